@@ -1,0 +1,191 @@
+"""Benchmark: time to 1e-6 relative KKT of the PDHG LP path on one MI355X,
+mixed FP32/FP64 vs FP64, and the HBM bandwidth of the SpMV kernels.
+
+Workload (the LP the metric is quoted on): m = 400,000 rows, n = 1,000,000
+columns, Zipf(1.5) column lengths on [2, 200] (~17.6M nonzeros), a random 10%
+of columns rescaled by 10^U(-2,2), 70% equality rows and 30% <= rows, 0 <= x <= 10,
+seeded (generator family 2 of include/vikalp_lpgen.h).
+
+A step is one solve from a cold start until the relative KKT error (original
+coordinates) is <= 1e-6, with the adaptive restarted PDHG on the fused HIP
+path.  `value` is the mixed-precision device time of the iteration phase (the
+LP is already resident in HBM when it starts; setup, scaling and postsolve
+are reported separately), `fp64_seconds` the same for the FP64 path.
+
+roofline: the dominant fused kernel (A' y with the residual terms and the
+next primal step, or A x with the dual update) timed with HIP events on the
+solver's stream; achieved = its algorithmic HBM bytes per launch / mean
+duration.  cpu_baseline: the CPU oracle (the reference's single-threaded
+algorithm restated in C) timed on a bounded sample of iterations of the same
+LP on one host core, extrapolated to the GPU's iteration count.
+
+Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = {"workload": "pdhg_lp_zipf", "m": 400_000, "n": 1_000_000, "family": 2,
+            "seed": 20261203, "tolerance": 1e-6, "policy": "adaptive"}
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(m, n, nnz, value_bytes, adaptive=True):
+    """Algorithmic (compulsory) HBM bytes per launch of the two fused kernels.
+
+    dual   (rows of A):  values+cols (s_v+4)*nnz, row offsets 4(m+1), x gathered
+                         once s_v*n; per row read ax, y, rl, ru, 1/r, y_last and
+                         y_sum, write ax, y, y_sum: 10*s_v*m.
+    primal (rows of A'): values+rows (s_v+4)*nnz, offsets 4(n+1), y gathered
+                         once s_v*m; per column read x, c, l, u, s, x_last,
+                         x_sum, write aty, x_next, x_sum: 10*s_v*n.
+    """
+    sv = value_bytes
+    per = 10 if adaptive else 7
+    dual = (sv + 4) * nnz + 4 * (m + 1) + sv * n + per * sv * m
+    primal = (sv + 4) * nnz + 4 * (n + 1) + sv * m + per * sv * n
+    return dual, primal
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--m", type=int, default=WORKLOAD["m"])
+    ap.add_argument("--n", type=int, default=WORKLOAD["n"])
+    ap.add_argument("--max-iterations", type=int, default=200_000)
+    ap.add_argument("--time-limit", type=float, default=100.0, help="seconds per solve")
+    ap.add_argument("--cpu-iterations", type=int, default=12)
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and the PDHG path "
+              "is built for one device (multi-GPU sharding is described in DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import vikalp_amd as V
+
+    V.load_libraries()
+    t0 = time.time()
+    lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
+    log(f"generated LP m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t0:.1f}s")
+
+    def options(precision):
+        return V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
+                         relative_kkt_tolerance=WORKLOAD["tolerance"],
+                         iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit)
+
+    # Kernel timing for the roofline (HIP events on the solver's stream).
+    prof = V.profile_kernels(lp, options(V.PdhgPrecision.FP64), warm_iterations=30, iterations=60)
+    prof32 = V.profile_kernels(lp, options(V.PdhgPrecision.Mixed), warm_iterations=30, iterations=60)
+    d64, p64 = kernel_bytes(lp.m, lp.n, lp.nnz, 8)
+    d32, p32 = kernel_bytes(lp.m, lp.n, lp.nnz, 4)
+    log(f"fp64 kernels: dual {prof['dual_ms']:.4f} ms ({d64 / prof['dual_ms'] / 1e6:.0f} GB/s), "
+        f"primal {prof['primal_ms']:.4f} ms ({p64 / prof['primal_ms'] / 1e6:.0f} GB/s), "
+        f"iteration {prof['iteration_ms']:.4f} ms")
+    log(f"mixed kernels: dual {prof32['dual_ms']:.4f} ms ({d32 / prof32['dual_ms'] / 1e6:.0f} GB/s), "
+        f"primal {prof32['primal_ms']:.4f} ms ({p32 / prof32['primal_ms'] / 1e6:.0f} GB/s), "
+        f"iteration {prof32['iteration_ms']:.4f} ms")
+
+    def run(precision, label):
+        log(f"solve {label} start")
+        r = V.solve_lp(lp, options(precision))
+        log(f"solve {label}: {r.status} iterations={r.iterations} (fp32 {r.low_precision_iterations}) "
+            f"device={r.device_seconds:.3f}s wall={r.solve_seconds:.3f}s relkkt={r.kkt_relative:.3e} "
+            f"obj={r.objective_value:.10g}")
+        return r
+
+    for _ in range(args.warmup):
+        run(V.PdhgPrecision.Mixed, "warmup")
+    fp64_runs, mixed_runs = [], []
+    for _ in range(args.steps):
+        fp64_runs.append(run(V.PdhgPrecision.FP64, "fp64"))
+        mixed_runs.append(run(V.PdhgPrecision.Mixed, "mixed"))
+
+    def summary(runs):
+        ok = [r for r in runs if r.status == "Optimal"]
+        dev = float(np.median([r.device_seconds for r in runs]))
+        return dev, bool(ok) and len(ok) == len(runs)
+
+    t64, ok64 = summary(fp64_runs)
+    t32, ok32 = summary(mixed_runs)
+    r32, r64 = mixed_runs[-1], fp64_runs[-1]
+    obj_err = abs(r32.objective_value - r64.objective_value) / max(1.0, abs(r64.objective_value))
+
+    dominant = "primal" if prof["primal_ms"] >= prof["dual_ms"] else "dual"
+    dom_ms = prof[dominant + "_ms"]
+    dom_bytes = p64 if dominant == "primal" else d64
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import oracle as O   # the checker, timed as the CPU baseline
+        olp = lp.as_oracle()
+        per_it = O.time_iterations(olp, args.cpu_iterations) / args.cpu_iterations
+        its = r64.iterations
+        cpu = {"value": per_it * its, "unit": "s (extrapolated)", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iterations} iterations of the reference loop (4 SpMVs/iteration) "
+                         f"on the same LP: {per_it:.3f} s/iteration x {its} iterations of the "
+                         f"FP64 GPU solve", "host_cpu": platform.processor() or platform.machine(),
+               "host_nproc": os.cpu_count()}
+        log(f"cpu baseline: {per_it:.3f} s/iteration")
+
+    line = {
+        "metric": "time to 1e-6 rel. KKT, mixed-FP32 vs FP64 PDHG, 1 MI355X; SpMV HBM GB/s",
+        "value": t32 if ok32 else None,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded generator, LP built on the host)",
+        "config": {**WORKLOAD, "m": lp.m, "n": lp.n, "nnz": lp.nnz},
+        "fp64_seconds": t64 if ok64 else None,
+        "speedup_mixed_vs_fp64": (t64 / t32) if (ok32 and ok64) else None,
+        "iterations_fp64": r64.iterations,
+        "iterations_mixed": r32.iterations,
+        "iterations_mixed_fp32_phase": r32.low_precision_iterations,
+        "status_fp64": r64.status,
+        "status_mixed": r32.status,
+        "rel_kkt_fp64": r64.kkt_relative,
+        "rel_kkt_mixed": r32.kkt_relative,
+        "objective_rel_diff_mixed_vs_fp64": obj_err,
+        "ms_per_iteration_fp64": prof["iteration_ms"],
+        "ms_per_iteration_mixed": prof32["iteration_ms"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_{dominant}<double>", "bytes_per_launch": dom_bytes,
+                     "mean_ms": dom_ms},
+        "kernels": {
+            "fp64": {"dual_ms": prof["dual_ms"], "primal_ms": prof["primal_ms"],
+                     "dual_GBs": d64 / prof["dual_ms"] / 1e6, "primal_GBs": p64 / prof["primal_ms"] / 1e6},
+            "mixed": {"dual_ms": prof32["dual_ms"], "primal_ms": prof32["primal_ms"],
+                      "dual_GBs": d32 / prof32["dual_ms"] / 1e6, "primal_GBs": p32 / prof32["primal_ms"] / 1e6},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,195 @@
+/*
+ * vikalp_hip.h — C ABI of the MI355X (gfx950) kernel library libvikalp_hip.so.
+ *
+ * This is the drop-in boundary under the reference's ExecutionBackend
+ * contract (include/vikalp/contracts/ExecutionBackend.hpp:36-64).  The CUDA
+ * backend it replaces (cuda/cuda_ops.cu) reaches its device through CUDA
+ * runtime calls, five kernels and cuSPARSE; here a C++ HipBackend (and any
+ * FFI caller) reaches the device only through these entry points.
+ *
+ * Rules for every entry point:
+ *   - returns 0 on success, a negative VK_E* code on failure; the message is
+ *     kept per thread and read with vk_hip_last_error();
+ *   - device buffers are plain pointers from vk_hip_alloc; sizes are element
+ *     counts (int64_t); `stream` is a hipStream_t passed as void* (NULL =
+ *     the library's own stream);
+ *   - FP64 unless the name says f32; device indices are int32 (host CSR
+ *     input is int64, like vikalp::Index, and is range-checked on upload);
+ *   - no entry point uses floating-point atomics: every reduction has a
+ *     fixed order and repeats bitwise.
+ */
+#ifndef VIKALP_HIP_H
+#define VIKALP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VK_OK = 0,
+    VK_EINVAL = -1,     /* bad argument (dimension, null, range)         */
+    VK_EDEVICE = -2,    /* HIP runtime error                             */
+    VK_ENOMEM = -3,     /* device or host allocation failed              */
+    VK_ESTATE = -4      /* call out of order / object in the wrong state */
+};
+
+/* ---- runtime --------------------------------------------------------- */
+
+/* cuda_ops.cu:314-317 (cudaSetDevice(0) + cusparseCreate in the backend
+ * constructor).  Selects `device` and creates the library stream. */
+int vk_hip_init(int device);
+/* Releases the library stream; device buffers stay valid until freed. */
+int vk_hip_shutdown(void);
+const char *vk_hip_last_error(void);
+/* The library's stream (hipStream_t as void*). */
+void *vk_hip_stream(void);
+
+/* cuda_ops.cu:89-124 (DeviceBuffer RAII) */
+int vk_hip_alloc(void **ptr, size_t bytes);
+int vk_hip_free(void *ptr);
+/* cuda_ops.cu:339-360 upload/download and :384-393 copy.  The h2d/d2h
+ * forms are synchronous on `stream` (the host buffer is free on return). */
+int vk_hip_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int vk_hip_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int vk_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+/* cuda_ops.cu:659-661 synchronize() */
+int vk_hip_sync(void *stream);
+
+/* ---- vector ops (ExecutionBackend.hpp:46-55) --------------------------- */
+
+/* fill: cuda_ops.cu:139-142 / cpu_ops.cpp:316-319 */
+int vk_hip_fill(double *y, int64_t n, double value, void *stream);
+/* y = alpha*x + beta*y, literally (x may alias y): cpu_ops.cpp:331-343 */
+int vk_hip_axpby(int64_t n, double alpha, const double *x, double beta, double *y,
+                 void *stream);
+/* x = min(max(x, lo), hi): cpu_ops.cpp:367-383 */
+int vk_hip_project_box(int64_t n, double *x, const double *lo, const double *hi,
+                       void *stream);
+/* max |x_i| (0 for n = 0), result to host; synchronous: cpu_ops.cpp:358-364,
+ * cuda_ops.cu:432-451 (finished on the device here, not on the host). */
+int vk_hip_reduce_absmax(int64_t n, const double *x, double *result, void *stream);
+/* sum x_i*y_i, result to host; synchronous: cpu_ops.cpp:346-355 */
+int vk_hip_reduce_dot(int64_t n, const double *x, const double *y, double *result,
+                      void *stream);
+
+/* ---- sparse matrices --------------------------------------------------- */
+
+/* A device-resident CSR matrix together with its explicitly stored
+ * transpose and the row tiling both SpMV directions use. */
+typedef struct vk_hip_csr vk_hip_csr;
+
+/* cuda_ops.cu:198-281 create_matrix (+ validate_pattern, cpu_ops.cpp:38-63).
+ * Takes the host pattern (int64 offsets/indices, columns strictly increasing
+ * per row), validates it, uploads it as int32 and builds the transpose
+ * pattern.  Values are zero until vk_hip_csr_set_values. */
+int vk_hip_csr_create(int64_t rows, int64_t cols, const int64_t *row_offsets,
+                      const int64_t *column_indices, vk_hip_csr **out);
+/* cuda_ops.cu:362-372 set_values: nnz host values in pattern order; the
+ * transpose values are gathered on the device. */
+int vk_hip_csr_set_values(vk_hip_csr *A, const double *values);
+int vk_hip_csr_destroy(vk_hip_csr *A);
+int vk_hip_csr_dims(const vk_hip_csr *A, int64_t *rows, int64_t *cols, int64_t *nnz);
+/* Copies the device transpose (int32 offsets n+1, int32 row indices nnz,
+ * values nnz) to host buffers; any pointer may be NULL. */
+int vk_hip_csr_transpose_export(const vk_hip_csr *A, int32_t *t_offsets,
+                                int32_t *t_rows, double *t_values);
+
+/* cuda_ops.cu:473-528 multiply: y = alpha*op(A)*x + beta*y, op = A when
+ * transpose == 0, A' otherwise.  A' runs on the stored transpose (row-
+ * parallel, no atomics).  y must not alias x. */
+int vk_hip_spmv_csr(const vk_hip_csr *A, int transpose, double alpha,
+                    const double *x, double beta, double *y, void *stream);
+
+/* Estimate ||A||_2 by `iterations` power iterations on A'A from a fixed
+ * seeded start vector (deterministic).  Result to host. */
+int vk_hip_csr_norm2_estimate(const vk_hip_csr *A, int iterations, double *result);
+
+/* ---- fused PDHG (new: the reference drives PDHG op by op, pdhg.cpp:87-151) */
+
+enum {
+    VK_PDHG_POLICY_REFERENCE = 0,  /* pdhg.cpp:75-150 step/restart rule      */
+    VK_PDHG_POLICY_ADAPTIVE = 1    /* restarted PDHG with averaging and a
+                                      primal weight (KKT restart criteria)   */
+};
+
+enum {
+    VK_PDHG_RUNNING = 0,
+    VK_PDHG_CONVERGED = 1,         /* termination test met                   */
+    VK_PDHG_ITERATION_LIMIT = 2
+};
+
+enum {
+    VK_PREC_FP64 = 0,              /* A, iterates and all vectors in FP64    */
+    VK_PREC_MIXED = 1              /* A and iterates FP32 (compensated row
+                                      sums); reductions and control in FP64  */
+};
+
+typedef struct {
+    int policy;                    /* VK_PDHG_POLICY_*                         */
+    int precision;                 /* VK_PREC_*                                */
+    int64_t iteration_limit;       /* SolverOptions::iteration_limit           */
+    double metric_tolerance;       /* reference test: max(r_p, r_d) <= this;
+                                      max(primal_tol, dual_tol) (pdhg.cpp:147) */
+    double relative_kkt_tolerance; /* > 0: stop on relative KKT <= this
+                                      instead of the reference test          */
+    double step_size;              /* reference: 0.99/max(1, bound) (pdhg.cpp:76);
+                                      adaptive: eta, with tau = eta/w, sigma = eta*w */
+    double primal_weight;          /* adaptive: initial w (> 0)               */
+    int64_t restart_period;        /* adaptive: iterations between restart
+                                      checks (default 64)                     */
+    double norm_b;                 /* ||bhat||_2 and ||c||_2 in original      */
+    double norm_c;                 /*   coordinates (relative KKT denominators) */
+    int use_graphs;                /* capture iteration blocks in hipGraphs   */
+    int grid_blocks;               /* 0 = auto; else blocks per SpMV launch   */
+} vk_pdhg_params;
+
+typedef struct {
+    int64_t iterations;            /* completed iterations                    */
+    int status;                    /* VK_PDHG_*                               */
+    int restarts;                  /* adaptive restarts taken                 */
+    double tau, sigma, primal_weight;
+    double metric;                 /* reference metric max(r_p, r_d), scaled  */
+    double primal_scaled, dual_scaled;   /* r_p, r_d (infinity norms)        */
+    double rel_primal, rel_dual, rel_gap, rel_kkt;  /* original coordinates   */
+    double primal_objective, dual_objective;        /* without offset         */
+    int converged_on_average;      /* adaptive: solution is the average      */
+    int reserved;
+} vk_pdhg_state;
+
+typedef struct vk_hip_pdhg vk_hip_pdhg;
+
+/* Allocates the solver on the device for the SCALED LP (the caller scales,
+ * like pdhg.cpp:25): A (values already set), objective c, variable bounds
+ * l,u, row bounds rl,ru (+-inf allowed), row_scale r and col_scale s
+ * (x = s*xhat, y = r*yhat, used for original-coordinate KKT), and the scaled
+ * start x0 (NULL = 0) and y0 (NULL = 0).  x0 is projected onto [l,u]
+ * (pdhg.cpp:71).  In VK_PREC_MIXED the FP32 copies are made on the device. */
+int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l,
+                       const double *u, const double *rl, const double *ru,
+                       const double *row_scale, const double *col_scale,
+                       const double *x0, const double *y0,
+                       const vk_pdhg_params *params, vk_hip_pdhg **out);
+/* Runs up to `max_iterations` more iterations (or until the device stops),
+ * polling the device state every iteration block.  Returns the VK_PDHG_*
+ * status through `status` (not an error). */
+int vk_hip_pdhg_run(vk_hip_pdhg *solver, int64_t max_iterations, int *status);
+int vk_hip_pdhg_read_state(vk_hip_pdhg *solver, vk_pdhg_state *state);
+/* Scaled iterates (FP64 on the host) of the current solution. */
+int vk_hip_pdhg_download(vk_hip_pdhg *solver, double *x, double *y);
+/* Milliseconds of device time of the last vk_hip_pdhg_run, measured with
+ * HIP events on the solver's stream. */
+int vk_hip_pdhg_last_run_ms(const vk_hip_pdhg *solver, double *ms);
+/* Runs `iterations` more iterations launched one by one, with HIP events on
+ * the solver's stream around the dual and primal kernels; returns their mean
+ * durations and the mean wall time per iteration (ms).  For measurement. */
+int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *dual_ms,
+                        double *primal_ms, double *iteration_ms);
+int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+/*
+ * vikalp_solve.h — C ABI of the host library libvikalp.so (C++20).
+ *
+ * A flat-argument form of vikalp::solve / vikalp::solver::solve_lp for FFI
+ * callers (ctypes, cgo, JNI).  It replaces, for the LP path:
+ *   vikalp::solve(model, options)            src/solver/Solver.cpp:92-99
+ *   vikalp::solver::solve_lp(model, backend, options, warm, lo, hi)
+ *                                            src/solver/lp/pdhg.cpp:269-293
+ * with the backend fixed to the HIP ExecutionBackend (make_hip_backend()).
+ * The model is the reference's canonical LP (include/vikalp/contracts/Model.hpp):
+ * CSR with int64 offsets/indices, +-inf bounds allowed.
+ */
+#ifndef VIKALP_SOLVE_H
+#define VIKALP_SOLVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* vikalp::SolverOptions (include/vikalp/contracts/SolverOptions.hpp:15-28),
+ * enums as ints in declaration order, then the HIP-path options. */
+typedef struct {
+    int backend;                    /* BackendPreference: 0 Auto, 1 CPU, 2 CUDA, 3 HIP */
+    int continuous_method;          /* 0 Auto, 1 PDHG, 2 RevisedSimplex, 3 InteriorPoint */
+    int64_t iteration_limit;
+    int64_t node_limit;
+    double time_limit_seconds;
+    double primal_tolerance;
+    double dual_tolerance;
+    double integrality_tolerance;
+    double absolute_gap_tolerance;
+    double relative_gap_tolerance;
+    int deterministic;
+    int pdhg_policy;                /* 0 Reference, 1 Adaptive */
+    int pdhg_precision;             /* 0 FP64, 1 Mixed */
+    int fused_iterations;
+    int use_graphs;
+    int reserved;
+    double relative_kkt_tolerance;
+    double mixed_phase_tolerance;
+    int64_t restart_period;
+} vk_solver_options;
+
+/* Fills `o` with the defaults of vikalp::SolverOptions. */
+void vk_solver_options_default(vk_solver_options *o);
+
+/* Scalar fields of vikalp::SolveResult (SolveResult.hpp:26-61) + the
+ * appended relative-KKT and device-time fields. */
+typedef struct {
+    int status;                     /* SolveStatus in declaration order */
+    int reserved;
+    int64_t iterations;
+    int64_t low_precision_iterations;
+    double objective_value;
+    double primal_bound, dual_bound;
+    double primal_residual;
+    double stationarity_residual;
+    double complementarity_residual;
+    double absolute_gap;
+    double relative_gap;
+    double solve_seconds;
+    double device_seconds;
+    double kkt_relative_primal, kkt_relative_dual, kkt_relative_gap, kkt_relative;
+    double kkt_primal_objective, kkt_dual_objective;
+    char backend[16];
+    char solver[32];
+    char message[160];
+} vk_solve_info;
+
+/* Solves the LP.  `mode` 0 = vikalp::solve (router: validates the model,
+ * honours options->backend), 1 = solver::solve_lp on a HIP backend directly
+ * (honours warm_start and the bound overrides; any may be NULL).
+ * x_out (n) receives the primal solution, y_out (m) the row duals in the
+ * reference sign (constraint_upper_dual - constraint_lower_dual); either may
+ * be NULL.  Returns 0 when a result was produced (see info->status),
+ * negative when the arguments were unusable. */
+int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
+                const int64_t *column_indices, const double *values,
+                const double *objective, double objective_offset,
+                const double *row_lower, const double *row_upper,
+                const double *var_lower, const double *var_upper,
+                const double *warm_start, const double *lower_override,
+                const double *upper_override, const vk_solver_options *options,
+                double *x_out, double *y_out, vk_solve_info *info);
+
+/* Measurement: builds the same scaled LP on the HIP backend, runs
+ * `warm_iterations` iterations, then `iterations` more launched one by one
+ * with HIP events around the two fused kernels on the solver's stream.
+ * out[0] = mean dual-kernel ms (A x with the dual update), out[1] = mean
+ * primal-kernel ms (A'y with the residuals and primal step), out[2] = mean
+ * ms per iteration, out[3] = the step size used.  Returns 0 on success. */
+int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
+                  const int64_t *column_indices, const double *values,
+                  const double *objective, const double *row_lower,
+                  const double *row_upper, const double *var_lower,
+                  const double *var_upper, const vk_solver_options *options,
+                  int64_t warm_iterations, int64_t iterations, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,280 @@
+"""Pins the CPU oracle (oracle/pdhg_oracle.c) before anything is checked
+against it.  CPU only.
+
+  - ops: the hand-computed values of the reference's tests/test_backend.cpp
+    and dense brute force on random CSR;
+  - scaling: the reference's literal O(n*nnz) column loop vs the O(nnz)
+    pass vs a dense numpy restatement, and the operator bound vs dense sums;
+  - BiCGSTAB: the reference's identity / 2x2 / breakdown cases;
+  - the solve: bitwise-equal trajectories against the reference's own
+    sources (oracle/_ref, when built) and against the committed fixtures
+    written from them; analytic optima and failure paths of
+    tests/solver/continuous/test_pdhg.cpp;
+  - relative KKT: zero at an exact optimum (vertex enumeration on a tiny LP),
+    the dual bound matches the LP dual at the HiGHS optimum, and every term
+    reacts to a planted error.
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import vikalp_amd as V
+
+INF = math.inf
+
+
+def csr_from_dense(D):
+    rp, ci, v = [0], [], []
+    for row in D:
+        for j, a in enumerate(row):
+            if a != 0.0:
+                ci.append(j)
+                v.append(a)
+        rp.append(len(ci))
+    return np.array(rp, np.int64), np.array(ci, np.int64), np.array(v)
+
+
+def random_csr(rng, m, n, density):
+    D = np.where(rng.random((m, n)) < density, rng.normal(size=(m, n)), 0.0)
+    return D, *csr_from_dense(D)
+
+
+# ---------------------------------------------------------------- ops
+
+def test_spmv_hand_values():
+    # test_backend.cpp:257-312: A = [1 2 0; 0 3 4]
+    rp, ci, v = np.array([0, 2, 4]), np.array([0, 1, 1, 2]), np.array([1.0, 2.0, 3.0, 4.0])
+    y = np.zeros(2)
+    O.spmv(rp, ci, v, 2, 3, 1.0, np.array([2.0, 1.0, 3.0]), 0.0, y, False)
+    assert list(y) == [4.0, 15.0]
+    O.spmv(rp, ci, v, 2, 3, 2.0, np.array([2.0, 1.0, 3.0]), 3.0, y, False)
+    assert list(y) == [20.0, 75.0]
+    yt = np.zeros(3)
+    O.spmv(rp, ci, v, 2, 3, 1.0, np.array([2.0, 1.0]), 0.0, yt, True)
+    assert list(yt) == [2.0, 7.0, 4.0]
+    O.spmv(rp, ci, v, 2, 3, 0.5, np.array([2.0, 1.0]), 2.0, yt, True)
+    assert list(yt) == [5.0, 17.5, 10.0]
+    # empty row (test_backend.cpp:337-360)
+    y3 = np.zeros(3)
+    O.spmv(np.array([0, 1, 1, 2]), np.array([0, 1]), np.array([1.0, 5.0]), 3, 2, 1.0,
+           np.array([2.0, 3.0]), 0.0, y3, False)
+    assert list(y3) == [2.0, 0.0, 15.0]
+
+
+def test_spmv_matches_dense_brute_force():
+    rng = np.random.default_rng(0)
+    for m, n, d in [(30, 40, 0.2), (1, 50, 0.9), (50, 1, 0.5), (200, 100, 0.03)]:
+        D, rp, ci, v = random_csr(rng, m, n, d)
+        x, yN = rng.normal(size=n), rng.normal(size=m)
+        exp = 1.5 * D @ x - 0.5 * yN
+        O.spmv(rp, ci, v, m, n, 1.5, x, -0.5, yN, False)
+        np.testing.assert_allclose(yN, exp, rtol=1e-13, atol=1e-13)
+        xt, yT = rng.normal(size=m), rng.normal(size=n)
+        exp = 0.7 * D.T @ xt + 2.0 * yT
+        O.spmv(rp, ci, v, m, n, 0.7, xt, 2.0, yT, True)
+        np.testing.assert_allclose(yT, exp, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference checker not built here")
+def test_spmv_bitwise_equals_reference_cpu_backend():
+    rng = np.random.default_rng(1)
+    D, rp, ci, v = random_csr(rng, 80, 60, 0.1)
+    for t in (False, True):
+        nx, ny = (80, 60) if t else (60, 80)
+        x, y = rng.normal(size=nx), rng.normal(size=ny)
+        a, b = y.copy(), y.copy()
+        O.spmv(rp, ci, v, 80, 60, 1.3, x, 0.4, a, t)
+        O.ref_spmv(rp, ci, v, 80, 60, 1.3, x, 0.4, b, t)
+        assert np.array_equal(a, b)
+
+
+def test_vector_ops():
+    L = O.lib()
+    x, y = np.array([1.0, -2.0, 3.0]), np.array([4.0, 5.0, -6.0])
+    L.orc_axpby(3, 2.5, O._ptr(x), -1.0, O._ptr(y))
+    assert list(y) == [-1.5, -10.0, 13.5]
+    a, b = np.array([1.0, -2.0, 0.5, 3.0]), np.array([-1.0, 3.0, 2.0, -0.5])
+    assert L.orc_dot(4, O._ptr(a), O._ptr(b)) == -7.5
+    assert L.orc_norm_inf(5, O._ptr(np.array([1.0, -7.5, 3.0, -2.0, 0.0]))) == 7.5
+    assert L.orc_norm_inf(0, O._ptr(np.zeros(1))) == 0.0
+    p = np.array([0.5, -1.0, 2.0])
+    L.orc_project_box(3, O._ptr(p), O._ptr(np.zeros(3)), O._ptr(np.ones(3)))
+    assert list(p) == [0.5, 0.0, 1.0]
+
+
+def test_validate_pattern():
+    L = O.lib()
+    ok = L.orc_validate_pattern(2, 3, 3, O._ptr(np.array([0, 2, 4], np.int64), O._lp), 4,
+                                O._ptr(np.array([0, 1, 1, 2], np.int64), O._lp))
+    assert ok == 0
+    bad = L.orc_validate_pattern(1, 1, 2, O._ptr(np.array([0, 2], np.int64), O._lp), 1,
+                                 O._ptr(np.array([0], np.int64), O._lp))
+    assert bad != 0
+    unordered = L.orc_validate_pattern(1, 3, 2, O._ptr(np.array([0, 2], np.int64), O._lp), 2,
+                                       O._ptr(np.array([2, 1], np.int64), O._lp))
+    assert unordered != 0
+
+
+def test_bicgstab_reference_cases():
+    # test_backend.cpp:380-448
+    conv, its, res, x = O.bicgstab([0, 1, 2], [0, 1], [1.0, 1.0], np.array([3.0, 5.0]), 1e-10, 100)
+    assert conv and np.allclose(x, [3.0, 5.0], atol=1e-8)
+    conv, its, res, x = O.bicgstab([0, 2, 4], [0, 1, 0, 1], [2.0, 1.0, 1.0, 3.0], np.array([4.0, 5.0]), 1e-10, 1000)
+    assert conv and np.allclose(x, [1.4, 1.2], atol=1e-7)
+    conv, its, res, x = O.bicgstab([0, 1, 2], [0, 1], [1.0, 1.0], np.zeros(2), 1e-10, 100)
+    assert conv and its == 0
+    conv, its, res, x = O.bicgstab([0, 2, 4], [0, 1, 0, 1], [-3.0, -3.0, -3.0, 0.0], np.array([-3.0, 0.0]), 1e-10, 10)
+    assert not conv and its == 1 and res == 3.0
+
+
+# ---------------------------------------------------------------- scaling
+
+def dense_scaling(D, c, rl, ru, l, u):
+    """continuous_common.hpp:133-205 restated densely (a second, independent form)."""
+    A = np.abs(D)
+    rmax, cmax = A.max(axis=1), A.max(axis=0)
+    r = np.where(rmax > 1.0, 1.0 / np.where(rmax > 0, rmax, 1), 1.0)
+    s = np.where(cmax > 1.0, 1.0 / np.where(cmax > 0, cmax, 1), 1.0)
+    with np.errstate(invalid="ignore"):
+        return dict(values=(D * (r[:, None] * s[None, :])), objective=c * s, row_scale=r, col_scale=s,
+                    row_lower=np.where(np.isfinite(rl), rl * r, rl), row_upper=np.where(np.isfinite(ru), ru * r, ru),
+                    var_lower=np.where(np.isfinite(l), l / s, l), var_upper=np.where(np.isfinite(u), u / s, u))
+
+
+def test_scaling_literal_fast_and_dense_agree():
+    rng = np.random.default_rng(2)
+    D = np.where(rng.random((25, 30)) < 0.2, rng.normal(scale=3.0, size=(25, 30)), 0.0)
+    D[3, :] = 0.0                      # empty row keeps factor 1
+    D[:, 5] *= 1e-3                    # small column keeps factor 1
+    rp, ci, v = csr_from_dense(D)
+    c, l, u = rng.normal(size=30), -rng.random(30), 1 + rng.random(30)
+    l[0], u[1] = -INF, INF
+    rl, ru = rng.normal(size=25) - 5, rng.normal(size=25) + 5
+    rl[2], ru[4] = -INF, INF
+    lp = O.LP(25, 30, rp, ci, v, c, rl, ru, l, u)
+    fast, lit = O.scale_lp(lp, literal=False), O.scale_lp(lp, literal=True)
+    for k in fast:
+        assert np.array_equal(fast[k], lit[k]), k
+    dense = dense_scaling(D, c, rl, ru, l, u)
+    Dv = dense["values"][D != 0.0]
+    assert np.array_equal(fast["values"], Dv)
+    for k in ("objective", "row_scale", "col_scale", "row_lower", "row_upper", "var_lower", "var_upper"):
+        assert np.array_equal(fast[k], dense[k]), k
+
+
+def test_operator_bound_dense():
+    rng = np.random.default_rng(3)
+    D, rp, ci, v = random_csr(rng, 40, 70, 0.1)
+    got = O.operator_bound(40, 70, rp, ci, v)
+    exp = math.sqrt(np.abs(D).sum(axis=1).max() * np.abs(D).sum(axis=0).max())
+    assert abs(got - exp) <= 1e-12 * exp
+    assert got >= np.linalg.norm(D, 2) * (1 - 1e-12)   # it bounds ||A||_2
+
+
+# ---------------------------------------------------------------- the solve
+
+def bounded_linear(c, lo, hi):
+    """test_pdhg.cpp:28-37"""
+    return O.LP(0, 1, np.array([0]), np.array([], np.int64), np.array([]), np.array([c]),
+                np.array([]), np.array([]), np.array([lo]), np.array([hi]))
+
+
+def test_pdhg_analytic_lps():
+    # test_pdhg.cpp:105-127
+    r = O.solve_lp(bounded_linear(-1.0, 0.0, 10.0), iteration_limit=5000, primal_tolerance=1e-5, dual_tolerance=1e-5)
+    assert r.status == "Optimal" and abs(r.x[0] - 10.0) <= 1e-5 and abs(r.objective_value + 10.0) <= 1e-5
+    lp = O.LP(1, 1, np.array([0, 1]), np.array([0]), np.array([1.0]), np.array([-1.0]),
+              np.array([-INF]), np.array([3.0]), np.array([0.0]), np.array([20.0]))
+    r = O.solve_lp(lp, iteration_limit=5000, primal_tolerance=1e-5, dual_tolerance=1e-5)
+    assert r.status == "Optimal" and abs(r.x[0] - 3.0) <= 1e-4
+    # LpRelaxationOracle with warm start 2 and upper override 4 (test_pdhg.cpp:129-136)
+    r = O.solve_lp(lp, iteration_limit=5000, primal_tolerance=1e-5, dual_tolerance=1e-5,
+                   warm_start=np.array([2.0]), upper_override=np.array([4.0]))
+    assert r.status == "Optimal" and abs(r.x[0] - 3.0) <= 1e-4
+
+
+def test_pdhg_failure_paths():
+    # test_pdhg.cpp:190-217
+    assert O.solve_lp(bounded_linear(1.0, 3.0, 2.0), iteration_limit=20).status == "InvalidModel"
+    assert O.solve_lp(bounded_linear(1.0, 0.0, 2.0), iteration_limit=20,
+                      warm_start=np.array([INF])).status == "InvalidModel"
+    assert O.solve_lp(bounded_linear(1.0, 0.0, 2.0), iteration_limit=-1).status == "InvalidModel"
+    r = O.solve_lp(bounded_linear(1.0, 0.0, 2.0), time_limit_seconds=0.0)
+    assert r.status == "TimeLimit" and r.iterations == 0
+    r = O.solve_lp(bounded_linear(1.0, 0.0, 2.0), iteration_limit=0)
+    assert r.status == "IterationLimit" and r.iterations == 0
+
+
+def _summary_matches(v, s):
+    assert float(np.sum(v)) == s["sum"] and float(np.sum(v * v)) == s["sumsq"]
+    assert [float(t) for t in v[:4]] == s["head"] and [float(t) for t in v[-4:]] == s["tail"]
+
+
+def test_pdhg_bitwise_against_reference_fixtures(golden_dir):
+    with open(os.path.join(golden_dir, "reference_trajectories.json")) as f:
+        cases = json.load(f)
+    for name, c in cases.items():
+        lp = V.generate_lp(c["family"], c["m"], c["n"], c["k"], c["seed"]).as_oracle()
+        assert lp.m == c["m"] and lp.n == c["n"] and len(lp.values) == c["nnz"], name
+        r = O.solve_lp(lp, **c["options"])
+        assert (r.status, r.iterations) == (c["status"], c["iterations"]), name
+        assert r.objective_value == c["objective"], name
+        assert r.primal_residual == c["primal_residual"], name
+        assert r.stationarity_residual == c["stationarity_residual"], name
+        assert r.complementarity_residual == c["complementarity_residual"], name
+        _summary_matches(r.x, c["x"])
+        _summary_matches(r.y, c["y"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference checker not built here")
+def test_pdhg_bitwise_against_reference_build():
+    for fam, m, n, k, seed, it in [(1, 50, 120, 3, 21, 700), (2, 300, 800, 0, 4, 400)]:
+        lp = V.generate_lp(fam, m, n, k, seed).as_oracle()
+        a = O.solve_lp(lp, iteration_limit=it)
+        b = O.ref_solve_lp(lp, iteration_limit=it)
+        assert (a.status, a.iterations) == (b.status, b.iterations)
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+        assert a.objective_value == b.objective_value
+
+
+# ---------------------------------------------------------------- relative KKT
+
+def tiny_lp():
+    # min -x1 - 2 x2  s.t.  x1 + x2 <= 4,  x1 + 3 x2 = 6 (as rl=ru),  0 <= x <= 5
+    D = np.array([[1.0, 1.0], [1.0, 3.0]])
+    rp, ci, v = csr_from_dense(D)
+    return O.LP(2, 2, rp, ci, v, np.array([-1.0, -2.0]), np.array([-INF, 6.0]),
+                np.array([4.0, 6.0]), np.zeros(2), np.full(2, 5.0)), D
+
+
+def test_relative_kkt_zero_at_vertex_enumerated_optimum():
+    lp, D = tiny_lp()
+    # brute force: enumerate vertices of {x1+x2<=4, x1+3x2=6, 0<=x<=5}
+    best = None
+    cons = [(np.array([1.0, 1.0]), 4.0), (np.array([1.0, 0.0]), 0.0), (np.array([0.0, 1.0]), 0.0),
+            (np.array([1.0, 0.0]), 5.0), (np.array([0.0, 1.0]), 5.0)]
+    for a, b in cons:
+        M = np.array([[1.0, 3.0], a])
+        if abs(np.linalg.det(M)) < 1e-12:
+            continue
+        x = np.linalg.solve(M, [6.0, b])
+        if x.min() >= -1e-12 and x.max() <= 5 + 1e-12 and x.sum() <= 4 + 1e-12:
+            val = -x[0] - 2 * x[1]
+            if best is None or val < best[0]:
+                best = (val, x)
+    val, x = best           # x = (3, 1), objective -5
+    # duals in the reference sign: c + A'y = 0 on the active rows
+    y = np.linalg.solve(D.T, -lp.objective)
+    rp_, rd_, gp_, po, do = O.relative_kkt(lp, x, y)
+    assert max(rp_, rd_, gp_) < 1e-14
+    assert abs(po - val) < 1e-14 and abs(do - val) < 1e-14
+    # planted errors move each term
+    assert O.relative_kkt(lp, x + [0.1, 0.0], y)[0] > 1e-3        # infeasible
+    assert O.relative_kkt(lp, x, y + [0.0, 0.1])[1] > 1e-3        # not stationary
+    assert O.relative_kkt(lp, x, -y)[2] > 1e-3                     # wrong sign -> gap
+    assert O.relative_kkt(lp, x, y + [-1.0, 0.0])[2] == 1.0        # y<0 on a <= row: D = -inf

@@ -1,0 +1,1074 @@
+// Fused PDHG iteration for gfx950.
+//
+// The reference drives one PDHG iteration (src/solver/lp/pdhg.cpp:87-151)
+// through ~30 backend calls: 4 SpMVs, ~25 vector ops and 2 host-synchronising
+// infinity norms.  Here one iteration is two kernels and never synchronises
+// with the host:
+//
+//   dual   (rows of A):   ax+ = A x+;  Axbar = 2 ax+ - ax  (linearity: A(2x+ - x);
+//                         = ax+ after a restart, pdhg.cpp:98-104);
+//                         v = y + sigma Axbar;  y+ = v - sigma P_[rl,ru](v/sigma)
+//                         (pdhg.cpp:105-113); primal residual |P(ax+) - ax+|
+//                         (pdhg.cpp:116-120) and the row terms of the KKT sums.
+//   primal (rows of A'):  aty = A' y+ (stored transpose, no atomics);
+//                         dual residual |x - P_X(x - (c + aty))| (pdhg.cpp:122-130)
+//                         and the column KKT terms; then, fused, the next
+//                         primal step x++ = P_X(x+ - tau c - tau aty)
+//                         (pdhg.cpp:92-96) whenever tau cannot change at this
+//                         iteration.  The last block to finish reduces all
+//                         block partials in a fixed order and runs the
+//                         step/restart/termination rule on the device.
+//
+// So A x+ and A' y+ are the only SpMVs: pdhg.cpp:105 is replaced by
+// linearity and pdhg.cpp:122 is the next iteration's :92.
+//
+// Two policies share the kernels:
+//   reference  pdhg.cpp:75-150: tau = sigma, every 50 iterations halve both
+//              and restart the extrapolation when the metric grew by 25%, or
+//              grow tau by 5% (capped) when it fell by 90%; stop when
+//              max(r_p, r_d) <= max(primal_tol, dual_tol);
+//   adaptive   restarted PDHG with iterate averaging and a primal weight w
+//              (tau = eta/w, sigma = eta*w; Applegate et al., "Practical
+//              large-scale linear programming using primal-dual hybrid
+//              gradient", NeurIPS 2021): every `restart_period` iterations the
+//              average since the last restart is evaluated (A xbar and A' ybar
+//              by two more SpMVs), the better of current/average by the
+//              weighted KKT error becomes the restart candidate, restarts fire
+//              on the 0.2 / 0.8 / 0.36 criteria, and w is smoothed towards
+//              ||dy||/||dx|| between restart points.
+// Either policy can stop on the relative KKT error instead (original
+// coordinates, evaluated every iteration from the fused partial sums).
+
+#include "runtime.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+namespace vk {
+namespace {
+
+constexpr int kDecisionPeriod = 50;   // pdhg.cpp:133
+
+// Device-resident solver state; written only by the last block of a
+// finalising kernel, read by every kernel at its start.
+struct DevState {
+    long long k;            // completed iterations
+    long long ks;           // iterations since the last restart
+    long long ns;           // terms in the running sums
+    long long ns_avg;       // terms of the average evaluated at the last check
+    long long limit;
+    long long period;       // adaptive restart-check period
+    int status;             // VK_PDHG_*
+    int policy;
+    int restart_flag;       // reference: the next dual step uses xbar = x+
+    int restart_apply;      // adaptive: the last check restarted
+    int restart_to_avg;     // adaptive: ... to the average
+    int final_from_avg;     // adaptive: converged on the average
+    int restarts;
+    unsigned int counter;   // last-block detection
+    double tau, sigma, inv_sigma, eta, omega, max_step;
+    double prev_metric, metric, rp, rd;
+    double metric_tol, rel_tol, norm_b, norm_c;
+    double rel_p, rel_d, rel_g, rel_kkt, pobj, dobj;
+    double cur_pr2, cur_dr2, cur_gap, cur_dx2, cur_dy2;
+    double kkt_ref, kkt_prev;
+};
+
+// Partial-slot layout (slot 0 combines with max, the rest with +).
+//   rows: 0 max|P(ax)-ax| (scaled)   1 sum (orig primal residual)^2
+//         2 sum (scaled primal residual)^2   3 dual-objective row terms
+//         4 count of -inf dual terms   5 sum (y - y_last)^2
+//   cols: 0 max|x-P(x-g)| (scaled)   1 sum (orig dual residual)^2
+//         2 sum (scaled dual residual)^2   3 dual-objective column terms
+//         4 count of -inf dual terms   5 c'x   6 sum (x - x_last)^2
+
+template <class T>
+struct Bufs {
+    T *x[2];            // x_k in x[k & 1]
+    T *y, *ax, *aty;    // current y, A x, A' y (in place)
+    const T *c, *l, *u, *rl, *ru;
+    const T *row_unscale, *col_scale;   // 1/r and s
+    T *xsum, *ysum, *xlast, *ylast, *ax_avg, *aty_avg;  // adaptive only
+    double *part_rows, *part_cols, *part_erows, *part_ecols;
+    DevState *st;
+    int m, n;
+    int grid_rows, grid_cols;
+};
+
+__device__ __forceinline__ double kkt_weighted(double w, double pr2, double dr2, double gap) {
+    return sqrt(w * w * pr2 + dr2 / (w * w) + gap * gap);
+}
+
+// ------------------------------------------------------------------ epilogues
+
+// Dual step on row i given acc = (A x+)_i.
+template <class T>
+struct DualEpi {
+    const Bufs<T> &b;
+    double sigma, inv_sigma;
+    bool restart;
+    __device__ void row(int i, double acc, Partials &p) const {
+        const double axn = acc;
+        const double ax_old = static_cast<double>(b.ax[i]);
+        const double axbar = restart ? axn : 2.0 * axn - ax_old;
+        const double lo = static_cast<double>(b.rl[i]), hi = static_cast<double>(b.ru[i]);
+        const double v = static_cast<double>(b.y[i]) + sigma * axbar;
+        const double proj = clamp_box(inv_sigma * v, lo, hi);
+        const double yn = v - sigma * proj;
+        b.y[i] = static_cast<T>(yn);
+        b.ax[i] = static_cast<T>(axn);
+        if (b.ysum) b.ysum[i] += static_cast<T>(yn);
+        const double d = clamp_box(axn, lo, hi) - axn;
+        p.v[0] = dmax(p.v[0], fabs(d));
+        const double du = d * static_cast<double>(b.row_unscale[i]);
+        p.v[1] += du * du;
+        p.v[2] += d * d;
+        if (yn > 0.0) {
+            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= yn * hi;
+        } else if (yn < 0.0) {
+            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= yn * lo;
+        }
+        if (b.ylast) {
+            const double dy = yn - static_cast<double>(b.ylast[i]);
+            p.v[5] += dy * dy;
+        }
+    }
+};
+
+// Column terms of the KKT sums for (x_j, g_j = c_j + aty_j) (shared by the
+// per-iteration and the average evaluation).
+template <class T>
+__device__ __forceinline__ void column_terms(const Bufs<T> &b, int j, double xj, double cj,
+                                             double g, double lj, double uj, Partials &p) {
+    const double d = xj - clamp_box(xj - g, lj, uj);
+    p.v[0] = dmax(p.v[0], fabs(d));
+    p.v[2] += d * d;
+    const double s = static_cast<double>(b.col_scale[j]);
+    const double dorig = s * (xj - clamp_box(xj - g / (s * s), lj, uj));
+    p.v[1] += dorig * dorig;
+    p.v[5] += cj * xj;
+    if (g > 0.0) {
+        if (isinf(lj)) p.v[4] += 1.0; else p.v[3] += g * lj;
+    } else if (g < 0.0) {
+        if (isinf(uj)) p.v[4] += 1.0; else p.v[3] += g * uj;
+    }
+    if (b.xlast) {
+        const double dx = xj - static_cast<double>(b.xlast[j]);
+        p.v[6] += dx * dx;
+    }
+}
+
+// A' y on column j, the residual terms, and (fused) the next primal step.
+template <class T>
+struct PrimalEpi {
+    const Bufs<T> &b;
+    const T *xcur;
+    T *xnext;
+    double tau;
+    bool fuse;
+    __device__ void row(int j, double acc, Partials &p) const {
+        b.aty[j] = static_cast<T>(acc);
+        const double xj = static_cast<double>(xcur[j]);
+        const double cj = static_cast<double>(b.c[j]);
+        const double lj = static_cast<double>(b.l[j]), uj = static_cast<double>(b.u[j]);
+        column_terms(b, j, xj, cj, cj + acc, lj, uj, p);
+        if (fuse) {
+            const double xn = clamp_box((xj - tau * cj) - tau * acc, lj, uj);
+            xnext[j] = static_cast<T>(xn);
+            if (b.xsum) b.xsum[j] += static_cast<T>(xn);
+        }
+    }
+};
+
+// Average evaluation, rows: acc = (A xsum)_i, scaled by 1/ns.
+template <class T>
+struct EvalRowsEpi {
+    const Bufs<T> &b;
+    double inv_ns;
+    __device__ void row(int i, double acc, Partials &p) const {
+        const double axa = acc * inv_ns;
+        b.ax_avg[i] = static_cast<T>(axa);
+        const double ya = static_cast<double>(b.ysum[i]) * inv_ns;
+        const double lo = static_cast<double>(b.rl[i]), hi = static_cast<double>(b.ru[i]);
+        const double d = clamp_box(axa, lo, hi) - axa;
+        p.v[0] = dmax(p.v[0], fabs(d));
+        const double du = d * static_cast<double>(b.row_unscale[i]);
+        p.v[1] += du * du;
+        p.v[2] += d * d;
+        if (ya > 0.0) {
+            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= ya * hi;
+        } else if (ya < 0.0) {
+            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= ya * lo;
+        }
+        const double dy = ya - static_cast<double>(b.ylast[i]);
+        p.v[5] += dy * dy;
+    }
+};
+
+template <class T>
+struct EvalColsEpi {
+    const Bufs<T> &b;
+    double inv_ns;
+    __device__ void row(int j, double acc, Partials &p) const {
+        const double atya = acc * inv_ns;
+        b.aty_avg[j] = static_cast<T>(atya);
+        const double xa = static_cast<double>(b.xsum[j]) * inv_ns;
+        const double cj = static_cast<double>(b.c[j]);
+        column_terms(b, j, xa, cj, cj + atya, static_cast<double>(b.l[j]),
+                     static_cast<double>(b.u[j]), p);
+    }
+};
+
+// ------------------------------------------------------------------ device helpers
+
+// Marks this block done; returns true in every thread of the last block.
+__device__ __forceinline__ bool last_block(DevState *st) {
+    __shared__ bool am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = atomicAdd(&st->counter, 1u);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (am_last) __threadfence();
+    return am_last;
+}
+
+struct KktPieces {
+    double max0, orig2, scaled2, dual, inf, pobj, dx2, dy2;
+};
+
+__device__ __forceinline__ KktPieces combine(const double *r, const double *c) {
+    KktPieces k;
+    k.max0 = dmax(r[0], c[0]);
+    k.orig2 = 0.0;
+    k.scaled2 = 0.0;
+    k.dual = r[3] + c[3];
+    k.inf = r[4] + c[4];
+    k.pobj = c[5];
+    k.dx2 = c[6];
+    k.dy2 = r[5];
+    return k;
+}
+
+__device__ __forceinline__ void relative_kkt(const DevState &s, const double *r, const double *c,
+                                             double &rel_p, double &rel_d, double &rel_g,
+                                             double &gap_abs, double &pobj, double &dobj) {
+    rel_p = sqrt(r[1]) / (1.0 + s.norm_b);
+    rel_d = sqrt(c[1]) / (1.0 + s.norm_c);
+    pobj = c[5];
+    const bool neg_inf = (r[4] + c[4]) > 0.0;
+    dobj = neg_inf ? -INFINITY : r[3] + c[3];
+    if (neg_inf) {
+        rel_g = 1.0;
+        gap_abs = INFINITY;
+    } else {
+        gap_abs = fabs(pobj - dobj);
+        rel_g = gap_abs / (1.0 + fabs(pobj) + fabs(dobj));
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dual(Bufs<T> b, CsrView<T> A) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds_prod[kTileNnz];
+    __shared__ double lds_rows[kTileRows];
+    __shared__ double scratch[kWaves];
+    const T *xin = b.x[(s.k + 1) & 1];
+    DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag};
+    Partials part;
+    part.zero();
+    tile_spmv(A, xin, epi, part, lds_prod, lds_rows, scratch);
+    store_partials(part, b.part_rows, scratch);
+}
+
+// Whether the primal kernel of iteration k may fuse the next primal step
+// (tau is known to stay the same).
+__device__ __forceinline__ bool may_fuse(const DevState &s) {
+    if (s.policy == VK_PDHG_POLICY_REFERENCE) return !(s.k > 0 && s.k % kDecisionPeriod == 0);
+    return (s.k + 1) % s.period != 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds_prod[kTileNnz];
+    __shared__ double lds_rows[kTileRows];
+    __shared__ double scratch[kWaves];
+    __shared__ double red_rows[kSlots], red_cols[kSlots];
+    const bool fuse = may_fuse(s);
+    PrimalEpi<T> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse};
+    Partials part;
+    part.zero();
+    tile_spmv(AT, b.y, epi, part, lds_prod, lds_rows, scratch);
+    store_partials(part, b.part_cols, scratch);
+    if (!last_block(b.st)) return;
+
+    reduce_partials(b.part_rows, b.grid_rows, red_rows, scratch);
+    reduce_partials(b.part_cols, b.grid_cols, red_cols, scratch);
+    if (threadIdx.x != 0) return;
+    DevState &S = *b.st;
+    const long long k = S.k;
+    const double rp = red_rows[0], rd = red_cols[0];
+    const double metric = dmax(rp, rd);   // pdhg.cpp:131
+    double rel_p, rel_d, rel_g, gap_abs, pobj, dobj;
+    relative_kkt(S, red_rows, red_cols, rel_p, rel_d, rel_g, gap_abs, pobj, dobj);
+    const double rel_kkt = dmax(rel_p, dmax(rel_d, rel_g));
+    S.rp = rp;
+    S.rd = rd;
+    S.metric = metric;
+    S.rel_p = rel_p;
+    S.rel_d = rel_d;
+    S.rel_g = rel_g;
+    S.rel_kkt = rel_kkt;
+    S.pobj = pobj;
+    S.dobj = dobj;
+    S.cur_pr2 = red_rows[2];
+    S.cur_dr2 = red_cols[2];
+    S.cur_gap = gap_abs;
+    S.cur_dx2 = red_cols[6];
+    S.cur_dy2 = red_rows[5];
+    bool converged;
+    if (S.policy == VK_PDHG_POLICY_REFERENCE) {
+        // pdhg.cpp:133-145; the restart flag was consumed by this iteration's
+        // dual step (pdhg.cpp:98-100).
+        S.restart_flag = 0;
+        if (k > 0 && k % kDecisionPeriod == 0) {
+            if (metric > S.prev_metric * 1.25) {
+                S.tau *= 0.5;
+                S.sigma *= 0.5;
+                S.restart_flag = 1;
+            } else if (metric < S.prev_metric * 0.1) {
+                const double grown = S.tau * 1.05;
+                S.tau = (S.max_step < grown) ? S.max_step : grown;
+                S.sigma = S.tau;
+            }
+            S.prev_metric = metric;
+        } else if (k == 0) {
+            S.prev_metric = metric;
+        }
+        S.inv_sigma = 1.0 / S.sigma;
+    }
+    converged = (S.rel_tol > 0.0) ? (rel_kkt <= S.rel_tol) : (metric <= S.metric_tol);  // pdhg.cpp:147
+    S.k = k + 1;
+    S.ks += 1;
+    S.ns += 1;
+    if (converged) S.status = VK_PDHG_CONVERGED;
+    else if (S.k >= S.limit) S.status = VK_PDHG_ITERATION_LIMIT;
+    S.counter = 0;
+}
+
+// Standalone primal step from x[k&1] into x[(k+1)&1] (setup, and after an
+// iteration whose step size may have changed).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pstep(Bufs<T> b) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    const T *xcur = b.x[s.k & 1];
+    T *xnext = b.x[(s.k + 1) & 1];
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
+        const double xj = static_cast<double>(xcur[j]);
+        const double xn = clamp_box((xj - s.tau * static_cast<double>(b.c[j])) -
+                                        s.tau * static_cast<double>(b.aty[j]),
+                                    static_cast<double>(b.l[j]), static_cast<double>(b.u[j]));
+        xnext[j] = static_cast<T>(xn);
+        if (b.xsum) b.xsum[j] += static_cast<T>(xn);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_rows(Bufs<T> b, CsrView<T> A) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds_prod[kTileNnz];
+    __shared__ double lds_rows[kTileRows];
+    __shared__ double scratch[kWaves];
+    EvalRowsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
+    Partials part;
+    part.zero();
+    tile_spmv(A, b.xsum, epi, part, lds_prod, lds_rows, scratch);
+    store_partials(part, b.part_erows, scratch);
+}
+
+// Average evaluation over columns; the last block runs the adaptive restart
+// rule.  `init` evaluates the start point instead and only records its KKT
+// error as the first restart reference.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_cols(Bufs<T> b, CsrView<T> AT, int init) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds_prod[kTileNnz];
+    __shared__ double lds_rows[kTileRows];
+    __shared__ double scratch[kWaves];
+    __shared__ double red_rows[kSlots], red_cols[kSlots];
+    EvalColsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
+    Partials part;
+    part.zero();
+    tile_spmv(AT, b.ysum, epi, part, lds_prod, lds_rows, scratch);
+    store_partials(part, b.part_ecols, scratch);
+    if (!last_block(b.st)) return;
+    reduce_partials(b.part_erows, b.grid_rows, red_rows, scratch);
+    reduce_partials(b.part_ecols, b.grid_cols, red_cols, scratch);
+    if (threadIdx.x != 0) return;
+    DevState &S = *b.st;
+    S.counter = 0;
+    double rel_p, rel_d, rel_g, gap_abs, pobj, dobj;
+    relative_kkt(S, red_rows, red_cols, rel_p, rel_d, rel_g, gap_abs, pobj, dobj);
+    const double w = S.omega;
+    const double kkt_avg = kkt_weighted(w, red_rows[2], red_cols[2], gap_abs);
+    if (init) {
+        S.kkt_ref = kkt_avg;
+        S.kkt_prev = INFINITY;
+        return;
+    }
+    const double rel_avg = dmax(rel_p, dmax(rel_d, rel_g));
+    S.ns_avg = S.ns;
+    S.restart_apply = 0;
+    S.restart_to_avg = 0;
+    if (S.rel_tol > 0.0 && rel_avg <= S.rel_tol) {
+        S.status = VK_PDHG_CONVERGED;
+        S.final_from_avg = 1;
+        S.rel_p = rel_p;
+        S.rel_d = rel_d;
+        S.rel_g = rel_g;
+        S.rel_kkt = rel_avg;
+        S.pobj = pobj;
+        S.dobj = dobj;
+        return;
+    }
+    const double kkt_cur = kkt_weighted(w, S.cur_pr2, S.cur_dr2, S.cur_gap);
+    const bool use_avg = kkt_avg < kkt_cur;
+    const double kc = use_avg ? kkt_avg : kkt_cur;
+    const bool restart = kc <= 0.2 * S.kkt_ref ||
+                         (kc <= 0.8 * S.kkt_ref && kc > S.kkt_prev) ||
+                         static_cast<double>(S.ks) >= 0.36 * static_cast<double>(S.k);
+    S.kkt_prev = kc;
+    if (!restart) return;
+    const double dx = sqrt(use_avg ? red_cols[6] : S.cur_dx2);
+    const double dy = sqrt(use_avg ? red_rows[5] : S.cur_dy2);
+    if (dx > 1e-10 && dy > 1e-10) S.omega = exp(0.5 * log(dy / dx) + 0.5 * log(S.omega));
+    S.tau = S.eta / S.omega;
+    S.sigma = S.eta * S.omega;
+    S.inv_sigma = 1.0 / S.sigma;
+    S.kkt_ref = use_avg ? kkt_weighted(S.omega, red_rows[2], red_cols[2], gap_abs)
+                        : kkt_weighted(S.omega, S.cur_pr2, S.cur_dr2, S.cur_gap);
+    S.kkt_prev = INFINITY;
+    S.ks = 0;
+    S.ns = 0;
+    S.restart_apply = 1;
+    S.restart_to_avg = use_avg ? 1 : 0;
+    S.restarts += 1;
+}
+
+// After a restart check: move to the average if chosen (or converged on it),
+// record the restart point, clear the sums, and take the next primal step.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_apply(Bufs<T> b) {
+    const DevState s = *b.st;
+    const bool copy_avg = s.final_from_avg || (s.restart_apply && s.restart_to_avg);
+    if (s.status != VK_PDHG_RUNNING && !s.final_from_avg) return;
+    const double inv = 1.0 / static_cast<double>(s.ns_avg > 0 ? s.ns_avg : 1);
+    T *xcur = b.x[s.k & 1];
+    T *xnext = b.x[(s.k + 1) & 1];
+    const int total = b.n > b.m ? b.n : b.m;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        if (t < b.n) {
+            if (copy_avg) {
+                xcur[t] = static_cast<T>(static_cast<double>(b.xsum[t]) * inv);
+                b.aty[t] = b.aty_avg[t];
+            }
+            if (s.restart_apply) {
+                b.xlast[t] = xcur[t];
+                b.xsum[t] = static_cast<T>(0);
+            }
+            if (s.status == VK_PDHG_RUNNING) {
+                const double xj = static_cast<double>(xcur[t]);
+                const double xn = clamp_box((xj - s.tau * static_cast<double>(b.c[t])) -
+                                                s.tau * static_cast<double>(b.aty[t]),
+                                            static_cast<double>(b.l[t]), static_cast<double>(b.u[t]));
+                xnext[t] = static_cast<T>(xn);
+                b.xsum[t] += static_cast<T>(xn);
+            }
+        }
+        if (t < b.m) {
+            if (copy_avg) {
+                b.y[t] = static_cast<T>(static_cast<double>(b.ysum[t]) * inv);
+                b.ax[t] = b.ax_avg[t];
+            }
+            if (s.restart_apply) {
+                b.ylast[t] = b.y[t];
+                b.ysum[t] = static_cast<T>(0);
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ void k_project_start(int n, T *x, const T *l, const T *u) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        x[j] = static_cast<T>(clamp_box(static_cast<double>(x[j]), static_cast<double>(l[j]),
+                                        static_cast<double>(u[j])));
+}
+
+template <class T>
+__global__ void k_copy(int n, const T *src, T *dst) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) dst[j] = src[j];
+}
+
+template <class T>
+__global__ void k_zero(int n, T *dst) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) dst[j] = T(0);
+}
+
+// Plain SpMV y = A x in the storage type (setup: A x0 and A' y0).
+template <class T>
+struct StoreEpi {
+    T *y;
+    __device__ void row(int r, double acc, Partials &) const { y[r] = static_cast<T>(acc); }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spmv_store(CsrView<T> A, const T *x, T *y) {
+    __shared__ double lds_prod[kTileNnz];
+    __shared__ double lds_rows[kTileRows];
+    __shared__ double scratch[kWaves];
+    StoreEpi<T> epi{y};
+    Partials part;
+    part.zero();
+    tile_spmv(A, x, epi, part, lds_prod, lds_rows, scratch);
+}
+
+__global__ void k_d2f(int n, const double *src, float *dst) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        dst[j] = static_cast<float>(src[j]);
+}
+
+__global__ void k_f2d(int n, const float *src, double *dst) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        dst[j] = static_cast<double>(src[j]);
+}
+
+int egrid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, 8192)); }
+
+}  // namespace
+}  // namespace vk
+
+using namespace vk;
+
+// The solver object behind the C ABI.
+struct vk_hip_pdhg {
+    vk_pdhg_params params{};
+    const vk_hip_csr *A = nullptr;
+    int m = 0, n = 0;
+    hipStream_t stream = nullptr;
+    DevState *st = nullptr;
+    DevState *host_state = nullptr;   // pinned
+    std::vector<void *> allocations;
+    Bufs<double> b64{};
+    Bufs<float> b32{};
+    bool mixed = false;
+    int grid_rows = 1, grid_cols = 1;
+    long long host_k = 0;             // iterations enqueued
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_snap[2] = {nullptr, nullptr};
+    DevState *snap[2] = {nullptr, nullptr};   // pinned
+    double last_ms = 0.0;
+
+    template <class T>
+    T *alloc(size_t count) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) return nullptr;
+        allocations.push_back(p);
+        return static_cast<T *>(p);
+    }
+};
+
+namespace {
+
+template <class T>
+const CsrView<T> view_a(const vk_hip_pdhg *s);
+template <>
+const CsrView<double> view_a<double>(const vk_hip_pdhg *s) { return s->A->a.view(); }
+template <>
+const CsrView<float> view_a<float>(const vk_hip_pdhg *s) { return s->A->a32.view(); }
+template <class T>
+const CsrView<T> view_at(const vk_hip_pdhg *s);
+template <>
+const CsrView<double> view_at<double>(const vk_hip_pdhg *s) { return s->A->at.view(); }
+template <>
+const CsrView<float> view_at<float>(const vk_hip_pdhg *s) { return s->A->at32.view(); }
+
+template <class T>
+Bufs<T> &bufs(vk_hip_pdhg *s);
+template <>
+Bufs<double> &bufs<double>(vk_hip_pdhg *s) { return s->b64; }
+template <>
+Bufs<float> &bufs<float>(vk_hip_pdhg *s) { return s->b32; }
+
+// Enqueues the kernels of iterations [k_begin, k_begin + count) without
+// synchronising; the schedule mirrors the device's own may_fuse rule.
+template <class T>
+void enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
+    Bufs<T> &b = bufs<T>(s);
+    const CsrView<T> A = view_a<T>(s), AT = view_at<T>(s);
+    const int ge = egrid(std::max(s->n, s->m));
+    for (long long k = k_begin; k < k_begin + count; ++k) {
+        k_dual<T><<<s->grid_rows, kBlock, 0, s->stream>>>(b, A);
+        k_primal<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT);
+        if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
+            if (k > 0 && k % kDecisionPeriod == 0) k_pstep<T><<<egrid(s->n), kBlock, 0, s->stream>>>(b);
+        } else if ((k + 1) % s->params.restart_period == 0) {
+            k_eval_rows<T><<<s->grid_rows, kBlock, 0, s->stream>>>(b, A);
+            k_eval_cols<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT, 0);
+            k_apply<T><<<ge, kBlock, 0, s->stream>>>(b);
+        }
+    }
+}
+
+// Length of the repeating iteration block and the first iteration it may
+// start at (the reference policy's iteration 0 has its own shape).
+long long block_length(const vk_hip_pdhg *s) {
+    return s->params.policy == VK_PDHG_POLICY_REFERENCE ? kDecisionPeriod : s->params.restart_period;
+}
+
+template <class T>
+int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double *u,
+                 const double *rl, const double *ru, const double *row_scale,
+                 const double *col_scale, const double *x0, const double *y0) {
+    const int m = s->m, n = s->n;
+    hipStream_t st = s->stream;
+    Bufs<T> &b = bufs<T>(s);
+    std::memset(&b, 0, sizeof(b));
+    b.m = m;
+    b.n = n;
+    b.x[0] = s->alloc<T>(n);
+    b.x[1] = s->alloc<T>(n);
+    b.y = s->alloc<T>(m);
+    b.ax = s->alloc<T>(m);
+    b.aty = s->alloc<T>(n);
+    T *c_d = s->alloc<T>(n), *l_d = s->alloc<T>(n), *u_d = s->alloc<T>(n);
+    T *rl_d = s->alloc<T>(m), *ru_d = s->alloc<T>(m);
+    T *runs = s->alloc<T>(m), *cs = s->alloc<T>(n);
+    b.c = c_d; b.l = l_d; b.u = u_d; b.rl = rl_d; b.ru = ru_d;
+    b.row_unscale = runs;
+    b.col_scale = cs;
+    const bool adaptive = s->params.policy == VK_PDHG_POLICY_ADAPTIVE;
+    if (adaptive) {
+        b.xsum = s->alloc<T>(n);
+        b.ysum = s->alloc<T>(m);
+        b.xlast = s->alloc<T>(n);
+        b.ylast = s->alloc<T>(m);
+        b.ax_avg = s->alloc<T>(m);
+        b.aty_avg = s->alloc<T>(n);
+    }
+    b.part_rows = s->alloc<double>(kSlots * s->grid_rows);
+    b.part_cols = s->alloc<double>(kSlots * s->grid_cols);
+    b.part_erows = s->alloc<double>(kSlots * s->grid_rows);
+    b.part_ecols = s->alloc<double>(kSlots * s->grid_cols);
+    b.st = s->st;
+    b.grid_rows = s->grid_rows;
+    b.grid_cols = s->grid_cols;
+    for (void *p : s->allocations)
+        if (!p) return VK_ENOMEM;
+    if (!b.x[0] || !b.x[1] || !b.y || !b.ax || !b.aty || !c_d || !l_d || !u_d || !rl_d ||
+        !ru_d || !runs || !cs || !b.part_rows || !b.part_cols || !b.part_erows || !b.part_ecols ||
+        (adaptive && (!b.xsum || !b.ysum || !b.xlast || !b.ylast || !b.ax_avg || !b.aty_avg))) {
+        set_error("vk_hip_pdhg_create: device allocation failed");
+        return VK_ENOMEM;
+    }
+    // Host vectors -> device (FP64 staging, converted on the device in mixed mode).
+    double *stage = nullptr;
+    VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * std::max(m, n) + 8));
+    std::vector<double> unscale(static_cast<size_t>(m));
+    for (int i = 0; i < m; ++i) unscale[i] = 1.0 / row_scale[i];
+    std::vector<double> zeros_n(static_cast<size_t>(n), 0.0), zeros_m(static_cast<size_t>(m), 0.0);
+    auto put = [&](T *dst, const double *src, int count) -> int {
+        if (count == 0) return VK_OK;
+        if constexpr (sizeof(T) == sizeof(double)) {
+            VK_HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyHostToDevice, st));
+        } else {
+            VK_HIP_CHECK(hipMemcpyAsync(stage, src, sizeof(double) * count, hipMemcpyHostToDevice, st));
+            k_d2f<<<egrid(count), kBlock, 0, st>>>(count, stage, reinterpret_cast<float *>(dst));
+            VK_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        return VK_OK;
+    };
+    int rc = VK_OK;
+    rc = rc ? rc : put(c_d, c, n);
+    rc = rc ? rc : put(l_d, l, n);
+    rc = rc ? rc : put(u_d, u, n);
+    rc = rc ? rc : put(rl_d, rl, m);
+    rc = rc ? rc : put(ru_d, ru, m);
+    rc = rc ? rc : put(runs, unscale.data(), m);
+    rc = rc ? rc : put(cs, col_scale, n);
+    rc = rc ? rc : put(b.x[0], x0 ? x0 : zeros_n.data(), n);
+    rc = rc ? rc : put(b.y, y0 ? y0 : zeros_m.data(), m);
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(stage);
+    if (rc) return rc;
+
+    // pdhg.cpp:71 project the start; A x0, A' y0.
+    k_project_start<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.l, b.u);
+    const CsrView<T> A = view_a<T>(s), AT = view_at<T>(s);
+    if (m > 0) {
+        k_zero<T><<<egrid(m), kBlock, 0, st>>>(m, b.ax);
+        if (A.ntiles > 0) k_spmv_store<T><<<s->grid_rows, kBlock, 0, st>>>(A, b.x[0], b.ax);
+    }
+    k_zero<T><<<egrid(n), kBlock, 0, st>>>(n, b.aty);
+    if (m > 0 && AT.ntiles > 0) k_spmv_store<T><<<s->grid_cols, kBlock, 0, st>>>(AT, b.y, b.aty);
+    if (adaptive) {
+        // KKT error of the start point = first restart reference: evaluate it
+        // through the average path with sums = (x0, y0) and ns = 1.
+        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xsum);
+        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xlast);
+        if (m > 0) {
+            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.ysum);
+            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.ylast);
+        }
+        DevState tmp = *s->host_state;
+        tmp.ns = 1;
+        VK_HIP_CHECK(hipMemcpyAsync(s->st, &tmp, sizeof(DevState), hipMemcpyHostToDevice, st));
+        VK_HIP_CHECK(hipStreamSynchronize(st));
+        k_eval_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, A);
+        k_eval_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, AT, 1);
+        k_zero<T><<<egrid(n), kBlock, 0, st>>>(n, b.xsum);
+        if (m > 0) k_zero<T><<<egrid(m), kBlock, 0, st>>>(m, b.ysum);
+        VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
+        VK_HIP_CHECK(hipStreamSynchronize(st));
+        s->host_state->ns = 0;
+        VK_HIP_CHECK(hipMemcpyAsync(s->st, s->host_state, sizeof(DevState), hipMemcpyHostToDevice, st));
+    }
+    // First primal step x1 = P(x0 - tau (c + A'y0)) into x[1].
+    k_pstep<T><<<egrid(n), kBlock, 0, st>>>(b);
+    VK_HIP_CHECK(hipGetLastError());
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    return VK_OK;
+}
+
+template <class T>
+int build_graph(vk_hip_pdhg *s, long long k_begin) {
+    if (s->graph) return VK_OK;
+    hipGraph_t g = nullptr;
+    VK_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_iterations<T>(s, k_begin, block_length(s));
+    VK_HIP_CHECK(hipStreamEndCapture(s->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    VK_HIP_CHECK(e);
+    return VK_OK;
+}
+
+template <class T>
+int run_solver(vk_hip_pdhg *s, long long max_iterations, int *status) {
+    hipStream_t st = s->stream;
+    VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    if (s->host_state->status != VK_PDHG_RUNNING) {
+        *status = s->host_state->status;
+        s->last_ms = 0.0;
+        return VK_OK;
+    }
+    s->host_k = s->host_state->k;
+    const long long target = s->host_k + max_iterations;
+    const long long L = block_length(s);
+    VK_HIP_CHECK(hipEventRecord(s->ev_start, st));
+    // Iterations up to the next block boundary run eagerly; whole blocks then
+    // run as one graph launch each.  After each launch the device state is
+    // copied to one of two pinned snapshots; the host reads the snapshot of
+    // the previous launch while the current one runs, so it never idles the
+    // device and overshoots a converged run by at most one block (whose
+    // kernels return at once).
+    long long first_boundary;
+    if (s->params.policy == VK_PDHG_POLICY_REFERENCE)
+        first_boundary = ((s->host_k - 1 + L - 1) / L) * L + 1;   // blocks start at k = 1 mod 50
+    else
+        first_boundary = ((s->host_k + L - 1) / L) * L;
+    if (s->host_k == 0 && s->params.policy == VK_PDHG_POLICY_REFERENCE) first_boundary = 1;
+    int pending = -1, cur = 0;
+    while (s->host_k < target) {
+        long long chunk;
+        if (s->host_k < first_boundary) {
+            chunk = std::min(target, first_boundary) - s->host_k;
+            enqueue_iterations<T>(s, s->host_k, chunk);
+        } else if (s->host_k + L <= target) {
+            chunk = L;
+            if (s->params.use_graphs) {
+                if (int rc = build_graph<T>(s, s->host_k)) return rc;
+                VK_HIP_CHECK(hipGraphLaunch(s->graph, st));
+            } else {
+                enqueue_iterations<T>(s, s->host_k, L);
+            }
+        } else {
+            chunk = target - s->host_k;
+            enqueue_iterations<T>(s, s->host_k, chunk);
+        }
+        VK_HIP_CHECK(hipGetLastError());
+        s->host_k += chunk;
+        VK_HIP_CHECK(hipMemcpyAsync(s->snap[cur], s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
+        VK_HIP_CHECK(hipEventRecord(s->ev_snap[cur], st));
+        if (pending >= 0) {
+            VK_HIP_CHECK(hipEventSynchronize(s->ev_snap[pending]));
+            if (s->snap[pending]->status != VK_PDHG_RUNNING) break;
+        }
+        pending = cur;
+        cur ^= 1;
+    }
+    VK_HIP_CHECK(hipEventRecord(s->ev_stop, st));
+    VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    VK_HIP_CHECK(hipEventElapsedTime(&ms, s->ev_start, s->ev_stop));
+    s->last_ms = ms;
+    *status = s->host_state->status;
+    return VK_OK;
+}
+
+template <class T>
+int download_solution(vk_hip_pdhg *s, double *x, double *y) {
+    Bufs<T> &b = bufs<T>(s);
+    VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+    VK_HIP_CHECK(hipStreamSynchronize(s->stream));
+    const T *xs = b.x[s->host_state->k & 1];
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (x && s->n) VK_HIP_CHECK(hipMemcpyAsync(x, xs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+        if (y && s->m) VK_HIP_CHECK(hipMemcpyAsync(y, b.y, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->stream));
+    } else {
+        double *stage = nullptr;
+        VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * (s->n + s->m) + 8));
+        if (s->n) k_f2d<<<egrid(s->n), kBlock, 0, s->stream>>>(s->n, xs, stage);
+        if (s->m) k_f2d<<<egrid(s->m), kBlock, 0, s->stream>>>(s->m, b.y, stage + s->n);
+        if (x && s->n) VK_HIP_CHECK(hipMemcpyAsync(x, stage, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+        if (y && s->m) VK_HIP_CHECK(hipMemcpyAsync(y, stage + s->n, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->stream));
+        VK_HIP_CHECK(hipStreamSynchronize(s->stream));
+        (void)hipFree(stage);
+    }
+    VK_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return VK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vk_hip_pdhg_destroy(vk_hip_pdhg *s) {
+    if (!s) return VK_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    for (void *p : s->allocations) (void)hipFree(p);
+    if (s->st) (void)hipFree(s->st);
+    if (s->host_state) (void)hipHostFree(s->host_state);
+    if (s->ev_start) (void)hipEventDestroy(s->ev_start);
+    if (s->ev_stop) (void)hipEventDestroy(s->ev_stop);
+    for (int i = 0; i < 2; ++i) {
+        if (s->ev_snap[i]) (void)hipEventDestroy(s->ev_snap[i]);
+        if (s->snap[i]) (void)hipHostFree(s->snap[i]);
+    }
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return VK_OK;
+}
+
+int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, const double *u,
+                       const double *rl, const double *ru, const double *row_scale,
+                       const double *col_scale, const double *x0, const double *y0,
+                       const vk_pdhg_params *params, vk_hip_pdhg **out) {
+    VK_REQUIRE(out && A && params, "vk_hip_pdhg_create: null argument");
+    *out = nullptr;
+    VK_REQUIRE(A->has_values, "vk_hip_pdhg_create: matrix values were never set");
+    VK_REQUIRE(A->cols > 0, "vk_hip_pdhg_create: the LP needs at least one variable");
+    VK_REQUIRE(c && l && u && col_scale, "vk_hip_pdhg_create: null column vector");
+    VK_REQUIRE(A->rows == 0 || (rl && ru && row_scale), "vk_hip_pdhg_create: null row vector");
+    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE,
+               "vk_hip_pdhg_create: unknown policy");
+    VK_REQUIRE(params->precision == VK_PREC_FP64 || params->precision == VK_PREC_MIXED,
+               "vk_hip_pdhg_create: unknown precision");
+    VK_REQUIRE(params->iteration_limit >= 0, "vk_hip_pdhg_create: negative iteration limit");
+    VK_REQUIRE(params->step_size > 0.0 && std::isfinite(params->step_size),
+               "vk_hip_pdhg_create: step size must be positive");
+    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE ||
+                   (params->primal_weight > 0.0 && params->restart_period > 0),
+               "vk_hip_pdhg_create: adaptive policy needs a primal weight and restart period");
+    if (int rc = ensure_init()) return rc;
+    auto *s = new (std::nothrow) vk_hip_pdhg();
+    if (!s) return VK_ENOMEM;
+    s->params = *params;
+    s->A = A;
+    s->m = static_cast<int>(A->rows);
+    s->n = static_cast<int>(A->cols);
+    s->mixed = params->precision == VK_PREC_MIXED;
+    const int cap = params->grid_blocks > 0 ? params->grid_blocks : 2048;
+    s->grid_rows = std::max(1, std::min(A->a.ntiles, cap));
+    s->grid_cols = std::max(1, std::min(A->at.ntiles, cap));
+    auto fail = [&](int rc) {
+        vk_hip_pdhg_destroy(s);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->ev_start) != hipSuccess || hipEventCreate(&s->ev_stop) != hipSuccess ||
+        hipMalloc(&s->st, sizeof(DevState)) != hipSuccess ||
+        hipHostMalloc(&s->host_state, sizeof(DevState)) != hipSuccess ||
+        hipHostMalloc(&s->snap[0], sizeof(DevState)) != hipSuccess ||
+        hipHostMalloc(&s->snap[1], sizeof(DevState)) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_snap[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_snap[1], hipEventDisableTiming) != hipSuccess) {
+        set_error("vk_hip_pdhg_create: runtime resource allocation failed");
+        return fail(VK_EDEVICE);
+    }
+    DevState init{};
+    init.limit = params->iteration_limit;
+    init.period = params->policy == VK_PDHG_POLICY_ADAPTIVE ? params->restart_period : kDecisionPeriod;
+    init.status = params->iteration_limit == 0 ? VK_PDHG_ITERATION_LIMIT : VK_PDHG_RUNNING;
+    init.policy = params->policy;
+    init.metric_tol = params->metric_tolerance;
+    init.rel_tol = params->relative_kkt_tolerance;
+    init.norm_b = params->norm_b;
+    init.norm_c = params->norm_c;
+    init.max_step = params->step_size;
+    init.prev_metric = std::numeric_limits<double>::infinity();
+    init.metric = std::numeric_limits<double>::infinity();
+    if (params->policy == VK_PDHG_POLICY_REFERENCE) {
+        init.tau = params->step_size;   // pdhg.cpp:76-78
+        init.sigma = params->step_size;
+        init.omega = 1.0;
+    } else {
+        init.eta = params->step_size;
+        init.omega = params->primal_weight;
+        init.tau = init.eta / init.omega;
+        init.sigma = init.eta * init.omega;
+    }
+    init.inv_sigma = 1.0 / init.sigma;
+    init.kkt_ref = std::numeric_limits<double>::infinity();
+    init.kkt_prev = std::numeric_limits<double>::infinity();
+    init.rel_kkt = std::numeric_limits<double>::infinity();
+    *s->host_state = init;
+    if (hipMemcpy(s->st, &init, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("vk_hip_pdhg_create: state upload failed");
+        return fail(VK_EDEVICE);
+    }
+    int rc;
+    if (s->mixed) {
+        rc = csr_make_f32(const_cast<vk_hip_csr *>(A), s->stream);
+        if (!rc) rc = setup_solver<float>(s, c, l, u, rl, ru, row_scale, col_scale, x0, y0);
+    } else {
+        rc = setup_solver<double>(s, c, l, u, rl, ru, row_scale, col_scale, x0, y0);
+    }
+    if (rc) return fail(rc);
+    *out = s;
+    return VK_OK;
+}
+
+int vk_hip_pdhg_run(vk_hip_pdhg *s, int64_t max_iterations, int *status) {
+    VK_REQUIRE(s && status && max_iterations >= 0, "vk_hip_pdhg_run: bad arguments");
+    return s->mixed ? run_solver<float>(s, max_iterations, status)
+                    : run_solver<double>(s, max_iterations, status);
+}
+
+int vk_hip_pdhg_read_state(vk_hip_pdhg *s, vk_pdhg_state *out) {
+    VK_REQUIRE(s && out, "vk_hip_pdhg_read_state: null argument");
+    VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+    VK_HIP_CHECK(hipStreamSynchronize(s->stream));
+    const DevState &d = *s->host_state;
+    std::memset(out, 0, sizeof(*out));
+    out->iterations = d.k;
+    out->status = d.status;
+    out->restarts = d.restarts;
+    out->tau = d.tau;
+    out->sigma = d.sigma;
+    out->primal_weight = d.omega;
+    out->metric = d.metric;
+    out->primal_scaled = d.rp;
+    out->dual_scaled = d.rd;
+    out->rel_primal = d.rel_p;
+    out->rel_dual = d.rel_d;
+    out->rel_gap = d.rel_g;
+    out->rel_kkt = d.rel_kkt;
+    out->primal_objective = d.pobj;
+    out->dual_objective = d.dobj;
+    out->converged_on_average = d.final_from_avg;
+    return VK_OK;
+}
+
+int vk_hip_pdhg_download(vk_hip_pdhg *s, double *x, double *y) {
+    VK_REQUIRE(s, "vk_hip_pdhg_download: null solver");
+    return s->mixed ? download_solution<float>(s, x, y) : download_solution<double>(s, x, y);
+}
+
+int vk_hip_pdhg_profile(vk_hip_pdhg *s, int64_t iterations, double *dual_ms, double *primal_ms,
+                        double *iteration_ms) {
+    VK_REQUIRE(s && dual_ms && primal_ms && iterations > 0, "vk_hip_pdhg_profile: bad arguments");
+    hipStream_t st = s->stream;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(3 * iterations + 1));
+    for (auto &e : ev) VK_HIP_CHECK(hipEventCreate(&e));
+    VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    long long k = s->host_state->k;
+    for (int64_t i = 0; i < iterations; ++i, ++k) {
+        VK_HIP_CHECK(hipEventRecord(ev[3 * i], st));
+        if (s->mixed) {
+            k_dual<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
+            VK_HIP_CHECK(hipEventRecord(ev[3 * i + 1], st));
+            k_primal<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s));
+        } else {
+            k_dual<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
+            VK_HIP_CHECK(hipEventRecord(ev[3 * i + 1], st));
+            k_primal<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s));
+        }
+        VK_HIP_CHECK(hipEventRecord(ev[3 * i + 2], st));
+        // keep the device schedule consistent (step-size / restart kernels)
+        if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
+            if (k > 0 && k % kDecisionPeriod == 0) {
+                if (s->mixed) k_pstep<float><<<egrid(s->n), kBlock, 0, st>>>(s->b32);
+                else k_pstep<double><<<egrid(s->n), kBlock, 0, st>>>(s->b64);
+            }
+        } else if ((k + 1) % s->params.restart_period == 0) {
+            const int ge = egrid(std::max(s->n, s->m));
+            if (s->mixed) {
+                k_eval_rows<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
+                k_eval_cols<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s), 0);
+                k_apply<float><<<ge, kBlock, 0, st>>>(s->b32);
+            } else {
+                k_eval_rows<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
+                k_eval_cols<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s), 0);
+                k_apply<double><<<ge, kBlock, 0, st>>>(s->b64);
+            }
+        }
+    }
+    VK_HIP_CHECK(hipEventRecord(ev[3 * iterations], st));
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    double dsum = 0.0, psum = 0.0;
+    for (int64_t i = 0; i < iterations; ++i) {
+        float a = 0.0f, b = 0.0f;
+        VK_HIP_CHECK(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+        VK_HIP_CHECK(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+        dsum += a;
+        psum += b;
+    }
+    float total = 0.0f;
+    VK_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[3 * iterations]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    *dual_ms = dsum / static_cast<double>(iterations);
+    *primal_ms = psum / static_cast<double>(iterations);
+    if (iteration_ms) *iteration_ms = total / static_cast<double>(iterations);
+    return VK_OK;
+}
+
+int vk_hip_pdhg_last_run_ms(const vk_hip_pdhg *s, double *ms) {
+    VK_REQUIRE(s && ms, "vk_hip_pdhg_last_run_ms: null argument");
+    *ms = s->last_ms;
+    return VK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Host-side runtime pieces shared by the C-ABI translation units.
+#pragma once
+
+#include "vikalp_hip.h"
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vk {
+
+void set_error(const std::string &message);
+hipStream_t default_stream();
+int ensure_init();
+
+inline hipStream_t as_stream(void *s) {
+    return s ? static_cast<hipStream_t>(s) : default_stream();
+}
+
+// Device-resident tiling of one CSR operand.
+template <class V>
+struct DeviceCsr {
+    int *row_ptr = nullptr;
+    int *col = nullptr;
+    V *val = nullptr;
+    int *tile_row = nullptr;
+    int rows = 0;
+    int ntiles = 0;
+    CsrView<V> view() const { return CsrView<V>{row_ptr, col, val, tile_row, rows, ntiles}; }
+};
+
+// Splits rows into tiles of whole rows with at most kTileNnz nonzeros and
+// kTileRows rows; a row longer than kTileNnz is a tile of its own.
+std::vector<int> build_tiles(const std::vector<int> &row_ptr, int rows);
+
+}  // namespace vk
+
+// The opaque matrix handle of the C ABI.
+struct vk_hip_csr {
+    std::int64_t rows = 0, cols = 0, nnz = 0;
+    vk::DeviceCsr<double> a, at;      // A and its stored transpose (FP64)
+    vk::DeviceCsr<float> a32, at32;   // FP32 copies (made on demand)
+    int *tperm = nullptr;             // at.val[k] = a.val[tperm[k]]
+    bool has_values = false;
+};
+
+#define VK_HIP_CHECK(expr)                                                      \
+    do {                                                                        \
+        hipError_t vk_err_ = (expr);                                            \
+        if (vk_err_ != hipSuccess) {                                            \
+            vk::set_error(std::string(#expr) + ": " + hipGetErrorString(vk_err_)); \
+            return VK_EDEVICE;                                                  \
+        }                                                                       \
+    } while (0)
+
+#define VK_REQUIRE(cond, msg)                 \
+    do {                                      \
+        if (!(cond)) {                        \
+            vk::set_error(msg);               \
+            return VK_EINVAL;                 \
+        }                                     \
+    } while (0)
+
+namespace vk {
+// Makes the FP32 copies of A and A' (used by the mixed-precision solver).
+int csr_make_f32(vk_hip_csr *A, hipStream_t stream);
+}

@@ -1,0 +1,189 @@
+// Flat C ABI over vikalp::solve / solver::solve_lp (include/vikalp_solve.h).
+
+#include "vikalp_solve.h"
+
+#include "vikalp/backend/HipBackend.hpp"
+#include "vikalp/solver/Solver.hpp"
+#include "vikalp/solver/continuous.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <span>
+#include <string>
+
+namespace {
+
+using namespace vikalp;
+
+SolverOptions to_options(const vk_solver_options &o) {
+    SolverOptions s;
+    s.backend = static_cast<BackendPreference>(o.backend);
+    s.continuous_method = static_cast<ContinuousMethod>(o.continuous_method);
+    s.iteration_limit = o.iteration_limit;
+    s.node_limit = o.node_limit;
+    s.time_limit_seconds = o.time_limit_seconds;
+    s.primal_tolerance = o.primal_tolerance;
+    s.dual_tolerance = o.dual_tolerance;
+    s.integrality_tolerance = o.integrality_tolerance;
+    s.absolute_gap_tolerance = o.absolute_gap_tolerance;
+    s.relative_gap_tolerance = o.relative_gap_tolerance;
+    s.deterministic = o.deterministic != 0;
+    s.pdhg_policy = o.pdhg_policy == 1 ? PdhgPolicy::Adaptive : PdhgPolicy::Reference;
+    s.pdhg_precision = o.pdhg_precision == 1 ? PdhgPrecision::Mixed : PdhgPrecision::FP64;
+    s.fused_iterations = o.fused_iterations != 0;
+    s.use_graphs = o.use_graphs != 0;
+    s.relative_kkt_tolerance = o.relative_kkt_tolerance;
+    s.mixed_phase_tolerance = o.mixed_phase_tolerance;
+    s.restart_period = o.restart_period;
+    return s;
+}
+
+void copy_text(char *dst, std::size_t cap, const std::string &src) {
+    const std::size_t n = std::min(cap - 1, src.size());
+    std::memcpy(dst, src.data(), n);
+    dst[n] = '\0';
+}
+
+void fill_info(const SolveResult &r, vk_solve_info &info) {
+    std::memset(&info, 0, sizeof(info));
+    info.status = static_cast<int>(r.status);
+    info.iterations = r.iterations;
+    info.low_precision_iterations = r.low_precision_iterations;
+    info.objective_value = r.objective_value;
+    info.primal_bound = r.primal_bound;
+    info.dual_bound = r.dual_bound;
+    info.primal_residual = r.primal_residual;
+    info.stationarity_residual = r.stationarity_residual;
+    info.complementarity_residual = r.complementarity_residual;
+    info.absolute_gap = r.absolute_gap;
+    info.relative_gap = r.relative_gap;
+    info.solve_seconds = r.solve_seconds;
+    info.device_seconds = r.device_seconds;
+    info.kkt_relative_primal = r.kkt_relative_primal;
+    info.kkt_relative_dual = r.kkt_relative_dual;
+    info.kkt_relative_gap = r.kkt_relative_gap;
+    info.kkt_relative = r.kkt_relative;
+    info.kkt_primal_objective = r.kkt_primal_objective;
+    info.kkt_dual_objective = r.kkt_dual_objective;
+    copy_text(info.backend, sizeof(info.backend), r.backend);
+    copy_text(info.solver, sizeof(info.solver), r.solver);
+    copy_text(info.message, sizeof(info.message), r.message);
+}
+
+}  // namespace
+
+extern "C" {
+
+void vk_solver_options_default(vk_solver_options *o) {
+    if (!o) return;
+    const SolverOptions d;
+    std::memset(o, 0, sizeof(*o));
+    o->backend = static_cast<int>(d.backend);
+    o->continuous_method = static_cast<int>(d.continuous_method);
+    o->iteration_limit = d.iteration_limit;
+    o->node_limit = d.node_limit;
+    o->time_limit_seconds = d.time_limit_seconds;
+    o->primal_tolerance = d.primal_tolerance;
+    o->dual_tolerance = d.dual_tolerance;
+    o->integrality_tolerance = d.integrality_tolerance;
+    o->absolute_gap_tolerance = d.absolute_gap_tolerance;
+    o->relative_gap_tolerance = d.relative_gap_tolerance;
+    o->deterministic = d.deterministic ? 1 : 0;
+    o->pdhg_policy = 0;
+    o->pdhg_precision = 0;
+    o->fused_iterations = d.fused_iterations ? 1 : 0;
+    o->use_graphs = d.use_graphs ? 1 : 0;
+    o->relative_kkt_tolerance = d.relative_kkt_tolerance;
+    o->mixed_phase_tolerance = d.mixed_phase_tolerance;
+    o->restart_period = d.restart_period;
+}
+
+static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,
+                        const int64_t *column_indices, const double *values, const double *objective,
+                        double objective_offset, const double *row_lower, const double *row_upper,
+                        const double *var_lower, const double *var_upper, Model &model) {
+    if (m < 0 || n < 0 || !row_offsets) return false;
+    if ((n > 0 && (!objective || !var_lower || !var_upper)) || (m > 0 && (!row_lower || !row_upper)))
+        return false;
+    const int64_t nnz = row_offsets[m];
+    if (nnz < 0 || (nnz > 0 && (!column_indices || !values))) return false;
+    model.name = "c-abi";
+    model.linear_objective.assign(objective, objective + n);
+    model.objective_offset = objective_offset;
+    model.variable_lower.assign(var_lower, var_lower + n);
+    model.variable_upper.assign(var_upper, var_upper + n);
+    model.variable_types.assign(static_cast<std::size_t>(n), VariableType::Continuous);
+    model.constraint_lower.assign(row_lower, row_lower + m);
+    model.constraint_upper.assign(row_upper, row_upper + m);
+    model.constraint_matrix.pattern.rows = m;
+    model.constraint_matrix.pattern.columns = n;
+    model.constraint_matrix.pattern.row_offsets.assign(row_offsets, row_offsets + m + 1);
+    model.constraint_matrix.pattern.column_indices.assign(column_indices, column_indices + nnz);
+    model.constraint_matrix.values.assign(values, values + nnz);
+    return true;
+}
+
+int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets, const int64_t *column_indices,
+                  const double *values, const double *objective, const double *row_lower,
+                  const double *row_upper, const double *var_lower, const double *var_upper,
+                  const vk_solver_options *options, int64_t warm_iterations, int64_t iterations,
+                  double *out) {
+    if (!options || !out) return -1;
+    Model model;
+    if (!build_model(m, n, row_offsets, column_indices, values, objective, 0.0, row_lower, row_upper,
+                     var_lower, var_upper, model))
+        return -1;
+    try {
+        const auto p = solver::profile_lp_kernels(model, to_options(*options), warm_iterations, iterations);
+        out[0] = p.dual_ms;
+        out[1] = p.primal_ms;
+        out[2] = p.iteration_ms;
+        out[3] = p.step_size;
+    } catch (const std::exception &e) {
+        return -2;
+    }
+    return 0;
+}
+
+int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
+                const int64_t *column_indices, const double *values, const double *objective,
+                double objective_offset, const double *row_lower, const double *row_upper,
+                const double *var_lower, const double *var_upper, const double *warm_start,
+                const double *lower_override, const double *upper_override,
+                const vk_solver_options *options, double *x_out, double *y_out, vk_solve_info *info) {
+    if (!info || !options) return -1;
+    Model model;
+    if (!build_model(m, n, row_offsets, column_indices, values, objective, objective_offset, row_lower,
+                     row_upper, var_lower, var_upper, model))
+        return -1;
+    const SolverOptions opts = to_options(*options);
+    SolveResult r;
+    if (mode == 0) {
+        r = solve(model, opts);
+    } else {
+        try {
+            auto backend = make_hip_backend();
+            const auto span_or_empty = [n](const double *p) {
+                return p ? std::span<const double>(p, static_cast<std::size_t>(n)) : std::span<const double>{};
+            };
+            r = solver::solve_lp(model, *backend, opts, span_or_empty(warm_start),
+                                 span_or_empty(lower_override), span_or_empty(upper_override));
+            r.backend = "hip";
+        } catch (const std::exception &e) {
+            r = SolveResult{};
+            r.status = SolveStatus::NumericalFailure;
+            r.message = e.what();
+        }
+    }
+    fill_info(r, *info);
+    if (x_out && r.primal_solution.size() == static_cast<std::size_t>(n))
+        std::memcpy(x_out, r.primal_solution.data(), sizeof(double) * static_cast<std::size_t>(n));
+    if (y_out && r.constraint_upper_dual.size() == static_cast<std::size_t>(m))
+        for (int64_t i = 0; i < m; ++i)
+            y_out[i] = r.constraint_upper_dual[static_cast<std::size_t>(i)] -
+                       r.constraint_lower_dual[static_cast<std::size_t>(i)];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+#pragma once
+// Solver options.  The first block is the reference contract
+// (include/vikalp/contracts/SolverOptions.hpp:7-28) field for field, with
+// BackendPreference::HIP added after CUDA; the HIP-path options are appended
+// after it with defaults, so existing aggregate initialisers keep compiling
+// and keep their meaning.
+
+#include "vikalp/contracts/Model.hpp"
+
+namespace vikalp {
+
+enum class BackendPreference { Auto, CPU, CUDA, HIP };
+enum class ContinuousMethod { Auto, PDHG, RevisedSimplex, InteriorPoint };
+
+// Step-size / restart rule of the PDHG loop.
+enum class PdhgPolicy {
+    Reference,   // the reference rule, src/solver/lp/pdhg.cpp:75-150
+    Adaptive     // restarted PDHG with averaging and a primal weight
+};
+
+// Arithmetic of the device iteration.
+enum class PdhgPrecision {
+    FP64,        // A, iterates and vectors in FP64 (the drop-in)
+    Mixed        // FP32 A and iterates, then a warm-started FP64 phase
+};
+
+struct SolverOptions {
+    BackendPreference backend = BackendPreference::Auto;
+    ContinuousMethod continuous_method = ContinuousMethod::Auto;
+    Index iteration_limit = 100'000;
+    Index node_limit = 100'000;
+    Scalar time_limit_seconds = Model::infinity();
+    // Feasibility and KKT residual limits in original, unscaled coordinates.
+    Scalar primal_tolerance = 1e-7;
+    Scalar dual_tolerance = 1e-7;
+    Scalar integrality_tolerance = 1e-6;
+    Scalar absolute_gap_tolerance = 1e-6;
+    Scalar relative_gap_tolerance = 1e-4;
+    bool deterministic = true;
+
+    // ---- HIP PDHG path -------------------------------------------------
+    PdhgPolicy pdhg_policy = PdhgPolicy::Reference;
+    PdhgPrecision pdhg_precision = PdhgPrecision::FP64;
+    // > 0: stop when the relative KKT error (original coordinates) is at or
+    // below this, instead of the reference's scaled infinity-norm test.
+    Scalar relative_kkt_tolerance = 0.0;
+    // Mixed precision: relative KKT target of the FP32 phase.
+    Scalar mixed_phase_tolerance = 1e-4;
+    // Adaptive policy: iterations between restart checks.
+    Index restart_period = 64;
+    // Run whole iterations on the device (two fused kernels) instead of the
+    // reference's op-by-op ExecutionBackend sequence.
+    bool fused_iterations = true;
+    // Capture iteration blocks in hipGraphs.
+    bool use_graphs = true;
+};
+
+}  // namespace vikalp

@@ -1,0 +1,31 @@
+#pragma once
+// Continuous LP entry point with the reference's signature
+// (include/vikalp/solver/continuous.hpp:10-15).
+
+#include "vikalp/contracts/ExecutionBackend.hpp"
+#include "vikalp/contracts/SolveResult.hpp"
+#include "vikalp/contracts/SolverOptions.hpp"
+
+#include <span>
+
+namespace vikalp::solver {
+
+// Restarted PDHG on `backend`.  On a HipBackend with
+// options.fused_iterations the whole iteration runs on the device; on any
+// other ExecutionBackend the reference's op-by-op sequence is issued.
+SolveResult solve_lp(const Model &model, ExecutionBackend &backend,
+                     const SolverOptions &options = {},
+                     std::span<const Scalar> warm_start = {},
+                     std::span<const Scalar> variable_lower_override = {},
+                     std::span<const Scalar> variable_upper_override = {});
+
+// Mean device time (ms, HIP events on the solver's stream) of the fused dual
+// and primal kernels and of a whole iteration, after `warm_iterations`
+// iterations, on the HIP backend.  For measurement only.
+struct KernelProfile {
+    double dual_ms = 0.0, primal_ms = 0.0, iteration_ms = 0.0, step_size = 0.0;
+};
+KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &options,
+                                 Index warm_iterations, Index iterations);
+
+}  // namespace vikalp::solver

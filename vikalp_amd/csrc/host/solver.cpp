@@ -1,0 +1,52 @@
+// vikalp::solve for the HIP library: the routing of src/solver/Solver.cpp:28-99
+// with BackendPreference::HIP.  Exceptions below become NumericalFailure
+// (Solver.cpp:81-86); the backend label is "hip" (Solver.cpp:79,96 use
+// "cpu"/"cuda").
+
+#include "vikalp/solver/Solver.hpp"
+
+#include "vikalp/backend/HipBackend.hpp"
+#include "vikalp/solver/continuous.hpp"
+
+#include <algorithm>
+#include <exception>
+#include <span>
+
+namespace vikalp {
+
+SolveResult solve(const Model &model, const SolverOptions &options) {
+    SolveResult result;
+    const auto errors = model.validate();
+    if (!errors.empty()) {
+        result.status = SolveStatus::InvalidModel;
+        result.message = errors.front();
+        return result;
+    }
+    if (options.backend == BackendPreference::CPU || options.backend == BackendPreference::CUDA) {
+        result.status = SolveStatus::UnsupportedModel;
+        result.message = "this library carries the HIP backend only; CPU and CUDA are served by the reference";
+        return result;
+    }
+    if (model.problem_class() != ProblemClass::LP) {
+        result.status = SolveStatus::UnsupportedModel;
+        result.message = "the HIP PDHG path solves LP models only";
+        return result;
+    }
+    try {
+        auto backend = make_hip_backend();
+        Model relaxation = model;
+        std::fill(relaxation.variable_types.begin(), relaxation.variable_types.end(),
+                  VariableType::Continuous);
+        result = solver::solve_lp(relaxation, *backend, options);
+        result.backend = "hip";
+    } catch (const std::exception &error) {
+        result = SolveResult{};
+        result.status = SolveStatus::NumericalFailure;
+        result.message = error.what();
+        result.backend = "hip";
+    }
+    result.solver = "vikalp-continuous";
+    return result;
+}
+
+}  // namespace vikalp

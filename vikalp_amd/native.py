@@ -1,0 +1,431 @@
+"""ctypes bindings of libvikalp_hip.so and libvikalp.so (see include/*.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+HIP_SO = os.path.join(LIB_DIR, "libvikalp_hip.so")
+HOST_SO = os.path.join(LIB_DIR, "libvikalp.so")
+
+_dp = C.POINTER(C.c_double)
+_lp = C.POINTER(C.c_int64)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+class BackendPreference(enum.IntEnum):
+    Auto = 0
+    CPU = 1
+    CUDA = 2
+    HIP = 3
+
+
+class ContinuousMethod(enum.IntEnum):
+    Auto = 0
+    PDHG = 1
+    RevisedSimplex = 2
+    InteriorPoint = 3
+
+
+class PdhgPolicy(enum.IntEnum):
+    Reference = 0
+    Adaptive = 1
+
+
+class PdhgPrecision(enum.IntEnum):
+    FP64 = 0
+    Mixed = 1
+
+
+class SolveStatus(enum.IntEnum):
+    NotSolved = 0
+    Optimal = 1
+    LocallyOptimal = 2
+    Feasible = 3
+    Infeasible = 4
+    Unbounded = 5
+    IterationLimit = 6
+    NodeLimit = 7
+    TimeLimit = 8
+    NumericalFailure = 9
+    UnsupportedModel = 10
+    InvalidModel = 11
+
+
+STATUS_NAMES = [s.name for s in SolveStatus]
+
+
+class _SolverOptions(C.Structure):
+    _fields_ = [
+        ("backend", C.c_int), ("continuous_method", C.c_int),
+        ("iteration_limit", C.c_int64), ("node_limit", C.c_int64),
+        ("time_limit_seconds", C.c_double), ("primal_tolerance", C.c_double),
+        ("dual_tolerance", C.c_double), ("integrality_tolerance", C.c_double),
+        ("absolute_gap_tolerance", C.c_double), ("relative_gap_tolerance", C.c_double),
+        ("deterministic", C.c_int), ("pdhg_policy", C.c_int), ("pdhg_precision", C.c_int),
+        ("fused_iterations", C.c_int), ("use_graphs", C.c_int), ("reserved", C.c_int),
+        ("relative_kkt_tolerance", C.c_double), ("mixed_phase_tolerance", C.c_double),
+        ("restart_period", C.c_int64),
+    ]
+
+
+class _SolveInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("reserved", C.c_int),
+        ("iterations", C.c_int64), ("low_precision_iterations", C.c_int64),
+        ("objective_value", C.c_double), ("primal_bound", C.c_double), ("dual_bound", C.c_double),
+        ("primal_residual", C.c_double), ("stationarity_residual", C.c_double),
+        ("complementarity_residual", C.c_double), ("absolute_gap", C.c_double),
+        ("relative_gap", C.c_double), ("solve_seconds", C.c_double), ("device_seconds", C.c_double),
+        ("kkt_relative_primal", C.c_double), ("kkt_relative_dual", C.c_double),
+        ("kkt_relative_gap", C.c_double), ("kkt_relative", C.c_double),
+        ("kkt_primal_objective", C.c_double), ("kkt_dual_objective", C.c_double),
+        ("backend", C.c_char * 16), ("solver", C.c_char * 32), ("message", C.c_char * 160),
+    ]
+
+
+class PdhgParams(C.Structure):
+    _fields_ = [
+        ("policy", C.c_int), ("precision", C.c_int), ("iteration_limit", C.c_int64),
+        ("metric_tolerance", C.c_double), ("relative_kkt_tolerance", C.c_double),
+        ("step_size", C.c_double), ("primal_weight", C.c_double), ("restart_period", C.c_int64),
+        ("norm_b", C.c_double), ("norm_c", C.c_double), ("use_graphs", C.c_int),
+        ("grid_blocks", C.c_int),
+    ]
+
+
+class PdhgState(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64), ("status", C.c_int), ("restarts", C.c_int),
+        ("tau", C.c_double), ("sigma", C.c_double), ("primal_weight", C.c_double),
+        ("metric", C.c_double), ("primal_scaled", C.c_double), ("dual_scaled", C.c_double),
+        ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
+        ("rel_kkt", C.c_double), ("primal_objective", C.c_double), ("dual_objective", C.c_double),
+        ("converged_on_average", C.c_int), ("reserved", C.c_int),
+    ]
+
+
+# Every exported symbol, with its ctypes signature (restype, argtypes).
+HIP_SYMBOLS = {
+    "vk_hip_init": (C.c_int, [C.c_int]),
+    "vk_hip_shutdown": (C.c_int, []),
+    "vk_hip_last_error": (C.c_char_p, []),
+    "vk_hip_stream": (_vp, []),
+    "vk_hip_alloc": (C.c_int, [C.POINTER(_vp), C.c_size_t]),
+    "vk_hip_free": (C.c_int, [_vp]),
+    "vk_hip_memcpy_h2d": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "vk_hip_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "vk_hip_memcpy_d2d": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "vk_hip_sync": (C.c_int, [_vp]),
+    "vk_hip_fill": (C.c_int, [_vp, C.c_int64, C.c_double, _vp]),
+    "vk_hip_axpby": (C.c_int, [C.c_int64, C.c_double, _vp, C.c_double, _vp, _vp]),
+    "vk_hip_project_box": (C.c_int, [C.c_int64, _vp, _vp, _vp, _vp]),
+    "vk_hip_reduce_absmax": (C.c_int, [C.c_int64, _vp, _dp, _vp]),
+    "vk_hip_reduce_dot": (C.c_int, [C.c_int64, _vp, _vp, _dp, _vp]),
+    "vk_hip_csr_create": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, C.POINTER(_vp)]),
+    "vk_hip_csr_set_values": (C.c_int, [_vp, _dp]),
+    "vk_hip_csr_destroy": (C.c_int, [_vp]),
+    "vk_hip_csr_dims": (C.c_int, [_vp, _lp, _lp, _lp]),
+    "vk_hip_csr_transpose_export": (C.c_int, [_vp, _ip, _ip, _dp]),
+    "vk_hip_spmv_csr": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp, _vp]),
+    "vk_hip_csr_norm2_estimate": (C.c_int, [_vp, C.c_int, _dp]),
+    "vk_hip_pdhg_create": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     C.POINTER(PdhgParams), C.POINTER(_vp)]),
+    "vk_hip_pdhg_run": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int)]),
+    "vk_hip_pdhg_read_state": (C.c_int, [_vp, C.POINTER(PdhgState)]),
+    "vk_hip_pdhg_download": (C.c_int, [_vp, _dp, _dp]),
+    "vk_hip_pdhg_last_run_ms": (C.c_int, [_vp, _dp]),
+    "vk_hip_pdhg_profile": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp]),
+    "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
+}
+
+HOST_SYMBOLS = {
+    "vk_solver_options_default": (None, [C.POINTER(_SolverOptions)]),
+    "vk_lp_solve": (C.c_int, [C.c_int, C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, C.c_double,
+                              _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(_SolverOptions),
+                              _dp, _dp, C.POINTER(_SolveInfo)]),
+    "vk_lp_profile": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                C.POINTER(_SolverOptions), C.c_int64, C.c_int64, _dp]),
+    "vk_lp_generate": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_vp)]),
+    "vk_lp_dims": (C.c_int, [_vp, _lp, _lp, _lp]),
+    "vk_lp_export": (C.c_int, [_vp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "vk_lp_free": (None, [_vp]),
+    "vk_lp_block_periods": (C.c_int64, [C.c_uint64, C.c_int64]),
+}
+
+_hip = None
+_host = None
+
+
+def _bind(lib, table):
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def hip_lib():
+    """libvikalp_hip.so (raises if it was not built)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_SO):
+            raise RuntimeError(f"{HIP_SO} is missing: run `python -m vikalp_amd.build`")
+        lib = C.CDLL(HIP_SO)
+        _bind(lib, HIP_SYMBOLS)
+        _hip = lib
+    return _hip
+
+
+def host_lib():
+    """libvikalp.so (raises if it was not built)."""
+    global _host
+    if _host is None:
+        hip_lib()
+        if not os.path.exists(HOST_SO):
+            raise RuntimeError(f"{HOST_SO} is missing: run `python -m vikalp_amd.build`")
+        lib = C.CDLL(HOST_SO)
+        _bind(lib, HOST_SYMBOLS)
+        _host = lib
+    return _host
+
+
+def load_libraries():
+    return hip_lib(), host_lib()
+
+
+def _ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = hip_lib().vk_hip_last_error()
+        raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+@dataclass
+class LP:
+    """An LP in the reference's canonical form (CSR, int64 indices)."""
+    m: int
+    n: int
+    row_offsets: np.ndarray
+    column_indices: np.ndarray
+    values: np.ndarray
+    objective: np.ndarray
+    row_lower: np.ndarray
+    row_upper: np.ndarray
+    var_lower: np.ndarray
+    var_upper: np.ndarray
+    objective_offset: float = 0.0
+    x0: np.ndarray | None = None
+    y0: np.ndarray | None = None
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_offsets[-1])
+
+    def as_oracle(self):
+        from oracle import oracle as orc  # test/bench only
+        return orc.LP(self.m, self.n, self.row_offsets, self.column_indices, self.values,
+                      self.objective, self.row_lower, self.row_upper, self.var_lower,
+                      self.var_upper, self.objective_offset)
+
+
+def generate_lp(family: int, m: int = 0, n: int = 0, k: int = 0, seed: int = 20261203) -> LP:
+    """Seeded generator of include/vikalp_lpgen.h (family 1 uniform, 2 zipf, 4 block)."""
+    L = host_lib()
+    h = _vp()
+    rc = L.vk_lp_generate(family, m, n, k, seed, C.byref(h))
+    if rc != 0:
+        raise ValueError(f"vk_lp_generate({family}, {m}, {n}, {k}) failed with {rc}")
+    try:
+        mm, nn, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        L.vk_lp_dims(h, C.byref(mm), C.byref(nn), C.byref(nz))
+        m, n, nnz = mm.value, nn.value, nz.value
+        a = dict(rp=np.zeros(m + 1, np.int64), ci=np.zeros(nnz, np.int64), v=np.zeros(nnz),
+                 c=np.zeros(n), rl=np.zeros(m), ru=np.zeros(m), l=np.zeros(n), u=np.zeros(n),
+                 x0=np.zeros(n), y0=np.zeros(m))
+        L.vk_lp_export(h, _ptr(a["rp"], _lp), _ptr(a["ci"], _lp), _ptr(a["v"]), _ptr(a["c"]),
+                       _ptr(a["rl"]), _ptr(a["ru"]), _ptr(a["l"]), _ptr(a["u"]), _ptr(a["x0"]),
+                       _ptr(a["y0"]))
+    finally:
+        L.vk_lp_free(h)
+    return LP(m, n, a["rp"], a["ci"], a["v"], a["c"], a["rl"], a["ru"], a["l"], a["u"],
+              0.0, a["x0"], a["y0"])
+
+
+@dataclass
+class Options:
+    """vikalp::SolverOptions (+ the HIP-path fields)."""
+    backend: int = BackendPreference.Auto
+    continuous_method: int = ContinuousMethod.Auto
+    iteration_limit: int = 100_000
+    node_limit: int = 100_000
+    time_limit_seconds: float = math.inf
+    primal_tolerance: float = 1e-7
+    dual_tolerance: float = 1e-7
+    integrality_tolerance: float = 1e-6
+    absolute_gap_tolerance: float = 1e-6
+    relative_gap_tolerance: float = 1e-4
+    deterministic: bool = True
+    pdhg_policy: int = PdhgPolicy.Reference
+    pdhg_precision: int = PdhgPrecision.FP64
+    fused_iterations: bool = True
+    use_graphs: bool = True
+    relative_kkt_tolerance: float = 0.0
+    mixed_phase_tolerance: float = 1e-4
+    restart_period: int = 64
+
+    def _c(self) -> _SolverOptions:
+        o = _SolverOptions()
+        for name, _ in _SolverOptions._fields_:
+            if name == "reserved":
+                continue
+            v = getattr(self, name)
+            setattr(o, name, int(v) if isinstance(v, (bool, enum.IntEnum)) else v)
+        return o
+
+
+@dataclass
+class Result:
+    status: str
+    iterations: int
+    objective_value: float
+    primal_residual: float
+    stationarity_residual: float
+    complementarity_residual: float
+    absolute_gap: float
+    relative_gap: float
+    solve_seconds: float
+    device_seconds: float
+    kkt_relative_primal: float
+    kkt_relative_dual: float
+    kkt_relative_gap: float
+    kkt_relative: float
+    kkt_primal_objective: float
+    kkt_dual_objective: float
+    low_precision_iterations: int
+    backend: str
+    solver: str
+    message: str
+    x: np.ndarray = field(repr=False, default=None)
+    y: np.ndarray = field(repr=False, default=None)
+
+
+def _solve(mode, lp: LP, options: Options, warm_start=None, lower_override=None,
+           upper_override=None) -> Result:
+    L = host_lib()
+    keep = [np.ascontiguousarray(lp.row_offsets, dtype=np.int64),
+            np.ascontiguousarray(lp.column_indices, dtype=np.int64)]
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in
+         (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
+    opt = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+           for a in (warm_start, lower_override, upper_override)]
+    x = np.zeros(lp.n)
+    y = np.zeros(lp.m)
+    info = _SolveInfo()
+    copt = options._c()
+    rc = L.vk_lp_solve(mode, lp.m, lp.n, _ptr(keep[0], _lp), _ptr(keep[1], _lp), _ptr(f[0]),
+                       _ptr(f[1]), float(lp.objective_offset), _ptr(f[2]), _ptr(f[3]), _ptr(f[4]),
+                       _ptr(f[5]), _ptr(opt[0]), _ptr(opt[1]), _ptr(opt[2]), C.byref(copt),
+                       _ptr(x), _ptr(y), C.byref(info))
+    if rc != 0:
+        raise ValueError(f"vk_lp_solve rejected its arguments ({rc})")
+    return Result(STATUS_NAMES[info.status], info.iterations, info.objective_value,
+                  info.primal_residual, info.stationarity_residual, info.complementarity_residual,
+                  info.absolute_gap, info.relative_gap, info.solve_seconds, info.device_seconds,
+                  info.kkt_relative_primal, info.kkt_relative_dual, info.kkt_relative_gap,
+                  info.kkt_relative, info.kkt_primal_objective, info.kkt_dual_objective,
+                  info.low_precision_iterations, info.backend.decode(), info.solver.decode(),
+                  info.message.decode(), x, y)
+
+
+def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int = 50,
+                    iterations: int = 50) -> dict:
+    """Mean fused-kernel device times (ms) measured with HIP events (vk_lp_profile)."""
+    L = host_lib()
+    rp = np.ascontiguousarray(lp.row_offsets, dtype=np.int64)
+    ci = np.ascontiguousarray(lp.column_indices, dtype=np.int64)
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in
+         (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
+    out = np.zeros(4)
+    copt = (options or Options())._c()
+    rc = L.vk_lp_profile(lp.m, lp.n, _ptr(rp, _lp), _ptr(ci, _lp), *[_ptr(a) for a in f],
+                         C.byref(copt), int(warm_iterations), int(iterations), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"vk_lp_profile failed ({rc}): {hip_lib().vk_hip_last_error()}")
+    return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3]}
+
+
+def solve(lp: LP, options: Options | None = None) -> Result:
+    """vikalp::solve(model, options) (Solver.cpp:92-99) on the HIP library."""
+    return _solve(0, lp, options or Options())
+
+
+def solve_lp(lp: LP, options: Options | None = None, warm_start=None, lower_override=None,
+             upper_override=None) -> Result:
+    """vikalp::solver::solve_lp(model, hip_backend, options, warm, lo, hi) (pdhg.cpp:269-293)."""
+    return _solve(1, lp, options or Options(), warm_start, lower_override, upper_override)
+
+
+class Device:
+    """Raw access to the kernel C ABI (include/vikalp_hip.h) with host numpy arrays."""
+
+    def __init__(self):
+        self.L = hip_lib()
+        check(self.L.vk_hip_init(0), "vk_hip_init")
+        self._owned = []
+
+    def alloc(self, count: int, itemsize: int = 8) -> int:
+        p = _vp()
+        check(self.L.vk_hip_alloc(C.byref(p), max(1, count) * itemsize), "vk_hip_alloc")
+        self._owned.append(p.value)
+        return p.value
+
+    def put(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        p = self.alloc(arr.size)
+        check(self.L.vk_hip_memcpy_h2d(p, arr.ctypes.data, arr.nbytes, None), "h2d")
+        return p
+
+    def get(self, p: int, count: int) -> np.ndarray:
+        out = np.zeros(count)
+        check(self.L.vk_hip_memcpy_d2h(out.ctypes.data, p, out.nbytes, None), "d2h")
+        return out
+
+    def free_all(self):
+        for p in self._owned:
+            self.L.vk_hip_free(p)
+        self._owned = []
+
+    def csr(self, rows, cols, row_offsets, column_indices, values=None) -> int:
+        h = _vp()
+        rp = np.ascontiguousarray(row_offsets, dtype=np.int64)
+        ci = np.ascontiguousarray(column_indices, dtype=np.int64)
+        check(self.L.vk_hip_csr_create(rows, cols, _ptr(rp, _lp), _ptr(ci, _lp), C.byref(h)),
+              "vk_hip_csr_create")
+        if values is not None:
+            v = np.ascontiguousarray(values, dtype=np.float64)
+            check(self.L.vk_hip_csr_set_values(h, _ptr(v)), "vk_hip_csr_set_values")
+        return h.value
+
+    def spmv(self, h, transpose, alpha, x, beta, y):
+        check(self.L.vk_hip_spmv_csr(h, int(transpose), float(alpha), x, float(beta), y, None), "spmv")
+        check(self.L.vk_hip_sync(None), "sync")
+
+    def absmax(self, p, n) -> float:
+        r = C.c_double()
+        check(self.L.vk_hip_reduce_absmax(n, p, C.byref(r), None), "absmax")
+        return r.value
+
+    def dot(self, p, q, n) -> float:
+        r = C.c_double()
+        check(self.L.vk_hip_reduce_dot(n, p, q, C.byref(r), None), "dot")
+        return r.value
