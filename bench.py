@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--time-limit", type=float, default=100.0, help="seconds per solve")
     ap.add_argument("--cpu-iterations", type=int, default=12)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-solve", action="store_true",
+                    help="kernel timing only (used for the rocprofv3 pass)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and the PDHG path "
@@ -111,10 +113,16 @@ def main():
             f"obj={r.objective_value:.10g}")
         return r
 
-    for _ in range(args.warmup):
+    for _ in range(args.warmup if not args.skip_solve else 0):
         run(V.PdhgPrecision.Mixed, "warmup")
     fp64_runs, mixed_runs = [], []
-    for _ in range(args.steps):
+    if args.skip_solve:
+        args.skip_cpu = True
+        empty = V.Result("NotSolved", 0, math.nan, math.inf, math.inf, math.inf, math.inf, math.inf,
+                         0.0, 0.0, math.inf, math.inf, math.inf, math.inf, math.nan, math.nan, 0,
+                         "hip", "", "solve skipped")
+        fp64_runs, mixed_runs = [empty], [empty]
+    for _ in range(args.steps if not args.skip_solve else 0):
         fp64_runs.append(run(V.PdhgPrecision.FP64, "fp64"))
         mixed_runs.append(run(V.PdhgPrecision.Mixed, "mixed"))
 

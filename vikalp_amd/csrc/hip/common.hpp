@@ -18,14 +18,14 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWaves = kBlock / kWave;
 
-// Tile geometry of the CSR-adaptive SpMV: a tile is a run of whole rows whose
-// nonzeros fit in kTileNnz LDS slots (or one row longer than that), with at
-// most kTileRows rows so that per-row sums fit in LDS too.
-constexpr int kTileNnz = 2048;
-constexpr int kTileRows = 1024;
+// Tile geometry of the CSR-adaptive SpMV: a tile is a run of whole rows with
+// at most kTileNnz nonzeros (8 per lane of a wave64) and at most kTileRows
+// rows (one per lane), or a single row longer than kTileNnz.  Each tile is
+// processed by ONE wave with a private LDS slice, so waves never wait on a
+// block barrier inside the SpMV.
+constexpr int kTileNnz = 512;
+constexpr int kTileRows = 64;
 
-// std::min(std::max(v, lo), hi) with the reference's argument order, so the
-// NaN behaviour and the choice between equal values match the CPU path.
 template <class T>
 __device__ __forceinline__ T clamp_box(T v, T lo, T hi) {
     T t = (v < lo) ? lo : v;
@@ -77,11 +77,19 @@ __device__ __forceinline__ double block_max(double v, double *scratch) {
 }
 
 // Lanes per row for a tile of `rows` rows: the largest power of two with
-// lanes*rows <= kBlock, capped at one wave so a group never spans waves.
+// lanes*rows <= kWave.
 __device__ __forceinline__ int lanes_per_row(int rows) {
     int g = 1;
-    while (g < kWave && g * 2 * rows <= kBlock) g *= 2;
+    while (g < kWave && g * 2 * rows <= kWave) g *= 2;
     return g;
+}
+
+// Orders this wave's LDS writes before its later LDS reads (and vice versa)
+// without a block barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Device-side view of one CSR operand (A or its stored transpose).
@@ -90,7 +98,7 @@ struct CsrView {
     const int *row_ptr;     // rows+1
     const int *col;         // nnz
     const V *val;           // nnz
-    const int *tile_row;    // ntiles+1, first row of each tile
+    const int2 *tiles;      // ntiles+1: {first row, first nonzero} of each tile
     int rows;
     int ntiles;
 };
@@ -117,67 +125,127 @@ __device__ __forceinline__ void store_partials(const Partials &p, double *partia
     }
 }
 
-// Reduce `nblocks` per-block partials in a fixed order; result in out[0..7]
-// (valid in every thread after the call).
-__device__ __forceinline__ void reduce_partials(const double *partials, int nblocks,
+// Reduce `nblocks` (<= 4 * blockDim.x) per-block partials in a fixed order;
+// result in out[0..7] (valid in every thread after the call).  All loads of
+// a thread are issued before any is used.  blockDim.x must be kFinalBlock.
+constexpr int kFinalBlock = 1024;
+constexpr int kMaxGrid = 4 * kFinalBlock;
+
+__device__ __forceinline__ double final_sum(double v, double *scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < kFinalBlock / kWave; ++w) r += scratch[w];
+    return r;
+}
+
+__device__ __forceinline__ double final_max(double v, double *scratch) {
+    v = wave_max(v);
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double r = scratch[0];
+    for (int w = 1; w < kFinalBlock / kWave; ++w) r = dmax(r, scratch[w]);
+    return r;
+}
+
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ partials, int nblocks,
                                                 double *out, double *scratch) {
-    for (int s = 0; s < kSlots; ++s) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += kBlock) {
-            const double v = partials[s * nblocks + b];
-            acc = (s == 0) ? dmax(acc, v) : acc + v;
+    double v[kSlots][4];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = threadIdx.x + u * kFinalBlock;
+            v[s][u] = b < nblocks ? partials[s * nblocks + b] : 0.0;
         }
-        const double r = (s == 0) ? block_max(acc, scratch) : block_sum(acc, scratch);
-        if (threadIdx.x == 0) out[s] = r;
-        __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        double acc = v[s][0];
+#pragma unroll
+        for (int u = 1; u < 4; ++u) acc = (s == 0) ? dmax(acc, v[s][u]) : acc + v[s][u];
+        const double r = (s == 0) ? final_max(acc, scratch) : final_sum(acc, scratch);
+        out[s] = r;
     }
 }
 
-// CSR-adaptive SpMV over the tiles assigned to this block (grid-stride).
-// Phase 1 streams the tile's nonzeros (coalesced) and stores val*x[col] in
-// LDS; phase 2 reduces each row with a fixed lane group; phase 3 hands each
-// row sum to the epilogue with consecutive threads on consecutive rows.
-// Rows longer than kTileNnz form a tile of their own and are reduced by the
-// whole block.  Epi::row(row, acc, partials) sees every row exactly once.
+// CSR-adaptive SpMV, one tile per wave (grid-stride over waves).
+//
+//   header   one 16-byte load gives the tile's first/last row and nonzero;
+//   phase 1  every lane issues its (up to) 8 column and value loads, the row
+//            offsets of the tile and the epilogue's per-row prefetch at once,
+//            then its 8 gathers of x, then 8 products into the wave's private
+//            LDS slice -- explicit batching so no load waits on another;
+//   phase 2  each row is reduced by a fixed power-of-two lane group and a
+//            shuffle tree, and row r's sum moves to lane r;
+//   phase 3  Epi::row on consecutive lanes for consecutive rows.
+// A row longer than kTileNnz is a tile of its own, summed by the whole wave
+// straight from memory.  No block barrier, no atomics; the summation order
+// depends only on the tiling, so results repeat bitwise.
+//
+// Epi provides `Pre prefetch(int row) const` (loads issued before the
+// gathers) and `row(int row, double acc, const Pre &, Partials &)`; every row
+// is seen exactly once.
 template <class V, class X, class Epi>
 __device__ __forceinline__ void tile_spmv(const CsrView<V> &A, const X *__restrict__ x,
-                                          Epi &epi, Partials &part, double *lds_prod,
-                                          double *lds_rows, double *scratch) {
-    for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
-        const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
-        const int k0 = A.row_ptr[r0], k1 = A.row_ptr[r1];
+                                          Epi &epi, Partials &part, double *lds_generic) {
+    using Pre = typename Epi::Pre;
+    // The slice is LDS: say so, so LDS stores do not order global loads.
+    auto *lds = (__attribute__((address_space(3))) double *)(lds_generic);
+    constexpr int kPer = kTileNnz / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int nwaves = gridDim.x * kWaves;
+    const int *__restrict__ colp = A.col;
+    const V *__restrict__ valp = A.val;
+    for (int t = blockIdx.x * kWaves + static_cast<int>(threadIdx.x) / kWave; t < A.ntiles; t += nwaves) {
+        const int2 head = A.tiles[t], tail = A.tiles[t + 1];
+        const int r0 = head.x, r1 = tail.x, k0 = head.y, k1 = tail.y;
         const int nnz = k1 - k0;
         const int nr = r1 - r0;
         if (nnz <= kTileNnz) {
-            for (int k = threadIdx.x; k < nnz; k += kBlock) {
-                const int kk = k0 + k;
-                lds_prod[k] = static_cast<double>(A.val[kk]) * static_cast<double>(x[A.col[kk]]);
+            int c[kPer];
+            V v[kPer];
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int k = lane + u * kWave;
+                c[u] = k < nnz ? colp[k0 + k] : 0;
+                v[u] = k < nnz ? valp[k0 + k] : V(0);
             }
-            __syncthreads();
+            const int rs = A.row_ptr[r0 + (lane < nr ? lane : nr)] - k0;   // lane nr: tile end
+            Pre pre{};
+            if (lane < nr) pre = epi.prefetch(r0 + lane);
+            X xv[kPer];
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) xv[u] = (lane + u * kWave < nnz) ? x[c[u]] : X(0);
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int k = lane + u * kWave;
+                if (k < nnz) lds[k] = static_cast<double>(v[u]) * static_cast<double>(xv[u]);
+            }
+            wave_sync();
             const int g = lanes_per_row(nr);
-            for (int base = 0; base < nr * g; base += kBlock) {
-                const int idx = base + threadIdx.x;
-                const int lr = idx / g, lane = idx % g;
-                double acc = 0.0;
-                if (lr < nr) {
-                    const int s = A.row_ptr[r0 + lr] - k0, e = A.row_ptr[r0 + lr + 1] - k0;
-                    for (int k = s + lane; k < e; k += g) acc += lds_prod[k];
-                }
-                for (int off = g / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, g);
-                if (lane == 0 && lr < nr) lds_rows[lr] = acc;
-            }
-            __syncthreads();
-            for (int lr = threadIdx.x; lr < nr; lr += kBlock) epi.row(r0 + lr, lds_rows[lr], part);
-            __syncthreads();
-        } else {
+            const int lr = lane / g, sub = lane % g;
+            const int s = __shfl(rs, lr < nr ? lr : nr, kWave);
+            const int e = __shfl(rs, lr < nr ? lr + 1 : nr, kWave);
             double acc = 0.0;
-            for (int k = k0 + threadIdx.x; k < k1; k += kBlock)
-                acc += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
-            acc = block_sum(acc, scratch);
-            if (threadIdx.x == 0) lds_rows[0] = acc;
-            __syncthreads();
-            if (threadIdx.x == 0) epi.row(r0, lds_rows[0], part);
-            __syncthreads();
+            if (lr < nr)
+                for (int k = s + sub; k < e; k += g) acc += lds[k];
+            for (int off = g / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, g);
+            const double row_sum = __shfl(acc, (lane * g) % kWave, kWave);
+            wave_sync();   // all lanes are done with the slice before the next tile
+            if (lane < nr) epi.row(r0 + lane, row_sum, pre, part);
+        } else {
+            Pre pre{};
+            if (lane == 0) pre = epi.prefetch(r0);
+            double acc = 0.0;
+            for (int k = k0 + lane; k < k1; k += kWave)
+                acc += static_cast<double>(valp[k]) * static_cast<double>(x[colp[k]]);
+            acc = wave_sum(acc);
+            if (lane == 0) epi.row(r0, acc, pre, part);
         }
     }
 }

@@ -46,25 +46,27 @@ int ensure_init() {
     return VK_OK;
 }
 
-std::vector<int> build_tiles(const std::vector<int> &row_ptr, int rows) {
-    std::vector<int> tiles{0};
+std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows) {
+    std::vector<int> starts{0};
     int cur_nnz = 0, cur_rows = 0;
     for (int r = 0; r < rows; ++r) {
         const int len = row_ptr[r + 1] - row_ptr[r];
         if (cur_rows > 0 && (cur_nnz + len > kTileNnz || cur_rows == kTileRows)) {
-            tiles.push_back(r);
+            starts.push_back(r);
             cur_nnz = 0;
             cur_rows = 0;
         }
         cur_nnz += len;
         ++cur_rows;
         if (len > kTileNnz) {
-            tiles.push_back(r + 1);
+            starts.push_back(r + 1);
             cur_nnz = 0;
             cur_rows = 0;
         }
     }
-    if (cur_rows > 0) tiles.push_back(rows);
+    if (cur_rows > 0) starts.push_back(rows);
+    std::vector<int2> tiles(starts.size());
+    for (size_t i = 0; i < starts.size(); ++i) tiles[i] = make_int2(starts[i], row_ptr[starts[i]]);
     return tiles;
 }
 
@@ -79,7 +81,10 @@ __global__ void k_fill(double *__restrict__ y, int64_t n, double v) {
 }
 
 // x may alias y (pdhg.cpp:109 calls axpby(a, y, 0, y)), so no __restrict__.
+// Two roundings (product, then sum), no fused multiply-add: the same
+// arithmetic as the CPU backend's loop (cpu_ops.cpp:341-342).
 __global__ void k_axpby(int64_t n, double alpha, const double *x, double beta, double *y) {
+#pragma clang fp contract(off)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         y[i] = alpha * x[i] + beta * y[i];
@@ -98,6 +103,7 @@ template <bool kDot>
 __global__ __launch_bounds__(kBlock) void k_reduce_stage1(int64_t n, const double *__restrict__ x,
                                                           const double *__restrict__ y,
                                                           double *__restrict__ partials) {
+#pragma clang fp contract(off)
     __shared__ double scratch[kWaves];
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
@@ -123,18 +129,21 @@ __global__ __launch_bounds__(kBlock) void k_reduce_stage2(int nblocks, const dou
 struct PlainEpi {
     double alpha, beta;
     double *y;
-    __device__ void row(int r, double acc, Partials &) const { y[r] = alpha * acc + beta * y[r]; }
+    using Pre = double;
+    __device__ Pre prefetch(int r) const { return y[r]; }
+    __device__ void row(int r, double acc, const Pre &y_old, Partials &) const {
+#pragma clang fp contract(off)
+        y[r] = alpha * acc + beta * y_old;   // cpu_ops.cpp:426, two roundings
+    }
 };
 
 __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<double> A, const double *__restrict__ x,
                                                  double alpha, double beta, double *y) {
-    __shared__ double lds_prod[kTileNnz];
-    __shared__ double lds_rows[kTileRows];
-    __shared__ double scratch[kWaves];
+    __shared__ double lds[kWaves][kTileNnz];
     PlainEpi epi{alpha, beta, y};
     Partials part;
     part.zero();
-    tile_spmv(A, x, epi, part, lds_prod, lds_rows, scratch);
+    tile_spmv(A, x, epi, part, lds[threadIdx.x / kWave]);
 }
 
 __global__ void k_gather(int64_t n, const double *__restrict__ src, const int *__restrict__ perm,
@@ -196,12 +205,12 @@ int upload_csr(DeviceCsr<V> &d, const std::vector<int> &rp, const std::vector<in
     VK_HIP_CHECK(hipMalloc(&d.row_ptr, sizeof(int) * rp.size()));
     VK_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * std::max<size_t>(1, ci.size())));
     VK_HIP_CHECK(hipMalloc(&d.val, sizeof(V) * std::max<size_t>(1, ci.size())));
-    VK_HIP_CHECK(hipMalloc(&d.tile_row, sizeof(int) * tiles.size()));
+    VK_HIP_CHECK(hipMalloc(&d.tiles, sizeof(int2) * tiles.size()));
     VK_HIP_CHECK(hipMemcpyAsync(d.row_ptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice, s));
     if (!ci.empty())
         VK_HIP_CHECK(hipMemcpyAsync(d.col, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice, s));
     VK_HIP_CHECK(hipMemsetAsync(d.val, 0, sizeof(V) * std::max<size_t>(1, ci.size()), s));
-    VK_HIP_CHECK(hipMemcpyAsync(d.tile_row, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, s));
+    VK_HIP_CHECK(hipMemcpyAsync(d.tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
     VK_HIP_CHECK(hipStreamSynchronize(s));
     return VK_OK;
 }
@@ -211,11 +220,11 @@ void free_csr(DeviceCsr<V> &d) {
     (void)hipFree(d.row_ptr);
     (void)hipFree(d.col);
     (void)hipFree(d.val);
-    (void)hipFree(d.tile_row);
+    (void)hipFree(d.tiles);
     d = DeviceCsr<V>{};
 }
 
-int spmv_grid(int ntiles) { return std::max(1, std::min(ntiles, 4096)); }
+int spmv_grid(int ntiles) { return std::max(1, std::min((ntiles + kWaves - 1) / kWaves, 4096)); }
 
 }  // namespace
 
@@ -225,7 +234,7 @@ int csr_make_f32(vk_hip_csr *A, hipStream_t s) {
     auto shallow = [&](const DeviceCsr<double> &src, DeviceCsr<float> &dst) -> int {
         dst.row_ptr = src.row_ptr;
         dst.col = src.col;
-        dst.tile_row = src.tile_row;
+        dst.tiles = src.tiles;
         dst.rows = src.rows;
         dst.ntiles = src.ntiles;
         VK_HIP_CHECK(hipMalloc(&dst.val, sizeof(float) * nnz));

@@ -72,7 +72,6 @@ struct DevState {
     int restart_to_avg;     // adaptive: ... to the average
     int final_from_avg;     // adaptive: converged on the average
     int restarts;
-    unsigned int counter;   // last-block detection
     double tau, sigma, inv_sigma, eta, omega, max_step;
     double prev_metric, metric, rp, rd;
     double metric_tol, rel_tol, norm_b, norm_c;
@@ -114,20 +113,45 @@ struct DualEpi {
     const Bufs<T> &b;
     double sigma, inv_sigma;
     bool restart;
-    __device__ void row(int i, double acc, Partials &p) const {
+    bool sign_exact;
+    struct Pre {
+        T ax, y, lo, hi, unscale, ysum, ylast;
+    };
+    __device__ Pre prefetch(int i) const {
+        Pre p;
+        p.ax = b.ax[i];
+        p.y = b.y[i];
+        p.lo = b.rl[i];
+        p.hi = b.ru[i];
+        p.unscale = b.row_unscale[i];
+        p.ysum = b.ysum ? b.ysum[i] : T(0);
+        p.ylast = b.ylast ? b.ylast[i] : T(0);
+        return p;
+    }
+    __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
         const double axn = acc;
-        const double ax_old = static_cast<double>(b.ax[i]);
+        const double ax_old = static_cast<double>(q.ax);
         const double axbar = restart ? axn : 2.0 * axn - ax_old;
-        const double lo = static_cast<double>(b.rl[i]), hi = static_cast<double>(b.ru[i]);
-        const double v = static_cast<double>(b.y[i]) + sigma * axbar;
-        const double proj = clamp_box(inv_sigma * v, lo, hi);
-        const double yn = v - sigma * proj;
+        const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
+        const double v = static_cast<double>(q.y) + sigma * axbar;
+        double yn;
+        if (sign_exact) {
+            // Same Moreau step, written so that y+ is exactly 0 inside the
+            // box and carries the sign of the active bound outside it (a
+            // one-sided row never gets a dual of the wrong sign from
+            // round-off, which would make the dual objective -inf).
+            const double t = inv_sigma * v;
+            yn = (t < lo) ? fmin(v - sigma * lo, 0.0) : ((hi < t) ? fmax(v - sigma * hi, 0.0) : 0.0);
+        } else {
+            const double proj = clamp_box(inv_sigma * v, lo, hi);   // pdhg.cpp:108-112
+            yn = v - sigma * proj;
+        }
         b.y[i] = static_cast<T>(yn);
         b.ax[i] = static_cast<T>(axn);
-        if (b.ysum) b.ysum[i] += static_cast<T>(yn);
+        if (b.ysum) b.ysum[i] = q.ysum + static_cast<T>(yn);
         const double d = clamp_box(axn, lo, hi) - axn;
         p.v[0] = dmax(p.v[0], fabs(d));
-        const double du = d * static_cast<double>(b.row_unscale[i]);
+        const double du = d * static_cast<double>(q.unscale);
         p.v[1] += du * du;
         p.v[2] += d * d;
         if (yn > 0.0) {
@@ -136,21 +160,19 @@ struct DualEpi {
             if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= yn * lo;
         }
         if (b.ylast) {
-            const double dy = yn - static_cast<double>(b.ylast[i]);
+            const double dy = yn - static_cast<double>(q.ylast);
             p.v[5] += dy * dy;
         }
     }
 };
 
 // Column terms of the KKT sums for (x_j, g_j = c_j + aty_j) (shared by the
-// per-iteration and the average evaluation).
-template <class T>
-__device__ __forceinline__ void column_terms(const Bufs<T> &b, int j, double xj, double cj,
-                                             double g, double lj, double uj, Partials &p) {
+// per-iteration and the average evaluation).  s = col_scale[j].
+__device__ __forceinline__ void column_terms(double xj, double cj, double g, double lj, double uj,
+                                             double s, bool have_last, double xlast, Partials &p) {
     const double d = xj - clamp_box(xj - g, lj, uj);
     p.v[0] = dmax(p.v[0], fabs(d));
     p.v[2] += d * d;
-    const double s = static_cast<double>(b.col_scale[j]);
     const double dorig = s * (xj - clamp_box(xj - g / (s * s), lj, uj));
     p.v[1] += dorig * dorig;
     p.v[5] += cj * xj;
@@ -159,8 +181,8 @@ __device__ __forceinline__ void column_terms(const Bufs<T> &b, int j, double xj,
     } else if (g < 0.0) {
         if (isinf(uj)) p.v[4] += 1.0; else p.v[3] += g * uj;
     }
-    if (b.xlast) {
-        const double dx = xj - static_cast<double>(b.xlast[j]);
+    if (have_last) {
+        const double dx = xj - xlast;
         p.v[6] += dx * dx;
     }
 }
@@ -173,16 +195,31 @@ struct PrimalEpi {
     T *xnext;
     double tau;
     bool fuse;
-    __device__ void row(int j, double acc, Partials &p) const {
+    struct Pre {
+        T x, c, l, u, s, xsum, xlast;
+    };
+    __device__ Pre prefetch(int j) const {
+        Pre p;
+        p.x = xcur[j];
+        p.c = b.c[j];
+        p.l = b.l[j];
+        p.u = b.u[j];
+        p.s = b.col_scale[j];
+        p.xsum = (fuse && b.xsum) ? b.xsum[j] : T(0);
+        p.xlast = b.xlast ? b.xlast[j] : T(0);
+        return p;
+    }
+    __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
         b.aty[j] = static_cast<T>(acc);
-        const double xj = static_cast<double>(xcur[j]);
-        const double cj = static_cast<double>(b.c[j]);
-        const double lj = static_cast<double>(b.l[j]), uj = static_cast<double>(b.u[j]);
-        column_terms(b, j, xj, cj, cj + acc, lj, uj, p);
+        const double xj = static_cast<double>(q.x);
+        const double cj = static_cast<double>(q.c);
+        const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
+        column_terms(xj, cj, cj + acc, lj, uj, static_cast<double>(q.s), b.xlast != nullptr,
+                     static_cast<double>(q.xlast), p);
         if (fuse) {
             const double xn = clamp_box((xj - tau * cj) - tau * acc, lj, uj);
             xnext[j] = static_cast<T>(xn);
-            if (b.xsum) b.xsum[j] += static_cast<T>(xn);
+            if (b.xsum) b.xsum[j] = q.xsum + static_cast<T>(xn);
         }
     }
 };
@@ -192,14 +229,20 @@ template <class T>
 struct EvalRowsEpi {
     const Bufs<T> &b;
     double inv_ns;
-    __device__ void row(int i, double acc, Partials &p) const {
+    struct Pre {
+        T ysum, lo, hi, unscale, ylast;
+    };
+    __device__ Pre prefetch(int i) const {
+        return Pre{b.ysum[i], b.rl[i], b.ru[i], b.row_unscale[i], b.ylast[i]};
+    }
+    __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
         const double axa = acc * inv_ns;
         b.ax_avg[i] = static_cast<T>(axa);
-        const double ya = static_cast<double>(b.ysum[i]) * inv_ns;
-        const double lo = static_cast<double>(b.rl[i]), hi = static_cast<double>(b.ru[i]);
+        const double ya = static_cast<double>(q.ysum) * inv_ns;
+        const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
         const double d = clamp_box(axa, lo, hi) - axa;
         p.v[0] = dmax(p.v[0], fabs(d));
-        const double du = d * static_cast<double>(b.row_unscale[i]);
+        const double du = d * static_cast<double>(q.unscale);
         p.v[1] += du * du;
         p.v[2] += d * d;
         if (ya > 0.0) {
@@ -207,7 +250,7 @@ struct EvalRowsEpi {
         } else if (ya < 0.0) {
             if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= ya * lo;
         }
-        const double dy = ya - static_cast<double>(b.ylast[i]);
+        const double dy = ya - static_cast<double>(q.ylast);
         p.v[5] += dy * dy;
     }
 };
@@ -216,48 +259,23 @@ template <class T>
 struct EvalColsEpi {
     const Bufs<T> &b;
     double inv_ns;
-    __device__ void row(int j, double acc, Partials &p) const {
+    struct Pre {
+        T xsum, c, l, u, s, xlast;
+    };
+    __device__ Pre prefetch(int j) const {
+        return Pre{b.xsum[j], b.c[j], b.l[j], b.u[j], b.col_scale[j], b.xlast[j]};
+    }
+    __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
         const double atya = acc * inv_ns;
         b.aty_avg[j] = static_cast<T>(atya);
-        const double xa = static_cast<double>(b.xsum[j]) * inv_ns;
-        const double cj = static_cast<double>(b.c[j]);
-        column_terms(b, j, xa, cj, cj + atya, static_cast<double>(b.l[j]),
-                     static_cast<double>(b.u[j]), p);
+        const double xa = static_cast<double>(q.xsum) * inv_ns;
+        const double cj = static_cast<double>(q.c);
+        column_terms(xa, cj, cj + atya, static_cast<double>(q.l), static_cast<double>(q.u),
+                     static_cast<double>(q.s), true, static_cast<double>(q.xlast), p);
     }
 };
 
 // ------------------------------------------------------------------ device helpers
-
-// Marks this block done; returns true in every thread of the last block.
-__device__ __forceinline__ bool last_block(DevState *st) {
-    __shared__ bool am_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(&st->counter, 1u);
-        am_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (am_last) __threadfence();
-    return am_last;
-}
-
-struct KktPieces {
-    double max0, orig2, scaled2, dual, inf, pobj, dx2, dy2;
-};
-
-__device__ __forceinline__ KktPieces combine(const double *r, const double *c) {
-    KktPieces k;
-    k.max0 = dmax(r[0], c[0]);
-    k.orig2 = 0.0;
-    k.scaled2 = 0.0;
-    k.dual = r[3] + c[3];
-    k.inf = r[4] + c[4];
-    k.pobj = c[5];
-    k.dx2 = c[6];
-    k.dy2 = r[5];
-    return k;
-}
 
 __device__ __forceinline__ void relative_kkt(const DevState &s, const double *r, const double *c,
                                              double &rel_p, double &rel_d, double &rel_g,
@@ -282,14 +300,14 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_dual(Bufs<T> b, CsrView<T> A) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
-    __shared__ double lds_prod[kTileNnz];
-    __shared__ double lds_rows[kTileRows];
+    __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kWaves];
     const T *xin = b.x[(s.k + 1) & 1];
-    DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag};
+    DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
+                   s.policy == VK_PDHG_POLICY_ADAPTIVE};
     Partials part;
     part.zero();
-    tile_spmv(A, xin, epi, part, lds_prod, lds_rows, scratch);
+    tile_spmv(A, xin, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_rows, scratch);
 }
 
@@ -304,18 +322,25 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
-    __shared__ double lds_prod[kTileNnz];
-    __shared__ double lds_rows[kTileRows];
+    __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kWaves];
-    __shared__ double red_rows[kSlots], red_cols[kSlots];
     const bool fuse = may_fuse(s);
     PrimalEpi<T> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse};
     Partials part;
     part.zero();
-    tile_spmv(AT, b.y, epi, part, lds_prod, lds_rows, scratch);
+    tile_spmv(AT, b.y, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_cols, scratch);
-    if (!last_block(b.st)) return;
+}
 
+// One block: reduces the partials of this iteration's dual and primal
+// kernels in a fixed order and runs the step / restart / termination rule.
+// (A separate launch orders it after every block of both kernels without a
+// device-scope fence in each block.)
+template <class T>
+__global__ __launch_bounds__(kFinalBlock) void k_finalize(Bufs<T> b) {
+    if (b.st->status != VK_PDHG_RUNNING) return;
+    __shared__ double scratch[kFinalBlock / kWave];
+    __shared__ double red_rows[kSlots], red_cols[kSlots];
     reduce_partials(b.part_rows, b.grid_rows, red_rows, scratch);
     reduce_partials(b.part_cols, b.grid_cols, red_cols, scratch);
     if (threadIdx.x != 0) return;
@@ -340,7 +365,6 @@ __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
     S.cur_gap = gap_abs;
     S.cur_dx2 = red_cols[6];
     S.cur_dy2 = red_rows[5];
-    bool converged;
     if (S.policy == VK_PDHG_POLICY_REFERENCE) {
         // pdhg.cpp:133-145; the restart flag was consumed by this iteration's
         // dual step (pdhg.cpp:98-100).
@@ -361,13 +385,12 @@ __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
         }
         S.inv_sigma = 1.0 / S.sigma;
     }
-    converged = (S.rel_tol > 0.0) ? (rel_kkt <= S.rel_tol) : (metric <= S.metric_tol);  // pdhg.cpp:147
+    const bool converged = (S.rel_tol > 0.0) ? (rel_kkt <= S.rel_tol) : (metric <= S.metric_tol);  // pdhg.cpp:147
     S.k = k + 1;
     S.ks += 1;
     S.ns += 1;
     if (converged) S.status = VK_PDHG_CONVERGED;
     else if (S.k >= S.limit) S.status = VK_PDHG_ITERATION_LIMIT;
-    S.counter = 0;
 }
 
 // Standalone primal step from x[k&1] into x[(k+1)&1] (setup, and after an
@@ -392,38 +415,41 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_rows(Bufs<T> b, CsrView<T> A) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
-    __shared__ double lds_prod[kTileNnz];
-    __shared__ double lds_rows[kTileRows];
+    __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kWaves];
     EvalRowsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
     Partials part;
     part.zero();
-    tile_spmv(A, b.xsum, epi, part, lds_prod, lds_rows, scratch);
+    tile_spmv(A, b.xsum, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_erows, scratch);
 }
 
-// Average evaluation over columns; the last block runs the adaptive restart
-// rule.  `init` evaluates the start point instead and only records its KKT
-// error as the first restart reference.
+// Average evaluation over columns.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_eval_cols(Bufs<T> b, CsrView<T> AT, int init) {
+__global__ __launch_bounds__(kBlock) void k_eval_cols(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
-    __shared__ double lds_prod[kTileNnz];
-    __shared__ double lds_rows[kTileRows];
+    __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kWaves];
-    __shared__ double red_rows[kSlots], red_cols[kSlots];
     EvalColsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
     Partials part;
     part.zero();
-    tile_spmv(AT, b.ysum, epi, part, lds_prod, lds_rows, scratch);
+    tile_spmv(AT, b.ysum, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_ecols, scratch);
-    if (!last_block(b.st)) return;
+}
+
+// One block: the adaptive restart rule on the evaluated average.  `init`
+// evaluates the start point instead and only records its KKT error as the
+// first restart reference.
+template <class T>
+__global__ __launch_bounds__(kFinalBlock) void k_check(Bufs<T> b, int init) {
+    if (b.st->status != VK_PDHG_RUNNING) return;
+    __shared__ double scratch[kFinalBlock / kWave];
+    __shared__ double red_rows[kSlots], red_cols[kSlots];
     reduce_partials(b.part_erows, b.grid_rows, red_rows, scratch);
     reduce_partials(b.part_ecols, b.grid_cols, red_cols, scratch);
     if (threadIdx.x != 0) return;
     DevState &S = *b.st;
-    S.counter = 0;
     double rel_p, rel_d, rel_g, gap_abs, pobj, dobj;
     relative_kkt(S, red_rows, red_cols, rel_p, rel_d, rel_g, gap_abs, pobj, dobj);
     const double w = S.omega;
@@ -536,18 +562,19 @@ __global__ void k_zero(int n, T *dst) {
 template <class T>
 struct StoreEpi {
     T *y;
-    __device__ void row(int r, double acc, Partials &) const { y[r] = static_cast<T>(acc); }
+    struct Pre {};
+    __device__ Pre prefetch(int) const { return Pre{}; }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = static_cast<T>(acc); }
 };
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_spmv_store(CsrView<T> A, const T *x, T *y) {
-    __shared__ double lds_prod[kTileNnz];
-    __shared__ double lds_rows[kTileRows];
+    __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kWaves];
     StoreEpi<T> epi{y};
     Partials part;
     part.zero();
-    tile_spmv(A, x, epi, part, lds_prod, lds_rows, scratch);
+    tile_spmv(A, x, epi, part, lds[threadIdx.x / kWave]);
 }
 
 __global__ void k_d2f(int n, const double *src, float *dst) {
@@ -628,11 +655,13 @@ void enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
     for (long long k = k_begin; k < k_begin + count; ++k) {
         k_dual<T><<<s->grid_rows, kBlock, 0, s->stream>>>(b, A);
         k_primal<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT);
+        k_finalize<T><<<1, kFinalBlock, 0, s->stream>>>(b);
         if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
             if (k > 0 && k % kDecisionPeriod == 0) k_pstep<T><<<egrid(s->n), kBlock, 0, s->stream>>>(b);
         } else if ((k + 1) % s->params.restart_period == 0) {
             k_eval_rows<T><<<s->grid_rows, kBlock, 0, s->stream>>>(b, A);
-            k_eval_cols<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT, 0);
+            k_eval_cols<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT);
+            k_check<T><<<1, kFinalBlock, 0, s->stream>>>(b, 0);
             k_apply<T><<<ge, kBlock, 0, s->stream>>>(b);
         }
     }
@@ -743,7 +772,8 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
         VK_HIP_CHECK(hipMemcpyAsync(s->st, &tmp, sizeof(DevState), hipMemcpyHostToDevice, st));
         VK_HIP_CHECK(hipStreamSynchronize(st));
         k_eval_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, A);
-        k_eval_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, AT, 1);
+        k_eval_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, AT);
+        k_check<T><<<1, kFinalBlock, 0, st>>>(b, 1);
         k_zero<T><<<egrid(n), kBlock, 0, st>>>(n, b.xsum);
         if (m > 0) k_zero<T><<<egrid(m), kBlock, 0, st>>>(m, b.ysum);
         VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
@@ -909,9 +939,10 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, co
     s->m = static_cast<int>(A->rows);
     s->n = static_cast<int>(A->cols);
     s->mixed = params->precision == VK_PREC_MIXED;
-    const int cap = params->grid_blocks > 0 ? params->grid_blocks : 2048;
-    s->grid_rows = std::max(1, std::min(A->a.ntiles, cap));
-    s->grid_cols = std::max(1, std::min(A->at.ntiles, cap));
+    const int cap = std::min(params->grid_blocks > 0 ? params->grid_blocks : 2048, kMaxGrid);
+    // One tile per wave, kWaves waves per block.
+    s->grid_rows = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, cap));
+    s->grid_cols = std::max(1, std::min((A->at.ntiles + kWaves - 1) / kWaves, cap));
     auto fail = [&](int rc) {
         vk_hip_pdhg_destroy(s);
         return rc;
@@ -1017,16 +1048,14 @@ int vk_hip_pdhg_profile(vk_hip_pdhg *s, int64_t iterations, double *dual_ms, dou
     long long k = s->host_state->k;
     for (int64_t i = 0; i < iterations; ++i, ++k) {
         VK_HIP_CHECK(hipEventRecord(ev[3 * i], st));
-        if (s->mixed) {
-            k_dual<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
-            VK_HIP_CHECK(hipEventRecord(ev[3 * i + 1], st));
-            k_primal<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s));
-        } else {
-            k_dual<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
-            VK_HIP_CHECK(hipEventRecord(ev[3 * i + 1], st));
-            k_primal<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s));
-        }
+        if (s->mixed) k_dual<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
+        else k_dual<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
+        VK_HIP_CHECK(hipEventRecord(ev[3 * i + 1], st));
+        if (s->mixed) k_primal<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s));
+        else k_primal<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s));
         VK_HIP_CHECK(hipEventRecord(ev[3 * i + 2], st));
+        if (s->mixed) k_finalize<float><<<1, kFinalBlock, 0, st>>>(s->b32);
+        else k_finalize<double><<<1, kFinalBlock, 0, st>>>(s->b64);
         // keep the device schedule consistent (step-size / restart kernels)
         if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
             if (k > 0 && k % kDecisionPeriod == 0) {
@@ -1037,11 +1066,13 @@ int vk_hip_pdhg_profile(vk_hip_pdhg *s, int64_t iterations, double *dual_ms, dou
             const int ge = egrid(std::max(s->n, s->m));
             if (s->mixed) {
                 k_eval_rows<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
-                k_eval_cols<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s), 0);
+                k_eval_cols<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s));
+                k_check<float><<<1, kFinalBlock, 0, st>>>(s->b32, 0);
                 k_apply<float><<<ge, kBlock, 0, st>>>(s->b32);
             } else {
                 k_eval_rows<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
-                k_eval_cols<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s), 0);
+                k_eval_cols<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s));
+                k_check<double><<<1, kFinalBlock, 0, st>>>(s->b64, 0);
                 k_apply<double><<<ge, kBlock, 0, st>>>(s->b64);
             }
         }

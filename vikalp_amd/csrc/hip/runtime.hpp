@@ -26,15 +26,15 @@ struct DeviceCsr {
     int *row_ptr = nullptr;
     int *col = nullptr;
     V *val = nullptr;
-    int *tile_row = nullptr;
+    int2 *tiles = nullptr;   // {first row, first nonzero}, ntiles+1 entries
     int rows = 0;
     int ntiles = 0;
-    CsrView<V> view() const { return CsrView<V>{row_ptr, col, val, tile_row, rows, ntiles}; }
+    CsrView<V> view() const { return CsrView<V>{row_ptr, col, val, tiles, rows, ntiles}; }
 };
 
 // Splits rows into tiles of whole rows with at most kTileNnz nonzeros and
 // kTileRows rows; a row longer than kTileNnz is a tile of its own.
-std::vector<int> build_tiles(const std::vector<int> &row_ptr, int rows);
+std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows);
 
 }  // namespace vk
 

@@ -29,6 +29,7 @@ namespace vikalp::solver {
 namespace {
 
 using Clock = std::chrono::steady_clock;
+constexpr int kRuizPasses = 10;
 constexpr Scalar kInf = std::numeric_limits<Scalar>::infinity();
 
 SolveResult status_result(SolveStatus status, const std::string &message) {
@@ -93,6 +94,64 @@ struct Scaled {
 Scalar scale_bound(Scalar v, Scalar f, bool multiply) {
     if (!std::isfinite(v)) return v;
     return multiply ? v * f : v / f;
+}
+
+// Further diagonal preconditioning used by the adaptive policy, on top of the
+// reference scaling: `ruiz` passes of Ruiz equilibration (divide each row and
+// column by the square root of its largest magnitude) followed by one
+// Pock-Chambolle step with alpha = 1 (divide row i by sqrt(sum_j |a_ij|) and
+// column j by sqrt(sum_i |a_ij|)) -- the default preconditioner of PDLP
+// (Applegate et al., NeurIPS 2021, sec. 3.3).  Factors accumulate into r, s;
+// c, bounds and row bounds are transformed as in the reference scaling.
+void equilibrate(const CsrPattern &p, Scaled &sc, int ruiz) {
+    const auto m = static_cast<std::size_t>(p.rows);
+    const auto n = static_cast<std::size_t>(p.columns);
+    std::vector<Scalar> rf(m), cf(n);
+    auto apply = [&]() {
+        for (std::size_t i = 0; i < m; ++i)
+            for (Index k = p.row_offsets[i]; k < p.row_offsets[i + 1]; ++k) {
+                const auto kk = static_cast<std::size_t>(k);
+                sc.values[kk] /= rf[i] * cf[static_cast<std::size_t>(p.column_indices[kk])];
+            }
+        for (std::size_t i = 0; i < m; ++i) {
+            sc.r[i] /= rf[i];
+            if (std::isfinite(sc.rl[i])) sc.rl[i] /= rf[i];
+            if (std::isfinite(sc.ru[i])) sc.ru[i] /= rf[i];
+        }
+        for (std::size_t j = 0; j < n; ++j) {
+            sc.s[j] /= cf[j];
+            sc.c[j] /= cf[j];
+            if (std::isfinite(sc.l[j])) sc.l[j] *= cf[j];
+            if (std::isfinite(sc.u[j])) sc.u[j] *= cf[j];
+        }
+    };
+    for (int pass = 0; pass < ruiz; ++pass) {
+        std::fill(rf.begin(), rf.end(), 0.0);
+        std::fill(cf.begin(), cf.end(), 0.0);
+        for (std::size_t i = 0; i < m; ++i)
+            for (Index k = p.row_offsets[i]; k < p.row_offsets[i + 1]; ++k) {
+                const auto kk = static_cast<std::size_t>(k);
+                const Scalar a = std::abs(sc.values[kk]);
+                rf[i] = std::max(rf[i], a);
+                auto &c = cf[static_cast<std::size_t>(p.column_indices[kk])];
+                c = std::max(c, a);
+            }
+        for (auto &v : rf) v = v > 0.0 ? std::sqrt(v) : 1.0;
+        for (auto &v : cf) v = v > 0.0 ? std::sqrt(v) : 1.0;
+        apply();
+    }
+    std::fill(rf.begin(), rf.end(), 0.0);
+    std::fill(cf.begin(), cf.end(), 0.0);
+    for (std::size_t i = 0; i < m; ++i)
+        for (Index k = p.row_offsets[i]; k < p.row_offsets[i + 1]; ++k) {
+            const auto kk = static_cast<std::size_t>(k);
+            const Scalar a = std::abs(sc.values[kk]);
+            rf[i] += a;
+            cf[static_cast<std::size_t>(p.column_indices[kk])] += a;
+        }
+    for (auto &v : rf) v = v > 0.0 ? std::sqrt(v) : 1.0;
+    for (auto &v : cf) v = v > 0.0 ? std::sqrt(v) : 1.0;
+    apply();
 }
 
 // continuous_common.hpp:133-205 with the column pass in O(nnz).
@@ -562,7 +621,9 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
     if (!resolve_bounds(model, {}, {}, bounds, message) || !model.validate().empty() ||
         model.num_variables() == 0 || iterations <= 0)
         throw std::invalid_argument("profile_lp_kernels: unusable model or arguments");
-    const Scaled sc = scale(model, bounds);
+    Scaled sc = scale(model, bounds);
+    if (options.pdhg_policy == PdhgPolicy::Adaptive)
+        equilibrate(model.constraint_matrix.pattern, sc, kRuizPasses);
     vk_hip_csr *A = nullptr;
     check_vk(vk_hip_csr_create(model.constraint_matrix.pattern.rows, model.constraint_matrix.pattern.columns,
                                model.constraint_matrix.pattern.row_offsets.data(),
@@ -609,7 +670,9 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend, const Solver
     if (options.continuous_method != ContinuousMethod::Auto && options.continuous_method != ContinuousMethod::PDHG)
         return status_result(SolveStatus::UnsupportedModel, "requested continuous method is not implemented for LP");
 
-    const Scaled sc = scale(model, bounds);
+    Scaled sc = scale(model, bounds);
+    if (options.pdhg_policy == PdhgPolicy::Adaptive)
+        equilibrate(model.constraint_matrix.pattern, sc, kRuizPasses);
     std::vector<Scalar> warm;
     if (!warm_start.empty()) {
         if (static_cast<Index>(warm_start.size()) != model.num_variables())
