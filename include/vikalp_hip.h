@@ -189,6 +189,41 @@ int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *dual_ms
                         double *primal_ms, double *iteration_ms);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
 
+/* ---- batched PDHG (new: one persistent launch for many LPs) ------------
+ *
+ * Replaces, for a set of relaxations that differ only in data, the serial
+ * loop of solve_lp calls the branch-and-bound engine makes
+ * (src/solver/mip/BranchAndBound.cpp:296-297; one backend and one solve per
+ * node, src/solver/Solver.cpp:37-51).  Instance i owns rows
+ * [row_starts[i], row_starts[i+1]) and columns [col_starts[i], col_starts[i+1])
+ * of one block-diagonal CSR matrix (global indices; a row may only touch its
+ * instance's columns).  All vectors are the SCALED data concatenated in the
+ * same order; row_scale/col_scale give original coordinates.  step_sizes[i]
+ * <= 0 (or NULL) asks the device for ||A_i||_2 by power iteration;
+ * primal_weights, norm_b, norm_c are per instance (NULL = 1, 0, 0).  params
+ * is shared (its step_size, primal_weight, norm_b, norm_c are ignored). */
+typedef struct vk_hip_batch vk_hip_batch;
+
+int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col_starts,
+                        const int64_t *row_offsets, const int64_t *column_indices,
+                        const double *values, const double *c, const double *l,
+                        const double *u, const double *rl, const double *ru,
+                        const double *row_scale, const double *col_scale,
+                        const double *x0, const double *y0, const double *step_sizes,
+                        const double *primal_weights, const double *norm_b,
+                        const double *norm_c, const vk_pdhg_params *params,
+                        vk_hip_batch **out);
+/* Launches the persistent kernel up to `max_launches` times, each running
+ * `iterations_per_launch` iterations of every instance still running; stops
+ * early when all have stopped.  *running = instances still running. */
+int vk_hip_batch_run(vk_hip_batch *batch, int64_t iterations_per_launch,
+                     int64_t max_launches, int *running);
+int vk_hip_batch_read_state(vk_hip_batch *batch, int instance, vk_pdhg_state *state);
+/* Scaled iterates of all instances, concatenated (FP64 on the host). */
+int vk_hip_batch_download(vk_hip_batch *batch, double *x, double *y);
+int vk_hip_batch_last_run_ms(const vk_hip_batch *batch, double *ms);
+int vk_hip_batch_destroy(vk_hip_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

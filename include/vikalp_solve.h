@@ -85,6 +85,20 @@ int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
                 const double *upper_override, const vk_solver_options *options,
                 double *x_out, double *y_out, vk_solve_info *info);
 
+/* Batched solve (solver::solve_lp_batch): `count` independent LPs given as
+ * one block-diagonal CSR with global indices; instance i owns rows
+ * [row_starts[i], row_starts[i+1]) and columns [col_starts[i], col_starts[i+1]).
+ * Vectors are concatenated in the same order.  x_out (total columns) and
+ * y_out (total rows) receive the concatenated solutions, info[count] the
+ * per-instance results.  Returns 0 when results were produced. */
+int vk_lp_solve_batch(int count, const int64_t *row_starts, const int64_t *col_starts,
+                      const int64_t *row_offsets, const int64_t *column_indices,
+                      const double *values, const double *objective,
+                      const double *row_lower, const double *row_upper,
+                      const double *var_lower, const double *var_upper,
+                      const vk_solver_options *options, double *x_out, double *y_out,
+                      vk_solve_info *info);
+
 /* Measurement: builds the same scaled LP on the HIP backend, runs
  * `warm_iterations` iterations, then `iterations` more launched one by one
  * with HIP events around the two fused kernels on the solver's stream.

@@ -1,0 +1,82 @@
+"""Batched persistent PDHG (one launch, many LPs) on the GPU.
+
+Reference policy: every instance of a batch must reproduce the CPU oracle's
+solve_lp on that instance alone -- same status and iteration count,
+objective to 1e-9 relative, x and y to 1e-8 -- the same bar as the
+single-LP tests, so batching changes execution only, not results.
+Adaptive policy (FP64 and mixed): each instance reaches relative KKT <= 1e-6,
+recomputed by the oracle on the returned point, and agrees with the
+single-instance HIP solve's objective to 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import vikalp_amd as V
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def instances():
+    lps = [V.generate_lp(1, 40, 90, 3, 3), V.generate_lp(4, 0, 0, 4, 9), V.generate_lp(1, 60, 150, 4, 7),
+           V.generate_lp(2, 400, 1000, 0, 5), V.generate_lp(4, 0, 0, 5, 11)]
+    return lps
+
+
+def test_reference_policy_batch_matches_oracle_per_instance():
+    lps = instances()
+    it = 600
+    print("batch reference policy", [(lp.m, lp.n, lp.nnz) for lp in lps], flush=True)
+    got = V.solve_batch(lps, V.Options(iteration_limit=it))
+    for lp, g in zip(lps, got):
+        ref = O.solve_lp(lp.as_oracle(), iteration_limit=it)
+        assert g.status == ref.status and g.iterations == ref.iterations, (g.status, ref.status, g.iterations)
+        assert abs(g.objective_value - ref.objective_value) <= 1e-9 * max(1.0, abs(ref.objective_value))
+        assert np.max(np.abs(g.x - ref.x)) <= 1e-8 and np.max(np.abs(g.y - ref.y)) <= 1e-8
+
+
+def test_reference_policy_batch_terminates_per_instance():
+    # a loose tolerance stops some instances early; each must match its own
+    # single solve exactly, including the iteration at which it stopped
+    lps = [V.generate_lp(1, 40, 90, 3, 3), V.generate_lp(1, 30, 70, 3, 4), bounded_linear(-1.0, 0.0, 10.0)]
+    opts = dict(iteration_limit=20000, primal_tolerance=1e-3, dual_tolerance=1e-3)
+    got = V.solve_batch(lps, V.Options(**opts))
+    for lp, g in zip(lps, got):
+        ref = O.solve_lp(lp.as_oracle(), **opts)
+        assert (g.status, g.iterations) == (ref.status, ref.iterations)
+        assert np.max(np.abs(g.x - ref.x)) <= 1e-8
+
+
+def bounded_linear(c, lo, hi):
+    return V.LP(0, 1, np.array([0]), np.array([], np.int64), np.array([]), np.array([c]),
+                np.array([]), np.array([]), np.array([lo]), np.array([hi]))
+
+
+@pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
+def test_adaptive_batch_reaches_relative_kkt(precision):
+    lps = [V.generate_lp(4, 0, 0, t, 20 + t) for t in (4, 5, 6, 4)] + [V.generate_lp(1, 300, 900, 5, 11)]
+    tol = 1e-6
+    opts = V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
+                     relative_kkt_tolerance=tol, iteration_limit=100_000)
+    print("adaptive batch", precision.name, flush=True)
+    got = V.solve_batch(lps, opts)
+    for lp, g in zip(lps, got):
+        assert g.status == "Optimal", g
+        rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), g.x, g.y)
+        assert max(rp, rd, gp) <= tol * 1.0001, (rp, rd, gp)
+        single = V.solve_lp(lp, V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, relative_kkt_tolerance=tol,
+                                          iteration_limit=100_000))
+        assert single.status == "Optimal"
+        assert abs(g.objective_value - single.objective_value) <= 1e-5 * max(1.0, abs(single.objective_value))
+
+
+def test_invalid_instance_does_not_stop_the_batch():
+    good = V.generate_lp(1, 40, 90, 3, 3)
+    bad = bounded_linear(1.0, 3.0, 2.0)            # lower > upper
+    got = V.solve_batch([good, bad, good], V.Options(iteration_limit=300))
+    assert got[1].status == "InvalidModel"
+    ref = O.solve_lp(good.as_oracle(), iteration_limit=300)
+    for g in (got[0], got[2]):
+        assert g.status == ref.status and np.max(np.abs(g.x - ref.x)) <= 1e-8

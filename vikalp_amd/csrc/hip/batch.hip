@@ -1,0 +1,672 @@
+// Batched PDHG: many independent LPs solved per launch by a persistent
+// kernel, one 512-thread workgroup (8 wave64s) per instance.
+//
+// The instances are stored as one block-diagonal CSR matrix (and its stored
+// transpose) whose tiles never cross an instance boundary, so a workgroup
+// streams only its own instance.  Inside the workgroup one iteration is the
+// same fused sequence as the single-LP solver (pdhg.hip): the dual epilogue
+// on the instance's rows of A, a workgroup barrier, the primal epilogue on
+// its rows of A', then the step / restart / termination rule of the shared
+// core (pdhg_core.hpp) run by one thread on the instance's own DevState.
+// Each instance therefore keeps its own step sizes, primal weight, restart
+// points and status on the device, and stops on its own; the kernel runs a
+// bounded number of iterations per launch and the host relaunches until
+// every instance has stopped.  This is the shape of the reference's
+// branch-and-bound node loop (src/solver/mip/BranchAndBound.cpp:296-297),
+// where only bounds and warm starts differ between relaxations, turned into
+// one launch instead of one serial solve_lp per node.
+
+#include "runtime.hpp"
+#include "pdhg_core.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+namespace vk {
+namespace {
+
+constexpr int kBatchBlock = 512;                  // threads per instance
+constexpr int kBatchWaves = kBatchBlock / kWave;  // 8
+
+struct InstanceRange {
+    int r0, r1, c0, c1;     // rows / columns of the instance
+    int ta0, ta1, tt0, tt1; // tile ranges in A and A'
+};
+
+template <class T>
+struct BatchArgs {
+    Bufs<T> b;
+    CsrView<T> A, AT;
+    CsrView<double> A64, AT64;   // for the step-size power iteration
+    const InstanceRange *inst;
+    DevState *states;
+    double *pw_rows, *pw_cols;   // power-iteration scratch (FP64)
+    int count;
+    int power_iterations;
+};
+
+// Workgroup reduction of the 8 slots into out[] (written by thread 0).
+__device__ __forceinline__ void reduce_slots(const Partials &p, double *out, double *scratch) {
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const double r = (s == 0) ? final_max<kBatchBlock>(p.v[s], scratch) : final_sum<kBatchBlock>(p.v[s], scratch);
+        if (threadIdx.x == 0) out[s] = r;
+    }
+    __syncthreads();
+}
+
+struct StoreEpi64 {
+    double *y;
+    struct Pre {};
+    __device__ Pre prefetch(int) const { return Pre{}; }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = acc; }
+};
+
+template <class T>
+struct StoreEpiT {
+    T *y;
+    struct Pre {};
+    __device__ Pre prefetch(int) const { return Pre{}; }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = static_cast<T>(acc); }
+};
+
+// Per instance: project the start, compute A x0 and A' y0, estimate ||A||_2
+// by power iteration when the host gave no step size, record the start
+// point's KKT error (adaptive policy) and take the first primal step.
+template <class T>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
+    const int id = blockIdx.x;
+    if (id >= a.count) return;
+    __shared__ double lds[kBatchWaves][kTileNnz];
+    __shared__ double scratch[kBatchWaves];
+    __shared__ double red_r[kSlots], red_c[kSlots];
+    __shared__ DevState S;
+    const InstanceRange R = a.inst[id];
+    const Bufs<T> &b = a.b;
+    const int tid = threadIdx.x, wave = tid / kWave;
+    double *my_lds = lds[wave];
+    if (tid == 0) S = a.states[id];
+    __syncthreads();
+    for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+        b.x[0][j] = static_cast<T>(clamp_box(static_cast<double>(b.x[0][j]), static_cast<double>(b.l[j]),
+                                             static_cast<double>(b.u[j])));
+    __syncthreads();
+    Partials p;
+    p.zero();
+    {
+        StoreEpiT<T> e{b.ax};
+        tile_spmv_range(a.A, b.x[0], e, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        StoreEpiT<T> f{b.aty};
+        tile_spmv_range(a.AT, b.y, f, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+    }
+    if (S.eta <= 0.0 && S.max_step <= 0.0) {
+        // sigma_max(A)^2 ~ ||A'A v|| with ||v|| = 1 (deterministic start)
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) {
+            uint64_t z = static_cast<uint64_t>(j - R.c0) * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z ^= z >> 31;
+            a.pw_cols[j] = static_cast<double>(z >> 11) * 0x1.0p-53 + 0.5;
+        }
+        __syncthreads();
+        double nrm2 = 0.0;
+        for (int it = 0; it <= a.power_iterations; ++it) {
+            double part = 0.0;
+            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) part += a.pw_cols[j] * a.pw_cols[j];
+            nrm2 = final_sum<kBatchBlock>(part, scratch);
+            if (it == a.power_iterations) break;
+            const double inv = 1.0 / sqrt(nrm2);
+            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) a.pw_cols[j] *= inv;
+            __syncthreads();
+            StoreEpi64 e{a.pw_rows};
+            tile_spmv_range(a.A64, a.pw_cols, e, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+            __syncthreads();
+            StoreEpi64 f{a.pw_cols};
+            tile_spmv_range(a.AT64, a.pw_rows, f, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const double sigma_max = sqrt(sqrt(nrm2));
+            const double eta = 0.99 / (sigma_max > 1e-12 ? sigma_max : 1e-12);
+            if (S.policy == VK_PDHG_POLICY_REFERENCE) {
+                S.max_step = eta;
+                S.tau = S.sigma = eta;
+            } else {
+                S.eta = eta;
+                S.tau = eta / S.omega;
+                S.sigma = eta * S.omega;
+            }
+            S.inv_sigma = 1.0 / S.sigma;
+        }
+    }
+    __syncthreads();
+    if (S.policy == VK_PDHG_POLICY_ADAPTIVE && S.status == VK_PDHG_RUNNING) {
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) {
+            b.xsum[j] = b.x[0][j];
+            b.xlast[j] = b.x[0][j];
+        }
+        for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) {
+            b.ysum[i] = b.y[i];
+            b.ylast[i] = b.y[i];
+        }
+        __syncthreads();
+        EvalRowsEpi<T> er{b, 1.0};
+        p.zero();
+        tile_spmv_range(a.A, b.xsum, er, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        reduce_slots(p, red_r, scratch);
+        EvalColsEpi<T> ec{b, 1.0};
+        p.zero();
+        tile_spmv_range(a.AT, b.ysum, ec, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        reduce_slots(p, red_c, scratch);
+        if (tid == 0) check_step(S, red_r, red_c, true);
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) b.xsum[j] = T(0);
+        for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) b.ysum[i] = T(0);
+        __syncthreads();
+    }
+    if (S.status == VK_PDHG_RUNNING)
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) pstep_element(b, S, j, b.x[0], b.x[1]);
+    __syncthreads();
+    if (tid == 0) a.states[id] = S;
+}
+
+// Up to `iterations` iterations of every instance still running.
+template <class T>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, int iterations) {
+    const int id = blockIdx.x;
+    if (id >= a.count) return;
+    __shared__ double lds[kBatchWaves][kTileNnz];
+    __shared__ double scratch[kBatchWaves];
+    __shared__ double red_r[kSlots], red_c[kSlots];
+    __shared__ DevState S;
+    const InstanceRange R = a.inst[id];
+    const Bufs<T> &b = a.b;
+    const int tid = threadIdx.x, wave = tid / kWave;
+    double *my_lds = lds[wave];
+    if (tid == 0) S = a.states[id];
+    __syncthreads();
+    for (int it = 0; it < iterations; ++it) {
+        if (S.status != VK_PDHG_RUNNING) break;
+        const long long k = S.k;
+        const bool ref = S.policy == VK_PDHG_POLICY_REFERENCE;
+        Partials p;
+        p.zero();
+        DualEpi<T> de{b, S.sigma, S.inv_sigma, ref && S.restart_flag, !ref};
+        tile_spmv_range(a.A, b.x[(k + 1) & 1], de, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        reduce_slots(p, red_r, scratch);
+        const bool fuse = may_fuse(S);
+        PrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse};
+        p.zero();
+        tile_spmv_range(a.AT, b.y, pe, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        reduce_slots(p, red_c, scratch);
+        if (tid == 0) finalize_step(S, red_r, red_c);
+        __syncthreads();
+        if (fuse) continue;
+        if (ref) {
+            if (S.status == VK_PDHG_RUNNING)
+                for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+                    pstep_element(b, S, j, b.x[S.k & 1], b.x[(S.k + 1) & 1]);
+            __syncthreads();
+            continue;
+        }
+        if (S.status != VK_PDHG_RUNNING) continue;
+        const double inv_ns = 1.0 / static_cast<double>(S.ns);
+        EvalRowsEpi<T> er{b, inv_ns};
+        p.zero();
+        tile_spmv_range(a.A, b.xsum, er, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        reduce_slots(p, red_r, scratch);
+        EvalColsEpi<T> ec{b, inv_ns};
+        p.zero();
+        tile_spmv_range(a.AT, b.ysum, ec, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        reduce_slots(p, red_c, scratch);
+        if (tid == 0) check_step(S, red_r, red_c, false);
+        __syncthreads();
+        if (S.status == VK_PDHG_RUNNING || S.final_from_avg) {
+            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+                apply_column(b, S, j, b.x[S.k & 1], b.x[(S.k + 1) & 1]);
+            for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) apply_row(b, S, i);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) a.states[id] = S;
+}
+
+__global__ void k_to_f32_batch(int64_t n, const double *__restrict__ src, float *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = static_cast<float>(src[i]);
+}
+
+__global__ void k_to_f64_batch(int64_t n, const float *__restrict__ src, double *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = static_cast<double>(src[i]);
+}
+
+int bgrid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
+
+// Tiles of the rows [begin, end) only (tiles never cross an instance).
+void append_tiles(const std::vector<int> &rp, int begin, int end, std::vector<int2> &tiles) {
+    int cur_nnz = 0, cur_rows = 0;
+    tiles.push_back(make_int2(begin, rp[begin]));
+    for (int r = begin; r < end; ++r) {
+        const int len = rp[r + 1] - rp[r];
+        if (cur_rows > 0 && (cur_nnz + len > kTileNnz || cur_rows == kTileRows)) {
+            tiles.push_back(make_int2(r, rp[r]));
+            cur_nnz = 0;
+            cur_rows = 0;
+        }
+        cur_nnz += len;
+        ++cur_rows;
+        if (len > kTileNnz) {
+            tiles.push_back(make_int2(r + 1, rp[r + 1]));
+            cur_nnz = 0;
+            cur_rows = 0;
+        }
+    }
+    if (cur_rows > 0) tiles.push_back(make_int2(end, rp[end]));
+    // tiles = [first, boundaries..., end]: tile t of the range spans entries
+    // t and t+1; the end entry doubles as the next range's first entry.
+}
+
+}  // namespace
+}  // namespace vk
+
+using namespace vk;
+
+struct vk_hip_batch {
+    vk_pdhg_params params{};
+    int count = 0;
+    int64_t m = 0, n = 0, nnz = 0;
+    bool mixed = false;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocations;
+    int *rp = nullptr, *ci = nullptr, *trp = nullptr, *tci = nullptr;
+    double *val = nullptr, *tval = nullptr;
+    float *val32 = nullptr, *tval32 = nullptr;
+    int2 *tiles_a = nullptr, *tiles_at = nullptr;
+    int ntiles_a = 0, ntiles_at = 0;
+    InstanceRange *inst = nullptr;
+    std::vector<InstanceRange> inst_host;
+    DevState *states = nullptr;
+    std::vector<DevState> states_host;
+    BatchArgs<double> a64{};
+    BatchArgs<float> a32{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+
+    template <class T>
+    T *alloc(size_t count_) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count_, 1)) != hipSuccess) return nullptr;
+        allocations.push_back(p);
+        return static_cast<T *>(p);
+    }
+};
+
+namespace {
+
+template <class T>
+int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *val_at,
+                     const std::vector<double> *host_vectors) {
+    const int64_t m = h->m, n = h->n;
+    Bufs<T> &b = a.b;
+    std::memset(&b, 0, sizeof(b));
+    b.m = static_cast<int>(m);
+    b.n = static_cast<int>(n);
+    b.x[0] = h->alloc<T>(n);
+    b.x[1] = h->alloc<T>(n);
+    b.y = h->alloc<T>(m);
+    b.ax = h->alloc<T>(m);
+    b.aty = h->alloc<T>(n);
+    T *c = h->alloc<T>(n), *l = h->alloc<T>(n), *u = h->alloc<T>(n), *rl = h->alloc<T>(m), *ru = h->alloc<T>(m);
+    T *runs = h->alloc<T>(m), *cs = h->alloc<T>(n);
+    b.c = c; b.l = l; b.u = u; b.rl = rl; b.ru = ru; b.row_unscale = runs; b.col_scale = cs;
+    if (h->params.policy == VK_PDHG_POLICY_ADAPTIVE) {
+        b.xsum = h->alloc<T>(n);
+        b.ysum = h->alloc<T>(m);
+        b.xlast = h->alloc<T>(n);
+        b.ylast = h->alloc<T>(m);
+        b.ax_avg = h->alloc<T>(m);
+        b.aty_avg = h->alloc<T>(n);
+    }
+    for (void *p : h->allocations)
+        if (!p) {
+            set_error("vk_hip_batch_create: device allocation failed");
+            return VK_ENOMEM;
+        }
+    // host_vectors: c, l, u, rl, ru, 1/r, s, x0, y0 (FP64)
+    T *dst[9] = {c, l, u, rl, ru, runs, cs, b.x[0], b.y};
+    double *stage = nullptr;
+    VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * std::max<int64_t>(std::max(m, n), 1)));
+    for (int v = 0; v < 9; ++v) {
+        const auto &src = host_vectors[v];
+        if (src.empty()) continue;
+        if constexpr (sizeof(T) == sizeof(double)) {
+            VK_HIP_CHECK(hipMemcpyAsync(dst[v], src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice, h->stream));
+        } else {
+            VK_HIP_CHECK(hipMemcpyAsync(stage, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice, h->stream));
+            k_to_f32_batch<<<bgrid(src.size()), 256, 0, h->stream>>>(static_cast<int64_t>(src.size()), stage,
+                                                                    reinterpret_cast<float *>(dst[v]));
+        }
+        VK_HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    (void)hipFree(stage);
+    a.A = CsrView<T>{h->rp, h->ci, val_a, h->tiles_a, static_cast<int>(m), h->ntiles_a};
+    a.AT = CsrView<T>{h->trp, h->tci, val_at, h->tiles_at, static_cast<int>(n), h->ntiles_at};
+    a.A64 = CsrView<double>{h->rp, h->ci, h->val, h->tiles_a, static_cast<int>(m), h->ntiles_a};
+    a.AT64 = CsrView<double>{h->trp, h->tci, h->tval, h->tiles_at, static_cast<int>(n), h->ntiles_at};
+    a.inst = h->inst;
+    a.states = h->states;
+    a.pw_rows = h->alloc<double>(m);
+    a.pw_cols = h->alloc<double>(n);
+    a.count = h->count;
+    a.power_iterations = 100;
+    if (!a.pw_rows || !a.pw_cols) return VK_ENOMEM;
+    k_batch_setup<T><<<h->count, kBatchBlock, 0, h->stream>>>(a);
+    VK_HIP_CHECK(hipGetLastError());
+    VK_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return VK_OK;
+}
+
+int read_states(vk_hip_batch *h) {
+    VK_HIP_CHECK(hipMemcpyAsync(h->states_host.data(), h->states, sizeof(DevState) * h->count,
+                                hipMemcpyDeviceToHost, h->stream));
+    VK_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return VK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vk_hip_batch_destroy(vk_hip_batch *h) {
+    if (!h) return VK_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocations) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return VK_OK;
+}
+
+int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col_starts,
+                        const int64_t *row_offsets, const int64_t *column_indices, const double *values,
+                        const double *c, const double *l, const double *u, const double *rl,
+                        const double *ru, const double *row_scale, const double *col_scale,
+                        const double *x0, const double *y0, const double *step_sizes,
+                        const double *primal_weights, const double *norm_b, const double *norm_c,
+                        const vk_pdhg_params *params, vk_hip_batch **out) {
+    VK_REQUIRE(out && params && count > 0 && row_starts && col_starts && row_offsets,
+               "vk_hip_batch_create: bad arguments");
+    *out = nullptr;
+    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE,
+               "vk_hip_batch_create: unknown policy");
+    VK_REQUIRE(params->precision == VK_PREC_FP64 || params->precision == VK_PREC_MIXED,
+               "vk_hip_batch_create: unknown precision");
+    VK_REQUIRE(params->iteration_limit >= 0 && params->restart_period > 0,
+               "vk_hip_batch_create: bad iteration limit or restart period");
+    const int64_t m = row_starts[count], n = col_starts[count];
+    VK_REQUIRE(row_starts[0] == 0 && col_starts[0] == 0 && m < INT32_MAX && n < INT32_MAX,
+               "vk_hip_batch_create: instance ranges must start at 0 and fit int32");
+    const int64_t nnz = row_offsets[m];
+    VK_REQUIRE(nnz >= 0 && nnz < INT32_MAX && row_offsets[0] == 0, "vk_hip_batch_create: bad row offsets");
+    VK_REQUIRE(n > 0 && c && l && u && col_scale, "vk_hip_batch_create: null column vector");
+    VK_REQUIRE(m == 0 || (rl && ru && row_scale), "vk_hip_batch_create: null row vector");
+    VK_REQUIRE(nnz == 0 || (column_indices && values), "vk_hip_batch_create: null matrix arrays");
+    for (int i = 0; i < count; ++i) {
+        VK_REQUIRE(row_starts[i + 1] >= row_starts[i] && col_starts[i + 1] > col_starts[i],
+                   "vk_hip_batch_create: every instance needs >= 1 column and ordered ranges");
+        for (int64_t r = row_starts[i]; r < row_starts[i + 1]; ++r) {
+            VK_REQUIRE(row_offsets[r] <= row_offsets[r + 1] && row_offsets[r + 1] <= nnz,
+                       "vk_hip_batch_create: row offsets must be ordered and in range");
+            int64_t prev = -1;
+            for (int64_t k = row_offsets[r]; k < row_offsets[r + 1]; ++k) {
+                const int64_t col = column_indices[k];
+                VK_REQUIRE(col >= col_starts[i] && col < col_starts[i + 1] && col > prev,
+                           "vk_hip_batch_create: a row leaves its instance's columns or is unordered");
+                prev = col;
+            }
+        }
+    }
+    if (int rc = ensure_init()) return rc;
+    auto *h = new (std::nothrow) vk_hip_batch();
+    if (!h) return VK_ENOMEM;
+    auto fail = [&](int rc) {
+        vk_hip_batch_destroy(h);
+        return rc;
+    };
+    h->params = *params;
+    h->count = count;
+    h->m = m;
+    h->n = n;
+    h->nnz = nnz;
+    h->mixed = params->precision == VK_PREC_MIXED;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        set_error("vk_hip_batch_create: stream/event creation failed");
+        return fail(VK_EDEVICE);
+    }
+    // Host CSR (int32) and its transpose by a stable counting sort.
+    std::vector<int> rp(static_cast<size_t>(m + 1)), ci(static_cast<size_t>(nnz));
+    for (int64_t r = 0; r <= m; ++r) rp[r] = static_cast<int>(row_offsets[r]);
+    for (int64_t k = 0; k < nnz; ++k) ci[k] = static_cast<int>(column_indices[k]);
+    std::vector<int> trp(static_cast<size_t>(n + 1), 0), tci(static_cast<size_t>(nnz));
+    std::vector<double> tv(static_cast<size_t>(nnz));
+    for (int64_t k = 0; k < nnz; ++k) ++trp[ci[k] + 1];
+    for (int64_t j = 0; j < n; ++j) trp[j + 1] += trp[j];
+    {
+        std::vector<int> next(trp.begin(), trp.end() - 1);
+        for (int64_t r = 0; r < m; ++r)
+            for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                const int pos = next[ci[k]]++;
+                tci[pos] = static_cast<int>(r);
+                tv[pos] = values[k];
+            }
+    }
+    std::vector<int2> ta, tt;
+    h->inst_host.resize(static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) {
+        InstanceRange &R = h->inst_host[i];
+        R.r0 = static_cast<int>(row_starts[i]);
+        R.r1 = static_cast<int>(row_starts[i + 1]);
+        R.c0 = static_cast<int>(col_starts[i]);
+        R.c1 = static_cast<int>(col_starts[i + 1]);
+        // Each range contributes its tile starts; its end is the next
+        // range's start (ranges are contiguous) or the final sentinel.
+        R.ta0 = static_cast<int>(ta.size());
+        std::vector<int2> tmp;
+        append_tiles(rp, R.r0, R.r1, tmp);
+        R.ta1 = R.ta0 + static_cast<int>(tmp.size()) - 1;
+        ta.insert(ta.end(), tmp.begin(), tmp.end() - 1);
+        R.tt0 = static_cast<int>(tt.size());
+        tmp.clear();
+        append_tiles(trp, R.c0, R.c1, tmp);
+        R.tt1 = R.tt0 + static_cast<int>(tmp.size()) - 1;
+        tt.insert(tt.end(), tmp.begin(), tmp.end() - 1);
+    }
+    ta.push_back(make_int2(static_cast<int>(m), static_cast<int>(nnz)));
+    tt.push_back(make_int2(static_cast<int>(n), static_cast<int>(nnz)));
+    h->ntiles_a = static_cast<int>(ta.size()) - 1;
+    h->ntiles_at = static_cast<int>(tt.size()) - 1;
+    h->rp = h->alloc<int>(m + 1);
+    h->ci = h->alloc<int>(nnz);
+    h->trp = h->alloc<int>(n + 1);
+    h->tci = h->alloc<int>(nnz);
+    h->val = h->alloc<double>(nnz);
+    h->tval = h->alloc<double>(nnz);
+    h->tiles_a = h->alloc<int2>(ta.size());
+    h->tiles_at = h->alloc<int2>(tt.size());
+    h->inst = h->alloc<InstanceRange>(count);
+    h->states = h->alloc<DevState>(count);
+    if (h->mixed) {
+        h->val32 = h->alloc<float>(nnz);
+        h->tval32 = h->alloc<float>(nnz);
+    }
+    for (void *p : h->allocations)
+        if (!p) {
+            set_error("vk_hip_batch_create: device allocation failed");
+            return fail(VK_ENOMEM);
+        }
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream) : hipSuccess;
+    };
+    if (up(h->rp, rp.data(), sizeof(int) * rp.size()) != hipSuccess ||
+        up(h->ci, ci.data(), sizeof(int) * ci.size()) != hipSuccess ||
+        up(h->trp, trp.data(), sizeof(int) * trp.size()) != hipSuccess ||
+        up(h->tci, tci.data(), sizeof(int) * tci.size()) != hipSuccess ||
+        up(h->val, values, sizeof(double) * nnz) != hipSuccess ||
+        up(h->tval, tv.data(), sizeof(double) * nnz) != hipSuccess ||
+        up(h->tiles_a, ta.data(), sizeof(int2) * ta.size()) != hipSuccess ||
+        up(h->tiles_at, tt.data(), sizeof(int2) * tt.size()) != hipSuccess ||
+        up(h->inst, h->inst_host.data(), sizeof(InstanceRange) * count) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        set_error("vk_hip_batch_create: upload failed");
+        return fail(VK_EDEVICE);
+    }
+    if (h->mixed && nnz > 0) {
+        k_to_f32_batch<<<bgrid(nnz), 256, 0, h->stream>>>(nnz, h->val, h->val32);
+        k_to_f32_batch<<<bgrid(nnz), 256, 0, h->stream>>>(nnz, h->tval, h->tval32);
+    }
+    // Per-instance state.
+    h->states_host.assign(static_cast<size_t>(count), DevState{});
+    for (int i = 0; i < count; ++i) {
+        DevState &S = h->states_host[i];
+        S.limit = params->iteration_limit;
+        S.period = params->policy == VK_PDHG_POLICY_ADAPTIVE ? params->restart_period : kDecisionPeriod;
+        S.status = params->iteration_limit == 0 ? VK_PDHG_ITERATION_LIMIT : VK_PDHG_RUNNING;
+        S.policy = params->policy;
+        S.metric_tol = params->metric_tolerance;
+        S.rel_tol = params->relative_kkt_tolerance;
+        S.norm_b = norm_b ? norm_b[i] : 0.0;
+        S.norm_c = norm_c ? norm_c[i] : 0.0;
+        S.prev_metric = S.metric = S.rel_kkt = std::numeric_limits<double>::infinity();
+        S.kkt_ref = S.kkt_prev = std::numeric_limits<double>::infinity();
+        const double step = step_sizes ? step_sizes[i] : 0.0;   // <= 0: power iteration
+        S.omega = (params->policy == VK_PDHG_POLICY_ADAPTIVE && primal_weights) ? primal_weights[i] : 1.0;
+        if (S.omega <= 0.0 || !std::isfinite(S.omega)) S.omega = 1.0;
+        if (params->policy == VK_PDHG_POLICY_REFERENCE) {
+            S.max_step = step;
+            S.tau = S.sigma = step;
+        } else {
+            S.eta = step;
+            S.tau = step / S.omega;
+            S.sigma = step * S.omega;
+        }
+        S.inv_sigma = step > 0.0 ? 1.0 / S.sigma : 0.0;
+    }
+    if (up(h->states, h->states_host.data(), sizeof(DevState) * count) != hipSuccess) {
+        set_error("vk_hip_batch_create: state upload failed");
+        return fail(VK_EDEVICE);
+    }
+    std::vector<double> hv[9];
+    hv[0].assign(c, c + n);
+    hv[1].assign(l, l + n);
+    hv[2].assign(u, u + n);
+    if (m > 0) {
+        hv[3].assign(rl, rl + m);
+        hv[4].assign(ru, ru + m);
+        hv[5].resize(static_cast<size_t>(m));
+        for (int64_t i = 0; i < m; ++i) hv[5][i] = 1.0 / row_scale[i];
+    }
+    hv[6].assign(col_scale, col_scale + n);
+    hv[7] = x0 ? std::vector<double>(x0, x0 + n) : std::vector<double>(static_cast<size_t>(n), 0.0);
+    if (m > 0) hv[8] = y0 ? std::vector<double>(y0, y0 + m) : std::vector<double>(static_cast<size_t>(m), 0.0);
+    const int rc = h->mixed ? setup_batch_bufs<float>(h, h->a32, h->val32, h->tval32, hv)
+                            : setup_batch_bufs<double>(h, h->a64, h->val, h->tval, hv);
+    if (rc) return fail(rc);
+    *out = h;
+    return VK_OK;
+}
+
+int vk_hip_batch_run(vk_hip_batch *h, int64_t iterations_per_launch, int64_t max_launches, int *running) {
+    VK_REQUIRE(h && running && iterations_per_launch > 0 && max_launches > 0, "vk_hip_batch_run: bad arguments");
+    VK_HIP_CHECK(hipEventRecord(h->ev0, h->stream));
+    int alive = 0;
+    for (int64_t launch = 0; launch < max_launches; ++launch) {
+        if (h->mixed)
+            k_batch_iterate<float><<<h->count, kBatchBlock, 0, h->stream>>>(h->a32, static_cast<int>(iterations_per_launch));
+        else
+            k_batch_iterate<double><<<h->count, kBatchBlock, 0, h->stream>>>(h->a64, static_cast<int>(iterations_per_launch));
+        VK_HIP_CHECK(hipGetLastError());
+        if (int rc = read_states(h)) return rc;
+        alive = 0;
+        for (const auto &S : h->states_host) alive += S.status == VK_PDHG_RUNNING ? 1 : 0;
+        if (alive == 0) break;
+    }
+    VK_HIP_CHECK(hipEventRecord(h->ev1, h->stream));
+    VK_HIP_CHECK(hipEventSynchronize(h->ev1));
+    float ms = 0.0f;
+    VK_HIP_CHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    *running = alive;
+    return VK_OK;
+}
+
+int vk_hip_batch_read_state(vk_hip_batch *h, int instance, vk_pdhg_state *out) {
+    VK_REQUIRE(h && out && instance >= 0 && instance < h->count, "vk_hip_batch_read_state: bad arguments");
+    if (int rc = read_states(h)) return rc;
+    const DevState &d = h->states_host[instance];
+    std::memset(out, 0, sizeof(*out));
+    out->iterations = d.k;
+    out->status = d.status;
+    out->restarts = d.restarts;
+    out->tau = d.tau;
+    out->sigma = d.sigma;
+    out->primal_weight = d.omega;
+    out->metric = d.metric;
+    out->primal_scaled = d.rp;
+    out->dual_scaled = d.rd;
+    out->rel_primal = d.rel_p;
+    out->rel_dual = d.rel_d;
+    out->rel_gap = d.rel_g;
+    out->rel_kkt = d.rel_kkt;
+    out->primal_objective = d.pobj;
+    out->dual_objective = d.dobj;
+    out->converged_on_average = d.final_from_avg;
+    return VK_OK;
+}
+
+int vk_hip_batch_download(vk_hip_batch *h, double *x, double *y) {
+    VK_REQUIRE(h, "vk_hip_batch_download: null batch");
+    if (int rc = read_states(h)) return rc;
+    std::vector<double> xs[2], ys(static_cast<size_t>(h->m));
+    for (int p = 0; p < 2; ++p) xs[p].resize(static_cast<size_t>(h->n));
+    double *stage = nullptr;
+    VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * std::max<int64_t>(std::max(h->m, h->n), 1)));
+    auto fetch = [&](const void *src, int64_t count, double *dst) -> int {
+        if (count == 0) return VK_OK;
+        if (h->mixed) {
+            k_to_f64_batch<<<bgrid(count), 256, 0, h->stream>>>(count, static_cast<const float *>(src), stage);
+            VK_HIP_CHECK(hipMemcpyAsync(dst, stage, sizeof(double) * count, hipMemcpyDeviceToHost, h->stream));
+        } else {
+            VK_HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToHost, h->stream));
+        }
+        VK_HIP_CHECK(hipStreamSynchronize(h->stream));
+        return VK_OK;
+    };
+    int rc = VK_OK;
+    for (int p = 0; p < 2 && !rc; ++p)
+        rc = h->mixed ? fetch(h->a32.b.x[p], h->n, xs[p].data()) : fetch(h->a64.b.x[p], h->n, xs[p].data());
+    if (!rc) rc = h->mixed ? fetch(h->a32.b.y, h->m, ys.data()) : fetch(h->a64.b.y, h->m, ys.data());
+    (void)hipFree(stage);
+    if (rc) return rc;
+    for (int i = 0; i < h->count; ++i) {
+        const auto &R = h->inst_host[i];
+        const auto &src = xs[h->states_host[i].k & 1];
+        if (x) std::memcpy(x + R.c0, src.data() + R.c0, sizeof(double) * (R.c1 - R.c0));
+    }
+    if (y && h->m) std::memcpy(y, ys.data(), sizeof(double) * h->m);
+    return VK_OK;
+}
+
+int vk_hip_batch_last_run_ms(const vk_hip_batch *h, double *ms) {
+    VK_REQUIRE(h && ms, "vk_hip_batch_last_run_ms: null argument");
+    *ms = h->last_ms;
+    return VK_OK;
+}
+
+}  // extern "C"

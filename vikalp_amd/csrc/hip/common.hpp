@@ -131,6 +131,9 @@ __device__ __forceinline__ void store_partials(const Partials &p, double *partia
 constexpr int kFinalBlock = 1024;
 constexpr int kMaxGrid = 4 * kFinalBlock;
 
+// Block-wide sum / max over a block of B threads; every thread receives the
+// result.  `scratch` holds B / kWave doubles.
+template <int B = kFinalBlock>
 __device__ __forceinline__ double final_sum(double v, double *scratch) {
     v = wave_sum(v);
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
@@ -138,10 +141,11 @@ __device__ __forceinline__ double final_sum(double v, double *scratch) {
     if (lane == 0) scratch[wave] = v;
     __syncthreads();
     double r = 0.0;
-    for (int w = 0; w < kFinalBlock / kWave; ++w) r += scratch[w];
+    for (int w = 0; w < B / kWave; ++w) r += scratch[w];
     return r;
 }
 
+template <int B = kFinalBlock>
 __device__ __forceinline__ double final_max(double v, double *scratch) {
     v = wave_max(v);
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
@@ -149,7 +153,7 @@ __device__ __forceinline__ double final_max(double v, double *scratch) {
     if (lane == 0) scratch[wave] = v;
     __syncthreads();
     double r = scratch[0];
-    for (int w = 1; w < kFinalBlock / kWave; ++w) r = dmax(r, scratch[w]);
+    for (int w = 1; w < B / kWave; ++w) r = dmax(r, scratch[w]);
     return r;
 }
 
@@ -168,7 +172,7 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ parti
         double acc = v[s][0];
 #pragma unroll
         for (int u = 1; u < 4; ++u) acc = (s == 0) ? dmax(acc, v[s][u]) : acc + v[s][u];
-        const double r = (s == 0) ? final_max(acc, scratch) : final_sum(acc, scratch);
+        const double r = (s == 0) ? final_max<>(acc, scratch) : final_sum<>(acc, scratch);
         out[s] = r;
     }
 }
@@ -191,17 +195,17 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ parti
 // gathers) and `row(int row, double acc, const Pre &, Partials &)`; every row
 // is seen exactly once.
 template <class V, class X, class Epi>
-__device__ __forceinline__ void tile_spmv(const CsrView<V> &A, const X *__restrict__ x,
-                                          Epi &epi, Partials &part, double *lds_generic) {
+__device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__restrict__ x, Epi &epi,
+                                                Partials &part, double *lds_generic, int t_begin,
+                                                int t_end, int wave, int nwaves) {
     using Pre = typename Epi::Pre;
     // The slice is LDS: say so, so LDS stores do not order global loads.
     auto *lds = (__attribute__((address_space(3))) double *)(lds_generic);
     constexpr int kPer = kTileNnz / kWave;
     const int lane = threadIdx.x % kWave;
-    const int nwaves = gridDim.x * kWaves;
     const int *__restrict__ colp = A.col;
     const V *__restrict__ valp = A.val;
-    for (int t = blockIdx.x * kWaves + static_cast<int>(threadIdx.x) / kWave; t < A.ntiles; t += nwaves) {
+    for (int t = t_begin + wave; t < t_end; t += nwaves) {
         const int2 head = A.tiles[t], tail = A.tiles[t + 1];
         const int r0 = head.x, r1 = tail.x, k0 = head.y, k1 = tail.y;
         const int nnz = k1 - k0;
@@ -215,7 +219,8 @@ __device__ __forceinline__ void tile_spmv(const CsrView<V> &A, const X *__restri
                 c[u] = k < nnz ? colp[k0 + k] : 0;
                 v[u] = k < nnz ? valp[k0 + k] : V(0);
             }
-            const int rs = A.row_ptr[r0 + (lane < nr ? lane : nr)] - k0;   // lane nr: tile end
+            // Row starts of the tile, one per lane (lane r holds row r's start).
+            const int rs = lane < nr ? A.row_ptr[r0 + lane] - k0 : nnz;
             Pre pre{};
             if (lane < nr) pre = epi.prefetch(r0 + lane);
             X xv[kPer];
@@ -229,8 +234,11 @@ __device__ __forceinline__ void tile_spmv(const CsrView<V> &A, const X *__restri
             wave_sync();
             const int g = lanes_per_row(nr);
             const int lr = lane / g, sub = lane % g;
-            const int s = __shfl(rs, lr < nr ? lr : nr, kWave);
-            const int e = __shfl(rs, lr < nr ? lr + 1 : nr, kWave);
+            // A tile can hold 64 rows, so the last row's end is the tile's
+            // nonzero count rather than a 65th lane.
+            const int s = __shfl(rs, lr < nr ? lr : 0, kWave);
+            const int e_next = __shfl(rs, (lr + 1 < nr) ? lr + 1 : 0, kWave);
+            const int e = (lr + 1 < nr) ? e_next : nnz;
             double acc = 0.0;
             if (lr < nr)
                 for (int k = s + sub; k < e; k += g) acc += lds[k];
@@ -248,6 +256,15 @@ __device__ __forceinline__ void tile_spmv(const CsrView<V> &A, const X *__restri
             if (lane == 0) epi.row(r0, acc, pre, part);
         }
     }
+}
+
+// All tiles of A, one per wave of the grid (grid-stride).
+template <class V, class X, class Epi>
+__device__ __forceinline__ void tile_spmv(const CsrView<V> &A, const X *__restrict__ x, Epi &epi,
+                                          Partials &part, double *lds) {
+    tile_spmv_range(A, x, epi, part, lds, 0, A.ntiles,
+                    static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x) / kWave,
+                    static_cast<int>(gridDim.x) * kWaves);
 }
 
 }  // namespace vk

@@ -1,0 +1,412 @@
+// PDHG iteration core shared by the single-LP solver (pdhg.hip) and the
+// batched persistent solver (batch.hip): device state, buffers, the fused
+// epilogues and the step / restart / termination rules.  See pdhg.hip for
+// the algorithm and its reference citations.
+#pragma once
+
+#include "common.hpp"
+#include "vikalp_hip.h"
+
+namespace vk {
+
+constexpr int kDecisionPeriod = 50;   // pdhg.cpp:133
+
+// Device-resident solver state: written by the one-block finalizer and
+// check kernels (one thread), read by every kernel at its start.
+struct DevState {
+    long long k;            // completed iterations
+    long long ks;           // iterations since the last restart
+    long long ns;           // terms in the running sums
+    long long ns_avg;       // terms of the average evaluated at the last check
+    long long limit;
+    long long period;       // adaptive restart-check period
+    int status;             // VK_PDHG_*
+    int policy;
+    int restart_flag;       // reference: the next dual step uses xbar = x+
+    int restart_apply;      // adaptive: the last check restarted
+    int restart_to_avg;     // adaptive: ... to the average
+    int final_from_avg;     // adaptive: converged on the average
+    int restarts;
+    double tau, sigma, inv_sigma, eta, omega, max_step;
+    double prev_metric, metric, rp, rd;
+    double metric_tol, rel_tol, norm_b, norm_c;
+    double rel_p, rel_d, rel_g, rel_kkt, pobj, dobj;
+    double cur_pr2, cur_dr2, cur_gap, cur_dx2, cur_dy2;
+    double kkt_ref, kkt_prev;
+};
+
+// Partial-slot layout (slot 0 combines with max, the rest with +).
+//   rows: 0 max|P(ax)-ax| (scaled)   1 sum (orig primal residual)^2
+//         2 sum (scaled primal residual)^2   3 dual-objective row terms
+//         4 count of -inf dual terms   5 sum (y - y_last)^2
+//   cols: 0 max|x-P(x-g)| (scaled)   1 sum (orig dual residual)^2
+//         2 sum (scaled dual residual)^2   3 dual-objective column terms
+//         4 count of -inf dual terms   5 c'x   6 sum (x - x_last)^2
+
+template <class T>
+struct Bufs {
+    T *x[2];            // x_k in x[k & 1]
+    T *y, *ax, *aty;    // current y, A x, A' y (in place)
+    const T *c, *l, *u, *rl, *ru;
+    const T *row_unscale, *col_scale;   // 1/r and s
+    T *xsum, *ysum, *xlast, *ylast, *ax_avg, *aty_avg;  // adaptive only
+    double *part_rows, *part_cols, *part_erows, *part_ecols;
+    DevState *st;
+    int m, n;
+    int grid_rows, grid_cols;
+};
+
+__device__ __forceinline__ double kkt_weighted(double w, double pr2, double dr2, double gap) {
+    return sqrt(w * w * pr2 + dr2 / (w * w) + gap * gap);
+}
+
+// ------------------------------------------------------------------ epilogues
+
+// Dual step on row i given acc = (A x+)_i.
+template <class T>
+struct DualEpi {
+    const Bufs<T> &b;
+    double sigma, inv_sigma;
+    bool restart;
+    bool sign_exact;
+    struct Pre {
+        T ax, y, lo, hi, unscale, ysum, ylast;
+    };
+    __device__ Pre prefetch(int i) const {
+        Pre p;
+        p.ax = b.ax[i];
+        p.y = b.y[i];
+        p.lo = b.rl[i];
+        p.hi = b.ru[i];
+        p.unscale = b.row_unscale[i];
+        p.ysum = b.ysum ? b.ysum[i] : T(0);
+        p.ylast = b.ylast ? b.ylast[i] : T(0);
+        return p;
+    }
+    __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
+        const double axn = acc;
+        const double ax_old = static_cast<double>(q.ax);
+        const double axbar = restart ? axn : 2.0 * axn - ax_old;
+        const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
+        const double v = static_cast<double>(q.y) + sigma * axbar;
+        double yn;
+        if (sign_exact) {
+            // Same Moreau step, written so that y+ is exactly 0 inside the
+            // box and carries the sign of the active bound outside it (a
+            // one-sided row never gets a dual of the wrong sign from
+            // round-off, which would make the dual objective -inf).
+            const double t = inv_sigma * v;
+            yn = (t < lo) ? fmin(v - sigma * lo, 0.0) : ((hi < t) ? fmax(v - sigma * hi, 0.0) : 0.0);
+        } else {
+            const double proj = clamp_box(inv_sigma * v, lo, hi);   // pdhg.cpp:108-112
+            yn = v - sigma * proj;
+        }
+        b.y[i] = static_cast<T>(yn);
+        b.ax[i] = static_cast<T>(axn);
+        if (b.ysum) b.ysum[i] = q.ysum + static_cast<T>(yn);
+        const double d = clamp_box(axn, lo, hi) - axn;
+        p.v[0] = dmax(p.v[0], fabs(d));
+        const double du = d * static_cast<double>(q.unscale);
+        p.v[1] += du * du;
+        p.v[2] += d * d;
+        if (yn > 0.0) {
+            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= yn * hi;
+        } else if (yn < 0.0) {
+            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= yn * lo;
+        }
+        if (b.ylast) {
+            const double dy = yn - static_cast<double>(q.ylast);
+            p.v[5] += dy * dy;
+        }
+    }
+};
+
+// Column terms of the KKT sums for (x_j, g_j = c_j + aty_j) (shared by the
+// per-iteration and the average evaluation).  s = col_scale[j].
+__device__ __forceinline__ void column_terms(double xj, double cj, double g, double lj, double uj,
+                                             double s, bool have_last, double xlast, Partials &p) {
+    const double d = xj - clamp_box(xj - g, lj, uj);
+    p.v[0] = dmax(p.v[0], fabs(d));
+    p.v[2] += d * d;
+    const double dorig = s * (xj - clamp_box(xj - g / (s * s), lj, uj));
+    p.v[1] += dorig * dorig;
+    p.v[5] += cj * xj;
+    if (g > 0.0) {
+        if (isinf(lj)) p.v[4] += 1.0; else p.v[3] += g * lj;
+    } else if (g < 0.0) {
+        if (isinf(uj)) p.v[4] += 1.0; else p.v[3] += g * uj;
+    }
+    if (have_last) {
+        const double dx = xj - xlast;
+        p.v[6] += dx * dx;
+    }
+}
+
+// A' y on column j, the residual terms, and (fused) the next primal step.
+template <class T>
+struct PrimalEpi {
+    const Bufs<T> &b;
+    const T *xcur;
+    T *xnext;
+    double tau;
+    bool fuse;
+    struct Pre {
+        T x, c, l, u, s, xsum, xlast;
+    };
+    __device__ Pre prefetch(int j) const {
+        Pre p;
+        p.x = xcur[j];
+        p.c = b.c[j];
+        p.l = b.l[j];
+        p.u = b.u[j];
+        p.s = b.col_scale[j];
+        p.xsum = (fuse && b.xsum) ? b.xsum[j] : T(0);
+        p.xlast = b.xlast ? b.xlast[j] : T(0);
+        return p;
+    }
+    __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
+        b.aty[j] = static_cast<T>(acc);
+        const double xj = static_cast<double>(q.x);
+        const double cj = static_cast<double>(q.c);
+        const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
+        column_terms(xj, cj, cj + acc, lj, uj, static_cast<double>(q.s), b.xlast != nullptr,
+                     static_cast<double>(q.xlast), p);
+        if (fuse) {
+            const double xn = clamp_box((xj - tau * cj) - tau * acc, lj, uj);
+            xnext[j] = static_cast<T>(xn);
+            if (b.xsum) b.xsum[j] = q.xsum + static_cast<T>(xn);
+        }
+    }
+};
+
+// Average evaluation, rows: acc = (A xsum)_i, scaled by 1/ns.
+template <class T>
+struct EvalRowsEpi {
+    const Bufs<T> &b;
+    double inv_ns;
+    struct Pre {
+        T ysum, lo, hi, unscale, ylast;
+    };
+    __device__ Pre prefetch(int i) const {
+        return Pre{b.ysum[i], b.rl[i], b.ru[i], b.row_unscale[i], b.ylast[i]};
+    }
+    __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
+        const double axa = acc * inv_ns;
+        b.ax_avg[i] = static_cast<T>(axa);
+        const double ya = static_cast<double>(q.ysum) * inv_ns;
+        const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
+        const double d = clamp_box(axa, lo, hi) - axa;
+        p.v[0] = dmax(p.v[0], fabs(d));
+        const double du = d * static_cast<double>(q.unscale);
+        p.v[1] += du * du;
+        p.v[2] += d * d;
+        if (ya > 0.0) {
+            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= ya * hi;
+        } else if (ya < 0.0) {
+            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= ya * lo;
+        }
+        const double dy = ya - static_cast<double>(q.ylast);
+        p.v[5] += dy * dy;
+    }
+};
+
+template <class T>
+struct EvalColsEpi {
+    const Bufs<T> &b;
+    double inv_ns;
+    struct Pre {
+        T xsum, c, l, u, s, xlast;
+    };
+    __device__ Pre prefetch(int j) const {
+        return Pre{b.xsum[j], b.c[j], b.l[j], b.u[j], b.col_scale[j], b.xlast[j]};
+    }
+    __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
+        const double atya = acc * inv_ns;
+        b.aty_avg[j] = static_cast<T>(atya);
+        const double xa = static_cast<double>(q.xsum) * inv_ns;
+        const double cj = static_cast<double>(q.c);
+        column_terms(xa, cj, cj + atya, static_cast<double>(q.l), static_cast<double>(q.u),
+                     static_cast<double>(q.s), true, static_cast<double>(q.xlast), p);
+    }
+};
+
+// ------------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ void relative_kkt(const DevState &s, const double *r, const double *c,
+                                             double &rel_p, double &rel_d, double &rel_g,
+                                             double &gap_abs, double &pobj, double &dobj) {
+    rel_p = sqrt(r[1]) / (1.0 + s.norm_b);
+    rel_d = sqrt(c[1]) / (1.0 + s.norm_c);
+    pobj = c[5];
+    const bool neg_inf = (r[4] + c[4]) > 0.0;
+    dobj = neg_inf ? -INFINITY : r[3] + c[3];
+    if (neg_inf) {
+        rel_g = 1.0;
+        gap_abs = INFINITY;
+    } else {
+        gap_abs = fabs(pobj - dobj);
+        rel_g = gap_abs / (1.0 + fabs(pobj) + fabs(dobj));
+    }
+}
+
+// Whether the primal kernel of iteration k may fuse the next primal step
+// (tau is known to stay the same).
+__device__ __forceinline__ bool may_fuse(const DevState &s) {
+    if (s.policy == VK_PDHG_POLICY_REFERENCE) return !(s.k > 0 && s.k % kDecisionPeriod == 0);
+    return (s.k + 1) % s.period != 0;
+}
+
+// Step / restart / termination rule after one iteration, from the reduced
+// dual (red_rows) and primal (red_cols) partial slots (pdhg.cpp:131-150 for
+// the reference policy).  Runs in one thread.
+__device__ inline void finalize_step(DevState &S, const double *red_rows, const double *red_cols) {
+    const long long k = S.k;
+    const double rp = red_rows[0], rd = red_cols[0];
+    const double metric = dmax(rp, rd);   // pdhg.cpp:131
+    double rel_p, rel_d, rel_g, gap_abs, pobj, dobj;
+    relative_kkt(S, red_rows, red_cols, rel_p, rel_d, rel_g, gap_abs, pobj, dobj);
+    const double rel_kkt = dmax(rel_p, dmax(rel_d, rel_g));
+    S.rp = rp;
+    S.rd = rd;
+    S.metric = metric;
+    S.rel_p = rel_p;
+    S.rel_d = rel_d;
+    S.rel_g = rel_g;
+    S.rel_kkt = rel_kkt;
+    S.pobj = pobj;
+    S.dobj = dobj;
+    S.cur_pr2 = red_rows[2];
+    S.cur_dr2 = red_cols[2];
+    S.cur_gap = gap_abs;
+    S.cur_dx2 = red_cols[6];
+    S.cur_dy2 = red_rows[5];
+    if (S.policy == VK_PDHG_POLICY_REFERENCE) {
+        // pdhg.cpp:133-145; the restart flag was consumed by this iteration's
+        // dual step (pdhg.cpp:98-100).
+        S.restart_flag = 0;
+        if (k > 0 && k % kDecisionPeriod == 0) {
+            if (metric > S.prev_metric * 1.25) {
+                S.tau *= 0.5;
+                S.sigma *= 0.5;
+                S.restart_flag = 1;
+            } else if (metric < S.prev_metric * 0.1) {
+                const double grown = S.tau * 1.05;
+                S.tau = (S.max_step < grown) ? S.max_step : grown;
+                S.sigma = S.tau;
+            }
+            S.prev_metric = metric;
+        } else if (k == 0) {
+            S.prev_metric = metric;
+        }
+        S.inv_sigma = 1.0 / S.sigma;
+    }
+    const bool converged = (S.rel_tol > 0.0) ? (rel_kkt <= S.rel_tol) : (metric <= S.metric_tol);  // pdhg.cpp:147
+    S.k = k + 1;
+    S.ks += 1;
+    S.ns += 1;
+    if (converged) S.status = VK_PDHG_CONVERGED;
+    else if (S.k >= S.limit) S.status = VK_PDHG_ITERATION_LIMIT;
+}
+
+// Adaptive restart rule on the evaluated average (er/ec: reduced partial
+// slots of the average's rows and columns).  `init` records the start
+// point's KKT error as the first restart reference.  Runs in one thread.
+__device__ inline void check_step(DevState &S, const double *red_rows, const double *red_cols, bool init) {
+    double rel_p, rel_d, rel_g, gap_abs, pobj, dobj;
+    relative_kkt(S, red_rows, red_cols, rel_p, rel_d, rel_g, gap_abs, pobj, dobj);
+    const double w = S.omega;
+    const double kkt_avg = kkt_weighted(w, red_rows[2], red_cols[2], gap_abs);
+    if (init) {
+        S.kkt_ref = kkt_avg;
+        S.kkt_prev = INFINITY;
+        return;
+    }
+    const double rel_avg = dmax(rel_p, dmax(rel_d, rel_g));
+    S.ns_avg = S.ns;
+    S.restart_apply = 0;
+    S.restart_to_avg = 0;
+    if (S.rel_tol > 0.0 && rel_avg <= S.rel_tol) {
+        S.status = VK_PDHG_CONVERGED;
+        S.final_from_avg = 1;
+        S.rel_p = rel_p;
+        S.rel_d = rel_d;
+        S.rel_g = rel_g;
+        S.rel_kkt = rel_avg;
+        S.pobj = pobj;
+        S.dobj = dobj;
+        return;
+    }
+    const double kkt_cur = kkt_weighted(w, S.cur_pr2, S.cur_dr2, S.cur_gap);
+    const bool use_avg = kkt_avg < kkt_cur;
+    const double kc = use_avg ? kkt_avg : kkt_cur;
+    const bool restart = kc <= 0.2 * S.kkt_ref ||
+                         (kc <= 0.8 * S.kkt_ref && kc > S.kkt_prev) ||
+                         static_cast<double>(S.ks) >= 0.36 * static_cast<double>(S.k);
+    S.kkt_prev = kc;
+    if (!restart) return;
+    const double dx = sqrt(use_avg ? red_cols[6] : S.cur_dx2);
+    const double dy = sqrt(use_avg ? red_rows[5] : S.cur_dy2);
+    if (dx > 1e-10 && dy > 1e-10) S.omega = exp(0.5 * log(dy / dx) + 0.5 * log(S.omega));
+    S.tau = S.eta / S.omega;
+    S.sigma = S.eta * S.omega;
+    S.inv_sigma = 1.0 / S.sigma;
+    S.kkt_ref = use_avg ? kkt_weighted(S.omega, red_rows[2], red_cols[2], gap_abs)
+                        : kkt_weighted(S.omega, S.cur_pr2, S.cur_dr2, S.cur_gap);
+    S.kkt_prev = INFINITY;
+    S.ks = 0;
+    S.ns = 0;
+    S.restart_apply = 1;
+    S.restart_to_avg = use_avg ? 1 : 0;
+    S.restarts += 1;
+}
+
+// The post-check update, split by index kind: move to the average if chosen
+// (or converged on it), record the restart point, clear the sums, and take
+// the next primal step (columns) / refresh y and A x (rows).
+template <class T>
+__device__ __forceinline__ void apply_column(const Bufs<T> &b, const DevState &s, int j, T *xcur, T *xnext) {
+    const bool copy_avg = s.final_from_avg || (s.restart_apply && s.restart_to_avg);
+    const double inv = 1.0 / static_cast<double>(s.ns_avg > 0 ? s.ns_avg : 1);
+    if (copy_avg) {
+        xcur[j] = static_cast<T>(static_cast<double>(b.xsum[j]) * inv);
+        b.aty[j] = b.aty_avg[j];
+    }
+    if (s.restart_apply) {
+        b.xlast[j] = xcur[j];
+        b.xsum[j] = static_cast<T>(0);
+    }
+    if (s.status == VK_PDHG_RUNNING) {
+        const double xj = static_cast<double>(xcur[j]);
+        const double xn = clamp_box((xj - s.tau * static_cast<double>(b.c[j])) -
+                                        s.tau * static_cast<double>(b.aty[j]),
+                                    static_cast<double>(b.l[j]), static_cast<double>(b.u[j]));
+        xnext[j] = static_cast<T>(xn);
+        b.xsum[j] += static_cast<T>(xn);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void apply_row(const Bufs<T> &b, const DevState &s, int i) {
+    const bool copy_avg = s.final_from_avg || (s.restart_apply && s.restart_to_avg);
+    const double inv = 1.0 / static_cast<double>(s.ns_avg > 0 ? s.ns_avg : 1);
+    if (copy_avg) {
+        b.y[i] = static_cast<T>(static_cast<double>(b.ysum[i]) * inv);
+        b.ax[i] = b.ax_avg[i];
+    }
+    if (s.restart_apply) {
+        b.ylast[i] = b.y[i];
+        b.ysum[i] = static_cast<T>(0);
+    }
+}
+
+// One element of the standalone primal step x[k+1] = P_X(x[k] - tau (c + aty)).
+template <class T>
+__device__ __forceinline__ void pstep_element(const Bufs<T> &b, const DevState &s, int j, const T *xcur, T *xnext) {
+    const double xj = static_cast<double>(xcur[j]);
+    const double xn = clamp_box((xj - s.tau * static_cast<double>(b.c[j])) - s.tau * static_cast<double>(b.aty[j]),
+                                static_cast<double>(b.l[j]), static_cast<double>(b.u[j]));
+    xnext[j] = static_cast<T>(xn);
+    if (b.xsum) b.xsum[j] += static_cast<T>(xn);
+}
+
+}  // namespace vk

@@ -11,6 +11,7 @@
 #include <exception>
 #include <span>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -142,6 +143,64 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets, const int64_
         out[3] = p.step_size;
     } catch (const std::exception &e) {
         return -2;
+    }
+    return 0;
+}
+
+int vk_lp_solve_batch(int count, const int64_t *row_starts, const int64_t *col_starts,
+                      const int64_t *row_offsets, const int64_t *column_indices, const double *values,
+                      const double *objective, const double *row_lower, const double *row_upper,
+                      const double *var_lower, const double *var_upper, const vk_solver_options *options,
+                      double *x_out, double *y_out, vk_solve_info *info) {
+    if (count <= 0 || !row_starts || !col_starts || !row_offsets || !options || !info) return -1;
+    if (row_starts[0] != 0 || col_starts[0] != 0) return -1;
+    for (int i = 0; i < count; ++i)
+        if (row_starts[i + 1] < row_starts[i] || col_starts[i + 1] < col_starts[i]) return -1;
+    std::vector<Model> models(static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) {
+        const int64_t r0 = row_starts[i], m = row_starts[i + 1] - r0;
+        const int64_t c0 = col_starts[i], n = col_starts[i + 1] - c0;
+        Model &mod = models[i];
+        mod.name = "batch-" + std::to_string(i);
+        mod.linear_objective.assign(objective + c0, objective + c0 + n);
+        mod.variable_lower.assign(var_lower + c0, var_lower + c0 + n);
+        mod.variable_upper.assign(var_upper + c0, var_upper + c0 + n);
+        mod.variable_types.assign(static_cast<size_t>(n), VariableType::Continuous);
+        mod.constraint_lower.assign(row_lower + r0, row_lower + r0 + m);
+        mod.constraint_upper.assign(row_upper + r0, row_upper + r0 + m);
+        auto &pat = mod.constraint_matrix.pattern;
+        pat.rows = m;
+        pat.columns = n;
+        pat.row_offsets.assign(1, 0);
+        for (int64_t r = r0; r < r0 + m; ++r) {
+            for (int64_t k = row_offsets[r]; k < row_offsets[r + 1]; ++k) {
+                pat.column_indices.push_back(column_indices[k] - c0);   // out-of-range -> InvalidModel
+                mod.constraint_matrix.values.push_back(values[k]);
+            }
+            pat.row_offsets.push_back(static_cast<Index>(pat.column_indices.size()));
+        }
+    }
+    std::vector<const Model *> ptrs;
+    for (const auto &mod : models) ptrs.push_back(&mod);
+    std::vector<SolveResult> res;
+    try {
+        res = solver::solve_lp_batch(ptrs, to_options(*options));
+    } catch (const std::exception &e) {
+        res.assign(static_cast<size_t>(count), SolveResult{});
+        for (auto &r : res) {
+            r.status = SolveStatus::NumericalFailure;
+            r.message = e.what();
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        fill_info(res[i], info[i]);
+        const auto &r = res[i];
+        const int64_t r0 = row_starts[i], c0 = col_starts[i];
+        if (x_out && r.primal_solution.size() == static_cast<size_t>(col_starts[i + 1] - c0))
+            std::memcpy(x_out + c0, r.primal_solution.data(), sizeof(double) * r.primal_solution.size());
+        if (y_out && r.constraint_upper_dual.size() == static_cast<size_t>(row_starts[i + 1] - r0))
+            for (size_t k = 0; k < r.constraint_upper_dual.size(); ++k)
+                y_out[r0 + k] = r.constraint_upper_dual[k] - r.constraint_lower_dual[k];
     }
     return 0;
 }

@@ -7,6 +7,7 @@
 #include "vikalp/contracts/SolverOptions.hpp"
 
 #include <span>
+#include <vector>
 
 namespace vikalp::solver {
 
@@ -18,6 +19,14 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend,
                      std::span<const Scalar> warm_start = {},
                      std::span<const Scalar> variable_lower_override = {},
                      std::span<const Scalar> variable_upper_override = {});
+
+// Solves every LP of `models` together on the batched persistent HIP kernel
+// (one launch runs many instances; each keeps its own step sizes, restart
+// state and status).  Results are in the order of `models`, each with
+// solve_lp's semantics; device_seconds is the batch's shared device time.
+// Instances that fail the entry checks get their status without running.
+std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models,
+                                        const SolverOptions &options = {});
 
 // Mean device time (ms, HIP events on the solver's stream) of the fused dual
 // and primal kernels and of a whole iteration, after `warm_iterations`

@@ -1,0 +1,45 @@
+#pragma once
+// Host helpers shared by the single-LP and the batched PDHG paths
+// (definitions in pdhg_lp.cpp).  Each follows the reference function named
+// beside it.
+
+#include "vikalp/contracts/Model.hpp"
+#include "vikalp/contracts/SolveResult.hpp"
+#include "vikalp/contracts/SolverOptions.hpp"
+
+#include <chrono>
+#include <span>
+#include <string>
+#include <vector>
+
+namespace vikalp::solver::detail {
+
+using Clock = std::chrono::steady_clock;
+constexpr int kRuizPasses = 10;
+
+struct Bounds {
+    std::vector<Scalar> lower, upper;
+};
+
+// Scaled LP data (continuous_common.hpp:24-34): values, c, l, u, rl, ru and
+// the row / column factors r, s (x = s * xhat, y = r * yhat).
+struct Scaled {
+    std::vector<Scalar> values, c, l, u, rl, ru, r, s;
+};
+
+SolveResult status_result(SolveStatus status, const std::string &message);
+bool valid_options(const SolverOptions &o);                              // continuous_common.hpp:50-56
+bool timed_out(Clock::time_point started, const SolverOptions &o);        // continuous_common.hpp:58-63
+bool resolve_bounds(const Model &model, std::span<const Scalar> lo, std::span<const Scalar> hi,
+                    Bounds &b, std::string &message);                     // continuous_common.hpp:80-110
+Scaled scale(const Model &model, const Bounds &bounds);                   // continuous_common.hpp:133-205
+void equilibrate(const CsrPattern &p, Scaled &sc, int ruiz);              // adaptive policy only
+Scalar operator_bound(const CsrPattern &p, const std::vector<Scalar> &v); // continuous_common.hpp:207-225
+Scalar row_bound_norm(const std::vector<Scalar> &lo, const std::vector<Scalar> &hi);
+Scalar norm2(const std::vector<Scalar> &v);
+void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
+               const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,
+               SolveResult &res);                                          // pdhg.cpp:153-264
+void check_vk(int rc, const char *what);
+
+}  // namespace vikalp::solver::detail

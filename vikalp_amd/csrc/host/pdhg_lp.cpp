@@ -13,6 +13,8 @@
 // reference's O(n*nnz) column loop (continuous_common.hpp:162-176): a maximum
 // over a set does not depend on visiting order.
 
+#include "lp_internal.hpp"
+
 #include "vikalp/backend/HipBackend.hpp"
 #include "vikalp/solver/continuous.hpp"
 
@@ -26,10 +28,8 @@
 #include <vector>
 
 namespace vikalp::solver {
-namespace {
+namespace detail {
 
-using Clock = std::chrono::steady_clock;
-constexpr int kRuizPasses = 10;
 constexpr Scalar kInf = std::numeric_limits<Scalar>::infinity();
 
 SolveResult status_result(SolveStatus status, const std::string &message) {
@@ -53,10 +53,6 @@ bool timed_out(Clock::time_point started, const SolverOptions &o) {
     return std::isfinite(o.time_limit_seconds) &&
            std::chrono::duration<Scalar>(Clock::now() - started).count() >= o.time_limit_seconds;
 }
-
-struct Bounds {
-    std::vector<Scalar> lower, upper;
-};
 
 // continuous_common.hpp:80-110
 bool resolve_bounds(const Model &model, std::span<const Scalar> lo, std::span<const Scalar> hi,
@@ -86,10 +82,6 @@ bool resolve_bounds(const Model &model, std::span<const Scalar> lo, std::span<co
         }
     return true;
 }
-
-struct Scaled {
-    std::vector<Scalar> values, c, l, u, rl, ru, r, s;
-};
 
 Scalar scale_bound(Scalar v, Scalar f, bool multiply) {
     if (!std::isfinite(v)) return v;
@@ -392,6 +384,12 @@ void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &op
 void check_vk(int rc, const char *what) {
     if (rc != VK_OK) throw std::runtime_error(std::string(what) + ": " + vk_hip_last_error());
 }
+
+}  // namespace detail
+
+namespace {
+
+using namespace detail;
 
 struct DeviceSolver {
     vk_hip_pdhg *h = nullptr;

@@ -144,6 +144,14 @@ HIP_SYMBOLS = {
     "vk_hip_pdhg_last_run_ms": (C.c_int, [_vp, _dp]),
     "vk_hip_pdhg_profile": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp]),
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
+    "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(PdhgParams),
+                                      C.POINTER(_vp)]),
+    "vk_hip_batch_run": (C.c_int, [_vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)]),
+    "vk_hip_batch_read_state": (C.c_int, [_vp, C.c_int, C.POINTER(PdhgState)]),
+    "vk_hip_batch_download": (C.c_int, [_vp, _dp, _dp]),
+    "vk_hip_batch_last_run_ms": (C.c_int, [_vp, _dp]),
+    "vk_hip_batch_destroy": (C.c_int, [_vp]),
 }
 
 HOST_SYMBOLS = {
@@ -151,6 +159,8 @@ HOST_SYMBOLS = {
     "vk_lp_solve": (C.c_int, [C.c_int, C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, C.c_double,
                               _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(_SolverOptions),
                               _dp, _dp, C.POINTER(_SolveInfo)]),
+    "vk_lp_solve_batch": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                    C.POINTER(_SolverOptions), _dp, _dp, C.POINTER(_SolveInfo)]),
     "vk_lp_profile": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 C.POINTER(_SolverOptions), C.c_int64, C.c_int64, _dp]),
     "vk_lp_generate": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_vp)]),
@@ -362,6 +372,45 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     if rc != 0:
         raise RuntimeError(f"vk_lp_profile failed ({rc}): {hip_lib().vk_hip_last_error()}")
     return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3]}
+
+
+def _result(info, x, y):
+    return Result(STATUS_NAMES[info.status], info.iterations, info.objective_value,
+                  info.primal_residual, info.stationarity_residual, info.complementarity_residual,
+                  info.absolute_gap, info.relative_gap, info.solve_seconds, info.device_seconds,
+                  info.kkt_relative_primal, info.kkt_relative_dual, info.kkt_relative_gap,
+                  info.kkt_relative, info.kkt_primal_objective, info.kkt_dual_objective,
+                  info.low_precision_iterations, info.backend.decode(), info.solver.decode(),
+                  info.message.decode(), x, y)
+
+
+def solve_batch(lps: list, options: Options | None = None) -> list:
+    """solver::solve_lp_batch on the HIP batched persistent kernel."""
+    L = host_lib()
+    count = len(lps)
+    rs = np.zeros(count + 1, np.int64)
+    cs = np.zeros(count + 1, np.int64)
+    for i, lp in enumerate(lps):
+        rs[i + 1] = rs[i] + lp.m
+        cs[i + 1] = cs[i] + lp.n
+    rp = [np.zeros(1, np.int64)]
+    nnz0 = 0
+    for lp in lps:
+        rp.append(np.asarray(lp.row_offsets[1:], np.int64) + nnz0)
+        nnz0 += lp.nnz
+    rp = np.concatenate(rp)
+    ci = np.concatenate([np.asarray(lp.column_indices, np.int64) + cs[i] for i, lp in enumerate(lps)])
+    cat = lambda k: np.ascontiguousarray(np.concatenate([np.asarray(getattr(lp, k), np.float64) for lp in lps]))
+    v, c, rl, ru, l, u = (cat(k) for k in ("values", "objective", "row_lower", "row_upper", "var_lower", "var_upper"))
+    x = np.zeros(int(cs[-1]))
+    y = np.zeros(int(rs[-1]))
+    infos = (_SolveInfo * count)()
+    copt = (options or Options())._c()
+    rc = L.vk_lp_solve_batch(count, _ptr(rs, _lp), _ptr(cs, _lp), _ptr(rp, _lp), _ptr(ci, _lp), _ptr(v), _ptr(c),
+                             _ptr(rl), _ptr(ru), _ptr(l), _ptr(u), C.byref(copt), _ptr(x), _ptr(y), infos)
+    if rc != 0:
+        raise ValueError(f"vk_lp_solve_batch rejected its arguments ({rc})")
+    return [_result(infos[i], x[cs[i]:cs[i + 1]].copy(), y[rs[i]:rs[i + 1]].copy()) for i in range(count)]
 
 
 def solve(lp: LP, options: Options | None = None) -> Result:
