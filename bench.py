@@ -43,21 +43,38 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(m, n, nnz, value_bytes, adaptive=True):
-    """Algorithmic (compulsory) HBM bytes per launch of the two fused kernels.
+def kernel_table(prof, m, n, nnz, sv, suffix):
+    """Algorithmic (compulsory) HBM bytes and achieved GB/s per kernel of one
+    iteration, for the layout the solver used.
 
-    dual   (rows of A):  values+cols (s_v+4)*nnz, row offsets 4(m+1), x gathered
-                         once s_v*n; per row read ax, y, rl, ru, 1/r, y_last and
-                         y_sum, write ax, y, y_sum: 10*s_v*m.
-    primal (rows of A'): values+rows (s_v+4)*nnz, offsets 4(n+1), y gathered
-                         once s_v*m; per column read x, c, l, u, s, x_last,
-                         x_sum, write aty, x_next, x_sum: 10*s_v*n.
+    SpMV y = A x (SpMV-only kernel of a split layout): values and column
+    indices (s_v + 4) nnz, row offsets 4 (rows + 1), x gathered once
+    s_v cols, FP64 sums written 8 rows.  Column blocks add offsets and
+    partial sums that are not counted (they lower the achieved fraction).
+    Row pass (dual): FP64 sums 8 m, plus per row read ax, y, rl, ru, 1/r,
+    y_last, write ax, y (8 s_v m) and read + write the FP64 y_sum (16 m).
+    Column pass (primal): FP64 sums 8 n, per column read x, c, l, u, s,
+    x_last, write aty, x_next (8 s_v n), FP64 x_sum (16 n).
+    A fused kernel counts its SpMV without the sum store plus its pass
+    without the sum load.
     """
-    sv = value_bytes
-    per = 10 if adaptive else 7
-    dual = (sv + 4) * nnz + 4 * (m + 1) + sv * n + per * sv * m
-    primal = (sv + 4) * nnz + 4 * (n + 1) + sv * m + per * sv * n
-    return dual, primal
+    spmv_a = (sv + 4) * nnz + 4 * (m + 1) + sv * n + 8 * m
+    spmv_at = (sv + 4) * nnz + 4 * (n + 1) + sv * m + 8 * n
+    pass_rows = 8 * m + 8 * sv * m + 16 * m
+    pass_cols = 8 * n + 8 * sv * n + 16 * n
+    rows = []
+    if prof["col_blocks"] > 1 or prof["split_passes"]:
+        rows.append(("k_blocks" if prof["col_blocks"] > 1 else "k_dual_spmv", prof["dual_spmv_ms"], spmv_a))
+        rows.append(("k_dual_rows", prof["dual_pass_ms"], pass_rows))
+    else:
+        rows.append(("k_dual", prof["dual_spmv_ms"], spmv_a - 8 * m + pass_rows - 8 * m))
+    if prof["split_passes"]:
+        rows.append(("k_primal_spmv", prof["primal_spmv_ms"], spmv_at))
+        rows.append(("k_primal_cols", prof["primal_pass_ms"], pass_cols))
+    else:
+        rows.append(("k_primal", prof["primal_spmv_ms"], spmv_at - 8 * n + pass_cols - 8 * n))
+    return {f"{name}<{suffix}>": {"ms": ms, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9}
+            for name, ms, b in rows}
 
 
 def main():
@@ -96,14 +113,12 @@ def main():
     # Kernel timing for the roofline (HIP events on the solver's stream).
     prof = V.profile_kernels(lp, options(V.PdhgPrecision.FP64), warm_iterations=30, iterations=60)
     prof32 = V.profile_kernels(lp, options(V.PdhgPrecision.Mixed), warm_iterations=30, iterations=60)
-    d64, p64 = kernel_bytes(lp.m, lp.n, lp.nnz, 8)
-    d32, p32 = kernel_bytes(lp.m, lp.n, lp.nnz, 4)
-    log(f"fp64 kernels: dual {prof['dual_ms']:.4f} ms ({d64 / prof['dual_ms'] / 1e6:.0f} GB/s), "
-        f"primal {prof['primal_ms']:.4f} ms ({p64 / prof['primal_ms'] / 1e6:.0f} GB/s), "
-        f"iteration {prof['iteration_ms']:.4f} ms")
-    log(f"mixed kernels: dual {prof32['dual_ms']:.4f} ms ({d32 / prof32['dual_ms'] / 1e6:.0f} GB/s), "
-        f"primal {prof32['primal_ms']:.4f} ms ({p32 / prof32['primal_ms'] / 1e6:.0f} GB/s), "
-        f"iteration {prof32['iteration_ms']:.4f} ms")
+    k64 = kernel_table(prof, lp.m, lp.n, lp.nnz, 8, "double")
+    k32 = kernel_table(prof32, lp.m, lp.n, lp.nnz, 4, "float")
+    for label, p, table in (("fp64", prof, k64), ("mixed", prof32, k32)):
+        log(f"{label}: iteration {p['iteration_ms']:.4f} ms, col_blocks={p['col_blocks']} "
+            f"split={p['split_passes']}; " +
+            ", ".join(f"{k} {v['ms'] * 1e3:.1f} us {v['GBs']:.0f} GB/s" for k, v in table.items()))
 
     def run(precision, label):
         log(f"solve {label} start")
@@ -136,10 +151,9 @@ def main():
     r32, r64 = mixed_runs[-1], fp64_runs[-1]
     obj_err = abs(r32.objective_value - r64.objective_value) / max(1.0, abs(r64.objective_value))
 
-    dominant = "primal" if prof["primal_ms"] >= prof["dual_ms"] else "dual"
-    dom_ms = prof[dominant + "_ms"]
-    dom_bytes = p64 if dominant == "primal" else d64
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    dominant = max(k64, key=lambda k: k64[k]["ms"])
+    dom_ms, dom_bytes = k64[dominant]["ms"], k64[dominant]["bytes"]
+    achieved = k64[dominant]["GBs"]
 
     cpu = None
     if not args.skip_cpu:
@@ -182,14 +196,10 @@ def main():
         "ms_per_iteration_mixed": prof32["iteration_ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_{dominant}<double>", "bytes_per_launch": dom_bytes,
-                     "mean_ms": dom_ms},
-        "kernels": {
-            "fp64": {"dual_ms": prof["dual_ms"], "primal_ms": prof["primal_ms"],
-                     "dual_GBs": d64 / prof["dual_ms"] / 1e6, "primal_GBs": p64 / prof["primal_ms"] / 1e6},
-            "mixed": {"dual_ms": prof32["dual_ms"], "primal_ms": prof32["primal_ms"],
-                      "dual_GBs": d32 / prof32["dual_ms"] / 1e6, "primal_GBs": p32 / prof32["primal_ms"] / 1e6},
-        },
+                     "kernel": dominant, "bytes_per_launch": dom_bytes, "mean_ms": dom_ms},
+        "layout": {"fp64": {"col_blocks": prof["col_blocks"], "split_passes": prof["split_passes"]},
+                   "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"]}},
+        "kernels": {"fp64": k64, "mixed": k32},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -144,6 +144,14 @@ typedef struct {
     double norm_c;                 /*   coordinates (relative KKT denominators) */
     int use_graphs;                /* capture iteration blocks in hipGraphs   */
     int grid_blocks;               /* 0 = auto; else blocks per SpMV launch   */
+    int col_blocks;                /* A x layout: 0 = auto (4 column blocks
+                                      once x outgrows 2 MiB, else none);
+                                      1 = plain row tiles; 2..16 = that many
+                                      column blocks (one XCD L2 slice each) */
+    int split_passes;              /* 0 = auto (split once m or n > 2^18);
+                                      1 = fused SpMV + epilogue kernels;
+                                      2 = SpMV-only kernels, then row and
+                                      column passes (same results bitwise) */
 } vk_pdhg_params;
 
 typedef struct {
@@ -183,11 +191,27 @@ int vk_hip_pdhg_download(vk_hip_pdhg *solver, double *x, double *y);
  * HIP events on the solver's stream. */
 int vk_hip_pdhg_last_run_ms(const vk_hip_pdhg *solver, double *ms);
 /* Runs `iterations` more iterations launched one by one, with HIP events on
- * the solver's stream around the dual and primal kernels; returns their mean
- * durations and the mean wall time per iteration (ms).  For measurement. */
-int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *dual_ms,
-                        double *primal_ms, double *iteration_ms);
+ * the solver's stream between the kernels.  ms[0] = mean duration of the
+ * dual SpMV kernel (the fused dual kernel when not split), ms[1] = the dual
+ * row pass (0 when fused), ms[2] = the primal SpMV kernel (or fused primal
+ * kernel), ms[3] = the primal column pass (0 when fused), ms[4] = mean
+ * device time per whole iteration.  For measurement. */
+int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *ms);
+/* The A x layout the solver chose: column blocks (1 = plain row tiles) and
+ * whether the SpMVs run split from their row / column passes (0 or 1). */
+int vk_hip_pdhg_layout(const vk_hip_pdhg *solver, int *col_blocks, int *split_passes);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
+
+/* Layout tuner (new; the reference hands SpMV to cuSPARSE,
+ * cuda_ops.cu:473-528, and has no layout choice): times y = A x in the
+ * storage precision of `precision` on the plain row tiles and on 2, 4, 8 and
+ * 16 column blocks (`repeats` launches each), writes the mean milliseconds
+ * of each candidate to ms[0..4] (may be NULL; +inf where a candidate does
+ * not apply) and the fastest block count to *best, for
+ * vk_pdhg_params.col_blocks.  The choice changes only the summation order
+ * of A x, never the iteration's results beyond round-off. */
+int vk_hip_csr_tune_col_blocks(vk_hip_csr *A, int precision, int repeats, int *best,
+                               double *ms);
 
 /* ---- batched PDHG (new: one persistent launch for many LPs) ------------
  *

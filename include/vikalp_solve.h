@@ -41,6 +41,10 @@ typedef struct {
     double relative_kkt_tolerance;
     double mixed_phase_tolerance;
     int64_t restart_period;
+    int spmv_col_blocks;            /* 0 auto, 1 plain tiles, 2..16 column blocks */
+    int spmv_autotune;              /* time the layouts and use the fastest */
+    int spmv_split_passes;          /* 0 auto, 1 fused, 2 split SpMV + passes */
+    int reserved2;
 } vk_solver_options;
 
 /* Fills `o` with the defaults of vikalp::SolverOptions. */
@@ -104,7 +108,12 @@ int vk_lp_solve_batch(int count, const int64_t *row_starts, const int64_t *col_s
  * with HIP events around the two fused kernels on the solver's stream.
  * out[0] = mean dual-kernel ms (A x with the dual update), out[1] = mean
  * primal-kernel ms (A'y with the residuals and primal step), out[2] = mean
- * ms per iteration, out[3] = the step size used.  Returns 0 on success. */
+ * ms per iteration, out[3] = the step size used, out[4] = column blocks
+ * used (1 = plain row tiles), out[5] = 1 when the SpMVs ran split from
+ * their passes, out[6..9] = mean ms of the dual SpMV (or fused dual kernel),
+ * the dual row pass, the primal SpMV (or fused primal kernel) and the primal
+ * column pass (vk_hip_pdhg_profile).  out holds 10 doubles.  Returns 0 on
+ * success. */
 int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
                   const int64_t *column_indices, const double *values,
                   const double *objective, const double *row_lower,

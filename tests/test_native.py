@@ -117,7 +117,7 @@ def test_options_defaults_match_the_contract():
     native.host_lib().vk_solver_options_default(o)
     d = V.Options()
     for name, _ in native._SolverOptions._fields_:
-        if name == "reserved":
+        if name.startswith("reserved"):
             continue
         got, want = getattr(o, name), getattr(d, name)
         if isinstance(want, float) and math.isinf(want):

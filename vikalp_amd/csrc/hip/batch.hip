@@ -51,13 +51,14 @@ struct BatchArgs {
     int power_iterations;
 };
 
-// Workgroup reduction of the 8 slots into out[] (written by thread 0).
+// Workgroup reduction of the 8 slots into out[] (shared memory, readable by
+// every thread after the call).  scratch holds kSlots * kBatchWaves doubles.
 __device__ __forceinline__ void reduce_slots(const Partials &p, double *out, double *scratch) {
+    double v[kSlots];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        const double r = (s == 0) ? final_max<kBatchBlock>(p.v[s], scratch) : final_sum<kBatchBlock>(p.v[s], scratch);
-        if (threadIdx.x == 0) out[s] = r;
-    }
+    for (int s = 0; s < kSlots; ++s) v[s] = p.v[s];
+    const double r = block_reduce_slots<kBatchBlock>(v, scratch);
+    if (threadIdx.x < kSlots) out[threadIdx.x] = r;
     __syncthreads();
 }
 
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
     const int id = blockIdx.x;
     if (id >= a.count) return;
     __shared__ double lds[kBatchWaves][kTileNnz];
-    __shared__ double scratch[kBatchWaves];
+    __shared__ double scratch[kSlots * kBatchWaves];
     __shared__ double red_r[kSlots], red_c[kSlots];
     __shared__ DevState S;
     const InstanceRange R = a.inst[id];
@@ -147,11 +148,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
     __syncthreads();
     if (S.policy == VK_PDHG_POLICY_ADAPTIVE && S.status == VK_PDHG_RUNNING) {
         for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) {
-            b.xsum[j] = b.x[0][j];
+            b.xsum[j] = static_cast<double>(b.x[0][j]);
             b.xlast[j] = b.x[0][j];
         }
         for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) {
-            b.ysum[i] = b.y[i];
+            b.ysum[i] = static_cast<double>(b.y[i]);
             b.ylast[i] = b.y[i];
         }
         __syncthreads();
@@ -164,8 +165,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
         tile_spmv_range(a.AT, b.ysum, ec, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
         reduce_slots(p, red_c, scratch);
         if (tid == 0) check_step(S, red_r, red_c, true);
-        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) b.xsum[j] = T(0);
-        for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) b.ysum[i] = T(0);
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) b.xsum[j] = 0.0;
+        for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) b.ysum[i] = 0.0;
         __syncthreads();
     }
     if (S.status == VK_PDHG_RUNNING)
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
     const int id = blockIdx.x;
     if (id >= a.count) return;
     __shared__ double lds[kBatchWaves][kTileNnz];
-    __shared__ double scratch[kBatchWaves];
+    __shared__ double scratch[kSlots * kBatchWaves];
     __shared__ double red_r[kSlots], red_c[kSlots];
     __shared__ DevState S;
     const InstanceRange R = a.inst[id];
@@ -247,30 +248,6 @@ __global__ void k_to_f64_batch(int64_t n, const float *__restrict__ src, double 
 
 int bgrid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
 
-// Tiles of the rows [begin, end) only (tiles never cross an instance).
-void append_tiles(const std::vector<int> &rp, int begin, int end, std::vector<int2> &tiles) {
-    int cur_nnz = 0, cur_rows = 0;
-    tiles.push_back(make_int2(begin, rp[begin]));
-    for (int r = begin; r < end; ++r) {
-        const int len = rp[r + 1] - rp[r];
-        if (cur_rows > 0 && (cur_nnz + len > kTileNnz || cur_rows == kTileRows)) {
-            tiles.push_back(make_int2(r, rp[r]));
-            cur_nnz = 0;
-            cur_rows = 0;
-        }
-        cur_nnz += len;
-        ++cur_rows;
-        if (len > kTileNnz) {
-            tiles.push_back(make_int2(r + 1, rp[r + 1]));
-            cur_nnz = 0;
-            cur_rows = 0;
-        }
-    }
-    if (cur_rows > 0) tiles.push_back(make_int2(end, rp[end]));
-    // tiles = [first, boundaries..., end]: tile t of the range spans entries
-    // t and t+1; the end entry doubles as the next range's first entry.
-}
-
 }  // namespace
 }  // namespace vk
 
@@ -325,8 +302,8 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
     T *runs = h->alloc<T>(m), *cs = h->alloc<T>(n);
     b.c = c; b.l = l; b.u = u; b.rl = rl; b.ru = ru; b.row_unscale = runs; b.col_scale = cs;
     if (h->params.policy == VK_PDHG_POLICY_ADAPTIVE) {
-        b.xsum = h->alloc<T>(n);
-        b.ysum = h->alloc<T>(m);
+        b.xsum = h->alloc<double>(n);
+        b.ysum = h->alloc<double>(m);
         b.xlast = h->alloc<T>(n);
         b.ylast = h->alloc<T>(m);
         b.ax_avg = h->alloc<T>(m);
@@ -479,12 +456,12 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
         // range's start (ranges are contiguous) or the final sentinel.
         R.ta0 = static_cast<int>(ta.size());
         std::vector<int2> tmp;
-        append_tiles(rp, R.r0, R.r1, tmp);
+        append_tiles(rp.data(), R.r0, R.r1, tmp);
         R.ta1 = R.ta0 + static_cast<int>(tmp.size()) - 1;
         ta.insert(ta.end(), tmp.begin(), tmp.end() - 1);
         R.tt0 = static_cast<int>(tt.size());
         tmp.clear();
-        append_tiles(trp, R.c0, R.c1, tmp);
+        append_tiles(trp.data(), R.c0, R.c1, tmp);
         R.tt1 = R.tt0 + static_cast<int>(tmp.size()) - 1;
         tt.insert(tt.end(), tmp.begin(), tmp.end() - 1);
     }

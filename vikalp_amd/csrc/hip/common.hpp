@@ -103,6 +103,33 @@ struct CsrView {
     int ntiles;
 };
 
+// Column-blocked copy of a CSR operand, for when the gathered vector
+// outgrows one XCD's L2.  Block p holds the entries of columns
+// [n*p/nb, n*(p+1)/nb) of every row as a CSR of its own (offsets into shared
+// col/val arrays, blocks stored one after another) and is tiled like the
+// plain operand.  Workgroup g works on block g % nb: workgroups are
+// dispatched round-robin over the 8 XCDs, so block p's slice of x is
+// gathered by one XCD (nb = 8) or two (nb = 4) and stays in their L2.  Every
+// block writes one partial sum per row; a row pass adds the nb partials in
+// block order, so results repeat bitwise.
+constexpr int kMaxColBlocks = 16;
+
+template <class V>
+struct BlockedView {
+    const int *row_ptr;   // nb * (rows + 1), global offsets into col/val
+    const int *col;
+    const V *val;
+    const int2 *tiles;    // block p: entries [tile_first[p], tile_first[p+1]),
+    int rows;             //   the last one its end sentinel
+    int nb;
+    int tile_first[kMaxColBlocks + 1];
+
+    __device__ CsrView<V> block(int p) const {
+        return CsrView<V>{row_ptr + static_cast<long>(p) * (rows + 1), col, val, tiles + tile_first[p], rows,
+                          tile_first[p + 1] - tile_first[p] - 1};
+    }
+};
+
 // Number of fixed partial slots every fused kernel writes per block.
 constexpr int kSlots = 8;
 
@@ -116,13 +143,49 @@ struct Partials {
     }
 };
 
+// Reduces all kSlots accumulators of a block of B threads at once (slot 0
+// with max, the others with +): interleaved shuffle trees in every wave, one
+// LDS exchange, then thread s folds slot s over the waves in wave order.
+// Returns slot threadIdx.x in threads 0..kSlots-1 (other threads: unused).
+// scratch holds kSlots * B / kWave doubles; no barrier follows.
+template <int B>
+__device__ __forceinline__ double block_reduce_slots(double (&v)[kSlots], double *scratch) {
+    constexpr int W = B / kWave;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const double o = __shfl_down(v[s], off, kWave);
+            v[s] = (s == 0) ? dmax(v[s], o) : v[s] + o;
+        }
+    }
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) scratch[s * W + wave] = v[s];
+    }
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x < kSlots) {
+        const int s = threadIdx.x;
+        r = scratch[s * W];
+        for (int w = 1; w < W; ++w) r = (s == 0) ? dmax(r, scratch[s * W + w]) : r + scratch[s * W + w];
+    }
+    return r;
+}
+
+// Shared-memory doubles a kBlock-thread kernel needs for store_partials.
+constexpr int kSlotScratch = kSlots * kWaves;
+
+// Block partials of a kBlock-thread kernel: slot s of block b goes to
+// partials[s * gridDim.x + b].
 __device__ __forceinline__ void store_partials(const Partials &p, double *partials,
                                                double *scratch) {
+    double v[kSlots];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        const double r = (s == 0) ? block_max(p.v[s], scratch) : block_sum(p.v[s], scratch);
-        if (threadIdx.x == 0) partials[s * gridDim.x + blockIdx.x] = r;
-    }
+    for (int s = 0; s < kSlots; ++s) v[s] = p.v[s];
+    const double r = block_reduce_slots<kBlock>(v, scratch);
+    if (threadIdx.x < kSlots) partials[threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
 // Reduce `nblocks` (<= 4 * blockDim.x) per-block partials in a fixed order;
@@ -157,6 +220,11 @@ __device__ __forceinline__ double final_max(double v, double *scratch) {
     return r;
 }
 
+// Shared-memory doubles reduce_partials needs.
+constexpr int kFinalScratch = kSlots * kFinalBlock / kWave;
+
+// out (shared memory) receives the 8 reduced slots; a barrier follows, so
+// every thread may read out[] and reuse scratch afterwards.
 __device__ __forceinline__ void reduce_partials(const double *__restrict__ partials, int nblocks,
                                                 double *out, double *scratch) {
     double v[kSlots][4];
@@ -167,14 +235,16 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ parti
             const int b = threadIdx.x + u * kFinalBlock;
             v[s][u] = b < nblocks ? partials[s * nblocks + b] : 0.0;
         }
+    double acc[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-        double acc = v[s][0];
+        acc[s] = v[s][0];
 #pragma unroll
-        for (int u = 1; u < 4; ++u) acc = (s == 0) ? dmax(acc, v[s][u]) : acc + v[s][u];
-        const double r = (s == 0) ? final_max<>(acc, scratch) : final_sum<>(acc, scratch);
-        out[s] = r;
+        for (int u = 1; u < 4; ++u) acc[s] = (s == 0) ? dmax(acc[s], v[s][u]) : acc[s] + v[s][u];
     }
+    const double r = block_reduce_slots<kFinalBlock>(acc, scratch);
+    if (threadIdx.x < kSlots) out[threadIdx.x] = r;
+    __syncthreads();
 }
 
 // CSR-adaptive SpMV, one tile per wave (grid-stride over waves).
@@ -216,11 +286,11 @@ __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__
 #pragma unroll
             for (int u = 0; u < kPer; ++u) {
                 const int k = lane + u * kWave;
-                c[u] = k < nnz ? colp[k0 + k] : 0;
-                v[u] = k < nnz ? valp[k0 + k] : V(0);
+                c[u] = k < nnz ? __builtin_nontemporal_load(colp + k0 + k) : 0;
+                v[u] = k < nnz ? __builtin_nontemporal_load(valp + k0 + k) : V(0);
             }
             // Row starts of the tile, one per lane (lane r holds row r's start).
-            const int rs = lane < nr ? A.row_ptr[r0 + lane] - k0 : nnz;
+            const int rs = lane < nr ? __builtin_nontemporal_load(A.row_ptr + r0 + lane) - k0 : nnz;
             Pre pre{};
             if (lane < nr) pre = epi.prefetch(r0 + lane);
             X xv[kPer];
@@ -251,7 +321,8 @@ __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__
             if (lane == 0) pre = epi.prefetch(r0);
             double acc = 0.0;
             for (int k = k0 + lane; k < k1; k += kWave)
-                acc += static_cast<double>(valp[k]) * static_cast<double>(x[colp[k]]);
+                acc += static_cast<double>(__builtin_nontemporal_load(valp + k)) *
+                       static_cast<double>(x[__builtin_nontemporal_load(colp + k)]);
             acc = wave_sum(acc);
             if (lane == 0) epi.row(r0, acc, pre, part);
         }

@@ -46,27 +46,30 @@ int ensure_init() {
     return VK_OK;
 }
 
-std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows) {
-    std::vector<int> starts{0};
+void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles) {
     int cur_nnz = 0, cur_rows = 0;
-    for (int r = 0; r < rows; ++r) {
-        const int len = row_ptr[r + 1] - row_ptr[r];
+    tiles.push_back(make_int2(begin, rp[begin]));
+    for (int r = begin; r < end; ++r) {
+        const int len = rp[r + 1] - rp[r];
         if (cur_rows > 0 && (cur_nnz + len > kTileNnz || cur_rows == kTileRows)) {
-            starts.push_back(r);
+            tiles.push_back(make_int2(r, rp[r]));
             cur_nnz = 0;
             cur_rows = 0;
         }
         cur_nnz += len;
         ++cur_rows;
         if (len > kTileNnz) {
-            starts.push_back(r + 1);
+            tiles.push_back(make_int2(r + 1, rp[r + 1]));
             cur_nnz = 0;
             cur_rows = 0;
         }
     }
-    if (cur_rows > 0) starts.push_back(rows);
-    std::vector<int2> tiles(starts.size());
-    for (size_t i = 0; i < starts.size(); ++i) tiles[i] = make_int2(starts[i], row_ptr[starts[i]]);
+    if (cur_rows > 0) tiles.push_back(make_int2(end, rp[end]));
+}
+
+std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows) {
+    std::vector<int2> tiles;
+    append_tiles(row_ptr.data(), 0, rows, tiles);
     return tiles;
 }
 

@@ -48,7 +48,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
+#include <string>
 #include <new>
 #include <vector>
 
@@ -57,12 +59,22 @@ namespace {
 
 // ------------------------------------------------------------------ kernels
 
+// Plain SpMV y = A x rounded to T (setup: A x0 and A' y0; the split passes
+// store FP64 sums with T = double).
+template <class T>
+struct StoreEpi {
+    T *y;
+    struct Pre {};
+    __device__ Pre prefetch(int) const { return Pre{}; }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = static_cast<T>(acc); }
+};
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dual(Bufs<T> b, CsrView<T> A) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    __shared__ double scratch[kWaves];
+    __shared__ double scratch[kSlotScratch];
     const T *xin = b.x[(s.k + 1) & 1];
     DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
                    s.policy == VK_PDHG_POLICY_ADAPTIVE};
@@ -77,13 +89,130 @@ __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    __shared__ double scratch[kWaves];
+    __shared__ double scratch[kSlotScratch];
     const bool fuse = may_fuse(s);
     PrimalEpi<T> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse};
     Partials part;
     part.zero();
     tile_spmv(AT, b.y, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_cols, scratch);
+}
+
+// Split form of the two fused kernels for large operands: the SpMV alone
+// (FP64 sums to memory, a light epilogue that keeps 8 waves per SIMD), then
+// a coalesced pass over rows or columns that runs the same epilogue on the
+// stored sums.  The iterates are computed exactly as by the fused kernels;
+// only the block partials of the KKT sums are grouped differently (rows per
+// workgroup), which the reference policy's max-norm tests do not see.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dual_spmv(Bufs<T> b, CsrView<T> A, double *part) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds[kWaves][kTileNnz];
+    StoreEpi<double> epi{part};
+    Partials unused;
+    unused.zero();
+    tile_spmv(A, b.x[(s.k + 1) & 1], epi, unused, lds[threadIdx.x / kWave]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_primal_spmv(Bufs<T> b, CsrView<T> AT) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds[kWaves][kTileNnz];
+    StoreEpi<double> epi{b.atyd};
+    Partials unused;
+    unused.zero();
+    tile_spmv(AT, b.y, epi, unused, lds[threadIdx.x / kWave]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double scratch[kSlotScratch];
+    const bool fuse = may_fuse(s);
+    PrimalEpi<T> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse, sizeof(T) != sizeof(double)};
+    Partials part;
+    part.zero();
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
+        const auto pre = epi.prefetch(j);
+        epi.row(j, b.atyd[j], pre, part);
+    }
+    store_partials(part, b.part_cols, scratch);
+}
+
+// Column-blocked A x into per-block partial row sums (part[p*m + row]).
+// `which` selects x: 0 = the primal iterate x[(k+1)&1] (dual step),
+// 1 = the running sum xsum (average evaluation).  The grid is a multiple of
+// the block count.
+template <class T>
+struct PartialEpi {
+    double *out;
+    struct Pre {};
+    __device__ Pre prefetch(int) const { return Pre{}; }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const { out[r] = acc; }
+};
+
+template <class V, class X>
+__device__ __forceinline__ void blocked_spmv(const BlockedView<V> &B, const X *__restrict__ x,
+                                             double *__restrict__ part, double *lds) {
+    const int p = static_cast<int>(blockIdx.x) % B.nb;
+    const int sub = static_cast<int>(blockIdx.x) / B.nb, nsub = static_cast<int>(gridDim.x) / B.nb;
+    const CsrView<V> A = B.block(p);
+    PartialEpi<V> epi{part + static_cast<long>(p) * B.rows};
+    Partials unused;
+    unused.zero();
+    tile_spmv_range(A, x, epi, unused, lds, 0, A.ntiles, sub * kWaves + static_cast<int>(threadIdx.x) / kWave,
+                    nsub * kWaves);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_blocks(Bufs<T> b, BlockedView<T> B, double *part, int which) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds[kWaves][kTileNnz];
+    if (which) blocked_spmv(B, b.xsum, part, lds[threadIdx.x / kWave]);
+    else blocked_spmv(B, b.x[(s.k + 1) & 1], part, lds[threadIdx.x / kWave]);
+}
+
+// Dual step over rows from the column-blocked partial sums (k_dual's
+// epilogue; the nb partials are added in block order).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *__restrict__ part, int nb) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double scratch[kSlotScratch];
+    DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
+                   s.policy == VK_PDHG_POLICY_ADAPTIVE};
+    Partials acc_part;
+    acc_part.zero();
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
+        const auto pre = epi.prefetch(i);
+        double acc = 0.0;
+        for (int p = 0; p < nb; ++p) acc += part[static_cast<long>(p) * b.m + i];
+        epi.row(i, acc, pre, acc_part);
+    }
+    store_partials(acc_part, b.part_rows, scratch);
+}
+
+// Average evaluation over rows from the column-blocked partial sums.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_rows_blocked(Bufs<T> b, const double *__restrict__ part, int nb) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double scratch[kSlotScratch];
+    EvalRowsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
+    Partials acc_part;
+    acc_part.zero();
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
+        const auto pre = epi.prefetch(i);
+        double acc = 0.0;
+        for (int p = 0; p < nb; ++p) acc += part[static_cast<long>(p) * b.m + i];
+        epi.row(i, acc, pre, acc_part);
+    }
+    store_partials(acc_part, b.part_erows, scratch);
 }
 
 // One block: reduces the partials of this iteration's dual and primal
@@ -93,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
 template <class T>
 __global__ __launch_bounds__(kFinalBlock) void k_finalize(Bufs<T> b) {
     if (b.st->status != VK_PDHG_RUNNING) return;
-    __shared__ double scratch[kFinalBlock / kWave];
+    __shared__ double scratch[kFinalScratch];
     __shared__ double red_rows[kSlots], red_cols[kSlots];
     reduce_partials(b.part_rows, b.grid_rows, red_rows, scratch);
     reduce_partials(b.part_cols, b.grid_cols, red_cols, scratch);
@@ -118,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Bufs<T> b, CsrView<T> A) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    __shared__ double scratch[kWaves];
+    __shared__ double scratch[kSlotScratch];
     EvalRowsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
     Partials part;
     part.zero();
@@ -132,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Bufs<T> b, CsrView<T> AT) 
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    __shared__ double scratch[kWaves];
+    __shared__ double scratch[kSlotScratch];
     EvalColsEpi<T> epi{b, 1.0 / static_cast<double>(s.ns)};
     Partials part;
     part.zero();
@@ -146,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Bufs<T> b, CsrView<T> AT) 
 template <class T>
 __global__ __launch_bounds__(kFinalBlock) void k_check(Bufs<T> b, int init) {
     if (b.st->status != VK_PDHG_RUNNING) return;
-    __shared__ double scratch[kFinalBlock / kWave];
+    __shared__ double scratch[kFinalScratch];
     __shared__ double red_rows[kSlots], red_cols[kSlots];
     reduce_partials(b.part_erows, b.grid_rows, red_rows, scratch);
     reduce_partials(b.part_ecols, b.grid_cols, red_cols, scratch);
@@ -182,27 +311,46 @@ __global__ void k_copy(int n, const T *src, T *dst) {
 }
 
 template <class T>
+__global__ void k_widen(int n, const T *src, double *dst) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) dst[j] = static_cast<double>(src[j]);
+}
+
+template <class T>
 __global__ void k_zero(int n, T *dst) {
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) dst[j] = T(0);
 }
 
-// Plain SpMV y = A x in the storage type (setup: A x0 and A' y0).
-template <class T>
-struct StoreEpi {
-    T *y;
-    struct Pre {};
-    __device__ Pre prefetch(int) const { return Pre{}; }
-    __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = static_cast<T>(acc); }
-};
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_spmv_store(CsrView<T> A, const T *x, T *y) {
     __shared__ double lds[kWaves][kTileNnz];
-    __shared__ double scratch[kWaves];
+    __shared__ double scratch[kSlotScratch];
     StoreEpi<T> epi{y};
     Partials part;
     part.zero();
     tile_spmv(A, x, epi, part, lds[threadIdx.x / kWave]);
+}
+
+// Column-blocked y = A x in the storage type (the layout tuner's timing
+// loop): partial sums, then the ordered row pass.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_blocks_plain(BlockedView<T> B, const T *x, double *part) {
+    __shared__ double lds[kWaves][kTileNnz];
+    blocked_spmv(B, x, part, lds[threadIdx.x / kWave]);
+}
+
+template <class T>
+__global__ void k_rows_plain(int m, int nb, const double *__restrict__ part, T *y) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int p = 0; p < nb; ++p) acc += part[static_cast<long>(p) * m + i];
+        y[i] = static_cast<T>(acc);
+    }
+}
+
+template <class T>
+__global__ void k_gather_perm(int count, const T *__restrict__ src, const int *__restrict__ perm, T *dst) {
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < count; k += gridDim.x * kBlock) dst[k] = src[perm[k]];
 }
 
 __global__ void k_d2f(int n, const double *src, float *dst) {
@@ -216,6 +364,83 @@ __global__ void k_f2d(int n, const float *src, double *dst) {
 }
 
 int egrid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, 8192)); }
+
+// Host plan of the column-blocked layout of a CSR pattern (rp, ci) with
+// m rows and n columns: block p takes columns [n*p/nb, n*(p+1)/nb) of every
+// row, rows in order, blocks one after another.  perm maps a blocked
+// position to the CSR position its value comes from.
+struct BlockPlan {
+    std::vector<int> row_ptr, perm, col;
+    std::vector<int2> tiles;
+    int tile_first[kMaxColBlocks + 1] = {};
+    int max_tiles = 0;   // most tiles of any block
+};
+
+void plan_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int m, int n, int nb, BlockPlan &P) {
+    const size_t stride = static_cast<size_t>(m) + 1;
+    std::vector<long long> bound(static_cast<size_t>(nb) + 1);
+    for (int p = 0; p <= nb; ++p) bound[p] = static_cast<long long>(n) * p / nb;
+    P.row_ptr.assign(stride * nb, 0);
+    // Columns are increasing within a row, so the block index only grows.
+    for (int r = 0; r < m; ++r) {
+        int p = 0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            while (ci[k] >= bound[p + 1]) ++p;
+            ++P.row_ptr[p * stride + r + 1];
+        }
+    }
+    int base = 0;
+    for (int p = 0; p < nb; ++p) {
+        int *q = &P.row_ptr[p * stride];
+        q[0] = base;
+        for (int r = 0; r < m; ++r) q[r + 1] += q[r];
+        base = q[m];
+    }
+    const size_t nnz = static_cast<size_t>(rp[m]);
+    P.perm.resize(nnz);
+    P.col.resize(nnz);
+    std::vector<int> cursor(static_cast<size_t>(nb));
+    for (int r = 0; r < m; ++r) {
+        for (int p = 0; p < nb; ++p) cursor[p] = P.row_ptr[p * stride + r];
+        int p = 0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            while (ci[k] >= bound[p + 1]) ++p;
+            const int pos = cursor[p]++;
+            P.perm[pos] = k;
+            P.col[pos] = ci[k];
+        }
+    }
+    P.tiles.clear();
+    P.max_tiles = 0;
+    for (int p = 0; p < nb; ++p) {
+        P.tile_first[p] = static_cast<int>(P.tiles.size());
+        append_tiles(&P.row_ptr[p * stride], 0, m, P.tiles);
+        P.max_tiles = std::max(P.max_tiles, static_cast<int>(P.tiles.size()) - P.tile_first[p] - 1);
+    }
+    P.tile_first[nb] = static_cast<int>(P.tiles.size());
+}
+
+// Workgroups of a column-blocked launch: a multiple of nb, one tile per wave
+// up to the grid cap.
+int blocked_grid(int nb, int max_tiles) {
+    const int per = std::max(1, std::min((max_tiles + kWaves - 1) / kWaves, kMaxGrid / nb));
+    return per * nb;
+}
+
+// Column blocks when the caller asks for automatic choice: split once the
+// gathered vector (n entries of `bytes` each) outgrows 2 MiB, half of one
+// XCD's 4 MB L2, into 4 blocks (each slice shared by two XCDs; measured on
+// the 1M-column LP, 4 blocks beat 8 by 3% per iteration: fewer partial sums
+// to write and add).
+int auto_col_blocks(long long n, int bytes, long long m) {
+    if (m == 0) return 1;
+    return n * bytes > (2ll << 20) ? 4 : 1;
+}
+
+// Split passes when the caller asks for automatic choice: once either
+// dimension exceeds 2^18 (launch count matters less than the fused
+// epilogues' register pressure at that size).
+bool auto_split(long long m, long long n) { return std::max(m, n) > (1ll << 18); }
 
 }  // namespace
 }  // namespace vk
@@ -241,6 +466,17 @@ struct vk_hip_pdhg {
     hipEvent_t ev_snap[2] = {nullptr, nullptr};
     DevState *snap[2] = {nullptr, nullptr};   // pinned
     double last_ms = 0.0;
+    // Column-blocked dual SpMV (col_blocks > 1): layout, launch width and the
+    // col_blocks x m partial row sums.
+    int col_blocks = 1;
+    int grid_blk = 0;
+    // Split passes: SpMV-only kernels (launch widths grid_spmv_*) followed by
+    // row / column passes over grid_rows / grid_cols workgroups.
+    bool split = false;
+    int grid_spmv_rows = 1, grid_spmv_cols = 1;
+    BlockedView<double> blk64{};
+    BlockedView<float> blk32{};
+    double *blk_part = nullptr;
 
     template <class T>
     T *alloc(size_t count) {
@@ -267,32 +503,133 @@ template <>
 const CsrView<float> view_at<float>(const vk_hip_pdhg *s) { return s->A->at32.view(); }
 
 template <class T>
+BlockedView<T> &blocked(vk_hip_pdhg *s);
+template <>
+BlockedView<double> &blocked<double>(vk_hip_pdhg *s) { return s->blk64; }
+template <>
+BlockedView<float> &blocked<float>(vk_hip_pdhg *s) { return s->blk32; }
+
+template <class T>
 Bufs<T> &bufs(vk_hip_pdhg *s);
 template <>
 Bufs<double> &bufs<double>(vk_hip_pdhg *s) { return s->b64; }
 template <>
 Bufs<float> &bufs<float>(vk_hip_pdhg *s) { return s->b32; }
 
+// A x+ and the dual step: one fused kernel on the plain tiles, or the
+// column-blocked SpMV and its row pass.
+// `part` selects the kernels: 1 = the SpMV (or the fused kernel), 2 = the
+// row pass of a split layout (nothing when fused), 3 = both.
+template <class T>
+void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+    Bufs<T> &b = bufs<T>(s);
+    if (s->col_blocks > 1) {
+        if (part & 1) k_blocks<T><<<s->grid_blk, kBlock, 0, st>>>(b, blocked<T>(s), s->blk_part, 0);
+        if (part & 2) k_dual_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
+    } else if (s->split) {
+        if (part & 1) k_dual_spmv<T><<<s->grid_spmv_rows, kBlock, 0, st>>>(b, view_a<T>(s), s->blk_part);
+        if (part & 2) k_dual_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, 1);
+    } else if (part & 1) {
+        k_dual<T><<<s->grid_rows, kBlock, 0, st>>>(b, view_a<T>(s));
+    }
+}
+
+// A' y+ with the residual terms and the fused next primal step (`part` as
+// for launch_dual).
+template <class T>
+void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+    Bufs<T> &b = bufs<T>(s);
+    if (s->split) {
+        if (part & 1) k_primal_spmv<T><<<s->grid_spmv_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        if (part & 2) k_primal_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b);
+    } else if (part & 1) {
+        k_primal<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+    }
+}
+
+// A xbar of the running average and its row terms.
+template <class T>
+void launch_eval_rows(vk_hip_pdhg *s, hipStream_t st) {
+    Bufs<T> &b = bufs<T>(s);
+    if (s->col_blocks > 1) {
+        k_blocks<T><<<s->grid_blk, kBlock, 0, st>>>(b, blocked<T>(s), s->blk_part, 1);
+        k_eval_rows_blocked<T><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
+    } else {
+        k_eval_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, view_a<T>(s));
+    }
+}
+
+// The kernels that follow the primal kernel of iteration k: the finalizer,
+// then the policy's step-size or restart kernels.
+template <class T>
+void launch_control(vk_hip_pdhg *s, long long k, hipStream_t st) {
+    Bufs<T> &b = bufs<T>(s);
+    k_finalize<T><<<1, kFinalBlock, 0, st>>>(b);
+    if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
+        if (k > 0 && k % kDecisionPeriod == 0) k_pstep<T><<<egrid(s->n), kBlock, 0, st>>>(b);
+    } else if ((k + 1) % s->params.restart_period == 0) {
+        launch_eval_rows<T>(s, st);
+        k_eval_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        k_check<T><<<1, kFinalBlock, 0, st>>>(b, 0);
+        k_apply<T><<<egrid(std::max(s->n, s->m)), kBlock, 0, st>>>(b);
+    }
+}
+
 // Enqueues the kernels of iterations [k_begin, k_begin + count) without
 // synchronising; the schedule mirrors the device's own may_fuse rule.
 template <class T>
 void enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
-    Bufs<T> &b = bufs<T>(s);
-    const CsrView<T> A = view_a<T>(s), AT = view_at<T>(s);
-    const int ge = egrid(std::max(s->n, s->m));
     for (long long k = k_begin; k < k_begin + count; ++k) {
-        k_dual<T><<<s->grid_rows, kBlock, 0, s->stream>>>(b, A);
-        k_primal<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT);
-        k_finalize<T><<<1, kFinalBlock, 0, s->stream>>>(b);
-        if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
-            if (k > 0 && k % kDecisionPeriod == 0) k_pstep<T><<<egrid(s->n), kBlock, 0, s->stream>>>(b);
-        } else if ((k + 1) % s->params.restart_period == 0) {
-            k_eval_rows<T><<<s->grid_rows, kBlock, 0, s->stream>>>(b, A);
-            k_eval_cols<T><<<s->grid_cols, kBlock, 0, s->stream>>>(b, AT);
-            k_check<T><<<1, kFinalBlock, 0, s->stream>>>(b, 0);
-            k_apply<T><<<ge, kBlock, 0, s->stream>>>(b);
-        }
+        launch_dual<T>(s, s->stream);
+        launch_primal<T>(s, s->stream);
+        launch_control<T>(s, k, s->stream);
     }
+}
+
+// Builds the column-blocked copy of A in the storage type T: the pattern is
+// planned on the host, the values are gathered on the device.
+template <class T>
+int build_blocked(vk_hip_pdhg *s) {
+    const int m = s->m, n = s->n, nb = s->col_blocks;
+    const CsrView<T> A = view_a<T>(s);
+    const int nnz = static_cast<int>(s->A->nnz);
+    std::vector<int> rp(static_cast<size_t>(m) + 1), ci(static_cast<size_t>(nnz));
+    VK_HIP_CHECK(hipMemcpy(rp.data(), A.row_ptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+    if (nnz) VK_HIP_CHECK(hipMemcpy(ci.data(), A.col, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+    BlockPlan P;
+    plan_blocks(rp, ci, m, n, nb, P);
+    int *row_ptr = s->alloc<int>(P.row_ptr.size());
+    int *col = s->alloc<int>(P.col.size());
+    T *val = s->alloc<T>(P.col.size());
+    int2 *tiles = s->alloc<int2>(P.tiles.size());
+    s->blk_part = s->alloc<double>(static_cast<size_t>(nb) * m);
+    int *perm = nullptr;
+    if (!row_ptr || !col || !val || !tiles || !s->blk_part ||
+        hipMalloc(&perm, sizeof(int) * std::max<size_t>(1, P.perm.size())) != hipSuccess) {
+        set_error("vk_hip_pdhg_create: column-blocked layout allocation failed");
+        return VK_ENOMEM;
+    }
+    hipStream_t st = s->stream;
+    VK_HIP_CHECK(hipMemcpyAsync(row_ptr, P.row_ptr.data(), sizeof(int) * P.row_ptr.size(), hipMemcpyHostToDevice, st));
+    VK_HIP_CHECK(hipMemcpyAsync(tiles, P.tiles.data(), sizeof(int2) * P.tiles.size(), hipMemcpyHostToDevice, st));
+    if (nnz) {
+        VK_HIP_CHECK(hipMemcpyAsync(col, P.col.data(), sizeof(int) * P.col.size(), hipMemcpyHostToDevice, st));
+        VK_HIP_CHECK(hipMemcpyAsync(perm, P.perm.data(), sizeof(int) * P.perm.size(), hipMemcpyHostToDevice, st));
+        k_gather_perm<T><<<egrid(nnz), kBlock, 0, st>>>(nnz, A.val, perm, val);
+        VK_HIP_CHECK(hipGetLastError());
+    }
+    VK_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(perm);
+    BlockedView<T> &B = blocked<T>(s);
+    B.row_ptr = row_ptr;
+    B.col = col;
+    B.val = val;
+    B.tiles = tiles;
+    B.rows = m;
+    B.nb = nb;
+    for (int p = 0; p <= nb; ++p) B.tile_first[p] = P.tile_first[p];
+    s->grid_blk = blocked_grid(nb, P.max_tiles);
+    return VK_OK;
 }
 
 // Length of the repeating iteration block and the first iteration it may
@@ -316,6 +653,8 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     b.y = s->alloc<T>(m);
     b.ax = s->alloc<T>(m);
     b.aty = s->alloc<T>(n);
+    if constexpr (sizeof(T) == sizeof(double)) b.atyd = reinterpret_cast<double *>(b.aty);
+    else b.atyd = s->alloc<double>(n);
     T *c_d = s->alloc<T>(n), *l_d = s->alloc<T>(n), *u_d = s->alloc<T>(n);
     T *rl_d = s->alloc<T>(m), *ru_d = s->alloc<T>(m);
     T *runs = s->alloc<T>(m), *cs = s->alloc<T>(n);
@@ -324,8 +663,8 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     b.col_scale = cs;
     const bool adaptive = s->params.policy == VK_PDHG_POLICY_ADAPTIVE;
     if (adaptive) {
-        b.xsum = s->alloc<T>(n);
-        b.ysum = s->alloc<T>(m);
+        b.xsum = s->alloc<double>(n);
+        b.ysum = s->alloc<double>(m);
         b.xlast = s->alloc<T>(n);
         b.ylast = s->alloc<T>(m);
         b.ax_avg = s->alloc<T>(m);
@@ -340,7 +679,7 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     b.grid_cols = s->grid_cols;
     for (void *p : s->allocations)
         if (!p) return VK_ENOMEM;
-    if (!b.x[0] || !b.x[1] || !b.y || !b.ax || !b.aty || !c_d || !l_d || !u_d || !rl_d ||
+    if (!b.x[0] || !b.x[1] || !b.y || !b.ax || !b.aty || !b.atyd || !c_d || !l_d || !u_d || !rl_d ||
         !ru_d || !runs || !cs || !b.part_rows || !b.part_cols || !b.part_erows || !b.part_ecols ||
         (adaptive && (!b.xsum || !b.ysum || !b.xlast || !b.ylast || !b.ax_avg || !b.aty_avg))) {
         set_error("vk_hip_pdhg_create: device allocation failed");
@@ -377,6 +716,16 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     (void)hipFree(stage);
     if (rc) return rc;
 
+    if (s->col_blocks > 1) {
+        if (int brc = build_blocked<T>(s)) return brc;
+    } else if (s->split) {
+        s->blk_part = s->alloc<double>(m);
+        if (!s->blk_part) {
+            set_error("vk_hip_pdhg_create: device allocation failed");
+            return VK_ENOMEM;
+        }
+    }
+
     // pdhg.cpp:71 project the start; A x0, A' y0.
     k_project_start<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.l, b.u);
     const CsrView<T> A = view_a<T>(s), AT = view_at<T>(s);
@@ -389,21 +738,21 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     if (adaptive) {
         // KKT error of the start point = first restart reference: evaluate it
         // through the average path with sums = (x0, y0) and ns = 1.
-        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xsum);
+        k_widen<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xsum);
         k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xlast);
         if (m > 0) {
-            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.ysum);
+            k_widen<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.ysum);
             k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.ylast);
         }
         DevState tmp = *s->host_state;
         tmp.ns = 1;
         VK_HIP_CHECK(hipMemcpyAsync(s->st, &tmp, sizeof(DevState), hipMemcpyHostToDevice, st));
         VK_HIP_CHECK(hipStreamSynchronize(st));
-        k_eval_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, A);
+        launch_eval_rows<T>(s, st);
         k_eval_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, AT);
         k_check<T><<<1, kFinalBlock, 0, st>>>(b, 1);
-        k_zero<T><<<egrid(n), kBlock, 0, st>>>(n, b.xsum);
-        if (m > 0) k_zero<T><<<egrid(m), kBlock, 0, st>>>(m, b.ysum);
+        k_zero<double><<<egrid(n), kBlock, 0, st>>>(n, b.xsum);
+        if (m > 0) k_zero<double><<<egrid(m), kBlock, 0, st>>>(m, b.ysum);
         VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
         VK_HIP_CHECK(hipStreamSynchronize(st));
         s->host_state->ns = 0;
@@ -517,6 +866,181 @@ int download_solution(vk_hip_pdhg *s, double *x, double *y) {
     return VK_OK;
 }
 
+// Runs iterations launched one by one with HIP events on the solver's
+// stream between the kernels (same schedule as a run).  out[0..3] = mean ms
+// of the dual SpMV (or fused dual kernel), the dual row pass, the primal
+// SpMV (or fused primal kernel) and the primal column pass (0 when fused);
+// out[4] = mean ms per whole iteration.
+template <class T>
+int profile_iterations(vk_hip_pdhg *s, int64_t iterations, double *out) {
+    hipStream_t st = s->stream;
+    constexpr int kMarks = 5;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(kMarks * iterations + 1), nullptr);
+    auto cleanup = [&] {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            cleanup();
+            set_error("vk_hip_pdhg_profile: event creation failed");
+            return VK_EDEVICE;
+        }
+    int rc = VK_OK;
+    auto check = [&](hipError_t e) {
+        if (e != hipSuccess && rc == VK_OK) {
+            set_error(std::string("vk_hip_pdhg_profile: ") + hipGetErrorString(e));
+            rc = VK_EDEVICE;
+        }
+    };
+    check(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
+    check(hipStreamSynchronize(st));
+    long long k = s->host_state->k;
+    for (int64_t i = 0; i < iterations && rc == VK_OK; ++i, ++k) {
+        hipEvent_t *e = &ev[static_cast<size_t>(kMarks * i)];
+        check(hipEventRecord(e[0], st));
+        launch_dual<T>(s, st, 1);
+        check(hipEventRecord(e[1], st));
+        launch_dual<T>(s, st, 2);
+        check(hipEventRecord(e[2], st));
+        launch_primal<T>(s, st, 1);
+        check(hipEventRecord(e[3], st));
+        launch_primal<T>(s, st, 2);
+        check(hipEventRecord(e[4], st));
+        launch_control<T>(s, k, st);
+    }
+    check(hipEventRecord(ev[static_cast<size_t>(kMarks * iterations)], st));
+    check(hipStreamSynchronize(st));
+    check(hipGetLastError());
+    double sum[kMarks - 1] = {};
+    for (int64_t i = 0; i < iterations && rc == VK_OK; ++i)
+        for (int j = 0; j < kMarks - 1; ++j) {
+            float t = 0.0f;
+            check(hipEventElapsedTime(&t, ev[static_cast<size_t>(kMarks * i + j)],
+                                      ev[static_cast<size_t>(kMarks * i + j + 1)]));
+            sum[j] += t;
+        }
+    float total = 0.0f;
+    check(hipEventElapsedTime(&total, ev[0], ev[static_cast<size_t>(kMarks * iterations)]));
+    cleanup();
+    if (rc) return rc;
+    for (int j = 0; j < kMarks - 1; ++j) out[j] = sum[j] / static_cast<double>(iterations);
+    out[4] = total / static_cast<double>(iterations);
+    return VK_OK;
+}
+
+// Times y = A x on the plain tiles (nb = 1) and on each column-blocked
+// layout nb in kTuneBlocks; ms[i] is the mean of `repeats` launches.
+constexpr int kTuneBlocks[] = {1, 2, 4, 8, 16};
+constexpr int kTuneCount = 5;
+
+template <class T>
+int tune_col_blocks(vk_hip_csr *A, const CsrView<T> &view, int repeats, int *best, double *ms_out) {
+    const int m = static_cast<int>(A->rows), n = static_cast<int>(A->cols), nnz = static_cast<int>(A->nnz);
+    hipStream_t st = default_stream();
+    std::vector<void *> owned;
+    auto release = [&] {
+        for (void *p : owned) (void)hipFree(p);
+        owned.clear();
+    };
+    auto dalloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        owned.push_back(p);
+        return p;
+    };
+    T *x = static_cast<T *>(dalloc(sizeof(T) * n));
+    T *y = static_cast<T *>(dalloc(sizeof(T) * m));
+    double *part = static_cast<double *>(dalloc(sizeof(double) * kMaxColBlocks * static_cast<size_t>(m)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!x || !y || !part || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        release();
+        if (e0) (void)hipEventDestroy(e0);
+        set_error("vk_hip_csr_tune_col_blocks: device allocation failed");
+        return VK_ENOMEM;
+    }
+    int rc = VK_OK;
+    auto check = [&](hipError_t e) {
+        if (e != hipSuccess && rc == VK_OK) {
+            set_error(std::string("vk_hip_csr_tune_col_blocks: ") + hipGetErrorString(e));
+            rc = VK_EDEVICE;
+        }
+    };
+    check(hipMemsetAsync(x, 0, sizeof(T) * n, st));
+    std::vector<int> rp(static_cast<size_t>(m) + 1), ci(static_cast<size_t>(nnz));
+    check(hipMemcpy(rp.data(), view.row_ptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+    if (nnz) check(hipMemcpy(ci.data(), view.col, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+    double best_ms = std::numeric_limits<double>::infinity();
+    *best = 1;
+    for (int i = 0; i < kTuneCount && rc == VK_OK; ++i) {
+        const int nb = kTuneBlocks[i];
+        double ms = std::numeric_limits<double>::infinity();
+        if (nb == 1 || (nb <= n && m > 0)) {
+            std::function<void()> launch;
+            std::vector<void *> layout;
+            BlockedView<T> B{};
+            if (nb == 1) {
+                const int grid = std::max(1, std::min((view.ntiles + kWaves - 1) / kWaves, 2048));
+                launch = [&, grid] {
+                    if (view.ntiles > 0) k_spmv_store<T><<<grid, kBlock, 0, st>>>(view, x, y);
+                };
+            } else {
+                BlockPlan P;
+                plan_blocks(rp, ci, m, n, nb, P);
+                const size_t before = owned.size();
+                int *brp = static_cast<int *>(dalloc(sizeof(int) * P.row_ptr.size()));
+                int *bcol = static_cast<int *>(dalloc(sizeof(int) * P.col.size()));
+                T *bval = static_cast<T *>(dalloc(sizeof(T) * P.col.size()));
+                int2 *btiles = static_cast<int2 *>(dalloc(sizeof(int2) * P.tiles.size()));
+                int *bperm = static_cast<int *>(dalloc(sizeof(int) * P.perm.size()));
+                if (!brp || !bcol || !bval || !btiles || !bperm) {
+                    set_error("vk_hip_csr_tune_col_blocks: device allocation failed");
+                    rc = VK_ENOMEM;
+                    break;
+                }
+                layout.assign(owned.begin() + static_cast<long>(before), owned.end());
+                check(hipMemcpyAsync(brp, P.row_ptr.data(), sizeof(int) * P.row_ptr.size(), hipMemcpyHostToDevice, st));
+                check(hipMemcpyAsync(btiles, P.tiles.data(), sizeof(int2) * P.tiles.size(), hipMemcpyHostToDevice, st));
+                if (nnz) {
+                    check(hipMemcpyAsync(bcol, P.col.data(), sizeof(int) * P.col.size(), hipMemcpyHostToDevice, st));
+                    check(hipMemcpyAsync(bperm, P.perm.data(), sizeof(int) * P.perm.size(), hipMemcpyHostToDevice, st));
+                    k_gather_perm<T><<<egrid(nnz), kBlock, 0, st>>>(nnz, view.val, bperm, bval);
+                }
+                B = BlockedView<T>{brp, bcol, bval, btiles, m, nb, {}};
+                for (int p = 0; p <= nb; ++p) B.tile_first[p] = P.tile_first[p];
+                const int grid = blocked_grid(nb, P.max_tiles);
+                launch = [&, grid, nb] {
+                    k_blocks_plain<T><<<grid, kBlock, 0, st>>>(B, x, part);
+                    k_rows_plain<T><<<egrid(m), kBlock, 0, st>>>(m, nb, part, y);
+                };
+            }
+            for (int w = 0; w < 2; ++w) launch();
+            check(hipEventRecord(e0, st));
+            for (int r = 0; r < repeats; ++r) launch();
+            check(hipEventRecord(e1, st));
+            check(hipEventSynchronize(e1));
+            check(hipGetLastError());
+            float t = 0.0f;
+            check(hipEventElapsedTime(&t, e0, e1));
+            ms = static_cast<double>(t) / repeats;
+            // Free this candidate's layout before the next one.
+            for (void *p : layout) {
+                (void)hipFree(p);
+                owned.erase(std::find(owned.begin(), owned.end(), p));
+            }
+        }
+        if (ms_out) ms_out[i] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = nb;
+        }
+    }
+    release();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -559,6 +1083,10 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, co
     VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE ||
                    (params->primal_weight > 0.0 && params->restart_period > 0),
                "vk_hip_pdhg_create: adaptive policy needs a primal weight and restart period");
+    VK_REQUIRE(params->col_blocks >= 0 && params->col_blocks <= kMaxColBlocks,
+               "vk_hip_pdhg_create: col_blocks must be in [0, 16]");
+    VK_REQUIRE(params->split_passes >= 0 && params->split_passes <= 2,
+               "vk_hip_pdhg_create: split_passes must be 0, 1 or 2");
     if (int rc = ensure_init()) return rc;
     auto *s = new (std::nothrow) vk_hip_pdhg();
     if (!s) return VK_ENOMEM;
@@ -571,6 +1099,19 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, co
     // One tile per wave, kWaves waves per block.
     s->grid_rows = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, cap));
     s->grid_cols = std::max(1, std::min((A->at.ntiles + kWaves - 1) / kWaves, cap));
+    s->col_blocks = params->col_blocks > 0 ? params->col_blocks
+                                           : auto_col_blocks(s->n, s->mixed ? 4 : 8, s->m);
+    s->col_blocks = std::max(1, std::min(s->col_blocks, s->n));
+    if (s->m == 0) s->col_blocks = 1;
+    // With column blocks the row pass (one row per thread) owns grid_rows.
+    s->split = params->split_passes == 2 ||
+               (params->split_passes == 0 && auto_split(s->m, s->n));
+    s->grid_spmv_rows = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, kMaxGrid));
+    s->grid_spmv_cols = std::max(1, std::min((A->at.ntiles + kWaves - 1) / kWaves, kMaxGrid));
+    // Row and column passes run one element per thread over grid_rows /
+    // grid_cols workgroups (which also size the partial buffers).
+    if (s->col_blocks > 1 || s->split) s->grid_rows = std::max(1, std::min((s->m + kBlock - 1) / kBlock, cap));
+    if (s->split) s->grid_cols = std::max(1, std::min((s->n + kBlock - 1) / kBlock, cap));
     auto fail = [&](int rc) {
         vk_hip_pdhg_destroy(s);
         return rc;
@@ -665,62 +1206,15 @@ int vk_hip_pdhg_download(vk_hip_pdhg *s, double *x, double *y) {
     return s->mixed ? download_solution<float>(s, x, y) : download_solution<double>(s, x, y);
 }
 
-int vk_hip_pdhg_profile(vk_hip_pdhg *s, int64_t iterations, double *dual_ms, double *primal_ms,
-                        double *iteration_ms) {
-    VK_REQUIRE(s && dual_ms && primal_ms && iterations > 0, "vk_hip_pdhg_profile: bad arguments");
-    hipStream_t st = s->stream;
-    std::vector<hipEvent_t> ev(static_cast<size_t>(3 * iterations + 1));
-    for (auto &e : ev) VK_HIP_CHECK(hipEventCreate(&e));
-    VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, st));
-    VK_HIP_CHECK(hipStreamSynchronize(st));
-    long long k = s->host_state->k;
-    for (int64_t i = 0; i < iterations; ++i, ++k) {
-        VK_HIP_CHECK(hipEventRecord(ev[3 * i], st));
-        if (s->mixed) k_dual<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
-        else k_dual<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
-        VK_HIP_CHECK(hipEventRecord(ev[3 * i + 1], st));
-        if (s->mixed) k_primal<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s));
-        else k_primal<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s));
-        VK_HIP_CHECK(hipEventRecord(ev[3 * i + 2], st));
-        if (s->mixed) k_finalize<float><<<1, kFinalBlock, 0, st>>>(s->b32);
-        else k_finalize<double><<<1, kFinalBlock, 0, st>>>(s->b64);
-        // keep the device schedule consistent (step-size / restart kernels)
-        if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
-            if (k > 0 && k % kDecisionPeriod == 0) {
-                if (s->mixed) k_pstep<float><<<egrid(s->n), kBlock, 0, st>>>(s->b32);
-                else k_pstep<double><<<egrid(s->n), kBlock, 0, st>>>(s->b64);
-            }
-        } else if ((k + 1) % s->params.restart_period == 0) {
-            const int ge = egrid(std::max(s->n, s->m));
-            if (s->mixed) {
-                k_eval_rows<float><<<s->grid_rows, kBlock, 0, st>>>(s->b32, view_a<float>(s));
-                k_eval_cols<float><<<s->grid_cols, kBlock, 0, st>>>(s->b32, view_at<float>(s));
-                k_check<float><<<1, kFinalBlock, 0, st>>>(s->b32, 0);
-                k_apply<float><<<ge, kBlock, 0, st>>>(s->b32);
-            } else {
-                k_eval_rows<double><<<s->grid_rows, kBlock, 0, st>>>(s->b64, view_a<double>(s));
-                k_eval_cols<double><<<s->grid_cols, kBlock, 0, st>>>(s->b64, view_at<double>(s));
-                k_check<double><<<1, kFinalBlock, 0, st>>>(s->b64, 0);
-                k_apply<double><<<ge, kBlock, 0, st>>>(s->b64);
-            }
-        }
-    }
-    VK_HIP_CHECK(hipEventRecord(ev[3 * iterations], st));
-    VK_HIP_CHECK(hipStreamSynchronize(st));
-    double dsum = 0.0, psum = 0.0;
-    for (int64_t i = 0; i < iterations; ++i) {
-        float a = 0.0f, b = 0.0f;
-        VK_HIP_CHECK(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
-        VK_HIP_CHECK(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
-        dsum += a;
-        psum += b;
-    }
-    float total = 0.0f;
-    VK_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[3 * iterations]));
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    *dual_ms = dsum / static_cast<double>(iterations);
-    *primal_ms = psum / static_cast<double>(iterations);
-    if (iteration_ms) *iteration_ms = total / static_cast<double>(iterations);
+int vk_hip_pdhg_profile(vk_hip_pdhg *s, int64_t iterations, double *ms) {
+    VK_REQUIRE(s && ms && iterations > 0, "vk_hip_pdhg_profile: bad arguments");
+    return s->mixed ? profile_iterations<float>(s, iterations, ms) : profile_iterations<double>(s, iterations, ms);
+}
+
+int vk_hip_pdhg_layout(const vk_hip_pdhg *s, int *col_blocks, int *split_passes) {
+    VK_REQUIRE(s && col_blocks && split_passes, "vk_hip_pdhg_layout: null argument");
+    *col_blocks = s->col_blocks;
+    *split_passes = s->split ? 1 : 0;
     return VK_OK;
 }
 
@@ -728,6 +1222,19 @@ int vk_hip_pdhg_last_run_ms(const vk_hip_pdhg *s, double *ms) {
     VK_REQUIRE(s && ms, "vk_hip_pdhg_last_run_ms: null argument");
     *ms = s->last_ms;
     return VK_OK;
+}
+
+int vk_hip_csr_tune_col_blocks(vk_hip_csr *A, int precision, int repeats, int *best, double *ms) {
+    VK_REQUIRE(A && best && repeats > 0, "vk_hip_csr_tune_col_blocks: bad arguments");
+    VK_REQUIRE(A->has_values, "vk_hip_csr_tune_col_blocks: matrix values were never set");
+    VK_REQUIRE(precision == VK_PREC_FP64 || precision == VK_PREC_MIXED,
+               "vk_hip_csr_tune_col_blocks: unknown precision");
+    if (int rc = ensure_init()) return rc;
+    if (precision == VK_PREC_MIXED) {
+        if (int rc = csr_make_f32(A, default_stream())) return rc;
+        return tune_col_blocks<float>(A, A->a32.view(), static_cast<int>(repeats), best, ms);
+    }
+    return tune_col_blocks<double>(A, A->a.view(), static_cast<int>(repeats), best, ms);
 }
 
 }  // extern "C"

@@ -47,9 +47,15 @@ template <class T>
 struct Bufs {
     T *x[2];            // x_k in x[k & 1]
     T *y, *ax, *aty;    // current y, A x, A' y (in place)
+    double *atyd;       // A' y before rounding to T (split primal pass; aliases
+                        // aty when T is double)
     const T *c, *l, *u, *rl, *ru;
     const T *row_unscale, *col_scale;   // 1/r and s
-    T *xsum, *ysum, *xlast, *ylast, *ax_avg, *aty_avg;  // adaptive only
+    // Adaptive only.  The running sums of the iterates stay FP64 in every
+    // storage precision: an FP32 sum of thousands of iterates would carry
+    // its own rounding into the average far above FP32 iterate precision.
+    double *xsum, *ysum;
+    T *xlast, *ylast, *ax_avg, *aty_avg;
     double *part_rows, *part_cols, *part_erows, *part_ecols;
     DevState *st;
     int m, n;
@@ -70,7 +76,8 @@ struct DualEpi {
     bool restart;
     bool sign_exact;
     struct Pre {
-        T ax, y, lo, hi, unscale, ysum, ylast;
+        T ax, y, lo, hi, unscale, ylast;
+        double ysum;
     };
     __device__ Pre prefetch(int i) const {
         Pre p;
@@ -79,7 +86,7 @@ struct DualEpi {
         p.lo = b.rl[i];
         p.hi = b.ru[i];
         p.unscale = b.row_unscale[i];
-        p.ysum = b.ysum ? b.ysum[i] : T(0);
+        p.ysum = b.ysum ? b.ysum[i] : 0.0;
         p.ylast = b.ylast ? b.ylast[i] : T(0);
         return p;
     }
@@ -101,9 +108,10 @@ struct DualEpi {
             const double proj = clamp_box(inv_sigma * v, lo, hi);   // pdhg.cpp:108-112
             yn = v - sigma * proj;
         }
-        b.y[i] = static_cast<T>(yn);
+        const T y_stored = static_cast<T>(yn);
+        b.y[i] = y_stored;
         b.ax[i] = static_cast<T>(axn);
-        if (b.ysum) b.ysum[i] = q.ysum + static_cast<T>(yn);
+        if (b.ysum) b.ysum[i] = q.ysum + static_cast<double>(y_stored);
         const double d = clamp_box(axn, lo, hi) - axn;
         p.v[0] = dmax(p.v[0], fabs(d));
         const double du = d * static_cast<double>(q.unscale);
@@ -150,8 +158,10 @@ struct PrimalEpi {
     T *xnext;
     double tau;
     bool fuse;
+    bool store_aty = true;   // false when aty already holds acc
     struct Pre {
-        T x, c, l, u, s, xsum, xlast;
+        T x, c, l, u, s, xlast;
+        double xsum;
     };
     __device__ Pre prefetch(int j) const {
         Pre p;
@@ -160,21 +170,21 @@ struct PrimalEpi {
         p.l = b.l[j];
         p.u = b.u[j];
         p.s = b.col_scale[j];
-        p.xsum = (fuse && b.xsum) ? b.xsum[j] : T(0);
+        p.xsum = (fuse && b.xsum) ? b.xsum[j] : 0.0;
         p.xlast = b.xlast ? b.xlast[j] : T(0);
         return p;
     }
     __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
-        b.aty[j] = static_cast<T>(acc);
+        if (store_aty) b.aty[j] = static_cast<T>(acc);
         const double xj = static_cast<double>(q.x);
         const double cj = static_cast<double>(q.c);
         const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
         column_terms(xj, cj, cj + acc, lj, uj, static_cast<double>(q.s), b.xlast != nullptr,
                      static_cast<double>(q.xlast), p);
         if (fuse) {
-            const double xn = clamp_box((xj - tau * cj) - tau * acc, lj, uj);
-            xnext[j] = static_cast<T>(xn);
-            if (b.xsum) b.xsum[j] = q.xsum + static_cast<T>(xn);
+            const T xn = static_cast<T>(clamp_box((xj - tau * cj) - tau * acc, lj, uj));
+            xnext[j] = xn;
+            if (b.xsum) b.xsum[j] = q.xsum + static_cast<double>(xn);
         }
     }
 };
@@ -185,7 +195,8 @@ struct EvalRowsEpi {
     const Bufs<T> &b;
     double inv_ns;
     struct Pre {
-        T ysum, lo, hi, unscale, ylast;
+        double ysum;
+        T lo, hi, unscale, ylast;
     };
     __device__ Pre prefetch(int i) const {
         return Pre{b.ysum[i], b.rl[i], b.ru[i], b.row_unscale[i], b.ylast[i]};
@@ -193,7 +204,7 @@ struct EvalRowsEpi {
     __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
         const double axa = acc * inv_ns;
         b.ax_avg[i] = static_cast<T>(axa);
-        const double ya = static_cast<double>(q.ysum) * inv_ns;
+        const double ya = q.ysum * inv_ns;
         const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
         const double d = clamp_box(axa, lo, hi) - axa;
         p.v[0] = dmax(p.v[0], fabs(d));
@@ -215,7 +226,8 @@ struct EvalColsEpi {
     const Bufs<T> &b;
     double inv_ns;
     struct Pre {
-        T xsum, c, l, u, s, xlast;
+        double xsum;
+        T c, l, u, s, xlast;
     };
     __device__ Pre prefetch(int j) const {
         return Pre{b.xsum[j], b.c[j], b.l[j], b.u[j], b.col_scale[j], b.xlast[j]};
@@ -223,7 +235,7 @@ struct EvalColsEpi {
     __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
         const double atya = acc * inv_ns;
         b.aty_avg[j] = static_cast<T>(atya);
-        const double xa = static_cast<double>(q.xsum) * inv_ns;
+        const double xa = q.xsum * inv_ns;
         const double cj = static_cast<double>(q.c);
         column_terms(xa, cj, cj + atya, static_cast<double>(q.l), static_cast<double>(q.u),
                      static_cast<double>(q.s), true, static_cast<double>(q.xlast), p);
@@ -368,20 +380,20 @@ __device__ __forceinline__ void apply_column(const Bufs<T> &b, const DevState &s
     const bool copy_avg = s.final_from_avg || (s.restart_apply && s.restart_to_avg);
     const double inv = 1.0 / static_cast<double>(s.ns_avg > 0 ? s.ns_avg : 1);
     if (copy_avg) {
-        xcur[j] = static_cast<T>(static_cast<double>(b.xsum[j]) * inv);
+        xcur[j] = static_cast<T>(b.xsum[j] * inv);
         b.aty[j] = b.aty_avg[j];
     }
     if (s.restart_apply) {
         b.xlast[j] = xcur[j];
-        b.xsum[j] = static_cast<T>(0);
+        b.xsum[j] = 0.0;
     }
     if (s.status == VK_PDHG_RUNNING) {
         const double xj = static_cast<double>(xcur[j]);
-        const double xn = clamp_box((xj - s.tau * static_cast<double>(b.c[j])) -
-                                        s.tau * static_cast<double>(b.aty[j]),
-                                    static_cast<double>(b.l[j]), static_cast<double>(b.u[j]));
-        xnext[j] = static_cast<T>(xn);
-        b.xsum[j] += static_cast<T>(xn);
+        const T xn = static_cast<T>(clamp_box((xj - s.tau * static_cast<double>(b.c[j])) -
+                                                  s.tau * static_cast<double>(b.aty[j]),
+                                              static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
+        xnext[j] = xn;
+        b.xsum[j] += static_cast<double>(xn);
     }
 }
 
@@ -390,12 +402,12 @@ __device__ __forceinline__ void apply_row(const Bufs<T> &b, const DevState &s, i
     const bool copy_avg = s.final_from_avg || (s.restart_apply && s.restart_to_avg);
     const double inv = 1.0 / static_cast<double>(s.ns_avg > 0 ? s.ns_avg : 1);
     if (copy_avg) {
-        b.y[i] = static_cast<T>(static_cast<double>(b.ysum[i]) * inv);
+        b.y[i] = static_cast<T>(b.ysum[i] * inv);
         b.ax[i] = b.ax_avg[i];
     }
     if (s.restart_apply) {
         b.ylast[i] = b.y[i];
-        b.ysum[i] = static_cast<T>(0);
+        b.ysum[i] = 0.0;
     }
 }
 
@@ -403,10 +415,11 @@ __device__ __forceinline__ void apply_row(const Bufs<T> &b, const DevState &s, i
 template <class T>
 __device__ __forceinline__ void pstep_element(const Bufs<T> &b, const DevState &s, int j, const T *xcur, T *xnext) {
     const double xj = static_cast<double>(xcur[j]);
-    const double xn = clamp_box((xj - s.tau * static_cast<double>(b.c[j])) - s.tau * static_cast<double>(b.aty[j]),
-                                static_cast<double>(b.l[j]), static_cast<double>(b.u[j]));
-    xnext[j] = static_cast<T>(xn);
-    if (b.xsum) b.xsum[j] += static_cast<T>(xn);
+    const T xn = static_cast<T>(clamp_box((xj - s.tau * static_cast<double>(b.c[j])) -
+                                              s.tau * static_cast<double>(b.aty[j]),
+                                          static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
+    xnext[j] = xn;
+    if (b.xsum) b.xsum[j] += static_cast<double>(xn);
 }
 
 }  // namespace vk

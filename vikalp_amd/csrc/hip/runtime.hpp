@@ -36,6 +36,11 @@ struct DeviceCsr {
 // kTileRows rows; a row longer than kTileNnz is a tile of its own.
 std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows);
 
+// Appends the tiles of rows [begin, end) of the CSR offsets `rp` to `tiles`
+// as [first, boundaries..., end] ({row, rp[row]} each): tile t spans entries
+// t and t+1, so the range's last entry is its end sentinel.
+void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles);
+
 }  // namespace vk
 
 // The opaque matrix handle of the C ABI.

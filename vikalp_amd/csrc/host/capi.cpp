@@ -37,6 +37,9 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.relative_kkt_tolerance = o.relative_kkt_tolerance;
     s.mixed_phase_tolerance = o.mixed_phase_tolerance;
     s.restart_period = o.restart_period;
+    s.spmv_col_blocks = o.spmv_col_blocks;
+    s.spmv_autotune = o.spmv_autotune != 0;
+    s.spmv_split_passes = o.spmv_split_passes;
     return s;
 }
 
@@ -98,6 +101,9 @@ void vk_solver_options_default(vk_solver_options *o) {
     o->relative_kkt_tolerance = d.relative_kkt_tolerance;
     o->mixed_phase_tolerance = d.mixed_phase_tolerance;
     o->restart_period = d.restart_period;
+    o->spmv_col_blocks = d.spmv_col_blocks;
+    o->spmv_autotune = d.spmv_autotune ? 1 : 0;
+    o->spmv_split_passes = d.spmv_split_passes;
 }
 
 static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,
@@ -141,6 +147,12 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets, const int64_
         out[1] = p.primal_ms;
         out[2] = p.iteration_ms;
         out[3] = p.step_size;
+        out[4] = p.col_blocks;
+        out[5] = p.split_passes;
+        out[6] = p.dual_spmv_ms;
+        out[7] = p.dual_pass_ms;
+        out[8] = p.primal_spmv_ms;
+        out[9] = p.primal_pass_ms;
     } catch (const std::exception &e) {
         return -2;
     }

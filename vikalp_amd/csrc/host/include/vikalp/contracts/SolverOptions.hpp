@@ -53,6 +53,15 @@ struct SolverOptions {
     bool fused_iterations = true;
     // Capture iteration blocks in hipGraphs.
     bool use_graphs = true;
+    // Layout of the A x SpMV: 0 = automatic (4 column blocks once x
+    // outgrows 2 MiB), 1 = plain row tiles, 2..16 = that many column blocks.
+    int spmv_col_blocks = 0;
+    // Time the layouts on the device before the solve and use the fastest
+    // (overrides spmv_col_blocks).
+    bool spmv_autotune = false;
+    // SpMV and epilogue kernels: 0 = automatic, 1 = fused, 2 = split into a
+    // SpMV-only kernel and a row / column pass (identical results).
+    int spmv_split_passes = 0;
 };
 
 }  // namespace vikalp

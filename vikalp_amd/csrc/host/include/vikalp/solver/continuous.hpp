@@ -33,6 +33,11 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
 // iterations, on the HIP backend.  For measurement only.
 struct KernelProfile {
     double dual_ms = 0.0, primal_ms = 0.0, iteration_ms = 0.0, step_size = 0.0;
+    // Per kernel: the SpMV (or fused) kernel and the row / column pass of a
+    // split layout (0 when fused).
+    double dual_spmv_ms = 0.0, dual_pass_ms = 0.0, primal_spmv_ms = 0.0, primal_pass_ms = 0.0;
+    int col_blocks = 1;   // layout the solver used
+    int split_passes = 0;
 };
 KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &options,
                                  Index warm_iterations, Index iterations);

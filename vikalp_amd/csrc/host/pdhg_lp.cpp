@@ -46,7 +46,8 @@ bool valid_options(const SolverOptions &o) {
            (!std::isfinite(o.time_limit_seconds) || o.time_limit_seconds >= 0.0) &&
            std::isfinite(o.relative_kkt_tolerance) && o.relative_kkt_tolerance >= 0.0 &&
            o.restart_period > 0 && std::isfinite(o.mixed_phase_tolerance) &&
-           o.mixed_phase_tolerance >= 0.0;
+           o.mixed_phase_tolerance >= 0.0 && o.spmv_col_blocks >= 0 && o.spmv_col_blocks <= 16 &&
+           o.spmv_split_passes >= 0 && o.spmv_split_passes <= 2;
 }
 
 bool timed_out(Clock::time_point started, const SolverOptions &o) {
@@ -444,6 +445,7 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     p.relative_kkt_tolerance = options.relative_kkt_tolerance;
     p.restart_period = options.restart_period;
     p.use_graphs = options.use_graphs ? 1 : 0;
+    p.split_passes = options.spmv_split_passes;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     if (p.policy == VK_PDHG_POLICY_REFERENCE) {
@@ -461,6 +463,15 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     return p;
 }
 
+// Column-block layout of A x for a device solve in `precision`.
+int col_blocks_for(const vk_hip_csr *A, const SolverOptions &options, int precision) {
+    if (!options.spmv_autotune) return options.spmv_col_blocks;
+    int best = 1;
+    check_vk(vk_hip_csr_tune_col_blocks(const_cast<vk_hip_csr *>(A), precision, 10, &best, nullptr),
+             "vk_hip_csr_tune_col_blocks");
+    return best;
+}
+
 SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &sc,
                         const vk_hip_csr *A, const SolverOptions &options,
                         const std::vector<Scalar> *warm_scaled, Clock::time_point started) {
@@ -475,20 +486,24 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
     int status;
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
         // FP32 phase to the mixed-phase tolerance, then an FP64 phase
-        // warm-started from its primal and dual iterates.
+        // warm-started from its primal and dual iterates.  The FP64 phase
+        // always runs: it certifies the returned point in FP64 (an FP32 phase
+        // that met the target on its FP64-evaluated average can miss it by
+        // the rounding of that average to FP32) and usually stops after its
+        // first iterations.
         vk_pdhg_params p32 = p;
         p32.precision = VK_PREC_MIXED;
+        p32.col_blocks = col_blocks_for(A, options, VK_PREC_MIXED);
         p32.relative_kkt_tolerance = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
         status = run_device(A, sc, p32, warm_scaled, nullptr, started, options, xs, ys, state, ms, timed);
         res.low_precision_iterations = state.iterations;
         Index used = state.iterations;
         double total_ms = ms;
-        const bool reached_final = status == VK_PDHG_CONVERGED &&
-                                   options.relative_kkt_tolerance >= options.mixed_phase_tolerance;
-        if (!timed && !reached_final && used < options.iteration_limit) {
+        if (!timed && used < options.iteration_limit) {
             const auto x32 = xs, y32 = ys;
             vk_pdhg_params p64 = p;
             p64.precision = VK_PREC_FP64;
+            p64.col_blocks = col_blocks_for(A, options, VK_PREC_FP64);
             p64.iteration_limit = options.iteration_limit - used;
             if (p.policy == VK_PDHG_POLICY_ADAPTIVE) p64.primal_weight = state.primal_weight;
             status = run_device(A, sc, p64, &x32, &y32, started, options, xs, ys, state, ms, timed);
@@ -499,6 +514,7 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
         ms = total_ms;
     } else {
         p.precision = VK_PREC_FP64;
+        p.col_blocks = col_blocks_for(A, options, VK_PREC_FP64);
         status = run_device(A, sc, p, warm_scaled, nullptr, started, options, xs, ys, state, ms, timed);
     }
     res.iterations = state.iterations;
@@ -634,6 +650,7 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
     check_vk(vk_hip_csr_set_values(A, sc.values.data()), "vk_hip_csr_set_values");
     vk_pdhg_params p = make_params(model, sc, A, options);
     p.precision = options.pdhg_precision == PdhgPrecision::Mixed ? VK_PREC_MIXED : VK_PREC_FP64;
+    p.col_blocks = col_blocks_for(A, options, p.precision);
     p.iteration_limit = warm_iterations + iterations + 1;
     p.relative_kkt_tolerance = 0.0;
     p.metric_tolerance = 0.0;
@@ -644,8 +661,16 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
     int status = 0;
     if (warm_iterations > 0) check_vk(vk_hip_pdhg_run(solver.h, warm_iterations, &status), "vk_hip_pdhg_run");
     KernelProfile out;
-    check_vk(vk_hip_pdhg_profile(solver.h, iterations, &out.dual_ms, &out.primal_ms, &out.iteration_ms),
-             "vk_hip_pdhg_profile");
+    double ms[5] = {};
+    check_vk(vk_hip_pdhg_profile(solver.h, iterations, ms), "vk_hip_pdhg_profile");
+    check_vk(vk_hip_pdhg_layout(solver.h, &out.col_blocks, &out.split_passes), "vk_hip_pdhg_layout");
+    out.dual_spmv_ms = ms[0];
+    out.dual_pass_ms = ms[1];
+    out.primal_spmv_ms = ms[2];
+    out.primal_pass_ms = ms[3];
+    out.dual_ms = ms[0] + ms[1];
+    out.primal_ms = ms[2] + ms[3];
+    out.iteration_ms = ms[4];
     out.step_size = p.step_size;
     return out;
 }

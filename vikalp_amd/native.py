@@ -72,7 +72,8 @@ class _SolverOptions(C.Structure):
         ("deterministic", C.c_int), ("pdhg_policy", C.c_int), ("pdhg_precision", C.c_int),
         ("fused_iterations", C.c_int), ("use_graphs", C.c_int), ("reserved", C.c_int),
         ("relative_kkt_tolerance", C.c_double), ("mixed_phase_tolerance", C.c_double),
-        ("restart_period", C.c_int64),
+        ("restart_period", C.c_int64), ("spmv_col_blocks", C.c_int), ("spmv_autotune", C.c_int),
+        ("spmv_split_passes", C.c_int), ("reserved2", C.c_int),
     ]
 
 
@@ -97,7 +98,7 @@ class PdhgParams(C.Structure):
         ("metric_tolerance", C.c_double), ("relative_kkt_tolerance", C.c_double),
         ("step_size", C.c_double), ("primal_weight", C.c_double), ("restart_period", C.c_int64),
         ("norm_b", C.c_double), ("norm_c", C.c_double), ("use_graphs", C.c_int),
-        ("grid_blocks", C.c_int),
+        ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
     ]
 
 
@@ -142,8 +143,10 @@ HIP_SYMBOLS = {
     "vk_hip_pdhg_read_state": (C.c_int, [_vp, C.POINTER(PdhgState)]),
     "vk_hip_pdhg_download": (C.c_int, [_vp, _dp, _dp]),
     "vk_hip_pdhg_last_run_ms": (C.c_int, [_vp, _dp]),
-    "vk_hip_pdhg_profile": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp]),
+    "vk_hip_pdhg_profile": (C.c_int, [_vp, C.c_int64, _dp]),
+    "vk_hip_pdhg_layout": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
+    "vk_hip_csr_tune_col_blocks": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(PdhgParams),
                                       C.POINTER(_vp)]),
@@ -292,11 +295,14 @@ class Options:
     relative_kkt_tolerance: float = 0.0
     mixed_phase_tolerance: float = 1e-4
     restart_period: int = 64
+    spmv_col_blocks: int = 0
+    spmv_autotune: bool = False
+    spmv_split_passes: int = 0
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
         for name, _ in _SolverOptions._fields_:
-            if name == "reserved":
+            if name.startswith("reserved"):
                 continue
             v = getattr(self, name)
             setattr(o, name, int(v) if isinstance(v, (bool, enum.IntEnum)) else v)
@@ -365,13 +371,15 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     ci = np.ascontiguousarray(lp.column_indices, dtype=np.int64)
     f = [np.ascontiguousarray(a, dtype=np.float64) for a in
          (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
-    out = np.zeros(4)
+    out = np.zeros(10)
     copt = (options or Options())._c()
     rc = L.vk_lp_profile(lp.m, lp.n, _ptr(rp, _lp), _ptr(ci, _lp), *[_ptr(a) for a in f],
                          C.byref(copt), int(warm_iterations), int(iterations), _ptr(out))
     if rc != 0:
         raise RuntimeError(f"vk_lp_profile failed ({rc}): {hip_lib().vk_hip_last_error()}")
-    return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3]}
+    return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3],
+            "col_blocks": int(out[4]), "split_passes": int(out[5]), "dual_spmv_ms": out[6],
+            "dual_pass_ms": out[7], "primal_spmv_ms": out[8], "primal_pass_ms": out[9]}
 
 
 def _result(info, x, y):
@@ -464,6 +472,14 @@ class Device:
             v = np.ascontiguousarray(values, dtype=np.float64)
             check(self.L.vk_hip_csr_set_values(h, _ptr(v)), "vk_hip_csr_set_values")
         return h.value
+
+    def tune_col_blocks(self, h, precision=PdhgPrecision.FP64, repeats=10):
+        """vk_hip_csr_tune_col_blocks: (fastest block count, {blocks: mean ms})."""
+        best = C.c_int()
+        ms = np.zeros(5)
+        check(self.L.vk_hip_csr_tune_col_blocks(h, int(precision), int(repeats), C.byref(best), _ptr(ms)),
+              "vk_hip_csr_tune_col_blocks")
+        return best.value, dict(zip((1, 2, 4, 8, 16), ms.tolist()))
 
     def spmv(self, h, transpose, alpha, x, beta, y):
         check(self.L.vk_hip_spmv_csr(h, int(transpose), float(alpha), x, float(beta), y, None), "spmv")
