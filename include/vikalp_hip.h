@@ -224,8 +224,10 @@ int vk_hip_csr_tune_col_blocks(vk_hip_csr *A, int precision, int repeats, int *b
  * instance's columns).  All vectors are the SCALED data concatenated in the
  * same order; row_scale/col_scale give original coordinates.  step_sizes[i]
  * <= 0 (or NULL) asks the device for ||A_i||_2 by power iteration;
- * primal_weights, norm_b, norm_c are per instance (NULL = 1, 0, 0).  params
- * is shared (its step_size, primal_weight, norm_b, norm_c are ignored). */
+ * primal_weights, norm_b, norm_c are per instance (NULL = 1, 0, 0);
+ * iteration_limits[i] (NULL = params->iteration_limit for all) bounds
+ * instance i's iterations.  params is shared (its step_size, primal_weight,
+ * norm_b, norm_c, col_blocks and split_passes are ignored). */
 typedef struct vk_hip_batch vk_hip_batch;
 
 int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col_starts,
@@ -235,8 +237,8 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
                         const double *row_scale, const double *col_scale,
                         const double *x0, const double *y0, const double *step_sizes,
                         const double *primal_weights, const double *norm_b,
-                        const double *norm_c, const vk_pdhg_params *params,
-                        vk_hip_batch **out);
+                        const double *norm_c, const int64_t *iteration_limits,
+                        const vk_pdhg_params *params, vk_hip_batch **out);
 /* Launches the persistent kernel up to `max_launches` times, each running
  * `iterations_per_launch` iterations of every instance still running; stops
  * early when all have stopped.  *running = instances still running. */
@@ -247,6 +249,34 @@ int vk_hip_batch_read_state(vk_hip_batch *batch, int instance, vk_pdhg_state *st
 int vk_hip_batch_download(vk_hip_batch *batch, double *x, double *y);
 int vk_hip_batch_last_run_ms(const vk_hip_batch *batch, double *ms);
 int vk_hip_batch_destroy(vk_hip_batch *batch);
+
+/* ---- unrolled PDHG and its adjoint (new: the reference has no
+ * differentiation) ---------------------------------------------------------
+ *
+ * K steps of the reference iteration (pdhg.cpp:92-113) with tau and sigma
+ * fixed and no restart, on the SCALED LP (A with values set, objective c,
+ * variable bounds l,u, row bounds rl,ru), saving (x, y) every
+ * checkpoint_every steps; the backward pass recomputes each segment from its
+ * checkpoint, records the projections' active sets and runs the adjoint
+ * recursion in reverse (A and A' SpMVs swapped).  All vectors are scaled
+ * FP64 on the host. */
+typedef struct vk_hip_unrolled vk_hip_unrolled;
+
+int vk_hip_unrolled_create(const vk_hip_csr *A, const double *c, const double *l,
+                           const double *u, const double *rl, const double *ru,
+                           double tau, double sigma, int64_t iterations,
+                           int64_t checkpoint_every, vk_hip_unrolled **out);
+/* Runs the K steps from (x0, y0); x_out / y_out (may be NULL) get (x_K, y_K). */
+int vk_hip_unrolled_forward(vk_hip_unrolled *u, const double *x0, const double *y0,
+                            double *x_out, double *y_out);
+/* Given dL/dx_K and dL/dy_K (NULL = 0), writes dL/dc, dL/drl and dL/dru
+ * through the K steps (any output may be NULL).  Needs a forward pass first;
+ * may be called again with other seeds. */
+int vk_hip_unrolled_backward(vk_hip_unrolled *u, const double *grad_x, const double *grad_y,
+                             double *grad_c, double *grad_row_lower, double *grad_row_upper);
+/* Device milliseconds of the last forward or backward call (HIP events). */
+int vk_hip_unrolled_last_ms(const vk_hip_unrolled *u, double *ms);
+int vk_hip_unrolled_destroy(vk_hip_unrolled *u);
 
 #ifdef __cplusplus
 }

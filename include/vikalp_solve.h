@@ -121,6 +121,34 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
                   const double *var_upper, const vk_solver_options *options,
                   int64_t warm_iterations, int64_t iterations, double *out);
 
+/* Message of the last failure reported by a negative return code of this
+ * header's functions on the calling thread. */
+const char *vk_solve_last_error(void);
+
+/* ---- unrolled PDHG: forward / backward pair (new; solver::UnrolledPdhg) ----
+ * K steps of the PDHG iteration (pdhg.cpp:92-113) with the reference's fixed
+ * step (pdhg.cpp:75-78), no restarts, from the projected zero start, on the
+ * LP in the canonical form of vk_lp_solve.  forward writes x_K (n) and y_K
+ * (m, reference sign).  backward takes dL/dx_K and dL/dy_K (either may be
+ * NULL = 0) and writes dL/dc (n), dL/d(row_lower) (m) and dL/d(row_upper)
+ * (m) through the K steps; an equality row's dL/db is the sum of the two.
+ * Checkpoints every checkpoint_every steps bound the memory.  Returns 0, -1
+ * for unusable arguments, -2 with vk_solve_last_error() otherwise. */
+typedef struct vk_unrolled vk_unrolled;
+
+int vk_unrolled_create(int64_t m, int64_t n, const int64_t *row_offsets,
+                       const int64_t *column_indices, const double *values,
+                       const double *objective, const double *row_lower,
+                       const double *row_upper, const double *var_lower,
+                       const double *var_upper, int64_t iterations,
+                       int64_t checkpoint_every, vk_unrolled **out);
+int vk_unrolled_forward(vk_unrolled *u, double *x, double *y);
+int vk_unrolled_backward(vk_unrolled *u, const double *grad_x, const double *grad_y,
+                         double *grad_c, double *grad_row_lower, double *grad_row_upper);
+/* The fixed step and the device milliseconds of the last forward/backward. */
+int vk_unrolled_info(const vk_unrolled *u, double *step_size, double *last_device_ms);
+int vk_unrolled_destroy(vk_unrolled *u);
+
 #ifdef __cplusplus
 }
 #endif

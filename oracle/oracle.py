@@ -111,6 +111,8 @@ def lib():
         L.orc_relative_kkt.argtypes = [C.POINTER(_Lp), _dp, _dp] + [_dp] * 5
         L.orc_time_iterations.argtypes = [C.POINTER(_Lp), C.c_int64]
         L.orc_time_iterations.restype = C.c_double
+        L.orc_unrolled.argtypes = [C.POINTER(_Lp), C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_unrolled.restype = C.c_int
         _lib = L
     return _lib
 
@@ -281,6 +283,25 @@ def relative_kkt(lp: LP, x, y):
     c_lp = lp._c()
     lib().orc_relative_kkt(C.byref(c_lp), _ptr(x), _ptr(y), *[C.byref(o) for o in out])
     return tuple(o.value for o in out)
+
+
+def unrolled(lp: LP, iterations: int, grad_x=None, grad_y=None):
+    """Unrolled fixed-step PDHG (orc_unrolled): returns (x_K, y_K, step) and,
+    when grad_x / grad_y (dL/dx_K, dL/dy_K) are given, also
+    (grad_c, grad_row_lower, grad_row_upper) through the iterations."""
+    x = np.zeros(lp.n)
+    y = np.zeros(lp.m)
+    step = C.c_double()
+    want = grad_x is not None or grad_y is not None
+    gc, grl, gru = (np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.m)) if want else (None, None, None)
+    gx = None if grad_x is None else np.ascontiguousarray(grad_x, dtype=np.float64)
+    gy = None if grad_y is None else np.ascontiguousarray(grad_y, dtype=np.float64)
+    c_lp = lp._c()
+    rc = lib().orc_unrolled(C.byref(c_lp), int(iterations), _ptr(gx), _ptr(gy), _ptr(x), _ptr(y),
+                            _ptr(gc), _ptr(grl), _ptr(gru), C.byref(step))
+    if rc != 0:
+        raise ValueError("orc_unrolled: bad input")
+    return (x, y, step.value) + ((gc, grl, gru) if want else ())
 
 
 def time_iterations(lp: LP, iterations: int) -> float:

@@ -669,3 +669,106 @@ double orc_time_iterations(const orc_lp *lp, int64_t iterations) {
     scaled_free(&s);
     return dt;
 }
+
+/* ------------------------------------------------- unrolled PDHG + adjoint --
+ * The fixed-step iteration a differentiable layer unrolls: the op sequence of
+ * pdhg.cpp:92-113 with tau = sigma = 0.99 / max(1, bound) (pdhg.cpp:75-78)
+ * held fixed and no restart branch (pdhg.cpp:98-100 never taken), from the
+ * projected zero start (pdhg.cpp:71) and y = 0.  The adjoint is derived here
+ * from that sequence; nothing in the reference differentiates.  Per step, with
+ * z = x - tau c - tau A'y, x+ = P_X(z), xbar = 2 x+ - x, w = y + sigma A xbar,
+ * t = w / sigma, p = P_[rl,ru](t), y+ = w - sigma p:
+ *   gw   = [row clamped] gy+            (y+ = w - sigma p: d/dw is 0 inside)
+ *   grl -= sigma [t < rl] gy+,  gru -= sigma [t > ru] gy+
+ *   gxb  = sigma A' gw
+ *   gz   = [l <= z <= u] (gx+ + 2 gxb)
+ *   gx   = gz - gxb,  gc -= tau gz,  gy = gw - tau A gz
+ * Masks follow project_box = min(max(v, lo), hi): v == lo or v == hi count
+ * as inside. */
+
+static void unrolled_step(const scaled_lp *s, double tau, double sigma, double *x, double *y,
+                          double *x_next, double *x_bar, double *aty, double *ax_bar, double *y_arg,
+                          double *y_proj, unsigned char *mx, unsigned char *my) {
+    const int64_t m = s->m, n = s->n;
+    orc_spmv(m, n, s->rp, s->ci, s->v, 1.0, y, 0.0, aty, 1);           /* :92 */
+    memcpy(x_next, x, (size_t)n * sizeof(double));                     /* :93-96 */
+    orc_axpby(n, -tau, s->c, 1.0, x_next);
+    orc_axpby(n, -tau, aty, 1.0, x_next);
+    if (mx)
+        for (int64_t j = 0; j < n; ++j) mx[j] = !(x_next[j] < s->l[j]) && !(s->u[j] < x_next[j]);
+    orc_project_box(n, x_next, s->l, s->u);
+    memcpy(x_bar, x, (size_t)n * sizeof(double));                      /* :102-103 */
+    orc_axpby(n, 2.0, x_next, -1.0, x_bar);
+    orc_spmv(m, n, s->rp, s->ci, s->v, 1.0, x_bar, 0.0, ax_bar, 0);   /* :105-113 */
+    memcpy(y_arg, y, (size_t)m * sizeof(double));
+    orc_axpby(m, sigma, ax_bar, 1.0, y_arg);
+    memcpy(y_proj, y_arg, (size_t)m * sizeof(double));
+    orc_axpby(m, 1.0 / sigma, y_proj, 0.0, y_proj);
+    if (my)
+        for (int64_t i = 0; i < m; ++i)
+            my[i] = (y_proj[i] < s->rl[i]) ? 1 : ((s->ru[i] < y_proj[i]) ? 2 : 0);
+    orc_project_box(m, y_proj, s->rl, s->ru);
+    memcpy(y, y_arg, (size_t)m * sizeof(double));
+    orc_axpby(m, -sigma, y_proj, 1.0, y);
+    memcpy(x, x_next, (size_t)n * sizeof(double));
+}
+
+int orc_unrolled(const orc_lp *lp, int64_t iterations, const double *grad_x, const double *grad_y,
+                 double *x_out, double *y_out, double *grad_c, double *grad_row_lower,
+                 double *grad_row_upper, double *step_out) {
+    if (iterations < 0 || lp->n <= 0) return -1;
+    scaled_lp s;
+    scaled_alloc(&s, lp);
+    orc_scale_lp(lp, lp->var_lower, lp->var_upper, 0, s.v, s.c, s.rl, s.ru, s.l, s.u, s.rs, s.cs);
+    const int64_t m = s.m, n = s.n;
+    const double step = 0.99 / smax(1.0, orc_operator_bound(m, n, s.rp, s.ci, s.v));
+    const double tau = step, sigma = step;
+    if (step_out) *step_out = step;
+    double *x = calloc((size_t)n + 1, sizeof(double)), *y = calloc((size_t)m + 1, sizeof(double));
+    double *xn = calloc((size_t)n + 1, sizeof(double)), *xb = calloc((size_t)n + 1, sizeof(double));
+    double *aty = calloc((size_t)n + 1, sizeof(double)), *axb = calloc((size_t)m + 1, sizeof(double));
+    double *ya = calloc((size_t)m + 1, sizeof(double)), *yp = calloc((size_t)m + 1, sizeof(double));
+    const int want_grad = grad_x || grad_y;
+    unsigned char *mx = want_grad ? malloc((size_t)iterations * n + 1) : NULL;
+    unsigned char *my = want_grad ? malloc((size_t)iterations * m + 1) : NULL;
+    orc_project_box(n, x, s.l, s.u);                                     /* :71 */
+    for (int64_t k = 0; k < iterations; ++k)
+        unrolled_step(&s, tau, sigma, x, y, xn, xb, aty, axb, ya, yp,
+                      mx ? mx + (size_t)k * n : NULL, my ? my + (size_t)k * m : NULL);
+    if (x_out) for (int64_t j = 0; j < n; ++j) x_out[j] = x[j] * s.cs[j];
+    if (y_out) for (int64_t i = 0; i < m; ++i) y_out[i] = y[i] * s.rs[i];
+    if (want_grad) {
+        /* adjoints of the scaled state: x = s xhat, y = r yhat */
+        double *gx = calloc((size_t)n + 1, sizeof(double)), *gy = calloc((size_t)m + 1, sizeof(double));
+        double *gw = calloc((size_t)m + 1, sizeof(double)), *gxb = calloc((size_t)n + 1, sizeof(double));
+        double *gz = calloc((size_t)n + 1, sizeof(double)), *agz = calloc((size_t)m + 1, sizeof(double));
+        double *gc = calloc((size_t)n + 1, sizeof(double));
+        double *grl = calloc((size_t)m + 1, sizeof(double)), *gru = calloc((size_t)m + 1, sizeof(double));
+        for (int64_t j = 0; j < n; ++j) gx[j] = grad_x ? grad_x[j] * s.cs[j] : 0.0;
+        for (int64_t i = 0; i < m; ++i) gy[i] = grad_y ? grad_y[i] * s.rs[i] : 0.0;
+        for (int64_t k = iterations - 1; k >= 0; --k) {
+            const unsigned char *kx = mx + (size_t)k * n, *ky = my + (size_t)k * m;
+            for (int64_t i = 0; i < m; ++i) {
+                gw[i] = ky[i] ? gy[i] : 0.0;
+                if (ky[i] == 1) grl[i] -= sigma * gy[i];
+                if (ky[i] == 2) gru[i] -= sigma * gy[i];
+            }
+            orc_spmv(m, n, s.rp, s.ci, s.v, sigma, gw, 0.0, gxb, 1);
+            for (int64_t j = 0; j < n; ++j) {
+                gz[j] = kx[j] ? gx[j] + 2.0 * gxb[j] : 0.0;
+                gx[j] = gz[j] - gxb[j];
+                gc[j] -= tau * gz[j];
+            }
+            orc_spmv(m, n, s.rp, s.ci, s.v, 1.0, gz, 0.0, agz, 0);
+            for (int64_t i = 0; i < m; ++i) gy[i] = gw[i] - tau * agz[i];
+        }
+        /* c hat = s c, rl hat = r rl, ru hat = r ru */
+        if (grad_c) for (int64_t j = 0; j < n; ++j) grad_c[j] = gc[j] * s.cs[j];
+        if (grad_row_lower) for (int64_t i = 0; i < m; ++i) grad_row_lower[i] = grl[i] * s.rs[i];
+        if (grad_row_upper) for (int64_t i = 0; i < m; ++i) grad_row_upper[i] = gru[i] * s.rs[i];
+        free(gx); free(gy); free(gw); free(gxb); free(gz); free(agz); free(gc); free(grl); free(gru);
+    }
+    free(x); free(y); free(xn); free(xb); free(aty); free(axb); free(ya); free(yp); free(mx); free(my);
+    scaled_free(&s);
+    return 0;
+}

@@ -122,6 +122,20 @@ int orc_solve_lp(const orc_lp *lp, const double *lower_override,
  * instances too large to solve to tolerance. */
 double orc_time_iterations(const orc_lp *lp, int64_t iterations);
 
+/* Unrolled fixed-step PDHG and its adjoint (not in the reference, which has
+ * no differentiation; see pdhg_oracle.c for the recursion).  Runs
+ * `iterations` steps of pdhg.cpp:92-113 with tau = sigma = 0.99/max(1,bound)
+ * fixed and no restart from the projected zero start, on the scaled LP.
+ * x_out (n) / y_out (m) get x_K, y_K in original coordinates (y in the
+ * reference sign); step_out the step.  When grad_x or grad_y is given
+ * (dL/dx_K, dL/dy_K in original coordinates, either may be NULL = 0),
+ * grad_c (n), grad_row_lower (m) and grad_row_upper (m) receive dL/dc,
+ * dL/d(row_lower), dL/d(row_upper) through the K iterations (the explicit
+ * dependence of L on c is the caller's).  Returns 0, or -1 for bad input. */
+int orc_unrolled(const orc_lp *lp, int64_t iterations, const double *grad_x, const double *grad_y,
+                 double *x_out, double *y_out, double *grad_c, double *grad_row_lower,
+                 double *grad_row_upper, double *step_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -278,3 +278,90 @@ def test_relative_kkt_zero_at_vertex_enumerated_optimum():
     assert O.relative_kkt(lp, x, y + [0.0, 0.1])[1] > 1e-3        # not stationary
     assert O.relative_kkt(lp, x, -y)[2] > 1e-3                     # wrong sign -> gap
     assert O.relative_kkt(lp, x, y + [-1.0, 0.0])[2] == 1.0        # y<0 on a <= row: D = -inf
+
+
+# ------------------------------------------------------- unrolled PDHG adjoint
+
+def test_unrolled_forward_is_the_reference_trajectory_before_any_step_decision():
+    # The reference policy first changes its step after iteration 50
+    # (pdhg.cpp:133), so for K <= 50 its iterates are the fixed-step ones.
+    for fam, m, n, k, seed, K in [(1, 60, 150, 4, 7, 50), (2, 400, 1000, 0, 5, 37)]:
+        lp = V.generate_lp(fam, m, n, k, seed).as_oracle()
+        x, y, _ = O.unrolled(lp, K)
+        r = O.solve_lp(lp, iteration_limit=K)
+        assert r.iterations == K
+        assert np.array_equal(x, r.x) and np.array_equal(y, r.y)
+
+
+def _unrolled_fd(lp, K, xt, directions, seed):
+    """Max relative |central difference - adjoint| over random unit
+    directions in (c, equality rhs, <= rhs) for L = c'x_K + |x_K - x_t|^2 / 2."""
+    x, _, _ = O.unrolled(lp, K)
+    _, _, _, gc, grl, gru = O.unrolled(lp, K, grad_x=lp.objective + x - xt)
+    gc = gc + x
+    eq = np.isfinite(lp.row_lower) & (lp.row_lower == lp.row_upper)
+    le = ~eq & np.isfinite(lp.row_upper)
+
+    def loss(dc, db, h):
+        rl, ru = lp.row_lower.copy(), lp.row_upper.copy()
+        rl[eq] += h * db[eq]
+        ru[eq] += h * db[eq]
+        ru[le] += h * db[le]
+        p = O.LP(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values, lp.objective + h * dc, rl, ru,
+                 lp.var_lower, lp.var_upper)
+        xk, _, _ = O.unrolled(p, K)
+        return float(p.objective @ xk + 0.5 * np.sum((xk - xt) ** 2))
+
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for _ in range(directions):
+        dc, db = rng.normal(size=lp.n), rng.normal(size=lp.m)
+        norm = math.sqrt(dc @ dc + db[eq | le] @ db[eq | le])
+        dc, db = dc / norm, db / norm
+        fd = (loss(dc, db, 1e-6) - loss(dc, db, -1e-6)) / 2e-6
+        an = gc @ dc + (grl + gru)[eq] @ db[eq] + gru[le] @ db[le]
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-12))
+    return worst
+
+
+def test_unrolled_adjoint_matches_central_differences():
+    # equality and <= rows (family 2), box constraints active and inactive
+    lp = V.generate_lp(2, 400, 1000, 0, 11).as_oracle()
+    xt = np.random.default_rng(1).uniform(0, 10, lp.n)
+    assert _unrolled_fd(lp, 300, xt, 8, 2) <= 1e-4
+
+
+def test_unrolled_difference_check_separates_right_from_wrong_gradients():
+    # The central-difference bar is tight enough to reject a gradient that is
+    # off by one term: the same directional derivative without the explicit
+    # c'x_K contribution misses by far more than the tolerance.
+    lp = V.generate_lp(2, 400, 1000, 0, 11).as_oracle()
+    K = 120
+    x, _, _ = O.unrolled(lp, K)
+    xt = np.random.default_rng(3).uniform(0, 10, lp.n)
+    _, _, _, gc, grl, gru = O.unrolled(lp, K, grad_x=lp.objective + x - xt)
+    # without the explicit c'x_K term the directional derivative is wrong
+    rng = np.random.default_rng(4)
+    dc = rng.normal(size=lp.n)
+    dc /= np.linalg.norm(dc)
+
+    def loss(h):
+        p = O.LP(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values, lp.objective + h * dc,
+                 lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)
+        xk, _, _ = O.unrolled(p, K)
+        return float(p.objective @ xk + 0.5 * np.sum((xk - xt) ** 2))
+
+    fd = (loss(1e-6) - loss(-1e-6)) / 2e-6
+    assert abs(fd - (gc + x) @ dc) <= 1e-4 * abs(fd)
+    assert abs(fd - gc @ dc) > 1e-2 * abs(fd)
+
+
+def test_unrolled_gradients_are_duality_exact_at_convergence():
+    # At an optimum, d obj/dc = x* and d obj/db = -y* (the reference's dual
+    # sign, g = c + A'y); K = 200,000 fixed steps converge this LP to 1e-13.
+    lp = V.generate_lp(1, 40, 90, 3, 3).as_oracle()
+    x, y, _, gc, grl, gru = O.unrolled(lp, 200_000, grad_x=lp.objective)
+    rp, rd, gp, _, _ = O.relative_kkt(lp, x, y)
+    assert max(rp, rd, gp) <= 1e-10
+    assert np.max(np.abs(gc)) <= 1e-8                  # implicit part vanishes: d obj/dc = x*
+    assert np.max(np.abs((grl + gru) + y)) <= 1e-8     # d obj/db = -y*

@@ -376,7 +376,8 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
                         const double *ru, const double *row_scale, const double *col_scale,
                         const double *x0, const double *y0, const double *step_sizes,
                         const double *primal_weights, const double *norm_b, const double *norm_c,
-                        const vk_pdhg_params *params, vk_hip_batch **out) {
+                        const int64_t *iteration_limits, const vk_pdhg_params *params,
+                        vk_hip_batch **out) {
     VK_REQUIRE(out && params && count > 0 && row_starts && col_starts && row_offsets,
                "vk_hip_batch_create: bad arguments");
     *out = nullptr;
@@ -393,6 +394,9 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     VK_REQUIRE(nnz >= 0 && nnz < INT32_MAX && row_offsets[0] == 0, "vk_hip_batch_create: bad row offsets");
     VK_REQUIRE(n > 0 && c && l && u && col_scale, "vk_hip_batch_create: null column vector");
     VK_REQUIRE(m == 0 || (rl && ru && row_scale), "vk_hip_batch_create: null row vector");
+    if (iteration_limits)
+        for (int i = 0; i < count; ++i)
+            VK_REQUIRE(iteration_limits[i] >= 0, "vk_hip_batch_create: negative instance iteration limit");
     VK_REQUIRE(nnz == 0 || (column_indices && values), "vk_hip_batch_create: null matrix arrays");
     for (int i = 0; i < count; ++i) {
         VK_REQUIRE(row_starts[i + 1] >= row_starts[i] && col_starts[i + 1] > col_starts[i],
@@ -512,9 +516,9 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     h->states_host.assign(static_cast<size_t>(count), DevState{});
     for (int i = 0; i < count; ++i) {
         DevState &S = h->states_host[i];
-        S.limit = params->iteration_limit;
+        S.limit = iteration_limits ? iteration_limits[i] : params->iteration_limit;
         S.period = params->policy == VK_PDHG_POLICY_ADAPTIVE ? params->restart_period : kDecisionPeriod;
-        S.status = params->iteration_limit == 0 ? VK_PDHG_ITERATION_LIMIT : VK_PDHG_RUNNING;
+        S.status = S.limit == 0 ? VK_PDHG_ITERATION_LIMIT : VK_PDHG_RUNNING;
         S.policy = params->policy;
         S.metric_tol = params->metric_tolerance;
         S.rel_tol = params->relative_kkt_tolerance;

@@ -37,7 +37,8 @@ struct DeviceBatch {
 void run_batch(const std::vector<const Model *> &models, const std::vector<Prepared> &prep,
                const SolverOptions &options, int precision, double rel_tol, Index iteration_limit,
                const std::vector<double> *x0, const std::vector<double> *y0,
-               const std::vector<double> *weights_in, Clock::time_point started,
+               const std::vector<double> *weights_in, const std::vector<int64_t> *limits,
+               Clock::time_point started,
                std::vector<vk_pdhg_state> &states, std::vector<double> &xs, std::vector<double> &ys,
                double &device_ms, bool &timed) {
     const int count = static_cast<int>(prep.size());
@@ -92,7 +93,8 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
                                  c.data(), l.data(), u.data(), rl.empty() ? nullptr : rl.data(),
                                  ru.empty() ? nullptr : ru.data(), r.empty() ? nullptr : r.data(), s.data(),
                                  x0 ? x0->data() : nullptr, y0 && !y0->empty() ? y0->data() : nullptr,
-                                 steps.data(), weights.data(), nb.data(), nc.data(), &p, &batch.h),
+                                 steps.data(), weights.data(), nb.data(), nc.data(),
+                                 limits ? limits->data() : nullptr, &p, &batch.h),
              "vk_hip_batch_create");
     // Launch granularity: 256 iterations per launch keeps each kernel short
     // and lets the host honour the time limit between launches.
@@ -165,26 +167,31 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
         const double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
         run_batch(models, prep, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr,
-                  nullptr, started, states, xs, ys, ms, timed);
+                  nullptr, nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
         for (std::size_t i = 0; i < prep.size(); ++i) used[i] = low[i] = states[i].iterations;
         if (!timed) {
+            // FP64 refinement of every instance, warm-started, each with the
+            // iterations its FP32 phase left (an instance that used its whole
+            // budget stops at once with IterationLimit).
             std::vector<double> w(prep.size());
-            Index max_used = 0;
+            std::vector<int64_t> remaining(prep.size());
+            Index max_remaining = 0;
             for (std::size_t i = 0; i < prep.size(); ++i) {
                 w[i] = states[i].primal_weight;
-                max_used = std::max(max_used, used[i]);
+                remaining[i] = std::max<Index>(0, options.iteration_limit - used[i]);
+                max_remaining = std::max<Index>(max_remaining, remaining[i]);
             }
             const auto x32 = xs, y32 = ys;
-            run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance,
-                      std::max<Index>(0, options.iteration_limit - max_used), &x32, &y32,
-                      options.pdhg_policy == PdhgPolicy::Adaptive ? &w : nullptr, started, states, xs, ys, ms, timed);
+            run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32,
+                      &y32, options.pdhg_policy == PdhgPolicy::Adaptive ? &w : nullptr, &remaining, started,
+                      states, xs, ys, ms, timed);
             total_ms += ms;
             for (std::size_t i = 0; i < prep.size(); ++i) used[i] += states[i].iterations;
         }
     } else {
         run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit,
-                  nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
+                  nullptr, nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
         for (std::size_t i = 0; i < prep.size(); ++i) used[i] = states[i].iterations;
     }

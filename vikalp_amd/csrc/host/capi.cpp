@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <span>
 #include <string>
 #include <vector>
@@ -16,6 +17,8 @@
 namespace {
 
 using namespace vikalp;
+
+thread_local std::string g_last_error;
 
 SolverOptions to_options(const vk_solver_options &o) {
     SolverOptions s;
@@ -254,6 +257,78 @@ int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
         for (int64_t i = 0; i < m; ++i)
             y_out[i] = r.constraint_upper_dual[static_cast<std::size_t>(i)] -
                        r.constraint_lower_dual[static_cast<std::size_t>(i)];
+    return 0;
+}
+
+const char *vk_solve_last_error(void) { return g_last_error.c_str(); }
+
+struct vk_unrolled {
+    std::unique_ptr<solver::UnrolledPdhg> u;
+    int64_t m = 0, n = 0;
+};
+
+int vk_unrolled_create(int64_t m, int64_t n, const int64_t *row_offsets, const int64_t *column_indices,
+                       const double *values, const double *objective, const double *row_lower,
+                       const double *row_upper, const double *var_lower, const double *var_upper,
+                       int64_t iterations, int64_t checkpoint_every, vk_unrolled **out) {
+    if (!out) return -1;
+    *out = nullptr;
+    Model model;
+    if (!build_model(m, n, row_offsets, column_indices, values, objective, 0.0, row_lower, row_upper, var_lower,
+                     var_upper, model)) {
+        g_last_error = "vk_unrolled_create: unusable arguments";
+        return -1;
+    }
+    try {
+        auto h = std::make_unique<vk_unrolled>();
+        h->u = std::make_unique<solver::UnrolledPdhg>(model, iterations, checkpoint_every);
+        h->m = m;
+        h->n = n;
+        *out = h.release();
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -2;
+    }
+    return 0;
+}
+
+int vk_unrolled_forward(vk_unrolled *h, double *x, double *y) {
+    if (!h || !x || (h->m > 0 && !y)) return -1;
+    try {
+        h->u->forward(std::span<double>(x, static_cast<std::size_t>(h->n)),
+                      std::span<double>(y, static_cast<std::size_t>(h->m)));
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -2;
+    }
+    return 0;
+}
+
+int vk_unrolled_backward(vk_unrolled *h, const double *grad_x, const double *grad_y, double *grad_c,
+                         double *grad_row_lower, double *grad_row_upper) {
+    if (!h || !grad_c || (h->m > 0 && (!grad_row_lower || !grad_row_upper))) return -1;
+    const auto n = static_cast<std::size_t>(h->n), m = static_cast<std::size_t>(h->m);
+    try {
+        h->u->backward(grad_x ? std::span<const double>(grad_x, n) : std::span<const double>{},
+                       grad_y ? std::span<const double>(grad_y, m) : std::span<const double>{},
+                       std::span<double>(grad_c, n), std::span<double>(grad_row_lower, m),
+                       std::span<double>(grad_row_upper, m));
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -2;
+    }
+    return 0;
+}
+
+int vk_unrolled_info(const vk_unrolled *h, double *step_size, double *last_device_ms) {
+    if (!h) return -1;
+    if (step_size) *step_size = h->u->step_size();
+    if (last_device_ms) *last_device_ms = h->u->last_device_ms();
+    return 0;
+}
+
+int vk_unrolled_destroy(vk_unrolled *h) {
+    delete h;
     return 0;
 }
 

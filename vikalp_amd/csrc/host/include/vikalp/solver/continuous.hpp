@@ -6,6 +6,7 @@
 #include "vikalp/contracts/SolveResult.hpp"
 #include "vikalp/contracts/SolverOptions.hpp"
 
+#include <memory>
 #include <span>
 #include <vector>
 
@@ -41,5 +42,34 @@ struct KernelProfile {
 };
 KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &options,
                                  Index warm_iterations, Index iterations);
+
+// K unrolled steps of the PDHG iteration (pdhg.cpp:92-113) with the
+// reference's step tau = sigma = 0.99 / max(1, bound) (pdhg.cpp:75-78) held
+// fixed and no restarts, from the projected zero start, as a differentiable
+// map of the objective and the row bounds, on the HIP device.  forward()
+// gives (x_K, y_K) in original coordinates (y in the reference's sign);
+// backward() takes dL/dx_K and dL/dy_K and returns dL/dc, dL/d(row_lower)
+// and dL/d(row_upper) through the K steps (L's own explicit dependence on c
+// is the caller's).  The backward pass keeps (x, y) every checkpoint_every
+// steps and recomputes each segment, so memory is O((K / N + N)(m + n)).
+// Throws std::invalid_argument for an unusable model or counts.
+class UnrolledPdhg {
+public:
+    UnrolledPdhg(const Model &model, Index iterations, Index checkpoint_every);
+    ~UnrolledPdhg();
+    UnrolledPdhg(const UnrolledPdhg &) = delete;
+    UnrolledPdhg &operator=(const UnrolledPdhg &) = delete;
+
+    void forward(std::span<Scalar> x, std::span<Scalar> y);
+    void backward(std::span<const Scalar> grad_x, std::span<const Scalar> grad_y, std::span<Scalar> grad_c,
+                  std::span<Scalar> grad_row_lower, std::span<Scalar> grad_row_upper);
+    Scalar step_size() const;
+    Index iterations() const;
+    double last_device_ms() const;   // device time of the last forward / backward
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 }  // namespace vikalp::solver

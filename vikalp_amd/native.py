@@ -148,13 +148,19 @@ HIP_SYMBOLS = {
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
     "vk_hip_csr_tune_col_blocks": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
-                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(PdhgParams),
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, _lp, C.POINTER(PdhgParams),
                                       C.POINTER(_vp)]),
     "vk_hip_batch_run": (C.c_int, [_vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)]),
     "vk_hip_batch_read_state": (C.c_int, [_vp, C.c_int, C.POINTER(PdhgState)]),
     "vk_hip_batch_download": (C.c_int, [_vp, _dp, _dp]),
     "vk_hip_batch_last_run_ms": (C.c_int, [_vp, _dp]),
     "vk_hip_batch_destroy": (C.c_int, [_vp]),
+    "vk_hip_unrolled_create": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int64,
+                                         C.c_int64, C.POINTER(_vp)]),
+    "vk_hip_unrolled_forward": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "vk_hip_unrolled_backward": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "vk_hip_unrolled_last_ms": (C.c_int, [_vp, _dp]),
+    "vk_hip_unrolled_destroy": (C.c_int, [_vp]),
 }
 
 HOST_SYMBOLS = {
@@ -171,6 +177,13 @@ HOST_SYMBOLS = {
     "vk_lp_export": (C.c_int, [_vp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "vk_lp_free": (None, [_vp]),
     "vk_lp_block_periods": (C.c_int64, [C.c_uint64, C.c_int64]),
+    "vk_solve_last_error": (C.c_char_p, []),
+    "vk_unrolled_create": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     C.c_int64, C.c_int64, C.POINTER(_vp)]),
+    "vk_unrolled_forward": (C.c_int, [_vp, _dp, _dp]),
+    "vk_unrolled_backward": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "vk_unrolled_info": (C.c_int, [_vp, _dp, _dp]),
+    "vk_unrolled_destroy": (C.c_int, [_vp]),
 }
 
 _hip = None
@@ -430,6 +443,58 @@ def solve_lp(lp: LP, options: Options | None = None, warm_start=None, lower_over
              upper_override=None) -> Result:
     """vikalp::solver::solve_lp(model, hip_backend, options, warm, lo, hi) (pdhg.cpp:269-293)."""
     return _solve(1, lp, options or Options(), warm_start, lower_override, upper_override)
+
+
+class UnrolledPdhg:
+    """solver::UnrolledPdhg: K fixed-step PDHG iterations as a differentiable
+    map of the objective and the row bounds (vk_unrolled_* of
+    include/vikalp_solve.h).  forward() -> (x_K, y_K) in original coordinates;
+    backward(grad_x, grad_y) -> (grad_c, grad_row_lower, grad_row_upper)
+    through the iterations (L's explicit dependence on c is the caller's)."""
+
+    def __init__(self, lp: LP, iterations: int, checkpoint_every: int = 50):
+        self.L = host_lib()
+        self.m, self.n = lp.m, lp.n
+        rp = np.ascontiguousarray(lp.row_offsets, dtype=np.int64)
+        ci = np.ascontiguousarray(lp.column_indices, dtype=np.int64)
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in
+             (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
+        h = _vp()
+        rc = self.L.vk_unrolled_create(lp.m, lp.n, _ptr(rp, _lp), _ptr(ci, _lp), *[_ptr(a) for a in f],
+                                       int(iterations), int(checkpoint_every), C.byref(h))
+        if rc != 0:
+            raise ValueError(f"vk_unrolled_create failed ({rc}): {self.L.vk_solve_last_error().decode()}")
+        self.h = h.value
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.L.vk_solve_last_error().decode()}")
+
+    def forward(self):
+        x, y = np.zeros(self.n), np.zeros(self.m)
+        self._check(self.L.vk_unrolled_forward(self.h, _ptr(x), _ptr(y)), "vk_unrolled_forward")
+        return x, y
+
+    def backward(self, grad_x=None, grad_y=None):
+        gx = None if grad_x is None else np.ascontiguousarray(grad_x, dtype=np.float64)
+        gy = None if grad_y is None else np.ascontiguousarray(grad_y, dtype=np.float64)
+        gc, grl, gru = np.zeros(self.n), np.zeros(self.m), np.zeros(self.m)
+        self._check(self.L.vk_unrolled_backward(self.h, _ptr(gx), _ptr(gy), _ptr(gc), _ptr(grl), _ptr(gru)),
+                    "vk_unrolled_backward")
+        return gc, grl, gru
+
+    def info(self):
+        step, ms = C.c_double(), C.c_double()
+        self._check(self.L.vk_unrolled_info(self.h, C.byref(step), C.byref(ms)), "vk_unrolled_info")
+        return {"step_size": step.value, "last_device_ms": ms.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vk_unrolled_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Device:
