@@ -19,6 +19,10 @@ duration.  cpu_baseline: the CPU oracle (the reference's single-threaded
 algorithm restated in C) timed on a bounded sample of iterations of the same
 LP on one host core, extrapolated to the GPU's iteration count.
 
+Also measured (extra objects of the JSON line): the batched solve of 512
+block-angular LPs ("batch") and the unrolled-PDHG adjoint on a 100,000 x
+250,000 LP with its central-difference check ("adjoint").
+
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
 """
 from __future__ import annotations
@@ -77,6 +81,86 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
             for name, ms, b in rows}
 
 
+def measure_batch(V, np, args):
+    """Batched solve of block-angular instances (T ~ U{4..48} periods of
+    150 x 400 with 6 nonzeros per column, 40 coupling rows between periods,
+    1-20 branching bounds each) in one persistent launch per phase, to 1e-6
+    relative KKT, FP64 and mixed."""
+    rng = np.random.default_rng(WORKLOAD["seed"])
+    periods = rng.integers(4, 49, size=args.batch_instances)
+    lps = [V.generate_lp(4, 0, 0, int(t), WORKLOAD["seed"] + i) for i, t in enumerate(periods)]
+    rows, cols, nnz = sum(l.m for l in lps), sum(l.n for l in lps), sum(l.nnz for l in lps)
+    log(f"batch: {len(lps)} instances, {rows} rows, {cols} columns, {nnz} nonzeros")
+    out = {"instances": len(lps), "rows": rows, "cols": cols, "nnz": nnz,
+           "tolerance": WORKLOAD["tolerance"], "iteration_limit": args.batch_iterations}
+    for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
+        o = V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
+                      relative_kkt_tolerance=WORKLOAD["tolerance"], iteration_limit=args.batch_iterations,
+                      mixed_phase_tolerance=1e-4, time_limit_seconds=args.time_limit)
+        t0 = time.time()
+        res = V.solve_batch(lps, o)
+        wall = time.time() - t0
+        optimal = sum(r.status == "Optimal" for r in res)
+        its = np.array([r.iterations for r in res])
+        dev = res[0].device_seconds
+        out[label] = {"device_seconds": dev, "wall_seconds": wall, "optimal": optimal,
+                      "optimal_per_second": optimal / dev if dev > 0 else None,
+                      "iterations_median": float(np.median(its)), "iterations_max": int(its.max())}
+        log(f"batch {label}: {optimal}/{len(lps)} optimal, device {dev:.2f}s, wall {wall:.2f}s, "
+            f"iterations median {np.median(its):.0f} max {its.max()}")
+    return out
+
+
+def measure_adjoint(V, np, args):
+    """Unrolled PDHG forward / backward on m = 100,000, n = 250,000, 6
+    nonzeros per column (generator family 1), K iterations with checkpoints
+    every 50; loss c'x_K + |x_K - x_t|^2 / 2; the gradient with respect to
+    (c, b) checked by central differences (h = 1e-6) along random unit
+    directions."""
+    lp = V.generate_lp(1, 100_000, 250_000, 6, WORKLOAD["seed"])
+    K = args.adjoint_iterations
+    rng = np.random.default_rng(WORKLOAD["seed"] + 5)
+    xt = rng.uniform(0, 10, lp.n)
+    u = V.UnrolledPdhg(lp, K, checkpoint_every=50)
+    x, _ = u.forward()
+    fwd_ms = u.info()["last_device_ms"]
+    gc, grl, gru = u.backward(lp.objective + x - xt)
+    bwd_ms = u.info()["last_device_ms"]
+    gc = gc + x
+    gb = grl + gru
+    v = V.UnrolledPdhg(lp, K, checkpoint_every=K)
+
+    def loss(dc, db, h):
+        c = lp.objective + h * dc
+        v.set_data(c, lp.row_lower + h * db, lp.row_upper + h * db)
+        xk, _ = v.forward()
+        return float(c @ xk + 0.5 * np.sum((xk - xt) ** 2))
+
+    # Two error measures per direction d: |fd - g'd| / |g'd| (ill-conditioned
+    # when d is nearly orthogonal to g, as random directions in 350,000
+    # dimensions are: |g'd| ~ |g| / 600) and |fd - g'd| / |g|.
+    gnorm = math.sqrt(float(gc @ gc + gb @ gb))
+    errs, errs_norm = [], []
+    loss0 = loss(np.zeros(lp.n), np.zeros(lp.m), 0.0)
+    for _ in range(args.fd_directions):
+        dc, db = rng.normal(size=lp.n), rng.normal(size=lp.m)
+        norm = math.sqrt(dc @ dc + db @ db)
+        dc, db = dc / norm, db / norm
+        fd = (loss(dc, db, 1e-6) - loss(dc, db, -1e-6)) / 2e-6
+        an = float(gc @ dc + gb @ db)
+        errs.append(abs(fd - an) / max(abs(an), 1e-300))
+        errs_norm.append(abs(fd - an) / gnorm)
+    log(f"adjoint: m={lp.m} n={lp.n} nnz={lp.nnz} K={K}: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, "
+        f"central differences over {len(errs)} directions: |fd-g'd|/|g'd| median {np.median(errs):.2e} "
+        f"max {max(errs):.2e}; |fd-g'd|/|g| max {max(errs_norm):.2e} (loss {loss0:.6g}, |g| {gnorm:.4g})")
+    return {"m": lp.m, "n": lp.n, "nnz": lp.nnz, "iterations": K, "checkpoint_every": 50,
+            "forward_ms": fwd_ms, "backward_ms": bwd_ms, "fd_directions": len(errs), "fd_step": 1e-6,
+            "fd_rel_err_directional_median": float(np.median(errs)),
+            "fd_rel_err_directional_max": max(errs),
+            "fd_err_rel_gradient_norm_max": max(errs_norm),
+            "loss": loss0, "gradient_norm": gnorm}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +170,15 @@ def main():
     ap.add_argument("--n", type=int, default=WORKLOAD["n"])
     ap.add_argument("--max-iterations", type=int, default=200_000)
     ap.add_argument("--time-limit", type=float, default=100.0, help="seconds per solve")
-    ap.add_argument("--cpu-iterations", type=int, default=12)
+    ap.add_argument("--cpu-iterations", type=int, default=200)
+    ap.add_argument("--phase-tolerance", type=float, default=2e-6,
+                    help="relative KKT target of the FP32 phase of the mixed solve")
+    ap.add_argument("--batch-instances", type=int, default=512,
+                    help="block-angular instances of the batched solve (0 = skip)")
+    ap.add_argument("--batch-iterations", type=int, default=40_000)
+    ap.add_argument("--adjoint-iterations", type=int, default=1000,
+                    help="unrolled iterations of the adjoint measurement (0 = skip)")
+    ap.add_argument("--fd-directions", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true",
                     help="kernel timing only (used for the rocprofv3 pass)")
@@ -108,6 +200,7 @@ def main():
     def options(precision):
         return V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
                          relative_kkt_tolerance=WORKLOAD["tolerance"],
+                         mixed_phase_tolerance=args.phase_tolerance,
                          iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit)
 
     # Kernel timing for the roofline (HIP events on the solver's stream).
@@ -168,6 +261,13 @@ def main():
                "host_nproc": os.cpu_count()}
         log(f"cpu baseline: {per_it:.3f} s/iteration")
 
+    batch = None
+    if args.batch_instances > 0 and not args.skip_solve:
+        batch = measure_batch(V, np, args)
+    adjoint = None
+    if args.adjoint_iterations > 0 and not args.skip_solve:
+        adjoint = measure_adjoint(V, np, args)
+
     line = {
         "metric": "time to 1e-6 rel. KKT, mixed-FP32 vs FP64 PDHG, 1 MI355X; SpMV HBM GB/s",
         "value": t32 if ok32 else None,
@@ -201,6 +301,8 @@ def main():
                    "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"]}},
         "kernels": {"fp64": k64, "mixed": k32},
         "cpu_baseline": cpu,
+        "batch": batch,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 

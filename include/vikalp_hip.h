@@ -274,6 +274,10 @@ int vk_hip_unrolled_forward(vk_hip_unrolled *u, const double *x0, const double *
  * may be called again with other seeds. */
 int vk_hip_unrolled_backward(vk_hip_unrolled *u, const double *grad_x, const double *grad_y,
                              double *grad_c, double *grad_row_lower, double *grad_row_upper);
+/* Replaces the scaled objective and row bounds (same A, bounds of the same
+ * kind); the next backward needs a new forward pass. */
+int vk_hip_unrolled_set_data(vk_hip_unrolled *u, const double *c, const double *rl,
+                             const double *ru);
 /* Device milliseconds of the last forward or backward call (HIP events). */
 int vk_hip_unrolled_last_ms(const vk_hip_unrolled *u, double *ms);
 int vk_hip_unrolled_destroy(vk_hip_unrolled *u);

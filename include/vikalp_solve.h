@@ -145,6 +145,10 @@ int vk_unrolled_create(int64_t m, int64_t n, const int64_t *row_offsets,
 int vk_unrolled_forward(vk_unrolled *u, double *x, double *y);
 int vk_unrolled_backward(vk_unrolled *u, const double *grad_x, const double *grad_y,
                          double *grad_c, double *grad_row_lower, double *grad_row_upper);
+/* New objective and row bounds for the same matrix (each row keeps which of
+ * its bounds are finite); the matrix, scaling and step are reused. */
+int vk_unrolled_set_data(vk_unrolled *u, const double *objective, const double *row_lower,
+                         const double *row_upper);
 /* The fixed step and the device milliseconds of the last forward/backward. */
 int vk_unrolled_info(const vk_unrolled *u, double *step_size, double *last_device_ms);
 int vk_unrolled_destroy(vk_unrolled *u);

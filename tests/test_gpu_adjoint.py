@@ -68,13 +68,13 @@ def test_central_differences():
     gc = gc + x
     gb = grl + gru                                           # equality rows: rl = ru = b
 
+    v = V.UnrolledPdhg(lp, K, checkpoint_every=K)
+
     def loss(dc, db, h):
-        p = V.LP(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values, lp.objective + h * dc,
-                 lp.row_lower + h * db, lp.row_upper + h * db, lp.var_lower, lp.var_upper)
-        v = V.UnrolledPdhg(p, K, checkpoint_every=K)
+        c = lp.objective + h * dc
+        v.set_data(c, lp.row_lower + h * db, lp.row_upper + h * db)
         xk, _ = v.forward()
-        v.close()
-        return float(p.objective @ xk + 0.5 * np.sum((xk - xt) ** 2))
+        return float(c @ xk + 0.5 * np.sum((xk - xt) ** 2))
 
     worst = 0.0
     for _ in range(8):
@@ -85,6 +85,36 @@ def test_central_differences():
         an = gc @ dc + gb @ db
         worst = max(worst, abs(fd - an) / max(abs(an), 1e-12))
     assert worst <= 1e-4, worst
+
+
+def test_set_data_equals_a_fresh_solver():
+    lp = V.generate_lp(2, 400, 1000, 0, 5)
+    rng = np.random.default_rng(8)
+    c2 = lp.objective + rng.normal(size=lp.n)
+    shift = rng.normal(size=lp.m)
+    rl2 = lp.row_lower + np.where(np.isfinite(lp.row_lower), shift, 0.0)
+    ru2 = lp.row_upper + np.where(np.isfinite(lp.row_upper), shift, 0.0)
+    u = V.UnrolledPdhg(lp, 120, checkpoint_every=40)
+    u.forward()
+    u.set_data(c2, rl2, ru2)
+    with pytest.raises(RuntimeError):
+        u.backward(c2)                                      # new data needs a new forward
+    x, y = u.forward()
+    g = u.backward(c2 + x)
+    lp2 = V.LP(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values, c2, rl2, ru2, lp.var_lower,
+               lp.var_upper)
+    f = V.UnrolledPdhg(lp2, 120, checkpoint_every=40)
+    x2, y2 = f.forward()
+    g2 = f.backward(c2 + x2)
+    assert np.array_equal(x, x2) and np.array_equal(y, y2)
+    for a, b in zip(g, g2):
+        assert np.array_equal(a, b)
+    with pytest.raises(ValueError):                         # NaN bound
+        u.set_data(c2, rl2, np.full(lp.m, np.nan))
+    opened = ru2.copy()
+    opened[np.isfinite(opened)] = np.inf                    # a finite bound may not become infinite
+    with pytest.raises(ValueError):
+        u.set_data(c2, rl2, opened)
 
 
 def test_duality_exact_at_convergence():

@@ -344,6 +344,21 @@ int vk_hip_unrolled_backward(vk_hip_unrolled *h, const double *grad_x, const dou
     return VK_OK;
 }
 
+int vk_hip_unrolled_set_data(vk_hip_unrolled *h, const double *c, const double *rl, const double *ru) {
+    VK_REQUIRE(h && c && (h->m == 0 || (rl && ru)), "vk_hip_unrolled_set_data: null argument");
+    VK_HIP_CHECK(hipMemcpyAsync(const_cast<double *>(h->a.c), c, sizeof(double) * h->n, hipMemcpyHostToDevice,
+                                h->stream));
+    if (h->m) {
+        VK_HIP_CHECK(hipMemcpyAsync(const_cast<double *>(h->a.rl), rl, sizeof(double) * h->m,
+                                    hipMemcpyHostToDevice, h->stream));
+        VK_HIP_CHECK(hipMemcpyAsync(const_cast<double *>(h->a.ru), ru, sizeof(double) * h->m,
+                                    hipMemcpyHostToDevice, h->stream));
+    }
+    VK_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->forward_done = false;
+    return VK_OK;
+}
+
 int vk_hip_unrolled_last_ms(const vk_hip_unrolled *h, double *ms) {
     VK_REQUIRE(h && ms, "vk_hip_unrolled_last_ms: null argument");
     *ms = h->last_ms;

@@ -320,6 +320,20 @@ int vk_unrolled_backward(vk_unrolled *h, const double *grad_x, const double *gra
     return 0;
 }
 
+int vk_unrolled_set_data(vk_unrolled *h, const double *objective, const double *row_lower,
+                         const double *row_upper) {
+    if (!h || !objective || (h->m > 0 && (!row_lower || !row_upper))) return -1;
+    const auto n = static_cast<std::size_t>(h->n), m = static_cast<std::size_t>(h->m);
+    try {
+        h->u->set_data(std::span<const double>(objective, n), std::span<const double>(row_lower, m),
+                       std::span<const double>(row_upper, m));
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -2;
+    }
+    return 0;
+}
+
 int vk_unrolled_info(const vk_unrolled *h, double *step_size, double *last_device_ms) {
     if (!h) return -1;
     if (step_size) *step_size = h->u->step_size();

@@ -61,6 +61,11 @@ public:
     UnrolledPdhg &operator=(const UnrolledPdhg &) = delete;
 
     void forward(std::span<Scalar> x, std::span<Scalar> y);
+    // New objective and row bounds for the same A (a row keeps its kind:
+    // finite / infinite bounds and equalities where they were).  The scaling
+    // depends on A only, so the device matrix is reused.
+    void set_data(std::span<const Scalar> objective, std::span<const Scalar> row_lower,
+                  std::span<const Scalar> row_upper);
     void backward(std::span<const Scalar> grad_x, std::span<const Scalar> grad_y, std::span<Scalar> grad_c,
                   std::span<Scalar> grad_row_lower, std::span<Scalar> grad_row_upper);
     Scalar step_size() const;

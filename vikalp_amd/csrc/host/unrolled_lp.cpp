@@ -11,6 +11,7 @@
 #include "vikalp_hip.h"
 
 #include <algorithm>
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -79,6 +80,32 @@ void UnrolledPdhg::forward(std::span<Scalar> x, std::span<Scalar> y) {
     for (Index j = 0; j < I.n; ++j) x[j] = xs[j] * I.sc.s[j];
     for (Index i = 0; i < I.m; ++i) y[i] = ys[i] * I.sc.r[i];
     I.forward_done = true;
+}
+
+void UnrolledPdhg::set_data(std::span<const Scalar> objective, std::span<const Scalar> row_lower,
+                            std::span<const Scalar> row_upper) {
+    Impl &I = *impl_;
+    if (static_cast<Index>(objective.size()) != I.n || static_cast<Index>(row_lower.size()) != I.m ||
+        static_cast<Index>(row_upper.size()) != I.m)
+        throw std::invalid_argument("UnrolledPdhg::set_data: sizes must be n and m");
+    for (Index j = 0; j < I.n; ++j)
+        if (!std::isfinite(objective[j])) throw std::invalid_argument("UnrolledPdhg::set_data: objective not finite");
+    for (Index i = 0; i < I.m; ++i) {
+        const Scalar lo = row_lower[i], hi = row_upper[i];
+        if (std::isnan(lo) || std::isnan(hi) || lo > hi || std::isfinite(lo) != std::isfinite(I.sc.rl[i]) ||
+            std::isfinite(hi) != std::isfinite(I.sc.ru[i]))
+            throw std::invalid_argument("UnrolledPdhg::set_data: row bounds must keep their kind and order");
+    }
+    // continuous_common.hpp:133-205: c hat = s c, row bounds times r.
+    for (Index j = 0; j < I.n; ++j) I.sc.c[j] = objective[j] * I.sc.s[j];
+    for (Index i = 0; i < I.m; ++i) {
+        I.sc.rl[i] = row_lower[i] * I.sc.r[i];
+        I.sc.ru[i] = row_upper[i] * I.sc.r[i];
+    }
+    check_vk(vk_hip_unrolled_set_data(I.h, I.sc.c.data(), I.sc.rl.empty() ? nullptr : I.sc.rl.data(),
+                                      I.sc.ru.empty() ? nullptr : I.sc.ru.data()),
+             "vk_hip_unrolled_set_data");
+    I.forward_done = false;
 }
 
 void UnrolledPdhg::backward(std::span<const Scalar> grad_x, std::span<const Scalar> grad_y,

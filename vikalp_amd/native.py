@@ -160,6 +160,7 @@ HIP_SYMBOLS = {
     "vk_hip_unrolled_forward": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "vk_hip_unrolled_backward": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "vk_hip_unrolled_last_ms": (C.c_int, [_vp, _dp]),
+    "vk_hip_unrolled_set_data": (C.c_int, [_vp, _dp, _dp, _dp]),
     "vk_hip_unrolled_destroy": (C.c_int, [_vp]),
 }
 
@@ -183,6 +184,7 @@ HOST_SYMBOLS = {
     "vk_unrolled_forward": (C.c_int, [_vp, _dp, _dp]),
     "vk_unrolled_backward": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "vk_unrolled_info": (C.c_int, [_vp, _dp, _dp]),
+    "vk_unrolled_set_data": (C.c_int, [_vp, _dp, _dp, _dp]),
     "vk_unrolled_destroy": (C.c_int, [_vp]),
 }
 
@@ -482,6 +484,13 @@ class UnrolledPdhg:
         self._check(self.L.vk_unrolled_backward(self.h, _ptr(gx), _ptr(gy), _ptr(gc), _ptr(grl), _ptr(gru)),
                     "vk_unrolled_backward")
         return gc, grl, gru
+
+    def set_data(self, objective, row_lower, row_upper):
+        """New objective and row bounds for the same matrix (vk_unrolled_set_data)."""
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (objective, row_lower, row_upper)]
+        rc = self.L.vk_unrolled_set_data(self.h, *[_ptr(a) for a in f])
+        if rc != 0:
+            raise ValueError(f"vk_unrolled_set_data failed ({rc}): {self.L.vk_solve_last_error().decode()}")
 
     def info(self):
         step, ms = C.c_double(), C.c_double()
