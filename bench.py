@@ -81,6 +81,28 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
             for name, ms, b in rows}
 
 
+# HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, scaled by a 256 MiB
+# device copy) measured with `bench.py --counters` under rocprofv3 --pmc;
+# profiles/pdhg_traffic_zipf_400k_x_1m.md holds the passes and commands.
+MEASURED_TRAFFIC = {"k_blocks<double>": 284.7e6, "k_dual_rows<double>": 45.3e6,
+                    "k_primal_spmv<double>": 292.2e6, "k_primal_cols<double>": 80.6e6}
+
+
+def counters_driver(V, args):
+    dev = V.Device()
+    nbytes = 256 << 20
+    a, b = dev.alloc(nbytes // 8), dev.alloc(nbytes // 8)
+    for _ in range(3):
+        V.native.check(dev.L.vk_hip_memcpy_d2d(b, a, nbytes, None), "vk_hip_memcpy_d2d")
+    V.native.check(dev.L.vk_hip_sync(None), "vk_hip_sync")
+    dev.free_all()
+    log("calibration copies done")
+    lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
+    p = V.profile_kernels(lp, V.Options(pdhg_policy=V.PdhgPolicy.Adaptive,
+                                        relative_kkt_tolerance=WORKLOAD["tolerance"]), 2, 6)
+    log(f"profiled: col_blocks={p['col_blocks']} split={p['split_passes']}")
+
+
 def measure_batch(V, np, args):
     """Batched solve of block-angular instances (T ~ U{4..48} periods of
     150 x 400 with 6 nonzeros per column, 40 coupling rows between periods,
@@ -182,6 +204,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true",
                     help="kernel timing only (used for the rocprofv3 pass)")
+    ap.add_argument("--counters", action="store_true",
+                    help="driver for the rocprofv3 --pmc passes: three 256 MiB device copies "
+                         "(counter calibration), then 2 + 6 profiled FP64 iterations; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and the PDHG path "
@@ -193,6 +218,9 @@ def main():
     import vikalp_amd as V
 
     V.load_libraries()
+    if args.counters:
+        counters_driver(V, args)
+        return
     t0 = time.time()
     lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
     log(f"generated LP m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t0:.1f}s")
@@ -295,7 +323,7 @@ def main():
         "ms_per_iteration_fp64": prof["iteration_ms"],
         "ms_per_iteration_mixed": prof32["iteration_ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dominant),
                      "kernel": dominant, "bytes_per_launch": dom_bytes, "mean_ms": dom_ms},
         "layout": {"fp64": {"col_blocks": prof["col_blocks"], "split_passes": prof["split_passes"]},
                    "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"]}},
