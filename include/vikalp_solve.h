@@ -121,6 +121,20 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
                   const double *var_upper, const vk_solver_options *options,
                   int64_t warm_iterations, int64_t iterations, double *out);
 
+/* PDQP (solver::solve_qp, pdqp.cpp:303-331): vk_lp_solve with the quadratic
+ * objective 0.5 x'Qx given as n x n CSR stored symmetrically (q_row_offsets
+ * NULL = none).  mode 0 = vikalp::solve (routes QP models to solve_qp), 1 =
+ * solve_qp on a HIP backend directly. */
+int vk_qp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
+                const int64_t *column_indices, const double *values,
+                const int64_t *q_row_offsets, const int64_t *q_column_indices,
+                const double *q_values, const double *objective, double objective_offset,
+                const double *row_lower, const double *row_upper,
+                const double *var_lower, const double *var_upper,
+                const double *warm_start, const double *lower_override,
+                const double *upper_override, const vk_solver_options *options,
+                double *x_out, double *y_out, vk_solve_info *info);
+
 /* Message of the last failure reported by a negative return code of this
  * header's functions on the calling thread. */
 const char *vk_solve_last_error(void);

@@ -112,6 +112,9 @@ def lib():
         L.orc_time_iterations.argtypes = [C.POINTER(_Lp), C.c_int64]
         L.orc_time_iterations.restype = C.c_double
         L.orc_unrolled.argtypes = [C.POINTER(_Lp), C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_solve_qp.argtypes = [C.POINTER(_Lp), _lp, _lp, _dp, _dp, _dp, _dp, C.POINTER(_Options),
+                                   _dp, _dp, C.POINTER(_Result)]
+        L.orc_solve_qp.restype = C.c_int
         L.orc_unrolled.restype = C.c_int
         _lib = L
     return _lib
@@ -132,6 +135,10 @@ def ref():
                                    C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.ref_solve_lp.restype = C.c_int
         L.ref_spmv.argtypes = [C.c_int64, C.c_int64, _lp, _lp, _dp, C.c_double, _dp, C.c_double, _dp, C.c_int]
+        L.ref_solve_qp.argtypes = [C.c_int64, C.c_int64, _lp, _lp, _dp, _lp, _lp, _dp, _dp, C.c_double,
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int64,
+                                   C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.ref_solve_qp.restype = C.c_int
         _ref = L
     return _ref
 
@@ -229,6 +236,59 @@ def ref_solve_lp(lp: LP, *, iteration_limit=100_000, time_limit_seconds=math.inf
                    int(iteration_limit), float(time_limit_seconds),
                    float(primal_tolerance), float(dual_tolerance), _ptr(x), _ptr(y),
                    _ptr(stats))
+    return Result(STATUS_NAMES[int(stats[0])], int(stats[1]), stats[2], stats[3],
+                  stats[4], stats[5], stats[6], stats[7], x, y)
+
+
+def _q_arrays(q):
+    qrp, qci, qv = q
+    return (np.ascontiguousarray(qrp, dtype=np.int64), np.ascontiguousarray(qci, dtype=np.int64),
+            np.ascontiguousarray(qv, dtype=np.float64))
+
+
+def solve_qp(lp: LP, q, *, iteration_limit=100_000, time_limit_seconds=math.inf,
+             primal_tolerance=1e-7, dual_tolerance=1e-7, warm_start=None, lower_override=None,
+             upper_override=None) -> Result:
+    """The restated PDQP solve (pdqp.cpp:303-331 + :20-300); q = (row_offsets,
+    column_indices, values) of the symmetric n x n Q."""
+    L = lib()
+    qrp, qci, qv = _q_arrays(q)
+    opt = _Options(int(iteration_limit), float(time_limit_seconds), float(primal_tolerance),
+                   float(dual_tolerance), 0.0, 0)
+    res = _Result()
+    x = np.zeros(lp.n)
+    y = np.zeros(lp.m)
+    arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+            for a in (lower_override, upper_override, warm_start)]
+    c_lp = lp._c()
+    L.orc_solve_qp(C.byref(c_lp), _ptr(qrp, _lp), _ptr(qci, _lp), _ptr(qv), _ptr(arrs[0]), _ptr(arrs[1]),
+                   _ptr(arrs[2]), C.byref(opt), _ptr(x), _ptr(y), C.byref(res))
+    return Result(STATUS_NAMES[res.status], res.iterations, res.objective_value,
+                  res.primal_residual, res.stationarity_residual,
+                  res.complementarity_residual, res.absolute_gap, res.relative_gap,
+                  x, y, res.rel_primal, res.rel_dual, res.rel_gap, res.rel_kkt,
+                  res.primal_objective, res.dual_objective, res.tau, res.sigma,
+                  res.last_metric, res.solve_seconds)
+
+
+def ref_solve_qp(lp: LP, q, *, iteration_limit=100_000, time_limit_seconds=math.inf,
+                 primal_tolerance=1e-7, dual_tolerance=1e-7, warm_start=None,
+                 lower_override=None, upper_override=None) -> Result:
+    """The reference's own solve_qp on its CpuBackend (oracle/_ref)."""
+    L = ref()
+    lp._c()
+    k = lp._keep
+    qrp, qci, qv = _q_arrays(q)
+    x = np.zeros(lp.n)
+    y = np.zeros(lp.m)
+    stats = np.zeros(8)
+    arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+            for a in (warm_start, lower_override, upper_override)]
+    L.ref_solve_qp(lp.m, lp.n, _ptr(k[0], _lp), _ptr(k[1], _lp), _ptr(k[2]), _ptr(qrp, _lp), _ptr(qci, _lp),
+                   _ptr(qv), _ptr(k[3]), float(lp.objective_offset), _ptr(k[4]), _ptr(k[5]), _ptr(k[6]),
+                   _ptr(k[7]), _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
+                   int(iteration_limit), float(time_limit_seconds),
+                   float(primal_tolerance), float(dual_tolerance), _ptr(x), _ptr(y), _ptr(stats))
     return Result(STATUS_NAMES[int(stats[0])], int(stats[1]), stats[2], stats[3],
                   stats[4], stats[5], stats[6], stats[7], x, y)
 

@@ -772,3 +772,275 @@ int orc_unrolled(const orc_lp *lp, int64_t iterations, const double *grad_x, con
     scaled_free(&s);
     return 0;
 }
+
+/* ------------------------------------------------------------------ PDQP --
+ * src/solver/qp/pdqp.cpp restated: solve_qp (:303-331) and solve_scaled_qp
+ * (:20-300), with positive_semidefinite (continuous_common.hpp:242-275) and
+ * objective_value (:284-299).  Q is n x n CSR stored symmetrically. */
+
+/* continuous_common.hpp:227-240 csr_to_dense */
+static double *dense_q(int64_t n, const int64_t *qrp, const int64_t *qci, const double *qv) {
+    double *a = calloc((size_t)n * (size_t)n + 1, sizeof(double));
+    if (!qrp) return a;
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t k = qrp[r]; k < qrp[r + 1]; ++k) a[(size_t)r * n + qci[k]] = qv[k];
+    return a;
+}
+
+/* continuous_common.hpp:242-275: dense Cholesky with a semidefinite pivot rule */
+static int q_positive_semidefinite(int64_t n, const int64_t *qrp, const int64_t *qci, const double *qv) {
+    double *a = dense_q(n, qrp, qci, qv);
+    const double tol = 1e-10;
+    int ok = 1;
+    for (int64_t k = 0; k < n && ok; ++k) {
+        double pivot = a[(size_t)k * n + k];
+        for (int64_t j = 0; j < k; ++j) pivot -= a[(size_t)k * n + j] * a[(size_t)k * n + j];
+        if (pivot < -tol * smax(1.0, fabs(a[(size_t)k * n + k]))) { ok = 0; break; }
+        if (pivot <= tol) {
+            for (int64_t i = k + 1; i < n; ++i) {
+                if (fabs(a[(size_t)i * n + k]) > tol) { ok = 0; break; }
+                a[(size_t)i * n + k] = 0.0;
+            }
+            continue;
+        }
+        const double d = sqrt(pivot);
+        a[(size_t)k * n + k] = d;
+        for (int64_t i = k + 1; i < n; ++i) {
+            double v = a[(size_t)i * n + k];
+            for (int64_t j = 0; j < k; ++j) v -= a[(size_t)i * n + j] * a[(size_t)k * n + j];
+            a[(size_t)i * n + k] = v / d;
+        }
+    }
+    free(a);
+    return ok;
+}
+
+/* Model.cpp:145-170: a square pattern matching the variables, entries stored
+ * symmetrically with equal values. */
+static int q_valid(int64_t n, const int64_t *qrp, const int64_t *qci, const double *qv) {
+    if (orc_validate_pattern(n, n, n + 1, qrp, qrp[n], qci) != 0) return 0;
+    for (int64_t k = 0; k < qrp[n]; ++k)
+        if (!isfinite(qv[k])) return 0;
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t k = qrp[r]; k < qrp[r + 1]; ++k) {
+            const int64_t c = qci[k];
+            int found = 0;
+            for (int64_t t = qrp[c]; t < qrp[c + 1]; ++t)
+                if (qci[t] == r) { found = qv[t] == qv[k]; break; }
+            if (!found) return 0;
+        }
+    return 1;
+}
+
+int orc_solve_qp(const orc_lp *lp, const int64_t *qrp, const int64_t *qci, const double *qv,
+                 const double *lower_override, const double *upper_override, const double *warm_start,
+                 const orc_options *options, double *x_out, double *y_out, orc_result *result) {
+    result_init(result);
+    const int64_t m = lp->m, n = lp->n;
+    const double *lower = lower_override ? lower_override : lp->var_lower;
+    const double *upper = upper_override ? upper_override : lp->var_upper;
+    /* :307-315 copy_bounds and validate_continuous */
+    for (int64_t j = 0; j < n; ++j)
+        if (isnan(lower[j]) || isnan(upper[j]) || lower[j] > upper[j]) {
+            result->status = ORC_INVALID_MODEL;
+            return result->status;
+        }
+    for (int64_t r = 0; r < m; ++r)
+        if (isnan(lp->row_lower[r]) || isnan(lp->row_upper[r]) || lp->row_lower[r] > lp->row_upper[r]) {
+            result->status = ORC_INVALID_MODEL;
+            return result->status;
+        }
+    if (orc_validate_pattern(m, n, m + 1, lp->row_offsets, lp->row_offsets[m], lp->column_indices) != 0 ||
+        !qrp || !q_valid(n, qrp, qci, qv)) {
+        result->status = ORC_INVALID_MODEL;
+        return result->status;
+    }
+    if (qrp[n] == 0) {                       /* no quadratic values: an LP */
+        result->status = ORC_UNSUPPORTED_MODEL;
+        return result->status;
+    }
+    if (n == 0) { result->status = ORC_INVALID_MODEL; return result->status; }             /* :316-318 */
+    if (!q_positive_semidefinite(n, qrp, qci, qv)) {                                        /* :319-321 */
+        result->status = ORC_UNSUPPORTED_MODEL;
+        return result->status;
+    }
+    if (!valid_options(options)) { result->status = ORC_INVALID_MODEL; return result->status; }   /* :322-324 */
+    const double started = now_seconds();
+    scaled_lp s;
+    scaled_alloc(&s, lp);
+    orc_scale_lp(lp, lower, upper, 0, s.v, s.c, s.rl, s.ru, s.l, s.u, s.rs, s.cs);
+    if (warm_start)                                                                         /* :29-34 */
+        for (int64_t j = 0; j < n; ++j)
+            if (!isfinite(warm_start[j])) {
+                scaled_free(&s);
+                result->status = ORC_INVALID_MODEL;
+                return result->status;
+            }
+    /* :36-45 Q scaled by s_row s_col */
+    const int64_t qnnz = qrp[n];
+    double *sq = malloc(((size_t)qnnz + 1) * sizeof(double));
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t k = qrp[r]; k < qrp[r + 1]; ++k) sq[k] = qv[k] * (s.cs[r] * s.cs[qci[k]]);
+    work w;
+    work_alloc(&w, m, n);
+    double *qx = calloc((size_t)n + 1, sizeof(double));
+    /* :75-85 */
+    if (warm_start)
+        for (int64_t j = 0; j < n; ++j) w.x[j] = warm_start[j] / s.cs[j];
+    orc_project_box(n, w.x, s.l, s.u);
+    /* :87-99 */
+    const double norm = orc_operator_bound(m, n, s.rp, s.ci, s.v);
+    double q_norm = 0.0;
+    for (int64_t r = 0; r < n; ++r) {
+        double row_sum = 0.0;
+        for (int64_t k = qrp[r]; k < qrp[r + 1]; ++k) row_sum += fabs(sq[k]);
+        q_norm = smax(q_norm, row_sum);
+    }
+    const double max_step = 0.99 / smax(1.0, norm * norm + q_norm);
+    loop_state st;
+    st.max_step = max_step;
+    st.tau = st.sigma = max_step;
+    st.prev_metric = INFINITY;
+    st.last_metric = INFINITY;
+    st.restart = 0;
+    int status = ORC_NOT_SOLVED;
+    int64_t iterations = 0;
+    for (int64_t iter = 0; iter < options->iteration_limit; ++iter) {
+        if (isfinite(options->time_limit_seconds) && now_seconds() - started >= options->time_limit_seconds) {
+            status = ORC_TIME_LIMIT;
+            break;
+        }
+        const double tau = st.tau, sigma = st.sigma;
+        /* :115-122 */
+        orc_spmv(m, n, s.rp, s.ci, s.v, 1.0, w.y, 0.0, w.aty, 1);
+        orc_spmv(n, n, qrp, qci, sq, 1.0, w.x, 0.0, qx, 0);
+        memcpy(w.grad, qx, (size_t)n * sizeof(double));
+        orc_axpby(n, 1.0, s.c, 1.0, w.grad);
+        orc_axpby(n, 1.0, w.aty, 1.0, w.grad);
+        memcpy(w.x_next, w.x, (size_t)n * sizeof(double));
+        orc_axpby(n, -tau, w.grad, 1.0, w.x_next);
+        orc_project_box(n, w.x_next, s.l, s.u);
+        /* :124-130 */
+        if (st.restart) {
+            memcpy(w.x_bar, w.x_next, (size_t)n * sizeof(double));
+            st.restart = 0;
+        } else {
+            memcpy(w.x_bar, w.x, (size_t)n * sizeof(double));
+            orc_axpby(n, 2.0, w.x_next, -1.0, w.x_bar);
+        }
+        /* :131-139 */
+        orc_spmv(m, n, s.rp, s.ci, s.v, 1.0, w.x_bar, 0.0, w.ax_bar, 0);
+        memcpy(w.y_arg, w.y, (size_t)m * sizeof(double));
+        orc_axpby(m, sigma, w.ax_bar, 1.0, w.y_arg);
+        memcpy(w.y_proj, w.y_arg, (size_t)m * sizeof(double));
+        orc_axpby(m, 1.0 / sigma, w.y_proj, 0.0, w.y_proj);
+        orc_project_box(m, w.y_proj, s.rl, s.ru);
+        memcpy(w.y, w.y_arg, (size_t)m * sizeof(double));
+        orc_axpby(m, -sigma, w.y_proj, 1.0, w.y);
+        memcpy(w.x, w.x_next, (size_t)n * sizeof(double));
+        iterations = iter + 1;
+        /* :142-146 */
+        orc_spmv(m, n, s.rp, s.ci, s.v, 1.0, w.x, 0.0, w.ax, 0);
+        memcpy(w.pax, w.ax, (size_t)m * sizeof(double));
+        orc_project_box(m, w.pax, s.rl, s.ru);
+        orc_axpby(m, 1.0, w.pax, -1.0, w.ax);
+        const double primal = orc_norm_inf(m, w.ax);
+        /* :148-158 */
+        orc_spmv(m, n, s.rp, s.ci, s.v, 1.0, w.y, 0.0, w.aty, 1);
+        orc_spmv(n, n, qrp, qci, sq, 1.0, w.x, 0.0, qx, 0);
+        memcpy(w.grad, qx, (size_t)n * sizeof(double));
+        orc_axpby(n, 1.0, s.c, 1.0, w.grad);
+        orc_axpby(n, 1.0, w.aty, 1.0, w.grad);
+        memcpy(w.px, w.x, (size_t)n * sizeof(double));
+        orc_axpby(n, -1.0, w.grad, 1.0, w.px);
+        orc_project_box(n, w.px, s.l, s.u);
+        memcpy(w.res, w.x, (size_t)n * sizeof(double));
+        orc_axpby(n, -1.0, w.px, 1.0, w.res);
+        const double metric = smax(primal, orc_norm_inf(n, w.res));
+        st.last_metric = metric;
+        step_rule(&st, iter, metric);                                                       /* :161-171 */
+        if (metric <= smax(options->primal_tolerance, options->dual_tolerance)) {           /* :172-175 */
+            status = ORC_OPTIMAL;
+            break;
+        }
+    }
+    result->iterations = iterations;
+    result->tau = st.tau;
+    result->sigma = st.sigma;
+    result->last_metric = st.last_metric;
+    /* :178-300 postsolve */
+    double *xo = malloc(((size_t)n + 1) * sizeof(double));
+    double *yo = malloc(((size_t)m + 1) * sizeof(double));
+    unscale(&s, &w, xo, yo);
+    double *qd = dense_q(n, qrp, qci, qv);
+    /* objective_value: offset + c'x + 0.5 sum_r sum_c x_r q_rc x_c */
+    double obj = lp->objective_offset + orc_dot(n, lp->objective, xo);
+    double quad = 0.0;
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t c = 0; c < n; ++c) quad += xo[r] * qd[(size_t)r * n + c] * xo[c];
+    obj += 0.5 * quad;
+    result->objective_value = obj;
+    if (!isfinite(obj)) status = ORC_NUMERICAL_FAILURE;
+    double *row_values = calloc((size_t)m + 1, sizeof(double));
+    for (int64_t r = 0; r < m; ++r)
+        for (int64_t k = lp->row_offsets[r]; k < lp->row_offsets[r + 1]; ++k)
+            row_values[r] += lp->values[k] * xo[lp->column_indices[k]];
+    double pv = 0.0;
+    for (int64_t r = 0; r < m; ++r) {
+        if (isfinite(lp->row_lower[r])) pv = smax(pv, lp->row_lower[r] - row_values[r]);
+        if (isfinite(lp->row_upper[r])) pv = smax(pv, row_values[r] - lp->row_upper[r]);
+    }
+    pv = smax(0.0, pv);
+    double bv = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        if (isfinite(lower[j])) bv = smax(bv, lower[j] - xo[j]);
+        if (isfinite(upper[j])) bv = smax(bv, xo[j] - upper[j]);
+    }
+    bv = smax(0.0, bv);
+    result->primal_residual = smax(pv, bv);
+    /* :216-232 gradient = c + Q x (dense rows) + A'y (row order) */
+    double *grad = malloc(((size_t)n + 1) * sizeof(double));
+    memcpy(grad, lp->objective, (size_t)n * sizeof(double));
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t c = 0; c < n; ++c) grad[r] += qd[(size_t)r * n + c] * xo[c];
+    for (int64_t r = 0; r < m; ++r)
+        for (int64_t k = lp->row_offsets[r]; k < lp->row_offsets[r + 1]; ++k)
+            grad[lp->column_indices[k]] += yo[r] * lp->values[k];
+    double sr = 0.0, cr = 0.0;
+    const double active = smax(10.0 * options->primal_tolerance, 1e-8);
+    for (int64_t j = 0; j < n; ++j) {
+        const double lo = lower[j], hi = upper[j];
+        const double t = xo[j] - grad[j];
+        const double proj = (t < lo) ? lo : ((hi < t) ? hi : t);
+        sr = smax(sr, fabs(xo[j] - proj));
+        if (isfinite(lo) && xo[j] - lo <= active) cr = smax(cr, fabs((xo[j] - lo) * smax(0.0, grad[j])));
+        if (isfinite(hi) && hi - xo[j] <= active) cr = smax(cr, fabs((hi - xo[j]) * smax(0.0, -grad[j])));
+    }
+    for (int64_t r = 0; r < m; ++r) {
+        const double v = row_values[r], lo = lp->row_lower[r], hi = lp->row_upper[r];
+        if (isfinite(lo)) cr = smax(cr, fabs((v - lo) * smax(0.0, -yo[r])));
+        if (isfinite(hi)) cr = smax(cr, fabs((hi - v) * smax(0.0, yo[r])));
+    }
+    result->stationarity_residual = sr;
+    result->complementarity_residual = cr;
+    if (status == ORC_NOT_SOLVED) status = ORC_ITERATION_LIMIT;
+    int finite = 1;
+    for (int64_t j = 0; j < n; ++j) if (!isfinite(xo[j])) { finite = 0; break; }
+    if (!finite) status = ORC_NUMERICAL_FAILURE;
+    else if (status == ORC_OPTIMAL &&
+             (result->primal_residual > options->primal_tolerance ||
+              result->stationarity_residual > options->dual_tolerance))
+        status = ORC_NUMERICAL_FAILURE;
+    if (status == ORC_OPTIMAL) {
+        result->absolute_gap = 0.0;
+        result->relative_gap = 0.0;
+    }
+    result->status = status;
+    result->solve_seconds = now_seconds() - started;
+    if (x_out) memcpy(x_out, xo, (size_t)n * sizeof(double));
+    if (y_out) memcpy(y_out, yo, (size_t)m * sizeof(double));
+    free(grad); free(row_values); free(qd); free(xo); free(yo); free(qx); free(sq);
+    work_free(&w);
+    scaled_free(&s);
+    return status;
+}

@@ -122,6 +122,15 @@ int orc_solve_lp(const orc_lp *lp, const double *lower_override,
  * instances too large to solve to tolerance. */
 double orc_time_iterations(const orc_lp *lp, int64_t iterations);
 
+/* ---- PDQP: solve_qp (src/solver/qp/pdqp.cpp:303-331, :20-300) ----
+ * Q (n x n CSR, symmetric) given by qrp/qci/qv.  Same outputs as
+ * orc_solve_lp; the relative-KKT options are ignored (the QP loop stops on
+ * the reference metric only).  A non-PSD Q gives UNSUPPORTED_MODEL
+ * (continuous_common.hpp:242-275, dense). */
+int orc_solve_qp(const orc_lp *lp, const int64_t *qrp, const int64_t *qci, const double *qv,
+                 const double *lower_override, const double *upper_override, const double *warm_start,
+                 const orc_options *options, double *x_out, double *y_out, orc_result *result);
+
 /* Unrolled fixed-step PDHG and its adjoint (not in the reference, which has
  * no differentiation; see pdhg_oracle.c for the recursion).  Runs
  * `iterations` steps of pdhg.cpp:92-113 with tau = sigma = 0.99/max(1,bound)

@@ -142,6 +142,19 @@ __global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b) {
     store_partials(part, b.part_cols, scratch);
 }
 
+// Q x+ for a quadratic objective (pdqp.cpp:116, :149): FP64 sums into b.qx,
+// before the primal kernel that uses them.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_qx(Bufs<T> b, CsrView<T> Q) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double lds[kWaves][kTileNnz];
+    StoreEpi<double> epi{b.qx};
+    Partials unused;
+    unused.zero();
+    tile_spmv(Q, b.x[(s.k + 1) & 1], epi, unused, lds[threadIdx.x / kWave]);
+}
+
 // Column-blocked A x into per-block partial row sums (part[p*m + row]).
 // `which` selects x: 0 = the primal iterate x[(k+1)&1] (dual step),
 // 1 = the running sum xsum (average evaluation).  The grid is a multiple of
@@ -451,6 +464,7 @@ using namespace vk;
 struct vk_hip_pdhg {
     vk_pdhg_params params{};
     const vk_hip_csr *A = nullptr;
+    const vk_hip_csr *Q = nullptr;   // quadratic objective (PDQP), FP64 only
     int m = 0, n = 0;
     hipStream_t stream = nullptr;
     DevState *st = nullptr;
@@ -539,6 +553,12 @@ void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
 template <class T>
 void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
     Bufs<T> &b = bufs<T>(s);
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (s->Q && (part & 1)) {
+            const CsrView<double> Q = s->Q->a.view();
+            k_qx<T><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(b, Q);
+        }
+    }
     if (s->split) {
         if (part & 1) k_primal_spmv<T><<<s->grid_spmv_cols, kBlock, 0, st>>>(b, view_at<T>(s));
         if (part & 2) k_primal_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b);
@@ -757,6 +777,21 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
         VK_HIP_CHECK(hipStreamSynchronize(st));
         s->host_state->ns = 0;
         VK_HIP_CHECK(hipMemcpyAsync(s->st, s->host_state, sizeof(DevState), hipMemcpyHostToDevice, st));
+    }
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (s->Q) {
+            // pdqp.cpp:116 Q x0 for the first primal step.
+            b.qx = s->alloc<double>(n);
+            if (!b.qx) {
+                set_error("vk_hip_pdqp_create: device allocation failed");
+                return VK_ENOMEM;
+            }
+            const CsrView<double> Q = s->Q->a.view();
+            k_zero<double><<<egrid(n), kBlock, 0, st>>>(n, b.qx);
+            if (Q.ntiles > 0)
+                k_spmv_store<double><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(
+                    Q, b.x[0], b.qx);
+        }
     }
     // First primal step x1 = P(x0 - tau (c + A'y0)) into x[1].
     k_pstep<T><<<egrid(n), kBlock, 0, st>>>(b);
@@ -1063,12 +1098,20 @@ int vk_hip_pdhg_destroy(vk_hip_pdhg *s) {
     return VK_OK;
 }
 
-int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, const double *u,
-                       const double *rl, const double *ru, const double *row_scale,
+int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c, const double *l,
+                       const double *u, const double *rl, const double *ru, const double *row_scale,
                        const double *col_scale, const double *x0, const double *y0,
                        const vk_pdhg_params *params, vk_hip_pdhg **out) {
-    VK_REQUIRE(out && A && params, "vk_hip_pdhg_create: null argument");
+    VK_REQUIRE(out && A && params, "vk_hip_pdqp_create: null argument");
     *out = nullptr;
+    if (Q) {
+        VK_REQUIRE(Q->has_values, "vk_hip_pdqp_create: Q values were never set");
+        VK_REQUIRE(Q->rows == A->cols && Q->cols == A->cols, "vk_hip_pdqp_create: Q must be n x n");
+        VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE && params->precision == VK_PREC_FP64 &&
+                       params->relative_kkt_tolerance == 0.0,
+                   "vk_hip_pdqp_create: a quadratic objective runs the reference policy in FP64 "
+                   "(pdqp.cpp) without the relative-KKT test");
+    }
     VK_REQUIRE(A->has_values, "vk_hip_pdhg_create: matrix values were never set");
     VK_REQUIRE(A->cols > 0, "vk_hip_pdhg_create: the LP needs at least one variable");
     VK_REQUIRE(c && l && u && col_scale, "vk_hip_pdhg_create: null column vector");
@@ -1092,6 +1135,7 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, co
     if (!s) return VK_ENOMEM;
     s->params = *params;
     s->A = A;
+    s->Q = Q;
     s->m = static_cast<int>(A->rows);
     s->n = static_cast<int>(A->cols);
     s->mixed = params->precision == VK_PREC_MIXED;
@@ -1168,6 +1212,13 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, co
     if (rc) return fail(rc);
     *out = s;
     return VK_OK;
+}
+
+int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, const double *u,
+                       const double *rl, const double *ru, const double *row_scale,
+                       const double *col_scale, const double *x0, const double *y0,
+                       const vk_pdhg_params *params, vk_hip_pdhg **out) {
+    return vk_hip_pdqp_create(A, nullptr, c, l, u, rl, ru, row_scale, col_scale, x0, y0, params, out);
 }
 
 int vk_hip_pdhg_run(vk_hip_pdhg *s, int64_t max_iterations, int *status) {

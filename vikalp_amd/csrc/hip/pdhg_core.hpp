@@ -49,6 +49,7 @@ struct Bufs {
     T *y, *ax, *aty;    // current y, A x, A' y (in place)
     double *atyd;       // A' y before rounding to T (split primal pass; aliases
                         // aty when T is double)
+    double *qx;         // Q x+ of a quadratic objective (PDQP); nullptr for an LP
     const T *c, *l, *u, *rl, *ru;
     const T *row_unscale, *col_scale;   // 1/r and s
     // Adaptive only.  The running sums of the iterates stay FP64 in every
@@ -161,10 +162,11 @@ struct PrimalEpi {
     bool store_aty = true;   // false when aty already holds acc
     struct Pre {
         T x, c, l, u, s, xlast;
-        double xsum;
+        double xsum, qx;
     };
     __device__ Pre prefetch(int j) const {
         Pre p;
+        p.qx = b.qx ? b.qx[j] : 0.0;
         p.x = xcur[j];
         p.c = b.c[j];
         p.l = b.l[j];
@@ -179,10 +181,14 @@ struct PrimalEpi {
         const double xj = static_cast<double>(q.x);
         const double cj = static_cast<double>(q.c);
         const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
-        column_terms(xj, cj, cj + acc, lj, uj, static_cast<double>(q.s), b.xlast != nullptr,
+        // pdhg.cpp:122-125 g = c + A'y; pdqp.cpp:148-152 g = A'y + (c + Q x).
+        const double g = b.qx ? acc + (cj + q.qx) : cj + acc;
+        column_terms(xj, cj, g, lj, uj, static_cast<double>(q.s), b.xlast != nullptr,
                      static_cast<double>(q.xlast), p);
         if (fuse) {
-            const T xn = static_cast<T>(clamp_box((xj - tau * cj) - tau * acc, lj, uj));
+            // pdhg.cpp:93-96 (x - tau c) - tau A'y;  pdqp.cpp:120-122 x - tau g.
+            const T xn = static_cast<T>(
+                clamp_box(b.qx ? -tau * g + xj : (xj - tau * cj) - tau * acc, lj, uj));
             xnext[j] = xn;
             if (b.xsum) b.xsum[j] = q.xsum + static_cast<double>(xn);
         }
@@ -411,13 +417,14 @@ __device__ __forceinline__ void apply_row(const Bufs<T> &b, const DevState &s, i
     }
 }
 
-// One element of the standalone primal step x[k+1] = P_X(x[k] - tau (c + aty)).
+// One element of the standalone primal step x[k+1] = P_X(x[k] - tau (c + aty))
+// (with Q: x[k] - tau (aty + (c + Q x[k])), pdqp.cpp:115-122).
 template <class T>
 __device__ __forceinline__ void pstep_element(const Bufs<T> &b, const DevState &s, int j, const T *xcur, T *xnext) {
     const double xj = static_cast<double>(xcur[j]);
-    const T xn = static_cast<T>(clamp_box((xj - s.tau * static_cast<double>(b.c[j])) -
-                                              s.tau * static_cast<double>(b.aty[j]),
-                                          static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
+    const double cj = static_cast<double>(b.c[j]), aj = static_cast<double>(b.aty[j]);
+    const double z = b.qx ? -s.tau * (aj + (cj + b.qx[j])) + xj : (xj - s.tau * cj) - s.tau * aj;
+    const T xn = static_cast<T>(clamp_box(z, static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
     xnext[j] = xn;
     if (b.xsum) b.xsum[j] += static_cast<double>(xn);
 }

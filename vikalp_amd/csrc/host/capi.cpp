@@ -260,6 +260,55 @@ int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
     return 0;
 }
 
+int vk_qp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets, const int64_t *column_indices,
+                const double *values, const int64_t *q_row_offsets, const int64_t *q_column_indices,
+                const double *q_values, const double *objective, double objective_offset, const double *row_lower,
+                const double *row_upper, const double *var_lower, const double *var_upper,
+                const double *warm_start, const double *lower_override, const double *upper_override,
+                const vk_solver_options *options, double *x_out, double *y_out, vk_solve_info *info) {
+    if (!info || !options) return -1;
+    Model model;
+    if (!build_model(m, n, row_offsets, column_indices, values, objective, objective_offset, row_lower,
+                     row_upper, var_lower, var_upper, model))
+        return -1;
+    if (q_row_offsets) {
+        const int64_t qnnz = q_row_offsets[n];
+        if (qnnz < 0 || (qnnz > 0 && (!q_column_indices || !q_values))) return -1;
+        model.quadratic_objective.pattern.rows = n;
+        model.quadratic_objective.pattern.columns = n;
+        model.quadratic_objective.pattern.row_offsets.assign(q_row_offsets, q_row_offsets + n + 1);
+        model.quadratic_objective.pattern.column_indices.assign(q_column_indices, q_column_indices + qnnz);
+        model.quadratic_objective.values.assign(q_values, q_values + qnnz);
+    }
+    const SolverOptions opts = to_options(*options);
+    SolveResult r;
+    if (mode == 0) {
+        r = solve(model, opts);
+    } else {
+        try {
+            auto backend = make_hip_backend();
+            const auto span_or_empty = [n](const double *p) {
+                return p ? std::span<const double>(p, static_cast<std::size_t>(n)) : std::span<const double>{};
+            };
+            r = solver::solve_qp(model, *backend, opts, span_or_empty(warm_start), span_or_empty(lower_override),
+                                 span_or_empty(upper_override));
+            r.backend = "hip";
+        } catch (const std::exception &e) {
+            r = SolveResult{};
+            r.status = SolveStatus::NumericalFailure;
+            r.message = e.what();
+        }
+    }
+    fill_info(r, *info);
+    if (x_out && r.primal_solution.size() == static_cast<std::size_t>(n))
+        std::memcpy(x_out, r.primal_solution.data(), sizeof(double) * static_cast<std::size_t>(n));
+    if (y_out && r.constraint_upper_dual.size() == static_cast<std::size_t>(m))
+        for (int64_t i = 0; i < m; ++i)
+            y_out[i] = r.constraint_upper_dual[static_cast<std::size_t>(i)] -
+                       r.constraint_lower_dual[static_cast<std::size_t>(i)];
+    return 0;
+}
+
 const char *vk_solve_last_error(void) { return g_last_error.c_str(); }
 
 struct vk_unrolled {

@@ -21,6 +21,17 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend,
                      std::span<const Scalar> variable_lower_override = {},
                      std::span<const Scalar> variable_upper_override = {});
 
+// PDQP (src/solver/qp/pdqp.cpp:303-331): the same loop with the gradient
+// Q x + c + A'y and the step 0.99 / max(1, bound^2 + |Q|_inf).  On a
+// HipBackend with options.fused_iterations the iteration runs on the device;
+// otherwise the reference's op-by-op sequence is issued.  The reference
+// policy in FP64 only.
+SolveResult solve_qp(const Model &model, ExecutionBackend &backend,
+                     const SolverOptions &options = {},
+                     std::span<const Scalar> warm_start = {},
+                     std::span<const Scalar> variable_lower_override = {},
+                     std::span<const Scalar> variable_upper_override = {});
+
 // Solves every LP of `models` together on the batched persistent HIP kernel
 // (one launch runs many instances; each keeps its own step sizes, restart
 // state and status).  Results are in the order of `models`, each with

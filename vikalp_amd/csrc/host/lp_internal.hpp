@@ -41,5 +41,8 @@ void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &op
                const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,
                SolveResult &res);                                          // pdhg.cpp:153-264
 void check_vk(int rc, const char *what);
+// continuous_common.hpp:242-275 (dense) up to n = 4096; beyond that a
+// diagonally dominant Q with a non-negative diagonal.  `message` says why not.
+bool positive_semidefinite(const CsrMatrix &q, std::string &message);
 
 }  // namespace vikalp::solver::detail

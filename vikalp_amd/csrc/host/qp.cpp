@@ -1,0 +1,425 @@
+// solver::solve_qp on the HIP backend: PDQP, the PDHG loop with the gradient
+// Q x + c + A'y (src/solver/qp/pdqp.cpp).  Entry checks follow pdqp.cpp:303-331
+// (bounds, model validation, class QP, a variable, a PSD Q, options, method),
+// the scaling continuous_common.hpp:133-205 with Q scaled by s_i s_j
+// (pdqp.cpp:36-45), the step 0.99 / max(1, bound^2 + |Q|_inf) (pdqp.cpp:87-99)
+// and the postsolve pdqp.cpp:178-300.  Two executions: the fused device
+// iteration (vk_hip_pdqp_create: one more SpMV per iteration, Q x+, ahead of
+// the primal kernel) and the reference's op-by-op sequence on any
+// ExecutionBackend.
+
+#include "lp_internal.hpp"
+
+#include "vikalp/backend/HipBackend.hpp"
+#include "vikalp/solver/continuous.hpp"
+#include "vikalp_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace vikalp::solver {
+namespace detail {
+
+// Largest n the dense check runs for; beyond it a diagonally dominant Q with
+// a non-negative diagonal is accepted (Gershgorin: every eigenvalue lies in
+// some [q_ii - r_i, q_ii + r_i] with r_i the off-diagonal row sum).
+constexpr Index kDensePsdLimit = 4096;
+
+bool positive_semidefinite(const CsrMatrix &q, std::string &message) {
+    const Index n = q.pattern.rows;
+    if (n > kDensePsdLimit) {
+        for (Index i = 0; i < n; ++i) {
+            Scalar diag = 0.0, off = 0.0;
+            for (Index k = q.pattern.row_offsets[i]; k < q.pattern.row_offsets[i + 1]; ++k) {
+                if (q.pattern.column_indices[k] == i) diag = q.values[k];
+                else off += std::abs(q.values[k]);
+            }
+            if (diag < off) {
+                message = "QP objective matrix is not diagonally dominant and too large for the dense "
+                          "positive-semidefinite check";
+                return false;
+            }
+        }
+        return true;
+    }
+    // continuous_common.hpp:242-275: dense Cholesky that lets a zero pivot
+    // through only when its column below is zero as well.
+    const auto nn = static_cast<std::size_t>(n);
+    std::vector<Scalar> a(nn * nn, 0.0);
+    for (Index r = 0; r < n; ++r)
+        for (Index k = q.pattern.row_offsets[r]; k < q.pattern.row_offsets[r + 1]; ++k)
+            a[static_cast<std::size_t>(r) * nn + static_cast<std::size_t>(q.pattern.column_indices[k])] = q.values[k];
+    const Scalar tol = 1e-10;
+    for (std::size_t k = 0; k < nn; ++k) {
+        Scalar pivot = a[k * nn + k];
+        for (std::size_t j = 0; j < k; ++j) pivot -= a[k * nn + j] * a[k * nn + j];
+        if (pivot < -tol * std::max<Scalar>(1.0, std::abs(a[k * nn + k]))) {
+            message = "QP objective matrix is not positive semidefinite";
+            return false;
+        }
+        if (pivot <= tol) {
+            for (std::size_t i = k + 1; i < nn; ++i) {
+                if (std::abs(a[i * nn + k]) > tol) {
+                    message = "QP objective matrix is not positive semidefinite";
+                    return false;
+                }
+                a[i * nn + k] = 0.0;
+            }
+            continue;
+        }
+        const Scalar d = std::sqrt(pivot);
+        a[k * nn + k] = d;
+        for (std::size_t i = k + 1; i < nn; ++i) {
+            Scalar v = a[i * nn + k];
+            for (std::size_t j = 0; j < k; ++j) v -= a[i * nn + j] * a[k * nn + j];
+            a[i * nn + k] = v / d;
+        }
+    }
+    return true;
+}
+
+}  // namespace detail
+
+namespace {
+
+using namespace detail;
+
+constexpr Scalar kInf = std::numeric_limits<Scalar>::infinity();
+
+struct ScaledQ {
+    std::vector<Scalar> values;   // q_ij s_i s_j
+    Scalar norm_inf = 0.0;        // max row sum of |q_ij s_i s_j|
+};
+
+ScaledQ scale_q(const CsrMatrix &q, const std::vector<Scalar> &s) {
+    ScaledQ out;
+    out.values = q.values;
+    for (Index r = 0; r < q.pattern.rows; ++r) {
+        Scalar row_sum = 0.0;
+        for (Index k = q.pattern.row_offsets[r]; k < q.pattern.row_offsets[r + 1]; ++k) {
+            out.values[k] *= s[r] * s[q.pattern.column_indices[k]];   // pdqp.cpp:42-44
+            row_sum += std::abs(out.values[k]);
+        }
+        out.norm_inf = std::max(out.norm_inf, row_sum);   // pdqp.cpp:90-97
+    }
+    return out;
+}
+
+// pdqp.cpp:178-300 on original-coordinate x, y (reference sign).
+void postsolve_qp(const Model &model, const Bounds &bounds, const SolverOptions &options,
+                  const std::vector<Scalar> &x, const std::vector<Scalar> &y, SolveResult &res) {
+    const auto &A = model.constraint_matrix;
+    const auto &Q = model.quadratic_objective;
+    const auto m = static_cast<std::size_t>(A.pattern.rows);
+    const auto n = static_cast<std::size_t>(A.pattern.columns);
+    res.primal_solution = x;
+    // continuous_common.hpp:284-299 (row-major over the stored entries; the
+    // dense loop adds only zero terms besides these).
+    Scalar value = model.objective_offset;
+    Scalar lin = 0.0;
+    for (std::size_t j = 0; j < n; ++j) lin += model.linear_objective[j] * x[j];
+    value += lin;
+    Scalar quad = 0.0;
+    for (std::size_t r = 0; r < n; ++r)
+        for (Index k = Q.pattern.row_offsets[r]; k < Q.pattern.row_offsets[r + 1]; ++k)
+            quad += x[r] * Q.values[k] * x[static_cast<std::size_t>(Q.pattern.column_indices[k])];
+    value += 0.5 * quad;
+    res.objective_value = value;
+    if (!std::isfinite(res.objective_value)) {
+        res.status = SolveStatus::NumericalFailure;
+        res.message = "PDQP produced a non-finite objective";
+    }
+    std::vector<Scalar> row_values(m, 0.0);
+    for (std::size_t i = 0; i < m; ++i)
+        for (Index k = A.pattern.row_offsets[i]; k < A.pattern.row_offsets[i + 1]; ++k)
+            row_values[i] += A.values[k] * x[static_cast<std::size_t>(A.pattern.column_indices[k])];
+    Scalar viol = 0.0, box = 0.0;
+    for (std::size_t i = 0; i < m; ++i) {
+        if (std::isfinite(model.constraint_lower[i])) viol = std::max(viol, model.constraint_lower[i] - row_values[i]);
+        if (std::isfinite(model.constraint_upper[i])) viol = std::max(viol, row_values[i] - model.constraint_upper[i]);
+    }
+    for (std::size_t j = 0; j < n; ++j) {
+        if (std::isfinite(bounds.lower[j])) box = std::max(box, bounds.lower[j] - x[j]);
+        if (std::isfinite(bounds.upper[j])) box = std::max(box, x[j] - bounds.upper[j]);
+    }
+    res.primal_residual = std::max(std::max<Scalar>(0.0, viol), std::max<Scalar>(0.0, box));
+    res.constraint_lower_dual.resize(m);
+    res.constraint_upper_dual.resize(m);
+    for (std::size_t i = 0; i < m; ++i) {
+        res.constraint_lower_dual[i] = std::max<Scalar>(0.0, -y[i]);
+        res.constraint_upper_dual[i] = std::max<Scalar>(0.0, y[i]);
+    }
+    // pdqp.cpp:214-232: g = c + Q x + A'y.
+    std::vector<Scalar> g(model.linear_objective);
+    for (std::size_t r = 0; r < n; ++r)
+        for (Index k = Q.pattern.row_offsets[r]; k < Q.pattern.row_offsets[r + 1]; ++k)
+            g[r] += Q.values[k] * x[static_cast<std::size_t>(Q.pattern.column_indices[k])];
+    for (std::size_t i = 0; i < m; ++i)
+        for (Index k = A.pattern.row_offsets[i]; k < A.pattern.row_offsets[i + 1]; ++k)
+            g[static_cast<std::size_t>(A.pattern.column_indices[k])] += y[i] * A.values[k];
+    res.stationarity_residual = 0.0;
+    res.complementarity_residual = 0.0;
+    res.variable_lower_dual.assign(n, 0.0);
+    res.variable_upper_dual.assign(n, 0.0);
+    const Scalar active = std::max<Scalar>(10.0 * options.primal_tolerance, 1e-8);
+    for (std::size_t j = 0; j < n; ++j) {
+        const Scalar lo = bounds.lower[j], hi = bounds.upper[j];
+        const Scalar proj = std::clamp(x[j] - g[j], lo, hi);
+        res.stationarity_residual = std::max(res.stationarity_residual, std::abs(x[j] - proj));
+        if (std::isfinite(lo) && x[j] - lo <= active) {
+            res.variable_lower_dual[j] = std::max<Scalar>(0.0, g[j]);
+            res.complementarity_residual =
+                std::max(res.complementarity_residual, std::abs((x[j] - lo) * res.variable_lower_dual[j]));
+        }
+        if (std::isfinite(hi) && hi - x[j] <= active) {
+            res.variable_upper_dual[j] = std::max<Scalar>(0.0, -g[j]);
+            res.complementarity_residual =
+                std::max(res.complementarity_residual, std::abs((hi - x[j]) * res.variable_upper_dual[j]));
+        }
+    }
+    for (std::size_t i = 0; i < m; ++i) {
+        const Scalar v = row_values[i], lo = model.constraint_lower[i], hi = model.constraint_upper[i];
+        if (std::isfinite(lo))
+            res.complementarity_residual =
+                std::max(res.complementarity_residual, std::abs((v - lo) * res.constraint_lower_dual[i]));
+        if (std::isfinite(hi))
+            res.complementarity_residual =
+                std::max(res.complementarity_residual, std::abs((hi - v) * res.constraint_upper_dual[i]));
+    }
+    if (res.status == SolveStatus::NotSolved) res.status = SolveStatus::IterationLimit;
+    bool finite = true;
+    for (const Scalar v : x) finite = finite && std::isfinite(v);
+    if (!finite) {
+        res.status = SolveStatus::NumericalFailure;
+        res.message = "PDQP produced a non-finite iterate";
+    } else if (res.status == SolveStatus::Optimal &&
+               (res.primal_residual > options.primal_tolerance ||
+                res.stationarity_residual > options.dual_tolerance)) {
+        res.status = SolveStatus::NumericalFailure;
+        res.message = "PDQP stopped without satisfying original-coordinate residual checks";
+    }
+    if (res.status == SolveStatus::Optimal) {
+        res.primal_bound = res.objective_value;
+        res.dual_bound = res.objective_value;
+        res.absolute_gap = 0.0;
+        res.relative_gap = 0.0;
+    }
+}
+
+// pdqp.cpp:110-176 issued call for call on an ExecutionBackend.
+void iterate_ops(const Model &model, const Scaled &sc, ExecutionBackend &be, BackendMatrix &A,
+                 BackendMatrix &Q, Scalar max_step, const SolverOptions &options,
+                 const std::vector<Scalar> *warm_scaled, Clock::time_point started, std::vector<Scalar> &xs,
+                 std::vector<Scalar> &ys, SolveResult &res) {
+    const Index m = model.num_constraints(), n = model.num_variables();
+    auto x = be.create_vector(n), x_next = be.create_vector(n), x_bar = be.create_vector(n);
+    auto y = be.create_vector(m), aty = be.create_vector(n), ax_bar = be.create_vector(m);
+    auto y_arg = be.create_vector(m), y_proj = be.create_vector(m), ax = be.create_vector(m);
+    auto pax = be.create_vector(m), qx = be.create_vector(n), grad = be.create_vector(n);
+    auto px = be.create_vector(n), resid = be.create_vector(n);
+    auto c = be.create_vector(n), l = be.create_vector(n), u = be.create_vector(n);
+    auto rl = be.create_vector(m), ru = be.create_vector(m);
+    be.upload(sc.c, *c);
+    be.upload(sc.l, *l);
+    be.upload(sc.u, *u);
+    be.upload(sc.rl, *rl);
+    be.upload(sc.ru, *ru);
+    if (warm_scaled) be.upload(*warm_scaled, *x);
+    else be.fill(*x, 0.0);
+    be.project_box(*x, *l, *u);
+    be.fill(*y, 0.0);
+    Scalar tau = max_step, sigma = max_step, prev = kInf;
+    bool restart = false;
+    for (Index iter = 0; iter < options.iteration_limit; ++iter) {
+        if (timed_out(started, options)) {
+            res.status = SolveStatus::TimeLimit;
+            break;
+        }
+        be.multiply(1.0, A, *y, 0.0, *aty, Transpose::Yes);
+        be.multiply(1.0, Q, *x, 0.0, *qx, Transpose::No);
+        be.copy(*qx, *grad);
+        be.axpby(1.0, *c, 1.0, *grad);
+        be.axpby(1.0, *aty, 1.0, *grad);
+        be.copy(*x, *x_next);
+        be.axpby(-tau, *grad, 1.0, *x_next);
+        be.project_box(*x_next, *l, *u);
+        if (restart) {
+            be.copy(*x_next, *x_bar);
+            restart = false;
+        } else {
+            be.copy(*x, *x_bar);
+            be.axpby(2.0, *x_next, -1.0, *x_bar);
+        }
+        be.multiply(1.0, A, *x_bar, 0.0, *ax_bar, Transpose::No);
+        be.copy(*y, *y_arg);
+        be.axpby(sigma, *ax_bar, 1.0, *y_arg);
+        be.copy(*y_arg, *y_proj);
+        be.axpby(1.0 / sigma, *y_proj, 0.0, *y_proj);
+        be.project_box(*y_proj, *rl, *ru);
+        be.copy(*y_arg, *y);
+        be.axpby(-sigma, *y_proj, 1.0, *y);
+        be.copy(*x_next, *x);
+        res.iterations = iter + 1;
+        be.multiply(1.0, A, *x, 0.0, *ax, Transpose::No);
+        be.copy(*ax, *pax);
+        be.project_box(*pax, *rl, *ru);
+        be.axpby(1.0, *pax, -1.0, *ax);
+        const Scalar primal = be.norm_inf(*ax);
+        be.multiply(1.0, A, *y, 0.0, *aty, Transpose::Yes);
+        be.multiply(1.0, Q, *x, 0.0, *qx, Transpose::No);
+        be.copy(*qx, *grad);
+        be.axpby(1.0, *c, 1.0, *grad);
+        be.axpby(1.0, *aty, 1.0, *grad);
+        be.copy(*x, *px);
+        be.axpby(-1.0, *grad, 1.0, *px);
+        be.project_box(*px, *l, *u);
+        be.copy(*x, *resid);
+        be.axpby(-1.0, *px, 1.0, *resid);
+        const Scalar metric = std::max(primal, be.norm_inf(*resid));
+        if (iter > 0 && iter % 50 == 0) {   // pdqp.cpp:161-171
+            if (metric > prev * 1.25) {
+                tau *= 0.5;
+                sigma *= 0.5;
+                restart = true;
+            } else if (metric < prev * 0.1) {
+                tau = std::min(tau * 1.05, max_step);
+                sigma = tau;
+            }
+            prev = metric;
+        } else if (iter == 0) {
+            prev = metric;
+        }
+        if (metric <= std::max(options.primal_tolerance, options.dual_tolerance)) {
+            res.status = SolveStatus::Optimal;
+            break;
+        }
+    }
+    be.synchronize();
+    xs.resize(static_cast<std::size_t>(n));
+    ys.resize(static_cast<std::size_t>(m));
+    be.download(*x, xs);
+    be.download(*y, ys);
+}
+
+struct DeviceSolverQ {
+    vk_hip_pdhg *h = nullptr;
+    ~DeviceSolverQ() { vk_hip_pdhg_destroy(h); }
+};
+
+// The fused device iteration with Q (reference policy, FP64).
+void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, const vk_hip_csr *Q,
+                   Scalar max_step, const SolverOptions &options, const std::vector<Scalar> *warm_scaled,
+                   Clock::time_point started, std::vector<Scalar> &xs, std::vector<Scalar> &ys, SolveResult &res) {
+    vk_pdhg_params p{};
+    p.policy = VK_PDHG_POLICY_REFERENCE;
+    p.precision = VK_PREC_FP64;
+    p.iteration_limit = options.iteration_limit;
+    p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
+    p.step_size = max_step;
+    p.primal_weight = 1.0;
+    p.restart_period = options.restart_period;
+    p.use_graphs = options.use_graphs ? 1 : 0;
+    p.col_blocks = options.spmv_col_blocks;
+    p.split_passes = options.spmv_split_passes;
+    p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
+    p.norm_c = norm2(model.linear_objective);
+    DeviceSolverQ solver;
+    check_vk(vk_hip_pdqp_create(A, Q, sc.c.data(), sc.l.data(), sc.u.data(), sc.rl.data(), sc.ru.data(), sc.r.data(),
+                                sc.s.data(), warm_scaled ? warm_scaled->data() : nullptr, nullptr, &p, &solver.h),
+             "vk_hip_pdqp_create");
+    int status = VK_PDHG_RUNNING;
+    const Index chunk = std::isfinite(options.time_limit_seconds) ? 512 : std::max<Index>(1, p.iteration_limit);
+    double device_ms = 0.0;
+    vk_pdhg_state state{};
+    check_vk(vk_hip_pdhg_read_state(solver.h, &state), "vk_hip_pdhg_read_state");
+    status = state.status;
+    while (status == VK_PDHG_RUNNING) {
+        if (timed_out(started, options)) {
+            res.status = SolveStatus::TimeLimit;
+            break;
+        }
+        check_vk(vk_hip_pdhg_run(solver.h, chunk, &status), "vk_hip_pdhg_run");
+        double ms = 0.0;
+        check_vk(vk_hip_pdhg_last_run_ms(solver.h, &ms), "vk_hip_pdhg_last_run_ms");
+        device_ms += ms;
+    }
+    check_vk(vk_hip_pdhg_read_state(solver.h, &state), "vk_hip_pdhg_read_state");
+    if (state.status == VK_PDHG_CONVERGED) res.status = SolveStatus::Optimal;
+    res.iterations = state.iterations;
+    res.device_seconds = device_ms * 1e-3;
+    xs.resize(static_cast<std::size_t>(model.num_variables()));
+    ys.resize(static_cast<std::size_t>(model.num_constraints()));
+    check_vk(vk_hip_pdhg_download(solver.h, xs.data(), ys.data()), "vk_hip_pdhg_download");
+}
+
+}  // namespace
+
+SolveResult solve_qp(const Model &model, ExecutionBackend &backend, const SolverOptions &options,
+                     std::span<const Scalar> warm_start, std::span<const Scalar> variable_lower_override,
+                     std::span<const Scalar> variable_upper_override) {
+    const auto started = Clock::now();
+    Bounds bounds;
+    std::string message;
+    if (!resolve_bounds(model, variable_lower_override, variable_upper_override, bounds, message))
+        return status_result(SolveStatus::InvalidModel, message);
+    const auto errors = model.validate();
+    if (!errors.empty()) return status_result(SolveStatus::InvalidModel, errors.front());
+    if (model.problem_class() != ProblemClass::QP)
+        return status_result(SolveStatus::UnsupportedModel, "model class does not match the selected continuous solver");
+    if (model.num_variables() == 0) return status_result(SolveStatus::InvalidModel, "QP requires at least one variable");
+    if (!positive_semidefinite(model.quadratic_objective, message))
+        return status_result(SolveStatus::UnsupportedModel, message);
+    if (!valid_options(options))
+        return status_result(SolveStatus::InvalidModel, "solver limits and tolerances must be non-negative");
+    if (options.continuous_method != ContinuousMethod::Auto && options.continuous_method != ContinuousMethod::PDHG)
+        return status_result(SolveStatus::UnsupportedModel, "requested continuous method is not implemented for QP");
+    if (options.pdhg_policy != PdhgPolicy::Reference || options.pdhg_precision != PdhgPrecision::FP64 ||
+        options.relative_kkt_tolerance > 0.0)
+        return status_result(SolveStatus::UnsupportedModel,
+                             "PDQP runs the reference policy in FP64 (pdqp.cpp); the adaptive, mixed-precision "
+                             "and relative-KKT options are for LPs");
+
+    const Scaled sc = scale(model, bounds);
+    std::vector<Scalar> warm;
+    if (!warm_start.empty()) {   // pdqp.cpp:29-34
+        if (static_cast<Index>(warm_start.size()) != model.num_variables())
+            return status_result(SolveStatus::InvalidModel, "warm start must be empty or finite and sized to the model");
+        for (const Scalar v : warm_start)
+            if (!std::isfinite(v))
+                return status_result(SolveStatus::InvalidModel,
+                                     "warm start must be empty or finite and sized to the model");
+        warm.assign(warm_start.begin(), warm_start.end());
+        for (std::size_t j = 0; j < warm.size(); ++j) warm[j] /= sc.s[j];   // pdqp.cpp:77-82
+    }
+    const std::vector<Scalar> *warm_ptr = warm_start.empty() ? nullptr : &warm;
+    const ScaledQ sq = scale_q(model.quadratic_objective, sc.s);
+    const Scalar bound = operator_bound(model.constraint_matrix.pattern, sc.values);
+    const Scalar max_step = 0.99 / std::max<Scalar>(1.0, bound * bound + sq.norm_inf);   // pdqp.cpp:98
+
+    auto matrix = backend.create_matrix(model.constraint_matrix.pattern);
+    auto quadratic = backend.create_matrix(model.quadratic_objective.pattern);
+    backend.set_values(sc.values, *matrix);
+    backend.set_values(sq.values, *quadratic);
+    const vk_hip_csr *a_handle = hip_matrix_handle(*matrix), *q_handle = hip_matrix_handle(*quadratic);
+    SolveResult res;
+    res.solver = "PDQP";
+    res.backend = "provided";
+    std::vector<Scalar> xs, ys;
+    if (a_handle && q_handle && options.fused_iterations)
+        iterate_fused(model, sc, a_handle, q_handle, max_step, options, warm_ptr, started, xs, ys, res);
+    else
+        iterate_ops(model, sc, backend, *matrix, *quadratic, max_step, options, warm_ptr, started, xs, ys, res);
+    std::vector<Scalar> x(xs.size()), y(ys.size());
+    for (std::size_t j = 0; j < xs.size(); ++j) x[j] = xs[j] * sc.s[j];
+    for (std::size_t i = 0; i < ys.size(); ++i) y[i] = ys[i] * sc.r[i];
+    postsolve_qp(model, bounds, options, x, y, res);
+    res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
+    return res;
+}
+
+}  // namespace vikalp::solver

@@ -27,9 +27,10 @@ SolveResult solve(const Model &model, const SolverOptions &options) {
         result.message = "this library carries the HIP backend only; CPU and CUDA are served by the reference";
         return result;
     }
-    if (model.problem_class() != ProblemClass::LP) {
+    const ProblemClass pc = model.problem_class();
+    if (pc != ProblemClass::LP && pc != ProblemClass::QP) {
         result.status = SolveStatus::UnsupportedModel;
-        result.message = "the HIP PDHG path solves LP models only";
+        result.message = "the HIP path solves LP (PDHG) and QP (PDQP) models only";
         return result;
     }
     try {
@@ -37,7 +38,9 @@ SolveResult solve(const Model &model, const SolverOptions &options) {
         Model relaxation = model;
         std::fill(relaxation.variable_types.begin(), relaxation.variable_types.end(),
                   VariableType::Continuous);
-        result = solver::solve_lp(relaxation, *backend, options);
+        // Solver.cpp:59-69: LP -> solve_lp, QP -> solve_qp.
+        result = pc == ProblemClass::LP ? solver::solve_lp(relaxation, *backend, options)
+                                        : solver::solve_qp(relaxation, *backend, options);
         result.backend = "hip";
     } catch (const std::exception &error) {
         result = SolveResult{};

@@ -139,6 +139,8 @@ HIP_SYMBOLS = {
     "vk_hip_csr_norm2_estimate": (C.c_int, [_vp, C.c_int, _dp]),
     "vk_hip_pdhg_create": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                      C.POINTER(PdhgParams), C.POINTER(_vp)]),
+    "vk_hip_pdqp_create": (C.c_int, [_vp, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     C.POINTER(PdhgParams), C.POINTER(_vp)]),
     "vk_hip_pdhg_run": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int)]),
     "vk_hip_pdhg_read_state": (C.c_int, [_vp, C.POINTER(PdhgState)]),
     "vk_hip_pdhg_download": (C.c_int, [_vp, _dp, _dp]),
@@ -179,6 +181,9 @@ HOST_SYMBOLS = {
     "vk_lp_free": (None, [_vp]),
     "vk_lp_block_periods": (C.c_int64, [C.c_uint64, C.c_int64]),
     "vk_solve_last_error": (C.c_char_p, []),
+    "vk_qp_solve": (C.c_int, [C.c_int, C.c_int64, C.c_int64, _lp, _lp, _dp, _lp, _lp, _dp, _dp, C.c_double,
+                              _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(_SolverOptions),
+                              _dp, _dp, C.POINTER(_SolveInfo)]),
     "vk_unrolled_create": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
                                      C.c_int64, C.c_int64, C.POINTER(_vp)]),
     "vk_unrolled_forward": (C.c_int, [_vp, _dp, _dp]),
@@ -351,7 +356,7 @@ class Result:
 
 
 def _solve(mode, lp: LP, options: Options, warm_start=None, lower_override=None,
-           upper_override=None) -> Result:
+           upper_override=None, quadratic=None) -> Result:
     L = host_lib()
     keep = [np.ascontiguousarray(lp.row_offsets, dtype=np.int64),
             np.ascontiguousarray(lp.column_indices, dtype=np.int64)]
@@ -363,12 +368,21 @@ def _solve(mode, lp: LP, options: Options, warm_start=None, lower_override=None,
     y = np.zeros(lp.m)
     info = _SolveInfo()
     copt = options._c()
-    rc = L.vk_lp_solve(mode, lp.m, lp.n, _ptr(keep[0], _lp), _ptr(keep[1], _lp), _ptr(f[0]),
-                       _ptr(f[1]), float(lp.objective_offset), _ptr(f[2]), _ptr(f[3]), _ptr(f[4]),
-                       _ptr(f[5]), _ptr(opt[0]), _ptr(opt[1]), _ptr(opt[2]), C.byref(copt),
-                       _ptr(x), _ptr(y), C.byref(info))
+    if quadratic is None:
+        rc = L.vk_lp_solve(mode, lp.m, lp.n, _ptr(keep[0], _lp), _ptr(keep[1], _lp), _ptr(f[0]),
+                           _ptr(f[1]), float(lp.objective_offset), _ptr(f[2]), _ptr(f[3]), _ptr(f[4]),
+                           _ptr(f[5]), _ptr(opt[0]), _ptr(opt[1]), _ptr(opt[2]), C.byref(copt),
+                           _ptr(x), _ptr(y), C.byref(info))
+    else:
+        q = [np.ascontiguousarray(quadratic[0], dtype=np.int64), np.ascontiguousarray(quadratic[1], dtype=np.int64),
+             np.ascontiguousarray(quadratic[2], dtype=np.float64)]
+        rc = L.vk_qp_solve(mode, lp.m, lp.n, _ptr(keep[0], _lp), _ptr(keep[1], _lp), _ptr(f[0]),
+                           _ptr(q[0], _lp), _ptr(q[1], _lp), _ptr(q[2]),
+                           _ptr(f[1]), float(lp.objective_offset), _ptr(f[2]), _ptr(f[3]), _ptr(f[4]),
+                           _ptr(f[5]), _ptr(opt[0]), _ptr(opt[1]), _ptr(opt[2]), C.byref(copt),
+                           _ptr(x), _ptr(y), C.byref(info))
     if rc != 0:
-        raise ValueError(f"vk_lp_solve rejected its arguments ({rc})")
+        raise ValueError(f"vk_lp_solve / vk_qp_solve rejected its arguments ({rc})")
     return Result(STATUS_NAMES[info.status], info.iterations, info.objective_value,
                   info.primal_residual, info.stationarity_residual, info.complementarity_residual,
                   info.absolute_gap, info.relative_gap, info.solve_seconds, info.device_seconds,
@@ -436,9 +450,17 @@ def solve_batch(lps: list, options: Options | None = None) -> list:
     return [_result(infos[i], x[cs[i]:cs[i + 1]].copy(), y[rs[i]:rs[i + 1]].copy()) for i in range(count)]
 
 
-def solve(lp: LP, options: Options | None = None) -> Result:
-    """vikalp::solve(model, options) (Solver.cpp:92-99) on the HIP library."""
-    return _solve(0, lp, options or Options())
+def solve(lp: LP, options: Options | None = None, quadratic=None) -> Result:
+    """vikalp::solve(model, options) (Solver.cpp:92-99) on the HIP library;
+    quadratic = (row_offsets, column_indices, values) of Q makes it a QP."""
+    return _solve(0, lp, options or Options(), quadratic=quadratic)
+
+
+def solve_qp(lp: LP, quadratic, options: Options | None = None, warm_start=None, lower_override=None,
+             upper_override=None) -> Result:
+    """vikalp::solver::solve_qp(model, hip_backend, options, warm, lo, hi) (pdqp.cpp:303-331);
+    quadratic = (row_offsets, column_indices, values) of the symmetric n x n Q."""
+    return _solve(1, lp, options or Options(), warm_start, lower_override, upper_override, quadratic)
 
 
 def solve_lp(lp: LP, options: Options | None = None, warm_start=None, lower_override=None,
