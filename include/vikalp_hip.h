@@ -152,6 +152,12 @@ typedef struct {
                                       1 = fused SpMV + epilogue kernels;
                                       2 = SpMV-only kernels, then row and
                                       column passes (same results bitwise) */
+    int cache_budget_mb;           /* MiB of matrix data (A x operand first,
+                                      then A') loaded so that the 256 MB MALL
+                                      keeps it across iterations; the rest is
+                                      streamed.  0 = auto (256 MB less the
+                                      gathered x and y less 4 MiB), -1 = none.
+                                      Load hints only: results do not change */
 } vk_pdhg_params;
 
 typedef struct {

@@ -44,7 +44,7 @@ typedef struct {
     int spmv_col_blocks;            /* 0 auto, 1 plain tiles, 2..16 column blocks */
     int spmv_autotune;              /* time the layouts and use the fastest */
     int spmv_split_passes;          /* 0 auto, 1 fused, 2 split SpMV + passes */
-    int reserved2;
+    int spmv_cache_budget_mb;       /* MALL-resident matrix MiB: 0 auto, -1 none */
 } vk_solver_options;
 
 /* Fills `o` with the defaults of vikalp::SolverOptions. */

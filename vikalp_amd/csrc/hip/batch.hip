@@ -331,10 +331,10 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
         VK_HIP_CHECK(hipStreamSynchronize(h->stream));
     }
     (void)hipFree(stage);
-    a.A = CsrView<T>{h->rp, h->ci, val_a, h->tiles_a, static_cast<int>(m), h->ntiles_a};
-    a.AT = CsrView<T>{h->trp, h->tci, val_at, h->tiles_at, static_cast<int>(n), h->ntiles_at};
-    a.A64 = CsrView<double>{h->rp, h->ci, h->val, h->tiles_a, static_cast<int>(m), h->ntiles_a};
-    a.AT64 = CsrView<double>{h->trp, h->tci, h->tval, h->tiles_at, static_cast<int>(n), h->ntiles_at};
+    a.A = CsrView<T>{h->rp, h->ci, val_a, h->tiles_a, static_cast<int>(m), h->ntiles_a, 0};
+    a.AT = CsrView<T>{h->trp, h->tci, val_at, h->tiles_at, static_cast<int>(n), h->ntiles_at, 0};
+    a.A64 = CsrView<double>{h->rp, h->ci, h->val, h->tiles_a, static_cast<int>(m), h->ntiles_a, 0};
+    a.AT64 = CsrView<double>{h->trp, h->tci, h->tval, h->tiles_at, static_cast<int>(n), h->ntiles_at, 0};
     a.inst = h->inst;
     a.states = h->states;
     a.pw_rows = h->alloc<double>(m);

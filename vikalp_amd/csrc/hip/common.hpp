@@ -93,6 +93,15 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Device-side view of one CSR operand (A or its stored transpose).
+//
+// `resident` splits the tiles by how their column indices, values and row
+// offsets are loaded: tiles [0, resident) with ordinary loads, which the
+// 256 MB memory-side cache (MALL) keeps, the rest with non-temporal loads,
+// which it does not allocate.  An iteration streams A and A' once each; when
+// both together exceed the MALL, ordinary loads evict each other every
+// iteration and nothing is reused, while marking only what fits keeps that
+// part in the MALL from one iteration to the next.  The split changes load
+// hints only, never results.
 template <class V>
 struct CsrView {
     const int *row_ptr;     // rows+1
@@ -101,7 +110,29 @@ struct CsrView {
     const int2 *tiles;      // ntiles+1: {first row, first nonzero} of each tile
     int rows;
     int ntiles;
+    int resident;           // tiles [0, resident) use cache-allocating loads
 };
+
+// Loads of matrix data: cache-allocating (Keep) or non-temporal.
+template <bool Keep, class P>
+__device__ __forceinline__ P matrix_load(const P *p) {
+    if constexpr (Keep) return *p;
+    else return __builtin_nontemporal_load(p);
+}
+
+// Loads and stores of vectors that the next kernel does not gather:
+// non-temporal in the split row / column passes (Stream), so the matrix
+// data marked resident stays in the MALL.
+template <bool Stream, class P>
+__device__ __forceinline__ P vec_load(const P *p) {
+    if constexpr (Stream) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool Stream, class P>
+__device__ __forceinline__ void vec_store(P *p, P v) {
+    if constexpr (Stream) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 // Column-blocked copy of a CSR operand, for when the gathered vector
 // outgrows one XCD's L2.  Block p holds the entries of columns
@@ -123,10 +154,12 @@ struct BlockedView {
     int rows;             //   the last one its end sentinel
     int nb;
     int tile_first[kMaxColBlocks + 1];
+    int resident;         // global tile index below which loads allocate in the MALL
 
     __device__ CsrView<V> block(int p) const {
-        return CsrView<V>{row_ptr + static_cast<long>(p) * (rows + 1), col, val, tiles + tile_first[p], rows,
-                          tile_first[p + 1] - tile_first[p] - 1};
+        const int nt = tile_first[p + 1] - tile_first[p] - 1;
+        return CsrView<V>{row_ptr + static_cast<long>(p) * (rows + 1), col, val, tiles + tile_first[p], rows, nt,
+                          max(0, min(nt, resident - tile_first[p]))};
     }
 };
 
@@ -264,6 +297,21 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ parti
 // Epi provides `Pre prefetch(int row) const` (loads issued before the
 // gathers) and `row(int row, double acc, const Pre &, Partials &)`; every row
 // is seen exactly once.
+// Phase 1's matrix loads for one tile of at most kTileNnz nonzeros.
+template <bool Keep, class V>
+__device__ __forceinline__ void load_tile(const int *__restrict__ colp, const V *__restrict__ valp,
+                                          const int *__restrict__ row_ptr, int k0, int nnz, int r0, int nr,
+                                          int lane, int (&c)[kTileNnz / kWave], V (&v)[kTileNnz / kWave],
+                                          int &rs) {
+#pragma unroll
+    for (int u = 0; u < kTileNnz / kWave; ++u) {
+        const int k = lane + u * kWave;
+        c[u] = k < nnz ? matrix_load<Keep>(colp + k0 + k) : 0;
+        v[u] = k < nnz ? matrix_load<Keep>(valp + k0 + k) : V(0);
+    }
+    rs = lane < nr ? matrix_load<Keep>(row_ptr + r0 + lane) - k0 : nnz;
+}
+
 template <class V, class X, class Epi>
 __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__restrict__ x, Epi &epi,
                                                 Partials &part, double *lds_generic, int t_begin,
@@ -280,17 +328,13 @@ __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__
         const int r0 = head.x, r1 = tail.x, k0 = head.y, k1 = tail.y;
         const int nnz = k1 - k0;
         const int nr = r1 - r0;
+        const bool keep = t < A.resident;
         if (nnz <= kTileNnz) {
             int c[kPer];
             V v[kPer];
-#pragma unroll
-            for (int u = 0; u < kPer; ++u) {
-                const int k = lane + u * kWave;
-                c[u] = k < nnz ? __builtin_nontemporal_load(colp + k0 + k) : 0;
-                v[u] = k < nnz ? __builtin_nontemporal_load(valp + k0 + k) : V(0);
-            }
-            // Row starts of the tile, one per lane (lane r holds row r's start).
-            const int rs = lane < nr ? __builtin_nontemporal_load(A.row_ptr + r0 + lane) - k0 : nnz;
+            int rs;   // row starts of the tile, one per lane (lane r holds row r's start)
+            if (keep) load_tile<true>(colp, valp, A.row_ptr, k0, nnz, r0, nr, lane, c, v, rs);
+            else load_tile<false>(colp, valp, A.row_ptr, k0, nnz, r0, nr, lane, c, v, rs);
             Pre pre{};
             if (lane < nr) pre = epi.prefetch(r0 + lane);
             X xv[kPer];
@@ -320,9 +364,13 @@ __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__
             Pre pre{};
             if (lane == 0) pre = epi.prefetch(r0);
             double acc = 0.0;
-            for (int k = k0 + lane; k < k1; k += kWave)
-                acc += static_cast<double>(__builtin_nontemporal_load(valp + k)) *
-                       static_cast<double>(x[__builtin_nontemporal_load(colp + k)]);
+            if (keep)
+                for (int k = k0 + lane; k < k1; k += kWave)
+                    acc += static_cast<double>(valp[k]) * static_cast<double>(x[colp[k]]);
+            else
+                for (int k = k0 + lane; k < k1; k += kWave)
+                    acc += static_cast<double>(__builtin_nontemporal_load(valp + k)) *
+                           static_cast<double>(x[__builtin_nontemporal_load(colp + k)]);
             acc = wave_sum(acc);
             if (lane == 0) epi.row(r0, acc, pre, part);
         }

@@ -61,12 +61,14 @@ namespace {
 
 // Plain SpMV y = A x rounded to T (setup: A x0 and A' y0; the split passes
 // store FP64 sums with T = double).
-template <class T>
+template <class T, bool Stream = false>
 struct StoreEpi {
     T *y;
     struct Pre {};
     __device__ Pre prefetch(int) const { return Pre{}; }
-    __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = static_cast<T>(acc); }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const {
+        vec_store<Stream>(y + r, static_cast<T>(acc));
+    }
 };
 
 template <class T>
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_spmv(Bufs<T> b, CsrView<T> A, d
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    StoreEpi<double> epi{part};
+    StoreEpi<double, true> epi{part};
     Partials unused;
     unused.zero();
     tile_spmv(A, b.x[(s.k + 1) & 1], epi, unused, lds[threadIdx.x / kWave]);
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_primal_spmv(Bufs<T> b, CsrView<T> AT
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    StoreEpi<double> epi{b.atyd};
+    StoreEpi<double, true> epi{b.atyd};
     Partials unused;
     unused.zero();
     tile_spmv(AT, b.y, epi, unused, lds[threadIdx.x / kWave]);
@@ -132,12 +134,12 @@ __global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b) {
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
     const bool fuse = may_fuse(s);
-    PrimalEpi<T> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse, sizeof(T) != sizeof(double)};
+    PrimalEpi<T, true> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse, sizeof(T) != sizeof(double)};
     Partials part;
     part.zero();
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
         const auto pre = epi.prefetch(j);
-        epi.row(j, b.atyd[j], pre, part);
+        epi.row(j, __builtin_nontemporal_load(b.atyd + j), pre, part);
     }
     store_partials(part, b.part_cols, scratch);
 }
@@ -164,7 +166,7 @@ struct PartialEpi {
     double *out;
     struct Pre {};
     __device__ Pre prefetch(int) const { return Pre{}; }
-    __device__ void row(int r, double acc, const Pre &, Partials &) const { out[r] = acc; }
+    __device__ void row(int r, double acc, const Pre &, Partials &) const { __builtin_nontemporal_store(acc, out + r); }
 };
 
 template <class V, class X>
@@ -197,14 +199,14 @@ __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *_
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
-    DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
-                   s.policy == VK_PDHG_POLICY_ADAPTIVE};
+    DualEpi<T, true> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
+                         s.policy == VK_PDHG_POLICY_ADAPTIVE};
     Partials acc_part;
     acc_part.zero();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
         const auto pre = epi.prefetch(i);
         double acc = 0.0;
-        for (int p = 0; p < nb; ++p) acc += part[static_cast<long>(p) * b.m + i];
+        for (int p = 0; p < nb; ++p) acc += __builtin_nontemporal_load(part + static_cast<long>(p) * b.m + i);
         epi.row(i, acc, pre, acc_part);
     }
     store_partials(acc_part, b.part_rows, scratch);
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows_blocked(Bufs<T> b, const d
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
         const auto pre = epi.prefetch(i);
         double acc = 0.0;
-        for (int p = 0; p < nb; ++p) acc += part[static_cast<long>(p) * b.m + i];
+        for (int p = 0; p < nb; ++p) acc += __builtin_nontemporal_load(part + static_cast<long>(p) * b.m + i);
         epi.row(i, acc, pre, acc_part);
     }
     store_partials(acc_part, b.part_erows, scratch);
@@ -455,6 +457,36 @@ int auto_col_blocks(long long n, int bytes, long long m) {
 // epilogues' register pressure at that size).
 bool auto_split(long long m, long long n) { return std::max(m, n) > (1ll << 18); }
 
+// MALL budget for matrix data when the caller asks for automatic choice:
+// the MALL's 256 MB less the gathered x and y (they are loaded with ordinary
+// loads too) and a 4 MiB margin for the remaining ordinary traffic.  On the
+// 400,000 x 1,000,000 LP this lands on the measured optimum of both
+// precisions (FP64 iteration 334 -> 298 us, FP32 315 -> 266 us); a budget
+// that overfills the MALL thrashes and loses most of the gain.
+constexpr long long kMallBytes = 256'000'000;
+constexpr long long kMallMargin = 4ll << 20;
+
+long long cache_budget_bytes(int mib, long long m, long long n, int value_bytes) {
+    if (mib < 0) return 0;
+    if (mib > 0) return static_cast<long long>(mib) << 20;
+    return std::max(0ll, kMallBytes - kMallMargin - (m + n) * value_bytes);
+}
+
+// Tiles [0, resident) of an operand of `bytes` bytes over `entries` tile
+// entries that fit in `left` bytes of budget (tiles hold near-equal nonzero
+// counts, so a prefix of tiles is that fraction of the bytes); takes the
+// bytes used from `left`.
+int resident_prefix(long long entries, long long bytes, long long &left) {
+    if (bytes <= 0 || left <= 0) return 0;
+    if (bytes <= left) {
+        left -= bytes;
+        return static_cast<int>(entries);
+    }
+    const long long keep = entries * left / bytes;
+    left = 0;
+    return static_cast<int>(keep);
+}
+
 }  // namespace
 }  // namespace vk
 
@@ -491,6 +523,8 @@ struct vk_hip_pdhg {
     BlockedView<double> blk64{};
     BlockedView<float> blk32{};
     double *blk_part = nullptr;
+    // MALL residency (CsrView::resident) of the A x and A' y operands.
+    int resident_a = 0, resident_at = 0;
 
     template <class T>
     T *alloc(size_t count) {
@@ -505,16 +539,23 @@ namespace {
 
 template <class T>
 const CsrView<T> view_a(const vk_hip_pdhg *s);
+template <class V>
+CsrView<V> with_resident(CsrView<V> v, int resident) {
+    v.resident = resident;
+    return v;
+}
 template <>
-const CsrView<double> view_a<double>(const vk_hip_pdhg *s) { return s->A->a.view(); }
+const CsrView<double> view_a<double>(const vk_hip_pdhg *s) { return with_resident(s->A->a.view(), s->resident_a); }
 template <>
-const CsrView<float> view_a<float>(const vk_hip_pdhg *s) { return s->A->a32.view(); }
+const CsrView<float> view_a<float>(const vk_hip_pdhg *s) { return with_resident(s->A->a32.view(), s->resident_a); }
 template <class T>
 const CsrView<T> view_at(const vk_hip_pdhg *s);
 template <>
-const CsrView<double> view_at<double>(const vk_hip_pdhg *s) { return s->A->at.view(); }
+const CsrView<double> view_at<double>(const vk_hip_pdhg *s) { return with_resident(s->A->at.view(), s->resident_at); }
 template <>
-const CsrView<float> view_at<float>(const vk_hip_pdhg *s) { return s->A->at32.view(); }
+const CsrView<float> view_at<float>(const vk_hip_pdhg *s) {
+    return with_resident(s->A->at32.view(), s->resident_at);
+}
 
 template <class T>
 BlockedView<T> &blocked(vk_hip_pdhg *s);
@@ -604,6 +645,29 @@ void enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
         launch_primal<T>(s, s->stream);
         launch_control<T>(s, k, s->stream);
     }
+}
+
+// Splits the MALL budget (vk_pdhg_params.cache_budget_mb) between the two
+// operands of an iteration: the A x operand (column-blocked or plain) first,
+// then A'.  Bytes per operand: column indices and values of every nonzero,
+// the row offsets (per column block) and the tile table.
+template <class T>
+void plan_residency(vk_hip_pdhg *s) {
+    long long left = cache_budget_bytes(s->params.cache_budget_mb, s->m, s->n, static_cast<int>(sizeof(T)));
+    const long long nnz = s->A->nnz;
+    const long long per_nz = static_cast<long long>(sizeof(T)) + 4;
+    s->resident_a = s->resident_at = 0;
+    if (s->col_blocks > 1) {
+        BlockedView<T> &B = blocked<T>(s);
+        const long long entries = B.tile_first[B.nb];
+        const long long bytes = nnz * per_nz + static_cast<long long>(B.nb) * (s->m + 1) * 4 + entries * 8;
+        B.resident = resident_prefix(entries, bytes, left);
+    } else {
+        const CsrView<T> A = view_a<T>(s);
+        s->resident_a = resident_prefix(A.ntiles, nnz * per_nz + (s->m + 1ll) * 4 + (A.ntiles + 1ll) * 8, left);
+    }
+    const CsrView<T> AT = view_at<T>(s);
+    s->resident_at = resident_prefix(AT.ntiles, nnz * per_nz + (s->n + 1ll) * 4 + (AT.ntiles + 1ll) * 8, left);
 }
 
 // Builds the column-blocked copy of A in the storage type T: the pattern is
@@ -745,6 +809,7 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
             return VK_ENOMEM;
         }
     }
+    plan_residency<T>(s);
 
     // pdhg.cpp:71 project the start; A x0, A' y0.
     k_project_start<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.l, b.u);
@@ -1130,6 +1195,7 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
                "vk_hip_pdhg_create: col_blocks must be in [0, 16]");
     VK_REQUIRE(params->split_passes >= 0 && params->split_passes <= 2,
                "vk_hip_pdhg_create: split_passes must be 0, 1 or 2");
+    VK_REQUIRE(params->cache_budget_mb >= -1, "vk_hip_pdhg_create: cache_budget_mb must be >= -1");
     if (int rc = ensure_init()) return rc;
     auto *s = new (std::nothrow) vk_hip_pdhg();
     if (!s) return VK_ENOMEM;

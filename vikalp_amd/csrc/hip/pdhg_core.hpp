@@ -69,8 +69,9 @@ __device__ __forceinline__ double kkt_weighted(double w, double pr2, double dr2,
 
 // ------------------------------------------------------------------ epilogues
 
-// Dual step on row i given acc = (A x+)_i.
-template <class T>
+// Dual step on row i given acc = (A x+)_i.  Stream: non-temporal loads and
+// stores for every vector but y, which the next kernel gathers.
+template <class T, bool Stream = false>
 struct DualEpi {
     const Bufs<T> &b;
     double sigma, inv_sigma;
@@ -82,13 +83,13 @@ struct DualEpi {
     };
     __device__ Pre prefetch(int i) const {
         Pre p;
-        p.ax = b.ax[i];
-        p.y = b.y[i];
-        p.lo = b.rl[i];
-        p.hi = b.ru[i];
-        p.unscale = b.row_unscale[i];
-        p.ysum = b.ysum ? b.ysum[i] : 0.0;
-        p.ylast = b.ylast ? b.ylast[i] : T(0);
+        p.ax = vec_load<Stream>(b.ax + i);
+        p.y = vec_load<Stream>(b.y + i);
+        p.lo = vec_load<Stream>(b.rl + i);
+        p.hi = vec_load<Stream>(b.ru + i);
+        p.unscale = vec_load<Stream>(b.row_unscale + i);
+        p.ysum = b.ysum ? vec_load<Stream>(b.ysum + i) : 0.0;
+        p.ylast = b.ylast ? vec_load<Stream>(b.ylast + i) : T(0);
         return p;
     }
     __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
@@ -111,8 +112,8 @@ struct DualEpi {
         }
         const T y_stored = static_cast<T>(yn);
         b.y[i] = y_stored;
-        b.ax[i] = static_cast<T>(axn);
-        if (b.ysum) b.ysum[i] = q.ysum + static_cast<double>(y_stored);
+        vec_store<Stream>(b.ax + i, static_cast<T>(axn));
+        if (b.ysum) vec_store<Stream>(b.ysum + i, q.ysum + static_cast<double>(y_stored));
         const double d = clamp_box(axn, lo, hi) - axn;
         p.v[0] = dmax(p.v[0], fabs(d));
         const double du = d * static_cast<double>(q.unscale);
@@ -152,7 +153,8 @@ __device__ __forceinline__ void column_terms(double xj, double cj, double g, dou
 }
 
 // A' y on column j, the residual terms, and (fused) the next primal step.
-template <class T>
+// Stream as for DualEpi; x+ (gathered next) keeps ordinary stores.
+template <class T, bool Stream = false>
 struct PrimalEpi {
     const Bufs<T> &b;
     const T *xcur;
@@ -166,18 +168,18 @@ struct PrimalEpi {
     };
     __device__ Pre prefetch(int j) const {
         Pre p;
-        p.qx = b.qx ? b.qx[j] : 0.0;
-        p.x = xcur[j];
-        p.c = b.c[j];
-        p.l = b.l[j];
-        p.u = b.u[j];
-        p.s = b.col_scale[j];
-        p.xsum = (fuse && b.xsum) ? b.xsum[j] : 0.0;
-        p.xlast = b.xlast ? b.xlast[j] : T(0);
+        p.qx = b.qx ? vec_load<Stream>(b.qx + j) : 0.0;
+        p.x = vec_load<Stream>(xcur + j);
+        p.c = vec_load<Stream>(b.c + j);
+        p.l = vec_load<Stream>(b.l + j);
+        p.u = vec_load<Stream>(b.u + j);
+        p.s = vec_load<Stream>(b.col_scale + j);
+        p.xsum = (fuse && b.xsum) ? vec_load<Stream>(b.xsum + j) : 0.0;
+        p.xlast = b.xlast ? vec_load<Stream>(b.xlast + j) : T(0);
         return p;
     }
     __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
-        if (store_aty) b.aty[j] = static_cast<T>(acc);
+        if (store_aty) vec_store<Stream>(b.aty + j, static_cast<T>(acc));
         const double xj = static_cast<double>(q.x);
         const double cj = static_cast<double>(q.c);
         const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
@@ -190,7 +192,7 @@ struct PrimalEpi {
             const T xn = static_cast<T>(
                 clamp_box(b.qx ? -tau * g + xj : (xj - tau * cj) - tau * acc, lj, uj));
             xnext[j] = xn;
-            if (b.xsum) b.xsum[j] = q.xsum + static_cast<double>(xn);
+            if (b.xsum) vec_store<Stream>(b.xsum + j, q.xsum + static_cast<double>(xn));
         }
     }
 };

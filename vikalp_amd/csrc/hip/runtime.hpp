@@ -29,7 +29,9 @@ struct DeviceCsr {
     int2 *tiles = nullptr;   // {first row, first nonzero}, ntiles+1 entries
     int rows = 0;
     int ntiles = 0;
-    CsrView<V> view() const { return CsrView<V>{row_ptr, col, val, tiles, rows, ntiles}; }
+    // All tiles streamed (non-temporal); a solver that budgets the MALL sets
+    // `resident` on its own copy of the view.
+    CsrView<V> view() const { return CsrView<V>{row_ptr, col, val, tiles, rows, ntiles, 0}; }
 };
 
 // Splits rows into tiles of whole rows with at most kTileNnz nonzeros and

@@ -43,6 +43,7 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.spmv_col_blocks = o.spmv_col_blocks;
     s.spmv_autotune = o.spmv_autotune != 0;
     s.spmv_split_passes = o.spmv_split_passes;
+    s.spmv_cache_budget_mb = o.spmv_cache_budget_mb;
     return s;
 }
 
@@ -107,6 +108,7 @@ void vk_solver_options_default(vk_solver_options *o) {
     o->spmv_col_blocks = d.spmv_col_blocks;
     o->spmv_autotune = d.spmv_autotune ? 1 : 0;
     o->spmv_split_passes = d.spmv_split_passes;
+    o->spmv_cache_budget_mb = d.spmv_cache_budget_mb;
 }
 
 static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,

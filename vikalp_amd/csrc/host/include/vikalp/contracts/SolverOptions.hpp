@@ -62,6 +62,10 @@ struct SolverOptions {
     // SpMV and epilogue kernels: 0 = automatic, 1 = fused, 2 = split into a
     // SpMV-only kernel and a row / column pass (identical results).
     int spmv_split_passes = 0;
+    // MiB of matrix data kept in the 256 MB memory-side cache across
+    // iterations (the rest is streamed): 0 = automatic, -1 = none.  Changes
+    // load hints only, never results.
+    int spmv_cache_budget_mb = 0;
 };
 
 }  // namespace vikalp

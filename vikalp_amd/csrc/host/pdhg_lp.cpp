@@ -47,7 +47,7 @@ bool valid_options(const SolverOptions &o) {
            std::isfinite(o.relative_kkt_tolerance) && o.relative_kkt_tolerance >= 0.0 &&
            o.restart_period > 0 && std::isfinite(o.mixed_phase_tolerance) &&
            o.mixed_phase_tolerance >= 0.0 && o.spmv_col_blocks >= 0 && o.spmv_col_blocks <= 16 &&
-           o.spmv_split_passes >= 0 && o.spmv_split_passes <= 2;
+           o.spmv_split_passes >= 0 && o.spmv_split_passes <= 2 && o.spmv_cache_budget_mb >= -1;
 }
 
 bool timed_out(Clock::time_point started, const SolverOptions &o) {
@@ -446,6 +446,7 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     p.restart_period = options.restart_period;
     p.use_graphs = options.use_graphs ? 1 : 0;
     p.split_passes = options.spmv_split_passes;
+    p.cache_budget_mb = options.spmv_cache_budget_mb;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     if (p.policy == VK_PDHG_POLICY_REFERENCE) {

@@ -326,6 +326,7 @@ void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, co
     p.use_graphs = options.use_graphs ? 1 : 0;
     p.col_blocks = options.spmv_col_blocks;
     p.split_passes = options.spmv_split_passes;
+    p.cache_budget_mb = options.spmv_cache_budget_mb;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     DeviceSolverQ solver;

@@ -73,7 +73,7 @@ class _SolverOptions(C.Structure):
         ("fused_iterations", C.c_int), ("use_graphs", C.c_int), ("reserved", C.c_int),
         ("relative_kkt_tolerance", C.c_double), ("mixed_phase_tolerance", C.c_double),
         ("restart_period", C.c_int64), ("spmv_col_blocks", C.c_int), ("spmv_autotune", C.c_int),
-        ("spmv_split_passes", C.c_int), ("reserved2", C.c_int),
+        ("spmv_split_passes", C.c_int), ("spmv_cache_budget_mb", C.c_int),
     ]
 
 
@@ -99,6 +99,7 @@ class PdhgParams(C.Structure):
         ("step_size", C.c_double), ("primal_weight", C.c_double), ("restart_period", C.c_int64),
         ("norm_b", C.c_double), ("norm_c", C.c_double), ("use_graphs", C.c_int),
         ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
+        ("cache_budget_mb", C.c_int),
     ]
 
 
@@ -318,6 +319,7 @@ class Options:
     spmv_col_blocks: int = 0
     spmv_autotune: bool = False
     spmv_split_passes: int = 0
+    spmv_cache_budget_mb: int = 0
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
