@@ -60,7 +60,9 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
     Column pass (primal): FP64 sums 8 n, per column read x, c, l, u, s,
     x_last, write aty, x_next (8 s_v n), FP64 x_sum (16 n).
     A fused kernel counts its SpMV without the sum store plus its pass
-    without the sum load.
+    without the sum load; k_primal_own (split A' side: SpMV, then the
+    column pass over the workgroup's own rows, one launch) stores and
+    reloads its sums, so it counts both.
     """
     spmv_a = (sv + 4) * nnz + 4 * (m + 1) + sv * n + 8 * m
     spmv_at = (sv + 4) * nnz + 4 * (n + 1) + sv * m + 8 * n
@@ -73,8 +75,7 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
     else:
         rows.append(("k_dual", prof["dual_spmv_ms"], spmv_a - 8 * m + pass_rows - 8 * m))
     if prof["split_passes"]:
-        rows.append(("k_primal_spmv", prof["primal_spmv_ms"], spmv_at))
-        rows.append(("k_primal_cols", prof["primal_pass_ms"], pass_cols))
+        rows.append(("k_primal_own", prof["primal_spmv_ms"], spmv_at + pass_cols))
     else:
         rows.append(("k_primal", prof["primal_spmv_ms"], spmv_at - 8 * n + pass_cols - 8 * n))
     return {f"{name}<{suffix}>": {"ms": ms, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9}
@@ -84,8 +85,8 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
 # HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, scaled by a 256 MiB
 # device copy) measured with `bench.py --counters` under rocprofv3 --pmc;
 # profiles/pdhg_traffic_zipf_400k_x_1m.md holds the passes and commands.
-MEASURED_TRAFFIC = {"k_blocks<double>": 284.7e6, "k_dual_rows<double>": 45.3e6,
-                    "k_primal_spmv<double>": 292.2e6, "k_primal_cols<double>": 80.6e6}
+MEASURED_TRAFFIC = {"k_blocks<double>": 256.6e6, "k_dual_rows<double>": 45.3e6,
+                    "k_primal_own<double>": 359.8e6}
 
 
 def counters_driver(V, args):

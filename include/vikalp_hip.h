@@ -208,9 +208,9 @@ int vk_hip_pdhg_last_run_ms(const vk_hip_pdhg *solver, double *ms);
 /* Runs `iterations` more iterations launched one by one, with HIP events on
  * the solver's stream between the kernels.  ms[0] = mean duration of the
  * dual SpMV kernel (the fused dual kernel when not split), ms[1] = the dual
- * row pass (0 when fused), ms[2] = the primal SpMV kernel (or fused primal
- * kernel), ms[3] = the primal column pass (0 when fused), ms[4] = mean
- * device time per whole iteration.  For measurement. */
+ * row pass (0 when fused), ms[2] = the primal kernel (A' y with its column
+ * pass: one kernel in every layout), ms[3] = 0, ms[4] = mean device time
+ * per whole iteration.  For measurement. */
 int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *ms);
 /* The A x layout the solver chose: column blocks (1 = plain row tiles) and
  * whether the SpMVs run split from their row / column passes (0 or 1). */

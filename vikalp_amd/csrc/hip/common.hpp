@@ -280,6 +280,49 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ parti
     __syncthreads();
 }
 
+// reduce_partials for two partial sets at once (rows and columns of one
+// iteration): every load of both sets is issued before the reductions, and
+// each set is combined in the same order as by reduce_partials.
+__device__ __forceinline__ void reduce_partials_pair(const double *__restrict__ pa, int na,
+                                                     const double *__restrict__ pb, int nb, double *out_a,
+                                                     double *out_b, double *scratch) {
+    double va[kSlots], vb[kSlots];
+    // Two of the four per-thread entries per round (32 loads in flight fit
+    // the 128 registers of a 1024-thread block); the second round only runs
+    // for more than 2 * kFinalBlock partials.
+    const int rounds = max(na, nb) > 2 * kFinalBlock ? 2 : 1;
+#pragma unroll 1
+    for (int r = 0; r < rounds; ++r) {
+        const int i0 = threadIdx.x + 2 * r * kFinalBlock, i1 = i0 + kFinalBlock;
+        double a0[kSlots], a1[kSlots], b0[kSlots], b1[kSlots];
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            a0[s] = i0 < na ? pa[s * na + i0] : 0.0;
+            a1[s] = i1 < na ? pa[s * na + i1] : 0.0;
+            b0[s] = i0 < nb ? pb[s * nb + i0] : 0.0;
+            b1[s] = i1 < nb ? pb[s * nb + i1] : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (r == 0) {
+                va[s] = (s == 0) ? dmax(a0[s], a1[s]) : a0[s] + a1[s];
+                vb[s] = (s == 0) ? dmax(b0[s], b1[s]) : b0[s] + b1[s];
+            } else {
+                va[s] = (s == 0) ? dmax(dmax(va[s], a0[s]), a1[s]) : (va[s] + a0[s]) + a1[s];
+                vb[s] = (s == 0) ? dmax(dmax(vb[s], b0[s]), b1[s]) : (vb[s] + b0[s]) + b1[s];
+            }
+        }
+    }
+    const double ra = block_reduce_slots<kFinalBlock>(va, scratch);
+    __syncthreads();   // threads 0..7 have read scratch before it is reused
+    const double rb = block_reduce_slots<kFinalBlock>(vb, scratch);
+    if (threadIdx.x < kSlots) {
+        out_a[threadIdx.x] = ra;
+        out_b[threadIdx.x] = rb;
+    }
+    __syncthreads();
+}
+
 // CSR-adaptive SpMV, one tile per wave (grid-stride over waves).
 //
 //   header   one 16-byte load gives the tile's first/last row and nonzero;

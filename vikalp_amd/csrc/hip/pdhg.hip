@@ -100,12 +100,14 @@ __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
     store_partials(part, b.part_cols, scratch);
 }
 
-// Split form of the two fused kernels for large operands: the SpMV alone
-// (FP64 sums to memory, a light epilogue that keeps 8 waves per SIMD), then
-// a coalesced pass over rows or columns that runs the same epilogue on the
-// stored sums.  The iterates are computed exactly as by the fused kernels;
-// only the block partials of the KKT sums are grouped differently (rows per
-// workgroup), which the reference policy's max-norm tests do not see.
+// Split form of the fused kernels for large operands: the SpMV alone (FP64
+// sums to memory, a light epilogue that keeps 8 waves per SIMD), then a
+// coalesced pass over rows or columns that runs the same epilogue on the
+// stored sums (the dual side: k_dual_spmv or k_blocks, then k_dual_rows; the
+// A' side: k_primal_own, both phases in one launch).  The iterates are
+// computed exactly as by the fused kernels; only the block partials of the
+// KKT sums are grouped differently (rows per workgroup), which the
+// reference policy's max-norm tests do not see.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dual_spmv(Bufs<T> b, CsrView<T> A, double *part) {
     const DevState s = *b.st;
@@ -117,32 +119,41 @@ __global__ __launch_bounds__(kBlock) void k_dual_spmv(Bufs<T> b, CsrView<T> A, d
     tile_spmv(A, b.x[(s.k + 1) & 1], epi, unused, lds[threadIdx.x / kWave]);
 }
 
+// Split A' y+ and column pass in one launch: workgroup w owns the
+// contiguous tiles [w nt / G, (w+1) nt / G) of A' and so the rows (columns of
+// A) those tiles cover.  Its waves store the FP64 sums of those rows, then,
+// after a workgroup barrier, the workgroup runs the column pass over exactly
+// those rows: no second launch, and the pass's streaming overlaps other
+// workgroups' gathers.  The sums and iterates are the split kernels' own.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_primal_spmv(Bufs<T> b, CsrView<T> AT) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_primal_own(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
-    StoreEpi<double, true> epi{b.atyd};
-    Partials unused;
-    unused.zero();
-    tile_spmv(AT, b.y, epi, unused, lds[threadIdx.x / kWave]);
-}
-
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b) {
-    const DevState s = *b.st;
-    if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
+    const long long G = gridDim.x, w = blockIdx.x;
+    const int t0 = static_cast<int>(w * AT.ntiles / G), t1 = static_cast<int>((w + 1) * AT.ntiles / G);
+    {
+        StoreEpi<double, true> epi{b.atyd};   // streamed: the MALL budget is spent on A
+        Partials unused;
+        unused.zero();
+        tile_spmv_range(AT, b.y, epi, unused, lds[threadIdx.x / kWave], t0, t1,
+                        static_cast<int>(threadIdx.x) / kWave, kWaves);
+    }
+    __syncthreads();   // the sums of this workgroup's rows are visible to all its threads
+    const int r0 = AT.tiles[t0].x, r1 = AT.tiles[t1].x;
     const bool fuse = may_fuse(s);
     PrimalEpi<T, true> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse, sizeof(T) != sizeof(double)};
     Partials part;
     part.zero();
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
+    for (int j = r0 + static_cast<int>(threadIdx.x); j < r1; j += kBlock) {
         const auto pre = epi.prefetch(j);
         epi.row(j, __builtin_nontemporal_load(b.atyd + j), pre, part);
     }
     store_partials(part, b.part_cols, scratch);
 }
+
 
 // Q x+ for a quadratic objective (pdqp.cpp:116, :149): FP64 sums into b.qx,
 // before the primal kernel that uses them.
@@ -234,15 +245,24 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows_blocked(Bufs<T> b, const d
 // kernels in a fixed order and runs the step / restart / termination rule.
 // (A separate launch orders it after every block of both kernels without a
 // device-scope fence in each block.)
+//
+// The state is read into LDS by one word per thread while the partials load,
+// and the rule runs on a register copy: one thread working on the state in
+// global memory would wait on a chain of dependent loads.
 template <class T>
 __global__ __launch_bounds__(kFinalBlock) void k_finalize(Bufs<T> b) {
-    if (b.st->status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kFinalScratch];
     __shared__ double red_rows[kSlots], red_cols[kSlots];
-    reduce_partials(b.part_rows, b.grid_rows, red_rows, scratch);
-    reduce_partials(b.part_cols, b.grid_cols, red_cols, scratch);
+    __shared__ DevState st_lds;
+    static_assert(sizeof(DevState) % sizeof(int) == 0 && sizeof(DevState) / sizeof(int) <= kFinalBlock);
+    if (threadIdx.x < sizeof(DevState) / sizeof(int))
+        reinterpret_cast<int *>(&st_lds)[threadIdx.x] = reinterpret_cast<const int *>(b.st)[threadIdx.x];
+    reduce_partials_pair(b.part_rows, b.grid_rows, b.part_cols, b.grid_cols, red_rows, red_cols, scratch);
     if (threadIdx.x != 0) return;
-    finalize_step(*b.st, red_rows, red_cols);
+    DevState S = st_lds;
+    if (S.status != VK_PDHG_RUNNING) return;
+    finalize_step(S, red_rows, red_cols);
+    *b.st = S;
 }
 
 // Standalone primal step from x[k&1] into x[(k+1)&1] (setup, and after an
@@ -519,7 +539,7 @@ struct vk_hip_pdhg {
     // Split passes: SpMV-only kernels (launch widths grid_spmv_*) followed by
     // row / column passes over grid_rows / grid_cols workgroups.
     bool split = false;
-    int grid_spmv_rows = 1, grid_spmv_cols = 1;
+    int grid_spmv_rows = 1;
     BlockedView<double> blk64{};
     BlockedView<float> blk32{};
     double *blk_part = nullptr;
@@ -601,8 +621,7 @@ void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
         }
     }
     if (s->split) {
-        if (part & 1) k_primal_spmv<T><<<s->grid_spmv_cols, kBlock, 0, st>>>(b, view_at<T>(s));
-        if (part & 2) k_primal_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b);
+        if (part & 1) k_primal_own<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
     } else if (part & 1) {
         k_primal<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
     }
@@ -1025,6 +1044,10 @@ int profile_iterations(vk_hip_pdhg *s, int64_t iterations, double *out) {
     cleanup();
     if (rc) return rc;
     for (int j = 0; j < kMarks - 1; ++j) out[j] = sum[j] / static_cast<double>(iterations);
+    // Intervals with no kernel in them (fused dual kernel; the A' side is one
+    // kernel in every layout) hold only event overhead.
+    if (!s->split && s->col_blocks == 1) out[1] = 0.0;
+    out[3] = 0.0;
     out[4] = total / static_cast<double>(iterations);
     return VK_OK;
 }
@@ -1217,11 +1240,11 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     s->split = params->split_passes == 2 ||
                (params->split_passes == 0 && auto_split(s->m, s->n));
     s->grid_spmv_rows = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, kMaxGrid));
-    s->grid_spmv_cols = std::max(1, std::min((A->at.ntiles + kWaves - 1) / kWaves, kMaxGrid));
-    // Row and column passes run one element per thread over grid_rows /
-    // grid_cols workgroups (which also size the partial buffers).
+    // The row pass runs one row per thread over grid_rows workgroups (which
+    // also size the row partial buffer).  In split mode the A' side is
+    // k_primal_own over grid_cols workgroups, each with a contiguous tile
+    // range (about kWaves tiles).
     if (s->col_blocks > 1 || s->split) s->grid_rows = std::max(1, std::min((s->m + kBlock - 1) / kBlock, cap));
-    if (s->split) s->grid_cols = std::max(1, std::min((s->n + kBlock - 1) / kBlock, cap));
     auto fail = [&](int rc) {
         vk_hip_pdhg_destroy(s);
         return rc;
