@@ -111,8 +111,11 @@ int vk_hip_csr_norm2_estimate(const vk_hip_csr *A, int iterations, double *resul
 
 enum {
     VK_PDHG_POLICY_REFERENCE = 0,  /* pdhg.cpp:75-150 step/restart rule      */
-    VK_PDHG_POLICY_ADAPTIVE = 1    /* restarted PDHG with averaging and a
+    VK_PDHG_POLICY_ADAPTIVE = 1,   /* restarted PDHG with averaging and a
                                       primal weight (KKT restart criteria)   */
+    VK_PDHG_POLICY_HALPERN = 2     /* restarted Halpern PDHG with reflection
+                                      and a primal weight (fixed-point
+                                      residual restart criteria)             */
 };
 
 enum {

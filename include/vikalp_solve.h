@@ -33,7 +33,7 @@ typedef struct {
     double absolute_gap_tolerance;
     double relative_gap_tolerance;
     int deterministic;
-    int pdhg_policy;                /* 0 Reference, 1 Adaptive */
+    int pdhg_policy;                /* 0 Reference, 1 Adaptive, 2 Halpern */
     int pdhg_precision;             /* 0 FP64, 1 Mixed */
     int fused_iterations;
     int use_graphs;

@@ -71,29 +71,54 @@ struct StoreEpi {
     }
 };
 
-template <class T>
+// The dual and primal epilogues of one iteration: H selects the Halpern
+// policy's, otherwise the reference / adaptive ones.
+template <class T, bool H, bool Stream>
+struct IterEpis;
+
+template <class T, bool Stream>
+struct IterEpis<T, false, Stream> {
+    static __device__ DualEpi<T, Stream> dual(const Bufs<T> &b, const DevState &s) {
+        return DualEpi<T, Stream>{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
+                                  s.policy == VK_PDHG_POLICY_ADAPTIVE};
+    }
+    static __device__ PrimalEpi<T, Stream> primal(const Bufs<T> &b, const DevState &s, bool store_aty) {
+        return PrimalEpi<T, Stream>{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, may_fuse(s), store_aty};
+    }
+};
+
+template <class T, bool Stream>
+struct IterEpis<T, true, Stream> {
+    static __device__ HalpernDualEpi<T, Stream> dual(const Bufs<T> &b, const DevState &s) {
+        return HalpernDualEpi<T, Stream>{b, s.sigma, s.inv_sigma, halpern_weights(s.ks), check_iteration(s)};
+    }
+    static __device__ HalpernPrimalEpi<T, Stream> primal(const Bufs<T> &b, const DevState &s, bool) {
+        return HalpernPrimalEpi<T, Stream>{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, may_fuse(s),
+                                           check_iteration(s), halpern_weights(s.ks)};
+    }
+};
+
+template <class T, bool H>
 __global__ __launch_bounds__(kBlock) void k_dual(Bufs<T> b, CsrView<T> A) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kSlotScratch];
     const T *xin = b.x[(s.k + 1) & 1];
-    DualEpi<T> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
-                   s.policy == VK_PDHG_POLICY_ADAPTIVE};
+    auto epi = IterEpis<T, H, false>::dual(b, s);
     Partials part;
     part.zero();
     tile_spmv(A, xin, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_rows, scratch);
 }
 
-template <class T>
+template <class T, bool H>
 __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kSlotScratch];
-    const bool fuse = may_fuse(s);
-    PrimalEpi<T> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse};
+    auto epi = IterEpis<T, H, false>::primal(b, s, true);
     Partials part;
     part.zero();
     tile_spmv(AT, b.y, epi, part, lds[threadIdx.x / kWave]);
@@ -125,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_spmv(Bufs<T> b, CsrView<T> A, d
 // after a workgroup barrier, the workgroup runs the column pass over exactly
 // those rows: no second launch, and the pass's streaming overlaps other
 // workgroups' gathers.  The sums and iterates are the split kernels' own.
-template <class T>
+template <class T, bool H>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
 void k_primal_own(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
@@ -143,8 +168,7 @@ void k_primal_own(Bufs<T> b, CsrView<T> AT) {
     }
     __syncthreads();   // the sums of this workgroup's rows are visible to all its threads
     const int r0 = AT.tiles[t0].x, r1 = AT.tiles[t1].x;
-    const bool fuse = may_fuse(s);
-    PrimalEpi<T, true> epi{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, fuse, sizeof(T) != sizeof(double)};
+    auto epi = IterEpis<T, H, true>::primal(b, s, sizeof(T) != sizeof(double));
     Partials part;
     part.zero();
     for (int j = r0 + static_cast<int>(threadIdx.x); j < r1; j += kBlock) {
@@ -205,13 +229,12 @@ void k_blocks(Bufs<T> b, BlockedView<T> B, double *part, int which) {
 
 // Dual step over rows from the column-blocked partial sums (k_dual's
 // epilogue; the nb partials are added in block order).
-template <class T>
+template <class T, bool H>
 __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *__restrict__ part, int nb) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
-    DualEpi<T, true> epi{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
-                         s.policy == VK_PDHG_POLICY_ADAPTIVE};
+    auto epi = IterEpis<T, H, true>::dual(b, s);
     Partials acc_part;
     acc_part.zero();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
@@ -330,6 +353,45 @@ __global__ __launch_bounds__(kBlock) void k_apply(Bufs<T> b) {
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
         if (t < b.n) apply_column(b, s, t, xcur, xnext);
         if (t < b.m) apply_row(b, s, t);
+    }
+}
+
+// Halpern policy, after the finalizer of a restart-check iteration (the
+// primal kernel did not fuse the next step): on a restart z = z0 = T(z_k),
+// i.e. x+ and y+ with the A x+ and A' y+ kept at the check, which also
+// become the primal weight's restart point; then the next PDHG primal step
+// x+ = P_X((xh - tau c) - tau A' yh) with the step of the current weight.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_halpern_apply(Bufs<T> b) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    const T *xp = b.x[s.k & 1];
+    T *xnext = b.x[(s.k + 1) & 1];
+    const int total = b.n > b.m ? b.n : b.m;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        if (t < b.n) {
+            T xh = b.xh[t], aty = b.aty[t];
+            if (s.restart_apply) {
+                xh = xp[t];
+                aty = b.aty_avg[t];
+                b.xh[t] = xh;
+                b.aty[t] = aty;
+                b.x0h[t] = xh;
+                b.aty0h[t] = aty;
+                b.xlast[t] = xh;
+            }
+            xnext[t] = static_cast<T>(clamp_box((static_cast<double>(xh) - s.tau * static_cast<double>(b.c[t])) -
+                                                    s.tau * static_cast<double>(aty),
+                                                static_cast<double>(b.l[t]), static_cast<double>(b.u[t])));
+        }
+        if (t < b.m && s.restart_apply) {
+            const T yh = b.y[t], ax = b.ax_avg[t];
+            b.yh[t] = yh;
+            b.ax[t] = ax;
+            b.y0h[t] = yh;
+            b.ax0h[t] = ax;
+            b.ylast[t] = yh;
+        }
     }
 }
 
@@ -595,18 +657,24 @@ Bufs<float> &bufs<float>(vk_hip_pdhg *s) { return s->b32; }
 // column-blocked SpMV and its row pass.
 // `part` selects the kernels: 1 = the SpMV (or the fused kernel), 2 = the
 // row pass of a split layout (nothing when fused), 3 = both.
-template <class T>
-void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+template <class T, bool H>
+void launch_dual_policy(vk_hip_pdhg *s, hipStream_t st, int part) {
     Bufs<T> &b = bufs<T>(s);
     if (s->col_blocks > 1) {
         if (part & 1) k_blocks<T><<<s->grid_blk, kBlock, 0, st>>>(b, blocked<T>(s), s->blk_part, 0);
-        if (part & 2) k_dual_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
+        if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
     } else if (s->split) {
         if (part & 1) k_dual_spmv<T><<<s->grid_spmv_rows, kBlock, 0, st>>>(b, view_a<T>(s), s->blk_part);
-        if (part & 2) k_dual_rows<T><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, 1);
+        if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, 1);
     } else if (part & 1) {
-        k_dual<T><<<s->grid_rows, kBlock, 0, st>>>(b, view_a<T>(s));
+        k_dual<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, view_a<T>(s));
     }
+}
+
+template <class T>
+void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+    if (s->params.policy == VK_PDHG_POLICY_HALPERN) launch_dual_policy<T, true>(s, st, part);
+    else launch_dual_policy<T, false>(s, st, part);
 }
 
 // A' y+ with the residual terms and the fused next primal step (`part` as
@@ -620,10 +688,15 @@ void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
             k_qx<T><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(b, Q);
         }
     }
+    const bool h = s->params.policy == VK_PDHG_POLICY_HALPERN;
     if (s->split) {
-        if (part & 1) k_primal_own<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        if (part & 1) {
+            if (h) k_primal_own<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+            else k_primal_own<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        }
     } else if (part & 1) {
-        k_primal<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        if (h) k_primal<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        else k_primal<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
     }
 }
 
@@ -647,6 +720,9 @@ void launch_control(vk_hip_pdhg *s, long long k, hipStream_t st) {
     k_finalize<T><<<1, kFinalBlock, 0, st>>>(b);
     if (s->params.policy == VK_PDHG_POLICY_REFERENCE) {
         if (k > 0 && k % kDecisionPeriod == 0) k_pstep<T><<<egrid(s->n), kBlock, 0, st>>>(b);
+    } else if (s->params.policy == VK_PDHG_POLICY_HALPERN) {
+        if ((k + 1) % s->params.restart_period == 0)
+            k_halpern_apply<T><<<egrid(std::max(s->n, s->m)), kBlock, 0, st>>>(b);
     } else if ((k + 1) % s->params.restart_period == 0) {
         launch_eval_rows<T>(s, st);
         k_eval_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
@@ -756,7 +832,9 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     b.y = s->alloc<T>(m);
     b.ax = s->alloc<T>(m);
     b.aty = s->alloc<T>(n);
-    if constexpr (sizeof(T) == sizeof(double)) b.atyd = reinterpret_cast<double *>(b.aty);
+    const bool halpern = s->params.policy == VK_PDHG_POLICY_HALPERN;
+    // Halpern keeps the iterate's A' y in aty while the primal kernel sums A' y+.
+    if constexpr (sizeof(T) == sizeof(double)) b.atyd = halpern ? s->alloc<double>(n) : reinterpret_cast<double *>(b.aty);
     else b.atyd = s->alloc<double>(n);
     T *c_d = s->alloc<T>(n), *l_d = s->alloc<T>(n), *u_d = s->alloc<T>(n);
     T *rl_d = s->alloc<T>(m), *ru_d = s->alloc<T>(m);
@@ -768,10 +846,20 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     if (adaptive) {
         b.xsum = s->alloc<double>(n);
         b.ysum = s->alloc<double>(m);
+    }
+    if (adaptive || halpern) {
         b.xlast = s->alloc<T>(n);
         b.ylast = s->alloc<T>(m);
         b.ax_avg = s->alloc<T>(m);
         b.aty_avg = s->alloc<T>(n);
+    }
+    if (halpern) {
+        b.xh = s->alloc<T>(n);
+        b.yh = s->alloc<T>(m);
+        b.x0h = s->alloc<T>(n);
+        b.y0h = s->alloc<T>(m);
+        b.ax0h = s->alloc<T>(m);
+        b.aty0h = s->alloc<T>(n);
     }
     b.part_rows = s->alloc<double>(kSlots * s->grid_rows);
     b.part_cols = s->alloc<double>(kSlots * s->grid_cols);
@@ -784,7 +872,8 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
         if (!p) return VK_ENOMEM;
     if (!b.x[0] || !b.x[1] || !b.y || !b.ax || !b.aty || !b.atyd || !c_d || !l_d || !u_d || !rl_d ||
         !ru_d || !runs || !cs || !b.part_rows || !b.part_cols || !b.part_erows || !b.part_ecols ||
-        (adaptive && (!b.xsum || !b.ysum || !b.xlast || !b.ylast || !b.ax_avg || !b.aty_avg))) {
+        (adaptive && (!b.xsum || !b.ysum)) || ((adaptive || halpern) && (!b.xlast || !b.ylast || !b.ax_avg || !b.aty_avg)) ||
+        (halpern && (!b.xh || !b.yh || !b.x0h || !b.y0h || !b.ax0h || !b.aty0h))) {
         set_error("vk_hip_pdhg_create: device allocation failed");
         return VK_ENOMEM;
     }
@@ -861,6 +950,20 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
         VK_HIP_CHECK(hipStreamSynchronize(st));
         s->host_state->ns = 0;
         VK_HIP_CHECK(hipMemcpyAsync(s->st, s->host_state, sizeof(DevState), hipMemcpyHostToDevice, st));
+    }
+    if (halpern) {
+        // z0 = (x0, y0) is the Halpern iterate, the anchor and the primal
+        // weight's restart point; A x0 and A' y0 are its products.
+        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xh);
+        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.x0h);
+        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.x[0], b.xlast);
+        k_copy<T><<<egrid(n), kBlock, 0, st>>>(n, b.aty, b.aty0h);
+        if (m > 0) {
+            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.yh);
+            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.y0h);
+            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.y, b.ylast);
+            k_copy<T><<<egrid(m), kBlock, 0, st>>>(m, b.ax, b.ax0h);
+        }
     }
     if constexpr (sizeof(T) == sizeof(double)) {
         if (s->Q) {
@@ -1204,7 +1307,8 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     VK_REQUIRE(A->cols > 0, "vk_hip_pdhg_create: the LP needs at least one variable");
     VK_REQUIRE(c && l && u && col_scale, "vk_hip_pdhg_create: null column vector");
     VK_REQUIRE(A->rows == 0 || (rl && ru && row_scale), "vk_hip_pdhg_create: null row vector");
-    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE,
+    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE ||
+                   params->policy == VK_PDHG_POLICY_HALPERN,
                "vk_hip_pdhg_create: unknown policy");
     VK_REQUIRE(params->precision == VK_PREC_FP64 || params->precision == VK_PREC_MIXED,
                "vk_hip_pdhg_create: unknown precision");
@@ -1213,7 +1317,7 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
                "vk_hip_pdhg_create: step size must be positive");
     VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE ||
                    (params->primal_weight > 0.0 && params->restart_period > 0),
-               "vk_hip_pdhg_create: adaptive policy needs a primal weight and restart period");
+               "vk_hip_pdhg_create: the adaptive and Halpern policies need a primal weight and restart period");
     VK_REQUIRE(params->col_blocks >= 0 && params->col_blocks <= kMaxColBlocks,
                "vk_hip_pdhg_create: col_blocks must be in [0, 16]");
     VK_REQUIRE(params->split_passes >= 0 && params->split_passes <= 2,
@@ -1262,7 +1366,7 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     }
     DevState init{};
     init.limit = params->iteration_limit;
-    init.period = params->policy == VK_PDHG_POLICY_ADAPTIVE ? params->restart_period : kDecisionPeriod;
+    init.period = params->policy == VK_PDHG_POLICY_REFERENCE ? kDecisionPeriod : params->restart_period;
     init.status = params->iteration_limit == 0 ? VK_PDHG_ITERATION_LIMIT : VK_PDHG_RUNNING;
     init.policy = params->policy;
     init.metric_tol = params->metric_tolerance;

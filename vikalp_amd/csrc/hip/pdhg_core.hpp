@@ -32,16 +32,19 @@ struct DevState {
     double metric_tol, rel_tol, norm_b, norm_c;
     double rel_p, rel_d, rel_g, rel_kkt, pobj, dobj;
     double cur_pr2, cur_dr2, cur_gap, cur_dx2, cur_dy2;
-    double kkt_ref, kkt_prev;
+    double kkt_ref, kkt_prev;   // adaptive: weighted KKT at the restart / last check;
+                                // Halpern: fixed-point residual at the restart / last check
 };
 
 // Partial-slot layout (slot 0 combines with max, the rest with +).
 //   rows: 0 max|P(ax)-ax| (scaled)   1 sum (orig primal residual)^2
 //         2 sum (scaled primal residual)^2   3 dual-objective row terms
 //         4 count of -inf dual terms   5 sum (y - y_last)^2
+//         6 (Halpern) sum (y+ - y_k)^2
 //   cols: 0 max|x-P(x-g)| (scaled)   1 sum (orig dual residual)^2
 //         2 sum (scaled dual residual)^2   3 dual-objective column terms
 //         4 count of -inf dual terms   5 c'x   6 sum (x - x_last)^2
+//         7 (Halpern) sum (x+ - x_k)^2
 
 template <class T>
 struct Bufs {
@@ -57,6 +60,11 @@ struct Bufs {
     // its own rounding into the average far above FP32 iterate precision.
     double *xsum, *ysum;
     T *xlast, *ylast, *ax_avg, *aty_avg;
+    // Halpern only: the Halpern iterate (xh, yh; its A x and A' y live in ax
+    // and aty) and the anchor of the current restart (x0h, y0h, ax0h,
+    // aty0h).  x[] and y then hold the PDHG outputs T(z), which the SpMVs
+    // gather; ax_avg / aty_avg keep A x+ and A' y+ at restart checks.
+    T *xh, *yh, *x0h, *y0h, *ax0h, *aty0h;
     double *part_rows, *part_cols, *part_erows, *part_ecols;
     DevState *st;
     int m, n;
@@ -131,6 +139,82 @@ struct DualEpi {
     }
 };
 
+// Restarted Halpern PDHG with reflection (Lu and Yang, "Restarted Halpern
+// PDHG for linear programming", 2024): with T the PDHG map and z0 the point
+// of the last restart, z_{k+1} = w1 (2 T(z_k) - z_k) + w0 z0 where, for the
+// ks-th iteration since the restart, w1 = (ks+1)/(ks+2) and w0 = 1/(ks+2).
+// The SpMVs run on T(z_k) = (x+, y+); A x_{k+1} and A' y_{k+1} follow by
+// linearity from A x+, A x_k and A x0 (and likewise for A'), so an iteration
+// still costs two SpMVs.  The KKT sums and the termination test are taken
+// on T(z_k), the point returned.
+struct HalpernWeights {
+    double w1, w0;
+};
+__device__ __forceinline__ HalpernWeights halpern_weights(long long ks) {
+    const double d = 1.0 / static_cast<double>(ks + 2);
+    return HalpernWeights{static_cast<double>(ks + 1) * d, d};
+}
+
+// Whether iteration k of a Halpern or adaptive solve ends at a restart check.
+__device__ __forceinline__ bool check_iteration(const DevState &s) { return (s.k + 1) % s.period == 0; }
+
+// Halpern dual half on row i given acc = (A x+)_i: y+ = the dual step from
+// (y_k, A x_k) with A xbar = 2 A x+ - A x_k; then the Halpern update of y
+// and of A x.  Row slots as DualEpi on (A x+, y+), plus slot 6
+// sum (y+ - y_k)^2 and, at checks, slot 5 sum (y+ - y_restart)^2.
+template <class T, bool Stream = false>
+struct HalpernDualEpi {
+    const Bufs<T> &b;
+    double sigma, inv_sigma;
+    HalpernWeights w;
+    bool check;
+    struct Pre {
+        T ax, yh, lo, hi, unscale, y0, ax0, ylast;
+    };
+    __device__ Pre prefetch(int i) const {
+        Pre p;
+        p.ax = vec_load<Stream>(b.ax + i);
+        p.yh = vec_load<Stream>(b.yh + i);
+        p.lo = vec_load<Stream>(b.rl + i);
+        p.hi = vec_load<Stream>(b.ru + i);
+        p.unscale = vec_load<Stream>(b.row_unscale + i);
+        p.y0 = vec_load<Stream>(b.y0h + i);
+        p.ax0 = vec_load<Stream>(b.ax0h + i);
+        p.ylast = check ? vec_load<Stream>(b.ylast + i) : T(0);
+        return p;
+    }
+    __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
+        const double axn = acc;
+        const double ax_old = static_cast<double>(q.ax), y_old = static_cast<double>(q.yh);
+        const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
+        const double v = y_old + sigma * (2.0 * axn - ax_old);
+        const double t = inv_sigma * v;   // sign-exact Moreau step, as for the adaptive policy
+        const double yn = (t < lo) ? fmin(v - sigma * lo, 0.0) : ((hi < t) ? fmax(v - sigma * hi, 0.0) : 0.0);
+        const T y_stored = static_cast<T>(yn);
+        b.y[i] = y_stored;
+        const double yp = static_cast<double>(y_stored);
+        vec_store<Stream>(b.yh + i, static_cast<T>(w.w1 * (2.0 * yp - y_old) + w.w0 * static_cast<double>(q.y0)));
+        vec_store<Stream>(b.ax + i, static_cast<T>(w.w1 * (2.0 * axn - ax_old) + w.w0 * static_cast<double>(q.ax0)));
+        const double d = clamp_box(axn, lo, hi) - axn;
+        p.v[0] = dmax(p.v[0], fabs(d));
+        const double du = d * static_cast<double>(q.unscale);
+        p.v[1] += du * du;
+        p.v[2] += d * d;
+        if (yn > 0.0) {
+            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= yn * hi;
+        } else if (yn < 0.0) {
+            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= yn * lo;
+        }
+        const double dz = yp - y_old;
+        p.v[6] += dz * dz;
+        if (check) {
+            vec_store<Stream>(b.ax_avg + i, static_cast<T>(axn));
+            const double dy = yp - static_cast<double>(q.ylast);
+            p.v[5] += dy * dy;
+        }
+    }
+};
+
 // Column terms of the KKT sums for (x_j, g_j = c_j + aty_j) (shared by the
 // per-iteration and the average evaluation).  s = col_scale[j].
 __device__ __forceinline__ void column_terms(double xj, double cj, double g, double lj, double uj,
@@ -194,6 +278,55 @@ struct PrimalEpi {
             xnext[j] = xn;
             if (b.xsum) vec_store<Stream>(b.xsum + j, q.xsum + static_cast<double>(xn));
         }
+    }
+};
+
+// Halpern primal half on column j given acc = (A' y+)_j: the KKT column
+// terms of (x+, c + A' y+) (slot 6 at checks: sum (x+ - x_restart)^2), slot 7
+// sum (x+ - x_k)^2, the Halpern update of x and of A' y, and (fused) the next
+// PDHG primal step x+ = P_X((x_{k+1} - tau c) - tau A' y_{k+1}).
+template <class T, bool Stream = false>
+struct HalpernPrimalEpi {
+    const Bufs<T> &b;
+    const T *xcur;   // x+ of this iteration
+    T *xnext;        // x+ of the next one (fused step)
+    double tau;
+    bool fuse, check;
+    HalpernWeights w;
+    struct Pre {
+        T xp, xh, aty, c, l, u, s, x0, aty0, xlast;
+    };
+    __device__ Pre prefetch(int j) const {
+        Pre p;
+        p.xp = vec_load<Stream>(xcur + j);
+        p.xh = vec_load<Stream>(b.xh + j);
+        p.aty = vec_load<Stream>(b.aty + j);
+        p.c = vec_load<Stream>(b.c + j);
+        p.l = vec_load<Stream>(b.l + j);
+        p.u = vec_load<Stream>(b.u + j);
+        p.s = vec_load<Stream>(b.col_scale + j);
+        p.x0 = vec_load<Stream>(b.x0h + j);
+        p.aty0 = vec_load<Stream>(b.aty0h + j);
+        p.xlast = check ? vec_load<Stream>(b.xlast + j) : T(0);
+        return p;
+    }
+    __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
+        const double xp = static_cast<double>(q.xp), x_old = static_cast<double>(q.xh);
+        const double cj = static_cast<double>(q.c);
+        const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
+        column_terms(xp, cj, cj + acc, lj, uj, static_cast<double>(q.s), check, static_cast<double>(q.xlast), p);
+        const double dz = xp - x_old;
+        p.v[7] += dz * dz;
+        const T xh_new = static_cast<T>(w.w1 * (2.0 * xp - x_old) + w.w0 * static_cast<double>(q.x0));
+        const T aty_new =
+            static_cast<T>(w.w1 * (2.0 * acc - static_cast<double>(q.aty)) + w.w0 * static_cast<double>(q.aty0));
+        vec_store<Stream>(b.xh + j, xh_new);
+        vec_store<Stream>(b.aty + j, aty_new);
+        if (check) vec_store<Stream>(b.aty_avg + j, static_cast<T>(acc));
+        if (fuse)
+            xnext[j] = static_cast<T>(clamp_box((static_cast<double>(xh_new) - tau * cj) -
+                                                    tau * static_cast<double>(aty_new),
+                                                lj, uj));
     }
 };
 
@@ -276,6 +409,34 @@ __device__ __forceinline__ bool may_fuse(const DevState &s) {
     return (s.k + 1) % s.period != 0;
 }
 
+// Halpern restart rule at the end of iteration S.k, from the reduced slots
+// (before S.k and S.ks advance).  The fixed-point residual
+// r = |z_k - T(z_k)| in the primal-weight norm (w |dx|^2 + |dy|^2 / w)^(1/2)
+// is taken every iteration; the first one after a restart is the reference
+// r0.  At a check: restart on r <= 0.2 r0, on r <= 0.8 r0 with r above the
+// previous check's, or once the iterations since the restart reach 0.36 of
+// all iterations; the primal weight then moves halfway (in log) to
+// |y+ - y_r| / |x+ - x_r| since the last restart point, and the restart
+// (k_halpern_apply) sets z = z0 = T(z_k).
+__device__ inline void halpern_rule(DevState &S, const double *red_rows, const double *red_cols, bool converged) {
+    const double r = sqrt(S.omega * red_cols[7] + red_rows[6] / S.omega);
+    if (S.ks == 0) S.kkt_ref = r;
+    S.restart_apply = 0;
+    if (converged || (S.k + 1) % S.period != 0) return;
+    const bool restart = r <= 0.2 * S.kkt_ref || (r <= 0.8 * S.kkt_ref && r > S.kkt_prev) ||
+                         static_cast<double>(S.ks) >= 0.36 * static_cast<double>(S.k + 1);
+    S.kkt_prev = r;
+    if (!restart) return;
+    const double dx = sqrt(red_cols[6]), dy = sqrt(red_rows[5]);
+    if (dx > 1e-10 && dy > 1e-10) S.omega = exp(0.5 * log(dy / dx) + 0.5 * log(S.omega));
+    S.tau = S.eta / S.omega;
+    S.sigma = S.eta * S.omega;
+    S.inv_sigma = 1.0 / S.sigma;
+    S.kkt_prev = INFINITY;
+    S.restart_apply = 1;
+    S.restarts += 1;
+}
+
 // Step / restart / termination rule after one iteration, from the reduced
 // dual (red_rows) and primal (red_cols) partial slots (pdhg.cpp:131-150 for
 // the reference policy).  Runs in one thread.
@@ -321,8 +482,9 @@ __device__ inline void finalize_step(DevState &S, const double *red_rows, const 
         S.inv_sigma = 1.0 / S.sigma;
     }
     const bool converged = (S.rel_tol > 0.0) ? (rel_kkt <= S.rel_tol) : (metric <= S.metric_tol);  // pdhg.cpp:147
+    if (S.policy == VK_PDHG_POLICY_HALPERN) halpern_rule(S, red_rows, red_cols, converged);
     S.k = k + 1;
-    S.ks += 1;
+    S.ks = (S.policy == VK_PDHG_POLICY_HALPERN && S.restart_apply) ? 0 : S.ks + 1;
     S.ns += 1;
     if (converged) S.status = VK_PDHG_CONVERGED;
     else if (S.k >= S.limit) S.status = VK_PDHG_ITERATION_LIMIT;

@@ -152,6 +152,11 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             results[i] = status_result(SolveStatus::InvalidModel, "solver limits and tolerances must be non-negative");
             continue;
         }
+        if (options.pdhg_policy == PdhgPolicy::Halpern) {
+            results[i] = status_result(SolveStatus::UnsupportedModel,
+                                       "the batched solver runs the reference and adaptive policies");
+            continue;
+        }
         pr.sc = scale(model, pr.bounds);
         if (options.pdhg_policy == PdhgPolicy::Adaptive)
             equilibrate(model.constraint_matrix.pattern, pr.sc, kRuizPasses);

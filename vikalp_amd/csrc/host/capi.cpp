@@ -33,7 +33,7 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.absolute_gap_tolerance = o.absolute_gap_tolerance;
     s.relative_gap_tolerance = o.relative_gap_tolerance;
     s.deterministic = o.deterministic != 0;
-    s.pdhg_policy = o.pdhg_policy == 1 ? PdhgPolicy::Adaptive : PdhgPolicy::Reference;
+    s.pdhg_policy = static_cast<PdhgPolicy>(o.pdhg_policy);   // out-of-range values fail valid_options
     s.pdhg_precision = o.pdhg_precision == 1 ? PdhgPrecision::Mixed : PdhgPrecision::FP64;
     s.fused_iterations = o.fused_iterations != 0;
     s.use_graphs = o.use_graphs != 0;

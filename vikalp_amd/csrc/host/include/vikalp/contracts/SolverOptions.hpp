@@ -15,7 +15,8 @@ enum class ContinuousMethod { Auto, PDHG, RevisedSimplex, InteriorPoint };
 // Step-size / restart rule of the PDHG loop.
 enum class PdhgPolicy {
     Reference,   // the reference rule, src/solver/lp/pdhg.cpp:75-150
-    Adaptive     // restarted PDHG with averaging and a primal weight
+    Adaptive,    // restarted PDHG with averaging and a primal weight
+    Halpern      // restarted Halpern PDHG with reflection and a primal weight
 };
 
 // Arithmetic of the device iteration.

@@ -47,7 +47,9 @@ bool valid_options(const SolverOptions &o) {
            std::isfinite(o.relative_kkt_tolerance) && o.relative_kkt_tolerance >= 0.0 &&
            o.restart_period > 0 && std::isfinite(o.mixed_phase_tolerance) &&
            o.mixed_phase_tolerance >= 0.0 && o.spmv_col_blocks >= 0 && o.spmv_col_blocks <= 16 &&
-           o.spmv_split_passes >= 0 && o.spmv_split_passes <= 2 && o.spmv_cache_budget_mb >= -1;
+           o.spmv_split_passes >= 0 && o.spmv_split_passes <= 2 && o.spmv_cache_budget_mb >= -1 &&
+           (o.pdhg_policy == PdhgPolicy::Reference || o.pdhg_policy == PdhgPolicy::Adaptive ||
+            o.pdhg_policy == PdhgPolicy::Halpern);
 }
 
 bool timed_out(Clock::time_point started, const SolverOptions &o) {
@@ -438,8 +440,9 @@ int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &para
 vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_csr *A,
                            const SolverOptions &options) {
     vk_pdhg_params p{};
-    p.policy = options.pdhg_policy == PdhgPolicy::Adaptive ? VK_PDHG_POLICY_ADAPTIVE
-                                                           : VK_PDHG_POLICY_REFERENCE;
+    p.policy = options.pdhg_policy == PdhgPolicy::Adaptive  ? VK_PDHG_POLICY_ADAPTIVE
+               : options.pdhg_policy == PdhgPolicy::Halpern ? VK_PDHG_POLICY_HALPERN
+                                                            : VK_PDHG_POLICY_REFERENCE;
     p.iteration_limit = options.iteration_limit;
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
     p.relative_kkt_tolerance = options.relative_kkt_tolerance;
@@ -506,7 +509,7 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
             p64.precision = VK_PREC_FP64;
             p64.col_blocks = col_blocks_for(A, options, VK_PREC_FP64);
             p64.iteration_limit = options.iteration_limit - used;
-            if (p.policy == VK_PDHG_POLICY_ADAPTIVE) p64.primal_weight = state.primal_weight;
+            if (p.policy != VK_PDHG_POLICY_REFERENCE) p64.primal_weight = state.primal_weight;
             status = run_device(A, sc, p64, &x32, &y32, started, options, xs, ys, state, ms, timed);
             total_ms += ms;
             used += state.iterations;
@@ -637,7 +640,7 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
         model.num_variables() == 0 || iterations <= 0)
         throw std::invalid_argument("profile_lp_kernels: unusable model or arguments");
     Scaled sc = scale(model, bounds);
-    if (options.pdhg_policy == PdhgPolicy::Adaptive)
+    if (options.pdhg_policy != PdhgPolicy::Reference)
         equilibrate(model.constraint_matrix.pattern, sc, kRuizPasses);
     vk_hip_csr *A = nullptr;
     check_vk(vk_hip_csr_create(model.constraint_matrix.pattern.rows, model.constraint_matrix.pattern.columns,
@@ -695,7 +698,7 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend, const Solver
         return status_result(SolveStatus::UnsupportedModel, "requested continuous method is not implemented for LP");
 
     Scaled sc = scale(model, bounds);
-    if (options.pdhg_policy == PdhgPolicy::Adaptive)
+    if (options.pdhg_policy != PdhgPolicy::Reference)
         equilibrate(model.constraint_matrix.pattern, sc, kRuizPasses);
     std::vector<Scalar> warm;
     if (!warm_start.empty()) {
@@ -720,7 +723,7 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend, const Solver
         res = solve_fused(model, bounds, sc, handle, options, warm_ptr, started);
     } else if (device_only) {
         return status_result(SolveStatus::UnsupportedModel,
-                             "adaptive, mixed-precision and relative-KKT PDHG run only fused on a HipBackend");
+                             "adaptive, Halpern, mixed-precision and relative-KKT PDHG run only fused on a HipBackend");
     } else {
         res = solve_ops(model, bounds, sc, backend, *matrix, options, warm_ptr, started);
     }

@@ -37,6 +37,7 @@ class ContinuousMethod(enum.IntEnum):
 class PdhgPolicy(enum.IntEnum):
     Reference = 0
     Adaptive = 1
+    Halpern = 2
 
 
 class PdhgPrecision(enum.IntEnum):
