@@ -7,8 +7,10 @@ of columns rescaled by 10^U(-2,2), 70% equality rows and 30% <= rows, 0 <= x <= 
 seeded (generator family 2 of include/vikalp_lpgen.h).
 
 A step is one solve from a cold start until the relative KKT error (original
-coordinates) is <= 1e-6, with the adaptive restarted PDHG on the fused HIP
-path.  `value` is the mixed-precision device time of the iteration phase (the
+coordinates) is <= 1e-6, with restarted Halpern PDHG (reflection, fixed-point
+residual restarts) on the fused HIP path; the FP32 phase of the mixed solve
+runs to 1e-5.  The same solve with the adaptive policy (restarted averaged
+PDHG) is reported beside it.  `value` is the mixed-precision device time of the iteration phase (the
 LP is already resident in HBM when it starts; setup, scaling and postsolve
 are reported separately), `fp64_seconds` the same for the FP64 path.
 
@@ -19,7 +21,8 @@ duration.  cpu_baseline: the CPU oracle (the reference's single-threaded
 algorithm restated in C) timed on a bounded sample of iterations of the same
 LP on one host core, extrapolated to the GPU's iteration count.
 
-Also measured (extra objects of the JSON line): the batched solve of 512
+Also measured (extra objects of the JSON line): the FP64 solve of the same
+LP with the adaptive policy ("adaptive_fp64"), the batched solve of 512
 block-angular LPs ("batch") and the unrolled-PDHG adjoint on a 100,000 x
 250,000 LP with its central-difference check ("adjoint").
 
@@ -40,14 +43,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = {"workload": "pdhg_lp_zipf", "m": 400_000, "n": 1_000_000, "family": 2,
-            "seed": 20261203, "tolerance": 1e-6, "policy": "adaptive"}
+            "seed": 20261203, "tolerance": 1e-6, "policy": "halpern"}
 
 
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_table(prof, m, n, nnz, sv, suffix):
+def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True):
     """Algorithmic (compulsory) HBM bytes and achieved GB/s per kernel of one
     iteration, for the layout the solver used.
 
@@ -55,10 +58,14 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
     indices (s_v + 4) nnz, row offsets 4 (rows + 1), x gathered once
     s_v cols, FP64 sums written 8 rows.  Column blocks add offsets and
     partial sums that are not counted (they lower the achieved fraction).
-    Row pass (dual): FP64 sums 8 m, plus per row read ax, y, rl, ru, 1/r,
-    y_last, write ax, y (8 s_v m) and read + write the FP64 y_sum (16 m).
-    Column pass (primal): FP64 sums 8 n, per column read x, c, l, u, s,
-    x_last, write aty, x_next (8 s_v n), FP64 x_sum (16 n).
+    Row pass (dual), Halpern: FP64 sums 8 m, plus per row read ax, y_h,
+    rl, ru, 1/r, y0, ax0 and write y+, y_h, ax (10 s_v m); adaptive: read ax,
+    y, rl, ru, 1/r, y_last, write ax, y (8 s_v m) and read + write the FP64
+    y_sum (16 m).  Column pass (primal), Halpern: FP64 sums 8 n, per column
+    read x+, x_h, aty, c, l, u, s, x0, aty0 and write x_h, aty, x_next
+    (12 s_v n); adaptive: read x, c, l, u, s, x_last, write aty, x_next
+    (8 s_v n) and the FP64 x_sum (16 n).  (Restart-check iterations, one in
+    64, touch a few vectors more; not counted.)
     A fused kernel counts its SpMV without the sum store plus its pass
     without the sum load; k_primal_own (split A' side: SpMV, then the
     column pass over the workgroup's own rows, one launch) stores and
@@ -66,8 +73,8 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
     """
     spmv_a = (sv + 4) * nnz + 4 * (m + 1) + sv * n + 8 * m
     spmv_at = (sv + 4) * nnz + 4 * (n + 1) + sv * m + 8 * n
-    pass_rows = 8 * m + 8 * sv * m + 16 * m
-    pass_cols = 8 * n + 8 * sv * n + 16 * n
+    pass_rows = 8 * m + (10 * sv * m if halpern else 8 * sv * m + 16 * m)
+    pass_cols = 8 * n + (12 * sv * n if halpern else 8 * sv * n + 16 * n)
     rows = []
     if prof["col_blocks"] > 1 or prof["split_passes"]:
         rows.append(("k_blocks" if prof["col_blocks"] > 1 else "k_dual_spmv", prof["dual_spmv_ms"], spmv_a))
@@ -85,8 +92,8 @@ def kernel_table(prof, m, n, nnz, sv, suffix):
 # HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, scaled by a 256 MiB
 # device copy) measured with `bench.py --counters` under rocprofv3 --pmc;
 # profiles/pdhg_traffic_zipf_400k_x_1m.md holds the passes and commands.
-MEASURED_TRAFFIC = {"k_blocks<double>": 256.6e6, "k_dual_rows<double>": 45.3e6,
-                    "k_primal_own<double>": 359.8e6}
+MEASURED_TRAFFIC = {"k_blocks<double>": 256.5e6, "k_dual_rows<double>": 45.3e6,
+                    "k_primal_own<double>": 386.5e6}
 
 
 def counters_driver(V, args):
@@ -99,7 +106,7 @@ def counters_driver(V, args):
     dev.free_all()
     log("calibration copies done")
     lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
-    p = V.profile_kernels(lp, V.Options(pdhg_policy=V.PdhgPolicy.Adaptive,
+    p = V.profile_kernels(lp, V.Options(pdhg_policy=V.PdhgPolicy.Halpern,
                                         relative_kkt_tolerance=WORKLOAD["tolerance"]), 2, 6)
     log(f"profiled: col_blocks={p['col_blocks']} split={p['split_passes']}")
 
@@ -194,7 +201,7 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=200_000)
     ap.add_argument("--time-limit", type=float, default=100.0, help="seconds per solve")
     ap.add_argument("--cpu-iterations", type=int, default=200)
-    ap.add_argument("--phase-tolerance", type=float, default=2e-6,
+    ap.add_argument("--phase-tolerance", type=float, default=1e-5,
                     help="relative KKT target of the FP32 phase of the mixed solve")
     ap.add_argument("--batch-instances", type=int, default=512,
                     help="block-angular instances of the batched solve (0 = skip)")
@@ -203,6 +210,8 @@ def main():
                     help="unrolled iterations of the adjoint measurement (0 = skip)")
     ap.add_argument("--fd-directions", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-adaptive", action="store_true",
+                    help="skip the FP64 solve with the adaptive policy (comparison)")
     ap.add_argument("--skip-solve", action="store_true",
                     help="kernel timing only (used for the rocprofv3 pass)")
     ap.add_argument("--counters", action="store_true",
@@ -226,8 +235,8 @@ def main():
     lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
     log(f"generated LP m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t0:.1f}s")
 
-    def options(precision):
-        return V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
+    def options(precision, policy=V.PdhgPolicy.Halpern):
+        return V.Options(pdhg_policy=policy, pdhg_precision=precision,
                          relative_kkt_tolerance=WORKLOAD["tolerance"],
                          mixed_phase_tolerance=args.phase_tolerance,
                          iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit)
@@ -242,9 +251,9 @@ def main():
             f"split={p['split_passes']}; " +
             ", ".join(f"{k} {v['ms'] * 1e3:.1f} us {v['GBs']:.0f} GB/s" for k, v in table.items()))
 
-    def run(precision, label):
+    def run(precision, label, policy=V.PdhgPolicy.Halpern):
         log(f"solve {label} start")
-        r = V.solve_lp(lp, options(precision))
+        r = V.solve_lp(lp, options(precision, policy))
         log(f"solve {label}: {r.status} iterations={r.iterations} (fp32 {r.low_precision_iterations}) "
             f"device={r.device_seconds:.3f}s wall={r.solve_seconds:.3f}s relkkt={r.kkt_relative:.3e} "
             f"obj={r.objective_value:.10g}")
@@ -276,6 +285,14 @@ def main():
     dominant = max(k64, key=lambda k: k64[k]["ms"])
     dom_ms, dom_bytes = k64[dominant]["ms"], k64[dominant]["bytes"]
     achieved = k64[dominant]["GBs"]
+
+    adaptive = None
+    if not args.skip_solve and not args.skip_adaptive:
+        ra = run(V.PdhgPrecision.FP64, "fp64 adaptive", V.PdhgPolicy.Adaptive)
+        adaptive = {"policy": "adaptive", "precision": "fp64", "status": ra.status, "iterations": ra.iterations,
+                    "device_seconds": ra.device_seconds, "rel_kkt": ra.kkt_relative,
+                    "objective_rel_diff_vs_halpern": abs(ra.objective_value - r64.objective_value) /
+                    max(1.0, abs(r64.objective_value))}
 
     cpu = None
     if not args.skip_cpu:
@@ -330,6 +347,7 @@ def main():
                    "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"]}},
         "kernels": {"fp64": k64, "mixed": k32},
         "cpu_baseline": cpu,
+        "adaptive_fp64": adaptive,
         "batch": batch,
         "adjoint": adjoint,
     }
