@@ -116,3 +116,29 @@ def test_bad_arguments_are_rejected():
                                  ci.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(h))
     assert rc == -1 and b"ordered" in dev.L.vk_hip_last_error()
     assert dev.L.vk_hip_fill(None, 5, 1.0, None) == -1
+
+
+def test_norm_estimate_ignores_stale_memory():
+    # Regression: the power iteration's scratch vector comes from fresh device
+    # memory and k_spmv forms alpha A x + beta y with beta = 0 literally, so a
+    # NaN left there by an earlier allocation must not reach the estimate.
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    m, n = 700, 1500
+    rp, ci, v = random_csr(rng, m, n, 0.01)
+    dev = V.Device()
+    for _ in range(4):                        # stale NaNs in blocks of the scratch size
+        p = dev.alloc(m)
+        dev.L.vk_hip_fill(p, m, float("nan"), None)
+    dev.L.vk_hip_sync(None)
+    dev.free_all()
+    h = dev.csr(m, n, rp, ci, v)
+    est = C.c_double()
+    assert dev.L.vk_hip_csr_norm2_estimate(h, 200, C.byref(est)) == 0
+    dense = np.zeros((m, n))
+    for r in range(m):
+        dense[r, ci[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
+    exact = np.linalg.norm(dense, 2)
+    # sqrt(|A'A v|) with |v| = 1 never exceeds sigma_max and approaches it from below
+    assert np.isfinite(est.value) and 0.95 * exact <= est.value <= exact * (1 + 1e-12), (est.value, exact)
+    dev.L.vk_hip_csr_destroy(h)

@@ -532,6 +532,9 @@ int vk_hip_csr_norm2_estimate(const vk_hip_csr *A, int iterations, double *resul
     double *v = nullptr, *w = nullptr;
     VK_HIP_CHECK(hipMalloc(&v, sizeof(double) * A->cols));
     VK_HIP_CHECK(hipMalloc(&w, sizeof(double) * A->rows));
+    // k_spmv forms alpha A x + beta y literally (cpu_ops.cpp:426), so with
+    // beta = 0 a NaN or Inf left in fresh memory would still reach the sum.
+    VK_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(double) * A->rows, s));
     k_seed_vector<<<elementwise_grid(A->cols), kBlock, 0, s>>>(A->cols, v);
     double norm2 = 0.0, nv = 0.0;
     int rc = VK_OK;

@@ -227,6 +227,19 @@ void k_blocks(Bufs<T> b, BlockedView<T> B, double *part, int which) {
     else blocked_spmv(B, b.x[(s.k + 1) & 1], part, lds[threadIdx.x / kWave]);
 }
 
+// k_blocks for the iteration whose A x+ runs beside the previous iteration's
+// finalizer (on another stream): x is passed by the host instead of read
+// from the device state, which the finalizer is writing; only the status
+// word is read (a converged solve makes this launch a no-op, or at worst one
+// unused SpMV).
+template <class T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_blocks_x(const DevState *st, BlockedView<T> B, const T *x, double *part) {
+    if (st->status != VK_PDHG_RUNNING) return;
+    __shared__ double lds[kWaves][kTileNnz];
+    blocked_spmv(B, x, part, lds[threadIdx.x / kWave]);
+}
+
 // Dual step over rows from the column-blocked partial sums (k_dual's
 // epilogue; the nb partials are added in block order).
 template <class T, bool H>
@@ -581,6 +594,11 @@ struct vk_hip_pdhg {
     const vk_hip_csr *Q = nullptr;   // quadratic objective (PDQP), FP64 only
     int m = 0, n = 0;
     hipStream_t stream = nullptr;
+    // Second stream (default priority) for the finalizer when it runs beside
+    // the next iteration's A x (column-blocked layout), and the fork / join
+    // events of that overlap.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevState *st = nullptr;
     DevState *host_state = nullptr;   // pinned
     std::vector<void *> allocations;
@@ -731,14 +749,48 @@ void launch_control(vk_hip_pdhg *s, long long k, hipStream_t st) {
     }
 }
 
+// Length of the repeating iteration block and the first iteration it may
+// start at (the reference policy's iteration 0 has its own shape).
+long long block_length(const vk_hip_pdhg *s) {
+    return s->params.policy == VK_PDHG_POLICY_REFERENCE ? kDecisionPeriod : s->params.restart_period;
+}
+
+// Whether launch_control runs more than the finalizer after iteration k.
+bool control_after(const vk_hip_pdhg *s, long long k) {
+    if (s->params.policy == VK_PDHG_POLICY_REFERENCE) return k > 0 && k % kDecisionPeriod == 0;
+    return (k + 1) % s->params.restart_period == 0;
+}
+
 // Enqueues the kernels of iterations [k_begin, k_begin + count) without
 // synchronising; the schedule mirrors the device's own may_fuse rule.
+//
+// Column-blocked layout: after an iteration with nothing but the finalizer
+// to follow, the finalizer (one workgroup, about 13 us) runs on the side
+// stream while the next iteration's A x+ (which needs only x+, already
+// written by the primal kernel) runs on the main stream; the next dual pass
+// waits for both.  The x+ buffer is chosen by iteration parity on the host,
+// so the overlap needs an even block length (graph replays then keep the
+// parity).  The same kernels compute the same sums either way.
 template <class T>
 void enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
+    Bufs<T> &b = bufs<T>(s);
+    const bool overlap = s->side && s->col_blocks > 1 && block_length(s) % 2 == 0;
+    bool spmv_done = false;   // iteration k's A x+ was launched beside the previous finalizer
     for (long long k = k_begin; k < k_begin + count; ++k) {
-        launch_dual<T>(s, s->stream);
+        launch_dual<T>(s, s->stream, spmv_done ? 2 : 3);
         launch_primal<T>(s, s->stream);
-        launch_control<T>(s, k, s->stream);
+        if (overlap && k + 1 < k_begin + count && !control_after(s, k)) {
+            (void)hipEventRecord(s->ev_fork, s->stream);
+            (void)hipStreamWaitEvent(s->side, s->ev_fork, 0);
+            k_finalize<T><<<1, kFinalBlock, 0, s->side>>>(b);
+            (void)hipEventRecord(s->ev_join, s->side);
+            k_blocks_x<T><<<s->grid_blk, kBlock, 0, s->stream>>>(b.st, blocked<T>(s), b.x[(k + 2) & 1], s->blk_part);
+            (void)hipStreamWaitEvent(s->stream, s->ev_join, 0);
+            spmv_done = true;
+        } else {
+            launch_control<T>(s, k, s->stream);
+            spmv_done = false;
+        }
     }
 }
 
@@ -811,11 +863,6 @@ int build_blocked(vk_hip_pdhg *s) {
     return VK_OK;
 }
 
-// Length of the repeating iteration block and the first iteration it may
-// start at (the reference policy's iteration 0 has its own shape).
-long long block_length(const vk_hip_pdhg *s) {
-    return s->params.policy == VK_PDHG_POLICY_REFERENCE ? kDecisionPeriod : s->params.restart_period;
-}
 
 template <class T>
 int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double *u,
@@ -1284,6 +1331,10 @@ int vk_hip_pdhg_destroy(vk_hip_pdhg *s) {
         if (s->ev_snap[i]) (void)hipEventDestroy(s->ev_snap[i]);
         if (s->snap[i]) (void)hipHostFree(s->snap[i]);
     }
+    if (s->side) (void)hipStreamSynchronize(s->side);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return VK_OK;
@@ -1354,6 +1405,9 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
         return rc;
     };
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&s->ev_start) != hipSuccess || hipEventCreate(&s->ev_stop) != hipSuccess ||
         hipMalloc(&s->st, sizeof(DevState)) != hipSuccess ||
         hipHostMalloc(&s->host_state, sizeof(DevState)) != hipSuccess ||

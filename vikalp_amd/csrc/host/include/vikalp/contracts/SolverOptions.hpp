@@ -45,9 +45,10 @@ struct SolverOptions {
     // > 0: stop when the relative KKT error (original coordinates) is at or
     // below this, instead of the reference's scaled infinity-norm test.
     Scalar relative_kkt_tolerance = 0.0;
-    // Mixed precision: relative KKT target of the FP32 phase.
+    // Mixed precision: relative KKT target of the FP32 phase (at least 1e-5
+    // under the Halpern policy, whose FP32 phase stalls near 1e-4 to 1e-5).
     Scalar mixed_phase_tolerance = 1e-4;
-    // Adaptive policy: iterations between restart checks.
+    // Adaptive and Halpern policies: iterations between restart checks.
     Index restart_period = 64;
     // Run whole iterations on the device (two fused kernels) instead of the
     // reference's op-by-op ExecutionBackend sequence.

@@ -16,6 +16,8 @@ namespace vikalp::solver::detail {
 
 using Clock = std::chrono::steady_clock;
 constexpr int kRuizPasses = 10;
+// Smallest relative KKT target of a Halpern FP32 phase (see solve_lp's mixed driver).
+constexpr Scalar kHalpernFp32Floor = 1e-5;
 
 struct Bounds {
     std::vector<Scalar> lower, upper;
