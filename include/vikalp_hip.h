@@ -121,7 +121,10 @@ enum {
 enum {
     VK_PDHG_RUNNING = 0,
     VK_PDHG_CONVERGED = 1,         /* termination test met                   */
-    VK_PDHG_ITERATION_LIMIT = 2
+    VK_PDHG_ITERATION_LIMIT = 2,
+    VK_PDHG_PRIMAL_INFEASIBLE = 3  /* adaptive / Halpern: the dual iterate
+                                      y is a Farkas certificate (its dual
+                                      objective without c is positive)     */
 };
 
 enum {

@@ -167,3 +167,20 @@ def test_automatic_layout_at_scale_matches_oracle():
     assert prof["col_blocks"] == 4 and prof["split_passes"] == 1
     # the row pass is its own kernel; the A' side is one kernel (k_primal_own)
     assert prof["dual_pass_ms"] > 0 and prof["primal_spmv_ms"] > 0 and prof["primal_pass_ms"] == 0
+
+
+@pytest.mark.parametrize("policy", [V.PdhgPolicy.Reference, V.PdhgPolicy.Halpern])
+def test_cache_budget_changes_load_hints_only(policy):
+    # The MALL budget picks cache-allocating or non-temporal loads per tile;
+    # every budget (none, tiny, partial, automatic) must give bitwise-equal
+    # results on the column-blocked split layout.
+    lp = V.generate_lp(1, 100_000, 300_000, 5, 7)
+    print(f"cache budget policy={policy.name}", flush=True)
+    opts = dict(pdhg_policy=policy, iteration_limit=300)
+    if policy != V.PdhgPolicy.Reference:
+        opts["relative_kkt_tolerance"] = 1e-12
+    runs = [V.solve_lp(lp, V.Options(spmv_cache_budget_mb=b, **opts)) for b in (-1, 1, 8, 0)]
+    for r in runs[1:]:
+        assert r.iterations == runs[0].iterations
+        assert np.array_equal(r.x, runs[0].x) and np.array_equal(r.y, runs[0].y)
+    assert V.solve_lp(lp, V.Options(spmv_cache_budget_mb=-2, **opts)).status == "InvalidModel"

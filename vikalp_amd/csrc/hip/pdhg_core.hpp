@@ -34,13 +34,16 @@ struct DevState {
     double cur_pr2, cur_dr2, cur_gap, cur_dx2, cur_dy2;
     double kkt_ref, kkt_prev;   // adaptive: weighted KKT at the restart / last check;
                                 // Halpern: fixed-point residual at the restart / last check
+    // Row part of the dual objective of the last dual iterate, sum of its
+    // term magnitudes and count of -inf terms (primal infeasibility test).
+    double d0_rows, d0_rows_abs, d0_rows_ninf;
 };
 
 // Partial-slot layout (slot 0 combines with max, the rest with +).
 //   rows: 0 max|P(ax)-ax| (scaled)   1 sum (orig primal residual)^2
 //         2 sum (scaled primal residual)^2   3 dual-objective row terms
 //         4 count of -inf dual terms   5 sum (y - y_last)^2
-//         6 (Halpern) sum (y+ - y_k)^2
+//         6 (Halpern) sum (y+ - y_k)^2   7 sum |dual-objective row terms|
 //   cols: 0 max|x-P(x-g)| (scaled)   1 sum (orig dual residual)^2
 //         2 sum (scaled dual residual)^2   3 dual-objective column terms
 //         4 count of -inf dual terms   5 c'x   6 sum (x - x_last)^2
@@ -128,9 +131,9 @@ struct DualEpi {
         p.v[1] += du * du;
         p.v[2] += d * d;
         if (yn > 0.0) {
-            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= yn * hi;
+            if (isinf(hi)) p.v[4] += 1.0; else { p.v[3] -= yn * hi; p.v[7] += fabs(yn * hi); }
         } else if (yn < 0.0) {
-            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= yn * lo;
+            if (isinf(lo)) p.v[4] += 1.0; else { p.v[3] -= yn * lo; p.v[7] += fabs(yn * lo); }
         }
         if (b.ylast) {
             const double dy = yn - static_cast<double>(q.ylast);
@@ -201,9 +204,9 @@ struct HalpernDualEpi {
         p.v[1] += du * du;
         p.v[2] += d * d;
         if (yn > 0.0) {
-            if (isinf(hi)) p.v[4] += 1.0; else p.v[3] -= yn * hi;
+            if (isinf(hi)) p.v[4] += 1.0; else { p.v[3] -= yn * hi; p.v[7] += fabs(yn * hi); }
         } else if (yn < 0.0) {
-            if (isinf(lo)) p.v[4] += 1.0; else p.v[3] -= yn * lo;
+            if (isinf(lo)) p.v[4] += 1.0; else { p.v[3] -= yn * lo; p.v[7] += fabs(yn * lo); }
         }
         const double dz = yp - y_old;
         p.v[6] += dz * dz;
@@ -461,6 +464,9 @@ __device__ inline void finalize_step(DevState &S, const double *red_rows, const 
     S.cur_gap = gap_abs;
     S.cur_dx2 = red_cols[6];
     S.cur_dy2 = red_rows[5];
+    S.d0_rows = red_rows[3];
+    S.d0_rows_abs = red_rows[7];
+    S.d0_rows_ninf = red_rows[4];
     if (S.policy == VK_PDHG_POLICY_REFERENCE) {
         // pdhg.cpp:133-145; the restart flag was consumed by this iteration's
         // dual step (pdhg.cpp:98-100).

@@ -38,6 +38,9 @@ Scaled scale(const Model &model, const Bounds &bounds);                   // con
 void equilibrate(const CsrPattern &p, Scaled &sc, int ruiz);              // adaptive policy only
 Scalar operator_bound(const CsrPattern &p, const std::vector<Scalar> &v); // continuous_common.hpp:207-225
 Scalar row_bound_norm(const std::vector<Scalar> &lo, const std::vector<Scalar> &hi);
+// Whether y (reference sign, original coordinates) proves the LP primal
+// infeasible (Farkas); d0 = its dual objective without c.
+bool farkas_certificate(const Model &model, const Bounds &bounds, const std::vector<Scalar> &y, Scalar &d0);
 Scalar norm2(const std::vector<Scalar> &v);
 void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
                const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,

@@ -213,6 +213,40 @@ Scalar operator_bound(const CsrPattern &p, const std::vector<Scalar> &v) {
     return std::sqrt(max_row * max_col);
 }
 
+// Farkas test of a dual vector y (reference sign, original coordinates):
+// D0(y) = sum_j min(g_j l_j, g_j u_j) - sum_i (max(y_i,0) ru_i -
+// max(-y_i,0) rl_i) with g = A'y.  A feasible LP has D0(y) <= 0 for every y,
+// so D0(y) > 0 with every term finite proves primal infeasibility.  Returns
+// whether y passes with D0 above kFarkasMargin times the sum of the term
+// magnitudes (far above FP64 rounding).
+bool farkas_certificate(const Model &model, const Bounds &bounds, const std::vector<Scalar> &y, Scalar &d0) {
+    constexpr Scalar kFarkasMargin = 1e-9;
+    const auto &A = model.constraint_matrix;
+    const auto m = static_cast<std::size_t>(A.pattern.rows);
+    const auto n = static_cast<std::size_t>(A.pattern.columns);
+    std::vector<Scalar> g(n, 0.0);
+    for (std::size_t i = 0; i < m; ++i)
+        for (Index k = A.pattern.row_offsets[i]; k < A.pattern.row_offsets[i + 1]; ++k)
+            g[static_cast<std::size_t>(A.pattern.column_indices[k])] += y[i] * A.values[static_cast<std::size_t>(k)];
+    Scalar sum = 0.0, mag = 0.0;
+    for (std::size_t j = 0; j < n; ++j) {
+        if (g[j] == 0.0) continue;
+        const Scalar bound = g[j] > 0.0 ? bounds.lower[j] : bounds.upper[j];
+        if (!std::isfinite(bound)) return false;
+        sum += g[j] * bound;
+        mag += std::abs(g[j] * bound);
+    }
+    for (std::size_t i = 0; i < m; ++i) {
+        if (y[i] == 0.0) continue;
+        const Scalar bound = y[i] > 0.0 ? model.constraint_upper[i] : model.constraint_lower[i];
+        if (!std::isfinite(bound)) return false;
+        sum -= y[i] * bound;
+        mag += std::abs(y[i] * bound);
+    }
+    d0 = sum;
+    return sum > 0.0 && sum > kFarkasMargin * mag;
+}
+
 // The finite row bound magnitude per row (relative-KKT denominator).
 Scalar row_bound_norm(const std::vector<Scalar> &lo, const std::vector<Scalar> &hi) {
     Scalar s = 0.0;
@@ -508,7 +542,7 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
         res.low_precision_iterations = state.iterations;
         Index used = state.iterations;
         double total_ms = ms;
-        if (!timed && used < options.iteration_limit) {
+        if (!timed && used < options.iteration_limit && status != VK_PDHG_PRIMAL_INFEASIBLE) {
             const auto x32 = xs, y32 = ys;
             vk_pdhg_params p64 = p;
             p64.precision = VK_PREC_FP64;
@@ -534,6 +568,22 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
     for (std::size_t j = 0; j < xs.size(); ++j) x[j] = xs[j] * sc.s[j];
     for (std::size_t i = 0; i < ys.size(); ++i) y[i] = ys[i] * sc.r[i];
     postsolve(model, bounds, options, x, y, options.relative_kkt_tolerance > 0.0, res);
+    if (status == VK_PDHG_PRIMAL_INFEASIBLE && !timed) {
+        // The device found a Farkas ray in the dual iterate; re-check it in
+        // FP64 on the original data before reporting infeasibility.
+        Scalar d0 = 0.0;
+        if (farkas_certificate(model, bounds, y, d0)) {
+            res.status = SolveStatus::Infeasible;
+            res.message = "primal infeasible: the returned row duals are a Farkas ray (dual objective without c = " +
+                          std::to_string(d0) + " > 0)";
+            res.objective_value = std::numeric_limits<Scalar>::infinity();
+            res.primal_bound = std::numeric_limits<Scalar>::infinity();
+            res.dual_bound = std::numeric_limits<Scalar>::infinity();
+        } else {
+            res.status = SolveStatus::NumericalFailure;
+            res.message = "the device's primal infeasibility certificate did not pass the FP64 check";
+        }
+    }
     return res;
 }
 
