@@ -131,13 +131,14 @@ def measure_batch(V, np, args):
         res = V.solve_batch(lps, o)
         wall = time.time() - t0
         optimal = sum(r.status == "Optimal" for r in res)
+        infeasible = sum(r.status == "Infeasible" for r in res)
         its = np.array([r.iterations for r in res])
         dev = res[0].device_seconds
-        out[label] = {"device_seconds": dev, "wall_seconds": wall, "optimal": optimal,
-                      "optimal_per_second": optimal / dev if dev > 0 else None,
+        out[label] = {"device_seconds": dev, "wall_seconds": wall, "optimal": optimal, "infeasible": infeasible,
+                      "decided_per_second": (optimal + infeasible) / dev if dev > 0 else None,
                       "iterations_median": float(np.median(its)), "iterations_max": int(its.max())}
-        log(f"batch {label}: {optimal}/{len(lps)} optimal, device {dev:.2f}s, wall {wall:.2f}s, "
-            f"iterations median {np.median(its):.0f} max {its.max()}")
+        log(f"batch {label}: {optimal}/{len(lps)} optimal, {infeasible} proved infeasible, device {dev:.2f}s, "
+            f"wall {wall:.2f}s, iterations median {np.median(its):.0f} max {its.max()}")
     return out
 
 

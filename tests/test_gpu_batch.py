@@ -83,17 +83,21 @@ def test_invalid_instance_does_not_stop_the_batch():
 
 
 def test_mixed_batch_gives_every_instance_its_own_budget():
-    # One instance cannot converge (contradictory rows: x0 = 1 and x0 = 2);
-    # it uses its whole budget in the FP32 phase.  The others must still get
-    # their FP64 refinement and converge.
+    # A feasible but slow instance (block-angular instance 345 of the batched
+    # benchmark, relative KKT ~1e-5 after 40,000 iterations) uses its whole
+    # budget; an instance with contradictory rows (x0 = 1 and x0 = 2) is
+    # proved infeasible early.  The others must still get their FP64
+    # refinement and converge.
     bad = V.LP(2, 1, np.array([0, 1, 2]), np.array([0, 0]), np.array([1.0, 1.0]), np.array([1.0]),
                np.array([1.0, 2.0]), np.array([1.0, 2.0]), np.array([0.0]), np.array([10.0]))
+    slow = V.generate_lp(4, 0, 0, 7, 20261203 + 345)
     good = [V.generate_lp(1, 40, 90, 3, 3), V.generate_lp(4, 0, 0, 4, 9)]
     opts = V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=V.PdhgPrecision.Mixed,
                      relative_kkt_tolerance=1e-6, iteration_limit=20_000)
-    got = V.solve_batch([good[0], bad, good[1]], opts)
-    assert got[1].status == "IterationLimit" and got[1].iterations == 20_000
-    for lp, g in ((good[0], got[0]), (good[1], got[2])):
+    got = V.solve_batch([good[0], bad, slow, good[1]], opts)
+    assert got[1].status == "Infeasible" and got[1].iterations < 20_000
+    assert got[2].status == "IterationLimit" and got[2].iterations == 20_000
+    for lp, g in ((good[0], got[0]), (good[1], got[3])):
         assert g.status == "Optimal", g
         rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), g.x, g.y)
         assert max(rp, rd, gp) <= 1e-6 * 1.0001

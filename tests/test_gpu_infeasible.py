@@ -73,3 +73,16 @@ def test_reference_policy_does_not_detect():
     lp = small_infeasible()
     r = V.solve_lp(lp, V.Options(iteration_limit=2000))
     assert r.status == "IterationLimit"
+
+
+@pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
+def test_batch_flags_infeasible_instances(precision):
+    feasible = [V.generate_lp(1, 40, 90, 3, 3), V.generate_lp(4, 0, 0, 4, 9)]
+    lps = [feasible[0], branched_infeasible(), small_infeasible(), feasible[1]]
+    res = V.solve_batch(lps, V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
+                                       relative_kkt_tolerance=1e-6, iteration_limit=100_000))
+    assert [r.status for r in res] == ["Optimal", "Infeasible", "Infeasible", "Optimal"], [r.status for r in res]
+    for lp, r in zip(lps[1:3], res[1:3]):
+        d0 = farkas_d0(lp, r.y)
+        assert np.isfinite(d0) and d0 > 0, d0
+        assert r.iterations < 100_000

@@ -215,6 +215,14 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
             continue;
         }
         if (S.status != VK_PDHG_RUNNING) continue;
+        // Primal infeasibility test on this iteration's y+ (A' y+ is in aty).
+        p.zero();
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+            farkas_column(static_cast<double>(b.aty[j]), static_cast<double>(b.l[j]), static_cast<double>(b.u[j]), p);
+        reduce_slots(p, red_c, scratch);
+        if (tid == 0) infeasibility_rule(S, red_c);
+        __syncthreads();
+        if (S.status != VK_PDHG_RUNNING) continue;
         const double inv_ns = 1.0 / static_cast<double>(S.ns);
         EvalRowsEpi<T> er{b, inv_ns};
         p.zero();

@@ -370,39 +370,19 @@ __global__ __launch_bounds__(kBlock) void k_apply(Bufs<T> b) {
 }
 
 // Primal infeasibility test at a restart check of the adaptive and Halpern
-// policies (Farkas): for any y, with g = A'y and D0(y) = sum_j min(g_j l_j,
-// g_j u_j) - sum_i (max(y_i,0) ru_i - max(-y_i,0) rl_i), a feasible LP has
-// D0(y) <= 0 (take a feasible x: min_x g'x <= y'Ax <= max_t y't).  So
-// D0(y) > 0 with every term finite proves the LP infeasible.  On an
-// infeasible LP the PDHG dual iterate grows along such a ray, so D0 of the
-// last dual iterate y+ becomes positive.  The row part comes from the last
-// iteration's dual pass (DevState::d0_rows*); this pass adds the column part
-// from g = A' y+ (partials: 1 sum of terms, 2 count of -inf terms,
-// 3 sum of term magnitudes).  Scaling leaves D0 unchanged.
+// policies (farkas_column / infeasibility_rule): the column part of D0 of the
+// last dual iterate y+ from g = A' y+, then the decision in one block.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_infeas_cols(Bufs<T> b, const T *__restrict__ g) {
     if (b.st->status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
     Partials part;
     part.zero();
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
-        const double gj = static_cast<double>(__builtin_nontemporal_load(g + j));
-        const double bound = gj > 0.0 ? static_cast<double>(b.l[j]) : static_cast<double>(b.u[j]);
-        if (gj == 0.0) continue;
-        if (isinf(bound)) {
-            part.v[2] += 1.0;
-        } else {
-            part.v[1] += gj * bound;
-            part.v[3] += fabs(gj * bound);
-        }
-    }
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock)
+        farkas_column(static_cast<double>(__builtin_nontemporal_load(g + j)), static_cast<double>(b.l[j]),
+                      static_cast<double>(b.u[j]), part);
     store_partials(part, b.part_ecols, scratch);
 }
-
-// Relative margin of the device test: D0 must exceed this fraction of the
-// sum of its term magnitudes, far above the rounding of the FP32 or FP64
-// terms (the host re-checks the certificate in FP64, original coordinates).
-constexpr double kInfeasMargin = 1e-6;
 
 template <class T>
 __global__ __launch_bounds__(kFinalBlock) void k_infeas_check(Bufs<T> b) {
@@ -410,12 +390,9 @@ __global__ __launch_bounds__(kFinalBlock) void k_infeas_check(Bufs<T> b) {
     __shared__ double red[kSlots];
     reduce_partials(b.part_ecols, b.grid_cols, red, scratch);
     if (threadIdx.x != 0) return;
-    const DevState S = *b.st;
-    if (S.status != VK_PDHG_RUNNING) return;
-    const double d0 = S.d0_rows + red[1];
-    const double scale = S.d0_rows_abs + red[3];
-    if (S.d0_rows_ninf == 0.0 && red[2] == 0.0 && d0 > 0.0 && d0 > kInfeasMargin * scale)
-        b.st->status = VK_PDHG_PRIMAL_INFEASIBLE;
+    DevState S = *b.st;
+    infeasibility_rule(S, red);
+    if (S.status == VK_PDHG_PRIMAL_INFEASIBLE) b.st->status = S.status;
 }
 
 // Halpern policy, after the finalizer of a restart-check iteration (the

@@ -412,6 +412,37 @@ __device__ __forceinline__ bool may_fuse(const DevState &s) {
     return (s.k + 1) % s.period != 0;
 }
 
+// Primal infeasibility test (Farkas).  For any y, with g = A'y and
+// D0(y) = sum_j min(g_j l_j, g_j u_j) - sum_i (max(y_i,0) ru_i -
+// max(-y_i,0) rl_i), a feasible LP has D0(y) <= 0 (take a feasible x:
+// min_x g'x <= y'Ax <= max_t y't), so D0(y) > 0 with every term finite proves
+// it infeasible.  On an infeasible LP the PDHG dual iterate grows along such
+// a ray.  The finalizer records the row part of the last dual iterate
+// (DevState::d0_rows*); farkas_column adds the column part of g = A'y+
+// (partial slots 1 sum of terms, 2 count of -inf terms, 3 sum of term
+// magnitudes); infeasibility_rule decides with a relative margin far above
+// the rounding of FP32 or FP64 terms.  Scaling leaves D0 unchanged.
+constexpr double kInfeasMargin = 1e-6;
+
+__device__ __forceinline__ void farkas_column(double gj, double lj, double uj, Partials &p) {
+    if (gj == 0.0) return;
+    const double bound = gj > 0.0 ? lj : uj;
+    if (isinf(bound)) {
+        p.v[2] += 1.0;
+    } else {
+        p.v[1] += gj * bound;
+        p.v[3] += fabs(gj * bound);
+    }
+}
+
+__device__ inline void infeasibility_rule(DevState &S, const double *red) {
+    if (S.status != VK_PDHG_RUNNING) return;
+    const double d0 = S.d0_rows + red[1];
+    const double scale = S.d0_rows_abs + red[3];
+    if (S.d0_rows_ninf == 0.0 && red[2] == 0.0 && d0 > 0.0 && d0 > kInfeasMargin * scale)
+        S.status = VK_PDHG_PRIMAL_INFEASIBLE;
+}
+
 // Halpern restart rule at the end of iteration S.k, from the reduced slots
 // (before S.k and S.ks advance).  The fixed-point residual
 // r = |z_k - T(z_k)| in the primal-weight norm (w |dx|^2 + |dy|^2 / w)^(1/2)

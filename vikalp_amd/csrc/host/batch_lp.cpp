@@ -169,6 +169,10 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     double ms = 0.0, total_ms = 0.0;
     bool timed = false;
     std::vector<Index> used(prep.size(), 0), low(prep.size(), 0);
+    // Instances the FP32 phase of a mixed batch proved infeasible, and their
+    // scaled row duals (the certificate) from that phase.
+    std::vector<char> infeasible32(prep.size(), 0);
+    std::vector<double> ys32;
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
         const double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
         run_batch(models, prep, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr,
@@ -184,9 +188,11 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             Index max_remaining = 0;
             for (std::size_t i = 0; i < prep.size(); ++i) {
                 w[i] = states[i].primal_weight;
-                remaining[i] = std::max<Index>(0, options.iteration_limit - used[i]);
+                infeasible32[i] = states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
+                remaining[i] = infeasible32[i] ? 0 : std::max<Index>(0, options.iteration_limit - used[i]);
                 max_remaining = std::max<Index>(max_remaining, remaining[i]);
             }
+            ys32 = ys;
             const auto x32 = xs, y32 = ys;
             run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32,
                       &y32, options.pdhg_policy == PdhgPolicy::Adaptive ? &w : nullptr, &remaining, started,
@@ -211,12 +217,15 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
         res.iterations = used[i];
         res.low_precision_iterations = low[i];
         res.device_seconds = total_ms * 1e-3;
-        if (timed && states[i].status == VK_PDHG_RUNNING) res.status = SolveStatus::TimeLimit;
-        else if (states[i].status == VK_PDHG_CONVERGED) res.status = SolveStatus::Optimal;
+        const bool infeasible = infeasible32[i] || states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
+        if (timed && states[i].status == VK_PDHG_RUNNING && !infeasible) res.status = SolveStatus::TimeLimit;
+        else if (states[i].status == VK_PDHG_CONVERGED && !infeasible) res.status = SolveStatus::Optimal;
+        const std::vector<double> &ysrc = infeasible32[i] ? ys32 : ys;
         std::vector<Scalar> x(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(m));
         for (Index j = 0; j < n; ++j) x[j] = xs[col0 + j] * sc.s[j];
-        for (Index r = 0; r < m; ++r) y[r] = ys[row0 + r] * sc.r[r];
+        for (Index r = 0; r < m; ++r) y[r] = ysrc[row0 + r] * sc.r[r];
         postsolve(model, prep[i].bounds, options, x, y, options.relative_kkt_tolerance > 0.0, res);
+        if (infeasible) report_infeasible(model, prep[i].bounds, y, res);
         res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
         results[prep[i].index] = std::move(res);
         row0 += m;

@@ -41,6 +41,9 @@ Scalar row_bound_norm(const std::vector<Scalar> &lo, const std::vector<Scalar> &
 // Whether y (reference sign, original coordinates) proves the LP primal
 // infeasible (Farkas); d0 = its dual objective without c.
 bool farkas_certificate(const Model &model, const Bounds &bounds, const std::vector<Scalar> &y, Scalar &d0);
+// Status Infeasible with y as the certificate, or NumericalFailure if y fails
+// farkas_certificate (a device-side detection re-checked in FP64).
+void report_infeasible(const Model &model, const Bounds &bounds, const std::vector<Scalar> &y, SolveResult &res);
 Scalar norm2(const std::vector<Scalar> &v);
 void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
                const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,

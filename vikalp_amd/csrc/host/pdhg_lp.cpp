@@ -247,6 +247,24 @@ bool farkas_certificate(const Model &model, const Bounds &bounds, const std::vec
     return sum > 0.0 && sum > kFarkasMargin * mag;
 }
 
+// After postsolve of a solve the device stopped as primal infeasible:
+// re-check the returned row duals in FP64 on the original data, then report
+// SolveStatus::Infeasible (or NumericalFailure if the ray does not pass).
+void report_infeasible(const Model &model, const Bounds &bounds, const std::vector<Scalar> &y, SolveResult &res) {
+    Scalar d0 = 0.0;
+    if (farkas_certificate(model, bounds, y, d0)) {
+        res.status = SolveStatus::Infeasible;
+        res.message = "primal infeasible: the returned row duals are a Farkas ray (dual objective without c = " +
+                      std::to_string(d0) + " > 0)";
+        res.objective_value = std::numeric_limits<Scalar>::infinity();
+        res.primal_bound = std::numeric_limits<Scalar>::infinity();
+        res.dual_bound = std::numeric_limits<Scalar>::infinity();
+    } else {
+        res.status = SolveStatus::NumericalFailure;
+        res.message = "the device's primal infeasibility certificate did not pass the FP64 check";
+    }
+}
+
 // The finite row bound magnitude per row (relative-KKT denominator).
 Scalar row_bound_norm(const std::vector<Scalar> &lo, const std::vector<Scalar> &hi) {
     Scalar s = 0.0;
@@ -568,22 +586,7 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
     for (std::size_t j = 0; j < xs.size(); ++j) x[j] = xs[j] * sc.s[j];
     for (std::size_t i = 0; i < ys.size(); ++i) y[i] = ys[i] * sc.r[i];
     postsolve(model, bounds, options, x, y, options.relative_kkt_tolerance > 0.0, res);
-    if (status == VK_PDHG_PRIMAL_INFEASIBLE && !timed) {
-        // The device found a Farkas ray in the dual iterate; re-check it in
-        // FP64 on the original data before reporting infeasibility.
-        Scalar d0 = 0.0;
-        if (farkas_certificate(model, bounds, y, d0)) {
-            res.status = SolveStatus::Infeasible;
-            res.message = "primal infeasible: the returned row duals are a Farkas ray (dual objective without c = " +
-                          std::to_string(d0) + " > 0)";
-            res.objective_value = std::numeric_limits<Scalar>::infinity();
-            res.primal_bound = std::numeric_limits<Scalar>::infinity();
-            res.dual_bound = std::numeric_limits<Scalar>::infinity();
-        } else {
-            res.status = SolveStatus::NumericalFailure;
-            res.message = "the device's primal infeasibility certificate did not pass the FP64 check";
-        }
-    }
+    if (status == VK_PDHG_PRIMAL_INFEASIBLE && !timed) report_infeasible(model, bounds, y, res);
     return res;
 }
 
