@@ -10,7 +10,8 @@ Not in the reference, so pinned three ways:
 - the relative-KKT certificate recomputed by the oracle on the returned
   point and the objective against the independent HiGHS optimum, in FP64
   and mixed precision;
-- bitwise repeats, and the refusal of paths that do not implement it.
+- bitwise repeats, the batched kernel's per-instance certificates, and the
+  refusal of the op-by-op path, which does not implement it.
 """
 import ctypes as C
 import json
@@ -175,9 +176,22 @@ def test_halpern_split_layout_certificate():
     assert max(rp, rd, gp) <= tol * 1.0001
 
 
-def test_halpern_refused_where_not_implemented():
+def test_halpern_refused_on_the_op_by_op_path():
     lp = V.generate_lp(1, 40, 90, 3, 3)
     o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=1e-6, fused_iterations=False)
-    assert V.solve_lp(lp, o).status == "UnsupportedModel"          # op-by-op path
-    res = V.solve_batch([lp, lp], V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=1e-6))
-    assert [r.status for r in res] == ["UnsupportedModel"] * 2
+    assert V.solve_lp(lp, o).status == "UnsupportedModel"
+
+
+@pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
+def test_halpern_batch_matches_single_solves(precision):
+    # The batched kernel runs the same Halpern iteration per instance (its
+    # own step and restart state): every instance meets the relative-KKT
+    # certificate recomputed by the oracle.
+    lps = [V.generate_lp(1, 40, 90, 3, 3), V.generate_lp(4, 0, 0, 4, 9), V.generate_lp(2, 400, 1000, 0, 5)]
+    o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision, relative_kkt_tolerance=1e-6,
+                  iteration_limit=100_000)
+    res = V.solve_batch(lps, o)
+    for lp, r in zip(lps, res):
+        assert r.status == "Optimal", r
+        rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), r.x, r.y)
+        assert max(rp, rd, gp) <= 1e-6 * 1.0001, (rp, rd, gp)

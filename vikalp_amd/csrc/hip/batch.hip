@@ -169,6 +169,18 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
         for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) b.ysum[i] = 0.0;
         __syncthreads();
     }
+    if (S.policy == VK_PDHG_POLICY_HALPERN) {
+        // z0 = (x0, y0): Halpern iterate, anchor and restart point.
+        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) {
+            b.xh[j] = b.x0h[j] = b.xlast[j] = b.x[0][j];
+            b.aty0h[j] = b.aty[j];
+        }
+        for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) {
+            b.yh[i] = b.y0h[i] = b.ylast[i] = b.y[i];
+            b.ax0h[i] = b.ax[i];
+        }
+        __syncthreads();
+    }
     if (S.status == VK_PDHG_RUNNING)
         for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) pstep_element(b, S, j, b.x[0], b.x[1]);
     __syncthreads();
@@ -194,15 +206,27 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
         if (S.status != VK_PDHG_RUNNING) break;
         const long long k = S.k;
         const bool ref = S.policy == VK_PDHG_POLICY_REFERENCE;
+        const bool hal = S.policy == VK_PDHG_POLICY_HALPERN;
+        const bool fuse = may_fuse(S);
         Partials p;
         p.zero();
-        DualEpi<T> de{b, S.sigma, S.inv_sigma, ref && S.restart_flag, !ref};
-        tile_spmv_range(a.A, b.x[(k + 1) & 1], de, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        if (hal) {
+            HalpernDualEpi<T> de{b, S.sigma, S.inv_sigma, halpern_weights(S.ks), check_iteration(S)};
+            tile_spmv_range(a.A, b.x[(k + 1) & 1], de, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        } else {
+            DualEpi<T> de{b, S.sigma, S.inv_sigma, ref && S.restart_flag, !ref};
+            tile_spmv_range(a.A, b.x[(k + 1) & 1], de, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        }
         reduce_slots(p, red_r, scratch);
-        const bool fuse = may_fuse(S);
-        PrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse};
         p.zero();
-        tile_spmv_range(a.AT, b.y, pe, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        if (hal) {
+            HalpernPrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse, check_iteration(S),
+                                   halpern_weights(S.ks)};
+            tile_spmv_range(a.AT, b.y, pe, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        } else {
+            PrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse};
+            tile_spmv_range(a.AT, b.y, pe, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        }
         reduce_slots(p, red_c, scratch);
         if (tid == 0) finalize_step(S, red_r, red_c);
         __syncthreads();
@@ -215,14 +239,25 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
             continue;
         }
         if (S.status != VK_PDHG_RUNNING) continue;
-        // Primal infeasibility test on this iteration's y+ (A' y+ is in aty).
+        // Primal infeasibility test on this iteration's y+ (A' y+ is in aty,
+        // or kept in aty_avg by the Halpern pass).
         p.zero();
-        for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
-            farkas_column(static_cast<double>(b.aty[j]), static_cast<double>(b.l[j]), static_cast<double>(b.u[j]), p);
+        {
+            const T *g = hal ? b.aty_avg : b.aty;
+            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+                farkas_column(static_cast<double>(g[j]), static_cast<double>(b.l[j]), static_cast<double>(b.u[j]), p);
+        }
         reduce_slots(p, red_c, scratch);
         if (tid == 0) infeasibility_rule(S, red_c);
         __syncthreads();
         if (S.status != VK_PDHG_RUNNING) continue;
+        if (hal) {
+            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+                halpern_apply_column(b, S, j, b.x[S.k & 1], b.x[(S.k + 1) & 1]);
+            for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) halpern_apply_row(b, S, i);
+            __syncthreads();
+            continue;
+        }
         const double inv_ns = 1.0 / static_cast<double>(S.ns);
         EvalRowsEpi<T> er{b, inv_ns};
         p.zero();
@@ -312,10 +347,20 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
     if (h->params.policy == VK_PDHG_POLICY_ADAPTIVE) {
         b.xsum = h->alloc<double>(n);
         b.ysum = h->alloc<double>(m);
+    }
+    if (h->params.policy != VK_PDHG_POLICY_REFERENCE) {
         b.xlast = h->alloc<T>(n);
         b.ylast = h->alloc<T>(m);
         b.ax_avg = h->alloc<T>(m);
         b.aty_avg = h->alloc<T>(n);
+    }
+    if (h->params.policy == VK_PDHG_POLICY_HALPERN) {
+        b.xh = h->alloc<T>(n);
+        b.yh = h->alloc<T>(m);
+        b.x0h = h->alloc<T>(n);
+        b.y0h = h->alloc<T>(m);
+        b.ax0h = h->alloc<T>(m);
+        b.aty0h = h->alloc<T>(n);
     }
     for (void *p : h->allocations)
         if (!p) {
@@ -389,7 +434,8 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     VK_REQUIRE(out && params && count > 0 && row_starts && col_starts && row_offsets,
                "vk_hip_batch_create: bad arguments");
     *out = nullptr;
-    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE,
+    VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE ||
+                   params->policy == VK_PDHG_POLICY_HALPERN,
                "vk_hip_batch_create: unknown policy");
     VK_REQUIRE(params->precision == VK_PREC_FP64 || params->precision == VK_PREC_MIXED,
                "vk_hip_batch_create: unknown precision");
@@ -525,7 +571,7 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     for (int i = 0; i < count; ++i) {
         DevState &S = h->states_host[i];
         S.limit = iteration_limits ? iteration_limits[i] : params->iteration_limit;
-        S.period = params->policy == VK_PDHG_POLICY_ADAPTIVE ? params->restart_period : kDecisionPeriod;
+        S.period = params->policy == VK_PDHG_POLICY_REFERENCE ? kDecisionPeriod : params->restart_period;
         S.status = S.limit == 0 ? VK_PDHG_ITERATION_LIMIT : VK_PDHG_RUNNING;
         S.policy = params->policy;
         S.metric_tol = params->metric_tolerance;
@@ -535,7 +581,7 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
         S.prev_metric = S.metric = S.rel_kkt = std::numeric_limits<double>::infinity();
         S.kkt_ref = S.kkt_prev = std::numeric_limits<double>::infinity();
         const double step = step_sizes ? step_sizes[i] : 0.0;   // <= 0: power iteration
-        S.omega = (params->policy == VK_PDHG_POLICY_ADAPTIVE && primal_weights) ? primal_weights[i] : 1.0;
+        S.omega = (params->policy != VK_PDHG_POLICY_REFERENCE && primal_weights) ? primal_weights[i] : 1.0;
         if (S.omega <= 0.0 || !std::isfinite(S.omega)) S.omega = 1.0;
         if (params->policy == VK_PDHG_POLICY_REFERENCE) {
             S.max_step = step;

@@ -408,29 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_halpern_apply(Bufs<T> b) {
     T *xnext = b.x[(s.k + 1) & 1];
     const int total = b.n > b.m ? b.n : b.m;
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
-        if (t < b.n) {
-            T xh = b.xh[t], aty = b.aty[t];
-            if (s.restart_apply) {
-                xh = xp[t];
-                aty = b.aty_avg[t];
-                b.xh[t] = xh;
-                b.aty[t] = aty;
-                b.x0h[t] = xh;
-                b.aty0h[t] = aty;
-                b.xlast[t] = xh;
-            }
-            xnext[t] = static_cast<T>(clamp_box((static_cast<double>(xh) - s.tau * static_cast<double>(b.c[t])) -
-                                                    s.tau * static_cast<double>(aty),
-                                                static_cast<double>(b.l[t]), static_cast<double>(b.u[t])));
-        }
-        if (t < b.m && s.restart_apply) {
-            const T yh = b.y[t], ax = b.ax_avg[t];
-            b.yh[t] = yh;
-            b.ax[t] = ax;
-            b.y0h[t] = yh;
-            b.ax0h[t] = ax;
-            b.ylast[t] = yh;
-        }
+        if (t < b.n) halpern_apply_column(b, s, t, xp, xnext);
+        if (t < b.m) halpern_apply_row(b, s, t);
     }
 }
 

@@ -618,6 +618,40 @@ __device__ __forceinline__ void apply_row(const Bufs<T> &b, const DevState &s, i
     }
 }
 
+// Halpern policy after a restart check (the primal pass did not fuse the
+// next step): on a restart z = z0 = T(z_k), i.e. x+ (xp) and the A' y+ kept
+// in aty_avg, which also become the primal weight's restart point; then the
+// next PDHG primal step x+ = P_X((xh - tau c) - tau A' yh) into xnext.
+template <class T>
+__device__ __forceinline__ void halpern_apply_column(const Bufs<T> &b, const DevState &s, int j, const T *xp,
+                                                     T *xnext) {
+    T xh = b.xh[j], aty = b.aty[j];
+    if (s.restart_apply) {
+        xh = xp[j];
+        aty = b.aty_avg[j];
+        b.xh[j] = xh;
+        b.aty[j] = aty;
+        b.x0h[j] = xh;
+        b.aty0h[j] = aty;
+        b.xlast[j] = xh;
+    }
+    xnext[j] = static_cast<T>(clamp_box((static_cast<double>(xh) - s.tau * static_cast<double>(b.c[j])) -
+                                            s.tau * static_cast<double>(aty),
+                                        static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
+}
+
+// Row half of the restart: y = y0 = y+, A x = A x0 = the A x+ kept in ax_avg.
+template <class T>
+__device__ __forceinline__ void halpern_apply_row(const Bufs<T> &b, const DevState &s, int i) {
+    if (!s.restart_apply) return;
+    const T yh = b.y[i], ax = b.ax_avg[i];
+    b.yh[i] = yh;
+    b.ax[i] = ax;
+    b.y0h[i] = yh;
+    b.ax0h[i] = ax;
+    b.ylast[i] = yh;
+}
+
 // One element of the standalone primal step x[k+1] = P_X(x[k] - tau (c + aty))
 // (with Q: x[k] - tau (aty + (c + Q x[k])), pdqp.cpp:115-122).
 template <class T>

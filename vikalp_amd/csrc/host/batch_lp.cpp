@@ -81,7 +81,9 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
         }
     }
     vk_pdhg_params p{};
-    p.policy = options.pdhg_policy == PdhgPolicy::Adaptive ? VK_PDHG_POLICY_ADAPTIVE : VK_PDHG_POLICY_REFERENCE;
+    p.policy = options.pdhg_policy == PdhgPolicy::Adaptive  ? VK_PDHG_POLICY_ADAPTIVE
+               : options.pdhg_policy == PdhgPolicy::Halpern ? VK_PDHG_POLICY_HALPERN
+                                                            : VK_PDHG_POLICY_REFERENCE;
     p.precision = precision;
     p.iteration_limit = iteration_limit;
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
@@ -152,13 +154,8 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             results[i] = status_result(SolveStatus::InvalidModel, "solver limits and tolerances must be non-negative");
             continue;
         }
-        if (options.pdhg_policy == PdhgPolicy::Halpern) {
-            results[i] = status_result(SolveStatus::UnsupportedModel,
-                                       "the batched solver runs the reference and adaptive policies");
-            continue;
-        }
         pr.sc = scale(model, pr.bounds);
-        if (options.pdhg_policy == PdhgPolicy::Adaptive)
+        if (options.pdhg_policy != PdhgPolicy::Reference)
             equilibrate(model.constraint_matrix.pattern, pr.sc, kRuizPasses);
         prep.push_back(std::move(pr));
     }
@@ -174,7 +171,8 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     std::vector<char> infeasible32(prep.size(), 0);
     std::vector<double> ys32;
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
-        const double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
+        double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
+        if (options.pdhg_policy == PdhgPolicy::Halpern) phase_tol = std::max(phase_tol, kHalpernFp32Floor);
         run_batch(models, prep, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr,
                   nullptr, nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
@@ -195,7 +193,7 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             ys32 = ys;
             const auto x32 = xs, y32 = ys;
             run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32,
-                      &y32, options.pdhg_policy == PdhgPolicy::Adaptive ? &w : nullptr, &remaining, started,
+                      &y32, options.pdhg_policy != PdhgPolicy::Reference ? &w : nullptr, &remaining, started,
                       states, xs, ys, ms, timed);
             total_ms += ms;
             for (std::size_t i = 0; i < prep.size(); ++i) used[i] += states[i].iterations;
