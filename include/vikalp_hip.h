@@ -107,6 +107,44 @@ int vk_hip_spmv_csr(const vk_hip_csr *A, int transpose, double alpha,
  * seeded start vector (deterministic).  Result to host. */
 int vk_hip_csr_norm2_estimate(const vk_hip_csr *A, int iterations, double *result);
 
+/* ---- scaling and postsolve passes on the device --------------------------
+ * The O(nnz) host passes around the reference's PDHG loop, run on the
+ * device with the host loops' arithmetic in their order (one thread per row,
+ * no contraction), so the results are bitwise equal to those loops. */
+
+/* Replaces continuous_common.hpp:133-205 scale_linear_model (called at
+ * pdhg.cpp:25).  Scales A in place, starting from the values last given to
+ * vk_hip_csr_set_values (kept on the device for
+ * vk_hip_csr_postsolve_products); A' and the FP32 copies follow:
+ *   1. row i by r_i = 1/max_j |a_ij| when that maximum exceeds 1 (else 1),
+ *      column j by s_j likewise; c*s, l/s, u/s, rl*r, ru*r (infinite bounds
+ *      stay infinite);
+ *   2. `ruiz_passes` Ruiz passes: rows and columns divided by the square
+ *      root of their largest |a| (1 if zero);
+ *   3. if `pock_chambolle` != 0, one Pock-Chambolle pass (alpha = 1): by the
+ *      square root of their sum of |a| (1 if zero).
+ * Steps 2-3 divide r, s, c, rl, ru by the factors and multiply l, u.  Host
+ * arrays: c, l, u (n), rl, ru (m) in; row_scale (m), col_scale (n) and the
+ * scaled c_out, l_out, u_out (n), rl_out, ru_out (m) out. */
+int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const double *u,
+                    const double *rl, const double *ru, int ruiz_passes, int pock_chambolle,
+                    double *row_scale, double *col_scale, double *c_out, double *l_out,
+                    double *u_out, double *rl_out, double *ru_out);
+
+/* Replaces continuous_common.hpp:207-225 scaled_operator_bound (pdhg.cpp:75):
+ * sqrt(max_i sum_j |a_ij| * max_j sum_i |a_ij|) over A's current values. */
+int vk_hip_csr_operator_bound(const vk_hip_csr *A, double *bound);
+
+/* Replaces the O(nnz) loops of the reference postsolve: linear_values
+ * (continuous_common.hpp:302-315, called at pdhg.cpp:168) and the
+ * stationarity accumulation (pdhg.cpp:188-196), on the values last given to
+ * vk_hip_csr_set_values (unscaled):
+ *   ax = A x   (each row summed from 0 in storage order),
+ *   g  = c + A'y (each column from c_j, its rows in increasing order).
+ * Host arrays x, c, g (n) and y, ax (m). */
+int vk_hip_csr_postsolve_products(const vk_hip_csr *A, const double *x, const double *y,
+                                  const double *c, double *ax, double *g);
+
 /* ---- fused PDHG (new: the reference drives PDHG op by op, pdhg.cpp:87-151) */
 
 enum {

@@ -470,6 +470,7 @@ int vk_hip_csr_set_values(vk_hip_csr *A, const double *values) {
     }
     VK_HIP_CHECK(hipStreamSynchronize(s));
     A->has_values = true;
+    A->scaled = false;
     return VK_OK;
 }
 
@@ -480,6 +481,7 @@ int vk_hip_csr_destroy(vk_hip_csr *A) {
     free_csr(A->a);
     free_csr(A->at);
     (void)hipFree(A->tperm);
+    (void)hipFree(A->orig);
     delete A;
     return VK_OK;
 }

@@ -52,6 +52,10 @@ struct vk_hip_csr {
     vk::DeviceCsr<float> a32, at32;   // FP32 copies (made on demand)
     int *tperm = nullptr;             // at.val[k] = a.val[tperm[k]]
     bool has_values = false;
+    // Once vk_hip_lp_scale has replaced the values with scaled ones, `orig`
+    // holds the values last given to vk_hip_csr_set_values.
+    double *orig = nullptr;
+    bool scaled = false;
 };
 
 #define VK_HIP_CHECK(expr)                                                      \

@@ -222,7 +222,8 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
         std::vector<Scalar> x(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(m));
         for (Index j = 0; j < n; ++j) x[j] = xs[col0 + j] * sc.s[j];
         for (Index r = 0; r < m; ++r) y[r] = ysrc[row0 + r] * sc.r[r];
-        postsolve(model, prep[i].bounds, options, x, y, options.relative_kkt_tolerance > 0.0, res);
+        postsolve(model, prep[i].bounds, options, x, y, host_products(model, x, y),
+                  options.relative_kkt_tolerance > 0.0, res);
         if (infeasible) report_infeasible(model, prep[i].bounds, y, res);
         res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
         results[prep[i].index] = std::move(res);

@@ -310,6 +310,11 @@ const vk_hip_csr *hip_matrix_handle(const BackendMatrix &matrix) noexcept {
     return m ? m->handle() : nullptr;
 }
 
+vk_hip_csr *hip_matrix_handle(BackendMatrix &matrix) noexcept {
+    auto *m = dynamic_cast<HipMatrix *>(&matrix);
+    return m ? m->handle() : nullptr;
+}
+
 double *hip_vector_data(BackendVector &vector) noexcept {
     auto *v = dynamic_cast<HipVector *>(&vector);
     return v ? v->data() : nullptr;

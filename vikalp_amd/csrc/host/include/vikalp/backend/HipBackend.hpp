@@ -23,6 +23,8 @@ namespace vikalp {
 // device pointer behind a HipBackend matrix/vector (nullptr when the object
 // does not belong to a HipBackend).
 [[nodiscard]] const vk_hip_csr *hip_matrix_handle(const BackendMatrix &matrix) noexcept;
+// The same handle, writable: the fused path scales the matrix in place.
+[[nodiscard]] vk_hip_csr *hip_matrix_handle(BackendMatrix &matrix) noexcept;
 [[nodiscard]] double *hip_vector_data(BackendVector &vector) noexcept;
 
 }  // namespace vikalp

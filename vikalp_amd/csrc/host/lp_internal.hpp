@@ -6,6 +6,7 @@
 #include "vikalp/contracts/Model.hpp"
 #include "vikalp/contracts/SolveResult.hpp"
 #include "vikalp/contracts/SolverOptions.hpp"
+#include "vikalp_hip.h"
 
 #include <chrono>
 #include <span>
@@ -45,9 +46,22 @@ bool farkas_certificate(const Model &model, const Bounds &bounds, const std::vec
 // farkas_certificate (a device-side detection re-checked in FP64).
 void report_infeasible(const Model &model, const Bounds &bounds, const std::vector<Scalar> &y, SolveResult &res);
 Scalar norm2(const std::vector<Scalar> &v);
+// The O(nnz) part of postsolve on the unscaled A: ax = A x
+// (continuous_common.hpp:302-315) and g = c + A'y (pdhg.cpp:188-196).
+struct Products {
+    std::vector<Scalar> ax, g;
+};
+Products host_products(const Model &model, const std::vector<Scalar> &x, const std::vector<Scalar> &y);
+// The same sums on the device (vk_hip_csr_postsolve_products; bitwise equal).
+Products device_products(const vk_hip_csr *A, const Model &model, const std::vector<Scalar> &x,
+                         const std::vector<Scalar> &y);
+// scale() and, when `equilibrated`, equilibrate(.., kRuizPasses) on the
+// device (vk_hip_lp_scale; bitwise equal): A's device values become the
+// scaled ones and the returned Scaled has no host values.
+Scaled device_scale(vk_hip_csr *A, const Model &model, const Bounds &bounds, bool equilibrated);
 void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
-               const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,
-               SolveResult &res);                                          // pdhg.cpp:153-264
+               const std::vector<Scalar> &x, const std::vector<Scalar> &y, const Products &products,
+               bool relative_mode, SolveResult &res);                      // pdhg.cpp:153-264
 void check_vk(int rc, const char *what);
 // continuous_common.hpp:242-275 (dense) up to n = 4096; beyond that a
 // diagonally dominant Q with a non-negative diagonal.  `message` says why not.

@@ -292,21 +292,14 @@ Scalar norm2(const std::vector<Scalar> &v) {
 //   D(y)  = sum_j min(g_j l_j, g_j u_j) - sum_i (max(y_i,0) ru_i - max(-y_i,0) rl_i)
 // with 0*inf = 0; D = -inf gives gap = 1.
 void relative_kkt(const Model &model, const Bounds &bounds, const std::vector<Scalar> &x,
-                  const std::vector<Scalar> &y, SolveResult &res) {
-    const auto &A = model.constraint_matrix;
-    const auto m = static_cast<std::size_t>(A.pattern.rows);
-    const auto n = static_cast<std::size_t>(A.pattern.columns);
-    std::vector<Scalar> g(model.linear_objective);
+                  const std::vector<Scalar> &y, const Products &pr, SolveResult &res) {
+    const auto m = static_cast<std::size_t>(model.constraint_matrix.pattern.rows);
+    const auto n = static_cast<std::size_t>(model.constraint_matrix.pattern.columns);
+    const std::vector<Scalar> &g = pr.g;
     Scalar pres = 0.0, dual = 0.0;
     bool neg_inf = false;
     for (std::size_t i = 0; i < m; ++i) {
-        Scalar ax = 0.0;
-        for (Index k = A.pattern.row_offsets[i]; k < A.pattern.row_offsets[i + 1]; ++k) {
-            const auto kk = static_cast<std::size_t>(k);
-            const auto j = static_cast<std::size_t>(A.pattern.column_indices[kk]);
-            ax += A.values[kk] * x[j];
-            g[j] += A.values[kk] * y[i];
-        }
+        const Scalar ax = pr.ax[i];
         const Scalar lo = model.constraint_lower[i], hi = model.constraint_upper[i];
         const Scalar d = ax - std::min(std::max(ax, lo), hi);
         pres += d * d;
@@ -342,14 +335,59 @@ void relative_kkt(const Model &model, const Bounds &bounds, const std::vector<Sc
     res.kkt_relative = std::max({res.kkt_relative_primal, res.kkt_relative_dual, res.kkt_relative_gap});
 }
 
-// pdhg.cpp:153-264: objective, residuals, duals and the final status, in
-// original coordinates.  `x`, `y` are unscaled.
-void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
-               const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,
-               SolveResult &res) {
+Products host_products(const Model &model, const std::vector<Scalar> &x, const std::vector<Scalar> &y) {
     const auto &A = model.constraint_matrix;
     const auto m = static_cast<std::size_t>(A.pattern.rows);
-    const auto n = static_cast<std::size_t>(A.pattern.columns);
+    Products pr;
+    pr.ax.assign(m, 0.0);
+    pr.g = model.linear_objective;
+    for (std::size_t i = 0; i < m; ++i)
+        for (Index k = A.pattern.row_offsets[i]; k < A.pattern.row_offsets[i + 1]; ++k) {
+            const auto kk = static_cast<std::size_t>(k);
+            const auto j = static_cast<std::size_t>(A.pattern.column_indices[kk]);
+            pr.ax[i] += A.values[kk] * x[j];
+            pr.g[j] += y[i] * A.values[kk];
+        }
+    return pr;
+}
+
+Products device_products(const vk_hip_csr *A, const Model &model, const std::vector<Scalar> &x,
+                         const std::vector<Scalar> &y) {
+    Products pr;
+    pr.ax.resize(y.size());
+    pr.g.resize(x.size());
+    check_vk(vk_hip_csr_postsolve_products(A, x.data(), y.data(), model.linear_objective.data(), pr.ax.data(),
+                                           pr.g.data()),
+             "vk_hip_csr_postsolve_products");
+    return pr;
+}
+
+Scaled device_scale(vk_hip_csr *A, const Model &model, const Bounds &bounds, bool equilibrated) {
+    const auto m = static_cast<std::size_t>(model.num_constraints());
+    const auto n = static_cast<std::size_t>(model.num_variables());
+    Scaled sc;
+    sc.r.resize(m);
+    sc.rl.resize(m);
+    sc.ru.resize(m);
+    sc.s.resize(n);
+    sc.c.resize(n);
+    sc.l.resize(n);
+    sc.u.resize(n);
+    check_vk(vk_hip_lp_scale(A, model.linear_objective.data(), bounds.lower.data(), bounds.upper.data(),
+                             model.constraint_lower.data(), model.constraint_upper.data(),
+                             equilibrated ? kRuizPasses : 0, equilibrated ? 1 : 0, sc.r.data(), sc.s.data(),
+                             sc.c.data(), sc.l.data(), sc.u.data(), sc.rl.data(), sc.ru.data()),
+             "vk_hip_lp_scale");
+    return sc;
+}
+
+// pdhg.cpp:153-264: objective, residuals, duals and the final status, in
+// original coordinates.  `x`, `y` are unscaled; `pr` holds A x and c + A'y.
+void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
+               const std::vector<Scalar> &x, const std::vector<Scalar> &y, const Products &pr,
+               bool relative_mode, SolveResult &res) {
+    const auto m = static_cast<std::size_t>(model.constraint_matrix.pattern.rows);
+    const auto n = static_cast<std::size_t>(model.constraint_matrix.pattern.columns);
     res.primal_solution = x;
     Scalar obj = 0.0;
     for (std::size_t j = 0; j < n; ++j) obj += model.linear_objective[j] * x[j];
@@ -358,15 +396,8 @@ void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &op
         res.status = SolveStatus::NumericalFailure;
         res.message = "PDHG produced a non-finite objective";
     }
-    std::vector<Scalar> row_values(m, 0.0);
-    std::vector<Scalar> stationarity(model.linear_objective);
-    for (std::size_t i = 0; i < m; ++i)
-        for (Index k = A.pattern.row_offsets[i]; k < A.pattern.row_offsets[i + 1]; ++k) {
-            const auto kk = static_cast<std::size_t>(k);
-            const auto j = static_cast<std::size_t>(A.pattern.column_indices[kk]);
-            row_values[i] += A.values[kk] * x[j];
-            stationarity[j] += y[i] * A.values[kk];
-        }
+    const std::vector<Scalar> &row_values = pr.ax;
+    const std::vector<Scalar> &stationarity = pr.g;
     Scalar viol = 0.0;
     for (std::size_t i = 0; i < m; ++i) {
         if (std::isfinite(model.constraint_lower[i])) viol = std::max(viol, model.constraint_lower[i] - row_values[i]);
@@ -433,7 +464,7 @@ void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &op
         res.absolute_gap = 0.0;
         res.relative_gap = 0.0;
     }
-    relative_kkt(model, bounds, x, y, res);
+    relative_kkt(model, bounds, x, y, pr, res);
 }
 
 void check_vk(int rc, const char *what) {
@@ -505,8 +536,10 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     if (p.policy == VK_PDHG_POLICY_REFERENCE) {
-        // pdhg.cpp:75-78
-        p.step_size = 0.99 / std::max<Scalar>(1.0, operator_bound(model.constraint_matrix.pattern, sc.values));
+        // pdhg.cpp:75-78, the bound over A's scaled device values
+        double bound = 0.0;
+        check_vk(vk_hip_csr_operator_bound(A, &bound), "vk_hip_csr_operator_bound");
+        p.step_size = 0.99 / std::max<Scalar>(1.0, bound);
         p.primal_weight = 1.0;
     } else {
         double norm = 0.0;
@@ -585,7 +618,8 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
     std::vector<Scalar> x(xs.size()), y(ys.size());
     for (std::size_t j = 0; j < xs.size(); ++j) x[j] = xs[j] * sc.s[j];
     for (std::size_t i = 0; i < ys.size(); ++i) y[i] = ys[i] * sc.r[i];
-    postsolve(model, bounds, options, x, y, options.relative_kkt_tolerance > 0.0, res);
+    postsolve(model, bounds, options, x, y, device_products(A, model, x, y), options.relative_kkt_tolerance > 0.0,
+              res);
     if (status == VK_PDHG_PRIMAL_INFEASIBLE && !timed) report_infeasible(model, bounds, y, res);
     return res;
 }
@@ -684,7 +718,7 @@ SolveResult solve_ops(const Model &model, const Bounds &bounds, const Scaled &sc
     be.download(*y, ys);
     for (Index j = 0; j < n; ++j) xs[static_cast<std::size_t>(j)] *= sc.s[static_cast<std::size_t>(j)];
     for (Index i = 0; i < m; ++i) ys[static_cast<std::size_t>(i)] *= sc.r[static_cast<std::size_t>(i)];
-    postsolve(model, bounds, options, xs, ys, false, res);
+    postsolve(model, bounds, options, xs, ys, host_products(model, xs, ys), false, res);
     return res;
 }
 
@@ -697,9 +731,6 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
     if (!resolve_bounds(model, {}, {}, bounds, message) || !model.validate().empty() ||
         model.num_variables() == 0 || iterations <= 0)
         throw std::invalid_argument("profile_lp_kernels: unusable model or arguments");
-    Scaled sc = scale(model, bounds);
-    if (options.pdhg_policy != PdhgPolicy::Reference)
-        equilibrate(model.constraint_matrix.pattern, sc, kRuizPasses);
     vk_hip_csr *A = nullptr;
     check_vk(vk_hip_csr_create(model.constraint_matrix.pattern.rows, model.constraint_matrix.pattern.columns,
                                model.constraint_matrix.pattern.row_offsets.data(),
@@ -709,7 +740,8 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
         vk_hip_csr *a;
         ~Guard() { vk_hip_csr_destroy(a); }
     } guard{A};
-    check_vk(vk_hip_csr_set_values(A, sc.values.data()), "vk_hip_csr_set_values");
+    check_vk(vk_hip_csr_set_values(A, model.constraint_matrix.values.data()), "vk_hip_csr_set_values");
+    const Scaled sc = device_scale(A, model, bounds, options.pdhg_policy != PdhgPolicy::Reference);
     vk_pdhg_params p = make_params(model, sc, A, options);
     p.precision = options.pdhg_precision == PdhgPrecision::Mixed ? VK_PREC_MIXED : VK_PREC_FP64;
     p.col_blocks = col_blocks_for(A, options, p.precision);
@@ -755,36 +787,36 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend, const Solver
     if (options.continuous_method != ContinuousMethod::Auto && options.continuous_method != ContinuousMethod::PDHG)
         return status_result(SolveStatus::UnsupportedModel, "requested continuous method is not implemented for LP");
 
-    Scaled sc = scale(model, bounds);
-    if (options.pdhg_policy != PdhgPolicy::Reference)
-        equilibrate(model.constraint_matrix.pattern, sc, kRuizPasses);
-    std::vector<Scalar> warm;
     if (!warm_start.empty()) {
         if (static_cast<Index>(warm_start.size()) != model.num_variables())
             return status_result(SolveStatus::InvalidModel, "warm start must be empty or finite and sized to the model");
         for (const Scalar v : warm_start)
             if (!std::isfinite(v))
                 return status_result(SolveStatus::InvalidModel, "warm start must be empty or finite and sized to the model");
-        warm.assign(warm_start.begin(), warm_start.end());
-        for (std::size_t j = 0; j < warm.size(); ++j) warm[j] /= sc.s[j];   // pdhg.cpp:64-69
     }
-    const std::vector<Scalar> *warm_ptr = warm_start.empty() ? nullptr : &warm;
-
-    auto matrix = backend.create_matrix(model.constraint_matrix.pattern);
-    backend.set_values(sc.values, *matrix);
-    const vk_hip_csr *handle = hip_matrix_handle(*matrix);
-    SolveResult res;
     const bool device_only = options.pdhg_policy != PdhgPolicy::Reference ||
                              options.pdhg_precision != PdhgPrecision::FP64 ||
                              options.relative_kkt_tolerance > 0.0;
-    if (handle && options.fused_iterations) {
-        res = solve_fused(model, bounds, sc, handle, options, warm_ptr, started);
-    } else if (device_only) {
+    auto matrix = backend.create_matrix(model.constraint_matrix.pattern);
+    vk_hip_csr *handle = options.fused_iterations ? hip_matrix_handle(*matrix) : nullptr;
+    if (!handle && device_only)
         return status_result(SolveStatus::UnsupportedModel,
                              "adaptive, Halpern, mixed-precision and relative-KKT PDHG run only fused on a HipBackend");
+    // The fused path scales on the device from the original values; the
+    // op-by-op path keeps the reference's host scaling (pdhg.cpp:25).
+    Scaled sc;
+    if (handle) {
+        backend.set_values(model.constraint_matrix.values, *matrix);
+        sc = device_scale(handle, model, bounds, options.pdhg_policy != PdhgPolicy::Reference);
     } else {
-        res = solve_ops(model, bounds, sc, backend, *matrix, options, warm_ptr, started);
+        sc = scale(model, bounds);
+        backend.set_values(sc.values, *matrix);
     }
+    std::vector<Scalar> warm(warm_start.begin(), warm_start.end());
+    for (std::size_t j = 0; j < warm.size(); ++j) warm[j] /= sc.s[j];   // pdhg.cpp:64-69
+    const std::vector<Scalar> *warm_ptr = warm_start.empty() ? nullptr : &warm;
+    SolveResult res = handle ? solve_fused(model, bounds, sc, handle, options, warm_ptr, started)
+                             : solve_ops(model, bounds, sc, backend, *matrix, options, warm_ptr, started);
     res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
     return res;
 }

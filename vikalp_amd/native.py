@@ -139,6 +139,10 @@ HIP_SYMBOLS = {
     "vk_hip_csr_transpose_export": (C.c_int, [_vp, _ip, _ip, _dp]),
     "vk_hip_spmv_csr": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp, _vp]),
     "vk_hip_csr_norm2_estimate": (C.c_int, [_vp, C.c_int, _dp]),
+    "vk_hip_lp_scale": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                  _dp, _dp]),
+    "vk_hip_csr_operator_bound": (C.c_int, [_vp, _dp]),
+    "vk_hip_csr_postsolve_products": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "vk_hip_pdhg_create": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                      C.POINTER(PdhgParams), C.POINTER(_vp)]),
     "vk_hip_pdqp_create": (C.c_int, [_vp, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -579,6 +583,45 @@ class Device:
         check(self.L.vk_hip_csr_tune_col_blocks(h, int(precision), int(repeats), C.byref(best), _ptr(ms)),
               "vk_hip_csr_tune_col_blocks")
         return best.value, dict(zip((1, 2, 4, 8, 16), ms.tolist()))
+
+    def lp_scale(self, h, lp, ruiz_passes=0, pock_chambolle=False):
+        """vk_hip_lp_scale on the matrix h (values set) with lp's vectors:
+        a dict shaped like oracle.scale_lp's, without the values."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (lp.objective, lp.var_lower, lp.var_upper, lp.row_lower, lp.row_upper)]
+        out = {k: np.zeros(lp.n) for k in ("col_scale", "objective", "var_lower", "var_upper")}
+        out.update({k: np.zeros(lp.m) for k in ("row_scale", "row_lower", "row_upper")})
+        check(self.L.vk_hip_lp_scale(h, *[_ptr(a) for a in arrs], int(ruiz_passes), int(bool(pock_chambolle)),
+                                     _ptr(out["row_scale"]), _ptr(out["col_scale"]), _ptr(out["objective"]),
+                                     _ptr(out["var_lower"]), _ptr(out["var_upper"]), _ptr(out["row_lower"]),
+                                     _ptr(out["row_upper"])), "vk_hip_lp_scale")
+        return out
+
+    def operator_bound(self, h) -> float:
+        r = C.c_double()
+        check(self.L.vk_hip_csr_operator_bound(h, C.byref(r)), "vk_hip_csr_operator_bound")
+        return r.value
+
+    def postsolve_products(self, h, x, y, c):
+        """vk_hip_csr_postsolve_products: (A x, c + A'y) on the unscaled values."""
+        x, y, c = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, c))
+        ax, g = np.zeros(y.size), np.zeros(x.size)
+        check(self.L.vk_hip_csr_postsolve_products(h, _ptr(x), _ptr(y), _ptr(c), _ptr(ax), _ptr(g)),
+              "vk_hip_csr_postsolve_products")
+        return ax, g
+
+    def transpose_values(self, h, nnz):
+        """A's current device values in transpose order (column by column,
+        rows increasing), with the transpose's offsets and row indices."""
+        n = C.c_int64()
+        check(self.L.vk_hip_csr_dims(h, None, C.byref(n), None), "vk_hip_csr_dims")
+        offs = np.zeros(n.value + 1, np.int32)
+        rows = np.zeros(max(1, nnz), np.int32)
+        vals = np.zeros(max(1, nnz))
+        check(self.L.vk_hip_csr_transpose_export(h, offs.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                 rows.ctypes.data_as(C.POINTER(C.c_int32)), _ptr(vals)),
+              "vk_hip_csr_transpose_export")
+        return vals[:nnz], offs, rows[:nnz]
 
     def spmv(self, h, transpose, alpha, x, beta, y):
         check(self.L.vk_hip_spmv_csr(h, int(transpose), float(alpha), x, float(beta), y, None), "spmv")
