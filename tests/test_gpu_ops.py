@@ -58,24 +58,41 @@ def test_spmv_both_directions(m, n, density, long_rows):
         dev.free_all()
 
 
-def test_transpose_is_exact():
+def transpose_cases():
     rng = np.random.default_rng(5)
-    m, n = 700, 400
-    rp, ci, v = random_csr(rng, m, n, 0.02, (3,))
+    out = [("random", 700, 400, *random_csr(rng, 700, 400, 0.02, (3,)))]
+    # a dense column, empty columns past the last used one, no rows at all
+    rp = np.arange(0, 2 * 3000 + 1, 2, dtype=np.int64)
+    ci = np.stack([np.zeros(3000, np.int64), np.arange(3000, dtype=np.int64) % 97 + 1], 1).ravel()
+    out.append(("dense column", 3000, 200, rp, ci, rng.normal(size=ci.size)))
+    out.append(("no rows", 0, 5, np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0)))
+    out.append(("no entries", 10, 7, np.zeros(11, np.int64), np.zeros(0, np.int64), np.zeros(0)))
+    # 2^20 + 3 columns: the sort needs 21 key bits
+    lp = V.generate_lp(1, 50_000, (1 << 20) + 3, 2, 8)
+    out.append(("wide", lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values))
+    return out
+
+
+@pytest.mark.parametrize("case", transpose_cases(), ids=lambda c: c[0])
+def test_transpose_is_exact(case):
+    # The stored transpose lists each column's rows in increasing order,
+    # with tperm-gathered values: compared with scipy's exact transpose.
+    name, m, n, rp, ci, v = case
+    print(f"transpose {name} m={m} n={n} nnz={len(v)}", flush=True)
     dev = V.Device()
     import ctypes as C
     import scipy.sparse as sp
     h = dev.csr(m, n, rp, ci, v)
     nnz = len(v)
-    trp, tri, tv = np.zeros(n + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz)
+    trp, tri, tv = np.zeros(n + 1, np.int32), np.zeros(max(1, nnz), np.int32), np.zeros(max(1, nnz))
     rc = dev.L.vk_hip_csr_transpose_export(h, trp.ctypes.data_as(C.POINTER(C.c_int32)),
                                           tri.ctypes.data_as(C.POINTER(C.c_int32)),
                                           tv.ctypes.data_as(C.POINTER(C.c_double)))
     assert rc == 0
     At = sp.csr_matrix((v, ci, rp), shape=(m, n)).T.tocsr()
     At.sort_indices()
-    assert np.array_equal(trp, At.indptr) and np.array_equal(tri, At.indices)
-    assert np.array_equal(tv, At.data)
+    assert np.array_equal(trp, At.indptr) and np.array_equal(tri[:nnz], At.indices)
+    assert np.array_equal(tv[:nnz], At.data)
     dev.L.vk_hip_csr_destroy(h)
     dev.free_all()
 

@@ -227,6 +227,31 @@ void free_csr(DeviceCsr<V> &d) {
     d = DeviceCsr<V>{};
 }
 
+// A's stored transpose, built on the device from A's device pattern
+// (columns list their rows in increasing order); the tiles are cut on the
+// host from the returned offsets.
+int build_transpose(vk_hip_csr *A, hipStream_t s) {
+    DeviceCsr<double> &t = A->at;
+    const int cols = static_cast<int>(A->cols), nnz = static_cast<int>(A->nnz);
+    const size_t slots = static_cast<size_t>(std::max(1, nnz));
+    VK_HIP_CHECK(hipMalloc(&t.row_ptr, sizeof(int) * (static_cast<size_t>(cols) + 1)));
+    VK_HIP_CHECK(hipMalloc(&t.col, sizeof(int) * slots));
+    VK_HIP_CHECK(hipMalloc(&t.val, sizeof(double) * slots));
+    VK_HIP_CHECK(hipMalloc(&A->tperm, sizeof(int) * slots));
+    std::vector<int> trp;
+    if (int rc = transpose_on_device(static_cast<int>(A->rows), cols, nnz, A->a.row_ptr, A->a.col, t.row_ptr, t.col,
+                                     A->tperm, trp, s))
+        return rc;
+    const auto tiles = build_tiles(trp, cols);
+    t.rows = cols;
+    t.ntiles = static_cast<int>(tiles.size()) - 1;
+    VK_HIP_CHECK(hipMalloc(&t.tiles, sizeof(int2) * tiles.size()));
+    VK_HIP_CHECK(hipMemcpyAsync(t.tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
+    VK_HIP_CHECK(hipMemsetAsync(t.val, 0, sizeof(double) * slots, s));
+    VK_HIP_CHECK(hipStreamSynchronize(s));
+    return VK_OK;
+}
+
 int spmv_grid(int ntiles) { return std::max(1, std::min((ntiles + kWaves - 1) / kWaves, 4096)); }
 
 }  // namespace
@@ -401,13 +426,18 @@ int vk_hip_csr_create(int64_t rows, int64_t cols, const int64_t *row_offsets,
     VK_REQUIRE(row_offsets[0] == 0 && nnz >= 0, "CsrPattern row_offsets do not describe column_indices");
     VK_REQUIRE(nnz < INT32_MAX, "nonzero count exceeds the int32 device range");
     VK_REQUIRE(nnz == 0 || column_indices, "CsrPattern column_indices missing");
+    // Validation and the int32 copies in one pass over the pattern.
+    std::vector<int> rp(static_cast<size_t>(rows + 1)), ci(static_cast<size_t>(nnz));
+    rp[0] = 0;
     for (int64_t r = 0; r < rows; ++r) {
         const int64_t b = row_offsets[r], e = row_offsets[r + 1];
         VK_REQUIRE(b >= 0 && b <= e && e <= nnz, "CsrPattern row_offsets must be ordered and in range");
+        rp[r + 1] = static_cast<int>(e);
         int64_t prev = -1;
         for (int64_t k = b; k < e; ++k) {
             const int64_t c = column_indices[k];
             VK_REQUIRE(c >= 0 && c < cols && c > prev, "CsrPattern columns must be ordered and in range");
+            ci[k] = static_cast<int>(c);
             prev = c;
         }
     }
@@ -417,33 +447,8 @@ int vk_hip_csr_create(int64_t rows, int64_t cols, const int64_t *row_offsets,
     A->rows = rows;
     A->cols = cols;
     A->nnz = nnz;
-    std::vector<int> rp(static_cast<size_t>(rows + 1)), ci(static_cast<size_t>(nnz));
-    for (int64_t r = 0; r <= rows; ++r) rp[r] = static_cast<int>(row_offsets[r]);
-    for (int64_t k = 0; k < nnz; ++k) ci[k] = static_cast<int>(column_indices[k]);
-    // Transpose pattern by a counting sort over columns; within a column the
-    // rows come out increasing, so A' is a valid CSR with sorted indices.
-    std::vector<int> trp(static_cast<size_t>(cols + 1), 0), tci(static_cast<size_t>(nnz)),
-        perm(static_cast<size_t>(nnz));
-    for (int64_t k = 0; k < nnz; ++k) ++trp[ci[k] + 1];
-    for (int64_t j = 0; j < cols; ++j) trp[j + 1] += trp[j];
-    std::vector<int> next(trp.begin(), trp.end() - 1);
-    for (int64_t r = 0; r < rows; ++r)
-        for (int k = rp[r]; k < rp[r + 1]; ++k) {
-            const int pos = next[ci[k]]++;
-            tci[pos] = static_cast<int>(r);
-            perm[pos] = k;
-        }
     int rc = upload_csr(A->a, rp, ci, s);
-    if (!rc) rc = upload_csr(A->at, trp, tci, s);
-    if (!rc) {
-        const hipError_t e1 = hipMalloc(&A->tperm, sizeof(int) * std::max<int64_t>(1, nnz));
-        const hipError_t e2 = nnz ? hipMemcpy(A->tperm, perm.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)
-                                  : hipSuccess;
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            set_error("vk_hip_csr_create: transpose permutation upload failed");
-            rc = VK_EDEVICE;
-        }
-    }
+    if (!rc) rc = build_transpose(A, s);
     if (rc) {
         vk_hip_csr_destroy(A);
         return rc;

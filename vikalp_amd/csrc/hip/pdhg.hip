@@ -480,59 +480,31 @@ __global__ void k_f2d(int n, const float *src, double *dst) {
 
 int egrid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, 8192)); }
 
-// Host plan of the column-blocked layout of a CSR pattern (rp, ci) with
-// m rows and n columns: block p takes columns [n*p/nb, n*(p+1)/nb) of every
-// row, rows in order, blocks one after another.  perm maps a blocked
-// position to the CSR position its value comes from.
-struct BlockPlan {
-    std::vector<int> row_ptr, perm, col;
+// Column-blocked layout of a device CSR pattern (rp, ci) with m rows and n
+// columns: block p takes columns [n*p/nb, n*(p+1)/nb) of every row, rows in
+// order, blocks one after another.  The pattern (row_ptr, col) and perm,
+// the CSR position each blocked entry's value comes from, are built on the
+// device; the tiles are cut on the host from the returned offsets.
+struct BlockTiles {
     std::vector<int2> tiles;
     int tile_first[kMaxColBlocks + 1] = {};
     int max_tiles = 0;   // most tiles of any block
 };
 
-void plan_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int m, int n, int nb, BlockPlan &P) {
+int plan_blocks(const int *rp, const int *ci, int m, int n, int nnz, int nb, int *row_ptr, int *col, int *perm,
+                BlockTiles &T, hipStream_t st) {
+    std::vector<int> host_rp;
+    if (int rc = plan_blocks_on_device(rp, ci, m, n, nnz, nb, row_ptr, col, perm, host_rp, st)) return rc;
     const size_t stride = static_cast<size_t>(m) + 1;
-    std::vector<long long> bound(static_cast<size_t>(nb) + 1);
-    for (int p = 0; p <= nb; ++p) bound[p] = static_cast<long long>(n) * p / nb;
-    P.row_ptr.assign(stride * nb, 0);
-    // Columns are increasing within a row, so the block index only grows.
-    for (int r = 0; r < m; ++r) {
-        int p = 0;
-        for (int k = rp[r]; k < rp[r + 1]; ++k) {
-            while (ci[k] >= bound[p + 1]) ++p;
-            ++P.row_ptr[p * stride + r + 1];
-        }
-    }
-    int base = 0;
+    T.tiles.clear();
+    T.max_tiles = 0;
     for (int p = 0; p < nb; ++p) {
-        int *q = &P.row_ptr[p * stride];
-        q[0] = base;
-        for (int r = 0; r < m; ++r) q[r + 1] += q[r];
-        base = q[m];
+        T.tile_first[p] = static_cast<int>(T.tiles.size());
+        append_tiles(&host_rp[p * stride], 0, m, T.tiles);
+        T.max_tiles = std::max(T.max_tiles, static_cast<int>(T.tiles.size()) - T.tile_first[p] - 1);
     }
-    const size_t nnz = static_cast<size_t>(rp[m]);
-    P.perm.resize(nnz);
-    P.col.resize(nnz);
-    std::vector<int> cursor(static_cast<size_t>(nb));
-    for (int r = 0; r < m; ++r) {
-        for (int p = 0; p < nb; ++p) cursor[p] = P.row_ptr[p * stride + r];
-        int p = 0;
-        for (int k = rp[r]; k < rp[r + 1]; ++k) {
-            while (ci[k] >= bound[p + 1]) ++p;
-            const int pos = cursor[p]++;
-            P.perm[pos] = k;
-            P.col[pos] = ci[k];
-        }
-    }
-    P.tiles.clear();
-    P.max_tiles = 0;
-    for (int p = 0; p < nb; ++p) {
-        P.tile_first[p] = static_cast<int>(P.tiles.size());
-        append_tiles(&P.row_ptr[p * stride], 0, m, P.tiles);
-        P.max_tiles = std::max(P.max_tiles, static_cast<int>(P.tiles.size()) - P.tile_first[p] - 1);
-    }
-    P.tile_first[nb] = static_cast<int>(P.tiles.size());
+    T.tile_first[nb] = static_cast<int>(T.tiles.size());
+    return VK_OK;
 }
 
 // Workgroups of a column-blocked launch: a multiple of nb, one tile per wave
@@ -828,40 +800,43 @@ void plan_residency(vk_hip_pdhg *s) {
     s->resident_at = resident_prefix(AT.ntiles, nnz * per_nz + (s->n + 1ll) * 4 + (AT.ntiles + 1ll) * 8, left);
 }
 
-// Builds the column-blocked copy of A in the storage type T: the pattern is
-// planned on the host, the values are gathered on the device.
+// Builds the column-blocked copy of A in the storage type T on the device.
 template <class T>
 int build_blocked(vk_hip_pdhg *s) {
     const int m = s->m, n = s->n, nb = s->col_blocks;
     const CsrView<T> A = view_a<T>(s);
     const int nnz = static_cast<int>(s->A->nnz);
-    std::vector<int> rp(static_cast<size_t>(m) + 1), ci(static_cast<size_t>(nnz));
-    VK_HIP_CHECK(hipMemcpy(rp.data(), A.row_ptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
-    if (nnz) VK_HIP_CHECK(hipMemcpy(ci.data(), A.col, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
-    BlockPlan P;
-    plan_blocks(rp, ci, m, n, nb, P);
-    int *row_ptr = s->alloc<int>(P.row_ptr.size());
-    int *col = s->alloc<int>(P.col.size());
-    T *val = s->alloc<T>(P.col.size());
-    int2 *tiles = s->alloc<int2>(P.tiles.size());
+    int *row_ptr = s->alloc<int>((static_cast<size_t>(m) + 1) * nb);
+    int *col = s->alloc<int>(nnz);
+    T *val = s->alloc<T>(nnz);
     s->blk_part = s->alloc<double>(static_cast<size_t>(nb) * m);
     int *perm = nullptr;
-    if (!row_ptr || !col || !val || !tiles || !s->blk_part ||
-        hipMalloc(&perm, sizeof(int) * std::max<size_t>(1, P.perm.size())) != hipSuccess) {
+    if (!row_ptr || !col || !val || !s->blk_part ||
+        hipMalloc(&perm, sizeof(int) * std::max(1, nnz)) != hipSuccess) {
         set_error("vk_hip_pdhg_create: column-blocked layout allocation failed");
         return VK_ENOMEM;
     }
     hipStream_t st = s->stream;
-    VK_HIP_CHECK(hipMemcpyAsync(row_ptr, P.row_ptr.data(), sizeof(int) * P.row_ptr.size(), hipMemcpyHostToDevice, st));
-    VK_HIP_CHECK(hipMemcpyAsync(tiles, P.tiles.data(), sizeof(int2) * P.tiles.size(), hipMemcpyHostToDevice, st));
-    if (nnz) {
-        VK_HIP_CHECK(hipMemcpyAsync(col, P.col.data(), sizeof(int) * P.col.size(), hipMemcpyHostToDevice, st));
-        VK_HIP_CHECK(hipMemcpyAsync(perm, P.perm.data(), sizeof(int) * P.perm.size(), hipMemcpyHostToDevice, st));
-        k_gather_perm<T><<<egrid(nnz), kBlock, 0, st>>>(nnz, A.val, perm, val);
-        VK_HIP_CHECK(hipGetLastError());
+    BlockTiles P;
+    int rc = plan_blocks(A.row_ptr, A.col, m, n, nnz, nb, row_ptr, col, perm, P, st);
+    int2 *tiles = rc ? nullptr : s->alloc<int2>(P.tiles.size());
+    if (!rc && !tiles) {
+        set_error("vk_hip_pdhg_create: column-blocked layout allocation failed");
+        rc = VK_ENOMEM;
     }
-    VK_HIP_CHECK(hipStreamSynchronize(st));
+    if (!rc) {
+        const hipError_t e1 = hipMemcpyAsync(tiles, P.tiles.data(), sizeof(int2) * P.tiles.size(),
+                                             hipMemcpyHostToDevice, st);
+        if (nnz) k_gather_perm<T><<<egrid(nnz), kBlock, 0, st>>>(nnz, A.val, perm, val);
+        const hipError_t e2 = hipGetLastError();
+        const hipError_t e3 = hipStreamSynchronize(st);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            set_error("vk_hip_pdhg_create: column-blocked layout upload failed");
+            rc = VK_EDEVICE;
+        }
+    }
     (void)hipFree(perm);
+    if (rc) return rc;
     BlockedView<T> &B = blocked<T>(s);
     B.row_ptr = row_ptr;
     B.col = col;
@@ -1251,9 +1226,6 @@ int tune_col_blocks(vk_hip_csr *A, const CsrView<T> &view, int repeats, int *bes
         }
     };
     check(hipMemsetAsync(x, 0, sizeof(T) * n, st));
-    std::vector<int> rp(static_cast<size_t>(m) + 1), ci(static_cast<size_t>(nnz));
-    check(hipMemcpy(rp.data(), view.row_ptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
-    if (nnz) check(hipMemcpy(ci.data(), view.col, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
     double best_ms = std::numeric_limits<double>::infinity();
     *best = 1;
     for (int i = 0; i < kTuneCount && rc == VK_OK; ++i) {
@@ -1269,27 +1241,27 @@ int tune_col_blocks(vk_hip_csr *A, const CsrView<T> &view, int repeats, int *bes
                     if (view.ntiles > 0) k_spmv_store<T><<<grid, kBlock, 0, st>>>(view, x, y);
                 };
             } else {
-                BlockPlan P;
-                plan_blocks(rp, ci, m, n, nb, P);
                 const size_t before = owned.size();
-                int *brp = static_cast<int *>(dalloc(sizeof(int) * P.row_ptr.size()));
-                int *bcol = static_cast<int *>(dalloc(sizeof(int) * P.col.size()));
-                T *bval = static_cast<T *>(dalloc(sizeof(T) * P.col.size()));
+                int *brp = static_cast<int *>(dalloc(sizeof(int) * (static_cast<size_t>(m) + 1) * nb));
+                int *bcol = static_cast<int *>(dalloc(sizeof(int) * nnz));
+                T *bval = static_cast<T *>(dalloc(sizeof(T) * nnz));
+                int *bperm = static_cast<int *>(dalloc(sizeof(int) * nnz));
+                BlockTiles P;
+                if (!brp || !bcol || !bval || !bperm) {
+                    set_error("vk_hip_csr_tune_col_blocks: device allocation failed");
+                    rc = VK_ENOMEM;
+                    break;
+                }
+                if ((rc = plan_blocks(view.row_ptr, view.col, m, n, nnz, nb, brp, bcol, bperm, P, st))) break;
                 int2 *btiles = static_cast<int2 *>(dalloc(sizeof(int2) * P.tiles.size()));
-                int *bperm = static_cast<int *>(dalloc(sizeof(int) * P.perm.size()));
-                if (!brp || !bcol || !bval || !btiles || !bperm) {
+                if (!btiles) {
                     set_error("vk_hip_csr_tune_col_blocks: device allocation failed");
                     rc = VK_ENOMEM;
                     break;
                 }
                 layout.assign(owned.begin() + static_cast<long>(before), owned.end());
-                check(hipMemcpyAsync(brp, P.row_ptr.data(), sizeof(int) * P.row_ptr.size(), hipMemcpyHostToDevice, st));
                 check(hipMemcpyAsync(btiles, P.tiles.data(), sizeof(int2) * P.tiles.size(), hipMemcpyHostToDevice, st));
-                if (nnz) {
-                    check(hipMemcpyAsync(bcol, P.col.data(), sizeof(int) * P.col.size(), hipMemcpyHostToDevice, st));
-                    check(hipMemcpyAsync(bperm, P.perm.data(), sizeof(int) * P.perm.size(), hipMemcpyHostToDevice, st));
-                    k_gather_perm<T><<<egrid(nnz), kBlock, 0, st>>>(nnz, view.val, bperm, bval);
-                }
+                if (nnz) k_gather_perm<T><<<egrid(nnz), kBlock, 0, st>>>(nnz, view.val, bperm, bval);
                 B = BlockedView<T>{brp, bcol, bval, btiles, m, nb, {}};
                 for (int p = 0; p <= nb; ++p) B.tile_first[p] = P.tile_first[p];
                 const int grid = blocked_grid(nb, P.max_tiles);

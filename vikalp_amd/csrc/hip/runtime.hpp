@@ -78,4 +78,18 @@ struct vk_hip_csr {
 namespace vk {
 // Makes the FP32 copies of A and A' (used by the mixed-precision solver).
 int csr_make_f32(vk_hip_csr *A, hipStream_t stream);
+
+// Device layout builders (layout.hip).  A's transpose pattern from its
+// device pattern (rp, ci): offsets trp (cols + 1), rows trow and the CSR
+// position of each entry tperm (nnz), columns listing rows in increasing
+// order; the offsets are also returned on the host.
+int transpose_on_device(int rows, int cols, int nnz, const int *rp, const int *ci, int *trp, int *trow,
+                        int *tperm, std::vector<int> &host_trp, hipStream_t s);
+// The column-blocked pattern of A: block p takes columns
+// [cols*p/nb, cols*(p+1)/nb) of every row, rows in order, blocks one after
+// another.  Fills row_ptr (nb blocks of rows + 1 offsets), col and perm
+// (the CSR position each blocked entry comes from) on the device; the
+// offsets are also returned on the host.
+int plan_blocks_on_device(const int *rp, const int *ci, int rows, int cols, int nnz, int nb, int *row_ptr,
+                          int *col, int *perm, std::vector<int> &host_row_ptr, hipStream_t s);
 }
