@@ -143,6 +143,23 @@ def measure_batch(V, np, args):
     return out
 
 
+def measure_mps(V, path, options):
+    """One real instance read from an MPS file (MpsReader::read), solved to
+    the benchmark tolerance in FP64 and mixed precision (Halpern policy)."""
+    t0 = time.time()
+    lp = V.read_mps(path)
+    out = {"file": os.path.basename(path), "name": lp.name, "m": lp.m, "n": lp.n, "nnz": lp.nnz,
+           "read_seconds": time.time() - t0}
+    for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
+        r = V.solve_lp(lp, options(precision))
+        out[label] = {"status": r.status, "iterations": r.iterations, "device_seconds": r.device_seconds,
+                      "solve_seconds": r.solve_seconds, "rel_kkt": r.kkt_relative,
+                      "objective": r.objective_value}
+        log(f"mps {lp.name} {label}: {r.status} iterations={r.iterations} device={r.device_seconds:.3f}s "
+            f"wall={r.solve_seconds:.3f}s relkkt={r.kkt_relative:.3e} obj={r.objective_value:.10g}")
+    return out
+
+
 def measure_adjoint(V, np, args):
     """Unrolled PDHG forward / backward on m = 100,000, n = 250,000, 6
     nonzeros per column (generator family 1), K iterations with checkpoints
@@ -216,6 +233,9 @@ def main():
                     help="skip the FP64 solve with the adaptive policy (comparison)")
     ap.add_argument("--skip-solve", action="store_true",
                     help="kernel timing only (used for the rocprofv3 pass)")
+    ap.add_argument("--mps", nargs="*", default=[],
+                    help="also solve these MPS files (FP64 and mixed, Halpern policy) and report them "
+                         "under \"mps\"")
     ap.add_argument("--counters", action="store_true",
                     help="driver for the rocprofv3 --pmc passes: three 256 MiB device copies "
                          "(counter calibration), then 2 + 6 profiled FP64 iterations; no JSON line")
@@ -315,6 +335,7 @@ def main():
     adjoint = None
     if args.adjoint_iterations > 0 and not args.skip_solve:
         adjoint = measure_adjoint(V, np, args)
+    mps = [measure_mps(V, path, options) for path in args.mps] if args.mps else None
 
     line = {
         "metric": "time to 1e-6 rel. KKT, mixed-FP32 vs FP64 PDHG, 1 MI355X; SpMV HBM GB/s",
@@ -331,6 +352,8 @@ def main():
         "data": "synthetic (seeded generator, LP built on the host)",
         "config": {**WORKLOAD, "m": lp.m, "n": lp.n, "nnz": lp.nnz},
         "fp64_seconds": t64 if ok64 else None,
+        "solve_wall_seconds": {"fp64": float(np.median([r.solve_seconds for r in fp64_runs])),
+                               "mixed": float(np.median([r.solve_seconds for r in mixed_runs]))},
         "speedup_mixed_vs_fp64": (t64 / t32) if (ok32 and ok64) else None,
         "iterations_fp64": r64.iterations,
         "iterations_mixed": r32.iterations,
@@ -352,6 +375,7 @@ def main():
         "adaptive_fp64": adaptive,
         "batch": batch,
         "adjoint": adjoint,
+        "mps": mps,
     }
     print(json.dumps(line), flush=True)
 

@@ -167,6 +167,30 @@ int vk_unrolled_set_data(vk_unrolled *u, const double *objective, const double *
 int vk_unrolled_info(const vk_unrolled *u, double *step_size, double *last_device_ms);
 int vk_unrolled_destroy(vk_unrolled *u);
 
+/* ---- MPS input --------------------------------------------------------------
+ * Replaces src/io/MpsReader.cpp:10-553 MpsReader::read
+ * (include/vikalp/io/MpsReader.hpp:17): reads a free-format MPS file into a
+ * model in the canonical form above.  Every file the reference reads gives
+ * the same model; files it rejects get the same messages.  In addition the
+ * standard RANGES section, MI / PL / LI / UI bounds, integer MARKER blocks
+ * and comment lines are read.  Returns 0 and a model handle, 1 when the
+ * file is rejected (vk_solve_last_error() lists every error as
+ * "line N: message", one per line), -1 for null arguments. */
+typedef struct vk_model vk_model;
+
+int vk_mps_read(const char *path, vk_model **out);
+int vk_model_dims(const vk_model *model, int64_t *m, int64_t *n, int64_t *nnz);
+/* Copies into caller buffers; any pointer may be NULL to skip that array.
+ * variable_types: 0 continuous, 1 integer, 2 binary (VariableType order). */
+int vk_model_export(const vk_model *model, int64_t *row_offsets, int64_t *column_indices,
+                    double *values, double *objective, double *row_lower, double *row_upper,
+                    double *var_lower, double *var_upper, int32_t *variable_types,
+                    double *objective_offset);
+/* The model's NAME into buf (NUL-terminated, truncated to size); returns
+ * the name's full length. */
+int64_t vk_model_name(const vk_model *model, char *buf, int64_t size);
+void vk_model_free(vk_model *model);
+
 #ifdef __cplusplus
 }
 #endif

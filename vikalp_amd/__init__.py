@@ -12,12 +12,12 @@ from __future__ import annotations
 from .native import (  # noqa: F401
     BackendPreference, ContinuousMethod, PdhgPolicy, PdhgPrecision, SolveStatus,
     STATUS_NAMES, LP, Options, Result, Device, hip_lib, host_lib, generate_lp,
-    solve, solve_lp, solve_qp, load_libraries, profile_kernels, solve_batch, UnrolledPdhg,
+    solve, solve_lp, solve_qp, load_libraries, profile_kernels, solve_batch, UnrolledPdhg, read_mps,
 )
 
 __all__ = [
     "BackendPreference", "ContinuousMethod", "PdhgPolicy", "PdhgPrecision", "SolveStatus",
     "STATUS_NAMES", "LP", "Options", "Result", "Device", "hip_lib", "host_lib",
     "generate_lp", "solve", "solve_lp", "load_libraries", "profile_kernels", "solve_batch",
-    "UnrolledPdhg", "solve_qp",
+    "UnrolledPdhg", "solve_qp", "read_mps",
 ]

@@ -33,7 +33,7 @@ CXX_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-para
              "-I" + INC, "-I" + os.path.join(HOST_SRC, "include"), "-I" + HOST_SRC]
 
 HIP_UNITS = ["ops.hip", "pdhg.hip", "batch.hip", "unrolled.hip", "presolve.hip", "layout.hip"]
-HOST_UNITS = ["model.cpp", "hip_backend.cpp", "pdhg_lp.cpp", "batch_lp.cpp", "unrolled_lp.cpp", "qp.cpp", "solver.cpp", "capi.cpp",
+HOST_UNITS = ["model.cpp", "hip_backend.cpp", "pdhg_lp.cpp", "batch_lp.cpp", "unrolled_lp.cpp", "qp.cpp", "solver.cpp", "capi.cpp", "mps_reader.cpp",
               "lpgen.cpp"]
 
 

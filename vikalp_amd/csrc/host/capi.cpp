@@ -3,6 +3,7 @@
 #include "vikalp_solve.h"
 
 #include "vikalp/backend/HipBackend.hpp"
+#include "vikalp/io/MpsReader.hpp"
 #include "vikalp/solver/Solver.hpp"
 #include "vikalp/solver/continuous.hpp"
 
@@ -396,5 +397,75 @@ int vk_unrolled_destroy(vk_unrolled *h) {
     delete h;
     return 0;
 }
+
+struct vk_model {
+    Model model;
+};
+
+int vk_mps_read(const char *path, vk_model **out) {
+    if (!path || !out) {
+        g_last_error = "vk_mps_read: null argument";
+        return -1;
+    }
+    *out = nullptr;
+    try {
+        auto h = std::make_unique<vk_model>();
+        MpsReader reader;
+        if (!reader.read(path, h->model)) {
+            std::string msg;
+            for (const auto &e : reader.errors()) msg += "line " + std::to_string(e.line) + ": " + e.message + "\n";
+            if (reader.errors().empty()) {
+                const auto problems = h->model.validate();
+                msg = "line 0: " + (problems.empty() ? std::string("invalid model") : problems.front()) + "\n";
+            }
+            g_last_error = msg;
+            return 1;
+        }
+        *out = h.release();
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = std::string("line 0: ") + e.what() + "\n";
+        return 1;
+    }
+}
+
+int vk_model_dims(const vk_model *h, int64_t *m, int64_t *n, int64_t *nnz) {
+    if (!h) return -1;
+    if (m) *m = h->model.num_constraints();
+    if (n) *n = h->model.num_variables();
+    if (nnz) *nnz = static_cast<int64_t>(h->model.constraint_matrix.values.size());
+    return 0;
+}
+
+int vk_model_export(const vk_model *h, int64_t *row_offsets, int64_t *column_indices, double *values,
+                    double *objective, double *row_lower, double *row_upper, double *var_lower,
+                    double *var_upper, int32_t *variable_types, double *objective_offset) {
+    if (!h) return -1;
+    const Model &md = h->model;
+    auto put = [](auto *dst, const auto &src) {
+        if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    put(row_offsets, md.constraint_matrix.pattern.row_offsets);
+    put(column_indices, md.constraint_matrix.pattern.column_indices);
+    put(values, md.constraint_matrix.values);
+    put(objective, md.linear_objective);
+    put(row_lower, md.constraint_lower);
+    put(row_upper, md.constraint_upper);
+    put(var_lower, md.variable_lower);
+    put(var_upper, md.variable_upper);
+    if (variable_types)
+        for (std::size_t j = 0; j < md.variable_types.size(); ++j)
+            variable_types[j] = static_cast<int32_t>(md.variable_types[j]);
+    if (objective_offset) *objective_offset = md.objective_offset;
+    return 0;
+}
+
+int64_t vk_model_name(const vk_model *h, char *buf, int64_t size) {
+    if (!h) return -1;
+    if (buf && size > 0) copy_text(buf, static_cast<std::size_t>(size), h->model.name);
+    return static_cast<int64_t>(h->model.name.size());
+}
+
+void vk_model_free(vk_model *h) { delete h; }
 
 }  // extern "C"

@@ -197,6 +197,11 @@ HOST_SYMBOLS = {
     "vk_unrolled_info": (C.c_int, [_vp, _dp, _dp]),
     "vk_unrolled_set_data": (C.c_int, [_vp, _dp, _dp, _dp]),
     "vk_unrolled_destroy": (C.c_int, [_vp]),
+    "vk_mps_read": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "vk_model_dims": (C.c_int, [_vp, _lp, _lp, _lp]),
+    "vk_model_export": (C.c_int, [_vp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]),
+    "vk_model_name": (C.c_int64, [_vp, C.c_char_p, C.c_int64]),
+    "vk_model_free": (None, [_vp]),
 }
 
 _hip = None
@@ -265,6 +270,8 @@ class LP:
     objective_offset: float = 0.0
     x0: np.ndarray | None = None
     y0: np.ndarray | None = None
+    name: str = ""
+    variable_types: np.ndarray | None = None   # 0 continuous, 1 integer, 2 binary
 
     @property
     def nnz(self) -> int:
@@ -275,6 +282,35 @@ class LP:
         return orc.LP(self.m, self.n, self.row_offsets, self.column_indices, self.values,
                       self.objective, self.row_lower, self.row_upper, self.var_lower,
                       self.var_upper, self.objective_offset)
+
+
+def read_mps(path: str) -> LP:
+    """MpsReader::read (src/io/MpsReader.cpp:10-553) through vk_mps_read: the
+    model as an LP with its NAME and variable types.  Raises ValueError with
+    every "line N: message" of a rejected file."""
+    L = host_lib()
+    h = _vp()
+    rc = L.vk_mps_read(os.fsencode(path), C.byref(h))
+    if rc != 0:
+        msg = L.vk_solve_last_error()
+        raise ValueError(f"vk_mps_read({path!r}) rejected the file:\n{msg.decode() if msg else ''}")
+    try:
+        mm, nn, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        L.vk_model_dims(h, C.byref(mm), C.byref(nn), C.byref(nz))
+        m, n, nnz = mm.value, nn.value, nz.value
+        a = dict(rp=np.zeros(m + 1, np.int64), ci=np.zeros(nnz, np.int64), v=np.zeros(nnz),
+                 c=np.zeros(n), rl=np.zeros(m), ru=np.zeros(m), l=np.zeros(n), u=np.zeros(n),
+                 t=np.zeros(n, np.int32))
+        off = C.c_double()
+        L.vk_model_export(h, _ptr(a["rp"], _lp), _ptr(a["ci"], _lp), _ptr(a["v"]), _ptr(a["c"]), _ptr(a["rl"]),
+                          _ptr(a["ru"]), _ptr(a["l"]), _ptr(a["u"]), _ptr(a["t"], _ip), C.byref(off))
+        size = L.vk_model_name(h, None, 0)
+        buf = C.create_string_buffer(int(size) + 1)
+        L.vk_model_name(h, buf, int(size) + 1)
+    finally:
+        L.vk_model_free(h)
+    return LP(m, n, a["rp"], a["ci"], a["v"], a["c"], a["rl"], a["ru"], a["l"], a["u"], off.value,
+              name=buf.value.decode(), variable_types=a["t"])
 
 
 def generate_lp(family: int, m: int = 0, n: int = 0, k: int = 0, seed: int = 20261203) -> LP:
