@@ -134,6 +134,12 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
 /* Replaces continuous_common.hpp:207-225 scaled_operator_bound (pdhg.cpp:75):
  * sqrt(max_i sum_j |a_ij| * max_j sum_i |a_ij|) over A's current values. */
 int vk_hip_csr_operator_bound(const vk_hip_csr *A, double *bound);
+/* The same bound for each diagonal block of a block-diagonal A (block i:
+ * rows [row_starts[i], row_starts[i+1]), columns [col_starts[i],
+ * col_starts[i+1])), as scaled_operator_bound gives for that block alone
+ * (pdhg.cpp:75 per batched instance).  bounds: count doubles. */
+int vk_hip_csr_operator_bounds(const vk_hip_csr *A, int count, const int64_t *row_starts,
+                               const int64_t *col_starts, double *bounds);
 
 /* Replaces the O(nnz) loops of the reference postsolve: linear_values
  * (continuous_common.hpp:302-315, called at pdhg.cpp:168) and the
@@ -298,6 +304,18 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
                         const double *primal_weights, const double *norm_b,
                         const double *norm_c, const int64_t *iteration_limits,
                         const vk_pdhg_params *params, vk_hip_batch **out);
+/* The same, with the block-diagonal matrix given as a CSR handle whose
+ * values are set (typically scaled in place by vk_hip_lp_scale): its
+ * pattern, transpose and values are copied on the device, and every row is
+ * checked to stay inside its instance's columns (VK_EINVAL otherwise). */
+int vk_hip_batch_create_csr(int count, const int64_t *row_starts, const int64_t *col_starts,
+                            const vk_hip_csr *A, const double *c, const double *l,
+                            const double *u, const double *rl, const double *ru,
+                            const double *row_scale, const double *col_scale,
+                            const double *x0, const double *y0, const double *step_sizes,
+                            const double *primal_weights, const double *norm_b,
+                            const double *norm_c, const int64_t *iteration_limits,
+                            const vk_pdhg_params *params, vk_hip_batch **out);
 /* Launches the persistent kernel up to `max_launches` times, each running
  * `iterations_per_launch` iterations of every instance still running; stops
  * early when all have stopped.  *running = instances still running. */

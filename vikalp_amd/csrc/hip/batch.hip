@@ -279,6 +279,28 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
     if (tid == 0) a.states[id] = S;
 }
 
+// *bad = 1 when a row of the block-diagonal batch matrix touches a column
+// outside its instance's range (instance of row r: the last i with
+// row_starts[i] <= r, which skips empty instances).
+__global__ void k_check_instances(int rows, int count, const int *__restrict__ rp, const int *__restrict__ ci,
+                                  const int64_t *__restrict__ row_starts, const int64_t *__restrict__ col_starts,
+                                  int *__restrict__ bad) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+        int lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (row_starts[mid] <= r) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t c0 = col_starts[lo], c1 = col_starts[lo + 1];
+        for (int k = rp[r]; k < rp[r + 1]; ++k)
+            if (ci[k] < c0 || ci[k] >= c1) {
+                *bad = 1;
+                break;
+            }
+    }
+}
+
 __global__ void k_to_f32_batch(int64_t n, const double *__restrict__ src, float *__restrict__ dst) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = static_cast<float>(src[i]);
@@ -408,30 +430,21 @@ int read_states(vk_hip_batch *h) {
     return VK_OK;
 }
 
-}  // namespace
+// The matrix of a batch: host CSR arrays (int64), or a device CSR handle
+// whose values are already set (and usually scaled on the device).
+struct MatrixSource {
+    const int64_t *row_offsets = nullptr, *column_indices = nullptr;
+    const double *values = nullptr;
+    const vk_hip_csr *A = nullptr;
+};
 
-extern "C" {
-
-int vk_hip_batch_destroy(vk_hip_batch *h) {
-    if (!h) return VK_OK;
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (void *p : h->allocations) (void)hipFree(p);
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
-    delete h;
-    return VK_OK;
-}
-
-int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col_starts,
-                        const int64_t *row_offsets, const int64_t *column_indices, const double *values,
-                        const double *c, const double *l, const double *u, const double *rl,
-                        const double *ru, const double *row_scale, const double *col_scale,
-                        const double *x0, const double *y0, const double *step_sizes,
-                        const double *primal_weights, const double *norm_b, const double *norm_c,
-                        const int64_t *iteration_limits, const vk_pdhg_params *params,
-                        vk_hip_batch **out) {
-    VK_REQUIRE(out && params && count > 0 && row_starts && col_starts && row_offsets,
+int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts, const MatrixSource &src,
+                 const double *c, const double *l, const double *u, const double *rl, const double *ru,
+                 const double *row_scale, const double *col_scale, const double *x0, const double *y0,
+                 const double *step_sizes, const double *primal_weights, const double *norm_b,
+                 const double *norm_c, const int64_t *iteration_limits, const vk_pdhg_params *params,
+                 vk_hip_batch **out) {
+    VK_REQUIRE(out && params && count > 0 && row_starts && col_starts && (src.row_offsets || src.A),
                "vk_hip_batch_create: bad arguments");
     *out = nullptr;
     VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE || params->policy == VK_PDHG_POLICY_ADAPTIVE ||
@@ -444,28 +457,36 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     const int64_t m = row_starts[count], n = col_starts[count];
     VK_REQUIRE(row_starts[0] == 0 && col_starts[0] == 0 && m < INT32_MAX && n < INT32_MAX,
                "vk_hip_batch_create: instance ranges must start at 0 and fit int32");
-    const int64_t nnz = row_offsets[m];
-    VK_REQUIRE(nnz >= 0 && nnz < INT32_MAX && row_offsets[0] == 0, "vk_hip_batch_create: bad row offsets");
     VK_REQUIRE(n > 0 && c && l && u && col_scale, "vk_hip_batch_create: null column vector");
     VK_REQUIRE(m == 0 || (rl && ru && row_scale), "vk_hip_batch_create: null row vector");
     if (iteration_limits)
         for (int i = 0; i < count; ++i)
             VK_REQUIRE(iteration_limits[i] >= 0, "vk_hip_batch_create: negative instance iteration limit");
-    VK_REQUIRE(nnz == 0 || (column_indices && values), "vk_hip_batch_create: null matrix arrays");
-    for (int i = 0; i < count; ++i) {
+    for (int i = 0; i < count; ++i)
         VK_REQUIRE(row_starts[i + 1] >= row_starts[i] && col_starts[i + 1] > col_starts[i],
                    "vk_hip_batch_create: every instance needs >= 1 column and ordered ranges");
-        for (int64_t r = row_starts[i]; r < row_starts[i + 1]; ++r) {
-            VK_REQUIRE(row_offsets[r] <= row_offsets[r + 1] && row_offsets[r + 1] <= nnz,
-                       "vk_hip_batch_create: row offsets must be ordered and in range");
-            int64_t prev = -1;
-            for (int64_t k = row_offsets[r]; k < row_offsets[r + 1]; ++k) {
-                const int64_t col = column_indices[k];
-                VK_REQUIRE(col >= col_starts[i] && col < col_starts[i + 1] && col > prev,
-                           "vk_hip_batch_create: a row leaves its instance's columns or is unordered");
-                prev = col;
+    int64_t nnz = 0;
+    if (src.A) {
+        VK_REQUIRE(src.A->rows == m && src.A->cols == n, "vk_hip_batch_create: matrix dimensions differ from the ranges");
+        VK_REQUIRE(src.A->has_values, "vk_hip_batch_create: matrix values not set");
+        nnz = src.A->nnz;
+    } else {
+        const int64_t *row_offsets = src.row_offsets, *column_indices = src.column_indices;
+        nnz = row_offsets[m];
+        VK_REQUIRE(nnz >= 0 && nnz < INT32_MAX && row_offsets[0] == 0, "vk_hip_batch_create: bad row offsets");
+        VK_REQUIRE(nnz == 0 || (column_indices && src.values), "vk_hip_batch_create: null matrix arrays");
+        for (int i = 0; i < count; ++i)
+            for (int64_t r = row_starts[i]; r < row_starts[i + 1]; ++r) {
+                VK_REQUIRE(row_offsets[r] <= row_offsets[r + 1] && row_offsets[r + 1] <= nnz,
+                           "vk_hip_batch_create: row offsets must be ordered and in range");
+                int64_t prev = -1;
+                for (int64_t k = row_offsets[r]; k < row_offsets[r + 1]; ++k) {
+                    const int64_t col = column_indices[k];
+                    VK_REQUIRE(col >= col_starts[i] && col < col_starts[i + 1] && col > prev,
+                               "vk_hip_batch_create: a row leaves its instance's columns or is unordered");
+                    prev = col;
+                }
             }
-        }
     }
     if (int rc = ensure_init()) return rc;
     auto *h = new (std::nothrow) vk_hip_batch();
@@ -485,22 +506,98 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
         set_error("vk_hip_batch_create: stream/event creation failed");
         return fail(VK_EDEVICE);
     }
-    // Host CSR (int32) and its transpose by a stable counting sort.
-    std::vector<int> rp(static_cast<size_t>(m + 1)), ci(static_cast<size_t>(nnz));
-    for (int64_t r = 0; r <= m; ++r) rp[r] = static_cast<int>(row_offsets[r]);
-    for (int64_t k = 0; k < nnz; ++k) ci[k] = static_cast<int>(column_indices[k]);
-    std::vector<int> trp(static_cast<size_t>(n + 1), 0), tci(static_cast<size_t>(nnz));
-    std::vector<double> tv(static_cast<size_t>(nnz));
-    for (int64_t k = 0; k < nnz; ++k) ++trp[ci[k] + 1];
-    for (int64_t j = 0; j < n; ++j) trp[j + 1] += trp[j];
-    {
-        std::vector<int> next(trp.begin(), trp.end() - 1);
-        for (int64_t r = 0; r < m; ++r)
-            for (int k = rp[r]; k < rp[r + 1]; ++k) {
-                const int pos = next[ci[k]]++;
-                tci[pos] = static_cast<int>(r);
-                tv[pos] = values[k];
-            }
+    h->rp = h->alloc<int>(m + 1);
+    h->ci = h->alloc<int>(nnz);
+    h->trp = h->alloc<int>(n + 1);
+    h->tci = h->alloc<int>(nnz);
+    h->val = h->alloc<double>(nnz);
+    h->tval = h->alloc<double>(nnz);
+    h->inst = h->alloc<InstanceRange>(count);
+    h->states = h->alloc<DevState>(count);
+    if (h->mixed) {
+        h->val32 = h->alloc<float>(nnz);
+        h->tval32 = h->alloc<float>(nnz);
+    }
+    for (void *p : h->allocations)
+        if (!p) {
+            set_error("vk_hip_batch_create: device allocation failed");
+            return fail(VK_ENOMEM);
+        }
+    // Host copies of both offset arrays: the tile tables are cut on the host.
+    std::vector<int> rp(static_cast<size_t>(m + 1)), trp(static_cast<size_t>(n + 1));
+    if (src.A) {
+        // The handle's A and A' (its transpose lists each column's rows in
+        // increasing order, as the host counting sort below does); a device
+        // check that every row stays inside its instance's columns.
+        const vk_hip_csr *A = src.A;
+        int64_t *starts = h->alloc<int64_t>(2 * (static_cast<size_t>(count) + 1));
+        int *bad = h->alloc<int>(1);
+        if (!starts || !bad) {
+            set_error("vk_hip_batch_create: device allocation failed");
+            return fail(VK_ENOMEM);
+        }
+        hipStream_t st = h->stream;
+        const size_t sb = sizeof(int64_t) * (static_cast<size_t>(count) + 1);
+        if (hipMemcpyAsync(starts, row_starts, sb, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(starts + count + 1, col_starts, sb, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemsetAsync(bad, 0, sizeof(int), st) != hipSuccess) {
+            set_error("vk_hip_batch_create: upload failed");
+            return fail(VK_EDEVICE);
+        }
+        if (m > 0) k_check_instances<<<bgrid(m), 256, 0, st>>>(static_cast<int>(m), count, A->a.row_ptr, A->a.col,
+                                                              starts, starts + count + 1, bad);
+        int bad_host = 0;
+        const auto d2d = hipMemcpyDeviceToDevice;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(h->rp, A->a.row_ptr, sizeof(int) * (m + 1), d2d, st) != hipSuccess ||
+            hipMemcpyAsync(h->trp, A->at.row_ptr, sizeof(int) * (n + 1), d2d, st) != hipSuccess ||
+            (nnz && (hipMemcpyAsync(h->ci, A->a.col, sizeof(int) * nnz, d2d, st) != hipSuccess ||
+                     hipMemcpyAsync(h->tci, A->at.col, sizeof(int) * nnz, d2d, st) != hipSuccess ||
+                     hipMemcpyAsync(h->val, A->a.val, sizeof(double) * nnz, d2d, st) != hipSuccess ||
+                     hipMemcpyAsync(h->tval, A->at.val, sizeof(double) * nnz, d2d, st) != hipSuccess)) ||
+            hipMemcpyAsync(rp.data(), A->a.row_ptr, sizeof(int) * (m + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(trp.data(), A->at.row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(&bad_host, bad, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_error("vk_hip_batch_create: device copy failed");
+            return fail(VK_EDEVICE);
+        }
+        if (bad_host) {
+            set_error("vk_hip_batch_create: a row leaves its instance's columns");
+            return fail(VK_EINVAL);
+        }
+    } else {
+        // Host CSR (int32) and its transpose by a stable counting sort.
+        const int64_t *row_offsets = src.row_offsets, *column_indices = src.column_indices;
+        std::vector<int> ci(static_cast<size_t>(nnz)), tci(static_cast<size_t>(nnz));
+        for (int64_t r = 0; r <= m; ++r) rp[r] = static_cast<int>(row_offsets[r]);
+        for (int64_t k = 0; k < nnz; ++k) ci[k] = static_cast<int>(column_indices[k]);
+        std::fill(trp.begin(), trp.end(), 0);
+        std::vector<double> tv(static_cast<size_t>(nnz));
+        for (int64_t k = 0; k < nnz; ++k) ++trp[ci[k] + 1];
+        for (int64_t j = 0; j < n; ++j) trp[j + 1] += trp[j];
+        {
+            std::vector<int> next(trp.begin(), trp.end() - 1);
+            for (int64_t r = 0; r < m; ++r)
+                for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                    const int pos = next[ci[k]]++;
+                    tci[pos] = static_cast<int>(r);
+                    tv[pos] = src.values[k];
+                }
+        }
+        auto up = [&](void *dst, const void *from, size_t bytes) {
+            return bytes ? hipMemcpyAsync(dst, from, bytes, hipMemcpyHostToDevice, h->stream) : hipSuccess;
+        };
+        if (up(h->rp, rp.data(), sizeof(int) * rp.size()) != hipSuccess ||
+            up(h->ci, ci.data(), sizeof(int) * ci.size()) != hipSuccess ||
+            up(h->trp, trp.data(), sizeof(int) * trp.size()) != hipSuccess ||
+            up(h->tci, tci.data(), sizeof(int) * tci.size()) != hipSuccess ||
+            up(h->val, src.values, sizeof(double) * nnz) != hipSuccess ||
+            up(h->tval, tv.data(), sizeof(double) * nnz) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            set_error("vk_hip_batch_create: upload failed");
+            return fail(VK_EDEVICE);
+        }
     }
     std::vector<int2> ta, tt;
     h->inst_host.resize(static_cast<size_t>(count));
@@ -527,35 +624,16 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     tt.push_back(make_int2(static_cast<int>(n), static_cast<int>(nnz)));
     h->ntiles_a = static_cast<int>(ta.size()) - 1;
     h->ntiles_at = static_cast<int>(tt.size()) - 1;
-    h->rp = h->alloc<int>(m + 1);
-    h->ci = h->alloc<int>(nnz);
-    h->trp = h->alloc<int>(n + 1);
-    h->tci = h->alloc<int>(nnz);
-    h->val = h->alloc<double>(nnz);
-    h->tval = h->alloc<double>(nnz);
     h->tiles_a = h->alloc<int2>(ta.size());
     h->tiles_at = h->alloc<int2>(tt.size());
-    h->inst = h->alloc<InstanceRange>(count);
-    h->states = h->alloc<DevState>(count);
-    if (h->mixed) {
-        h->val32 = h->alloc<float>(nnz);
-        h->tval32 = h->alloc<float>(nnz);
+    if (!h->tiles_a || !h->tiles_at) {
+        set_error("vk_hip_batch_create: device allocation failed");
+        return fail(VK_ENOMEM);
     }
-    for (void *p : h->allocations)
-        if (!p) {
-            set_error("vk_hip_batch_create: device allocation failed");
-            return fail(VK_ENOMEM);
-        }
-    auto up = [&](void *dst, const void *src, size_t bytes) {
-        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream) : hipSuccess;
+    auto up = [&](void *dst, const void *from, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, from, bytes, hipMemcpyHostToDevice, h->stream) : hipSuccess;
     };
-    if (up(h->rp, rp.data(), sizeof(int) * rp.size()) != hipSuccess ||
-        up(h->ci, ci.data(), sizeof(int) * ci.size()) != hipSuccess ||
-        up(h->trp, trp.data(), sizeof(int) * trp.size()) != hipSuccess ||
-        up(h->tci, tci.data(), sizeof(int) * tci.size()) != hipSuccess ||
-        up(h->val, values, sizeof(double) * nnz) != hipSuccess ||
-        up(h->tval, tv.data(), sizeof(double) * nnz) != hipSuccess ||
-        up(h->tiles_a, ta.data(), sizeof(int2) * ta.size()) != hipSuccess ||
+    if (up(h->tiles_a, ta.data(), sizeof(int2) * ta.size()) != hipSuccess ||
         up(h->tiles_at, tt.data(), sizeof(int2) * tt.size()) != hipSuccess ||
         up(h->inst, h->inst_host.data(), sizeof(InstanceRange) * count) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess) {
@@ -615,6 +693,53 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     if (rc) return fail(rc);
     *out = h;
     return VK_OK;
+}
+
+
+}  // namespace
+
+extern "C" {
+
+int vk_hip_batch_destroy(vk_hip_batch *h) {
+    if (!h) return VK_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocations) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return VK_OK;
+}
+
+
+int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col_starts,
+                        const int64_t *row_offsets, const int64_t *column_indices, const double *values,
+                        const double *c, const double *l, const double *u, const double *rl,
+                        const double *ru, const double *row_scale, const double *col_scale,
+                        const double *x0, const double *y0, const double *step_sizes,
+                        const double *primal_weights, const double *norm_b, const double *norm_c,
+                        const int64_t *iteration_limits, const vk_pdhg_params *params,
+                        vk_hip_batch **out) {
+    MatrixSource src;
+    src.row_offsets = row_offsets;
+    src.column_indices = column_indices;
+    src.values = values;
+    return batch_create(count, row_starts, col_starts, src, c, l, u, rl, ru, row_scale, col_scale, x0, y0,
+                        step_sizes, primal_weights, norm_b, norm_c, iteration_limits, params, out);
+}
+
+int vk_hip_batch_create_csr(int count, const int64_t *row_starts, const int64_t *col_starts,
+                            const vk_hip_csr *A, const double *c, const double *l, const double *u,
+                            const double *rl, const double *ru, const double *row_scale,
+                            const double *col_scale, const double *x0, const double *y0,
+                            const double *step_sizes, const double *primal_weights, const double *norm_b,
+                            const double *norm_c, const int64_t *iteration_limits,
+                            const vk_pdhg_params *params, vk_hip_batch **out) {
+    VK_REQUIRE(A, "vk_hip_batch_create_csr: null matrix");
+    MatrixSource src;
+    src.A = A;
+    return batch_create(count, row_starts, col_starts, src, c, l, u, rl, ru, row_scale, col_scale, x0, y0,
+                        step_sizes, primal_weights, norm_b, norm_c, iteration_limits, params, out);
 }
 
 int vk_hip_batch_run(vk_hip_batch *h, int64_t iterations_per_launch, int64_t max_launches, int *running) {

@@ -334,6 +334,39 @@ int vk_hip_csr_operator_bound(const vk_hip_csr *A, double *bound) {
     return VK_OK;
 }
 
+int vk_hip_csr_operator_bounds(const vk_hip_csr *A, int count, const int64_t *row_starts,
+                               const int64_t *col_starts, double *bounds) {
+    VK_REQUIRE(A && row_starts && col_starts && bounds && count > 0, "vk_hip_csr_operator_bounds: null argument");
+    VK_REQUIRE(A->has_values, "vk_hip_csr_operator_bounds: matrix values not set");
+    const int m = static_cast<int>(A->rows), n = static_cast<int>(A->cols);
+    VK_REQUIRE(row_starts[0] == 0 && col_starts[0] == 0 && row_starts[count] == m && col_starts[count] == n,
+               "vk_hip_csr_operator_bounds: block ranges must cover the matrix");
+    for (int i = 0; i < count; ++i)
+        VK_REQUIRE(row_starts[i + 1] >= row_starts[i] && col_starts[i + 1] >= col_starts[i],
+                   "vk_hip_csr_operator_bounds: block ranges must be ordered");
+    hipStream_t s = default_stream();
+    Scratch sb;
+    double *rows = sb.get<double>(m), *cols = sb.get<double>(n);
+    if (!rows || !cols) {
+        set_error("vk_hip_csr_operator_bounds: scratch allocation failed");
+        return VK_ENOMEM;
+    }
+    k_row_stat<RowStat::AbsSum><<<pre_grid(m), kPreBlock, 0, s>>>(m, A->a.row_ptr, A->a.val, rows);
+    k_row_stat<RowStat::AbsSum><<<pre_grid(n), kPreBlock, 0, s>>>(n, A->at.row_ptr, A->at.val, cols);
+    VK_HIP_CHECK(hipGetLastError());
+    std::vector<double> hr(static_cast<size_t>(m)), hc(static_cast<size_t>(n));
+    if (m) VK_HIP_CHECK(hipMemcpyAsync(hr.data(), rows, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    if (n) VK_HIP_CHECK(hipMemcpyAsync(hc.data(), cols, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    VK_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < count; ++i) {
+        double mr = 0.0, mc = 0.0;
+        for (int64_t r = row_starts[i]; r < row_starts[i + 1]; ++r) mr = std::max(mr, hr[static_cast<size_t>(r)]);
+        for (int64_t j = col_starts[i]; j < col_starts[i + 1]; ++j) mc = std::max(mc, hc[static_cast<size_t>(j)]);
+        bounds[i] = std::sqrt(mr * mc);
+    }
+    return VK_OK;
+}
+
 int vk_hip_csr_postsolve_products(const vk_hip_csr *A, const double *x, const double *y, const double *c,
                                   double *ax, double *g) {
     VK_REQUIRE(A, "vk_hip_csr_postsolve_products: null matrix");

@@ -1,8 +1,10 @@
 // solve_lp_batch: many LPs solved together on the batched persistent HIP
 // kernel (vk_hip_batch_*).  Every instance follows solve_lp's semantics:
 // the same entry checks (pdhg.cpp:269-293), scaling (continuous_common.hpp:
-// 133-205, plus equilibration for the adaptive policy), step rule, postsolve
-// (pdhg.cpp:153-264) and status mapping; only the execution is shared.
+// 133-205, plus equilibration for the adaptive and Halpern policies), step
+// rule, postsolve (pdhg.cpp:153-264) and status mapping; only the execution
+// is shared.  The instances are stacked into one block-diagonal matrix that
+// is transposed, scaled and multiplied for the postsolve on the device.
 
 #include "lp_internal.hpp"
 
@@ -24,62 +26,108 @@ using namespace detail;
 struct Prepared {
     std::size_t index;   // position in the caller's list
     Bounds bounds;
-    Scaled sc;
 };
+
+// The instances as one block-diagonal LP: original data on the host, the
+// matrix on the device (scaled there in place), the scaled vectors back on
+// the host.  Block scaling is per instance: no row or column of the
+// concatenation touches two instances.
+struct Stacked {
+    std::vector<int64_t> row_starts, col_starts;
+    Scaled sc;                                   // concatenated; no host values
+    std::vector<double> steps, weights, nb, nc;  // per instance
+    vk_hip_csr *A = nullptr;
+    ~Stacked() { vk_hip_csr_destroy(A); }
+};
+
+void stack(const std::vector<const Model *> &models, const std::vector<Prepared> &prep, const SolverOptions &options,
+           Stacked &st) {
+    const int count = static_cast<int>(prep.size());
+    st.row_starts.assign(count + 1, 0);
+    st.col_starts.assign(count + 1, 0);
+    int64_t nnz = 0;
+    for (int i = 0; i < count; ++i) {
+        const Model &mod = *models[prep[i].index];
+        st.row_starts[i + 1] = st.row_starts[i] + mod.num_constraints();
+        st.col_starts[i + 1] = st.col_starts[i] + mod.num_variables();
+        nnz += static_cast<int64_t>(mod.constraint_matrix.values.size());
+    }
+    const int64_t m = st.row_starts[count], n = st.col_starts[count];
+    std::vector<int64_t> rp(static_cast<size_t>(m + 1), 0), ci;
+    std::vector<double> val, c, l, u, rl, ru;
+    ci.reserve(static_cast<size_t>(nnz));
+    val.reserve(static_cast<size_t>(nnz));
+    for (int i = 0; i < count; ++i) {
+        const Model &mod = *models[prep[i].index];
+        const auto &pat = mod.constraint_matrix.pattern;
+        const int64_t base = static_cast<int64_t>(ci.size());
+        for (Index row = 0; row < pat.rows; ++row) rp[st.row_starts[i] + row + 1] = base + pat.row_offsets[row + 1];
+        for (const Index col : pat.column_indices) ci.push_back(col + st.col_starts[i]);
+        val.insert(val.end(), mod.constraint_matrix.values.begin(), mod.constraint_matrix.values.end());
+        c.insert(c.end(), mod.linear_objective.begin(), mod.linear_objective.end());
+        l.insert(l.end(), prep[i].bounds.lower.begin(), prep[i].bounds.lower.end());
+        u.insert(u.end(), prep[i].bounds.upper.begin(), prep[i].bounds.upper.end());
+        rl.insert(rl.end(), mod.constraint_lower.begin(), mod.constraint_lower.end());
+        ru.insert(ru.end(), mod.constraint_upper.begin(), mod.constraint_upper.end());
+    }
+    check_vk(vk_hip_csr_create(m, n, rp.data(), ci.empty() ? nullptr : ci.data(), &st.A), "vk_hip_csr_create");
+    check_vk(vk_hip_csr_set_values(st.A, val.empty() ? nullptr : val.data()), "vk_hip_csr_set_values");
+    // continuous_common.hpp:133-205 per instance (plus the equilibration of
+    // the adaptive and Halpern policies), on the device.
+    Scaled &sc = st.sc;
+    sc.r.resize(static_cast<size_t>(m));
+    sc.rl.resize(static_cast<size_t>(m));
+    sc.ru.resize(static_cast<size_t>(m));
+    sc.s.resize(static_cast<size_t>(n));
+    sc.c.resize(static_cast<size_t>(n));
+    sc.l.resize(static_cast<size_t>(n));
+    sc.u.resize(static_cast<size_t>(n));
+    const bool equilibrated = options.pdhg_policy != PdhgPolicy::Reference;
+    check_vk(vk_hip_lp_scale(st.A, c.data(), l.data(), u.data(), rl.data(), ru.data(), equilibrated ? kRuizPasses : 0,
+                             equilibrated ? 1 : 0, sc.r.data(), sc.s.data(), sc.c.data(), sc.l.data(), sc.u.data(),
+                             sc.rl.data(), sc.ru.data()),
+             "vk_hip_lp_scale");
+    st.steps.assign(count, 0.0);   // 0: power iteration on the device
+    st.weights.assign(count, 1.0);
+    st.nb.resize(count);
+    st.nc.resize(count);
+    if (!equilibrated) {
+        std::vector<double> bounds(count);
+        check_vk(vk_hip_csr_operator_bounds(st.A, count, st.row_starts.data(), st.col_starts.data(), bounds.data()),
+                 "vk_hip_csr_operator_bounds");
+        for (int i = 0; i < count; ++i) st.steps[i] = 0.99 / std::max<Scalar>(1.0, bounds[i]);   // pdhg.cpp:75-76
+    }
+    for (int i = 0; i < count; ++i) {
+        const Model &mod = *models[prep[i].index];
+        st.nb[i] = row_bound_norm(mod.constraint_lower, mod.constraint_upper);
+        st.nc[i] = norm2(mod.linear_objective);
+        if (equilibrated) {
+            const auto c0 = static_cast<size_t>(st.col_starts[i]), c1 = static_cast<size_t>(st.col_starts[i + 1]);
+            const auto r0 = static_cast<size_t>(st.row_starts[i]), r1 = static_cast<size_t>(st.row_starts[i + 1]);
+            const Scalar cn = norm2(std::vector<Scalar>(sc.c.begin() + c0, sc.c.begin() + c1));
+            const Scalar bn = row_bound_norm(std::vector<Scalar>(sc.rl.begin() + r0, sc.rl.begin() + r1),
+                                             std::vector<Scalar>(sc.ru.begin() + r0, sc.ru.begin() + r1));
+            st.weights[i] = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+        }
+    }
+}
 
 struct DeviceBatch {
     vk_hip_batch *h = nullptr;
     ~DeviceBatch() { vk_hip_batch_destroy(h); }
 };
 
-// Runs one device batch over `prep` (FP64 or mixed), optionally warm-started
-// from scaled iterates; returns per-instance states and scaled iterates.
-void run_batch(const std::vector<const Model *> &models, const std::vector<Prepared> &prep,
-               const SolverOptions &options, int precision, double rel_tol, Index iteration_limit,
-               const std::vector<double> *x0, const std::vector<double> *y0,
+// Runs one device batch over the stacked instances (FP64 or mixed),
+// optionally warm-started from scaled iterates; returns per-instance states
+// and scaled iterates.
+void run_batch(const Stacked &st, const SolverOptions &options, int precision, double rel_tol,
+               Index iteration_limit, const std::vector<double> *x0, const std::vector<double> *y0,
                const std::vector<double> *weights_in, const std::vector<int64_t> *limits,
-               Clock::time_point started,
-               std::vector<vk_pdhg_state> &states, std::vector<double> &xs, std::vector<double> &ys,
-               double &device_ms, bool &timed) {
-    const int count = static_cast<int>(prep.size());
-    std::vector<int64_t> row_starts(count + 1, 0), col_starts(count + 1, 0);
-    for (int i = 0; i < count; ++i) {
-        const Model &mod = *models[prep[i].index];
-        row_starts[i + 1] = row_starts[i] + mod.num_constraints();
-        col_starts[i + 1] = col_starts[i] + mod.num_variables();
-    }
-    const int64_t m = row_starts[count], n = col_starts[count];
-    std::vector<int64_t> rp(static_cast<size_t>(m + 1), 0), ci;
-    std::vector<double> val, c, l, u, rl, ru, r, s, steps(count), weights(count), nb(count), nc(count);
-    for (int i = 0; i < count; ++i) {
-        const Model &mod = *models[prep[i].index];
-        const Scaled &sc = prep[i].sc;
-        const auto &pat = mod.constraint_matrix.pattern;
-        for (Index row = 0; row < pat.rows; ++row) {
-            for (Index k = pat.row_offsets[row]; k < pat.row_offsets[row + 1]; ++k) {
-                ci.push_back(pat.column_indices[k] + col_starts[i]);
-                val.push_back(sc.values[k]);
-            }
-            rp[row_starts[i] + row + 1] = static_cast<int64_t>(ci.size());
-        }
-        c.insert(c.end(), sc.c.begin(), sc.c.end());
-        l.insert(l.end(), sc.l.begin(), sc.l.end());
-        u.insert(u.end(), sc.u.begin(), sc.u.end());
-        rl.insert(rl.end(), sc.rl.begin(), sc.rl.end());
-        ru.insert(ru.end(), sc.ru.begin(), sc.ru.end());
-        r.insert(r.end(), sc.r.begin(), sc.r.end());
-        s.insert(s.end(), sc.s.begin(), sc.s.end());
-        nb[i] = row_bound_norm(mod.constraint_lower, mod.constraint_upper);
-        nc[i] = norm2(mod.linear_objective);
-        if (options.pdhg_policy == PdhgPolicy::Reference) {
-            steps[i] = 0.99 / std::max<Scalar>(1.0, operator_bound(pat, sc.values));   // pdhg.cpp:75-76
-            weights[i] = 1.0;
-        } else {
-            steps[i] = 0.0;   // power iteration on the device
-            const Scalar cn = norm2(sc.c), bn = row_bound_norm(sc.rl, sc.ru);
-            weights[i] = weights_in ? (*weights_in)[i] : ((cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0);
-        }
-    }
+               Clock::time_point started, std::vector<vk_pdhg_state> &states, std::vector<double> &xs,
+               std::vector<double> &ys, double &device_ms, bool &timed) {
+    const int count = static_cast<int>(st.steps.size());
+    const int64_t m = st.row_starts[count], n = st.col_starts[count];
+    const Scaled &sc = st.sc;
     vk_pdhg_params p{};
     p.policy = options.pdhg_policy == PdhgPolicy::Adaptive  ? VK_PDHG_POLICY_ADAPTIVE
                : options.pdhg_policy == PdhgPolicy::Halpern ? VK_PDHG_POLICY_HALPERN
@@ -89,15 +137,15 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
     p.relative_kkt_tolerance = rel_tol;
     p.restart_period = options.restart_period;
+    const std::vector<double> &weights = weights_in ? *weights_in : st.weights;
     DeviceBatch batch;
-    check_vk(vk_hip_batch_create(count, row_starts.data(), col_starts.data(), rp.data(),
-                                 ci.empty() ? nullptr : ci.data(), val.empty() ? nullptr : val.data(),
-                                 c.data(), l.data(), u.data(), rl.empty() ? nullptr : rl.data(),
-                                 ru.empty() ? nullptr : ru.data(), r.empty() ? nullptr : r.data(), s.data(),
-                                 x0 ? x0->data() : nullptr, y0 && !y0->empty() ? y0->data() : nullptr,
-                                 steps.data(), weights.data(), nb.data(), nc.data(),
-                                 limits ? limits->data() : nullptr, &p, &batch.h),
-             "vk_hip_batch_create");
+    check_vk(vk_hip_batch_create_csr(count, st.row_starts.data(), st.col_starts.data(), st.A, sc.c.data(),
+                                     sc.l.data(), sc.u.data(), m ? sc.rl.data() : nullptr,
+                                     m ? sc.ru.data() : nullptr, m ? sc.r.data() : nullptr, sc.s.data(),
+                                     x0 ? x0->data() : nullptr, y0 && !y0->empty() ? y0->data() : nullptr,
+                                     st.steps.data(), weights.data(), st.nb.data(), st.nc.data(),
+                                     limits ? limits->data() : nullptr, &p, &batch.h),
+             "vk_hip_batch_create_csr");
     // Launch granularity: 256 iterations per launch keeps each kernel short
     // and lets the host honour the time limit between launches.
     constexpr int64_t kPerLaunch = 256;
@@ -154,12 +202,11 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             results[i] = status_result(SolveStatus::InvalidModel, "solver limits and tolerances must be non-negative");
             continue;
         }
-        pr.sc = scale(model, pr.bounds);
-        if (options.pdhg_policy != PdhgPolicy::Reference)
-            equilibrate(model.constraint_matrix.pattern, pr.sc, kRuizPasses);
         prep.push_back(std::move(pr));
     }
     if (prep.empty()) return results;
+    Stacked st;
+    stack(models, prep, options, st);
 
     std::vector<vk_pdhg_state> states;
     std::vector<double> xs, ys;
@@ -173,8 +220,8 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
         double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
         if (options.pdhg_policy == PdhgPolicy::Halpern) phase_tol = std::max(phase_tol, kHalpernFp32Floor);
-        run_batch(models, prep, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr,
-                  nullptr, nullptr, started, states, xs, ys, ms, timed);
+        run_batch(st, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr, nullptr,
+                  nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
         for (std::size_t i = 0; i < prep.size(); ++i) used[i] = low[i] = states[i].iterations;
         if (!timed) {
@@ -192,22 +239,39 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             }
             ys32 = ys;
             const auto x32 = xs, y32 = ys;
-            run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32,
-                      &y32, options.pdhg_policy != PdhgPolicy::Reference ? &w : nullptr, &remaining, started,
-                      states, xs, ys, ms, timed);
+            run_batch(st, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32, &y32,
+                      options.pdhg_policy != PdhgPolicy::Reference ? &w : nullptr, &remaining, started, states, xs,
+                      ys, ms, timed);
             total_ms += ms;
             for (std::size_t i = 0; i < prep.size(); ++i) used[i] += states[i].iterations;
         }
     } else {
-        run_batch(models, prep, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit,
-                  nullptr, nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
+        run_batch(st, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit, nullptr,
+                  nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
         for (std::size_t i = 0; i < prep.size(); ++i) used[i] = states[i].iterations;
     }
-    Index row0 = 0, col0 = 0;
+    // Unscaled iterates of all instances, then the postsolve products of
+    // the stacked matrix on the device (each instance's rows and columns
+    // see only its own entries).
+    const Scaled &sc = st.sc;
+    std::vector<Scalar> xall(xs.size()), yall(ys.size()), c_all;
+    for (std::size_t j = 0; j < xs.size(); ++j) xall[j] = xs[j] * sc.s[j];
     for (std::size_t i = 0; i < prep.size(); ++i) {
         const Model &model = *models[prep[i].index];
-        const Scaled &sc = prep[i].sc;
+        c_all.insert(c_all.end(), model.linear_objective.begin(), model.linear_objective.end());
+        const auto r0 = static_cast<std::size_t>(st.row_starts[i]), r1 = static_cast<std::size_t>(st.row_starts[i + 1]);
+        const std::vector<double> &ysrc = infeasible32[i] ? ys32 : ys;
+        for (std::size_t r = r0; r < r1; ++r) yall[r] = ysrc[r] * sc.r[r];
+    }
+    Products all;
+    all.ax.resize(yall.size());
+    all.g.resize(xall.size());
+    check_vk(vk_hip_csr_postsolve_products(st.A, xall.data(), yall.data(), c_all.data(), all.ax.data(), all.g.data()),
+             "vk_hip_csr_postsolve_products");
+    for (std::size_t i = 0; i < prep.size(); ++i) {
+        const Model &model = *models[prep[i].index];
+        const auto row0 = st.row_starts[i], col0 = st.col_starts[i];
         const Index m = model.num_constraints(), n = model.num_variables();
         SolveResult res;
         res.solver = "PDHG";
@@ -218,17 +282,15 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
         const bool infeasible = infeasible32[i] || states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
         if (timed && states[i].status == VK_PDHG_RUNNING && !infeasible) res.status = SolveStatus::TimeLimit;
         else if (states[i].status == VK_PDHG_CONVERGED && !infeasible) res.status = SolveStatus::Optimal;
-        const std::vector<double> &ysrc = infeasible32[i] ? ys32 : ys;
-        std::vector<Scalar> x(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(m));
-        for (Index j = 0; j < n; ++j) x[j] = xs[col0 + j] * sc.s[j];
-        for (Index r = 0; r < m; ++r) y[r] = ysrc[row0 + r] * sc.r[r];
-        postsolve(model, prep[i].bounds, options, x, y, host_products(model, x, y),
-                  options.relative_kkt_tolerance > 0.0, res);
+        const std::vector<Scalar> x(xall.begin() + col0, xall.begin() + col0 + n);
+        const std::vector<Scalar> y(yall.begin() + row0, yall.begin() + row0 + m);
+        Products pr;
+        pr.ax.assign(all.ax.begin() + row0, all.ax.begin() + row0 + m);
+        pr.g.assign(all.g.begin() + col0, all.g.begin() + col0 + n);
+        postsolve(model, prep[i].bounds, options, x, y, pr, options.relative_kkt_tolerance > 0.0, res);
         if (infeasible) report_infeasible(model, prep[i].bounds, y, res);
         res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
         results[prep[i].index] = std::move(res);
-        row0 += m;
-        col0 += n;
     }
     return results;
 }
