@@ -224,3 +224,63 @@ def test_scale_requires_values():
             dev.lp_scale(h, lp)
     finally:
         dev.L.vk_hip_csr_destroy(h)
+
+
+def stacked(lps):
+    """Block-diagonal stack of LPs: (m, n, rp, ci, values, row_starts, col_starts)."""
+    rs, cs = np.zeros(len(lps) + 1, np.int64), np.zeros(len(lps) + 1, np.int64)
+    for i, lp in enumerate(lps):
+        rs[i + 1], cs[i + 1] = rs[i] + lp.m, cs[i] + lp.n
+    rp = [np.zeros(1, np.int64)]
+    base = 0
+    for lp in lps:
+        rp.append(np.asarray(lp.row_offsets[1:], np.int64) + base)
+        base += lp.nnz
+    ci = np.concatenate([np.asarray(lp.column_indices, np.int64) + cs[i] for i, lp in enumerate(lps)])
+    v = np.concatenate([lp.values for lp in lps])
+    return int(rs[-1]), int(cs[-1]), np.concatenate(rp), ci, v, rs, cs
+
+
+def test_block_operator_bounds_match_each_instance():
+    lps = [V.generate_lp(1, 300, 900, 5, 4), V.generate_lp(4, 0, 0, 4, 9), V.generate_lp(1, 40, 90, 3, 3)]
+    m, n, rp, ci, v, rs, cs = stacked(lps)
+    cat = lambda k: np.concatenate([getattr(lp, k) for lp in lps])
+    big = V.LP(m, n, rp, ci, v, cat("objective"), cat("row_lower"), cat("row_upper"), cat("var_lower"),
+               cat("var_upper"))
+    dev = V.Device()
+    h = dev.csr(m, n, rp, ci, v)
+    try:
+        dev.lp_scale(h, big)
+        got = np.zeros(len(lps))
+        N.check(dev.L.vk_hip_csr_operator_bounds(h, len(lps), rs.ctypes.data_as(N._lp), cs.ctypes.data_as(N._lp),
+                                                 got.ctypes.data_as(N._dp)), "operator_bounds")
+        for i, lp in enumerate(lps):
+            ref = O.scale_lp(lp.as_oracle())
+            assert got[i] == O.operator_bound(lp.m, lp.n, lp.row_offsets, lp.column_indices, ref["values"]), i
+    finally:
+        dev.L.vk_hip_csr_destroy(h)
+
+
+def test_batch_from_handle_rejects_a_row_leaving_its_instance():
+    import ctypes as C
+    lps = [V.generate_lp(1, 40, 90, 3, 3), V.generate_lp(1, 40, 90, 3, 5)]
+    m, n, rp, ci, v, rs, cs = stacked(lps)
+    # the first row of instance 0 gains an entry in instance 1's last column
+    ci = ci.copy()
+    ci[rp[1] - 1] = n - 1
+    dev = V.Device()
+    h = dev.csr(m, n, rp, ci, v)
+    try:
+        p = N.PdhgParams()
+        p.policy, p.precision, p.iteration_limit, p.restart_period = 0, 0, 10, 64
+        vec = lambda k: np.ascontiguousarray(k, dtype=np.float64)
+        ones_n, ones_m, zeros_n = vec(np.ones(n)), vec(np.ones(m)), vec(np.zeros(n))
+        out = C.c_void_p()
+        rc = dev.L.vk_hip_batch_create_csr(2, rs.ctypes.data_as(N._lp), cs.ctypes.data_as(N._lp), h,
+                                           N._ptr(zeros_n), N._ptr(zeros_n), N._ptr(ones_n), N._ptr(ones_m),
+                                           N._ptr(ones_m), N._ptr(ones_m), N._ptr(ones_n), None, None, None,
+                                           None, None, None, None, C.byref(p), C.byref(out))
+        assert rc == -1 and not out.value   # VK_EINVAL
+        assert b"leaves its instance" in dev.L.vk_hip_last_error()
+    finally:
+        dev.L.vk_hip_csr_destroy(h)

@@ -142,6 +142,7 @@ HIP_SYMBOLS = {
     "vk_hip_lp_scale": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
                                   _dp, _dp]),
     "vk_hip_csr_operator_bound": (C.c_int, [_vp, _dp]),
+    "vk_hip_csr_operator_bounds": (C.c_int, [_vp, C.c_int, _lp, _lp, _dp]),
     "vk_hip_csr_postsolve_products": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "vk_hip_pdhg_create": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                      C.POINTER(PdhgParams), C.POINTER(_vp)]),
@@ -158,6 +159,8 @@ HIP_SYMBOLS = {
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, _dp, _dp, _dp, _dp, _dp, _dp, _lp, C.POINTER(PdhgParams),
                                       C.POINTER(_vp)]),
+    "vk_hip_batch_create_csr": (C.c_int, [C.c_int, _lp, _lp, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          _dp, _dp, _dp, _dp, _lp, C.POINTER(PdhgParams), C.POINTER(_vp)]),
     "vk_hip_batch_run": (C.c_int, [_vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)]),
     "vk_hip_batch_read_state": (C.c_int, [_vp, C.c_int, C.POINTER(PdhgState)]),
     "vk_hip_batch_download": (C.c_int, [_vp, _dp, _dp]),
