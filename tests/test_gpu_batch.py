@@ -101,3 +101,33 @@ def test_mixed_batch_gives_every_instance_its_own_budget():
         assert g.status == "Optimal", g
         rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), g.x, g.y)
         assert max(rp, rd, gp) <= 1e-6 * 1.0001
+
+@pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
+def test_tail_finishes_on_the_single_solver(precision):
+    # 40 instances: once at most 8 are still running, each remaining one
+    # continues on the single-LP solver from its batch iterate.  Every
+    # instance must still meet its certificate (or prove infeasibility),
+    # the slow ones must keep their iteration budget, and an instance that
+    # hits the limit reports exactly the limit.
+    lps = [V.generate_lp(4, 0, 0, 4 + (i % 5), 20261203 + 700 + i) for i in range(36)]
+    lps += [V.generate_lp(4, 0, 0, 7, 20261203 + 345), V.generate_lp(4, 0, 0, 30, 20261203 + 39),
+            V.generate_lp(2, 400, 1000, 0, 5), V.generate_lp(1, 300, 900, 5, 11)]
+    print(f"tail handoff batch of {len(lps)} precision={precision.name}", flush=True)
+    limit = 12_000
+    opts = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision, relative_kkt_tolerance=1e-6,
+                     iteration_limit=limit)
+    got = V.solve_batch(lps, opts)
+    from test_gpu_infeasible import farkas_d0
+    for lp, g in zip(lps, got):
+        assert g.status in ("Optimal", "Infeasible", "IterationLimit"), g.status
+        assert g.iterations <= limit
+        if g.status == "Optimal":
+            rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), g.x, g.y)
+            assert max(rp, rd, gp) <= 1e-6 * 1.0001, (rp, rd, gp)
+        elif g.status == "Infeasible":
+            d0 = farkas_d0(lp, g.y)
+            assert np.isfinite(d0) and d0 > 0, d0
+        else:
+            assert g.iterations == limit
+    # the slow feasible instance (instance 345) runs past the batch's tail point
+    assert got[36].status in ("Optimal", "IterationLimit") and got[36].iterations > 1000

@@ -49,6 +49,11 @@ struct BatchArgs {
     double *pw_rows, *pw_cols;   // power-iteration scratch (FP64)
     int count;
     int power_iterations;
+    // Capacity (entries of T) of the dynamic LDS buffer the iterate kernel
+    // gathers from: an instance with max(rows, columns) <= lds_elems copies
+    // x, then y, into it before each SpMV; a larger one gathers from global
+    // memory.  0: no buffer.
+    int lds_elems;
 };
 
 // Workgroup reduction of the 8 slots into out[] (shared memory, readable by
@@ -188,6 +193,23 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
 }
 
 // Up to `iterations` iterations of every instance still running.
+// Copies entries [first, last) of v into the LDS buffer and waits for the
+// whole workgroup.
+template <class T>
+__device__ __forceinline__ void stage_vector(T *lv, const T *v, int first, int last) {
+    for (int i = first + static_cast<int>(threadIdx.x); i < last; i += kBatchBlock) lv[i - first] = v[i];
+    __syncthreads();
+}
+
+// One SpMV pass over tiles [t0, t1) gathering v (entries [first, ...) of
+// the instance), from the LDS copy when the instance fits.
+template <class T, class Epi>
+__device__ __forceinline__ void batch_pass(const CsrView<T> &M, const T *v, bool in_lds, T *lv, int first,
+                                           Epi &epi, Partials &p, double *my_lds, int t0, int t1, int wave) {
+    if (in_lds) tile_spmv_range<T, T, Epi, true>(M, lv, epi, p, my_lds, t0, t1, wave, kBatchWaves, first);
+    else tile_spmv_range(M, v, epi, p, my_lds, t0, t1, wave, kBatchWaves);
+}
+
 template <class T>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, int iterations) {
     const int id = blockIdx.x;
@@ -196,10 +218,15 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
     __shared__ double scratch[kSlots * kBatchWaves];
     __shared__ double red_r[kSlots], red_c[kSlots];
     __shared__ DevState S;
+    extern __shared__ double gathered[];   // a.lds_elems entries of T
     const InstanceRange R = a.inst[id];
     const Bufs<T> &b = a.b;
     const int tid = threadIdx.x, wave = tid / kWave;
     double *my_lds = lds[wave];
+    T *lv = reinterpret_cast<T *>(gathered);
+    // Rows and columns share the buffer: x is gathered only by the dual
+    // pass and y only by the primal pass, each copied in just before.
+    const bool in_lds = R.c1 - R.c0 <= a.lds_elems && R.r1 - R.r0 <= a.lds_elems;
     if (tid == 0) S = a.states[id];
     __syncthreads();
     for (int it = 0; it < iterations; ++it) {
@@ -210,22 +237,29 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_iterate(BatchArgs<T> a, i
         const bool fuse = may_fuse(S);
         Partials p;
         p.zero();
+        // Every write of the previous iteration is complete here (each path
+        // of the loop ends with a workgroup barrier).
+        const T *xg = b.x[(k + 1) & 1];
+        if (in_lds) stage_vector(lv, xg, R.c0, R.c1);
         if (hal) {
             HalpernDualEpi<T> de{b, S.sigma, S.inv_sigma, halpern_weights(S.ks), check_iteration(S)};
-            tile_spmv_range(a.A, b.x[(k + 1) & 1], de, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+            batch_pass(a.A, xg, in_lds, lv, R.c0, de, p, my_lds, R.ta0, R.ta1, wave);
         } else {
             DualEpi<T> de{b, S.sigma, S.inv_sigma, ref && S.restart_flag, !ref};
-            tile_spmv_range(a.A, b.x[(k + 1) & 1], de, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+            batch_pass(a.A, xg, in_lds, lv, R.c0, de, p, my_lds, R.ta0, R.ta1, wave);
         }
+        // Ends with a barrier: the dual pass no longer reads the buffer and
+        // every y+ is written.
         reduce_slots(p, red_r, scratch);
+        if (in_lds) stage_vector(lv, b.y, R.r0, R.r1);
         p.zero();
         if (hal) {
             HalpernPrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse, check_iteration(S),
                                    halpern_weights(S.ks)};
-            tile_spmv_range(a.AT, b.y, pe, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+            batch_pass(a.AT, b.y, in_lds, lv, R.r0, pe, p, my_lds, R.tt0, R.tt1, wave);
         } else {
             PrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse};
-            tile_spmv_range(a.AT, b.y, pe, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+            batch_pass(a.AT, b.y, in_lds, lv, R.r0, pe, p, my_lds, R.tt0, R.tt1, wave);
         }
         reduce_slots(p, red_c, scratch);
         if (tid == 0) finalize_step(S, red_r, red_c);
@@ -417,6 +451,28 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
     a.count = h->count;
     a.power_iterations = 100;
     if (!a.pw_rows || !a.pw_cols) return VK_ENOMEM;
+    // Dynamic LDS for the gathered vector: what the largest instance needs,
+    // up to what the device leaves beside the kernel's static LDS.
+    int max_dim = 0;
+    for (const auto &R : h->inst_host) max_dim = std::max({max_dim, R.c1 - R.c0, R.r1 - R.r0});
+    int dev = 0, per_block = 0, optin = 0;
+    VK_HIP_CHECK(hipGetDevice(&dev));
+    VK_HIP_CHECK(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+    hipFuncAttributes fa{};
+    VK_HIP_CHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_batch_iterate<T>)));
+    auto elems_for = [&](int limit) {
+        const long long room = static_cast<long long>(limit) - static_cast<long long>(fa.sharedSizeBytes);
+        return room > 0 ? static_cast<int>(std::min<long long>(max_dim, room / static_cast<long long>(sizeof(T)))) : 0;
+    };
+    a.lds_elems = elems_for(std::max(per_block, optin));
+    if (a.lds_elems > elems_for(per_block) &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_iterate<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(sizeof(T)) * a.lds_elems) != hipSuccess) {
+        (void)hipGetLastError();   // no opt-in: stay within the default per-block limit
+        a.lds_elems = elems_for(per_block);
+    }
     k_batch_setup<T><<<h->count, kBatchBlock, 0, h->stream>>>(a);
     VK_HIP_CHECK(hipGetLastError());
     VK_HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -748,9 +804,11 @@ int vk_hip_batch_run(vk_hip_batch *h, int64_t iterations_per_launch, int64_t max
     int alive = 0;
     for (int64_t launch = 0; launch < max_launches; ++launch) {
         if (h->mixed)
-            k_batch_iterate<float><<<h->count, kBatchBlock, 0, h->stream>>>(h->a32, static_cast<int>(iterations_per_launch));
+            k_batch_iterate<float><<<h->count, kBatchBlock, sizeof(float) * h->a32.lds_elems, h->stream>>>(
+                h->a32, static_cast<int>(iterations_per_launch));
         else
-            k_batch_iterate<double><<<h->count, kBatchBlock, 0, h->stream>>>(h->a64, static_cast<int>(iterations_per_launch));
+            k_batch_iterate<double><<<h->count, kBatchBlock, sizeof(double) * h->a64.lds_elems, h->stream>>>(
+                h->a64, static_cast<int>(iterations_per_launch));
         VK_HIP_CHECK(hipGetLastError());
         if (int rc = read_states(h)) return rc;
         alive = 0;

@@ -355,13 +355,21 @@ __device__ __forceinline__ void load_tile(const int *__restrict__ colp, const V 
     rs = lane < nr ? matrix_load<Keep>(row_ptr + r0 + lane) - k0 : nnz;
 }
 
-template <class V, class X, class Epi>
+//
+// XLds: x points into the workgroup's LDS and holds entries [xoff, ...) of
+// the gathered vector (x[i - xoff] is entry i); the gathers then read LDS
+// and never touch the cache hierarchy.
+template <class V, class X, class Epi, bool XLds = false>
 __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__restrict__ x, Epi &epi,
                                                 Partials &part, double *lds_generic, int t_begin,
-                                                int t_end, int wave, int nwaves) {
+                                                int t_end, int wave, int nwaves, int xoff = 0) {
     using Pre = typename Epi::Pre;
     // The slice is LDS: say so, so LDS stores do not order global loads.
     auto *lds = (__attribute__((address_space(3))) double *)(lds_generic);
+    auto gather = [&](int col) -> X {
+        if constexpr (XLds) return ((__attribute__((address_space(3))) const X *)(x))[col - xoff];
+        else return x[col];
+    };
     constexpr int kPer = kTileNnz / kWave;
     const int lane = threadIdx.x % kWave;
     const int *__restrict__ colp = A.col;
@@ -382,7 +390,7 @@ __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__
             if (lane < nr) pre = epi.prefetch(r0 + lane);
             X xv[kPer];
 #pragma unroll
-            for (int u = 0; u < kPer; ++u) xv[u] = (lane + u * kWave < nnz) ? x[c[u]] : X(0);
+            for (int u = 0; u < kPer; ++u) xv[u] = (lane + u * kWave < nnz) ? gather(c[u]) : X(0);
 #pragma unroll
             for (int u = 0; u < kPer; ++u) {
                 const int k = lane + u * kWave;
@@ -409,11 +417,11 @@ __device__ __forceinline__ void tile_spmv_range(const CsrView<V> &A, const X *__
             double acc = 0.0;
             if (keep)
                 for (int k = k0 + lane; k < k1; k += kWave)
-                    acc += static_cast<double>(valp[k]) * static_cast<double>(x[colp[k]]);
+                    acc += static_cast<double>(valp[k]) * static_cast<double>(gather(colp[k]));
             else
                 for (int k = k0 + lane; k < k1; k += kWave)
                     acc += static_cast<double>(__builtin_nontemporal_load(valp + k)) *
-                           static_cast<double>(x[__builtin_nontemporal_load(colp + k)]);
+                           static_cast<double>(gather(__builtin_nontemporal_load(colp + k)));
             acc = wave_sum(acc);
             if (lane == 0) epi.row(r0, acc, pre, part);
         }

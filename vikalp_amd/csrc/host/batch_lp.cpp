@@ -117,11 +117,71 @@ struct DeviceBatch {
     ~DeviceBatch() { vk_hip_batch_destroy(h); }
 };
 
+// Instances still running when the batch hands its tail to the single-LP
+// solver.  The batch kernel runs one workgroup per instance: once a few
+// instances are left, each runs on one CU while the rest of the GPU idles
+// (257 us per iteration for the 18,400-column block LP, against 26 us for
+// the same instance on the whole GPU).  Solving them one after another on
+// the single-LP path is then faster while fewer than about 10 are left.
+constexpr int kTailInstances = 8;
+
+// Continues running instance i of the stacked batch on the single-LP device
+// solver: same scaled data (scaled again on the device from the model,
+// bitwise the stacked scaling), same step and primal weight, warm-started
+// from the batch's scaled iterates (a restart of the Halpern or adaptive
+// sequence at that point), up to the instance's remaining iterations.
+void finish_on_single_solver(const Model &model, const Prepared &pr, const Stacked &st, int i,
+                             const SolverOptions &options, int precision, double rel_tol, Index limit,
+                             Clock::time_point started, vk_pdhg_state &state, std::vector<double> &xs,
+                             std::vector<double> &ys, double &device_ms, bool &timed) {
+    vk_hip_csr *A = nullptr;
+    check_vk(vk_hip_csr_create(model.constraint_matrix.pattern.rows, model.constraint_matrix.pattern.columns,
+                               model.constraint_matrix.pattern.row_offsets.data(),
+                               model.constraint_matrix.pattern.column_indices.data(), &A),
+             "vk_hip_csr_create");
+    struct Guard {
+        vk_hip_csr *a;
+        ~Guard() { vk_hip_csr_destroy(a); }
+    } guard{A};
+    check_vk(vk_hip_csr_set_values(A, model.constraint_matrix.values.data()), "vk_hip_csr_set_values");
+    const Scaled sc = device_scale(A, model, pr.bounds, options.pdhg_policy != PdhgPolicy::Reference);
+    vk_pdhg_params p{};
+    p.policy = options.pdhg_policy == PdhgPolicy::Adaptive ? VK_PDHG_POLICY_ADAPTIVE : VK_PDHG_POLICY_HALPERN;
+    p.precision = precision;
+    p.iteration_limit = std::max<Index>(0, limit - state.iterations);
+    p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
+    p.relative_kkt_tolerance = rel_tol;
+    p.restart_period = options.restart_period;
+    p.use_graphs = options.use_graphs ? 1 : 0;
+    p.split_passes = options.spmv_split_passes;
+    p.cache_budget_mb = options.spmv_cache_budget_mb;
+    p.col_blocks = options.spmv_col_blocks;
+    p.norm_b = st.nb[i];
+    p.norm_c = st.nc[i];
+    p.step_size = std::sqrt(state.tau * state.sigma);   // tau = eta / omega, sigma = eta * omega
+    p.primal_weight = state.primal_weight;
+    const auto c0 = static_cast<std::size_t>(st.col_starts[i]), c1 = static_cast<std::size_t>(st.col_starts[i + 1]);
+    const auto r0 = static_cast<std::size_t>(st.row_starts[i]), r1 = static_cast<std::size_t>(st.row_starts[i + 1]);
+    const std::vector<Scalar> x0(xs.begin() + c0, xs.begin() + c1), y0(ys.begin() + r0, ys.begin() + r1);
+    std::vector<Scalar> xo, yo;
+    vk_pdhg_state s1{};
+    double ms = 0.0;
+    run_device(A, sc, p, &x0, &y0, started, options, xo, yo, s1, ms, timed);
+    device_ms += ms;
+    std::copy(xo.begin(), xo.end(), xs.begin() + c0);
+    std::copy(yo.begin(), yo.end(), ys.begin() + r0);
+    const Index before = state.iterations;
+    state = s1;
+    state.iterations += before;
+}
+
 // Runs one device batch over the stacked instances (FP64 or mixed),
 // optionally warm-started from scaled iterates; returns per-instance states
-// and scaled iterates.
-void run_batch(const Stacked &st, const SolverOptions &options, int precision, double rel_tol,
-               Index iteration_limit, const std::vector<double> *x0, const std::vector<double> *y0,
+// and scaled iterates.  Under the adaptive and Halpern policies the last
+// kTailInstances running instances finish on the single-LP solver.
+void run_batch(const std::vector<const Model *> &models, const std::vector<Prepared> &prep, const Stacked &st,
+               const SolverOptions &options, int precision, double rel_tol, Index iteration_limit,
+               const std::vector<double> *x0, const std::vector<double> *y0,
                const std::vector<double> *weights_in, const std::vector<int64_t> *limits,
                Clock::time_point started, std::vector<vk_pdhg_state> &states, std::vector<double> &xs,
                std::vector<double> &ys, double &device_ms, bool &timed) {
@@ -153,6 +213,9 @@ void run_batch(const Stacked &st, const SolverOptions &options, int precision, d
     timed = false;
     device_ms = 0.0;
     int running = count;
+    // The reference policy stays in the batch: its instances must repeat
+    // their own solves bitwise.  Small batches stay too (no idle GPU).
+    const bool tail = options.pdhg_policy != PdhgPolicy::Reference && count >= 4 * kTailInstances;
     for (int64_t launch = 0; launch < launches && running > 0; ++launch) {
         if (timed_out(started, options)) {
             timed = true;
@@ -162,12 +225,19 @@ void run_batch(const Stacked &st, const SolverOptions &options, int precision, d
         double ms = 0.0;
         check_vk(vk_hip_batch_last_run_ms(batch.h, &ms), "vk_hip_batch_last_run_ms");
         device_ms += ms;
+        if (tail && running <= kTailInstances) break;
     }
     states.resize(static_cast<size_t>(count));
     for (int i = 0; i < count; ++i) check_vk(vk_hip_batch_read_state(batch.h, i, &states[i]), "vk_hip_batch_read_state");
     xs.assign(static_cast<size_t>(n), 0.0);
     ys.assign(static_cast<size_t>(m), 0.0);
     check_vk(vk_hip_batch_download(batch.h, xs.data(), ys.empty() ? nullptr : ys.data()), "vk_hip_batch_download");
+    if (!tail || timed) return;
+    for (int i = 0; i < count && !timed; ++i)
+        if (states[i].status == VK_PDHG_RUNNING)
+            finish_on_single_solver(*models[prep[i].index], prep[i], st, i, options, precision, rel_tol,
+                                    limits ? (*limits)[i] : iteration_limit, started, states[i], xs, ys, device_ms,
+                                    timed);
 }
 
 }  // namespace
@@ -220,7 +290,7 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
         double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
         if (options.pdhg_policy == PdhgPolicy::Halpern) phase_tol = std::max(phase_tol, kHalpernFp32Floor);
-        run_batch(st, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr, nullptr,
+        run_batch(models, prep, st, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr, nullptr,
                   nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
         for (std::size_t i = 0; i < prep.size(); ++i) used[i] = low[i] = states[i].iterations;
@@ -239,14 +309,14 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
             }
             ys32 = ys;
             const auto x32 = xs, y32 = ys;
-            run_batch(st, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32, &y32,
+            run_batch(models, prep, st, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32, &y32,
                       options.pdhg_policy != PdhgPolicy::Reference ? &w : nullptr, &remaining, started, states, xs,
                       ys, ms, timed);
             total_ms += ms;
             for (std::size_t i = 0; i < prep.size(); ++i) used[i] += states[i].iterations;
         }
     } else {
-        run_batch(st, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit, nullptr,
+        run_batch(models, prep, st, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit, nullptr,
                   nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;
         for (std::size_t i = 0; i < prep.size(); ++i) used[i] = states[i].iterations;

@@ -63,6 +63,18 @@ void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &op
                const std::vector<Scalar> &x, const std::vector<Scalar> &y, const Products &products,
                bool relative_mode, SolveResult &res);                      // pdhg.cpp:153-264
 void check_vk(int rc, const char *what);
+// Owns a device solver handle.
+struct DeviceSolver {
+    vk_hip_pdhg *h = nullptr;
+    ~DeviceSolver() { vk_hip_pdhg_destroy(h); }
+};
+// One device solve of the scaled LP on A with `params` (pdhg_lp.cpp),
+// optionally warm-started from scaled iterates; returns the VK_PDHG_*
+// status, the scaled iterates, the final state and the device time.
+int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &params,
+               const std::vector<Scalar> *x0, const std::vector<Scalar> *y0, Clock::time_point started,
+               const SolverOptions &options, std::vector<Scalar> &xs, std::vector<Scalar> &ys,
+               vk_pdhg_state &state, double &device_ms, bool &timed);
 // continuous_common.hpp:242-275 (dense) up to n = 4096; beyond that a
 // diagonally dominant Q with a non-negative diagonal.  `message` says why not.
 bool positive_semidefinite(const CsrMatrix &q, std::string &message);

@@ -471,17 +471,6 @@ void check_vk(int rc, const char *what) {
     if (rc != VK_OK) throw std::runtime_error(std::string(what) + ": " + vk_hip_last_error());
 }
 
-}  // namespace detail
-
-namespace {
-
-using namespace detail;
-
-struct DeviceSolver {
-    vk_hip_pdhg *h = nullptr;
-    ~DeviceSolver() { vk_hip_pdhg_destroy(h); }
-};
-
 // Runs one device solve; returns the VK_PDHG_* status and fills the scaled
 // iterates.  Honours the time limit between blocks of iterations.
 int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &params,
@@ -519,6 +508,12 @@ int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &para
     check_vk(vk_hip_pdhg_download(solver.h, xs.data(), ys.data()), "vk_hip_pdhg_download");
     return state.status;
 }
+
+}  // namespace detail
+
+namespace {
+
+using namespace detail;
 
 vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_csr *A,
                            const SolverOptions &options) {
