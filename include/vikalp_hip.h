@@ -316,6 +316,19 @@ int vk_hip_batch_create_csr(int count, const int64_t *row_starts, const int64_t 
                             const double *primal_weights, const double *norm_b,
                             const double *norm_c, const int64_t *iteration_limits,
                             const vk_pdhg_params *params, vk_hip_batch **out);
+/* Node batch on one shared matrix: `count` instances that all have A's
+ * matrix (B&B relaxations of one LP differ only in their bounds,
+ * BranchAndBound.cpp:296-297).  A is stored once; every vector argument is
+ * per instance, instance-major (c, l, u, col_scale, x0: count * cols;
+ * rl, ru, row_scale, y0: count * rows), so the instances may also differ in
+ * objective and row bounds.  Otherwise as vk_hip_batch_create. */
+int vk_hip_batch_create_shared(int count, const vk_hip_csr *A, const double *c, const double *l,
+                               const double *u, const double *rl, const double *ru,
+                               const double *row_scale, const double *col_scale,
+                               const double *x0, const double *y0, const double *step_sizes,
+                               const double *primal_weights, const double *norm_b,
+                               const double *norm_c, const int64_t *iteration_limits,
+                               const vk_pdhg_params *params, vk_hip_batch **out);
 /* Launches the persistent kernel up to `max_launches` times, each running
  * `iterations_per_launch` iterations of every instance still running; stops
  * early when all have stopped.  *running = instances still running. */

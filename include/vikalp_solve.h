@@ -103,6 +103,24 @@ int vk_lp_solve_batch(int count, const int64_t *row_starts, const int64_t *col_s
                       const vk_solver_options *options, double *x_out, double *y_out,
                       vk_solve_info *info);
 
+/* Branch-and-bound node relaxations of one LP (solver::solve_lp_nodes):
+ * replaces the per-node RelaxationOracle::solve calls of the node loop
+ * (src/solver/mip/BranchAndBound.cpp:296-297; Solver.cpp:37-51 builds a
+ * backend per node) with one batch on one device-resident matrix.  The LP
+ * is given as in vk_lp_solve; node i uses node_lower[i*n .. i*n+n) and
+ * node_upper[...] as its variable bounds (NULL: the LP's own for every
+ * node) and warm_start[i*n ...] (NULL: cold starts).  x_out (count * n),
+ * y_out (count * m, reference sign) and info (count) per node.  Returns 0,
+ * or -1 for unusable arguments. */
+int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets,
+                      const int64_t *column_indices, const double *values,
+                      const double *objective, double objective_offset,
+                      const double *row_lower, const double *row_upper,
+                      const double *var_lower, const double *var_upper, int count,
+                      const double *node_lower, const double *node_upper,
+                      const double *warm_start, const vk_solver_options *options,
+                      double *x_out, double *y_out, vk_solve_info *info);
+
 /* Measurement: builds the same scaled LP on the HIP backend, runs
  * `warm_iterations` iterations, then `iterations` more launched one by one
  * with HIP events around the two fused kernels on the solver's stream.

@@ -49,6 +49,10 @@ struct BatchArgs {
     double *pw_rows, *pw_cols;   // power-iteration scratch (FP64)
     int count;
     int power_iterations;
+    // 1: every instance runs on the same matrix (rows and columns numbered
+    // from 0), with its own slice of the vectors starting at its r0 / c0;
+    // 0: the matrix is block-diagonal in global numbering.
+    int shared;
     // Capacity (entries of T) of the dynamic LDS buffer the iterate kernel
     // gathers from: an instance with max(rows, columns) <= lds_elems copies
     // x, then y, into it before each SpMV; a larger one gathers from global
@@ -82,6 +86,27 @@ struct StoreEpiT {
     __device__ void row(int r, double acc, const Pre &, Partials &) const { y[r] = static_cast<T>(acc); }
 };
 
+// An epilogue seen through a row offset: a shared matrix numbers its rows
+// from 0 while the instance's vectors start at `off`.
+template <class Epi>
+struct Shifted {
+    Epi &inner;
+    int off;
+    using Pre = typename Epi::Pre;
+    __device__ Pre prefetch(int r) const { return inner.prefetch(r + off); }
+    __device__ void row(int r, double acc, const Pre &pre, Partials &part) const { inner.row(r + off, acc, pre, part); }
+};
+
+// One SpMV pass of an instance over tiles [t0, t1) of M, gathering v.  On a
+// shared matrix the gathered entries start at vfirst and the pass's rows
+// at rfirst; on the block-diagonal matrix both are global already.
+template <class V, class X, class Epi>
+__device__ __forceinline__ void instance_spmv(const CsrView<V> &M, const X *v, int vfirst, int rfirst, bool shared,
+                                              Epi &epi, Partials &p, double *my_lds, int t0, int t1, int wave) {
+    Shifted<Epi> se{epi, shared ? rfirst : 0};
+    tile_spmv_range(M, shared ? v + vfirst : v, se, p, my_lds, t0, t1, wave, kBatchWaves);
+}
+
 // Per instance: project the start, compute A x0 and A' y0, estimate ||A||_2
 // by power iteration when the host gave no step size, record the start
 // point's KKT error (adaptive policy) and take the first primal step.
@@ -107,9 +132,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
     p.zero();
     {
         StoreEpiT<T> e{b.ax};
-        tile_spmv_range(a.A, b.x[0], e, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        instance_spmv(a.A, b.x[0], R.c0, R.r0, a.shared, e, p, my_lds, R.ta0, R.ta1, wave);
         StoreEpiT<T> f{b.aty};
-        tile_spmv_range(a.AT, b.y, f, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        instance_spmv(a.AT, b.y, R.r0, R.c0, a.shared, f, p, my_lds, R.tt0, R.tt1, wave);
     }
     if (S.eta <= 0.0 && S.max_step <= 0.0) {
         // sigma_max(A)^2 ~ ||A'A v|| with ||v|| = 1 (deterministic start)
@@ -130,10 +155,10 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
             for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) a.pw_cols[j] *= inv;
             __syncthreads();
             StoreEpi64 e{a.pw_rows};
-            tile_spmv_range(a.A64, a.pw_cols, e, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+            instance_spmv(a.A64, a.pw_cols, R.c0, R.r0, a.shared, e, p, my_lds, R.ta0, R.ta1, wave);
             __syncthreads();
             StoreEpi64 f{a.pw_cols};
-            tile_spmv_range(a.AT64, a.pw_rows, f, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+            instance_spmv(a.AT64, a.pw_rows, R.r0, R.c0, a.shared, f, p, my_lds, R.tt0, R.tt1, wave);
             __syncthreads();
         }
         if (tid == 0) {
@@ -163,11 +188,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
         __syncthreads();
         EvalRowsEpi<T> er{b, 1.0};
         p.zero();
-        tile_spmv_range(a.A, b.xsum, er, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        instance_spmv(a.A, b.xsum, R.c0, R.r0, a.shared, er, p, my_lds, R.ta0, R.ta1, wave);
         reduce_slots(p, red_r, scratch);
         EvalColsEpi<T> ec{b, 1.0};
         p.zero();
-        tile_spmv_range(a.AT, b.ysum, ec, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        instance_spmv(a.AT, b.ysum, R.r0, R.c0, a.shared, ec, p, my_lds, R.tt0, R.tt1, wave);
         reduce_slots(p, red_c, scratch);
         if (tid == 0) check_step(S, red_r, red_c, true);
         for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock) b.xsum[j] = 0.0;
@@ -201,13 +226,22 @@ __device__ __forceinline__ void stage_vector(T *lv, const T *v, int first, int l
     __syncthreads();
 }
 
-// One SpMV pass over tiles [t0, t1) gathering v (entries [first, ...) of
-// the instance), from the LDS copy when the instance fits.
+// One SpMV pass over tiles [t0, t1) gathering v (the instance's entries
+// start at vfirst, its pass rows at rfirst), from the LDS copy when the
+// instance fits.
 template <class T, class Epi>
-__device__ __forceinline__ void batch_pass(const CsrView<T> &M, const T *v, bool in_lds, T *lv, int first,
-                                           Epi &epi, Partials &p, double *my_lds, int t0, int t1, int wave) {
-    if (in_lds) tile_spmv_range<T, T, Epi, true>(M, lv, epi, p, my_lds, t0, t1, wave, kBatchWaves, first);
-    else tile_spmv_range(M, v, epi, p, my_lds, t0, t1, wave, kBatchWaves);
+__device__ __forceinline__ void batch_pass(const CsrView<T> &M, const T *v, bool in_lds, T *lv, int vfirst,
+                                           int rfirst, bool shared, Epi &epi, Partials &p, double *my_lds, int t0,
+                                           int t1, int wave) {
+    if (in_lds) {
+        // The buffer holds entries [vfirst, ...): a shared matrix's indices
+        // are already instance-local, a block-diagonal one's are global.
+        Shifted<Epi> se{epi, shared ? rfirst : 0};
+        tile_spmv_range<T, T, Shifted<Epi>, true>(M, lv, se, p, my_lds, t0, t1, wave, kBatchWaves,
+                                                  shared ? 0 : vfirst);
+    } else {
+        instance_spmv(M, v, vfirst, rfirst, shared, epi, p, my_lds, t0, t1, wave);
+    }
 }
 
 // Two workgroups per CU (4 waves per SIMD, at most 128 VGPRs, a few dozen
@@ -247,10 +281,10 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_batch_iterate(BatchArgs<T> a
         if (in_lds) stage_vector(lv, xg, R.c0, R.c1);
         if (hal) {
             HalpernDualEpi<T> de{b, S.sigma, S.inv_sigma, halpern_weights(S.ks), check_iteration(S)};
-            batch_pass(a.A, xg, in_lds, lv, R.c0, de, p, my_lds, R.ta0, R.ta1, wave);
+            batch_pass(a.A, xg, in_lds, lv, R.c0, R.r0, a.shared, de, p, my_lds, R.ta0, R.ta1, wave);
         } else {
             DualEpi<T> de{b, S.sigma, S.inv_sigma, ref && S.restart_flag, !ref};
-            batch_pass(a.A, xg, in_lds, lv, R.c0, de, p, my_lds, R.ta0, R.ta1, wave);
+            batch_pass(a.A, xg, in_lds, lv, R.c0, R.r0, a.shared, de, p, my_lds, R.ta0, R.ta1, wave);
         }
         // Ends with a barrier: the dual pass no longer reads the buffer and
         // every y+ is written.
@@ -260,10 +294,10 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_batch_iterate(BatchArgs<T> a
         if (hal) {
             HalpernPrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse, check_iteration(S),
                                    halpern_weights(S.ks)};
-            batch_pass(a.AT, b.y, in_lds, lv, R.r0, pe, p, my_lds, R.tt0, R.tt1, wave);
+            batch_pass(a.AT, b.y, in_lds, lv, R.r0, R.c0, a.shared, pe, p, my_lds, R.tt0, R.tt1, wave);
         } else {
             PrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse};
-            batch_pass(a.AT, b.y, in_lds, lv, R.r0, pe, p, my_lds, R.tt0, R.tt1, wave);
+            batch_pass(a.AT, b.y, in_lds, lv, R.r0, R.c0, a.shared, pe, p, my_lds, R.tt0, R.tt1, wave);
         }
         reduce_slots(p, red_c, scratch);
         if (tid == 0) finalize_step(S, red_r, red_c);
@@ -299,11 +333,11 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_batch_iterate(BatchArgs<T> a
         const double inv_ns = 1.0 / static_cast<double>(S.ns);
         EvalRowsEpi<T> er{b, inv_ns};
         p.zero();
-        tile_spmv_range(a.A, b.xsum, er, p, my_lds, R.ta0, R.ta1, wave, kBatchWaves);
+        instance_spmv(a.A, b.xsum, R.c0, R.r0, a.shared, er, p, my_lds, R.ta0, R.ta1, wave);
         reduce_slots(p, red_r, scratch);
         EvalColsEpi<T> ec{b, inv_ns};
         p.zero();
-        tile_spmv_range(a.AT, b.ysum, ec, p, my_lds, R.tt0, R.tt1, wave, kBatchWaves);
+        instance_spmv(a.AT, b.ysum, R.r0, R.c0, a.shared, ec, p, my_lds, R.tt0, R.tt1, wave);
         reduce_slots(p, red_c, scratch);
         if (tid == 0) check_step(S, red_r, red_c, false);
         __syncthreads();
@@ -359,7 +393,9 @@ using namespace vk;
 struct vk_hip_batch {
     vk_pdhg_params params{};
     int count = 0;
-    int64_t m = 0, n = 0, nnz = 0;
+    int64_t m = 0, n = 0, nnz = 0;     // stacked vector lengths; the matrix's nonzeros
+    int64_t mat_rows = 0, mat_cols = 0;  // the stored matrix (one instance's when shared)
+    bool shared = false;
     bool mixed = false;
     hipStream_t stream = nullptr;
     std::vector<void *> allocations;
@@ -444,10 +480,12 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
         VK_HIP_CHECK(hipStreamSynchronize(h->stream));
     }
     (void)hipFree(stage);
-    a.A = CsrView<T>{h->rp, h->ci, val_a, h->tiles_a, static_cast<int>(m), h->ntiles_a, 0};
-    a.AT = CsrView<T>{h->trp, h->tci, val_at, h->tiles_at, static_cast<int>(n), h->ntiles_at, 0};
-    a.A64 = CsrView<double>{h->rp, h->ci, h->val, h->tiles_a, static_cast<int>(m), h->ntiles_a, 0};
-    a.AT64 = CsrView<double>{h->trp, h->tci, h->tval, h->tiles_at, static_cast<int>(n), h->ntiles_at, 0};
+    const int mr = static_cast<int>(h->mat_rows), mc = static_cast<int>(h->mat_cols);
+    a.A = CsrView<T>{h->rp, h->ci, val_a, h->tiles_a, mr, h->ntiles_a, 0};
+    a.AT = CsrView<T>{h->trp, h->tci, val_at, h->tiles_at, mc, h->ntiles_at, 0};
+    a.A64 = CsrView<double>{h->rp, h->ci, h->val, h->tiles_a, mr, h->ntiles_a, 0};
+    a.AT64 = CsrView<double>{h->trp, h->tci, h->tval, h->tiles_at, mc, h->ntiles_at, 0};
+    a.shared = h->shared ? 1 : 0;
     a.inst = h->inst;
     a.states = h->states;
     a.pw_rows = h->alloc<double>(m);
@@ -497,6 +535,7 @@ struct MatrixSource {
     const int64_t *row_offsets = nullptr, *column_indices = nullptr;
     const double *values = nullptr;
     const vk_hip_csr *A = nullptr;
+    bool shared = false;   // A is every instance's matrix (equal instance sizes)
 };
 
 int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts, const MatrixSource &src,
@@ -526,8 +565,16 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
     for (int i = 0; i < count; ++i)
         VK_REQUIRE(row_starts[i + 1] >= row_starts[i] && col_starts[i + 1] > col_starts[i],
                    "vk_hip_batch_create: every instance needs >= 1 column and ordered ranges");
-    int64_t nnz = 0;
-    if (src.A) {
+    int64_t nnz = 0, mat_rows = m, mat_cols = n;
+    if (src.A && src.shared) {
+        mat_rows = src.A->rows;
+        mat_cols = src.A->cols;
+        for (int i = 0; i < count; ++i)
+            VK_REQUIRE(row_starts[i + 1] - row_starts[i] == mat_rows && col_starts[i + 1] - col_starts[i] == mat_cols,
+                       "vk_hip_batch_create: every instance must have the shared matrix's dimensions");
+        VK_REQUIRE(src.A->has_values, "vk_hip_batch_create: matrix values not set");
+        nnz = src.A->nnz;
+    } else if (src.A) {
         VK_REQUIRE(src.A->rows == m && src.A->cols == n, "vk_hip_batch_create: matrix dimensions differ from the ranges");
         VK_REQUIRE(src.A->has_values, "vk_hip_batch_create: matrix values not set");
         nnz = src.A->nnz;
@@ -561,15 +608,18 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
     h->m = m;
     h->n = n;
     h->nnz = nnz;
+    h->mat_rows = mat_rows;
+    h->mat_cols = mat_cols;
+    h->shared = src.shared;
     h->mixed = params->precision == VK_PREC_MIXED;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
         set_error("vk_hip_batch_create: stream/event creation failed");
         return fail(VK_EDEVICE);
     }
-    h->rp = h->alloc<int>(m + 1);
+    h->rp = h->alloc<int>(mat_rows + 1);
     h->ci = h->alloc<int>(nnz);
-    h->trp = h->alloc<int>(n + 1);
+    h->trp = h->alloc<int>(mat_cols + 1);
     h->tci = h->alloc<int>(nnz);
     h->val = h->alloc<double>(nnz);
     h->tval = h->alloc<double>(nnz);
@@ -585,7 +635,7 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
             return fail(VK_ENOMEM);
         }
     // Host copies of both offset arrays: the tile tables are cut on the host.
-    std::vector<int> rp(static_cast<size_t>(m + 1)), trp(static_cast<size_t>(n + 1));
+    std::vector<int> rp(static_cast<size_t>(mat_rows + 1)), trp(static_cast<size_t>(mat_cols + 1));
     if (src.A) {
         // The handle's A and A' (its transpose lists each column's rows in
         // increasing order, as the host counting sort below does); a device
@@ -605,19 +655,20 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
             set_error("vk_hip_batch_create: upload failed");
             return fail(VK_EDEVICE);
         }
-        if (m > 0) k_check_instances<<<bgrid(m), 256, 0, st>>>(static_cast<int>(m), count, A->a.row_ptr, A->a.col,
-                                                              starts, starts + count + 1, bad);
+        if (m > 0 && !src.shared)
+            k_check_instances<<<bgrid(m), 256, 0, st>>>(static_cast<int>(m), count, A->a.row_ptr, A->a.col, starts,
+                                                        starts + count + 1, bad);
         int bad_host = 0;
         const auto d2d = hipMemcpyDeviceToDevice;
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(h->rp, A->a.row_ptr, sizeof(int) * (m + 1), d2d, st) != hipSuccess ||
-            hipMemcpyAsync(h->trp, A->at.row_ptr, sizeof(int) * (n + 1), d2d, st) != hipSuccess ||
+            hipMemcpyAsync(h->rp, A->a.row_ptr, sizeof(int) * (mat_rows + 1), d2d, st) != hipSuccess ||
+            hipMemcpyAsync(h->trp, A->at.row_ptr, sizeof(int) * (mat_cols + 1), d2d, st) != hipSuccess ||
             (nnz && (hipMemcpyAsync(h->ci, A->a.col, sizeof(int) * nnz, d2d, st) != hipSuccess ||
                      hipMemcpyAsync(h->tci, A->at.col, sizeof(int) * nnz, d2d, st) != hipSuccess ||
                      hipMemcpyAsync(h->val, A->a.val, sizeof(double) * nnz, d2d, st) != hipSuccess ||
                      hipMemcpyAsync(h->tval, A->at.val, sizeof(double) * nnz, d2d, st) != hipSuccess)) ||
-            hipMemcpyAsync(rp.data(), A->a.row_ptr, sizeof(int) * (m + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(trp.data(), A->at.row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(rp.data(), A->a.row_ptr, sizeof(int) * (mat_rows + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(trp.data(), A->at.row_ptr, sizeof(int) * (mat_cols + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(&bad_host, bad, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess) {
             set_error("vk_hip_batch_create: device copy failed");
@@ -662,12 +713,26 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
     }
     std::vector<int2> ta, tt;
     h->inst_host.resize(static_cast<size_t>(count));
+    if (src.shared) {
+        // One tile table over the shared matrix, used by every instance.
+        append_tiles(rp.data(), 0, static_cast<int>(mat_rows), ta);
+        append_tiles(trp.data(), 0, static_cast<int>(mat_cols), tt);
+        ta.pop_back();
+        tt.pop_back();
+    }
     for (int i = 0; i < count; ++i) {
         InstanceRange &R = h->inst_host[i];
         R.r0 = static_cast<int>(row_starts[i]);
         R.r1 = static_cast<int>(row_starts[i + 1]);
         R.c0 = static_cast<int>(col_starts[i]);
         R.c1 = static_cast<int>(col_starts[i + 1]);
+        if (src.shared) {
+            R.ta0 = 0;
+            R.ta1 = static_cast<int>(ta.size());
+            R.tt0 = 0;
+            R.tt1 = static_cast<int>(tt.size());
+            continue;
+        }
         // Each range contributes its tile starts; its end is the next
         // range's start (ranges are contiguous) or the final sentinel.
         R.ta0 = static_cast<int>(ta.size());
@@ -681,8 +746,8 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
         R.tt1 = R.tt0 + static_cast<int>(tmp.size()) - 1;
         tt.insert(tt.end(), tmp.begin(), tmp.end() - 1);
     }
-    ta.push_back(make_int2(static_cast<int>(m), static_cast<int>(nnz)));
-    tt.push_back(make_int2(static_cast<int>(n), static_cast<int>(nnz)));
+    ta.push_back(make_int2(static_cast<int>(mat_rows), static_cast<int>(nnz)));
+    tt.push_back(make_int2(static_cast<int>(mat_cols), static_cast<int>(nnz)));
     h->ntiles_a = static_cast<int>(ta.size()) - 1;
     h->ntiles_at = static_cast<int>(tt.size()) - 1;
     h->tiles_a = h->alloc<int2>(ta.size());
@@ -787,6 +852,25 @@ int vk_hip_batch_create(int count, const int64_t *row_starts, const int64_t *col
     src.values = values;
     return batch_create(count, row_starts, col_starts, src, c, l, u, rl, ru, row_scale, col_scale, x0, y0,
                         step_sizes, primal_weights, norm_b, norm_c, iteration_limits, params, out);
+}
+
+int vk_hip_batch_create_shared(int count, const vk_hip_csr *A, const double *c, const double *l, const double *u,
+                               const double *rl, const double *ru, const double *row_scale,
+                               const double *col_scale, const double *x0, const double *y0,
+                               const double *step_sizes, const double *primal_weights, const double *norm_b,
+                               const double *norm_c, const int64_t *iteration_limits,
+                               const vk_pdhg_params *params, vk_hip_batch **out) {
+    VK_REQUIRE(A && count > 0, "vk_hip_batch_create_shared: null matrix or no instance");
+    std::vector<int64_t> rs(static_cast<size_t>(count) + 1), cs(static_cast<size_t>(count) + 1);
+    for (int i = 0; i <= count; ++i) {
+        rs[i] = static_cast<int64_t>(i) * A->rows;
+        cs[i] = static_cast<int64_t>(i) * A->cols;
+    }
+    MatrixSource src;
+    src.A = A;
+    src.shared = true;
+    return batch_create(count, rs.data(), cs.data(), src, c, l, u, rl, ru, row_scale, col_scale, x0, y0, step_sizes,
+                        primal_weights, norm_b, norm_c, iteration_limits, params, out);
 }
 
 int vk_hip_batch_create_csr(int count, const int64_t *row_starts, const int64_t *col_starts,

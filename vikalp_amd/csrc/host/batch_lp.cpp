@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <span>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -36,9 +37,75 @@ struct Stacked {
     std::vector<int64_t> row_starts, col_starts;
     Scaled sc;                                   // concatenated; no host values
     std::vector<double> steps, weights, nb, nc;  // per instance
+    std::vector<double> warm;                    // scaled start x of every instance, or empty
     vk_hip_csr *A = nullptr;
+    bool shared = false;   // A is one matrix every instance runs on
     ~Stacked() { vk_hip_csr_destroy(A); }
 };
+
+// Node batch: every instance is `model` with its own variable bounds (and
+// warm start).  The matrix is uploaded, transposed and scaled once; the
+// scaling of each node's bounds is the device scaling solve_lp applies to
+// them, so each node starts from exactly the data of its own solve_lp.
+void stack_shared(const Model &model, const std::vector<Prepared> &prep, const std::vector<std::vector<Scalar>> &warm,
+                  const SolverOptions &options, Stacked &st) {
+    const int count = static_cast<int>(prep.size());
+    const int64_t m = model.num_constraints(), n = model.num_variables();
+    st.shared = true;
+    st.row_starts.resize(count + 1);
+    st.col_starts.resize(count + 1);
+    for (int i = 0; i <= count; ++i) {
+        st.row_starts[i] = i * m;
+        st.col_starts[i] = i * n;
+    }
+    const auto &pat = model.constraint_matrix.pattern;
+    check_vk(vk_hip_csr_create(pat.rows, pat.columns, pat.row_offsets.data(),
+                               pat.column_indices.empty() ? nullptr : pat.column_indices.data(), &st.A),
+             "vk_hip_csr_create");
+    check_vk(vk_hip_csr_set_values(st.A, model.constraint_matrix.values.empty() ? nullptr
+                                                                               : model.constraint_matrix.values.data()),
+             "vk_hip_csr_set_values");
+    const bool equilibrated = options.pdhg_policy != PdhgPolicy::Reference;
+    Scaled &sc = st.sc;
+    bool any_warm = false;
+    for (int i = 0; i < count; ++i) {
+        // Scaling A depends on A only: every call leaves the same scaled
+        // matrix; the bounds come back scaled by this node's own sequence.
+        const Scaled s = device_scale(st.A, model, prep[i].bounds, equilibrated);
+        for (auto [dst, src] : {std::pair{&sc.c, &s.c}, std::pair{&sc.l, &s.l}, std::pair{&sc.u, &s.u},
+                                std::pair{&sc.s, &s.s}, std::pair{&sc.rl, &s.rl}, std::pair{&sc.ru, &s.ru},
+                                std::pair{&sc.r, &s.r}})
+            dst->insert(dst->end(), src->begin(), src->end());
+        any_warm = any_warm || !warm[static_cast<std::size_t>(prep[i].index)].empty();
+    }
+    if (any_warm) {
+        st.warm.assign(static_cast<std::size_t>(count * n), 0.0);
+        for (int i = 0; i < count; ++i) {
+            const auto &w = warm[static_cast<std::size_t>(prep[i].index)];
+            for (std::size_t j = 0; j < w.size(); ++j)
+                st.warm[static_cast<std::size_t>(i * n) + j] = w[j] / sc.s[static_cast<std::size_t>(i * n) + j];   // pdhg.cpp:64-69
+        }
+    }
+    // The step of solve_lp: the reference operator bound (pdhg.cpp:75-78),
+    // or 0.99 over the power-iteration estimate of ||A||_2.
+    double step = 0.0;
+    if (!equilibrated) {
+        double bound = 0.0;
+        check_vk(vk_hip_csr_operator_bound(st.A, &bound), "vk_hip_csr_operator_bound");
+        step = 0.99 / std::max<Scalar>(1.0, bound);
+    } else {
+        double norm = 0.0;
+        check_vk(vk_hip_csr_norm2_estimate(st.A, 100, &norm), "vk_hip_csr_norm2_estimate");
+        step = 0.99 / std::max(norm, 1e-12);
+    }
+    st.steps.assign(count, step);
+    const std::vector<Scalar> c0(sc.c.begin(), sc.c.begin() + n), rl0(sc.rl.begin(), sc.rl.begin() + m),
+        ru0(sc.ru.begin(), sc.ru.begin() + m);
+    const Scalar cn = norm2(c0), bn = row_bound_norm(rl0, ru0);
+    st.weights.assign(count, equilibrated && cn > 1e-10 && bn > 1e-10 ? cn / bn : 1.0);
+    st.nb.assign(count, row_bound_norm(model.constraint_lower, model.constraint_upper));
+    st.nc.assign(count, norm2(model.linear_objective));
+}
 
 void stack(const std::vector<const Model *> &models, const std::vector<Prepared> &prep, const SolverOptions &options,
            Stacked &st) {
@@ -199,13 +266,19 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
     p.restart_period = options.restart_period;
     const std::vector<double> &weights = weights_in ? *weights_in : st.weights;
     DeviceBatch batch;
-    check_vk(vk_hip_batch_create_csr(count, st.row_starts.data(), st.col_starts.data(), st.A, sc.c.data(),
-                                     sc.l.data(), sc.u.data(), m ? sc.rl.data() : nullptr,
-                                     m ? sc.ru.data() : nullptr, m ? sc.r.data() : nullptr, sc.s.data(),
-                                     x0 ? x0->data() : nullptr, y0 && !y0->empty() ? y0->data() : nullptr,
-                                     st.steps.data(), weights.data(), st.nb.data(), st.nc.data(),
-                                     limits ? limits->data() : nullptr, &p, &batch.h),
-             "vk_hip_batch_create_csr");
+    const double *rl = m ? sc.rl.data() : nullptr, *ru = m ? sc.ru.data() : nullptr, *r = m ? sc.r.data() : nullptr;
+    const double *xs0 = x0 ? x0->data() : nullptr, *ys0 = y0 && !y0->empty() ? y0->data() : nullptr;
+    if (st.shared)
+        check_vk(vk_hip_batch_create_shared(count, st.A, sc.c.data(), sc.l.data(), sc.u.data(), rl, ru, r, sc.s.data(),
+                                            xs0, ys0, st.steps.data(), weights.data(), st.nb.data(), st.nc.data(),
+                                            limits ? limits->data() : nullptr, &p, &batch.h),
+                 "vk_hip_batch_create_shared");
+    else
+        check_vk(vk_hip_batch_create_csr(count, st.row_starts.data(), st.col_starts.data(), st.A, sc.c.data(),
+                                         sc.l.data(), sc.u.data(), rl, ru, r, sc.s.data(), xs0, ys0, st.steps.data(),
+                                         weights.data(), st.nb.data(), st.nc.data(),
+                                         limits ? limits->data() : nullptr, &p, &batch.h),
+                 "vk_hip_batch_create_csr");
     // Launch granularity: 256 iterations per launch keeps each kernel short
     // and lets the host honour the time limit between launches.
     constexpr int64_t kPerLaunch = 256;
@@ -238,6 +311,109 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
             finish_on_single_solver(*models[prep[i].index], prep[i], st, i, options, precision, rel_tol,
                                     limits ? (*limits)[i] : iteration_limit, started, states[i], xs, ys, device_ms,
                                     timed);
+}
+
+// The phases of a stacked batch (FP32 then FP64 for mixed precision) and
+// each instance's postsolve, into results[prep[i].index].
+void run_stacked(const std::vector<const Model *> &models, const std::vector<Prepared> &prep, const Stacked &st,
+                 const SolverOptions &options, Clock::time_point started, std::vector<SolveResult> &results) {
+    const std::vector<double> *warm0 = st.warm.empty() ? nullptr : &st.warm;
+
+    std::vector<vk_pdhg_state> states;
+    std::vector<double> xs, ys;
+    double ms = 0.0, total_ms = 0.0;
+    bool timed = false;
+    std::vector<Index> used(prep.size(), 0), low(prep.size(), 0);
+    // Instances the FP32 phase of a mixed batch proved infeasible, and their
+    // scaled row duals (the certificate) from that phase.
+    std::vector<char> infeasible32(prep.size(), 0);
+    std::vector<double> ys32;
+    if (options.pdhg_precision == PdhgPrecision::Mixed) {
+        double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
+        if (options.pdhg_policy == PdhgPolicy::Halpern) phase_tol = std::max(phase_tol, kHalpernFp32Floor);
+        run_batch(models, prep, st, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, warm0, nullptr, nullptr,
+                  nullptr, started, states, xs, ys, ms, timed);
+        total_ms += ms;
+        for (std::size_t i = 0; i < prep.size(); ++i) used[i] = low[i] = states[i].iterations;
+        if (!timed) {
+            // FP64 refinement of every instance, warm-started, each with the
+            // iterations its FP32 phase left (an instance that used its whole
+            // budget stops at once with IterationLimit).
+            std::vector<double> w(prep.size());
+            std::vector<int64_t> remaining(prep.size());
+            Index max_remaining = 0;
+            for (std::size_t i = 0; i < prep.size(); ++i) {
+                w[i] = states[i].primal_weight;
+                infeasible32[i] = states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
+                remaining[i] = infeasible32[i] ? 0 : std::max<Index>(0, options.iteration_limit - used[i]);
+                max_remaining = std::max<Index>(max_remaining, remaining[i]);
+            }
+            ys32 = ys;
+            const auto x32 = xs, y32 = ys;
+            run_batch(models, prep, st, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32, &y32,
+                      options.pdhg_policy != PdhgPolicy::Reference ? &w : nullptr, &remaining, started, states, xs,
+                      ys, ms, timed);
+            total_ms += ms;
+            for (std::size_t i = 0; i < prep.size(); ++i) used[i] += states[i].iterations;
+        }
+    } else {
+        run_batch(models, prep, st, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit, warm0,
+                  nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
+        total_ms += ms;
+        for (std::size_t i = 0; i < prep.size(); ++i) used[i] = states[i].iterations;
+    }
+    // Unscaled iterates of all instances, then the postsolve products of
+    // the stacked matrix on the device (each instance's rows and columns
+    // see only its own entries).
+    const Scaled &sc = st.sc;
+    std::vector<Scalar> xall(xs.size()), yall(ys.size()), c_all;
+    for (std::size_t j = 0; j < xs.size(); ++j) xall[j] = xs[j] * sc.s[j];
+    for (std::size_t i = 0; i < prep.size(); ++i) {
+        const Model &model = *models[prep[i].index];
+        c_all.insert(c_all.end(), model.linear_objective.begin(), model.linear_objective.end());
+        const auto r0 = static_cast<std::size_t>(st.row_starts[i]), r1 = static_cast<std::size_t>(st.row_starts[i + 1]);
+        const std::vector<double> &ysrc = infeasible32[i] ? ys32 : ys;
+        for (std::size_t r = r0; r < r1; ++r) yall[r] = ysrc[r] * sc.r[r];
+    }
+    Products all;
+    all.ax.resize(yall.size());
+    all.g.resize(xall.size());
+    if (st.shared) {
+        // One matrix: one product pair per node.
+        for (std::size_t i = 0; i < prep.size(); ++i) {
+            const auto r0 = static_cast<std::size_t>(st.row_starts[i]), c0 = static_cast<std::size_t>(st.col_starts[i]);
+            check_vk(vk_hip_csr_postsolve_products(st.A, xall.data() + c0, yall.data() + r0, c_all.data() + c0,
+                                                   all.ax.data() + r0, all.g.data() + c0),
+                     "vk_hip_csr_postsolve_products");
+        }
+    } else {
+        check_vk(vk_hip_csr_postsolve_products(st.A, xall.data(), yall.data(), c_all.data(), all.ax.data(),
+                                               all.g.data()),
+                 "vk_hip_csr_postsolve_products");
+    }
+    for (std::size_t i = 0; i < prep.size(); ++i) {
+        const Model &model = *models[prep[i].index];
+        const auto row0 = st.row_starts[i], col0 = st.col_starts[i];
+        const Index m = model.num_constraints(), n = model.num_variables();
+        SolveResult res;
+        res.solver = "PDHG";
+        res.backend = "hip";
+        res.iterations = used[i];
+        res.low_precision_iterations = low[i];
+        res.device_seconds = total_ms * 1e-3;
+        const bool infeasible = infeasible32[i] || states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
+        if (timed && states[i].status == VK_PDHG_RUNNING && !infeasible) res.status = SolveStatus::TimeLimit;
+        else if (states[i].status == VK_PDHG_CONVERGED && !infeasible) res.status = SolveStatus::Optimal;
+        const std::vector<Scalar> x(xall.begin() + col0, xall.begin() + col0 + n);
+        const std::vector<Scalar> y(yall.begin() + row0, yall.begin() + row0 + m);
+        Products pr;
+        pr.ax.assign(all.ax.begin() + row0, all.ax.begin() + row0 + m);
+        pr.g.assign(all.g.begin() + col0, all.g.begin() + col0 + n);
+        postsolve(model, prep[i].bounds, options, x, y, pr, options.relative_kkt_tolerance > 0.0, res);
+        if (infeasible) report_infeasible(model, prep[i].bounds, y, res);
+        res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
+        results[prep[i].index] = std::move(res);
+    }
 }
 
 }  // namespace
@@ -277,91 +453,67 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     if (prep.empty()) return results;
     Stacked st;
     stack(models, prep, options, st);
+    run_stacked(models, prep, st, options, started, results);
+    return results;
+}
 
-    std::vector<vk_pdhg_state> states;
-    std::vector<double> xs, ys;
-    double ms = 0.0, total_ms = 0.0;
-    bool timed = false;
-    std::vector<Index> used(prep.size(), 0), low(prep.size(), 0);
-    // Instances the FP32 phase of a mixed batch proved infeasible, and their
-    // scaled row duals (the certificate) from that phase.
-    std::vector<char> infeasible32(prep.size(), 0);
-    std::vector<double> ys32;
-    if (options.pdhg_precision == PdhgPrecision::Mixed) {
-        double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
-        if (options.pdhg_policy == PdhgPolicy::Halpern) phase_tol = std::max(phase_tol, kHalpernFp32Floor);
-        run_batch(models, prep, st, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, nullptr, nullptr, nullptr,
-                  nullptr, started, states, xs, ys, ms, timed);
-        total_ms += ms;
-        for (std::size_t i = 0; i < prep.size(); ++i) used[i] = low[i] = states[i].iterations;
-        if (!timed) {
-            // FP64 refinement of every instance, warm-started, each with the
-            // iterations its FP32 phase left (an instance that used its whole
-            // budget stops at once with IterationLimit).
-            std::vector<double> w(prep.size());
-            std::vector<int64_t> remaining(prep.size());
-            Index max_remaining = 0;
-            for (std::size_t i = 0; i < prep.size(); ++i) {
-                w[i] = states[i].primal_weight;
-                infeasible32[i] = states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
-                remaining[i] = infeasible32[i] ? 0 : std::max<Index>(0, options.iteration_limit - used[i]);
-                max_remaining = std::max<Index>(max_remaining, remaining[i]);
-            }
-            ys32 = ys;
-            const auto x32 = xs, y32 = ys;
-            run_batch(models, prep, st, options, VK_PREC_FP64, options.relative_kkt_tolerance, max_remaining, &x32, &y32,
-                      options.pdhg_policy != PdhgPolicy::Reference ? &w : nullptr, &remaining, started, states, xs,
-                      ys, ms, timed);
-            total_ms += ms;
-            for (std::size_t i = 0; i < prep.size(); ++i) used[i] += states[i].iterations;
+std::vector<SolveResult> solve_lp_nodes(const Model &model, const std::vector<std::vector<Scalar>> &lower,
+                                        const std::vector<std::vector<Scalar>> &upper,
+                                        const std::vector<std::vector<Scalar>> &warm_start,
+                                        const SolverOptions &options) {
+    const auto started = Clock::now();
+    const std::size_t count = std::max(lower.size(), upper.size());
+    std::vector<SolveResult> results(count);
+    if (count == 0) return results;
+    auto node_vec = [](const std::vector<std::vector<Scalar>> &v, std::size_t i) -> std::span<const Scalar> {
+        return i < v.size() ? std::span<const Scalar>(v[i]) : std::span<const Scalar>();
+    };
+    // solve_lp's entry checks (pdhg.cpp:269-293): the model's once, each
+    // node's bounds and warm start on their own.
+    std::string model_error;
+    SolveStatus model_status = SolveStatus::NotSolved;
+    if (const auto errors = model.validate(); !errors.empty()) {
+        model_status = SolveStatus::InvalidModel;
+        model_error = errors.front();
+    } else if (model.problem_class() != ProblemClass::LP && model.problem_class() != ProblemClass::MILP) {
+        model_status = SolveStatus::UnsupportedModel;
+        model_error = "model class does not match the selected continuous solver";
+    } else if (model.num_variables() == 0) {
+        model_status = SolveStatus::InvalidModel;
+        model_error = "LP requires at least one variable";
+    } else if (!valid_options(options)) {
+        model_status = SolveStatus::InvalidModel;
+        model_error = "solver limits and tolerances must be non-negative";
+    }
+    std::vector<Prepared> prep;
+    std::vector<std::vector<Scalar>> warm(count);
+    for (std::size_t i = 0; i < count; ++i) {
+        if (model_status != SolveStatus::NotSolved) {
+            results[i] = status_result(model_status, model_error);
+            continue;
         }
-    } else {
-        run_batch(models, prep, st, options, VK_PREC_FP64, options.relative_kkt_tolerance, options.iteration_limit, nullptr,
-                  nullptr, nullptr, nullptr, started, states, xs, ys, ms, timed);
-        total_ms += ms;
-        for (std::size_t i = 0; i < prep.size(); ++i) used[i] = states[i].iterations;
+        Prepared pr;
+        pr.index = i;
+        std::string message;
+        if (!resolve_bounds(model, node_vec(lower, i), node_vec(upper, i), pr.bounds, message)) {
+            results[i] = status_result(SolveStatus::InvalidModel, message);
+            continue;
+        }
+        const auto w = node_vec(warm_start, i);
+        bool ok = w.empty() || static_cast<Index>(w.size()) == model.num_variables();
+        for (const Scalar v : w) ok = ok && std::isfinite(v);
+        if (!ok) {
+            results[i] = status_result(SolveStatus::InvalidModel, "warm start must be empty or finite and sized to the model");
+            continue;
+        }
+        warm[i].assign(w.begin(), w.end());
+        prep.push_back(std::move(pr));
     }
-    // Unscaled iterates of all instances, then the postsolve products of
-    // the stacked matrix on the device (each instance's rows and columns
-    // see only its own entries).
-    const Scaled &sc = st.sc;
-    std::vector<Scalar> xall(xs.size()), yall(ys.size()), c_all;
-    for (std::size_t j = 0; j < xs.size(); ++j) xall[j] = xs[j] * sc.s[j];
-    for (std::size_t i = 0; i < prep.size(); ++i) {
-        const Model &model = *models[prep[i].index];
-        c_all.insert(c_all.end(), model.linear_objective.begin(), model.linear_objective.end());
-        const auto r0 = static_cast<std::size_t>(st.row_starts[i]), r1 = static_cast<std::size_t>(st.row_starts[i + 1]);
-        const std::vector<double> &ysrc = infeasible32[i] ? ys32 : ys;
-        for (std::size_t r = r0; r < r1; ++r) yall[r] = ysrc[r] * sc.r[r];
-    }
-    Products all;
-    all.ax.resize(yall.size());
-    all.g.resize(xall.size());
-    check_vk(vk_hip_csr_postsolve_products(st.A, xall.data(), yall.data(), c_all.data(), all.ax.data(), all.g.data()),
-             "vk_hip_csr_postsolve_products");
-    for (std::size_t i = 0; i < prep.size(); ++i) {
-        const Model &model = *models[prep[i].index];
-        const auto row0 = st.row_starts[i], col0 = st.col_starts[i];
-        const Index m = model.num_constraints(), n = model.num_variables();
-        SolveResult res;
-        res.solver = "PDHG";
-        res.backend = "hip";
-        res.iterations = used[i];
-        res.low_precision_iterations = low[i];
-        res.device_seconds = total_ms * 1e-3;
-        const bool infeasible = infeasible32[i] || states[i].status == VK_PDHG_PRIMAL_INFEASIBLE;
-        if (timed && states[i].status == VK_PDHG_RUNNING && !infeasible) res.status = SolveStatus::TimeLimit;
-        else if (states[i].status == VK_PDHG_CONVERGED && !infeasible) res.status = SolveStatus::Optimal;
-        const std::vector<Scalar> x(xall.begin() + col0, xall.begin() + col0 + n);
-        const std::vector<Scalar> y(yall.begin() + row0, yall.begin() + row0 + m);
-        Products pr;
-        pr.ax.assign(all.ax.begin() + row0, all.ax.begin() + row0 + m);
-        pr.g.assign(all.g.begin() + col0, all.g.begin() + col0 + n);
-        postsolve(model, prep[i].bounds, options, x, y, pr, options.relative_kkt_tolerance > 0.0, res);
-        if (infeasible) report_infeasible(model, prep[i].bounds, y, res);
-        res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
-        results[prep[i].index] = std::move(res);
-    }
+    if (prep.empty()) return results;
+    Stacked st;
+    stack_shared(model, prep, warm, options, st);
+    const std::vector<const Model *> models(count, &model);
+    run_stacked(models, prep, st, options, started, results);
     return results;
 }
 

@@ -223,6 +223,48 @@ int vk_lp_solve_batch(int count, const int64_t *row_starts, const int64_t *col_s
     return 0;
 }
 
+int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets, const int64_t *column_indices,
+                      const double *values, const double *objective, double objective_offset,
+                      const double *row_lower, const double *row_upper, const double *var_lower,
+                      const double *var_upper, int count, const double *node_lower, const double *node_upper,
+                      const double *warm_start, const vk_solver_options *options, double *x_out, double *y_out,
+                      vk_solve_info *info) {
+    if (!info || !options || count <= 0) return -1;
+    Model model;
+    if (!build_model(m, n, row_offsets, column_indices, values, objective, objective_offset, row_lower,
+                     row_upper, var_lower, var_upper, model))
+        return -1;
+    const auto per_node = [&](const double *p) {
+        std::vector<std::vector<double>> v;
+        if (p)
+            for (int i = 0; i < count; ++i) v.emplace_back(p + static_cast<int64_t>(i) * n, p + static_cast<int64_t>(i + 1) * n);
+        return v;
+    };
+    auto lower = per_node(node_lower), upper = per_node(node_upper), warm = per_node(warm_start);
+    if (lower.empty() && upper.empty()) lower.assign(static_cast<std::size_t>(count), std::vector<double>{});
+    std::vector<SolveResult> res;
+    try {
+        res = solver::solve_lp_nodes(model, lower, upper, warm, to_options(*options));
+    } catch (const std::exception &e) {
+        res.assign(static_cast<std::size_t>(count), SolveResult{});
+        for (auto &r : res) {
+            r.status = SolveStatus::NumericalFailure;
+            r.message = e.what();
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        const auto &r = res[static_cast<std::size_t>(i)];
+        fill_info(r, info[i]);
+        if (x_out && r.primal_solution.size() == static_cast<std::size_t>(n))
+            std::memcpy(x_out + static_cast<int64_t>(i) * n, r.primal_solution.data(), sizeof(double) * n);
+        if (y_out && r.constraint_upper_dual.size() == static_cast<std::size_t>(m))
+            for (int64_t k = 0; k < m; ++k)
+                y_out[static_cast<int64_t>(i) * m + k] = r.constraint_upper_dual[static_cast<std::size_t>(k)] -
+                                                         r.constraint_lower_dual[static_cast<std::size_t>(k)];
+    }
+    return 0;
+}
+
 int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
                 const int64_t *column_indices, const double *values, const double *objective,
                 double objective_offset, const double *row_lower, const double *row_upper,

@@ -40,6 +40,19 @@ SolveResult solve_qp(const Model &model, ExecutionBackend &backend,
 std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models,
                                         const SolverOptions &options = {});
 
+// Relaxations of one model that differ only in variable bounds (and warm
+// start): the branch-and-bound node loop (BranchAndBound.cpp:296-297 calls
+// RelaxationOracle::solve once per node, with a new backend each time,
+// Solver.cpp:37-51) as one batch on one device-resident, once-scaled
+// matrix.  Node i uses lower[i] / upper[i] (empty: the model's bounds) and
+// warm_start[i] (empty: cold start), as solve_lp would; the number of nodes
+// is the longer of lower and upper.  Integer variable types are relaxed.
+// Results in node order with solve_lp's semantics.
+std::vector<SolveResult> solve_lp_nodes(const Model &model, const std::vector<std::vector<Scalar>> &lower,
+                                        const std::vector<std::vector<Scalar>> &upper,
+                                        const std::vector<std::vector<Scalar>> &warm_start = {},
+                                        const SolverOptions &options = {});
+
 // Mean device time (ms, HIP events on the solver's stream) of the fused dual
 // and primal kernels and of a whole iteration, after `warm_iterations`
 // iterations, on the HIP backend.  For measurement only.

@@ -161,6 +161,8 @@ HIP_SYMBOLS = {
                                       C.POINTER(_vp)]),
     "vk_hip_batch_create_csr": (C.c_int, [C.c_int, _lp, _lp, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                           _dp, _dp, _dp, _dp, _lp, C.POINTER(PdhgParams), C.POINTER(_vp)]),
+    "vk_hip_batch_create_shared": (C.c_int, [C.c_int, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _lp, C.POINTER(PdhgParams), C.POINTER(_vp)]),
     "vk_hip_batch_run": (C.c_int, [_vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)]),
     "vk_hip_batch_read_state": (C.c_int, [_vp, C.c_int, C.POINTER(PdhgState)]),
     "vk_hip_batch_download": (C.c_int, [_vp, _dp, _dp]),
@@ -182,6 +184,9 @@ HOST_SYMBOLS = {
                               _dp, _dp, C.POINTER(_SolveInfo)]),
     "vk_lp_solve_batch": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
                                     C.POINTER(_SolverOptions), _dp, _dp, C.POINTER(_SolveInfo)]),
+    "vk_lp_solve_nodes": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
+                                    C.c_int, _dp, _dp, _dp, C.POINTER(_SolverOptions), _dp, _dp,
+                                    C.POINTER(_SolveInfo)]),
     "vk_lp_profile": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 C.POINTER(_SolverOptions), C.c_int64, C.c_int64, _dp]),
     "vk_lp_generate": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_vp)]),
@@ -494,6 +499,34 @@ def solve_batch(lps: list, options: Options | None = None) -> list:
     if rc != 0:
         raise ValueError(f"vk_lp_solve_batch rejected its arguments ({rc})")
     return [_result(infos[i], x[cs[i]:cs[i + 1]].copy(), y[rs[i]:rs[i + 1]].copy()) for i in range(count)]
+
+
+def solve_nodes(lp: LP, lower=None, upper=None, warm_start=None, options: Options | None = None) -> list:
+    """solver::solve_lp_nodes through vk_lp_solve_nodes: the LP with each
+    node's variable bounds (lower / upper: count x n arrays, None = the LP's
+    own) and warm starts (count x n or None), as one batch on one matrix."""
+    L = host_lib()
+    arrs = [a for a in (lower, upper, warm_start) if a is not None]
+    if not arrs:
+        raise ValueError("solve_nodes needs node bounds")
+    count = len(arrs[0])
+    node = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).reshape(count, lp.n))
+    lo, hi, ws = node(lower), node(upper), node(warm_start)
+    k = [np.ascontiguousarray(a, dtype=t) for a, t in (
+        (lp.row_offsets, np.int64), (lp.column_indices, np.int64), (lp.values, np.float64),
+        (lp.objective, np.float64), (lp.row_lower, np.float64), (lp.row_upper, np.float64),
+        (lp.var_lower, np.float64), (lp.var_upper, np.float64))]
+    x = np.zeros(count * lp.n)
+    y = np.zeros(count * lp.m)
+    infos = (_SolveInfo * count)()
+    copt = (options or Options())._c()
+    rc = L.vk_lp_solve_nodes(lp.m, lp.n, _ptr(k[0], _lp), _ptr(k[1], _lp), _ptr(k[2]), _ptr(k[3]),
+                             float(lp.objective_offset), _ptr(k[4]), _ptr(k[5]), _ptr(k[6]), _ptr(k[7]), count,
+                             _ptr(lo), _ptr(hi), _ptr(ws), C.byref(copt), _ptr(x), _ptr(y), infos)
+    if rc != 0:
+        raise ValueError(f"vk_lp_solve_nodes rejected its arguments ({rc})")
+    return [_result(infos[i], x[i * lp.n:(i + 1) * lp.n].copy(), y[i * lp.m:(i + 1) * lp.m].copy())
+            for i in range(count)]
 
 
 def solve(lp: LP, options: Options | None = None, quadratic=None) -> Result:
