@@ -22,9 +22,12 @@ algorithm restated in C) timed on a bounded sample of iterations of the same
 LP on one host core, extrapolated to the GPU's iteration count.
 
 Also measured (extra objects of the JSON line): the FP64 solve of the same
-LP with the adaptive policy ("adaptive_fp64"), the batched solve of 512
-block-angular LPs ("batch") and the unrolled-PDHG adjoint on a 100,000 x
-250,000 LP with its central-difference check ("adjoint").
+LP with the adaptive policy ("adaptive_fp64"), the wall time of each solve
+including setup and postsolve ("solve_wall_seconds"), the batched solve of
+512 block-angular LPs ("batch"), 256 branch-and-bound node relaxations of
+one LP on one device matrix against per-node solve_lp calls ("nodes"), the
+unrolled-PDHG adjoint on a 100,000 x 250,000 LP with its central-difference
+check ("adjoint"), and MPS files given with --mps ("mps").
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
 """
@@ -143,6 +146,45 @@ def measure_batch(V, np, args):
     return out
 
 
+def measure_nodes(V, np, args):
+    """Branch-and-bound node relaxations: `args.nodes` nodes of one
+    block-angular LP (T = 24 periods), each with 1-20 variables branched to
+    floor / ceil of x0 (the node loop of BranchAndBound.cpp:296-297), solved
+    as one batch on one device-resident matrix (solve_lp_nodes) against one
+    solve_lp call per node on a sample of the nodes, Halpern policy, FP64,
+    to 1e-6 relative KKT."""
+    rng = np.random.default_rng(WORKLOAD["seed"] + 1)
+    lp = V.generate_lp(4, 0, 0, 24, WORKLOAD["seed"] + 24)
+    lo = np.tile(lp.var_lower, (args.nodes, 1))
+    hi = np.tile(lp.var_upper, (args.nodes, 1))
+    for k in range(args.nodes):
+        for j in rng.choice(lp.n, size=rng.integers(1, 21), replace=False):
+            if rng.random() < 0.5:
+                hi[k, j] = np.floor(lp.x0[j])
+            else:
+                lo[k, j] = np.ceil(lp.x0[j])
+    o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=WORKLOAD["tolerance"],
+                  iteration_limit=args.batch_iterations, time_limit_seconds=args.time_limit)
+    t0 = time.time()
+    res = V.solve_nodes(lp, lo, hi, options=o)
+    wall = time.time() - t0
+    decided = sum(r.status in ("Optimal", "Infeasible") for r in res)
+    sample = min(args.nodes, 16)
+    t1 = time.time()
+    single = [V.solve_lp(V.LP(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values, lp.objective,
+                              lp.row_lower, lp.row_upper, lo[k], hi[k]), o) for k in range(sample)]
+    per_node = (time.time() - t1) / sample
+    agree = sum(a.status == b.status for a, b in zip(res[:sample], single))
+    out = {"nodes": args.nodes, "m": lp.m, "n": lp.n, "nnz": lp.nnz, "decided": decided,
+           "batch_device_seconds": res[0].device_seconds, "batch_wall_seconds": wall,
+           "single_wall_seconds_per_node": per_node, "single_sample": sample,
+           "speedup_vs_single_calls": per_node * args.nodes / wall, "status_agreement_on_sample": agree}
+    log(f"nodes: {args.nodes} nodes of a {lp.m} x {lp.n} LP: batch wall {wall:.2f}s (device "
+        f"{res[0].device_seconds:.2f}s), {decided} decided; single solve_lp {per_node * 1e3:.0f} ms/node "
+        f"-> {out['speedup_vs_single_calls']:.1f}x")
+    return out
+
+
 def measure_mps(V, path, options):
     """One real instance read from an MPS file (MpsReader::read), solved to
     the benchmark tolerance in FP64 and mixed precision (Halpern policy)."""
@@ -225,6 +267,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=512,
                     help="block-angular instances of the batched solve (0 = skip)")
     ap.add_argument("--batch-iterations", type=int, default=40_000)
+    ap.add_argument("--nodes", type=int, default=256,
+                    help="branch-and-bound node relaxations of one LP on one matrix (0 = skip)")
     ap.add_argument("--adjoint-iterations", type=int, default=1000,
                     help="unrolled iterations of the adjoint measurement (0 = skip)")
     ap.add_argument("--fd-directions", type=int, default=64)
@@ -336,6 +380,7 @@ def main():
     if args.adjoint_iterations > 0 and not args.skip_solve:
         adjoint = measure_adjoint(V, np, args)
     mps = [measure_mps(V, path, options) for path in args.mps] if args.mps else None
+    nodes = measure_nodes(V, np, args) if args.nodes > 0 and not args.skip_solve else None
 
     line = {
         "metric": "time to 1e-6 rel. KKT, mixed-FP32 vs FP64 PDHG, 1 MI355X; SpMV HBM GB/s",
@@ -376,6 +421,7 @@ def main():
         "batch": batch,
         "adjoint": adjoint,
         "mps": mps,
+        "nodes": nodes,
     }
     print(json.dumps(line), flush=True)
 
