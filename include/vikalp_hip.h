@@ -131,6 +131,13 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
                     double *row_scale, double *col_scale, double *c_out, double *l_out,
                     double *u_out, double *rl_out, double *ru_out);
 
+/* Variable bounds of `count` bound sets (l, u: count * cols, set-major)
+ * scaled as the last vk_hip_lp_scale on A scaled its l and u (the same
+ * operations in the same order, so bitwise the same result): for branch-
+ * and-bound nodes that share A and differ only in bounds. */
+int vk_hip_lp_scale_bounds(const vk_hip_csr *A, int count, const double *l, const double *u,
+                           double *l_out, double *u_out);
+
 /* Replaces continuous_common.hpp:207-225 scaled_operator_bound (pdhg.cpp:75):
  * sqrt(max_i sum_j |a_ij| * max_j sum_i |a_ij|) over A's current values. */
 int vk_hip_csr_operator_bound(const vk_hip_csr *A, double *bound);

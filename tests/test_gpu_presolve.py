@@ -284,3 +284,34 @@ def test_batch_from_handle_rejects_a_row_leaving_its_instance():
         assert b"leaves its instance" in dev.L.vk_hip_last_error()
     finally:
         dev.L.vk_hip_csr_destroy(h)
+
+
+@pytest.mark.parametrize("ruiz,pc", [(10, True), (0, False)])
+def test_bound_sets_scale_like_their_own_scaling(ruiz, pc):
+    # vk_hip_lp_scale_bounds applies the recorded column passes of the last
+    # scaling to many bound sets: bitwise what scaling with each set gives.
+    lp = odd_lp() if ruiz == 0 else V.generate_lp(1, 300, 900, 5, 4)
+    rng = np.random.default_rng(9)
+    count = 5
+    lo = np.tile(lp.var_lower, (count, 1))
+    hi = np.tile(lp.var_upper, (count, 1))
+    lo[:, ::3] = rng.uniform(-2, 1, size=lo[:, ::3].shape)
+    hi[1, ::4] = np.inf
+    dev = V.Device()
+    h = dev.csr(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values)
+    try:
+        exp = []
+        for k in range(count):
+            lk = V.LP(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values, lp.objective, lp.row_lower,
+                      lp.row_upper, lo[k], hi[k])
+            out = dev.lp_scale(h, lk, ruiz, pc)
+            exp.append((out["var_lower"], out["var_upper"]))
+        dev.lp_scale(h, lp, ruiz, pc)
+        lo_c, hi_c = np.ascontiguousarray(lo), np.ascontiguousarray(hi)
+        gl, gu = np.zeros_like(lo_c), np.zeros_like(hi_c)
+        N.check(dev.L.vk_hip_lp_scale_bounds(h, count, lo_c.ctypes.data_as(N._dp), hi_c.ctypes.data_as(N._dp),
+                                             gl.ctypes.data_as(N._dp), gu.ctypes.data_as(N._dp)), "scale_bounds")
+        for k in range(count):
+            assert np.array_equal(gl[k], exp[k][0]) and np.array_equal(gu[k], exp[k][1]), k
+    finally:
+        dev.L.vk_hip_csr_destroy(h)

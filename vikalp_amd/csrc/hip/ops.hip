@@ -487,6 +487,7 @@ int vk_hip_csr_destroy(vk_hip_csr *A) {
     free_csr(A->at);
     (void)hipFree(A->tperm);
     (void)hipFree(A->orig);
+    (void)hipFree(A->col_factors);
     delete A;
     return VK_OK;
 }

@@ -124,6 +124,27 @@ __global__ void k_scale_rows(int m, const double *__restrict__ f, double *__rest
     }
 }
 
+// Bounds of `count` variable-bound sets through the recorded column passes:
+// l / f_0 then l * f_p for each later pass, finite entries only (the same
+// operations k_scale_cols applies to one set).
+__global__ void k_scale_bound_sets(int n, int count, int passes, const double *__restrict__ f,
+                                   double *__restrict__ l, double *__restrict__ u) {
+    const int64_t total = static_cast<int64_t>(n) * count;
+    for (int64_t i = blockIdx.x * (int64_t)kPreBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kPreBlock) {
+        const int j = static_cast<int>(i % n);
+        double lo = l[i], hi = u[i];
+        lo = scale_finite(lo, f[j], false);
+        hi = scale_finite(hi, f[j], false);
+        for (int p = 1; p < passes; ++p) {
+            const double fp = f[static_cast<int64_t>(p) * n + j];
+            lo = scale_finite(lo, fp, true);
+            hi = scale_finite(hi, fp, true);
+        }
+        l[i] = lo;
+        u[i] = hi;
+    }
+}
+
 __global__ void k_take(int64_t n, const double *__restrict__ src, const int *__restrict__ perm,
                        double *__restrict__ dst) {
     for (int64_t i = blockIdx.x * (int64_t)kPreBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPreBlock)
@@ -223,10 +244,11 @@ int propagate_values(vk_hip_csr *A, hipStream_t s) {
 // Pock-Chambolle pass.
 template <RowStat S>
 int equilibration_pass(vk_hip_csr *A, double *rf, double *cf, double *r, double *sc, double *c,
-                       double *l, double *u, double *rl, double *ru, hipStream_t s) {
+                       double *l, double *u, double *rl, double *ru, double *record, hipStream_t s) {
     const int m = static_cast<int>(A->rows), n = static_cast<int>(A->cols);
     k_row_stat<S><<<pre_grid(m), kPreBlock, 0, s>>>(m, A->a.row_ptr, A->a.val, rf);
     k_row_stat<S><<<pre_grid(n), kPreBlock, 0, s>>>(n, A->at.row_ptr, A->at.val, cf);
+    if (n) VK_HIP_CHECK(hipMemcpyAsync(record, cf, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_scale_values<true><<<pre_grid(m), kPreBlock, 0, s>>>(m, A->a.row_ptr, A->a.col, A->a.val, rf, cf);
     k_scale_rows<false><<<pre_grid(m), kPreBlock, 0, s>>>(m, rf, r, rl, ru);
     k_scale_cols<false><<<pre_grid(n), kPreBlock, 0, s>>>(n, cf, sc, c, l, u);
@@ -282,20 +304,31 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
         VK_HIP_CHECK(hipMemcpyAsync(dru, ru, sizeof(double) * m, hipMemcpyHostToDevice, s));
     }
     const int mi = static_cast<int>(m), ni = static_cast<int>(n);
+    const int passes = 1 + ruiz_passes + (pock_chambolle ? 1 : 0);
+    (void)hipFree(A->col_factors);
+    A->col_factors = nullptr;
+    A->col_passes = 0;
+    VK_HIP_CHECK(hipMalloc(&A->col_factors, sizeof(double) * std::max<int64_t>(1, n * passes)));
     // continuous_common.hpp:133-205 on the original values (the transpose
     // holds them too: set_values and the restore above both propagate).
     if (int rc = propagate_values(A, s)) return rc;
     k_row_stat<RowStat::RefFactor><<<pre_grid(m), kPreBlock, 0, s>>>(mi, A->a.row_ptr, A->a.val, rf);
     k_row_stat<RowStat::RefFactor><<<pre_grid(n), kPreBlock, 0, s>>>(ni, A->at.row_ptr, A->at.val, cf);
+    if (n) VK_HIP_CHECK(hipMemcpyAsync(A->col_factors, cf, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_scale_values<false><<<pre_grid(m), kPreBlock, 0, s>>>(mi, A->a.row_ptr, A->a.col, A->a.val, rf, cf);
     k_scale_rows<true><<<pre_grid(m), kPreBlock, 0, s>>>(mi, rf, dr, drl, dru);
     k_scale_cols<true><<<pre_grid(n), kPreBlock, 0, s>>>(ni, cf, ds, dc, dl, du);
     VK_HIP_CHECK(hipGetLastError());
     if (int rc = propagate_values(A, s)) return rc;
     for (int p = 0; p < ruiz_passes; ++p)
-        if (int rc = equilibration_pass<RowStat::RuizFactor>(A, rf, cf, dr, ds, dc, dl, du, drl, dru, s)) return rc;
+        if (int rc = equilibration_pass<RowStat::RuizFactor>(A, rf, cf, dr, ds, dc, dl, du, drl, dru,
+                                                             A->col_factors + (1 + p) * n, s))
+            return rc;
     if (pock_chambolle)
-        if (int rc = equilibration_pass<RowStat::PcFactor>(A, rf, cf, dr, ds, dc, dl, du, drl, dru, s)) return rc;
+        if (int rc = equilibration_pass<RowStat::PcFactor>(A, rf, cf, dr, ds, dc, dl, du, drl, dru,
+                                                           A->col_factors + (1 + ruiz_passes) * n, s))
+            return rc;
+    A->col_passes = passes;
     if (n) {
         VK_HIP_CHECK(hipMemcpyAsync(col_scale, ds, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         VK_HIP_CHECK(hipMemcpyAsync(c_out, dc, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -307,6 +340,31 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
         VK_HIP_CHECK(hipMemcpyAsync(rl_out, drl, sizeof(double) * m, hipMemcpyDeviceToHost, s));
         VK_HIP_CHECK(hipMemcpyAsync(ru_out, dru, sizeof(double) * m, hipMemcpyDeviceToHost, s));
     }
+    VK_HIP_CHECK(hipStreamSynchronize(s));
+    return VK_OK;
+}
+
+int vk_hip_lp_scale_bounds(const vk_hip_csr *A, int count, const double *l, const double *u, double *l_out,
+                           double *u_out) {
+    VK_REQUIRE(A && count >= 0, "vk_hip_lp_scale_bounds: bad arguments");
+    VK_REQUIRE(A->scaled && A->col_passes > 0, "vk_hip_lp_scale_bounds: the matrix has not been scaled");
+    const int64_t n = A->cols, total = n * count;
+    if (total == 0) return VK_OK;
+    VK_REQUIRE(l && u && l_out && u_out, "vk_hip_lp_scale_bounds: null bounds");
+    hipStream_t s = default_stream();
+    Scratch sb;
+    double *dl = sb.get<double>(total), *du = sb.get<double>(total);
+    if (!dl || !du) {
+        set_error("vk_hip_lp_scale_bounds: scratch allocation failed");
+        return VK_ENOMEM;
+    }
+    VK_HIP_CHECK(hipMemcpyAsync(dl, l, sizeof(double) * total, hipMemcpyHostToDevice, s));
+    VK_HIP_CHECK(hipMemcpyAsync(du, u, sizeof(double) * total, hipMemcpyHostToDevice, s));
+    k_scale_bound_sets<<<pre_grid(total), kPreBlock, 0, s>>>(static_cast<int>(n), count, A->col_passes,
+                                                             A->col_factors, dl, du);
+    VK_HIP_CHECK(hipGetLastError());
+    VK_HIP_CHECK(hipMemcpyAsync(l_out, dl, sizeof(double) * total, hipMemcpyDeviceToHost, s));
+    VK_HIP_CHECK(hipMemcpyAsync(u_out, du, sizeof(double) * total, hipMemcpyDeviceToHost, s));
     VK_HIP_CHECK(hipStreamSynchronize(s));
     return VK_OK;
 }

@@ -56,6 +56,11 @@ struct vk_hip_csr {
     // holds the values last given to vk_hip_csr_set_values.
     double *orig = nullptr;
     bool scaled = false;
+    // The column factors of the last scaling, pass by pass (pass 0: the
+    // reference step's s, divided into l and u; passes 1..: equilibration
+    // factors, multiplied into them): col_passes rows of cols doubles.
+    double *col_factors = nullptr;
+    int col_passes = 0;
 };
 
 #define VK_HIP_CHECK(expr)                                                      \

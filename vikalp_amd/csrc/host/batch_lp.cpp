@@ -44,9 +44,10 @@ struct Stacked {
 };
 
 // Node batch: every instance is `model` with its own variable bounds (and
-// warm start).  The matrix is uploaded, transposed and scaled once; the
-// scaling of each node's bounds is the device scaling solve_lp applies to
-// them, so each node starts from exactly the data of its own solve_lp.
+// warm start).  The matrix is uploaded, transposed and scaled once; each
+// node's bounds are scaled by the same operations solve_lp applies to them
+// (vk_hip_lp_scale_bounds), so each node starts from exactly the data of its
+// own solve_lp.
 void stack_shared(const Model &model, const std::vector<Prepared> &prep, const std::vector<std::vector<Scalar>> &warm,
                   const SolverOptions &options, Stacked &st) {
     const int count = static_cast<int>(prep.size());
@@ -67,14 +68,22 @@ void stack_shared(const Model &model, const std::vector<Prepared> &prep, const s
              "vk_hip_csr_set_values");
     const bool equilibrated = options.pdhg_policy != PdhgPolicy::Reference;
     Scaled &sc = st.sc;
+    // Scaling depends on A only: once, with the first node's bounds; every
+    // node's bounds then go through the same recorded column factors.
+    const Scaled s = device_scale(st.A, model, prep[0].bounds, equilibrated);
+    std::vector<double> lo, hi;
+    for (int i = 0; i < count; ++i) {
+        lo.insert(lo.end(), prep[i].bounds.lower.begin(), prep[i].bounds.lower.end());
+        hi.insert(hi.end(), prep[i].bounds.upper.begin(), prep[i].bounds.upper.end());
+    }
+    sc.l.resize(lo.size());
+    sc.u.resize(hi.size());
+    check_vk(vk_hip_lp_scale_bounds(st.A, count, lo.data(), hi.data(), sc.l.data(), sc.u.data()),
+             "vk_hip_lp_scale_bounds");
     bool any_warm = false;
     for (int i = 0; i < count; ++i) {
-        // Scaling A depends on A only: every call leaves the same scaled
-        // matrix; the bounds come back scaled by this node's own sequence.
-        const Scaled s = device_scale(st.A, model, prep[i].bounds, equilibrated);
-        for (auto [dst, src] : {std::pair{&sc.c, &s.c}, std::pair{&sc.l, &s.l}, std::pair{&sc.u, &s.u},
-                                std::pair{&sc.s, &s.s}, std::pair{&sc.rl, &s.rl}, std::pair{&sc.ru, &s.ru},
-                                std::pair{&sc.r, &s.r}})
+        for (auto [dst, src] : {std::pair{&sc.c, &s.c}, std::pair{&sc.s, &s.s}, std::pair{&sc.rl, &s.rl},
+                                std::pair{&sc.ru, &s.ru}, std::pair{&sc.r, &s.r}})
             dst->insert(dst->end(), src->begin(), src->end());
         any_warm = any_warm || !warm[static_cast<std::size_t>(prep[i].index)].empty();
     }

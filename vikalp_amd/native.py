@@ -141,6 +141,7 @@ HIP_SYMBOLS = {
     "vk_hip_csr_norm2_estimate": (C.c_int, [_vp, C.c_int, _dp]),
     "vk_hip_lp_scale": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
                                   _dp, _dp]),
+    "vk_hip_lp_scale_bounds": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "vk_hip_csr_operator_bound": (C.c_int, [_vp, _dp]),
     "vk_hip_csr_operator_bounds": (C.c_int, [_vp, C.c_int, _lp, _lp, _dp]),
     "vk_hip_csr_postsolve_products": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
