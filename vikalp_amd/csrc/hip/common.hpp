@@ -221,9 +221,8 @@ __device__ __forceinline__ void store_partials(const Partials &p, double *partia
     if (threadIdx.x < kSlots) partials[threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
-// Reduce `nblocks` (<= 4 * blockDim.x) per-block partials in a fixed order;
-// result in out[0..7] (valid in every thread after the call).  All loads of
-// a thread are issued before any is used.  blockDim.x must be kFinalBlock.
+// Threads of the one-block kernels that reduce per-block partials (the
+// finalizer and the checks), and the most blocks an SpMV grid uses.
 constexpr int kFinalBlock = 1024;
 constexpr int kMaxGrid = 4 * kFinalBlock;
 
@@ -253,74 +252,91 @@ __device__ __forceinline__ double final_max(double v, double *scratch) {
     return r;
 }
 
-// Shared-memory doubles reduce_partials needs.
-constexpr int kFinalScratch = kSlots * kFinalBlock / kWave;
+// Shared-memory doubles reduce_partials and reduce_partials_pair need.
+constexpr int kFinalScratch = 2 * kSlots * kFinalBlock / kWave;
 
-// out (shared memory) receives the 8 reduced slots; a barrier follows, so
-// every thread may read out[] and reuse scratch afterwards.
-__device__ __forceinline__ void reduce_partials(const double *__restrict__ partials, int nblocks,
-                                                double *out, double *scratch) {
-    double v[kSlots][4];
+// Final reduction of per-block partial slots (partials[s * nblocks + b],
+// slot 0 a maximum of non-negative values, the others sums) by one
+// kFinalBlock-thread block.  Thread t reduces slot t % kSlots over blocks
+// t / kSlots, t / kSlots + kFinalBlock / kSlots, ... in order; the kSlots
+// lanes apart that share a slot in a wave are then combined by a 3-level
+// shuffle, and the waves' results in index order through LDS.  One value
+// per shuffle instead of all kSlots keeps the cross-lane traffic small: the
+// whole reduction costs about what its loads do.  The order is fixed, so
+// results repeat bitwise.  scratch holds kFinalScratch doubles; `sets`
+// reductions (1 or 2) run together, set i from (p[i], n[i]) into out[i].
+template <int Sets>
+__device__ __forceinline__ void reduce_partial_sets(const double *const (&p)[Sets], const int (&n)[Sets],
+                                                    double *const (&out)[Sets], double *scratch) {
+    constexpr int kPerSlot = kFinalBlock / kSlots;   // threads per slot
+    constexpr int kW = kFinalBlock / kWave;          // waves
+    const int s = threadIdx.x % kSlots, q = threadIdx.x / kSlots;
+    double acc[Sets];
+    int nmax = 0;
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int b = threadIdx.x + u * kFinalBlock;
-            v[s][u] = b < nblocks ? partials[s * nblocks + b] : 0.0;
-        }
-    double acc[kSlots];
-#pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        acc[s] = v[s][0];
-#pragma unroll
-        for (int u = 1; u < 4; ++u) acc[s] = (s == 0) ? dmax(acc[s], v[s][u]) : acc[s] + v[s][u];
+    for (int i = 0; i < Sets; ++i) {
+        acc[i] = 0.0;
+        nmax = max(nmax, n[i]);
     }
-    const double r = block_reduce_slots<kFinalBlock>(acc, scratch);
-    if (threadIdx.x < kSlots) out[threadIdx.x] = r;
+    // kChunk loads per set in flight at a time, combined in block order
+    // (out-of-range entries are 0, the identity of both max and sum here).
+    constexpr int kChunk = 8;
+#pragma unroll 1
+    for (int b0 = q; b0 < nmax; b0 += kChunk * kPerSlot) {
+        double v[Sets][kChunk];
+#pragma unroll
+        for (int i = 0; i < Sets; ++i)
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int b = b0 + u * kPerSlot;
+                v[i][u] = b < n[i] ? p[i][s * n[i] + b] : 0.0;
+            }
+#pragma unroll
+        for (int i = 0; i < Sets; ++i)
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) acc[i] = (s == 0) ? dmax(acc[i], v[i][u]) : acc[i] + v[i][u];
+    }
+#pragma unroll
+    for (int off = kWave / 2; off >= kSlots; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < Sets; ++i) {
+            const double o = __shfl_down(acc[i], off, kWave);
+            acc[i] = (s == 0) ? dmax(acc[i], o) : acc[i] + o;
+        }
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    if (lane < kSlots)
+#pragma unroll
+        for (int i = 0; i < Sets; ++i) scratch[(i * kSlots + lane) * kW + wave] = acc[i];
+    __syncthreads();
+    if (threadIdx.x < Sets * kSlots) {
+        const int i = threadIdx.x / kSlots, t = threadIdx.x % kSlots;
+        const double *w = scratch + (i * kSlots + t) * kW;
+        double r = w[0];
+        for (int k = 1; k < kW; ++k) r = (t == 0) ? dmax(r, w[k]) : r + w[k];
+#pragma unroll
+        for (int j = 0; j < Sets; ++j)
+            if (j == i) out[j][t] = r;
+    }
     __syncthreads();
 }
 
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ partials, int nblocks,
+                                                double *out, double *scratch) {
+    const double *p[1] = {partials};
+    const int n[1] = {nblocks};
+    double *o[1] = {out};
+    reduce_partial_sets<1>(p, n, o, scratch);
+}
+
 // reduce_partials for two partial sets at once (rows and columns of one
-// iteration): every load of both sets is issued before the reductions, and
-// each set is combined in the same order as by reduce_partials.
+// iteration), their loads interleaved.
 __device__ __forceinline__ void reduce_partials_pair(const double *__restrict__ pa, int na,
                                                      const double *__restrict__ pb, int nb, double *out_a,
                                                      double *out_b, double *scratch) {
-    double va[kSlots], vb[kSlots];
-    // Two of the four per-thread entries per round (32 loads in flight fit
-    // the 128 registers of a 1024-thread block); the second round only runs
-    // for more than 2 * kFinalBlock partials.
-    const int rounds = max(na, nb) > 2 * kFinalBlock ? 2 : 1;
-#pragma unroll 1
-    for (int r = 0; r < rounds; ++r) {
-        const int i0 = threadIdx.x + 2 * r * kFinalBlock, i1 = i0 + kFinalBlock;
-        double a0[kSlots], a1[kSlots], b0[kSlots], b1[kSlots];
-#pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-            a0[s] = i0 < na ? pa[s * na + i0] : 0.0;
-            a1[s] = i1 < na ? pa[s * na + i1] : 0.0;
-            b0[s] = i0 < nb ? pb[s * nb + i0] : 0.0;
-            b1[s] = i1 < nb ? pb[s * nb + i1] : 0.0;
-        }
-#pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-            if (r == 0) {
-                va[s] = (s == 0) ? dmax(a0[s], a1[s]) : a0[s] + a1[s];
-                vb[s] = (s == 0) ? dmax(b0[s], b1[s]) : b0[s] + b1[s];
-            } else {
-                va[s] = (s == 0) ? dmax(dmax(va[s], a0[s]), a1[s]) : (va[s] + a0[s]) + a1[s];
-                vb[s] = (s == 0) ? dmax(dmax(vb[s], b0[s]), b1[s]) : (vb[s] + b0[s]) + b1[s];
-            }
-        }
-    }
-    const double ra = block_reduce_slots<kFinalBlock>(va, scratch);
-    __syncthreads();   // threads 0..7 have read scratch before it is reused
-    const double rb = block_reduce_slots<kFinalBlock>(vb, scratch);
-    if (threadIdx.x < kSlots) {
-        out_a[threadIdx.x] = ra;
-        out_b[threadIdx.x] = rb;
-    }
-    __syncthreads();
+    const double *p[2] = {pa, pb};
+    const int n[2] = {na, nb};
+    double *o[2] = {out_a, out_b};
+    reduce_partial_sets<2>(p, n, o, scratch);
 }
 
 // CSR-adaptive SpMV, one tile per wave (grid-stride over waves).
