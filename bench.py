@@ -130,7 +130,8 @@ def measure_batch(V, np, args):
     for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
         o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision,
                       relative_kkt_tolerance=WORKLOAD["tolerance"], iteration_limit=args.batch_iterations,
-                      mixed_phase_tolerance=1e-4, time_limit_seconds=args.time_limit)
+                      mixed_phase_tolerance=1e-4, time_limit_seconds=args.time_limit,
+                      use_graphs=not getattr(args, "no_graphs", False))
         t0 = time.time()
         res = V.solve_batch(lps, o)
         wall = time.time() - t0
@@ -164,7 +165,8 @@ def measure_nodes(V, np, args):
             else:
                 lo[k, j] = np.ceil(lp.x0[j])
     o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=WORKLOAD["tolerance"],
-                  iteration_limit=args.batch_iterations, time_limit_seconds=args.time_limit)
+                  iteration_limit=args.batch_iterations, time_limit_seconds=args.time_limit,
+                  use_graphs=not args.no_graphs)
     t0 = time.time()
     res = V.solve_nodes(lp, lo, hi, options=o)
     wall = time.time() - t0
@@ -280,6 +282,9 @@ def main():
     ap.add_argument("--mps", nargs="*", default=[],
                     help="also solve these MPS files (FP64 and mixed, Halpern policy) and report them "
                          "under \"mps\"")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel directly instead of replaying HIP graphs (for kernel tracing: "
+                         "rocprofv3 --kernel-trace does not follow graph replays with a forked stream)")
     ap.add_argument("--counters", action="store_true",
                     help="driver for the rocprofv3 --pmc passes: three 256 MiB device copies "
                          "(counter calibration), then 2 + 6 profiled FP64 iterations; no JSON line")
@@ -305,7 +310,8 @@ def main():
         return V.Options(pdhg_policy=policy, pdhg_precision=precision,
                          relative_kkt_tolerance=WORKLOAD["tolerance"],
                          mixed_phase_tolerance=args.phase_tolerance,
-                         iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit)
+                         iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit,
+                         use_graphs=not args.no_graphs)
 
     # Kernel timing for the roofline (HIP events on the solver's stream).
     prof = V.profile_kernels(lp, options(V.PdhgPrecision.FP64), warm_iterations=30, iterations=60)
