@@ -27,7 +27,9 @@ including setup and postsolve ("solve_wall_seconds"), the batched solve of
 512 block-angular LPs ("batch"), 256 branch-and-bound node relaxations of
 one LP on one device matrix against per-node solve_lp calls ("nodes"), the
 unrolled-PDHG adjoint on a 100,000 x 250,000 LP with its central-difference
-check ("adjoint"), and MPS files given with --mps ("mps").
+check ("adjoint"), a fixed count of PDQP iterations on the benchmark LP plus
+a sparse convex Q against the same count without Q ("qp"), and MPS files
+given with --mps ("mps").
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
 """
@@ -187,6 +189,44 @@ def measure_nodes(V, np, args):
     return out
 
 
+def measure_qp(V, np, lp, args):
+    """PDQP (pdqp.cpp:20-331) on the benchmark LP plus a sparse convex Q:
+    every column gets one off-diagonal partner with a value in
+    (-0.25, 0.25), mirrored, and a diagonal of 0.2 + its row's absolute
+    off-diagonal sum + U(0, 1) (strictly diagonally dominant, so positive
+    definite).  PDQP runs the reference policy in FP64, which does not reach
+    the benchmark tolerance, so the measure is a fixed count of iterations,
+    timed beside the same count of the LP without Q (same policy, same
+    layout rules): the cost of Q x per iteration."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(WORKLOAD["seed"] + 7)
+    n = lp.n
+    i = np.arange(n)
+    j = rng.integers(0, n, n)
+    keep = i != j
+    i, j = i[keep], j[keep]
+    v = rng.uniform(-0.25, 0.25, i.size)
+    off = sp.coo_matrix((np.concatenate([v, v]), (np.concatenate([i, j]), np.concatenate([j, i]))),
+                        shape=(n, n)).tocsr()
+    off.sum_duplicates()
+    diag = 0.2 + np.asarray(abs(off).sum(axis=1)).ravel() + rng.uniform(0, 1, n)
+    q = (off + sp.diags(diag)).tocsr()
+    q.sort_indices()
+    quadratic = (q.indptr.astype(np.int64), q.indices.astype(np.int64), q.data.astype(np.float64))
+    o = V.Options(iteration_limit=args.qp_iterations, time_limit_seconds=args.time_limit,
+                  use_graphs=not args.no_graphs)
+    rq = V.solve_qp(lp, quadratic, o)
+    rl = V.solve_lp(lp, o)
+    out = {"n": n, "q_nnz": int(q.nnz), "iterations": rq.iterations, "status": rq.status,
+           "qp_device_seconds": rq.device_seconds, "lp_device_seconds": rl.device_seconds,
+           "qp_ms_per_iteration": rq.device_seconds / max(1, rq.iterations) * 1e3,
+           "lp_ms_per_iteration": rl.device_seconds / max(1, rl.iterations) * 1e3,
+           "qp_wall_seconds": rq.solve_seconds, "policy": "reference", "precision": "fp64"}
+    log(f"qp: {rq.iterations} iterations with Q ({q.nnz} nonzeros): {out['qp_ms_per_iteration']:.3f} ms/iteration "
+        f"vs LP {out['lp_ms_per_iteration']:.3f}; {rq.status}, wall {rq.solve_seconds:.2f}s")
+    return out
+
+
 def measure_mps(V, path, options):
     """One real instance read from an MPS file (MpsReader::read), solved to
     the benchmark tolerance in FP64 and mixed precision (Halpern policy)."""
@@ -274,6 +314,8 @@ def main():
     ap.add_argument("--adjoint-iterations", type=int, default=1000,
                     help="unrolled iterations of the adjoint measurement (0 = skip)")
     ap.add_argument("--fd-directions", type=int, default=64)
+    ap.add_argument("--qp-iterations", type=int, default=2000,
+                    help="fixed PDQP iteration count on the benchmark LP plus Q (0 skips)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adaptive", action="store_true",
                     help="skip the FP64 solve with the adaptive policy (comparison)")
@@ -387,6 +429,7 @@ def main():
         adjoint = measure_adjoint(V, np, args)
     mps = [measure_mps(V, path, options) for path in args.mps] if args.mps else None
     nodes = measure_nodes(V, np, args) if args.nodes > 0 and not args.skip_solve else None
+    qp = measure_qp(V, np, lp, args) if args.qp_iterations > 0 and not args.skip_solve else None
 
     line = {
         "metric": "time to 1e-6 rel. KKT, mixed-FP32 vs FP64 PDHG, 1 MI355X; SpMV HBM GB/s",
@@ -428,6 +471,7 @@ def main():
         "adjoint": adjoint,
         "mps": mps,
         "nodes": nodes,
+        "qp": qp,
     }
     print(json.dumps(line), flush=True)
 
