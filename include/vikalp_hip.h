@@ -254,7 +254,9 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
                        const double *y0, const vk_pdhg_params *params, vk_hip_pdhg **out);
 /* Runs up to `max_iterations` more iterations (or until the device stops),
  * polling the device state every iteration block.  Returns the VK_PDHG_*
- * status through `status` (not an error). */
+ * status through `status` (not an error).  run, read_state and download may
+ * be called from any host thread; calls on one solver must not overlap,
+ * while different solvers (each with its own streams) may run at once. */
 int vk_hip_pdhg_run(vk_hip_pdhg *solver, int64_t max_iterations, int *status);
 int vk_hip_pdhg_read_state(vk_hip_pdhg *solver, vk_pdhg_state *state);
 /* Scaled iterates (FP64 on the host) of the current solution. */

@@ -104,8 +104,9 @@ def test_mixed_batch_gives_every_instance_its_own_budget():
 
 @pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
 def test_tail_finishes_on_the_single_solver(precision):
-    # 40 instances: once at most 8 are still running, each remaining one
-    # continues on the single-LP solver from its batch iterate.  Every
+    # 40 instances: once at most 10 (a quarter) are still running, each
+    # remaining one continues on the single-LP solver from its batch iterate
+    # (several solvers driven at once from host threads).  Every
     # instance must still meet its certificate (or prove infeasibility),
     # the slow ones must keep their iteration budget, and an instance that
     # hits the limit reports exactly the limit.

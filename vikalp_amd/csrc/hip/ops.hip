@@ -32,6 +32,17 @@ void set_error(const std::string &message) { g_error = message; }
 
 hipStream_t default_stream() { return g_stream; }
 
+int bind_device() {
+    // The current device is per host thread: a thread other than the one
+    // that initialised the library starts on device 0.
+    thread_local int bound = -1;
+    if (bound != g_device) {
+        VK_HIP_CHECK(hipSetDevice(g_device));
+        bound = g_device;
+    }
+    return VK_OK;
+}
+
 int ensure_init() {
     std::lock_guard<std::mutex> lock(g_mutex);
     if (g_ready) return VK_OK;

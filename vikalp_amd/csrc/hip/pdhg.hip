@@ -1453,12 +1453,14 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l, co
 
 int vk_hip_pdhg_run(vk_hip_pdhg *s, int64_t max_iterations, int *status) {
     VK_REQUIRE(s && status && max_iterations >= 0, "vk_hip_pdhg_run: bad arguments");
+    if (int rc = bind_device()) return rc;
     return s->mixed ? run_solver<float>(s, max_iterations, status)
                     : run_solver<double>(s, max_iterations, status);
 }
 
 int vk_hip_pdhg_read_state(vk_hip_pdhg *s, vk_pdhg_state *out) {
     VK_REQUIRE(s && out, "vk_hip_pdhg_read_state: null argument");
+    if (int rc = bind_device()) return rc;
     VK_HIP_CHECK(hipMemcpyAsync(s->host_state, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
     VK_HIP_CHECK(hipStreamSynchronize(s->stream));
     const DevState &d = *s->host_state;
@@ -1484,6 +1486,7 @@ int vk_hip_pdhg_read_state(vk_hip_pdhg *s, vk_pdhg_state *out) {
 
 int vk_hip_pdhg_download(vk_hip_pdhg *s, double *x, double *y) {
     VK_REQUIRE(s, "vk_hip_pdhg_download: null solver");
+    if (int rc = bind_device()) return rc;
     return s->mixed ? download_solution<float>(s, x, y) : download_solution<double>(s, x, y);
 }
 

@@ -13,6 +13,9 @@
 namespace vk {
 
 void set_error(const std::string &message);
+// Makes the library's device current on the calling host thread (once per
+// thread); for entry points that may run on threads of the caller's pool.
+int bind_device();
 hipStream_t default_stream();
 int ensure_init();
 

@@ -12,11 +12,15 @@
 #include "vikalp_hip.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <memory>
 #include <span>
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace vikalp::solver {
@@ -197,31 +201,53 @@ struct DeviceBatch {
 // solver.  The batch kernel runs one workgroup per instance: once a few
 // instances are left, each runs on one CU while the rest of the GPU idles
 // (257 us per iteration for the 18,400-column block LP, against 26 us for
-// the same instance on the whole GPU).  Solving them one after another on
-// the single-LP path is then faster while fewer than about 10 are left.
-constexpr int kTailInstances = 8;
+// the same instance on the whole GPU).  The hand-off comes once at most
+// kTailInstances (and at most a quarter of the batch) are left; a batch of
+// fewer than kTailMinBatch instances stays in the batch kernel.  Measured
+// on 512 block LPs with 4 workers: 8 / 16 / 32 left gave 9.17 / 8.75 /
+// 8.63 s (FP64) and 9.30 / 8.56 / 8.82 s (mixed).
+constexpr int kTailInstances = 16;
+constexpr int kTailMinBatch = 32;
+// Host threads driving the tail's single-LP solvers at once.  Each solver
+// has its own streams, and a small LP's iteration leaves most of the GPU
+// free, so they overlap (the device runs four hardware queues).
+constexpr int kTailWorkers = 4;
 
-// Continues running instance i of the stacked batch on the single-LP device
-// solver: same scaled data (scaled again on the device from the model,
-// bitwise the stacked scaling), same step and primal weight, warm-started
-// from the batch's scaled iterates (a restart of the Halpern or adaptive
-// sequence at that point), up to the instance's remaining iterations.
-void finish_on_single_solver(const Model &model, const Prepared &pr, const Stacked &st, int i,
-                             const SolverOptions &options, int precision, double rel_tol, Index limit,
-                             Clock::time_point started, vk_pdhg_state &state, std::vector<double> &xs,
-                             std::vector<double> &ys, double &device_ms, bool &timed) {
-    vk_hip_csr *A = nullptr;
+// One tail instance on the single-LP device solver: same scaled data
+// (scaled again on the device from the model, bitwise the stacked scaling),
+// same step and primal weight, warm-started from the batch's scaled
+// iterates (a restart of the Halpern or adaptive sequence at that point),
+// up to the instance's remaining iterations.
+struct TailJob {
+    struct Matrix {
+        vk_hip_csr *a = nullptr;
+        ~Matrix() { vk_hip_csr_destroy(a); }
+    };
+    int i = 0;
+    Matrix A;   // declared before the solver, so destroyed after it
+    Scaled sc;
+    vk_pdhg_params p{};
+    DeviceSolver solver;
+    std::vector<Scalar> xo, yo;
+    vk_pdhg_state state{};
+    double ms = 0.0;
+    bool timed = false;
+};
+
+// Uploads and scales instance i and creates its solver (calling thread).
+std::unique_ptr<TailJob> prepare_tail(const Model &model, const Prepared &pr, const Stacked &st, int i,
+                                      const SolverOptions &options, int precision, double rel_tol, Index limit,
+                                      const vk_pdhg_state &state, const std::vector<double> &xs,
+                                      const std::vector<double> &ys) {
+    auto job = std::make_unique<TailJob>();
+    job->i = i;
     check_vk(vk_hip_csr_create(model.constraint_matrix.pattern.rows, model.constraint_matrix.pattern.columns,
                                model.constraint_matrix.pattern.row_offsets.data(),
-                               model.constraint_matrix.pattern.column_indices.data(), &A),
+                               model.constraint_matrix.pattern.column_indices.data(), &job->A.a),
              "vk_hip_csr_create");
-    struct Guard {
-        vk_hip_csr *a;
-        ~Guard() { vk_hip_csr_destroy(a); }
-    } guard{A};
-    check_vk(vk_hip_csr_set_values(A, model.constraint_matrix.values.data()), "vk_hip_csr_set_values");
-    const Scaled sc = device_scale(A, model, pr.bounds, options.pdhg_policy != PdhgPolicy::Reference);
-    vk_pdhg_params p{};
+    check_vk(vk_hip_csr_set_values(job->A.a, model.constraint_matrix.values.data()), "vk_hip_csr_set_values");
+    job->sc = device_scale(job->A.a, model, pr.bounds, options.pdhg_policy != PdhgPolicy::Reference);
+    vk_pdhg_params &p = job->p;
     p.policy = options.pdhg_policy == PdhgPolicy::Adaptive ? VK_PDHG_POLICY_ADAPTIVE : VK_PDHG_POLICY_HALPERN;
     p.precision = precision;
     p.iteration_limit = std::max<Index>(0, limit - state.iterations);
@@ -239,16 +265,59 @@ void finish_on_single_solver(const Model &model, const Prepared &pr, const Stack
     const auto c0 = static_cast<std::size_t>(st.col_starts[i]), c1 = static_cast<std::size_t>(st.col_starts[i + 1]);
     const auto r0 = static_cast<std::size_t>(st.row_starts[i]), r1 = static_cast<std::size_t>(st.row_starts[i + 1]);
     const std::vector<Scalar> x0(xs.begin() + c0, xs.begin() + c1), y0(ys.begin() + r0, ys.begin() + r1);
-    std::vector<Scalar> xo, yo;
-    vk_pdhg_state s1{};
-    double ms = 0.0;
-    run_device(A, sc, p, &x0, &y0, started, options, xo, yo, s1, ms, timed);
-    device_ms += ms;
-    std::copy(xo.begin(), xo.end(), xs.begin() + c0);
-    std::copy(yo.begin(), yo.end(), ys.begin() + r0);
-    const Index before = state.iterations;
-    state = s1;
-    state.iterations += before;
+    const Scaled &sc = job->sc;
+    check_vk(vk_hip_pdhg_create(job->A.a, sc.c.data(), sc.l.data(), sc.u.data(), sc.rl.data(), sc.ru.data(),
+                                sc.r.data(), sc.s.data(), x0.data(), y0.data(), &p, &job->solver.h),
+             "vk_hip_pdhg_create");
+    return job;
+}
+
+// Finishes the tail instances: solvers are prepared one after another on
+// this thread, then kTailWorkers threads drive them concurrently.  The
+// device time added is the wall time of the concurrent phase (the GPU is
+// busy throughout; per-solver times overlap).
+void finish_tail(const std::vector<const Model *> &models, const std::vector<Prepared> &prep, const Stacked &st,
+                 const std::vector<int> &tail, const SolverOptions &options, int precision, double rel_tol,
+                 const std::vector<int64_t> *limits, Index iteration_limit, Clock::time_point started,
+                 std::vector<vk_pdhg_state> &states, std::vector<double> &xs, std::vector<double> &ys,
+                 double &device_ms, bool &timed) {
+    std::vector<std::unique_ptr<TailJob>> jobs;
+    for (int i : tail)
+        jobs.push_back(prepare_tail(*models[prep[i].index], prep[i], st, i, options, precision, rel_tol,
+                                    limits ? (*limits)[i] : iteration_limit, states[i], xs, ys));
+    std::atomic<std::size_t> next{0};
+    const int workers = static_cast<int>(std::min<std::size_t>(kTailWorkers, jobs.size()));
+    std::vector<std::exception_ptr> errors(static_cast<std::size_t>(workers));
+    auto work = [&](int w) {
+        try {
+            for (std::size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+                TailJob &J = *jobs[j];
+                drive_device(J.solver, J.p, J.sc.c.size(), J.sc.r.size(), started, options, J.xo, J.yo, J.state,
+                             J.ms, J.timed);
+            }
+        } catch (...) {
+            errors[static_cast<std::size_t>(w)] = std::current_exception();
+        }
+    };
+    const auto t0 = Clock::now();
+    {
+        std::vector<std::thread> pool;
+        for (int w = 1; w < workers; ++w) pool.emplace_back(work, w);
+        work(0);
+        for (auto &t : pool) t.join();
+    }
+    device_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    for (const auto &e : errors)
+        if (e) std::rethrow_exception(e);
+    for (const auto &J : jobs) {
+        const int i = J->i;
+        std::copy(J->xo.begin(), J->xo.end(), xs.begin() + st.col_starts[i]);
+        std::copy(J->yo.begin(), J->yo.end(), ys.begin() + st.row_starts[i]);
+        const Index before = states[i].iterations;
+        states[i] = J->state;
+        states[i].iterations += before;
+        timed = timed || J->timed;
+    }
 }
 
 // Runs one device batch over the stacked instances (FP64 or mixed),
@@ -297,7 +366,8 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
     int running = count;
     // The reference policy stays in the batch: its instances must repeat
     // their own solves bitwise.  Small batches stay too (no idle GPU).
-    const bool tail = options.pdhg_policy != PdhgPolicy::Reference && count >= 4 * kTailInstances;
+    const bool tail = options.pdhg_policy != PdhgPolicy::Reference && count >= kTailMinBatch;
+    const int tail_at = std::min(kTailInstances, count / 4);
     for (int64_t launch = 0; launch < launches && running > 0; ++launch) {
         if (timed_out(started, options)) {
             timed = true;
@@ -307,7 +377,7 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
         double ms = 0.0;
         check_vk(vk_hip_batch_last_run_ms(batch.h, &ms), "vk_hip_batch_last_run_ms");
         device_ms += ms;
-        if (tail && running <= kTailInstances) break;
+        if (tail && running <= tail_at) break;
     }
     states.resize(static_cast<size_t>(count));
     for (int i = 0; i < count; ++i) check_vk(vk_hip_batch_read_state(batch.h, i, &states[i]), "vk_hip_batch_read_state");
@@ -315,11 +385,12 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
     ys.assign(static_cast<size_t>(m), 0.0);
     check_vk(vk_hip_batch_download(batch.h, xs.data(), ys.empty() ? nullptr : ys.data()), "vk_hip_batch_download");
     if (!tail || timed) return;
-    for (int i = 0; i < count && !timed; ++i)
-        if (states[i].status == VK_PDHG_RUNNING)
-            finish_on_single_solver(*models[prep[i].index], prep[i], st, i, options, precision, rel_tol,
-                                    limits ? (*limits)[i] : iteration_limit, started, states[i], xs, ys, device_ms,
-                                    timed);
+    std::vector<int> left;
+    for (int i = 0; i < count; ++i)
+        if (states[i].status == VK_PDHG_RUNNING) left.push_back(i);
+    if (!left.empty())
+        finish_tail(models, prep, st, left, options, precision, rel_tol, limits, iteration_limit, started, states, xs,
+                    ys, device_ms, timed);
 }
 
 // The phases of a stacked batch (FP32 then FP64 for mixed precision) and

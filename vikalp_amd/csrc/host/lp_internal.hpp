@@ -75,6 +75,12 @@ int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &para
                const std::vector<Scalar> *x0, const std::vector<Scalar> *y0, Clock::time_point started,
                const SolverOptions &options, std::vector<Scalar> &xs, std::vector<Scalar> &ys,
                vk_pdhg_state &state, double &device_ms, bool &timed);
+// run_device's iteration loop on a solver already created with `params`:
+// runs to a final status, the iteration limit or the time limit, then reads
+// the state and the scaled iterates (n columns, m rows).
+int drive_device(DeviceSolver &solver, const vk_pdhg_params &params, std::size_t n, std::size_t m,
+                 Clock::time_point started, const SolverOptions &options, std::vector<Scalar> &xs,
+                 std::vector<Scalar> &ys, vk_pdhg_state &state, double &device_ms, bool &timed);
 // continuous_common.hpp:242-275 (dense) up to n = 4096; beyond that a
 // diagonally dominant Q with a non-negative diagonal.  `message` says why not.
 bool positive_semidefinite(const CsrMatrix &q, std::string &message);

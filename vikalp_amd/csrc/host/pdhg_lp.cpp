@@ -482,6 +482,12 @@ int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &para
                                 sc.r.data(), sc.s.data(), x0 ? x0->data() : nullptr,
                                 y0 ? y0->data() : nullptr, &params, &solver.h),
              "vk_hip_pdhg_create");
+    return drive_device(solver, params, sc.c.size(), sc.r.size(), started, options, xs, ys, state, device_ms, timed);
+}
+
+int drive_device(DeviceSolver &solver, const vk_pdhg_params &params, std::size_t n, std::size_t m,
+                 Clock::time_point started, const SolverOptions &options, std::vector<Scalar> &xs,
+                 std::vector<Scalar> &ys, vk_pdhg_state &state, double &device_ms, bool &timed) {
     int status = VK_PDHG_RUNNING;
     timed = false;
     device_ms = 0.0;
@@ -503,8 +509,8 @@ int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &para
         done = state.iterations;
     }
     check_vk(vk_hip_pdhg_read_state(solver.h, &state), "vk_hip_pdhg_read_state");
-    xs.resize(sc.c.size());
-    ys.resize(sc.r.size());
+    xs.resize(n);
+    ys.resize(m);
     check_vk(vk_hip_pdhg_download(solver.h, xs.data(), ys.data()), "vk_hip_pdhg_download");
     return state.status;
 }
