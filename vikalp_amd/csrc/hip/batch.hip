@@ -33,6 +33,9 @@ namespace {
 
 constexpr int kBatchBlock = 512;                  // threads per instance
 constexpr int kBatchWaves = kBatchBlock / kWave;  // 8
+// Threads per instance once no more instances run than the device has CUs:
+// each then gets a CU of its own and twice the waves for its tiles.
+constexpr int kWideBlock = 1024;
 
 struct InstanceRange {
     int r0, r1, c0, c1;     // rows / columns of the instance
@@ -61,12 +64,13 @@ struct BatchArgs {
 };
 
 // Workgroup reduction of the 8 slots into out[] (shared memory, readable by
-// every thread after the call).  scratch holds kSlots * kBatchWaves doubles.
+// every thread after the call).  scratch holds kSlots * Block / kWave doubles.
+template <int Block = kBatchBlock>
 __device__ __forceinline__ void reduce_slots(const Partials &p, double *out, double *scratch) {
     double v[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) v[s] = p.v[s];
-    const double r = block_reduce_slots<kBatchBlock>(v, scratch);
+    const double r = block_reduce_slots<Block>(v, scratch);
     if (threadIdx.x < kSlots) out[threadIdx.x] = r;
     __syncthreads();
 }
@@ -100,11 +104,11 @@ struct Shifted {
 // One SpMV pass of an instance over tiles [t0, t1) of M, gathering v.  On a
 // shared matrix the gathered entries start at vfirst and the pass's rows
 // at rfirst; on the block-diagonal matrix both are global already.
-template <class V, class X, class Epi>
+template <int Waves = kBatchWaves, class V, class X, class Epi>
 __device__ __forceinline__ void instance_spmv(const CsrView<V> &M, const X *v, int vfirst, int rfirst, bool shared,
                                               Epi &epi, Partials &p, double *my_lds, int t0, int t1, int wave) {
     Shifted<Epi> se{epi, shared ? rfirst : 0};
-    tile_spmv_range(M, shared ? v + vfirst : v, se, p, my_lds, t0, t1, wave, kBatchWaves);
+    tile_spmv_range(M, shared ? v + vfirst : v, se, p, my_lds, t0, t1, wave, Waves);
 }
 
 // Per instance: project the start, compute A x0 and A' y0, estimate ||A||_2
@@ -220,16 +224,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_setup(BatchArgs<T> a) {
 // Up to `iterations` iterations of every instance still running.
 // Copies entries [first, last) of v into the LDS buffer and waits for the
 // whole workgroup.
-template <class T>
+template <int Block, class T>
 __device__ __forceinline__ void stage_vector(T *lv, const T *v, int first, int last) {
-    for (int i = first + static_cast<int>(threadIdx.x); i < last; i += kBatchBlock) lv[i - first] = v[i];
+    for (int i = first + static_cast<int>(threadIdx.x); i < last; i += Block) lv[i - first] = v[i];
     __syncthreads();
 }
 
 // One SpMV pass over tiles [t0, t1) gathering v (the instance's entries
 // start at vfirst, its pass rows at rfirst), from the LDS copy when the
 // instance fits.
-template <class T, class Epi>
+template <int Waves, class T, class Epi>
 __device__ __forceinline__ void batch_pass(const CsrView<T> &M, const T *v, bool in_lds, T *lv, int vfirst,
                                            int rfirst, bool shared, Epi &epi, Partials &p, double *my_lds, int t0,
                                            int t1, int wave) {
@@ -237,23 +241,27 @@ __device__ __forceinline__ void batch_pass(const CsrView<T> &M, const T *v, bool
         // The buffer holds entries [vfirst, ...): a shared matrix's indices
         // are already instance-local, a block-diagonal one's are global.
         Shifted<Epi> se{epi, shared ? rfirst : 0};
-        tile_spmv_range<T, T, Shifted<Epi>, true>(M, lv, se, p, my_lds, t0, t1, wave, kBatchWaves,
+        tile_spmv_range<T, T, Shifted<Epi>, true>(M, lv, se, p, my_lds, t0, t1, wave, Waves,
                                                   shared ? 0 : vfirst);
     } else {
-        instance_spmv(M, v, vfirst, rfirst, shared, epi, p, my_lds, t0, t1, wave);
+        instance_spmv<Waves>(M, v, vfirst, rfirst, shared, epi, p, my_lds, t0, t1, wave);
     }
 }
 
-// Two workgroups per CU (4 waves per SIMD, at most 128 VGPRs, a few dozen
-// spilled): the barriers and the one-thread finalize of one instance
-// overlap with the other's tiles.  Measured on the 512-instance benchmark:
-// 10.1 s against 10.7 s with one workgroup per CU and 180 VGPRs.
-template <class T>
-__global__ __launch_bounds__(kBatchBlock, 4) void k_batch_iterate(BatchArgs<T> a, int iterations) {
+// Block = kBatchBlock: two workgroups per CU (4 waves per SIMD, at most 128
+// VGPRs, a few dozen spilled): the barriers and the one-thread finalize of
+// one instance overlap with the other's tiles.  Measured on the
+// 512-instance benchmark: 10.1 s against 10.7 s with one workgroup per CU
+// and 180 VGPRs.  Block = kWideBlock once no more instances run than there
+// are CUs: each instance's chain of dependent tiles is then split over 16
+// waves instead of 8 (same 128-VGPR budget), 8.76 -> 7.65 s (FP64).
+template <class T, int Block>
+__global__ __launch_bounds__(Block, 4) void k_batch_iterate(BatchArgs<T> a, int iterations) {
+    constexpr int Waves = Block / kWave;
     const int id = blockIdx.x;
     if (id >= a.count) return;
-    __shared__ double lds[kBatchWaves][kTileNnz];
-    __shared__ double scratch[kSlots * kBatchWaves];
+    __shared__ double lds[Waves][kTileNnz];
+    __shared__ double scratch[kSlots * Waves];
     __shared__ double red_r[kSlots], red_c[kSlots];
     __shared__ DevState S;
     extern __shared__ double gathered[];   // a.lds_elems entries of T
@@ -278,34 +286,34 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_batch_iterate(BatchArgs<T> a
         // Every write of the previous iteration is complete here (each path
         // of the loop ends with a workgroup barrier).
         const T *xg = b.x[(k + 1) & 1];
-        if (in_lds) stage_vector(lv, xg, R.c0, R.c1);
+        if (in_lds) stage_vector<Block>(lv, xg, R.c0, R.c1);
         if (hal) {
             HalpernDualEpi<T> de{b, S.sigma, S.inv_sigma, halpern_weights(S.ks), check_iteration(S)};
-            batch_pass(a.A, xg, in_lds, lv, R.c0, R.r0, a.shared, de, p, my_lds, R.ta0, R.ta1, wave);
+            batch_pass<Waves>(a.A, xg, in_lds, lv, R.c0, R.r0, a.shared, de, p, my_lds, R.ta0, R.ta1, wave);
         } else {
             DualEpi<T> de{b, S.sigma, S.inv_sigma, ref && S.restart_flag, !ref};
-            batch_pass(a.A, xg, in_lds, lv, R.c0, R.r0, a.shared, de, p, my_lds, R.ta0, R.ta1, wave);
+            batch_pass<Waves>(a.A, xg, in_lds, lv, R.c0, R.r0, a.shared, de, p, my_lds, R.ta0, R.ta1, wave);
         }
         // Ends with a barrier: the dual pass no longer reads the buffer and
         // every y+ is written.
-        reduce_slots(p, red_r, scratch);
-        if (in_lds) stage_vector(lv, b.y, R.r0, R.r1);
+        reduce_slots<Block>(p, red_r, scratch);
+        if (in_lds) stage_vector<Block>(lv, b.y, R.r0, R.r1);
         p.zero();
         if (hal) {
             HalpernPrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse, check_iteration(S),
                                    halpern_weights(S.ks)};
-            batch_pass(a.AT, b.y, in_lds, lv, R.r0, R.c0, a.shared, pe, p, my_lds, R.tt0, R.tt1, wave);
+            batch_pass<Waves>(a.AT, b.y, in_lds, lv, R.r0, R.c0, a.shared, pe, p, my_lds, R.tt0, R.tt1, wave);
         } else {
             PrimalEpi<T> pe{b, b.x[(k + 1) & 1], b.x[k & 1], S.tau, fuse};
-            batch_pass(a.AT, b.y, in_lds, lv, R.r0, R.c0, a.shared, pe, p, my_lds, R.tt0, R.tt1, wave);
+            batch_pass<Waves>(a.AT, b.y, in_lds, lv, R.r0, R.c0, a.shared, pe, p, my_lds, R.tt0, R.tt1, wave);
         }
-        reduce_slots(p, red_c, scratch);
+        reduce_slots<Block>(p, red_c, scratch);
         if (tid == 0) finalize_step(S, red_r, red_c);
         __syncthreads();
         if (fuse) continue;
         if (ref) {
             if (S.status == VK_PDHG_RUNNING)
-                for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+                for (int j = R.c0 + tid; j < R.c1; j += Block)
                     pstep_element(b, S, j, b.x[S.k & 1], b.x[(S.k + 1) & 1]);
             __syncthreads();
             continue;
@@ -316,35 +324,35 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_batch_iterate(BatchArgs<T> a
         p.zero();
         {
             const T *g = hal ? b.aty_avg : b.aty;
-            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+            for (int j = R.c0 + tid; j < R.c1; j += Block)
                 farkas_column(static_cast<double>(g[j]), static_cast<double>(b.l[j]), static_cast<double>(b.u[j]), p);
         }
-        reduce_slots(p, red_c, scratch);
+        reduce_slots<Block>(p, red_c, scratch);
         if (tid == 0) infeasibility_rule(S, red_c);
         __syncthreads();
         if (S.status != VK_PDHG_RUNNING) continue;
         if (hal) {
-            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+            for (int j = R.c0 + tid; j < R.c1; j += Block)
                 halpern_apply_column(b, S, j, b.x[S.k & 1], b.x[(S.k + 1) & 1]);
-            for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) halpern_apply_row(b, S, i);
+            for (int i = R.r0 + tid; i < R.r1; i += Block) halpern_apply_row(b, S, i);
             __syncthreads();
             continue;
         }
         const double inv_ns = 1.0 / static_cast<double>(S.ns);
         EvalRowsEpi<T> er{b, inv_ns};
         p.zero();
-        instance_spmv(a.A, b.xsum, R.c0, R.r0, a.shared, er, p, my_lds, R.ta0, R.ta1, wave);
-        reduce_slots(p, red_r, scratch);
+        instance_spmv<Waves>(a.A, b.xsum, R.c0, R.r0, a.shared, er, p, my_lds, R.ta0, R.ta1, wave);
+        reduce_slots<Block>(p, red_r, scratch);
         EvalColsEpi<T> ec{b, inv_ns};
         p.zero();
-        instance_spmv(a.AT, b.ysum, R.r0, R.c0, a.shared, ec, p, my_lds, R.tt0, R.tt1, wave);
-        reduce_slots(p, red_c, scratch);
+        instance_spmv<Waves>(a.AT, b.ysum, R.r0, R.c0, a.shared, ec, p, my_lds, R.tt0, R.tt1, wave);
+        reduce_slots<Block>(p, red_c, scratch);
         if (tid == 0) check_step(S, red_r, red_c, false);
         __syncthreads();
         if (S.status == VK_PDHG_RUNNING || S.final_from_avg) {
-            for (int j = R.c0 + tid; j < R.c1; j += kBatchBlock)
+            for (int j = R.c0 + tid; j < R.c1; j += Block)
                 apply_column(b, S, j, b.x[S.k & 1], b.x[(S.k + 1) & 1]);
-            for (int i = R.r0 + tid; i < R.r1; i += kBatchBlock) apply_row(b, S, i);
+            for (int i = R.r0 + tid; i < R.r1; i += Block) apply_row(b, S, i);
         }
         __syncthreads();
     }
@@ -412,6 +420,9 @@ struct vk_hip_batch {
     BatchArgs<float> a32{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    int running = 0;     // instances running after the last launch
+    int cus = 0;         // compute units of the device
+    bool wide = false;   // the kWideBlock kernel fits (LDS) for this batch's precision
 
     template <class T>
     T *alloc(size_t count_) {
@@ -502,7 +513,7 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
     VK_HIP_CHECK(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
     hipFuncAttributes fa{};
-    VK_HIP_CHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_batch_iterate<T>)));
+    VK_HIP_CHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_batch_iterate<T, kBatchBlock>)));
     auto elems_for = [&](int limit) {
         const long long room = static_cast<long long>(limit) - static_cast<long long>(fa.sharedSizeBytes);
         return room > 0 ? static_cast<int>(std::min<long long>(max_dim, room / static_cast<long long>(sizeof(T)))) : 0;
@@ -510,12 +521,28 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
     // Half the LDS per workgroup, so two workgroups fit a CU.
     a.lds_elems = elems_for(std::max(per_block, optin) / 2);
     if (a.lds_elems > elems_for(per_block) &&
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_iterate<T>),
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_iterate<T, kBatchBlock>),
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(sizeof(T)) * a.lds_elems) != hipSuccess) {
         (void)hipGetLastError();   // no opt-in: stay within the default per-block limit
         a.lds_elems = elems_for(per_block);
     }
+    // The wide kernel (one instance per CU) takes the same dynamic buffer
+    // beside its larger static LDS, or is not used.
+    hipFuncAttributes fw{};
+    VK_HIP_CHECK(hipFuncGetAttributes(&fw, reinterpret_cast<const void *>(&k_batch_iterate<T, kWideBlock>)));
+    const long long wide_lds = static_cast<long long>(fw.sharedSizeBytes) +
+                               static_cast<long long>(sizeof(T)) * a.lds_elems;
+    h->wide = fw.maxThreadsPerBlock >= kWideBlock && wide_lds <= std::max(per_block, optin);
+    if (h->wide && wide_lds > per_block &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_iterate<T, kWideBlock>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(sizeof(T)) * a.lds_elems) != hipSuccess) {
+        (void)hipGetLastError();
+        h->wide = false;
+    }
+    VK_HIP_CHECK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    h->running = h->count;
     k_batch_setup<T><<<h->count, kBatchBlock, 0, h->stream>>>(a);
     VK_HIP_CHECK(hipGetLastError());
     VK_HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -892,16 +919,24 @@ int vk_hip_batch_run(vk_hip_batch *h, int64_t iterations_per_launch, int64_t max
     VK_HIP_CHECK(hipEventRecord(h->ev0, h->stream));
     int alive = 0;
     for (int64_t launch = 0; launch < max_launches; ++launch) {
-        if (h->mixed)
-            k_batch_iterate<float><<<h->count, kBatchBlock, sizeof(float) * h->a32.lds_elems, h->stream>>>(
-                h->a32, static_cast<int>(iterations_per_launch));
-        else
-            k_batch_iterate<double><<<h->count, kBatchBlock, sizeof(double) * h->a64.lds_elems, h->stream>>>(
-                h->a64, static_cast<int>(iterations_per_launch));
+        // Instances that have stopped leave at once, so with at most one
+        // running instance per CU the wide kernel gives each a CU.
+        const bool wide = h->wide && h->running <= h->cus;
+        const int its = static_cast<int>(iterations_per_launch);
+        if (h->mixed) {
+            const size_t lds = sizeof(float) * h->a32.lds_elems;
+            if (wide) k_batch_iterate<float, kWideBlock><<<h->count, kWideBlock, lds, h->stream>>>(h->a32, its);
+            else k_batch_iterate<float, kBatchBlock><<<h->count, kBatchBlock, lds, h->stream>>>(h->a32, its);
+        } else {
+            const size_t lds = sizeof(double) * h->a64.lds_elems;
+            if (wide) k_batch_iterate<double, kWideBlock><<<h->count, kWideBlock, lds, h->stream>>>(h->a64, its);
+            else k_batch_iterate<double, kBatchBlock><<<h->count, kBatchBlock, lds, h->stream>>>(h->a64, its);
+        }
         VK_HIP_CHECK(hipGetLastError());
         if (int rc = read_states(h)) return rc;
         alive = 0;
         for (const auto &S : h->states_host) alive += S.status == VK_PDHG_RUNNING ? 1 : 0;
+        h->running = alive;
         if (alive == 0) break;
     }
     VK_HIP_CHECK(hipEventRecord(h->ev1, h->stream));
