@@ -240,6 +240,24 @@ void k_blocks_x(const DevState *st, BlockedView<T> B, const T *x, double *part) 
     blocked_spmv(B, x, part, lds[threadIdx.x / kWave]);
 }
 
+// Sum over the nb column blocks of row i's partial sums, in block order
+// (from 0.0, as a plain loop adds them), with the loads issued four at a
+// time so that a row waits on one memory latency per four blocks, not one
+// per block.
+__device__ __forceinline__ double block_partials_sum(const double *__restrict__ part, long m, int i, int nb) {
+    double acc = 0.0;
+    int p = 0;
+    for (; p + 4 <= nb; p += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(part + static_cast<long>(p + u) * m + i);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += v[u];
+    }
+    for (; p < nb; ++p) acc += __builtin_nontemporal_load(part + static_cast<long>(p) * m + i);
+    return acc;
+}
+
 // Dual step over rows from the column-blocked partial sums (k_dual's
 // epilogue; the nb partials are added in block order).
 template <class T, bool H>
@@ -252,9 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *_
     acc_part.zero();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
         const auto pre = epi.prefetch(i);
-        double acc = 0.0;
-        for (int p = 0; p < nb; ++p) acc += __builtin_nontemporal_load(part + static_cast<long>(p) * b.m + i);
-        epi.row(i, acc, pre, acc_part);
+        epi.row(i, block_partials_sum(part, b.m, i, nb), pre, acc_part);
     }
     store_partials(acc_part, b.part_rows, scratch);
 }
@@ -270,9 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows_blocked(Bufs<T> b, const d
     acc_part.zero();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
         const auto pre = epi.prefetch(i);
-        double acc = 0.0;
-        for (int p = 0; p < nb; ++p) acc += __builtin_nontemporal_load(part + static_cast<long>(p) * b.m + i);
-        epi.row(i, acc, pre, acc_part);
+        epi.row(i, block_partials_sum(part, b.m, i, nb), pre, acc_part);
     }
     store_partials(acc_part, b.part_erows, scratch);
 }
