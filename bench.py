@@ -29,7 +29,7 @@ one LP on one device matrix against per-node solve_lp calls ("nodes"), the
 unrolled-PDHG adjoint on a 100,000 x 250,000 LP with its central-difference
 check ("adjoint"), a fixed count of PDQP iterations on the benchmark LP plus
 a sparse convex Q against the same count without Q ("qp"), and MPS files
-given with --mps ("mps").
+given with --mps or written by --mps-generated ("mps").
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
 """
@@ -227,6 +227,33 @@ def measure_qp(V, np, lp, args):
     return out
 
 
+def write_generated_mps(V, np, path):
+    """Writes the seeded 100,000 x 250,000 uniform LP (6 nonzeros per column,
+    equality rows, 0 <= x <= 10) as free MPS, every double by its shortest
+    repr so that reading it back gives the same bits: an input of realistic
+    size for the MPS path, since no netlib or MIPLIB file is available."""
+    lp = V.generate_lp(1, 100_000, 250_000, 6, WORKLOAD["seed"])
+    assert (lp.var_lower == 0).all() and (lp.row_lower == lp.row_upper).all()
+    rows = np.repeat(np.arange(lp.m), np.diff(lp.row_offsets))
+    order = np.lexsort((rows, lp.column_indices))
+    rows, cols, vals = rows[order], lp.column_indices[order], lp.values[order]
+    starts = np.searchsorted(cols, np.arange(lp.n + 1))
+    r = lambda v: repr(float(v))
+    with open(path, "w") as f:
+        f.write("NAME UNIF100K\nROWS\n N OBJ\n")
+        f.writelines(f" E R{i}\n" for i in range(lp.m))
+        f.write("COLUMNS\n")
+        for j in range(lp.n):
+            f.write(f" X{j} OBJ {r(lp.objective[j])}\n")
+            f.writelines(f" X{j} R{rows[k]} {r(vals[k])}\n" for k in range(starts[j], starts[j + 1]))
+        f.write("RHS\n")
+        f.writelines(f" B R{i} {r(lp.row_lower[i])}\n" for i in range(lp.m))
+        f.write("BOUNDS\n")
+        f.writelines(f" UP B X{j} {r(lp.var_upper[j])}\n" for j in range(lp.n))
+        f.write("ENDATA\n")
+    log(f"wrote {path}: {lp.m} rows, {lp.n} columns, {lp.nnz} nonzeros, {os.path.getsize(path) / 1e6:.0f} MB")
+
+
 def measure_mps(V, path, options):
     """One real instance read from an MPS file (MpsReader::read), solved to
     the benchmark tolerance in FP64 and mixed precision (Halpern policy)."""
@@ -324,6 +351,8 @@ def main():
     ap.add_argument("--mps", nargs="*", default=[],
                     help="also solve these MPS files (FP64 and mixed, Halpern policy) and report them "
                          "under \"mps\"")
+    ap.add_argument("--mps-generated", action="store_true",
+                    help="also write the seeded 100,000 x 250,000 uniform LP as MPS and measure it like --mps")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel tracing: "
                          "rocprofv3 --kernel-trace does not follow graph replays with a forked stream)")
@@ -427,6 +456,11 @@ def main():
     adjoint = None
     if args.adjoint_iterations > 0 and not args.skip_solve:
         adjoint = measure_adjoint(V, np, args)
+    if args.mps_generated:
+        import tempfile
+        path = os.path.join(tempfile.mkdtemp(), "unif100k.mps")
+        write_generated_mps(V, np, path)
+        args.mps = list(args.mps) + [path]
     mps = [measure_mps(V, path, options) for path in args.mps] if args.mps else None
     nodes = measure_nodes(V, np, args) if args.nodes > 0 and not args.skip_solve else None
     qp = measure_qp(V, np, lp, args) if args.qp_iterations > 0 and not args.skip_solve else None
