@@ -132,3 +132,19 @@ def test_tail_finishes_on_the_single_solver(precision):
             assert g.iterations == limit
     # the slow feasible instance (instance 345) runs past the batch's tail point
     assert got[36].status in ("Optimal", "IterationLimit") and got[36].iterations > 1000
+
+
+def test_tail_repeats_bitwise():
+    # The tail's solvers run on host threads in whatever order the threads
+    # take them; each instance's result must not depend on that order (nor
+    # on the batch kernel's 512- or 1024-thread launches), so two runs agree
+    # bit for bit.
+    lps = [V.generate_lp(4, 0, 0, 4 + (i % 5), 20261203 + 700 + i) for i in range(36)]
+    lps += [V.generate_lp(4, 0, 0, 7, 20261203 + 345), V.generate_lp(4, 0, 0, 30, 20261203 + 39)]
+    print(f"tail repeat batch of {len(lps)}", flush=True)
+    opts = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=1e-6, iteration_limit=8_000)
+    a = V.solve_batch(lps, opts)
+    b = V.solve_batch(lps, opts)
+    for ra, rb in zip(a, b):
+        assert (ra.status, ra.iterations) == (rb.status, rb.iterations)
+        assert np.array_equal(ra.x, rb.x) and np.array_equal(ra.y, rb.y)
