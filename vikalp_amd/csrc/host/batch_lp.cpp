@@ -322,8 +322,9 @@ void finish_tail(const std::vector<const Model *> &models, const std::vector<Pre
 
 // Runs one device batch over the stacked instances (FP64 or mixed),
 // optionally warm-started from scaled iterates; returns per-instance states
-// and scaled iterates.  Under the adaptive and Halpern policies the last
-// kTailInstances running instances finish on the single-LP solver.
+// and scaled iterates.  Under the adaptive and Halpern policies a batch of
+// at least kTailMinBatch instances finishes its last running instances (at
+// most kTailInstances, at most a quarter of the batch) on single-LP solvers.
 void run_batch(const std::vector<const Model *> &models, const std::vector<Prepared> &prep, const Stacked &st,
                const SolverOptions &options, int precision, double rel_tol, Index iteration_limit,
                const std::vector<double> *x0, const std::vector<double> *y0,
