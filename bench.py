@@ -456,12 +456,17 @@ def main():
     adjoint = None
     if args.adjoint_iterations > 0 and not args.skip_solve:
         adjoint = measure_adjoint(V, np, args)
+    generated_dir = None
     if args.mps_generated:
         import tempfile
-        path = os.path.join(tempfile.mkdtemp(), "unif100k.mps")
+        generated_dir = tempfile.mkdtemp()
+        path = os.path.join(generated_dir, "unif100k.mps")
         write_generated_mps(V, np, path)
         args.mps = list(args.mps) + [path]
     mps = [measure_mps(V, path, options) for path in args.mps] if args.mps else None
+    if generated_dir:
+        import shutil
+        shutil.rmtree(generated_dir, ignore_errors=True)
     nodes = measure_nodes(V, np, args) if args.nodes > 0 and not args.skip_solve else None
     qp = measure_qp(V, np, lp, args) if args.qp_iterations > 0 and not args.skip_solve else None
 
