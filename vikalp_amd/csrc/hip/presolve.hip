@@ -69,10 +69,14 @@ __global__ __launch_bounds__(kPreBlock) void k_scale_values(int rows, const int 
                                                             const double *__restrict__ rf,
                                                             const double *__restrict__ cf) {
 #pragma clang fp contract(off)
-    for (int r = blockIdx.x * kPreBlock + threadIdx.x; r < rows; r += gridDim.x * kPreBlock) {
+    // Elementwise, so a wave per row (coalesced) rounds every entry exactly
+    // as the host loop does.
+    constexpr int waves = kPreBlock / kWave;
+    const int lane = threadIdx.x % kWave;
+    for (int r = blockIdx.x * waves + threadIdx.x / kWave; r < rows; r += gridDim.x * waves) {
         const double f = rf[r];
         const int e = rp[r + 1];
-        for (int k = rp[r]; k < e; ++k) {
+        for (int k = rp[r] + lane; k < e; k += kWave) {
             const double g = f * cf[ci[k]];
             val[k] = Divide ? val[k] / g : val[k] * g;
         }
@@ -249,7 +253,7 @@ int equilibration_pass(vk_hip_csr *A, double *rf, double *cf, double *r, double 
     k_row_stat<S><<<pre_grid(m), kPreBlock, 0, s>>>(m, A->a.row_ptr, A->a.val, rf);
     k_row_stat<S><<<pre_grid(n), kPreBlock, 0, s>>>(n, A->at.row_ptr, A->at.val, cf);
     if (n) VK_HIP_CHECK(hipMemcpyAsync(record, cf, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    k_scale_values<true><<<pre_grid(m), kPreBlock, 0, s>>>(m, A->a.row_ptr, A->a.col, A->a.val, rf, cf);
+    k_scale_values<true><<<pre_grid(static_cast<int64_t>(m) * kWave), kPreBlock, 0, s>>>(m, A->a.row_ptr, A->a.col, A->a.val, rf, cf);
     k_scale_rows<false><<<pre_grid(m), kPreBlock, 0, s>>>(m, rf, r, rl, ru);
     k_scale_cols<false><<<pre_grid(n), kPreBlock, 0, s>>>(n, cf, sc, c, l, u);
     VK_HIP_CHECK(hipGetLastError());
@@ -315,7 +319,7 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
     k_row_stat<RowStat::RefFactor><<<pre_grid(m), kPreBlock, 0, s>>>(mi, A->a.row_ptr, A->a.val, rf);
     k_row_stat<RowStat::RefFactor><<<pre_grid(n), kPreBlock, 0, s>>>(ni, A->at.row_ptr, A->at.val, cf);
     if (n) VK_HIP_CHECK(hipMemcpyAsync(A->col_factors, cf, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    k_scale_values<false><<<pre_grid(m), kPreBlock, 0, s>>>(mi, A->a.row_ptr, A->a.col, A->a.val, rf, cf);
+    k_scale_values<false><<<pre_grid(static_cast<int64_t>(mi) * kWave), kPreBlock, 0, s>>>(mi, A->a.row_ptr, A->a.col, A->a.val, rf, cf);
     k_scale_rows<true><<<pre_grid(m), kPreBlock, 0, s>>>(mi, rf, dr, drl, dru);
     k_scale_cols<true><<<pre_grid(n), kPreBlock, 0, s>>>(ni, cf, ds, dc, dl, du);
     VK_HIP_CHECK(hipGetLastError());
