@@ -361,9 +361,8 @@ def main():
                          "(counter calibration), then 2 + 6 profiled FP64 iterations; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X and the PDHG path "
-              "is built for one device (multi-GPU sharding is described in DESIGN.md).",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU (--gpus {args.gpus} asked): the PDHG path runs on one "
+              "MI355X and has no multi-GPU sharding.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
