@@ -204,8 +204,9 @@ struct DeviceBatch {
 // the same instance on the whole GPU).  The hand-off comes once at most
 // kTailInstances (and at most a quarter of the batch) are left; a batch of
 // fewer than kTailMinBatch instances stays in the batch kernel.  Measured
-// on 512 block LPs with 4 workers: 8 / 16 / 32 left gave 9.17 / 8.75 /
-// 8.63 s (FP64) and 9.30 / 8.56 / 8.82 s (mixed).
+// on 512 block LPs with 4 workers and the 1024-thread batch kernel for the
+// last instances: 8 / 16 / 32 left gave 7.79 / 7.65 / 7.94 s (FP64) and
+// 7.99 / 7.78 / 8.33 s (mixed).
 constexpr int kTailInstances = 16;
 constexpr int kTailMinBatch = 32;
 // Host threads driving the tail's single-LP solvers at once.  Each solver
