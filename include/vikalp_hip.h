@@ -215,6 +215,16 @@ typedef struct {
                                       streamed.  0 = auto (256 MB less the
                                       gathered x and y less 4 MiB), -1 = none.
                                       Load hints only: results do not change */
+    int spmv_panels;               /* panel layout (LDS-staged gathers): 0 =
+                                      auto (once m or n > 2^18, if the matrix
+                                      fits the layout), 1 = never, 2 = always
+                                      (VK_EINVAL when it does not fit).  When
+                                      used it replaces col_blocks/split_passes */
+    int panel_groups;              /* column groups of A x (0 = auto; 1, 2, 4, 8):
+                                      partial sums per group, added in order */
+    int panel_groups_t;            /* the same for A' y                       */
+    int panel_threads;             /* lanes per panel workgroup: 0 = auto,
+                                      256 or 512 (results do not depend on it) */
 } vk_pdhg_params;
 
 typedef struct {
@@ -274,6 +284,10 @@ int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *ms);
 /* The A x layout the solver chose: column blocks (1 = plain row tiles) and
  * whether the SpMVs run split from their row / column passes (0 or 1). */
 int vk_hip_pdhg_layout(const vk_hip_pdhg *solver, int *col_blocks, int *split_passes);
+/* The panel layouts the solver uses (zeros when it does not): info[0..2] =
+ * groups, lanes per workgroup and most entries per lane and panel of A x,
+ * info[3..5] the same for A' y. */
+int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *solver, int *info);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
 
 /* Layout tuner (new; the reference hands SpMV to cuSPARSE,
@@ -286,6 +300,18 @@ int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
  * of A x, never the iteration's results beyond round-off. */
 int vk_hip_csr_tune_col_blocks(vk_hip_csr *A, int precision, int repeats, int *best,
                                double *ms);
+
+/* Panel SpMV (new; the layout behind vk_pdhg_params.spmv_panels): builds
+ * the panel layout of A (transpose = 0; rows x cols) or A' (transpose = 1)
+ * with `groups` column groups (1, 2, 4 or 8) and `threads` lanes per
+ * workgroup (256 or 512), in FP64 or, for VK_PREC_MIXED, FP32 values and x
+ * (x rounded to FP32 first), and writes the FP64 partial sums of every group
+ * to the host array `partial` (groups x rows, group-major): partial[g*rows+r]
+ * is the sequential sum, in column order, of row r's products over group
+ * g's columns.  `repeats` > 0 also times that many launches (mean ms into
+ * *ms, may be NULL).  VK_EINVAL when the operand does not fit the layout. */
+int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int precision, int groups, int threads,
+                          const double *x, double *partial, int repeats, double *ms);
 
 /* ---- batched PDHG (new: one persistent launch for many LPs) ------------
  *

@@ -45,6 +45,10 @@ typedef struct {
     int spmv_autotune;              /* time the layouts and use the fastest */
     int spmv_split_passes;          /* 0 auto, 1 fused, 2 split SpMV + passes */
     int spmv_cache_budget_mb;       /* MALL-resident matrix MiB: 0 auto, -1 none */
+    int spmv_panels;                /* panel layout: 0 auto, 1 never, 2 always */
+    int spmv_panel_groups;          /* A x column groups: 0 auto, 1, 2, 4, 8 */
+    int spmv_panel_groups_t;        /* A' y column groups: 0 auto, 1, 2, 4, 8 */
+    int spmv_panel_threads;         /* lanes per panel workgroup: 0 auto, 256, 512 */
 } vk_solver_options;
 
 /* Fills `o` with the defaults of vikalp::SolverOptions. */
@@ -130,8 +134,9 @@ int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets,
  * used (1 = plain row tiles), out[5] = 1 when the SpMVs ran split from
  * their passes, out[6..9] = mean ms of the dual SpMV (or fused dual kernel),
  * the dual row pass, the primal SpMV (or fused primal kernel) and the primal
- * column pass (vk_hip_pdhg_profile).  out holds 10 doubles.  Returns 0 on
- * success. */
+ * column pass (vk_hip_pdhg_profile), out[10..15] = the panel layout used
+ * (vk_hip_pdhg_panel_layout; zeros when none).  out holds 16 doubles.
+ * Returns 0 on success. */
 int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
                   const int64_t *column_indices, const double *values,
                   const double *objective, const double *row_lower,

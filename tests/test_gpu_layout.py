@@ -156,14 +156,14 @@ def test_split_passes_adaptive_certificate(precision):
 
 
 def test_automatic_layout_at_scale_matches_oracle():
-    # n = 300,000 > 2^18 and 2.4 MB of FP64 x: the automatic rule picks
-    # 4 column blocks and split passes
+    # n = 300,000 > 2^18 and 2.4 MB of FP64 x: with the panel layout off,
+    # the automatic rule picks 4 column blocks and split passes
     lp = V.generate_lp(1, 100_000, 300_000, 5, 7)
     print(f"auto layout m={lp.m} n={lp.n} nnz={lp.nnz}", flush=True)
-    got = V.solve_lp(lp, V.Options(iteration_limit=200))
+    got = V.solve_lp(lp, V.Options(iteration_limit=200, spmv_panels=1))
     ref = O.solve_lp(lp.as_oracle(), iteration_limit=200)
     close(got, ref)
-    prof = V.profile_kernels(lp, V.Options(), warm_iterations=2, iterations=2)
+    prof = V.profile_kernels(lp, V.Options(spmv_panels=1), warm_iterations=2, iterations=2)
     assert prof["col_blocks"] == 4 and prof["split_passes"] == 1
     # the row pass is its own kernel; the A' side is one kernel (k_primal_own)
     assert prof["dual_pass_ms"] > 0 and prof["primal_spmv_ms"] > 0 and prof["primal_pass_ms"] == 0
@@ -176,7 +176,7 @@ def test_cache_budget_changes_load_hints_only(policy):
     # results on the column-blocked split layout.
     lp = V.generate_lp(1, 100_000, 300_000, 5, 7)
     print(f"cache budget policy={policy.name}", flush=True)
-    opts = dict(pdhg_policy=policy, iteration_limit=300)
+    opts = dict(pdhg_policy=policy, iteration_limit=300, spmv_panels=1)
     if policy != V.PdhgPolicy.Reference:
         opts["relative_kkt_tolerance"] = 1e-12
     runs = [V.solve_lp(lp, V.Options(spmv_cache_budget_mb=b, **opts)) for b in (-1, 1, 8, 0)]

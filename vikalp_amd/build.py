@@ -32,7 +32,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
 CXX_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
              "-I" + INC, "-I" + os.path.join(HOST_SRC, "include"), "-I" + HOST_SRC]
 
-HIP_UNITS = ["ops.hip", "pdhg.hip", "batch.hip", "unrolled.hip", "presolve.hip", "layout.hip"]
+HIP_UNITS = ["ops.hip", "pdhg.hip", "batch.hip", "unrolled.hip", "presolve.hip", "layout.hip", "panel.hip"]
 HOST_UNITS = ["model.cpp", "hip_backend.cpp", "pdhg_lp.cpp", "batch_lp.cpp", "unrolled_lp.cpp", "qp.cpp", "solver.cpp", "capi.cpp", "mps_reader.cpp",
               "lpgen.cpp"]
 

@@ -41,6 +41,7 @@
 
 #include "runtime.hpp"
 #include "pdhg_core.hpp"
+#include "panel.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -273,6 +274,24 @@ __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *_
         epi.row(i, block_partials_sum(part, b.m, i, nb), pre, acc_part);
     }
     store_partials(acc_part, b.part_rows, scratch);
+}
+
+// Primal step over columns from the panel layout's partial sums of A' y+
+// (k_primal_own's column pass; the nb group partials are added in group
+// order).
+template <class T, bool H>
+__global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b, const double *__restrict__ part, int nb) {
+    const DevState s = *b.st;
+    if (s.status != VK_PDHG_RUNNING) return;
+    __shared__ double scratch[kSlotScratch];
+    auto epi = IterEpis<T, H, true>::primal(b, s, true);
+    Partials acc_part;
+    acc_part.zero();
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
+        const auto pre = epi.prefetch(j);
+        epi.row(j, block_partials_sum(part, b.n, j, nb), pre, acc_part);
+    }
+    store_partials(acc_part, b.part_cols, scratch);
 }
 
 // Average evaluation over rows from the column-blocked partial sums.
@@ -573,6 +592,13 @@ int resident_prefix(long long entries, long long bytes, long long &left) {
     return static_cast<int>(keep);
 }
 
+// Panel layout when the caller asks for automatic choice: once either
+// dimension exceeds 2^18 (where the gathers' miss rate bounds the tile
+// kernels), if the matrix fits the layout.
+bool auto_panels(long long m, long long n) { return m > 0 && std::max(m, n) > (1ll << 18); }
+
+bool valid_panel_groups(int g) { return g == 0 || g == 1 || g == 2 || g == 4 || g == 8; }
+
 }  // namespace
 }  // namespace vk
 
@@ -616,6 +642,12 @@ struct vk_hip_pdhg {
     double *blk_part = nullptr;
     // MALL residency (CsrView::resident) of the A x and A' y operands.
     int resident_a = 0, resident_at = 0;
+    // Panel layout (panels): A x into blk_part (pan_a.groups x m partial
+    // sums, then k_dual_rows) and A' y into tpart (pan_at.groups x n, then
+    // k_primal_cols).
+    bool panels = false;
+    vk::PanelLayout pan_a, pan_at;
+    double *tpart = nullptr;
 
     template <class T>
     T *alloc(size_t count) {
@@ -669,7 +701,10 @@ Bufs<float> &bufs<float>(vk_hip_pdhg *s) { return s->b32; }
 template <class T, bool H>
 void launch_dual_policy(vk_hip_pdhg *s, hipStream_t st, int part) {
     Bufs<T> &b = bufs<T>(s);
-    if (s->col_blocks > 1) {
+    if (s->panels) {
+        if (part & 1) (void)panel_spmv_launch(s->pan_a, b.st, 1, b.x[0], b.x[1], s->blk_part, st);
+        if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->pan_a.groups);
+    } else if (s->col_blocks > 1) {
         if (part & 1) k_blocks<T><<<s->grid_blk, kBlock, 0, st>>>(b, blocked<T>(s), s->blk_part, 0);
         if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
     } else if (s->split) {
@@ -698,7 +733,13 @@ void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
         }
     }
     const bool h = s->params.policy == VK_PDHG_POLICY_HALPERN;
-    if (s->split) {
+    if (s->panels) {
+        if (part & 1) (void)panel_spmv_launch(s->pan_at, b.st, 0, b.y, b.y, s->tpart, st);
+        if (part & 2) {
+            if (h) k_primal_cols<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, s->tpart, s->pan_at.groups);
+            else k_primal_cols<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, s->tpart, s->pan_at.groups);
+        }
+    } else if (s->split) {
         if (part & 1) {
             if (h) k_primal_own<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
             else k_primal_own<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
@@ -769,26 +810,30 @@ bool control_after(const vk_hip_pdhg *s, long long k) {
 // so the overlap needs an even block length (graph replays then keep the
 // parity).  The same kernels compute the same sums either way.
 template <class T>
-void enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
+int enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
     Bufs<T> &b = bufs<T>(s);
-    const bool overlap = s->side && s->col_blocks > 1 && block_length(s) % 2 == 0;
+    const bool overlap = s->side && (s->col_blocks > 1 || s->panels) && block_length(s) % 2 == 0;
     bool spmv_done = false;   // iteration k's A x+ was launched beside the previous finalizer
     for (long long k = k_begin; k < k_begin + count; ++k) {
         launch_dual<T>(s, s->stream, spmv_done ? 2 : 3);
         launch_primal<T>(s, s->stream);
         if (overlap && k + 1 < k_begin + count && !control_after(s, k)) {
-            (void)hipEventRecord(s->ev_fork, s->stream);
-            (void)hipStreamWaitEvent(s->side, s->ev_fork, 0);
+            VK_HIP_CHECK(hipEventRecord(s->ev_fork, s->stream));
+            VK_HIP_CHECK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
             k_finalize<T><<<1, kFinalBlock, 0, s->side>>>(b);
-            (void)hipEventRecord(s->ev_join, s->side);
-            k_blocks_x<T><<<s->grid_blk, kBlock, 0, s->stream>>>(b.st, blocked<T>(s), b.x[(k + 2) & 1], s->blk_part);
-            (void)hipStreamWaitEvent(s->stream, s->ev_join, 0);
+            VK_HIP_CHECK(hipEventRecord(s->ev_join, s->side));
+            if (s->panels)
+                (void)panel_spmv_launch(s->pan_a, b.st, 0, b.x[(k + 2) & 1], b.x[(k + 2) & 1], s->blk_part, s->stream);
+            else
+                k_blocks_x<T><<<s->grid_blk, kBlock, 0, s->stream>>>(b.st, blocked<T>(s), b.x[(k + 2) & 1], s->blk_part);
+            VK_HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
             spmv_done = true;
         } else {
             launch_control<T>(s, k, s->stream);
             spmv_done = false;
         }
     }
+    return VK_OK;
 }
 
 // Splits the MALL budget (vk_pdhg_params.cache_budget_mb) between the two
@@ -864,6 +909,36 @@ int build_blocked(vk_hip_pdhg *s) {
 }
 
 
+// Builds the panel layouts of A (A x) and A' (A' y) in the solver's storage
+// precision.  Lanes per workgroup: the caller's, or 256 (FP64; FP32 A' y)
+// and 512 (FP32 A x), the faster of the two measured on the 400k x 1M LP;
+// 512 falls back to 256 when a panel needs more than 16 entries per lane.
+// VK_EINVAL: the matrix does not fit the layout (nothing is kept).
+int build_panels(vk_hip_pdhg *s) {
+    const vk_hip_csr *A = s->A;
+    const int vb = s->mixed ? 4 : 8;
+    const int nnz = static_cast<int>(A->nnz);
+    const int ga = s->params.panel_groups > 0 ? s->params.panel_groups : panel_auto_groups(s->m, s->n, vb);
+    const int gt = s->params.panel_groups_t > 0 ? s->params.panel_groups_t : panel_auto_groups(s->n, s->m, vb);
+    const int ta = s->params.panel_threads > 0 ? s->params.panel_threads : (s->mixed ? 512 : 256);
+    const int tt = s->params.panel_threads > 0 ? s->params.panel_threads : 256;
+    auto build = [&](const int *rp, const int *ci, const void *val, int rows, int cols, int groups, int threads,
+                     PanelLayout &out) {
+        int rc = panel_build(rp, ci, val, vb, rows, cols, nnz, groups, threads, out, s->stream);
+        if (rc == VK_EINVAL && threads == 512) rc = panel_build(rp, ci, val, vb, rows, cols, nnz, groups, 256, out, s->stream);
+        return rc;
+    };
+    const void *va = s->mixed ? static_cast<const void *>(A->a32.val) : static_cast<const void *>(A->a.val);
+    const void *vat = s->mixed ? static_cast<const void *>(A->at32.val) : static_cast<const void *>(A->at.val);
+    int rc = build(A->a.row_ptr, A->a.col, va, s->m, s->n, ga, ta, s->pan_a);
+    if (!rc) rc = build(A->at.row_ptr, A->at.col, vat, s->n, s->m, gt, tt, s->pan_at);
+    if (rc) {
+        s->pan_a.release();
+        s->pan_at.release();
+    }
+    return rc;
+}
+
 template <class T>
 int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double *u,
                  const double *rl, const double *ru, const double *row_scale,
@@ -874,9 +949,11 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     std::memset(&b, 0, sizeof(b));
     b.m = m;
     b.n = n;
-    b.x[0] = s->alloc<T>(n);
-    b.x[1] = s->alloc<T>(n);
-    b.y = s->alloc<T>(m);
+    // Padded by 4 entries: the panel SpMV reads the gathered vectors in whole
+    // 16-byte vectors.
+    b.x[0] = s->alloc<T>(n + 4);
+    b.x[1] = s->alloc<T>(n + 4);
+    b.y = s->alloc<T>(m + 4);
     b.ax = s->alloc<T>(m);
     b.aty = s->alloc<T>(n);
     const bool halpern = s->params.policy == VK_PDHG_POLICY_HALPERN;
@@ -955,7 +1032,14 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     (void)hipFree(stage);
     if (rc) return rc;
 
-    if (s->col_blocks > 1) {
+    if (s->panels) {
+        s->blk_part = s->alloc<double>(static_cast<size_t>(s->pan_a.groups) * m);
+        s->tpart = s->alloc<double>(static_cast<size_t>(s->pan_at.groups) * n);
+        if (!s->blk_part || !s->tpart) {
+            set_error("vk_hip_pdhg_create: device allocation failed");
+            return VK_ENOMEM;
+        }
+    } else if (s->col_blocks > 1) {
         if (int brc = build_blocked<T>(s)) return brc;
     } else if (s->split) {
         s->blk_part = s->alloc<double>(m);
@@ -1039,8 +1123,12 @@ int build_graph(vk_hip_pdhg *s, long long k_begin) {
     if (s->graph) return VK_OK;
     hipGraph_t g = nullptr;
     VK_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_iterations<T>(s, k_begin, block_length(s));
+    const int rc = enqueue_iterations<T>(s, k_begin, block_length(s));
     VK_HIP_CHECK(hipStreamEndCapture(s->stream, &g));
+    if (rc) {
+        (void)hipGraphDestroy(g);
+        return rc;
+    }
     const hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     VK_HIP_CHECK(e);
@@ -1078,18 +1166,18 @@ int run_solver(vk_hip_pdhg *s, long long max_iterations, int *status) {
         long long chunk;
         if (s->host_k < first_boundary) {
             chunk = std::min(target, first_boundary) - s->host_k;
-            enqueue_iterations<T>(s, s->host_k, chunk);
+            if (int rc = enqueue_iterations<T>(s, s->host_k, chunk)) return rc;
         } else if (s->host_k + L <= target) {
             chunk = L;
             if (s->params.use_graphs) {
                 if (int rc = build_graph<T>(s, s->host_k)) return rc;
                 VK_HIP_CHECK(hipGraphLaunch(s->graph, st));
             } else {
-                enqueue_iterations<T>(s, s->host_k, L);
+                if (int rc = enqueue_iterations<T>(s, s->host_k, L)) return rc;
             }
         } else {
             chunk = target - s->host_k;
-            enqueue_iterations<T>(s, s->host_k, chunk);
+            if (int rc = enqueue_iterations<T>(s, s->host_k, chunk)) return rc;
         }
         VK_HIP_CHECK(hipGetLastError());
         s->host_k += chunk;
@@ -1197,7 +1285,7 @@ int profile_iterations(vk_hip_pdhg *s, int64_t iterations, double *out) {
     // Intervals with no kernel in them (fused dual kernel; the A' side is one
     // kernel in every layout) hold only event overhead.
     if (!s->split && s->col_blocks == 1) out[1] = 0.0;
-    out[3] = 0.0;
+    if (!s->panels) out[3] = 0.0;   // k_primal_own: A' y and its column pass are one kernel
     out[4] = total / static_cast<double>(iterations);
     return VK_OK;
 }
@@ -1320,6 +1408,8 @@ int vk_hip_pdhg_destroy(vk_hip_pdhg *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     for (void *p : s->allocations) (void)hipFree(p);
+    s->pan_a.release();
+    s->pan_at.release();
     if (s->st) (void)hipFree(s->st);
     if (s->host_state) (void)hipHostFree(s->host_state);
     if (s->ev_start) (void)hipEventDestroy(s->ev_start);
@@ -1371,6 +1461,11 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     VK_REQUIRE(params->split_passes >= 0 && params->split_passes <= 2,
                "vk_hip_pdhg_create: split_passes must be 0, 1 or 2");
     VK_REQUIRE(params->cache_budget_mb >= -1, "vk_hip_pdhg_create: cache_budget_mb must be >= -1");
+    VK_REQUIRE(params->spmv_panels >= 0 && params->spmv_panels <= 2, "vk_hip_pdhg_create: spmv_panels must be 0, 1 or 2");
+    VK_REQUIRE(valid_panel_groups(params->panel_groups) && valid_panel_groups(params->panel_groups_t),
+               "vk_hip_pdhg_create: panel groups must be 0 (automatic), 1, 2, 4 or 8");
+    VK_REQUIRE(params->panel_threads == 0 || params->panel_threads == 256 || params->panel_threads == 512,
+               "vk_hip_pdhg_create: panel_threads must be 0 (automatic), 256 or 512");
     if (int rc = ensure_init()) return rc;
     auto *s = new (std::nothrow) vk_hip_pdhg();
     if (!s) return VK_ENOMEM;
@@ -1446,13 +1541,28 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
         set_error("vk_hip_pdhg_create: state upload failed");
         return fail(VK_EDEVICE);
     }
-    int rc;
-    if (s->mixed) {
-        rc = csr_make_f32(const_cast<vk_hip_csr *>(A), s->stream);
-        if (!rc) rc = setup_solver<float>(s, c, l, u, rl, ru, row_scale, col_scale, x0, y0);
-    } else {
-        rc = setup_solver<double>(s, c, l, u, rl, ru, row_scale, col_scale, x0, y0);
+    int rc = s->mixed ? csr_make_f32(const_cast<vk_hip_csr *>(A), s->stream) : VK_OK;
+    if (rc) return fail(rc);
+    if (params->spmv_panels == 2 || (params->spmv_panels == 0 && auto_panels(s->m, s->n))) {
+        rc = build_panels(s);
+        if (rc == VK_OK) {
+            // Row pass over k_dual_rows' grid, column pass over k_primal_cols'.
+            s->panels = true;
+            s->col_blocks = 1;
+            s->split = true;
+            s->grid_rows = std::max(1, std::min((s->m + kBlock - 1) / kBlock, cap));
+            s->grid_cols = std::max(1, std::min((s->n + kBlock - 1) / kBlock, cap));
+        } else if (rc == VK_EINVAL && params->spmv_panels == 0) {
+            s->pan_a.release();
+            s->pan_at.release();
+        } else {
+            if (rc == VK_EINVAL) set_error("vk_hip_pdhg_create: the matrix does not fit the panel layout "
+                                           "(a row with too many entries in one panel, or too many panels)");
+            return fail(rc);
+        }
     }
+    rc = s->mixed ? setup_solver<float>(s, c, l, u, rl, ru, row_scale, col_scale, x0, y0)
+                  : setup_solver<double>(s, c, l, u, rl, ru, row_scale, col_scale, x0, y0);
     if (rc) return fail(rc);
     *out = s;
     return VK_OK;
@@ -1507,6 +1617,17 @@ int vk_hip_pdhg_download(vk_hip_pdhg *s, double *x, double *y) {
 int vk_hip_pdhg_profile(vk_hip_pdhg *s, int64_t iterations, double *ms) {
     VK_REQUIRE(s && ms && iterations > 0, "vk_hip_pdhg_profile: bad arguments");
     return s->mixed ? profile_iterations<float>(s, iterations, ms) : profile_iterations<double>(s, iterations, ms);
+}
+
+int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *s, int *info) {
+    VK_REQUIRE(s && info, "vk_hip_pdhg_panel_layout: null argument");
+    const PanelLayout *L[2] = {&s->pan_a, &s->pan_at};
+    for (int i = 0; i < 2; ++i) {
+        info[3 * i] = s->panels ? L[i]->groups : 0;
+        info[3 * i + 1] = s->panels ? L[i]->threads : 0;
+        info[3 * i + 2] = s->panels ? L[i]->steps : 0;
+    }
+    return VK_OK;
 }
 
 int vk_hip_pdhg_layout(const vk_hip_pdhg *s, int *col_blocks, int *split_passes) {

@@ -259,6 +259,10 @@ std::unique_ptr<TailJob> prepare_tail(const Model &model, const Prepared &pr, co
     p.split_passes = options.spmv_split_passes;
     p.cache_budget_mb = options.spmv_cache_budget_mb;
     p.col_blocks = options.spmv_col_blocks;
+    p.spmv_panels = options.spmv_panels;
+    p.panel_groups = options.spmv_panel_groups;
+    p.panel_groups_t = options.spmv_panel_groups_t;
+    p.panel_threads = options.spmv_panel_threads;
     p.norm_b = st.nb[i];
     p.norm_c = st.nc[i];
     p.step_size = std::sqrt(state.tau * state.sigma);   // tau = eta / omega, sigma = eta * omega

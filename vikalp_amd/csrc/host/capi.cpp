@@ -45,6 +45,10 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.spmv_autotune = o.spmv_autotune != 0;
     s.spmv_split_passes = o.spmv_split_passes;
     s.spmv_cache_budget_mb = o.spmv_cache_budget_mb;
+    s.spmv_panels = o.spmv_panels;
+    s.spmv_panel_groups = o.spmv_panel_groups;
+    s.spmv_panel_groups_t = o.spmv_panel_groups_t;
+    s.spmv_panel_threads = o.spmv_panel_threads;
     return s;
 }
 
@@ -110,6 +114,10 @@ void vk_solver_options_default(vk_solver_options *o) {
     o->spmv_autotune = d.spmv_autotune ? 1 : 0;
     o->spmv_split_passes = d.spmv_split_passes;
     o->spmv_cache_budget_mb = d.spmv_cache_budget_mb;
+    o->spmv_panels = d.spmv_panels;
+    o->spmv_panel_groups = d.spmv_panel_groups;
+    o->spmv_panel_groups_t = d.spmv_panel_groups_t;
+    o->spmv_panel_threads = d.spmv_panel_threads;
 }
 
 static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,
@@ -159,6 +167,7 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets, const int64_
         out[7] = p.dual_pass_ms;
         out[8] = p.primal_spmv_ms;
         out[9] = p.primal_pass_ms;
+        for (int i = 0; i < 6; ++i) out[10 + i] = p.panel[i];
     } catch (const std::exception &e) {
         return -2;
     }

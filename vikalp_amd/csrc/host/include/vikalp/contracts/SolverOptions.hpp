@@ -68,6 +68,15 @@ struct SolverOptions {
     // iterations (the rest is streamed): 0 = automatic, -1 = none.  Changes
     // load hints only, never results.
     int spmv_cache_budget_mb = 0;
+    // Panel layout (the gathered vector staged in LDS panel by panel):
+    // 0 = automatic (once m or n exceeds 2^18, if the matrix fits), 1 = never,
+    // 2 = always.  Groups of columns of A x / A' y (0 = automatic; 1, 2, 4, 8)
+    // set the partial-sum order; lanes per workgroup (0 = automatic, 256 or
+    // 512) never change results.
+    int spmv_panels = 0;
+    int spmv_panel_groups = 0;
+    int spmv_panel_groups_t = 0;
+    int spmv_panel_threads = 0;
 };
 
 }  // namespace vikalp

@@ -39,6 +39,8 @@ SolveResult status_result(SolveStatus status, const std::string &message) {
     return r;
 }
 
+bool valid_groups(int g) { return g == 0 || g == 1 || g == 2 || g == 4 || g == 8; }
+
 // continuous_common.hpp:50-56
 bool valid_options(const SolverOptions &o) {
     return o.iteration_limit >= 0 && std::isfinite(o.primal_tolerance) &&
@@ -48,6 +50,9 @@ bool valid_options(const SolverOptions &o) {
            o.restart_period > 0 && std::isfinite(o.mixed_phase_tolerance) &&
            o.mixed_phase_tolerance >= 0.0 && o.spmv_col_blocks >= 0 && o.spmv_col_blocks <= 16 &&
            o.spmv_split_passes >= 0 && o.spmv_split_passes <= 2 && o.spmv_cache_budget_mb >= -1 &&
+           o.spmv_panels >= 0 && o.spmv_panels <= 2 && valid_groups(o.spmv_panel_groups) &&
+           valid_groups(o.spmv_panel_groups_t) &&
+           (o.spmv_panel_threads == 0 || o.spmv_panel_threads == 256 || o.spmv_panel_threads == 512) &&
            (o.pdhg_policy == PdhgPolicy::Reference || o.pdhg_policy == PdhgPolicy::Adaptive ||
             o.pdhg_policy == PdhgPolicy::Halpern);
 }
@@ -534,6 +539,10 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     p.use_graphs = options.use_graphs ? 1 : 0;
     p.split_passes = options.spmv_split_passes;
     p.cache_budget_mb = options.spmv_cache_budget_mb;
+    p.spmv_panels = options.spmv_panels;
+    p.panel_groups = options.spmv_panel_groups;
+    p.panel_groups_t = options.spmv_panel_groups_t;
+    p.panel_threads = options.spmv_panel_threads;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     if (p.policy == VK_PDHG_POLICY_REFERENCE) {
@@ -759,6 +768,7 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
     double ms[5] = {};
     check_vk(vk_hip_pdhg_profile(solver.h, iterations, ms), "vk_hip_pdhg_profile");
     check_vk(vk_hip_pdhg_layout(solver.h, &out.col_blocks, &out.split_passes), "vk_hip_pdhg_layout");
+    check_vk(vk_hip_pdhg_panel_layout(solver.h, out.panel), "vk_hip_pdhg_panel_layout");
     out.dual_spmv_ms = ms[0];
     out.dual_pass_ms = ms[1];
     out.primal_spmv_ms = ms[2];

@@ -327,6 +327,10 @@ void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, co
     p.col_blocks = options.spmv_col_blocks;
     p.split_passes = options.spmv_split_passes;
     p.cache_budget_mb = options.spmv_cache_budget_mb;
+    p.spmv_panels = options.spmv_panels;
+    p.panel_groups = options.spmv_panel_groups;
+    p.panel_groups_t = options.spmv_panel_groups_t;
+    p.panel_threads = options.spmv_panel_threads;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     DeviceSolverQ solver;

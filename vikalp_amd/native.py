@@ -75,6 +75,8 @@ class _SolverOptions(C.Structure):
         ("relative_kkt_tolerance", C.c_double), ("mixed_phase_tolerance", C.c_double),
         ("restart_period", C.c_int64), ("spmv_col_blocks", C.c_int), ("spmv_autotune", C.c_int),
         ("spmv_split_passes", C.c_int), ("spmv_cache_budget_mb", C.c_int),
+        ("spmv_panels", C.c_int), ("spmv_panel_groups", C.c_int), ("spmv_panel_groups_t", C.c_int),
+        ("spmv_panel_threads", C.c_int),
     ]
 
 
@@ -100,7 +102,8 @@ class PdhgParams(C.Structure):
         ("step_size", C.c_double), ("primal_weight", C.c_double), ("restart_period", C.c_int64),
         ("norm_b", C.c_double), ("norm_c", C.c_double), ("use_graphs", C.c_int),
         ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
-        ("cache_budget_mb", C.c_int),
+        ("cache_budget_mb", C.c_int), ("spmv_panels", C.c_int), ("panel_groups", C.c_int),
+        ("panel_groups_t", C.c_int), ("panel_threads", C.c_int),
     ]
 
 
@@ -155,6 +158,8 @@ HIP_SYMBOLS = {
     "vk_hip_pdhg_last_run_ms": (C.c_int, [_vp, _dp]),
     "vk_hip_pdhg_profile": (C.c_int, [_vp, C.c_int64, _dp]),
     "vk_hip_pdhg_layout": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vk_hip_pdhg_panel_layout": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "vk_hip_csr_panel_spmv": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]),
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
     "vk_hip_csr_tune_col_blocks": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -370,6 +375,10 @@ class Options:
     spmv_autotune: bool = False
     spmv_split_passes: int = 0
     spmv_cache_budget_mb: int = 0
+    spmv_panels: int = 0
+    spmv_panel_groups: int = 0
+    spmv_panel_groups_t: int = 0
+    spmv_panel_threads: int = 0
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
@@ -452,7 +461,7 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     ci = np.ascontiguousarray(lp.column_indices, dtype=np.int64)
     f = [np.ascontiguousarray(a, dtype=np.float64) for a in
          (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
-    out = np.zeros(10)
+    out = np.zeros(16)
     copt = (options or Options())._c()
     rc = L.vk_lp_profile(lp.m, lp.n, _ptr(rp, _lp), _ptr(ci, _lp), *[_ptr(a) for a in f],
                          C.byref(copt), int(warm_iterations), int(iterations), _ptr(out))
@@ -460,7 +469,8 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
         raise RuntimeError(f"vk_lp_profile failed ({rc}): {hip_lib().vk_hip_last_error()}")
     return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3],
             "col_blocks": int(out[4]), "split_passes": int(out[5]), "dual_spmv_ms": out[6],
-            "dual_pass_ms": out[7], "primal_spmv_ms": out[8], "primal_pass_ms": out[9]}
+            "dual_pass_ms": out[7], "primal_spmv_ms": out[8], "primal_pass_ms": out[9],
+            "panels": [int(v) for v in out[10:16]]}
 
 
 def _result(info, x, y):
@@ -656,6 +666,20 @@ class Device:
         check(self.L.vk_hip_csr_tune_col_blocks(h, int(precision), int(repeats), C.byref(best), _ptr(ms)),
               "vk_hip_csr_tune_col_blocks")
         return best.value, dict(zip((1, 2, 4, 8, 16), ms.tolist()))
+
+    def panel_spmv(self, h, rows, cols, x, transpose=False, precision=PdhgPrecision.FP64, groups=1,
+                   threads=256, repeats=0):
+        """vk_hip_csr_panel_spmv: the (groups, rows) FP64 partial sums of A x
+        (A' x when transpose) on the panel layout, and the mean ms of
+        `repeats` timed launches (None when repeats == 0)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out_rows = cols if transpose else rows
+        part = np.zeros((groups, out_rows))
+        ms = C.c_double()
+        check(self.L.vk_hip_csr_panel_spmv(h, int(bool(transpose)), int(precision), int(groups), int(threads),
+                                           _ptr(x), _ptr(part), int(repeats), C.byref(ms)),
+              "vk_hip_csr_panel_spmv")
+        return part, (ms.value if repeats else None)
 
     def lp_scale(self, h, lp, ruiz_passes=0, pock_chambolle=False):
         """vk_hip_lp_scale on the matrix h (values set) with lp's vectors:
