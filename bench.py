@@ -75,12 +75,23 @@ def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True):
     without the sum load; k_primal_own (split A' side: SpMV, then the
     column pass over the workgroup's own rows, one launch) stores and
     reloads its sums, so it counts both.
+    Panel layout: k_panel_spmv counts the same SpMV bytes for A x and A' y
+    (the packed entries are s_v + 4 bytes each; there are no row offsets to
+    read, and the partial sums of the groups beyond the first are not
+    counted), then k_dual_rows and k_primal_cols the row and column passes.
     """
     spmv_a = (sv + 4) * nnz + 4 * (m + 1) + sv * n + 8 * m
     spmv_at = (sv + 4) * nnz + 4 * (n + 1) + sv * m + 8 * n
     pass_rows = 8 * m + (10 * sv * m if halpern else 8 * sv * m + 16 * m)
     pass_cols = 8 * n + (12 * sv * n if halpern else 8 * sv * n + 16 * n)
     rows = []
+    if prof.get("panels") and prof["panels"][0] > 0:
+        rows.append(("k_panel_spmv[A x]", prof["dual_spmv_ms"], spmv_a))
+        rows.append(("k_dual_rows", prof["dual_pass_ms"], pass_rows))
+        rows.append(("k_panel_spmv[A'y]", prof["primal_spmv_ms"], spmv_at))
+        rows.append(("k_primal_cols", prof["primal_pass_ms"], pass_cols))
+        return {f"{name}<{suffix}>": {"ms": ms, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9}
+                for name, ms, b in rows}
     if prof["col_blocks"] > 1 or prof["split_passes"]:
         rows.append(("k_blocks" if prof["col_blocks"] > 1 else "k_dual_spmv", prof["dual_spmv_ms"], spmv_a))
         rows.append(("k_dual_rows", prof["dual_pass_ms"], pass_rows))
@@ -390,7 +401,7 @@ def main():
     k32 = kernel_table(prof32, lp.m, lp.n, lp.nnz, 4, "float")
     for label, p, table in (("fp64", prof, k64), ("mixed", prof32, k32)):
         log(f"{label}: iteration {p['iteration_ms']:.4f} ms, col_blocks={p['col_blocks']} "
-            f"split={p['split_passes']}; " +
+            f"split={p['split_passes']} panels={p['panels']}; " +
             ", ".join(f"{k} {v['ms'] * 1e3:.1f} us {v['GBs']:.0f} GB/s" for k, v in table.items()))
 
     def run(precision, label, policy=V.PdhgPolicy.Halpern):
@@ -500,8 +511,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dominant),
                      "kernel": dominant, "bytes_per_launch": dom_bytes, "mean_ms": dom_ms},
-        "layout": {"fp64": {"col_blocks": prof["col_blocks"], "split_passes": prof["split_passes"]},
-                   "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"]}},
+        "layout": {"fp64": {"col_blocks": prof["col_blocks"], "split_passes": prof["split_passes"],
+                            "panels": prof["panels"]},
+                   "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"],
+                             "panels": prof32["panels"]}},
         "kernels": {"fp64": k64, "mixed": k32},
         "cpu_baseline": cpu,
         "adaptive_fp64": adaptive,

@@ -355,7 +355,9 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
         }
         // Row ranges balanced by entries + lambda per row, lambda raised (or
         // workgroups added) until no range exceeds the LDS accumulators.
-        int nw = std::max(base_nw, (rows + kPanelMaxRows * 15 / 16 - 1) / (kPanelMaxRows * 15 / 16));
+        // (More than 256 workgroups in all run in two rounds: only add them
+        // when the rows cannot be balanced under the accumulator limit.)
+        int nw = std::max(base_nw, (rows + kPanelMaxRows - 1) / kPanelMaxRows);
         std::vector<int> bounds;
         for (bool ok = false; !ok;) {
             for (long long lambda : {2ll, 8ll, 32ll, 1ll << 30}) {
