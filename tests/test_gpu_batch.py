@@ -6,7 +6,8 @@ objective to 1e-9 relative, x and y to 1e-8 -- the same bar as the
 single-LP tests, so batching changes execution only, not results.
 Adaptive policy (FP64 and mixed): each instance reaches relative KKT <= 1e-6,
 recomputed by the oracle on the returned point, and agrees with the
-single-instance HIP solve's objective to 1e-5.
+single-instance HIP solve's objective to twice the bound a 1e-6 certificate
+allows (test_gpu_pdhg.objective_bar).
 """
 import math
 
@@ -15,6 +16,8 @@ import pytest
 
 import vikalp_amd as V
 from oracle import oracle as O
+
+from test_gpu_pdhg import objective_bar
 
 pytestmark = pytest.mark.gpu
 
@@ -69,7 +72,9 @@ def test_adaptive_batch_reaches_relative_kkt(precision):
         single = V.solve_lp(lp, V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, relative_kkt_tolerance=tol,
                                           iteration_limit=100_000))
         assert single.status == "Optimal"
-        assert abs(g.objective_value - single.objective_value) <= 1e-5 * max(1.0, abs(single.objective_value))
+        # both meet the certificate: each is within objective_bar of the optimum
+        assert abs(g.objective_value - single.objective_value) / max(1.0, abs(single.objective_value)) <= \
+            2 * objective_bar(tol)
 
 
 def test_invalid_instance_does_not_stop_the_batch():

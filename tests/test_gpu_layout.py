@@ -19,7 +19,7 @@ import pytest
 import vikalp_amd as V
 from oracle import oracle as O
 
-from test_gpu_pdhg import close
+from test_gpu_pdhg import close, objective_bar
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -95,7 +95,7 @@ def test_blocked_adaptive_reaches_relative_kkt(precision):
     assert r.status == "Optimal", r
     rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), r.x, r.y)
     assert max(rp, rd, gp) <= tol * 1.0001, (rp, rd, gp)
-    assert abs(r.objective_value - g["objective"]) / max(1.0, abs(g["objective"])) <= 1e-5
+    assert abs(r.objective_value - g["objective"]) / max(1.0, abs(g["objective"])) <= objective_bar(tol)
 
 
 @pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
@@ -152,7 +152,8 @@ def test_split_passes_adaptive_certificate(precision):
         rp, rd, gp, _, _ = O.relative_kkt(lp.as_oracle(), r.x, r.y)
         assert max(rp, rd, gp) <= tol * 1.0001, (split, rp, rd, gp)
         res.append(r)
-    assert abs(res[0].objective_value - res[1].objective_value) <= 1e-5 * max(1.0, abs(g["objective"]))
+    for r in res:
+        assert abs(r.objective_value - g["objective"]) / max(1.0, abs(g["objective"])) <= objective_bar(tol)
 
 
 def test_automatic_layout_at_scale_matches_oracle():

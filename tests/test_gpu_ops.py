@@ -23,11 +23,11 @@ def random_csr(rng, m, n, density, long_rows=()):
 
 
 CASES = [
-    # m, n, density, long rows  (long rows exceed one 2048-nonzero tile)
+    # m, n, density, long rows  (long rows exceed one 512-nonzero tile)
     (200, 300, 0.05, ()),
     (5000, 3000, 0.002, (7, 4000)),
     (3, 9000, 0.01, (1,)),
-    (20000, 50, 0.3, ()),          # many short rows: tiles hit the 1024-row cap
+    (20000, 50, 0.3, ()),          # many short rows: tiles hit the 64-row cap
     (64, 64, 0.0, ()),             # all rows empty
 ]
 

@@ -103,13 +103,23 @@ def test_router_and_failure_paths():
 HIGHS = json.load(open(os.path.join(GOLDEN, "highs_optima.json")))
 
 
+def objective_bar(tol, obj=None):
+    """Relative objective error against HiGHS accepted at a relative-KKT
+    target `tol`.  The certificate bounds the gap term by tol, but its
+    primal term is relative to 1 + |bhat|_2, so the objective of a point at
+    1e-6 can be off by ~1e-5 (measured 8.5e-6 on the 1000 x 3000 LP); the
+    1e-6 objective bar is met from a target of 1e-8 on (and is checked
+    there)."""
+    return 1e-5 if tol >= 1e-6 else 1e-6
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-8])
 @pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
 @pytest.mark.parametrize("name", ["uniform_300x900_k5", "uniform_1000x3000_k5"])
-def test_adaptive_policy_reaches_relative_kkt(name, precision):
+def test_adaptive_policy_reaches_relative_kkt(name, precision, tol):
     g = HIGHS[name]
-    print(f"adaptive {name} precision={precision.name}", flush=True)
+    print(f"adaptive {name} precision={precision.name} tol={tol}", flush=True)
     lp = V.generate_lp(g["family"], g["m"], g["n"], g["k"], g["seed"])
-    tol = 1e-6
     r = V.solve_lp(lp, V.Options(pdhg_policy=V.PdhgPolicy.Adaptive, pdhg_precision=precision,
                                  relative_kkt_tolerance=tol, iteration_limit=200_000))
     assert r.status == "Optimal", r
@@ -119,6 +129,7 @@ def test_adaptive_policy_reaches_relative_kkt(name, precision):
     assert abs(max(rp, rd, gp) - r.kkt_relative) <= 1e-9
     # objective against the independent HiGHS optimum
     rel_err = abs(r.objective_value - g["objective"]) / max(1.0, abs(g["objective"]))
-    assert rel_err <= 1e-5, rel_err
+    print(f"objective relative error vs HiGHS: {rel_err:.3e}", flush=True)
+    assert rel_err <= objective_bar(tol), rel_err
     if precision == V.PdhgPrecision.Mixed:
         assert r.low_precision_iterations > 0

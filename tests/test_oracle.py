@@ -280,6 +280,53 @@ def test_relative_kkt_zero_at_vertex_enumerated_optimum():
     assert O.relative_kkt(lp, x, y + [-1.0, 0.0])[2] == 1.0        # y<0 on a <= row: D = -inf
 
 
+def test_relative_kkt_normalisation_at_a_non_optimal_point():
+    # Hand values at a point that is neither feasible nor stationary, so every
+    # denominator is exercised:
+    #   rows:  x1 + x2 = 1 (both bounds finite, equal), x1 - x2 <= 0.5
+    #          (one-sided), -2 <= x2 <= 1 (a range: bhat is the larger
+    #          magnitude, -2);  0 <= x <= (2, 3);  c = (1, 2)
+    #   x = (0.8, 0.5), y = (0.3, 0.2, 0)
+    # A x = (1.3, 0.3, 0.5): only row 0 is violated, by 0.3, so
+    #   rel_p = 0.3 / (1 + |(1, 0.5, -2)|_2) = 0.3 / (1 + sqrt(5.25)).
+    # g = c + A'y = (1.5, 2.1); x - P_X(x - g) = (0.8, 0.5), so
+    #   rel_d = sqrt(0.89) / (1 + sqrt(5)).
+    # c'x = 1.8; D(y) = sum_j min(g_j l_j, g_j u_j) - (0.3 * 1 + 0.2 * 0.5) = -0.4,
+    #   gap = |1.8 + 0.4| / (1 + 1.8 + 0.4) = 2.2 / 3.2.
+    D = np.array([[1.0, 1.0], [1.0, -1.0], [0.0, 1.0]])
+    rp, ci, v = csr_from_dense(D)
+    lp = O.LP(3, 2, rp, ci, v, np.array([1.0, 2.0]), np.array([1.0, -INF, -2.0]), np.array([1.0, 0.5, 1.0]),
+              np.zeros(2), np.array([2.0, 3.0]))
+    rel_p, rel_d, gap, pobj, dobj = O.relative_kkt(lp, np.array([0.8, 0.5]), np.array([0.3, 0.2, 0.0]))
+    assert rel_p == pytest.approx(0.3 / (1.0 + math.sqrt(5.25)), rel=1e-14)
+    assert rel_d == pytest.approx(math.sqrt(0.89) / (1.0 + math.sqrt(5.0)), rel=1e-14)
+    assert gap == pytest.approx(2.2 / 3.2, rel=1e-14)
+    assert pobj == pytest.approx(1.8, rel=1e-15) and dobj == pytest.approx(-0.4, rel=1e-14)
+
+
+def test_relative_kkt_dual_bound_at_the_highs_optimum():
+    # HiGHS (scipy.optimize.linprog, an LP solver independent of the
+    # reference and of this repository) solves the seeded 300 x 900 LP; at its
+    # primal point and its equality duals (reference sign: y = -marginal, so
+    # g = c + A'y are the reduced costs) the oracle's dual objective must be
+    # the LP optimum and every relative KKT term must vanish.
+    scipy_optimize = pytest.importorskip("scipy.optimize")
+    import scipy.sparse as sp
+    lp = V.generate_lp(1, 300, 900, 5, 11)
+    A = sp.csr_matrix((lp.values, lp.column_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    assert np.array_equal(lp.row_lower, lp.row_upper)
+    res = scipy_optimize.linprog(lp.objective, A_eq=A, b_eq=lp.row_lower,
+                                 bounds=np.column_stack([lp.var_lower, lp.var_upper]), method="highs",
+                                 options={"primal_feasibility_tolerance": 1e-10, "dual_feasibility_tolerance": 1e-10})
+    assert res.status == 0
+    y = -res.eqlin.marginals
+    rel_p, rel_d, gap, pobj, dobj = O.relative_kkt(lp.as_oracle(), res.x, y)
+    assert max(rel_p, rel_d, gap) <= 1e-9, (rel_p, rel_d, gap)
+    assert abs(pobj - res.fun) <= 1e-9 * abs(res.fun) and abs(dobj - res.fun) <= 1e-9 * abs(res.fun)
+    # a dual point with a perturbed equality dual is no longer optimal
+    assert O.relative_kkt(lp.as_oracle(), res.x, y + 1e-3 * (np.arange(lp.m) == 0))[2] > 1e-8
+
+
 # ------------------------------------------------------- unrolled PDHG adjoint
 
 def test_unrolled_forward_is_the_reference_trajectory_before_any_step_decision():
