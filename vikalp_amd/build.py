@@ -1,16 +1,22 @@
-"""Builds the native libraries in-tree (no JIT cache, nothing installed).
+"""Builds the native libraries in-tree (no JIT cache, nothing installed),
+through the project's CMakeLists.txt (Ninja):
 
   vikalp_amd/lib/libvikalp_hip.so   HIP kernels for gfx950 + their C ABI
                                     (include/vikalp_hip.h)
   vikalp_amd/lib/libvikalp.so       C++20 host: contracts, HipBackend,
                                     solve_lp, vikalp::solve, generators
                                     (include/vikalp_solve.h, vikalp_lpgen.h)
+  tests/cpp/build/hip_backend_test  the HipBackend consistency test (CTest
+                                    hip_consistency), with its oracle copy
+
+The CMake build tree is vikalp_amd/lib/cmake-build.  Every call configures
+and compiles everything again (clean=False lets Ninja skip what is up to
+date).
 
 Usage:  python -m vikalp_amd.build   (or vikalp_amd.build.build())
 """
 from __future__ import annotations
 
-import concurrent.futures as cf
 import os
 import subprocess
 import sys
@@ -18,56 +24,30 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "vikalp_amd")
 LIB = os.path.join(PKG, "lib")
-OBJ = os.path.join(PKG, "lib", "obj")
-INC = os.path.join(ROOT, "include")
-HIP_SRC = os.path.join(PKG, "csrc", "hip")
-HOST_SRC = os.path.join(PKG, "csrc", "host")
+BUILD_DIR = os.path.join(LIB, "cmake-build")
+TEST_DIR = os.path.join(ROOT, "tests", "cpp", "build")
 ARCH = os.environ.get("VIKALP_OFFLOAD_ARCH", "gfx950")
-
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-# Packed FP32 ops are disabled for every HIP translation unit.
-HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-I" + INC, "-I" + HIP_SRC]
-CXX_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
-             "-I" + INC, "-I" + os.path.join(HOST_SRC, "include"), "-I" + HOST_SRC]
-
-HIP_UNITS = ["ops.hip", "pdhg.hip", "batch.hip", "unrolled.hip", "presolve.hip", "layout.hip", "panel.hip"]
-HOST_UNITS = ["model.cpp", "hip_backend.cpp", "pdhg_lp.cpp", "batch_lp.cpp", "unrolled_lp.cpp", "qp.cpp", "solver.cpp", "capi.cpp", "mps_reader.cpp",
-              "lpgen.cpp"]
+HIP_COMPILER = os.environ.get("VIKALP_HIP_COMPILER", "/opt/rocm/llvm/bin/clang++")
 
 
 def _run(cmd):
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
-        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + proc.stdout + proc.stderr)
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + proc.stdout[-6000:] + proc.stderr[-6000:])
     return proc
 
 
-def _compile_hip(src):
-    out = os.path.join(OBJ, os.path.basename(src) + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-c", src, "-o", out])
-    return out
-
-
-def _compile_cxx(src):
-    out = os.path.join(OBJ, os.path.basename(src) + ".o")
-    _run(["g++", *CXX_FLAGS, "-c", src, "-o", out])
-    return out
-
-
-def build(verbose: bool = True) -> None:
-    os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as pool:
-        hip_objs = list(pool.map(_compile_hip, [os.path.join(HIP_SRC, u) for u in HIP_UNITS]))
-        host_objs = list(pool.map(_compile_cxx, [os.path.join(HOST_SRC, u) for u in HOST_UNITS]))
-    hip_so = os.path.join(LIB, "libvikalp_hip.so")
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", hip_so, *hip_objs])
-    host_so = os.path.join(LIB, "libvikalp.so")
-    _run(["g++", "-shared", "-fPIC", "-o", host_so, *host_objs, "-L" + LIB, "-lvikalp_hip",
-          "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"])
+def build(verbose: bool = True, clean: bool = True) -> None:
+    os.makedirs(LIB, exist_ok=True)
+    os.makedirs(TEST_DIR, exist_ok=True)
+    _run(["cmake", "-S", ROOT, "-B", BUILD_DIR, "-G", "Ninja", "-DCMAKE_BUILD_TYPE=Release",
+          f"-DCMAKE_HIP_COMPILER={HIP_COMPILER}", f"-DCMAKE_HIP_ARCHITECTURES={ARCH}",
+          f"-DVIKALP_LIB_DIR={LIB}", f"-DVIKALP_TEST_DIR={TEST_DIR}", "-DBUILD_TESTING=ON"])
+    jobs = str(max(1, min(16, os.cpu_count() or 4)))
+    _run(["cmake", "--build", BUILD_DIR, "-j", jobs] + (["--clean-first"] if clean else []))
     if verbose:
-        print("built", hip_so, host_so)
+        print("built", os.path.join(LIB, "libvikalp_hip.so"), os.path.join(LIB, "libvikalp.so"),
+              os.path.join(TEST_DIR, "hip_backend_test"))
 
 
 if __name__ == "__main__":
