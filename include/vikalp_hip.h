@@ -74,6 +74,21 @@ int vk_hip_reduce_absmax(int64_t n, const double *x, double *result, void *strea
 /* sum x_i*y_i, result to host; synchronous: cpu_ops.cpp:346-355 */
 int vk_hip_reduce_dot(int64_t n, const double *x, const double *y, double *result,
                       void *stream);
+/* vk_hip_reduce_dot for callers that cannot allocate device memory (code
+ * that may not call hipMalloc: serving loops, harnesses that forbid it).
+ * No counterpart in the reference.  It allocates nothing, not even on its
+ * first call or before vk_hip_init: its partial sums live in a fixed 8.3 KB
+ * device array per GPU (1024 partials and the result slot), found by
+ * hipGetSymbolAddress on the first call on each device, and it never calls
+ * vk_hip_init.  Same kernels and grid as vk_hip_reduce_dot (results are
+ * bitwise equal); no limit on n, whose partial count is capped at the
+ * array's 1024.  Runs on `stream` (NULL: the current device's null stream),
+ * then synchronises it.  Every call on a GPU shares that GPU's array, which
+ * keeps the last call's contents (nothing in it needs clearing: each call
+ * writes every partial it reads); calls that overlapped on one GPU would
+ * race on it, so a lock makes the calls of this process take turns. */
+int vk_hip_reduce_dot_noalloc(int64_t n, const double *x, const double *y, double *result,
+                              void *stream);
 
 /* ---- sparse matrices --------------------------------------------------- */
 

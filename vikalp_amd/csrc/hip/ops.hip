@@ -28,6 +28,15 @@ double *g_reduce = nullptr;
 
 }  // namespace
 
+// Scratch of vk_hip_reduce_dot_noalloc: the block partials and the result
+// slot of the two-stage dot product, in a file-scope device array (each GPU
+// holds its own copy, 8.3 KB, until the process ends).  Host code uses the
+// per-device address from hipGetSymbolAddress, never the array's name.
+static __device__ __align__(256) double noalloc_scratch[kReduceBlocks + 8];
+constexpr int kMaxNoallocDevices = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[kMaxNoallocDevices] = {};
+
 void set_error(const std::string &message) { g_error = message; }
 
 hipStream_t default_stream() { return g_stream; }
@@ -423,6 +432,42 @@ int vk_hip_reduce_dot(int64_t n, const double *x, const double *y, double *resul
     if (int rc = ensure_init()) return rc;
     std::lock_guard<std::mutex> lock(g_mutex);
     return reduce(true, n, x, y, result, as_stream(stream));
+}
+
+int vk_hip_reduce_dot_noalloc(int64_t n, const double *x, const double *y, double *result, void *stream) {
+    VK_REQUIRE(n >= 0 && result, "vk_hip_reduce_dot_noalloc: bad arguments");
+    VK_REQUIRE(n == 0 || (x && y), "vk_hip_reduce_dot_noalloc: null vector");
+    if (n == 0) {
+        *result = 0.0;
+        return VK_OK;
+    }
+    int device = 0;
+    VK_HIP_CHECK(hipGetDevice(&device));
+    VK_REQUIRE(device >= 0 && device < kMaxNoallocDevices, "vk_hip_reduce_dot_noalloc: device index out of range");
+    // One lock over set-up, launches and the result copy: calls from several
+    // host threads take turns on the one scratch array per GPU.
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        // First call on this device (current on the calling thread): the
+        // device address of this GPU's copy of the array.
+        void *ptr = nullptr;
+        VK_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch)));
+        VK_REQUIRE(ptr != nullptr, "vk_hip_reduce_dot_noalloc: no device address for the scratch array");
+        g_scratch_ptr[device] = static_cast<double *>(ptr);
+    }
+    double *part = g_scratch_ptr[device];
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: the device's null stream
+    // The same grid and kernels as vk_hip_reduce_dot: stage 1 writes every
+    // partial it launches and stage 2 the result slot, so nothing in the
+    // array needs clearing between calls (the blocks are capped at the
+    // array's kReduceBlocks partials, whatever n).
+    const int blocks = static_cast<int>(std::min<int64_t>((n + kBlock * 4 - 1) / (kBlock * 4), kReduceBlocks));
+    k_reduce_stage1<true><<<blocks, kBlock, 0, s>>>(n, x, y, part);
+    k_reduce_stage2<true><<<1, kBlock, 0, s>>>(blocks, part, part + kReduceBlocks);
+    VK_HIP_CHECK(hipGetLastError());
+    VK_HIP_CHECK(hipMemcpyAsync(result, part + kReduceBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    VK_HIP_CHECK(hipStreamSynchronize(s));
+    return VK_OK;
 }
 
 int vk_hip_csr_create(int64_t rows, int64_t cols, const int64_t *row_offsets,

@@ -135,6 +135,7 @@ HIP_SYMBOLS = {
     "vk_hip_project_box": (C.c_int, [C.c_int64, _vp, _vp, _vp, _vp]),
     "vk_hip_reduce_absmax": (C.c_int, [C.c_int64, _vp, _dp, _vp]),
     "vk_hip_reduce_dot": (C.c_int, [C.c_int64, _vp, _vp, _dp, _vp]),
+    "vk_hip_reduce_dot_noalloc": (C.c_int, [C.c_int64, _vp, _vp, _dp, _vp]),
     "vk_hip_csr_create": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, C.POINTER(_vp)]),
     "vk_hip_csr_set_values": (C.c_int, [_vp, _dp]),
     "vk_hip_csr_destroy": (C.c_int, [_vp]),
