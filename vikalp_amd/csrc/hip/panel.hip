@@ -22,7 +22,7 @@ void PanelLayout::release() {
     meta = nullptr;
     hdr = nullptr;
     ent = nullptr;
-    groups = threads = steps = nwg = rows = cols = value_bytes = 0;
+    groups = threads = steps = fast_steps = nwg = rows = cols = value_bytes = 0;
     ent_bytes = 0;
 }
 
@@ -141,6 +141,25 @@ __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevSta
         for (int j = 0; j < KM; ++j) {
             run = ((j > 0 && rows[j] == rows[j - 1]) ? run : old[j]) + prod[j];
             if (lane < step_count(wc, j)) acc[rows[j]] = run;
+        }
+        // Steps beyond the unrolled KM (a wave with a long row in this
+        // panel; rare by the choice of KM): loaded here, each entry added to
+        // its row's accumulator after the row's earlier entries, as above.
+        if (wc[1] > KM) {
+            const int *hp = pv.hdr + 8 * (hfirst + p * kW + wv);
+            int off = wc[0];
+#pragma unroll
+            for (int j = 0; j < KM; ++j) off += step_count(wc, j);
+            for (int j = KM; j < wc[1]; ++j) {
+                const int c = (hp[2 + j / 4] >> (8 * (j % 4))) & 0xff;
+                if (lane < c) {
+                    const E e = pv.ent[off + lane];
+                    const int r = min(static_cast<int>(e.x >> kPanelColBits), kPanelMaxRows - 1);
+                    const int col = min(static_cast<int>(e.x & ((1u << kPanelColBits) - 1u)), P - 1);
+                    acc[r] = acc[r] + static_cast<double>(entry_value(e)) * static_cast<double>(xb[col]);
+                }
+                off += c;
+            }
         }
         store_panel((p + 1) & 1);
         __syncthreads();
@@ -294,11 +313,10 @@ void launch_kernel(const PanelLayout &L, const DevState *state, int select, cons
 template <class V, int NT>
 void launch_steps(const PanelLayout &L, const DevState *state, int select, const void *x0, const void *x1,
                   double *out, hipStream_t st) {
-    if (L.steps <= 4) launch_kernel<V, NT, 4>(L, state, select, x0, x1, out, st);
-    else if (L.steps <= 8) launch_kernel<V, NT, 8>(L, state, select, x0, x1, out, st);
-    else if (L.steps <= 12) launch_kernel<V, NT, 12>(L, state, select, x0, x1, out, st);
-    else if (L.steps <= 16) launch_kernel<V, NT, 16>(L, state, select, x0, x1, out, st);
-    else if constexpr (NT == 256) launch_kernel<V, NT, 24>(L, state, select, x0, x1, out, st);
+    if (L.fast_steps <= 4) launch_kernel<V, NT, 4>(L, state, select, x0, x1, out, st);
+    else if (L.fast_steps <= 8) launch_kernel<V, NT, 8>(L, state, select, x0, x1, out, st);
+    else if (L.fast_steps <= 12) launch_kernel<V, NT, 12>(L, state, select, x0, x1, out, st);
+    else launch_kernel<V, NT, 16>(L, state, select, x0, x1, out, st);
 }
 
 }  // namespace
@@ -433,7 +451,28 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
         steps = std::max(steps, built[i].steps);
     }
     if (ne + 64 > static_cast<std::size_t>(INT32_MAX)) return VK_EINVAL;
-    if (threads == 512 && steps > 16) return VK_EINVAL;
+    // Unrolled steps: the fewest of 4, 8, 12, 16 that cover all but 3% of
+    // the (wave, panel) chunks; the rest run their extra steps unrolled no
+    // further (the kernel's overflow loop).
+    int fast_steps = 16;
+    {
+        std::vector<long long> hist(kPanelStepCap + 1, 0);
+        long long chunks = 0;
+        for (const auto &b : built)
+            for (const PanelHeader &h : b.hdr) {
+                ++hist[static_cast<std::size_t>(h.steps)];
+                ++chunks;
+            }
+        for (int km : {4, 8, 12, 16}) {
+            long long over = 0;
+            for (int k = km + 1; k <= kPanelStepCap; ++k) over += hist[static_cast<std::size_t>(k)];
+            if (over * 100 <= chunks * 3) {
+                fast_steps = km;
+                break;
+            }
+        }
+        fast_steps = std::min(fast_steps, std::max(4, steps));
+    }
     std::vector<PanelHeader> hdr;
     hdr.reserve(nh);
     std::vector<unsigned> idx;
@@ -478,6 +517,7 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
     L.groups = groups;
     L.threads = threads;
     L.steps = steps;
+    L.fast_steps = fast_steps;
     L.nwg = nwg;
     L.rows = rows;
     L.cols = cols;

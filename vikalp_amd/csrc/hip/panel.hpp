@@ -103,7 +103,9 @@ struct PanelView {
 struct PanelLayout {
     int groups = 0;          // G (0: not built)
     int threads = 0;         // lanes per workgroup (256 or 512)
-    int steps = 0;           // most entries any lane has in one panel (kernel unroll)
+    int steps = 0;           // most entries any lane has in one panel
+    int fast_steps = 0;      // steps the kernel unrolls (4, 8, 12 or 16); a chunk with
+                             // more runs the rest in a loop
     int nwg = 0;             // workgroups
     int rows = 0, cols = 0;  // output rows, gathered entries
     int value_bytes = 0;
