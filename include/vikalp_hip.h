@@ -240,6 +240,10 @@ typedef struct {
     int panel_groups_t;            /* the same for A' y                       */
     int panel_threads;             /* lanes per panel workgroup: 0 = auto,
                                       256 or 512 (results do not depend on it) */
+    int panel_tune;                /* 1: time both lane counts and the unroll
+                                      depths around the automatic one on the
+                                      matrix at setup and keep the fastest
+                                      (results do not depend on them)         */
 } vk_pdhg_params;
 
 typedef struct {
@@ -299,9 +303,10 @@ int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *ms);
 /* The A x layout the solver chose: column blocks (1 = plain row tiles) and
  * whether the SpMVs run split from their row / column passes (0 or 1). */
 int vk_hip_pdhg_layout(const vk_hip_pdhg *solver, int *col_blocks, int *split_passes);
-/* The panel layouts the solver uses (zeros when it does not): info[0..2] =
- * groups, lanes per workgroup and most entries per lane and panel of A x,
- * info[3..5] the same for A' y. */
+/* The panel layouts the solver uses (zeros when it does not): info[0..3] =
+ * groups, lanes per workgroup, most entries per lane and panel, and the
+ * kernel's unrolled entries per lane and panel of A x; info[4..7] the same
+ * for A' y. */
 int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *solver, int *info);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
 

@@ -134,8 +134,8 @@ int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets,
  * used (1 = plain row tiles), out[5] = 1 when the SpMVs ran split from
  * their passes, out[6..9] = mean ms of the dual SpMV (or fused dual kernel),
  * the dual row pass, the primal SpMV (or fused primal kernel) and the primal
- * column pass (vk_hip_pdhg_profile), out[10..15] = the panel layout used
- * (vk_hip_pdhg_panel_layout; zeros when none).  out holds 16 doubles.
+ * column pass (vk_hip_pdhg_profile), out[10..17] = the panel layout used
+ * (vk_hip_pdhg_panel_layout; zeros when none).  out holds 18 doubles.
  * Returns 0 on success. */
 int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
                   const int64_t *column_indices, const double *values,

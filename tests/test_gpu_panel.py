@@ -181,6 +181,22 @@ def test_panel_runs_repeat_bitwise_and_ignore_lane_count():
         assert np.array_equal(r.x, a.x) and np.array_equal(r.y, a.y)
 
 
+@pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
+def test_panel_tuner_changes_no_result(precision):
+    # The tuner picks the lane count and the unroll depth per operand by
+    # timing them on the matrix; neither changes a bit of the iterates.
+    lp = V.generate_lp(1, 12_000, 40_000, 5, 21)
+    kw = dict(spmv_panels=2, pdhg_precision=precision, iteration_limit=500)
+    a = V.solve_lp(lp, V.Options(**kw))
+    b = V.solve_lp(lp, V.Options(spmv_autotune=True, **kw))
+    assert (a.status, a.iterations) == (b.status, b.iterations)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    prof = V.profile_kernels(lp, V.Options(spmv_autotune=True, spmv_panels=2, pdhg_precision=precision),
+                             warm_iterations=2, iterations=2)
+    for i in (0, 4):
+        assert prof["panels"][i + 1] in (256, 512) and prof["panels"][i + 3] in (4, 8, 12, 16), prof
+
+
 def test_panel_automatic_at_scale_matches_oracle():
     # n = 300,000 > 2^18: the automatic rule picks the panel layout
     lp = V.generate_lp(1, 100_000, 300_000, 5, 7)
@@ -189,7 +205,7 @@ def test_panel_automatic_at_scale_matches_oracle():
     ref = O.solve_lp(lp.as_oracle(), iteration_limit=200)
     close(got, ref)
     prof = V.profile_kernels(lp, V.Options(), warm_iterations=2, iterations=2)
-    assert prof["panels"][0] > 0 and prof["panels"][3] > 0, prof
+    assert prof["panels"][0] > 0 and prof["panels"][4] > 0, prof
     assert prof["dual_spmv_ms"] > 0 and prof["dual_pass_ms"] > 0
     assert prof["primal_spmv_ms"] > 0 and prof["primal_pass_ms"] > 0
 

@@ -558,6 +558,52 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
     return VK_OK;
 }
 
+int panel_tune_steps(PanelLayout &L, int repeats, double *best_ms, hipStream_t st) {
+    void *x = nullptr;
+    double *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = VK_OK;
+    auto check = [&](hipError_t e) {
+        if (e != hipSuccess && rc == VK_OK) {
+            set_error(std::string("panel tuner: ") + hipGetErrorString(e));
+            rc = e == hipErrorOutOfMemory ? VK_ENOMEM : VK_EDEVICE;
+        }
+    };
+    check(hipMalloc(&x, static_cast<std::size_t>(L.value_bytes) * (L.cols + 4)));
+    check(hipMalloc(&out, sizeof(double) * L.groups * static_cast<std::size_t>(L.rows)));
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    if (rc == VK_OK) check(hipMemsetAsync(x, 0, static_cast<std::size_t>(L.value_bytes) * (L.cols + 4), st));
+    const int automatic = L.fast_steps;
+    int best = automatic;
+    double best_t = 1e300;
+    for (int km : {4, 8, 12, 16}) {
+        if (rc != VK_OK) break;
+        if (km + 4 < automatic || km > std::max(automatic + 4, 4) || km > 16) continue;
+        L.fast_steps = km;
+        panel_spmv_launch(L, nullptr, 0, x, x, out, st);   // warm-up
+        check(hipEventRecord(e0, st));
+        for (int r = 0; r < repeats; ++r) panel_spmv_launch(L, nullptr, 0, x, x, out, st);
+        check(hipEventRecord(e1, st));
+        check(hipEventSynchronize(e1));
+        check(hipGetLastError());
+        float t = 0.0f;
+        check(hipEventElapsedTime(&t, e0, e1));
+        const double ms = static_cast<double>(t) / std::max(1, repeats);
+        if (rc == VK_OK && ms < best_t) {
+            best_t = ms;
+            best = km;
+        }
+    }
+    L.fast_steps = best;
+    if (best_ms) *best_ms = best_t;
+    if (x) (void)hipFree(x);
+    if (out) (void)hipFree(out);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int panel_spmv_launch(const PanelLayout &L, const void *state, int select, const void *x0, const void *x1,
                       double *out, hipStream_t st) {
     const DevState *s = static_cast<const DevState *>(state);

@@ -124,6 +124,12 @@ struct PanelLayout {
 int panel_build(const int *rp, const int *ci, const void *val, int value_bytes, int rows, int cols, int nnz,
                 int groups, int threads, PanelLayout &out, hipStream_t st);
 
+// Times `repeats` launches of L (on a zero vector) for each unroll depth
+// among 4, 8, 12, 16 not below L's automatic choice by more than one step,
+// and keeps the fastest in L.fast_steps; returns the mean ms of that choice.
+// The depth never changes results.
+int panel_tune_steps(PanelLayout &L, int repeats, double *best_ms, hipStream_t st);
+
 // Automatic group count for an operand (rows x cols, gathered entries of
 // value_bytes): the smallest of 1, 2, 4, 8 that minimises the modelled
 // staging + partial-sum traffic.

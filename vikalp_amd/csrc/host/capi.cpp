@@ -167,7 +167,7 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets, const int64_
         out[7] = p.dual_pass_ms;
         out[8] = p.primal_spmv_ms;
         out[9] = p.primal_pass_ms;
-        for (int i = 0; i < 6; ++i) out[10 + i] = p.panel[i];
+        for (int i = 0; i < 8; ++i) out[10 + i] = p.panel[i];
     } catch (const std::exception &e) {
         return -2;
     }

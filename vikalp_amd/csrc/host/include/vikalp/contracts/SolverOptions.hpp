@@ -59,7 +59,8 @@ struct SolverOptions {
     // outgrows 2 MiB), 1 = plain row tiles, 2..16 = that many column blocks.
     int spmv_col_blocks = 0;
     // Time the layouts on the device before the solve and use the fastest
-    // (overrides spmv_col_blocks).
+    // (overrides spmv_col_blocks; for the panel layout, the lane count and
+    // unroll depth, which never change results).
     bool spmv_autotune = false;
     // SpMV and epilogue kernels: 0 = automatic, 1 = fused, 2 = split into a
     // SpMV-only kernel and a row / column pass (identical results).

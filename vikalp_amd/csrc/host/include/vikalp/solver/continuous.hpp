@@ -63,9 +63,9 @@ struct KernelProfile {
     double dual_spmv_ms = 0.0, dual_pass_ms = 0.0, primal_spmv_ms = 0.0, primal_pass_ms = 0.0;
     int col_blocks = 1;   // layout the solver used
     int split_passes = 0;
-    // Panel layout (zeros when not used): groups, lanes per workgroup and
-    // most entries per lane and panel, of A x then A' y.
-    int panel[6] = {};
+    // Panel layout (zeros when not used): groups, lanes per workgroup, most
+    // entries per lane and panel and the unrolled entries, of A x then A' y.
+    int panel[8] = {};
 };
 KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &options,
                                  Index warm_iterations, Index iterations);

@@ -103,7 +103,7 @@ class PdhgParams(C.Structure):
         ("norm_b", C.c_double), ("norm_c", C.c_double), ("use_graphs", C.c_int),
         ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
         ("cache_budget_mb", C.c_int), ("spmv_panels", C.c_int), ("panel_groups", C.c_int),
-        ("panel_groups_t", C.c_int), ("panel_threads", C.c_int),
+        ("panel_groups_t", C.c_int), ("panel_threads", C.c_int), ("panel_tune", C.c_int),
     ]
 
 
@@ -462,7 +462,7 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     ci = np.ascontiguousarray(lp.column_indices, dtype=np.int64)
     f = [np.ascontiguousarray(a, dtype=np.float64) for a in
          (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
-    out = np.zeros(16)
+    out = np.zeros(18)
     copt = (options or Options())._c()
     rc = L.vk_lp_profile(lp.m, lp.n, _ptr(rp, _lp), _ptr(ci, _lp), *[_ptr(a) for a in f],
                          C.byref(copt), int(warm_iterations), int(iterations), _ptr(out))
@@ -471,7 +471,7 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3],
             "col_blocks": int(out[4]), "split_passes": int(out[5]), "dual_spmv_ms": out[6],
             "dual_pass_ms": out[7], "primal_spmv_ms": out[8], "primal_pass_ms": out[9],
-            "panels": [int(v) for v in out[10:16]]}
+            "panels": [int(v) for v in out[10:18]]}
 
 
 def _result(info, x, y):
