@@ -55,6 +55,24 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu_model():
+    """The host CPU as /proc/cpuinfo names it (model name, family / model
+    numbers and clock of the first processor), or the machine type."""
+    fields = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if not line.strip():
+                    break
+                key, _, value = line.partition(":")
+                fields[key.strip()] = value.strip()
+    except OSError:
+        pass
+    name = fields.get("model name") or platform.processor() or platform.machine()
+    extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "cpu MHz") if k in fields]
+    return f"{name} ({', '.join(extra)})" if extra else name
+
+
 def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True):
     """Algorithmic (compulsory) HBM bytes and achieved GB/s per kernel of one
     iteration, for the layout the solver used.
@@ -105,6 +123,16 @@ def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True):
             for name, ms, b in rows}
 
 
+def fused_iteration_bytes(m, n, nnz, sv, halpern=True):
+    """Algorithmic bytes of one whole iteration run as fused kernels (the
+    batched kernel's form): both SpMVs without sum stores, with the row and
+    column passes' vectors (kernel_table's formulas)."""
+    spmv = 2 * (sv + 4) * nnz + 4 * (m + 1) + 4 * (n + 1) + sv * n + sv * m
+    rows = 10 * sv * m if halpern else 8 * sv * m + 16 * m
+    cols = 12 * sv * n if halpern else 8 * sv * n + 16 * n
+    return spmv + rows + cols
+
+
 # HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, scaled by a 256 MiB
 # device copy) measured with `bench.py --counters` under rocprofv3 --pmc;
 # profiles/pdhg_traffic_zipf_400k_x_1m.md holds the passes and commands.
@@ -152,11 +180,60 @@ def measure_batch(V, np, args):
         infeasible = sum(r.status == "Infeasible" for r in res)
         its = np.array([r.iterations for r in res])
         dev = res[0].device_seconds
+        # Algorithmic bytes of every instance's iterations (FP32 phase at 4-byte
+        # values), over the batch's device time: the whole-batch rate, tail
+        # hand-off included.
+        nbytes = 0.0
+        for lp_i, r in zip(lps, res):
+            lo = r.low_precision_iterations if precision == V.PdhgPrecision.Mixed else 0
+            nbytes += lo * fused_iteration_bytes(lp_i.m, lp_i.n, lp_i.nnz, 4)
+            nbytes += (r.iterations - lo) * fused_iteration_bytes(lp_i.m, lp_i.n, lp_i.nnz, 8)
+        gbs = nbytes / dev / 1e9 if dev > 0 else None
         out[label] = {"device_seconds": dev, "wall_seconds": wall, "optimal": optimal, "infeasible": infeasible,
                       "decided_per_second": (optimal + infeasible) / dev if dev > 0 else None,
-                      "iterations_median": float(np.median(its)), "iterations_max": int(its.max())}
+                      "iterations_median": float(np.median(its)), "iterations_max": int(its.max()),
+                      "algorithmic_bytes": nbytes,
+                      "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": gbs / HBM_PEAK_GBS if gbs else None,
+                                   "note": "fused-iteration bytes of every instance / batch device time"}}
         log(f"batch {label}: {optimal}/{len(lps)} optimal, {infeasible} proved infeasible, device {dev:.2f}s, "
-            f"wall {wall:.2f}s, iterations median {np.median(its):.0f} max {its.max()}")
+            f"wall {wall:.2f}s, iterations median {np.median(its):.0f} max {its.max()}, "
+            f"{gbs:.0f} GB/s algorithmic")
+    return out
+
+
+def measure_config1(V, np, args):
+    """The 1,000 x 3,000 LP (5 nonzeros per column, seed 20261203): the CPU
+    oracle's full solve with the reference's defaults (tolerances 1e-7,
+    100,000 iterations; it runs to the limit) timed as the median of
+    `args.c1_cpu_runs` runs on one host core, beside the same solve on the
+    GPU's drop-in path (same status and iteration count) and the Halpern
+    solves to 1e-6 relative KKT in FP64 and mixed precision."""
+    from oracle import oracle as O   # the checker, timed as the CPU baseline
+    lp = V.generate_lp(1, 1000, 3000, 5, WORKLOAD["seed"])
+    cpu_times, ref = [], None
+    for _ in range(args.c1_cpu_runs):
+        t0 = time.perf_counter()
+        ref = O.solve_lp(lp.as_oracle())
+        cpu_times.append(time.perf_counter() - t0)
+    gpu = V.solve_lp(lp, V.Options())
+    out = {"m": lp.m, "n": lp.n, "nnz": lp.nnz, "cpu_runs": len(cpu_times),
+           "cpu_reference_seconds_median": float(np.median(cpu_times)) if cpu_times else None,
+           "cpu_status": ref.status if ref else None, "cpu_iterations": ref.iterations if ref else None,
+           "cpu_cores": 1, "host_cpu": host_cpu_model(),
+           "gpu_reference_status": gpu.status, "gpu_reference_iterations": gpu.iterations,
+           "gpu_reference_device_seconds": gpu.device_seconds, "gpu_reference_wall_seconds": gpu.solve_seconds}
+    for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
+        r = V.solve_lp(lp, V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision,
+                                     relative_kkt_tolerance=WORKLOAD["tolerance"], mixed_phase_tolerance=1e-4,
+                                     iteration_limit=args.max_iterations))
+        out[f"halpern_{label}"] = {"status": r.status, "iterations": r.iterations,
+                                   "device_seconds": r.device_seconds, "wall_seconds": r.solve_seconds,
+                                   "rel_kkt": r.kkt_relative}
+    log(f"config 1: CPU reference solve median {out['cpu_reference_seconds_median']} s "
+        f"({out['cpu_status']}, {out['cpu_iterations']} iterations); GPU drop-in {gpu.status} "
+        f"{gpu.iterations} iterations in {gpu.solve_seconds:.3f} s wall; Halpern 1e-6 FP64 "
+        f"{out['halpern_fp64']['device_seconds']:.3f} s, mixed {out['halpern_mixed']['device_seconds']:.3f} s")
     return out
 
 
@@ -309,9 +386,14 @@ def measure_adjoint(V, np, args):
 
     # Two error measures per direction d: |fd - g'd| / |g'd| (ill-conditioned
     # when d is nearly orthogonal to g, as random directions in 350,000
-    # dimensions are: |g'd| ~ |g| / 600) and |fd - g'd| / |g|.
+    # dimensions are: |g'd| ~ |g| / 600) and |fd - g'd| / |g|.  Acceptance:
+    # |fd - g'd| <= 1e-4 |g'd| + 1e-6 |g|.  The floor is the central
+    # difference's own rounding noise: each loss carries a relative error of
+    # a few ulps of |L| through K iterations, divided by h = 1e-6 that is
+    # ~1e-9 |L| / h ~ 1e-6 |g| here; a direction with |g'd| below 1e-2 |g|
+    # cannot be resolved to 1e-4 by this difference in FP64.
     gnorm = math.sqrt(float(gc @ gc + gb @ gb))
-    errs, errs_norm = [], []
+    errs, errs_norm, fails = [], [], 0
     loss0 = loss(np.zeros(lp.n), np.zeros(lp.m), 0.0)
     for _ in range(args.fd_directions):
         dc, db = rng.normal(size=lp.n), rng.normal(size=lp.m)
@@ -321,14 +403,35 @@ def measure_adjoint(V, np, args):
         an = float(gc @ dc + gb @ db)
         errs.append(abs(fd - an) / max(abs(an), 1e-300))
         errs_norm.append(abs(fd - an) / gnorm)
+        fails += abs(fd - an) > 1e-4 * abs(an) + 1e-6 * gnorm
     log(f"adjoint: m={lp.m} n={lp.n} nnz={lp.nnz} K={K}: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, "
         f"central differences over {len(errs)} directions: |fd-g'd|/|g'd| median {np.median(errs):.2e} "
         f"max {max(errs):.2e}; |fd-g'd|/|g| max {max(errs_norm):.2e} (loss {loss0:.6g}, |g| {gnorm:.4g})")
+    # Algorithmic bytes per unrolled step: forward = A'y with the primal
+    # update (x, c, l, u read, x written) and A xbar with the dual update (y,
+    # rl, ru read, y written), each SpMV (8 + 4) nnz + offsets + its gathered
+    # vector once; backward = the recomputed forward step plus its active-set
+    # bits, then the adjoint step's two SpMVs and their vector traffic
+    # (g_x, g_y, g_xbar, g_c in and out, the bits read).
+    m, n, nnz = lp.m, lp.n, lp.nnz
+    spmvs = 2 * 12 * nnz + 4 * (m + 1) + 4 * (n + 1) + 8 * m + 8 * n
+    fwd_step = spmvs + 5 * 8 * n + 4 * 8 * m
+    bits = (m + n) / 8
+    bwd_step = fwd_step + bits + spmvs + 6 * 8 * n + 4 * 8 * m + bits
+    fwd_gbs = K * fwd_step / (fwd_ms * 1e-3) / 1e9
+    bwd_gbs = K * bwd_step / (bwd_ms * 1e-3) / 1e9
+    log(f"adjoint: forward {fwd_gbs:.0f} GB/s, backward {bwd_gbs:.0f} GB/s algorithmic; "
+        f"{sum(e > 1e-4 for e in errs)} of {len(errs)} directions above 1e-4 relative, {fails} outside "
+        f"1e-4 |g'd| + 1e-6 |g|")
     return {"m": lp.m, "n": lp.n, "nnz": lp.nnz, "iterations": K, "checkpoint_every": 50,
             "forward_ms": fwd_ms, "backward_ms": bwd_ms, "fd_directions": len(errs), "fd_step": 1e-6,
+            "forward_GBs": fwd_gbs, "backward_GBs": bwd_gbs,
             "fd_rel_err_directional_median": float(np.median(errs)),
             "fd_rel_err_directional_max": max(errs),
+            "fd_directions_above_1e-4": int(sum(e > 1e-4 for e in errs)),
             "fd_err_rel_gradient_norm_max": max(errs_norm),
+            "fd_acceptance": "|fd - g'd| <= 1e-4 |g'd| + 1e-6 |g|",
+            "fd_directions_failing_acceptance": int(fails),
             "loss": loss0, "gradient_norm": gnorm}
 
 
@@ -342,6 +445,9 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=200_000)
     ap.add_argument("--time-limit", type=float, default=100.0, help="seconds per solve")
     ap.add_argument("--cpu-iterations", type=int, default=200)
+    ap.add_argument("--c1-cpu-runs", type=int, default=5,
+                    help="runs of the CPU oracle's full solve of the 1,000 x 3,000 LP (median; 0 skips "
+                         "the config-1 measurement)")
     ap.add_argument("--phase-tolerance", type=float, default=1e-5,
                     help="relative KKT target of the FP32 phase of the mixed solve")
     ap.add_argument("--batch-instances", type=int, default=512,
@@ -456,10 +562,14 @@ def main():
         cpu = {"value": per_it * its, "unit": "s (extrapolated)", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_iterations} iterations of the reference loop (4 SpMVs/iteration) "
                          f"on the same LP: {per_it:.3f} s/iteration x {its} iterations of the "
-                         f"FP64 GPU solve", "host_cpu": platform.processor() or platform.machine(),
-               "host_nproc": os.cpu_count()}
+                         f"FP64 GPU solve (the GPU runs the Halpern policy, 2 SpMVs/iteration: the "
+                         f"extrapolation prices the reference algorithm at the GPU's iteration count)",
+               "host_cpu": host_cpu_model(), "host_nproc": os.cpu_count()}
         log(f"cpu baseline: {per_it:.3f} s/iteration")
 
+    config1 = None
+    if args.c1_cpu_runs > 0 and not args.skip_solve and not args.skip_cpu:
+        config1 = measure_config1(V, np, args)
     batch = None
     if args.batch_instances > 0 and not args.skip_solve:
         batch = measure_batch(V, np, args)
@@ -517,6 +627,7 @@ def main():
                              "panels": prof32["panels"]}},
         "kernels": {"fp64": k64, "mixed": k32},
         "cpu_baseline": cpu,
+        "config1": config1,
         "adaptive_fp64": adaptive,
         "batch": batch,
         "adjoint": adjoint,

@@ -18,8 +18,8 @@
 // so the SpMVs of the forward step run in the opposite order on the other
 // operand.  Only the projections' active sets enter: each segment of
 // checkpoint_every steps is recomputed from its checkpoint with the masks
-// recorded (one byte per column and per row per step), then swept in
-// reverse.  The recomputation runs the forward kernels on the same inputs,
+// recorded as bitsets (one bit per column -- inside its box -- and two per
+// row -- below / above its bounds -- per step), then swept in reverse.  The recomputation runs the forward kernels on the same inputs,
 // so the masks equal the forward pass's bitwise.  Every accumulation runs in
 // a fixed order; results repeat bitwise.
 
@@ -41,11 +41,43 @@ namespace {
 struct UnrolledArgs {
     double *x, *y, *xbar;
     const double *c, *l, *u, *rl, *ru;
-    uint8_t *mx, *my;        // masks of this step (nullptr: not recorded)
+    // Bitsets of this step (nullptr: not recorded): column j inside [l, u];
+    // row i below rl / above ru.  Bit b of word w is entry 64 w + b.
+    unsigned long long *mx, *my_lo, *my_hi;
     double *gx, *gy, *gw, *gz, *gc, *grl, *gru;
     double tau, sigma, inv_sigma;
     int m, n;
 };
+
+// Writes the bits of entries [r0, r0 + 64) that `own` selects (bit b: entry
+// r0 + b) of bitset `set` to `val`.  A word can straddle two tiles whose
+// waves write their own bits concurrently, so each word takes an atomic AND
+// (clearing the owned bits that become 0) and an atomic OR (setting those
+// that become 1); bits the wave does not own are left alone.
+__device__ __forceinline__ void store_bits(unsigned long long *set, int r0, unsigned long long own,
+                                           unsigned long long val) {
+    auto put = [](unsigned long long *w, unsigned long long o, unsigned long long v) {
+        if (!o) return;
+        atomicAnd(w, ~o | v);
+        atomicOr(w, v);
+    };
+    const int w = r0 >> 6, s = r0 & 63;
+    put(set + w, own << s, val << s);
+    if (s) put(set + w + 1, own >> (64 - s), val >> (64 - s));
+}
+
+// The epilogue runs on consecutive lanes for consecutive rows starting at
+// lane 0 (tile_spmv's phase 3, or lane 0 alone for a long row), so the
+// wave's ballots are the bits of rows [j - lane, j - lane + 64).
+__device__ __forceinline__ void record_bit(unsigned long long *set, int j, bool bit) {
+    const unsigned long long own = __ballot(1), val = __ballot(bit);
+    const int lane = static_cast<int>(threadIdx.x % kWave);
+    if (lane == 0) store_bits(set, j, own, val);
+}
+
+__device__ __forceinline__ bool read_bit(const unsigned long long *set, int j) {
+    return (set[j >> 6] >> (j & 63)) & 1ull;
+}
 
 // pdhg.cpp:92-96 and :102-103 on column j given acc = (A'y)_j.
 struct FwdPrimalEpi {
@@ -57,7 +89,7 @@ struct FwdPrimalEpi {
     __device__ void row(int j, double acc, const Pre &q, Partials &) const {
         double z = -a.tau * q.c + q.x;
         z = -a.tau * acc + z;
-        if (a.mx) a.mx[j] = static_cast<uint8_t>(!(z < q.l) && !(q.u < z));
+        if (a.mx) record_bit(a.mx, j, !(z < q.l) && !(q.u < z));
         const double xn = clamp_box(z, q.l, q.u);
         a.xbar[j] = 2.0 * xn + -1.0 * q.x;
         a.x[j] = xn;
@@ -74,7 +106,10 @@ struct FwdDualEpi {
     __device__ void row(int i, double acc, const Pre &q, Partials &) const {
         const double w = a.sigma * acc + q.y;
         const double t = a.inv_sigma * w;
-        if (a.my) a.my[i] = static_cast<uint8_t>((t < q.lo) ? 1 : ((q.hi < t) ? 2 : 0));
+        if (a.my_lo) {
+            record_bit(a.my_lo, i, t < q.lo);
+            record_bit(a.my_hi, i, q.hi < t);
+        }
         const double p = clamp_box(t, q.lo, q.hi);
         a.y[i] = -a.sigma * p + w;
     }
@@ -85,9 +120,9 @@ struct BwdPrimalEpi {
     const UnrolledArgs &a;
     struct Pre {
         double gx, gc;
-        uint8_t in;
+        bool in;
     };
-    __device__ Pre prefetch(int j) const { return Pre{a.gx[j], a.gc[j], a.mx[j]}; }
+    __device__ Pre prefetch(int j) const { return Pre{a.gx[j], a.gc[j], read_bit(a.mx, j)}; }
     __device__ void row(int j, double acc, const Pre &q, Partials &) const {
         const double gxb = a.sigma * acc;
         const double gz = q.in ? q.gx + 2.0 * gxb : 0.0;
@@ -119,11 +154,11 @@ __global__ __launch_bounds__(kBlock) void k_unrolled_spmv(UnrolledArgs a, CsrVie
 // Backward, rows, elementwise: gw and the row-bound adjoints of one step.
 __global__ void k_unrolled_rows(UnrolledArgs a) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.m; i += gridDim.x * kBlock) {
-        const uint8_t code = a.my[i];
+        const bool below = read_bit(a.my_lo, i), above = read_bit(a.my_hi, i);
         const double g = a.gy[i];
-        a.gw[i] = code ? g : 0.0;
-        if (code == 1) a.grl[i] = a.grl[i] - a.sigma * g;
-        if (code == 2) a.gru[i] = a.gru[i] - a.sigma * g;
+        a.gw[i] = (below || above) ? g : 0.0;
+        if (below) a.grl[i] = a.grl[i] - a.sigma * g;
+        if (above) a.gru[i] = a.gru[i] - a.sigma * g;
     }
 }
 
@@ -147,7 +182,10 @@ struct vk_hip_unrolled {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     UnrolledArgs a{};
     double *ckpt = nullptr;   // segments x (n + m): x then y of each checkpoint
-    uint8_t *masks_x = nullptr, *masks_y = nullptr;   // every x n, every x m
+    // Step slot s: column bits at masks_x + s * wx, row bits (below, above)
+    // at masks_y + s * 2 * wy and + wy (wx, wy: 64-bit words of n and m).
+    unsigned long long *masks_x = nullptr, *masks_y = nullptr;
+    size_t wx = 0, wy = 0;
     bool forward_done = false;
     double last_ms = 0.0;
     std::vector<void *> allocations;
@@ -163,6 +201,17 @@ struct vk_hip_unrolled {
 
 namespace {
 
+// Points a's bitsets at step slot `slot` (-1: none recorded).
+void set_mask_slot(const vk_hip_unrolled *h, UnrolledArgs &a, long long slot) {
+    if (slot < 0) {
+        a.mx = a.my_lo = a.my_hi = nullptr;
+        return;
+    }
+    a.mx = h->masks_x + static_cast<size_t>(slot) * h->wx;
+    a.my_lo = h->masks_y + static_cast<size_t>(slot) * 2 * h->wy;
+    a.my_hi = a.my_lo + h->wy;
+}
+
 // Steps [k0, k1) from the current (x, y); masks of step k go to slot k - k0
 // when `record`, and (x, y) is saved at every checkpoint boundary when
 // `save`.
@@ -175,8 +224,7 @@ void forward_steps(vk_hip_unrolled *h, long long k0, long long k1, bool record, 
             (void)hipMemcpyAsync(dst, a.x, sizeof(double) * h->n, hipMemcpyDeviceToDevice, h->stream);
             if (h->m) (void)hipMemcpyAsync(dst + h->n, a.y, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream);
         }
-        a.mx = record ? h->masks_x + static_cast<size_t>(k - k0) * h->n : nullptr;
-        a.my = record ? h->masks_y + static_cast<size_t>(k - k0) * h->m : nullptr;
+        set_mask_slot(h, a, record ? k - k0 : -1);
         k_unrolled_spmv<FwdPrimalEpi><<<sgrid(AT.ntiles), kBlock, 0, h->stream>>>(a, AT, a.y);
         if (h->m) k_unrolled_spmv<FwdDualEpi><<<sgrid(A.ntiles), kBlock, 0, h->stream>>>(a, A, a.xbar);
     }
@@ -252,8 +300,10 @@ int vk_hip_unrolled_create(const vk_hip_csr *A, const double *c, const double *l
     a.grl = h->alloc<double>(m);
     a.gru = h->alloc<double>(m);
     h->ckpt = h->alloc<double>(static_cast<size_t>(std::max<long long>(1, h->segments)) * (n + m));
-    h->masks_x = h->alloc<uint8_t>(static_cast<size_t>(h->every) * n);
-    h->masks_y = h->alloc<uint8_t>(static_cast<size_t>(h->every) * std::max(m, 1));
+    h->wx = (static_cast<size_t>(n) + 63) / 64;
+    h->wy = (static_cast<size_t>(std::max(m, 1)) + 63) / 64;
+    h->masks_x = h->alloc<unsigned long long>(static_cast<size_t>(h->every) * h->wx);
+    h->masks_y = h->alloc<unsigned long long>(static_cast<size_t>(h->every) * 2 * h->wy);
     for (void *p : h->allocations)
         if (!p) {
             set_error("vk_hip_unrolled_create: device allocation failed");
@@ -321,8 +371,7 @@ int vk_hip_unrolled_backward(vk_hip_unrolled *h, const double *grad_x, const dou
         if (m) VK_HIP_CHECK(hipMemcpyAsync(a.y, src + n, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
         forward_steps(h, k0, k1, true, false);
         for (long long k = k1 - 1; k >= k0; --k) {
-            a.mx = h->masks_x + static_cast<size_t>(k - k0) * n;
-            a.my = h->masks_y + static_cast<size_t>(k - k0) * m;
+            set_mask_slot(h, a, k - k0);
             if (m) k_unrolled_rows<<<ugrid(m), kBlock, 0, st>>>(a);
             // A' has a tile for every column (empty ones included), so the
             // column update runs for all of them even without rows.
