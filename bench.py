@@ -150,9 +150,10 @@ def counters_driver(V, args):
     dev.free_all()
     log("calibration copies done")
     lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
-    p = V.profile_kernels(lp, V.Options(pdhg_policy=V.PdhgPolicy.Halpern,
-                                        relative_kkt_tolerance=WORKLOAD["tolerance"]), 2, 6)
-    log(f"profiled: col_blocks={p['col_blocks']} split={p['split_passes']}")
+    for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
+        p = V.profile_kernels(lp, V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision,
+                                            relative_kkt_tolerance=WORKLOAD["tolerance"]), 2, 6)
+        log(f"profiled {label}: col_blocks={p['col_blocks']} split={p['split_passes']} panels={p['panels']}")
 
 
 def measure_batch(V, np, args):
@@ -475,7 +476,8 @@ def main():
                          "rocprofv3 --kernel-trace does not follow graph replays with a forked stream)")
     ap.add_argument("--counters", action="store_true",
                     help="driver for the rocprofv3 --pmc passes: three 256 MiB device copies "
-                         "(counter calibration), then 2 + 6 profiled FP64 iterations; no JSON line")
+                         "(counter calibration), then 2 + 6 profiled iterations in FP64, then in "
+                         "FP32 (the mixed solve's first phase); no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (--gpus {args.gpus} asked): the PDHG path runs on one "

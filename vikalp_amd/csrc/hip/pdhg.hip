@@ -259,8 +259,39 @@ __device__ __forceinline__ double block_partials_sum(const double *__restrict__ 
     return acc;
 }
 
-// Dual step over rows from the column-blocked partial sums (k_dual's
-// epilogue; the nb partials are added in block order).
+// Items a thread of the panel layout's row / column passes takes at once.
+constexpr int kPassItems = 4;
+
+// The epilogue over items [0, count) from their nb partial sums (added in
+// partial order), kPassItems items per thread and step: the loads of all of
+// them (the epilogue's prefetch and the partials) are issued before the first
+// item's arithmetic, and the workgroup's 8-slot reduction is shared by
+// kPassItems * kBlock items.
+template <class Epi>
+__device__ __forceinline__ void pass_items(Epi &epi, const double *__restrict__ part, int count, int nb,
+                                           Partials &acc) {
+    constexpr int U = kPassItems;
+    for (int i0 = blockIdx.x * (kBlock * U) + threadIdx.x; i0 < count; i0 += gridDim.x * (kBlock * U)) {
+        typename Epi::Pre pre[U];
+        double sum[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i < count) {
+                pre[u] = epi.prefetch(i);
+                sum[u] = block_partials_sum(part, count, i, nb);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i < count) epi.row(i, sum[u], pre[u], acc);
+        }
+    }
+}
+
+// Dual step over rows from the column-blocked (or panel group) partial sums
+// (k_dual's epilogue; the nb partials are added in block order).
 template <class T, bool H>
 __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *__restrict__ part, int nb) {
     const DevState s = *b.st;
@@ -269,10 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *_
     auto epi = IterEpis<T, H, true>::dual(b, s);
     Partials acc_part;
     acc_part.zero();
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < b.m; i += gridDim.x * kBlock) {
-        const auto pre = epi.prefetch(i);
-        epi.row(i, block_partials_sum(part, b.m, i, nb), pre, acc_part);
-    }
+    pass_items(epi, part, b.m, nb, acc_part);
     store_partials(acc_part, b.part_rows, scratch);
 }
 
@@ -287,10 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b, const double 
     auto epi = IterEpis<T, H, true>::primal(b, s, true);
     Partials acc_part;
     acc_part.zero();
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
-        const auto pre = epi.prefetch(j);
-        epi.row(j, block_partials_sum(part, b.n, j, nb), pre, acc_part);
-    }
+    pass_items(epi, part, b.n, nb, acc_part);
     store_partials(acc_part, b.part_cols, scratch);
 }
 
@@ -1507,15 +1532,17 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
                                            : auto_col_blocks(s->n, s->mixed ? 4 : 8, s->m);
     s->col_blocks = std::max(1, std::min(s->col_blocks, s->n));
     if (s->m == 0) s->col_blocks = 1;
-    // With column blocks the row pass (one row per thread) owns grid_rows.
+    // With column blocks the row pass (kPassItems rows per thread) owns
+    // grid_rows.
     s->split = params->split_passes == 2 ||
                (params->split_passes == 0 && auto_split(s->m, s->n));
     s->grid_spmv_rows = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, kMaxGrid));
-    // The row pass runs one row per thread over grid_rows workgroups (which
-    // also size the row partial buffer).  In split mode the A' side is
+    // The row pass runs kPassItems rows per thread over grid_rows workgroups
+    // (which also size the row partial buffer).  In split mode the A' side is
     // k_primal_own over grid_cols workgroups, each with a contiguous tile
     // range (about kWaves tiles).
-    if (s->col_blocks > 1 || s->split) s->grid_rows = std::max(1, std::min((s->m + kBlock - 1) / kBlock, cap));
+    if (s->col_blocks > 1 || s->split)
+        s->grid_rows = std::max(1, std::min((s->m + kBlock * kPassItems - 1) / (kBlock * kPassItems), cap));
     auto fail = [&](int rc) {
         vk_hip_pdhg_destroy(s);
         return rc;
@@ -1570,12 +1597,13 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     if (params->spmv_panels == 2 || (params->spmv_panels == 0 && auto_panels(s->m, s->n))) {
         rc = build_panels(s);
         if (rc == VK_OK) {
-            // Row pass over k_dual_rows' grid, column pass over k_primal_cols'.
+            // Row pass over k_dual_rows' grid, column pass over k_primal_cols'
+            // (kPassItems items per thread).
             s->panels = true;
             s->col_blocks = 1;
             s->split = true;
-            s->grid_rows = std::max(1, std::min((s->m + kBlock - 1) / kBlock, cap));
-            s->grid_cols = std::max(1, std::min((s->n + kBlock - 1) / kBlock, cap));
+            s->grid_rows = std::max(1, std::min((s->m + kBlock * kPassItems - 1) / (kBlock * kPassItems), cap));
+            s->grid_cols = std::max(1, std::min((s->n + kBlock * kPassItems - 1) / (kBlock * kPassItems), cap));
         } else if (rc == VK_EINVAL && params->spmv_panels == 0) {
             s->pan_a.release();
             s->pan_at.release();
