@@ -239,11 +239,18 @@ typedef struct {
                                       partial sums per group, added in order */
     int panel_groups_t;            /* the same for A' y                       */
     int panel_threads;             /* lanes per panel workgroup: 0 = auto,
-                                      256 or 512 (results do not depend on it) */
-    int panel_tune;                /* 1: time both lane counts and the unroll
-                                      depths around the automatic one on the
+                                      256 (48 KB panels only) or 512 (results
+                                      do not depend on it)                    */
+    int panel_tune;                /* 1: time the automatic shape and, with
+                                      its group count, 48 KB panels with 512
+                                      and with 256 lanes, each over the unroll
+                                      depths around its automatic one, on the
                                       matrix at setup and keep the fastest
                                       (results do not depend on them)         */
+    int panel_kb;                  /* KB per staged panel, both operands: 0 =
+                                      auto (per operand, with the group count
+                                      when that is auto too), 48, 24 or 16
+                                      (results do not depend on it)          */
 } vk_pdhg_params;
 
 typedef struct {
@@ -303,10 +310,10 @@ int vk_hip_pdhg_profile(vk_hip_pdhg *solver, int64_t iterations, double *ms);
 /* The A x layout the solver chose: column blocks (1 = plain row tiles) and
  * whether the SpMVs run split from their row / column passes (0 or 1). */
 int vk_hip_pdhg_layout(const vk_hip_pdhg *solver, int *col_blocks, int *split_passes);
-/* The panel layouts the solver uses (zeros when it does not): info[0..3] =
- * groups, lanes per workgroup, most entries per lane and panel, and the
- * kernel's unrolled entries per lane and panel of A x; info[4..7] the same
- * for A' y. */
+/* The panel layouts the solver uses (zeros when it does not): info[0..4] =
+ * groups, lanes per workgroup, most entries per lane and panel, the
+ * kernel's unrolled entries per lane and panel, and KB per panel, of A x;
+ * info[5..9] the same for A' y. */
 int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *solver, int *info);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
 
@@ -323,15 +330,18 @@ int vk_hip_csr_tune_col_blocks(vk_hip_csr *A, int precision, int repeats, int *b
 
 /* Panel SpMV (new; the layout behind vk_pdhg_params.spmv_panels): builds
  * the panel layout of A (transpose = 0; rows x cols) or A' (transpose = 1)
- * with `groups` column groups (1, 2, 4 or 8) and `threads` lanes per
- * workgroup (256 or 512), in FP64 or, for VK_PREC_MIXED, FP32 values and x
- * (x rounded to FP32 first), and writes the FP64 partial sums of every group
- * to the host array `partial` (groups x rows, group-major): partial[g*rows+r]
- * is the sequential sum, in column order, of row r's products over group
- * g's columns.  `repeats` > 0 also times that many launches (mean ms into
- * *ms, may be NULL).  VK_EINVAL when the operand does not fit the layout. */
-int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int precision, int groups, int threads,
-                          const double *x, double *partial, int repeats, double *ms);
+ * with `groups` column groups (1, 2, 4 or 8), panels of `panel_kb` KB (48,
+ * 24, 16; 0 = the solver's automatic choice for that group count) and
+ * `threads` lanes per workgroup (512, or 256 with 48 KB panels), in FP64 or,
+ * for VK_PREC_MIXED, FP32 values and x (x rounded to FP32 first), and
+ * writes the FP64 partial sums of every group to the host array `partial`
+ * (groups x rows, group-major): partial[g*rows+r] is the sequential sum, in
+ * column order, of row r's products over group g's columns (the same bits
+ * for every panel size and lane count).  `repeats` > 0 also times that many
+ * launches (mean ms into *ms, may be NULL).  VK_EINVAL when the operand does
+ * not fit the layout. */
+int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int precision, int groups, int panel_kb,
+                          int threads, const double *x, double *partial, int repeats, double *ms);
 
 /* ---- batched PDHG (new: one persistent launch for many LPs) ------------
  *

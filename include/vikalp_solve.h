@@ -49,6 +49,7 @@ typedef struct {
     int spmv_panel_groups;          /* A x column groups: 0 auto, 1, 2, 4, 8 */
     int spmv_panel_groups_t;        /* A' y column groups: 0 auto, 1, 2, 4, 8 */
     int spmv_panel_threads;         /* lanes per panel workgroup: 0 auto, 256, 512 */
+    int spmv_panel_kb;              /* KB per staged panel: 0 auto, 48, 24, 16 */
 } vk_solver_options;
 
 /* Fills `o` with the defaults of vikalp::SolverOptions. */
@@ -134,8 +135,8 @@ int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets,
  * used (1 = plain row tiles), out[5] = 1 when the SpMVs ran split from
  * their passes, out[6..9] = mean ms of the dual SpMV (or fused dual kernel),
  * the dual row pass, the primal SpMV (or fused primal kernel) and the primal
- * column pass (vk_hip_pdhg_profile), out[10..17] = the panel layout used
- * (vk_hip_pdhg_panel_layout; zeros when none).  out holds 18 doubles.
+ * column pass (vk_hip_pdhg_profile), out[10..19] = the panel layout used
+ * (vk_hip_pdhg_panel_layout; zeros when none).  out holds 20 doubles.
  * Returns 0 on success. */
 int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets,
                   const int64_t *column_indices, const double *values,

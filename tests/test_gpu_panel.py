@@ -106,10 +106,10 @@ def test_panel_spmv_is_the_sequential_group_sum(case, precision):
                     exp = expected_partials(trp, tci, tv, n, m, xr, G)
                 else:
                     exp = expected_partials(rp, ci, vv, m, n, xr, G)
-                for threads in (256, 512):
+                for threads, kb in ((256, 48), (512, 48), (512, 24), (512, 16)):
                     got, _ = dev.panel_spmv(h, m, n, x, transpose=transpose, precision=precision, groups=G,
-                                            threads=threads)
-                    assert np.array_equal(got, exp), (name, transpose, G, threads,
+                                            threads=threads, panel_kb=kb)
+                    assert np.array_equal(got, exp), (name, transpose, G, threads, kb,
                                                       float(np.max(np.abs(got - exp))))
     finally:
         dev.L.vk_hip_csr_destroy(h)
@@ -170,13 +170,14 @@ def test_panel_halpern_certificate_and_optimum(precision):
     assert abs(r.objective_value - g["objective"]) / max(1.0, abs(g["objective"])) <= 1e-6
 
 
-def test_panel_runs_repeat_bitwise_and_ignore_lane_count():
+def test_panel_runs_repeat_bitwise_and_ignore_lane_count_and_panel_size():
     lp = V.generate_lp(2, 400, 1000, 0, 5)
     kw = dict(iteration_limit=777, spmv_panels=2)
     a = V.solve_lp(lp, V.Options(use_graphs=True, spmv_panel_threads=256, **kw))
-    b = V.solve_lp(lp, V.Options(use_graphs=False, spmv_panel_threads=512, **kw))
-    c = V.solve_lp(lp, V.Options(use_graphs=True, spmv_panel_threads=512, **kw))
-    for r in (b, c):
+    runs = [V.solve_lp(lp, V.Options(use_graphs=False, spmv_panel_threads=512, **kw)),
+            V.solve_lp(lp, V.Options(use_graphs=True, spmv_panel_threads=512, **kw))]
+    runs += [V.solve_lp(lp, V.Options(spmv_panel_kb=kb, **kw)) for kb in (48, 24, 16)]
+    for r in runs:
         assert r.iterations == a.iterations
         assert np.array_equal(r.x, a.x) and np.array_equal(r.y, a.y)
 
@@ -193,8 +194,9 @@ def test_panel_tuner_changes_no_result(precision):
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
     prof = V.profile_kernels(lp, V.Options(spmv_autotune=True, spmv_panels=2, pdhg_precision=precision),
                              warm_iterations=2, iterations=2)
-    for i in (0, 4):
+    for i in (0, 5):
         assert prof["panels"][i + 1] in (256, 512) and prof["panels"][i + 3] in (4, 8, 12, 16), prof
+        assert prof["panels"][i + 4] in (48, 24, 16), prof
 
 
 def test_panel_automatic_at_scale_matches_oracle():
@@ -205,7 +207,7 @@ def test_panel_automatic_at_scale_matches_oracle():
     ref = O.solve_lp(lp.as_oracle(), iteration_limit=200)
     close(got, ref)
     prof = V.profile_kernels(lp, V.Options(), warm_iterations=2, iterations=2)
-    assert prof["panels"][0] > 0 and prof["panels"][4] > 0, prof
+    assert prof["panels"][0] > 0 and prof["panels"][5] > 0, prof
     assert prof["dual_spmv_ms"] > 0 and prof["dual_pass_ms"] > 0
     assert prof["primal_spmv_ms"] > 0 and prof["primal_pass_ms"] > 0
 
@@ -224,5 +226,6 @@ def test_forced_panels_on_a_matrix_that_does_not_fit_fail_cleanly():
 def test_invalid_panel_options_are_rejected():
     lp = V.generate_lp(1, 40, 90, 3, 3)
     for kw in (dict(spmv_panels=3), dict(spmv_panel_groups=3), dict(spmv_panel_groups_t=16),
-               dict(spmv_panel_threads=128)):
+               dict(spmv_panel_threads=128), dict(spmv_panel_kb=32),
+               dict(spmv_panel_threads=256, spmv_panel_kb=16)):
         assert V.solve_lp(lp, V.Options(**kw)).status == "InvalidModel", kw

@@ -22,7 +22,7 @@ void PanelLayout::release() {
     meta = nullptr;
     hdr = nullptr;
     ent = nullptr;
-    groups = threads = steps = fast_steps = nwg = rows = cols = value_bytes = 0;
+    groups = panel_kb = threads = steps = fast_steps = nwg = rows = cols = value_bytes = 0;
     ent_bytes = 0;
 }
 
@@ -41,20 +41,21 @@ __device__ __forceinline__ float entry_value(const panel_u32x2 &e) { return __ui
 // buffer and a barrier ends the panel.  Entries a lane does not own (lane
 // >= the step's count) are loaded from a clamped, valid address and never
 // stored, so every load is unconditional and in bounds.
-template <class V, int NT, int KM>
+template <class V, int NT, int KM, int KB>
 __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevState *__restrict__ state, int select,
                                                    const V *__restrict__ x0, const V *__restrict__ x1,
                                                    double *__restrict__ out) {
 #pragma clang fp contract(off)
     using E = typename PanelEntry<V>::E;
-    constexpr int P = panel_width<V>();
+    constexpr int P = panel_width<V>(KB);
+    constexpr int kRows = panel_max_rows(KB);
     constexpr int kW = NT / kWave;
     constexpr int kVec = 16 / static_cast<int>(sizeof(V));
     constexpr int kPer = P / kVec / NT;   // 16-byte loads per thread and panel
     static_assert(kPer * kVec * NT == P, "panel width must split evenly over the lanes");
     constexpr int kHW = 2 + (KM + 3) / 4;   // header words used
     __shared__ __attribute__((aligned(16))) V xbuf[2][P];
-    __shared__ double acc[kPanelMaxRows];
+    __shared__ double acc[kRows];
 
     const V *x = x0;
     if (state) {
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevSta
         for (int j = 0; j < KM; ++j) {
             const unsigned e0 = ce[j].x;
             const int col = min(static_cast<int>(e0 & ((1u << kPanelColBits) - 1u)), P - 1);
-            rows[j] = min(static_cast<int>(e0 >> kPanelColBits), kPanelMaxRows - 1);
+            rows[j] = min(static_cast<int>(e0 >> kPanelColBits), kRows - 1);
             prod[j] = static_cast<double>(entry_value(ce[j])) * static_cast<double>(xb[col]);
         }
         // The last panel reloads itself: the loop body has no branch around
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevSta
                 const int c = (hp[2 + j / 4] >> (8 * (j % 4))) & 0xff;
                 if (lane < c) {
                     const E e = pv.ent[off + lane];
-                    const int r = min(static_cast<int>(e.x >> kPanelColBits), kPanelMaxRows - 1);
+                    const int r = min(static_cast<int>(e.x >> kPanelColBits), kRows - 1);
                     const int col = min(static_cast<int>(e.x & ((1u << kPanelColBits) - 1u)), P - 1);
                     acc[r] = acc[r] + static_cast<double>(entry_value(e)) * static_cast<double>(xb[col]);
                 }
@@ -301,30 +302,49 @@ bool build_workgroup(const std::vector<int> &rp, const std::vector<int> &ci, con
     return true;
 }
 
-template <class V, int NT, int KM>
+template <class V, int NT, int KM, int KB>
 void launch_kernel(const PanelLayout &L, const DevState *state, int select, const void *x0, const void *x1,
                    double *out, hipStream_t st) {
     PanelView<V> v{L.meta, L.hdr, static_cast<const typename PanelEntry<V>::E *>(L.ent), L.rows,
                    (L.cols * static_cast<int>(sizeof(V)) + 15) / 16};
-    k_panel_spmv<V, NT, KM><<<L.nwg, NT, 0, st>>>(v, state, select, static_cast<const V *>(x0),
-                                                  static_cast<const V *>(x1), out);
+    k_panel_spmv<V, NT, KM, KB><<<L.nwg, NT, 0, st>>>(v, state, select, static_cast<const V *>(x0),
+                                                      static_cast<const V *>(x1), out);
 }
 
-template <class V, int NT>
+template <class V, int NT, int KB>
 void launch_steps(const PanelLayout &L, const DevState *state, int select, const void *x0, const void *x1,
                   double *out, hipStream_t st) {
-    if (L.fast_steps <= 4) launch_kernel<V, NT, 4>(L, state, select, x0, x1, out, st);
-    else if (L.fast_steps <= 8) launch_kernel<V, NT, 8>(L, state, select, x0, x1, out, st);
-    else if (L.fast_steps <= 12) launch_kernel<V, NT, 12>(L, state, select, x0, x1, out, st);
-    else launch_kernel<V, NT, 16>(L, state, select, x0, x1, out, st);
+    if (L.fast_steps <= 4) launch_kernel<V, NT, 4, KB>(L, state, select, x0, x1, out, st);
+    else if (L.fast_steps <= 8) launch_kernel<V, NT, 8, KB>(L, state, select, x0, x1, out, st);
+    else if (L.fast_steps <= 12) launch_kernel<V, NT, 12, KB>(L, state, select, x0, x1, out, st);
+    else launch_kernel<V, NT, 16, KB>(L, state, select, x0, x1, out, st);
 }
 
-}  // namespace
+// The instantiated shapes: 48 KB panels with 256 or 512 lanes, 24 and 16 KB
+// panels with 512.
+template <class V>
+void launch_shape(const PanelLayout &L, const DevState *state, int select, const void *x0, const void *x1,
+                  double *out, hipStream_t st) {
+    if (L.panel_kb == 48) {
+        if (L.threads == 256) launch_steps<V, 256, 48>(L, state, select, x0, x1, out, st);
+        else launch_steps<V, 512, 48>(L, state, select, x0, x1, out, st);
+    } else if (L.panel_kb == 24) {
+        launch_steps<V, 512, 24>(L, state, select, x0, x1, out, st);
+    } else {
+        launch_steps<V, 512, 16>(L, state, select, x0, x1, out, st);
+    }
+}
 
-int panel_auto_groups(long long rows, long long cols, int value_bytes) {
-    // Staging: every workgroup (~256) loads its group's slice of x through L2
-    // (weighted 0.1 against HBM bytes); partial sums: G x rows FP64 written
-    // and read back.  Measured optimum on the 400k x 1M LP: 4 for A x, 2 for A' y.
+// Partial-sum bytes (stored and read back) plus half the staged bytes (the
+// staging is served by L2), per launch.
+double shape_traffic(long long rows, long long cols, int value_bytes, int G) {
+    return 16.0 * G * static_cast<double>(rows) + 0.5 * 256.0 * static_cast<double>(cols) * value_bytes / G;
+}
+
+// Staging alone weighted 0.1: the group count of 48 KB panels when no
+// shape qualifies (the measured optimum of 48 KB panels on the 400k x 1M
+// LP: 4 for A x, 2 for A' y).
+int groups_48kb(long long rows, long long cols, int value_bytes) {
     int best = 1;
     double best_cost = 0.0;
     for (int G : {1, 2, 4, 8}) {
@@ -337,14 +357,52 @@ int panel_auto_groups(long long rows, long long cols, int value_bytes) {
     return best;
 }
 
+}  // namespace
+
+void panel_auto_shape(long long rows, long long cols, long long nnz, int value_bytes, int threads, int *groups,
+                      int *panel_kb) {
+    const int fixed_g = *groups, fixed_kb = *panel_kb;
+    int best_g = 0, best_kb = 0;
+    double best_cost = 0.0;
+    for (int G : {1, 2, 4, 8}) {
+        if (fixed_g > 0 && G != fixed_g) continue;
+        const long long R = (rows * G + 255) / 256;
+        for (int kb : kPanelKbs) {
+            if ((fixed_kb > 0 && kb != fixed_kb) || (threads != 512 && kb != 48)) continue;
+            const long long P = kb * 1024ll / value_bytes;
+            const long long panels = (cols / G + P - 1) / P;
+            if (R > panel_max_rows(kb) || panels > kPanelMaxPanels) continue;
+            // The largest panel that holds the rows; skip G if it leaves the
+            // lanes too little work per panel.
+            if (static_cast<double>(nnz) < 3.0 * 256.0 * threads * static_cast<double>(panels)) break;
+            const double cost = shape_traffic(rows, cols, value_bytes, G);
+            if (best_g == 0 || cost < best_cost) {
+                best_g = G;
+                best_kb = kb;
+                best_cost = cost;
+            }
+            break;
+        }
+    }
+    if (best_g == 0) {
+        best_g = fixed_g > 0 ? fixed_g : groups_48kb(rows, cols, value_bytes);
+        best_kb = fixed_kb > 0 ? fixed_kb : 48;
+    }
+    *groups = best_g;
+    *panel_kb = best_kb;
+}
+
 int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_bytes, int rows, int cols, int nnz,
-                int groups, int threads, PanelLayout &out, hipStream_t st) {
+                int groups, int panel_kb, int threads, PanelLayout &out, hipStream_t st) {
     out.release();
     VK_REQUIRE(value_bytes == 8 || value_bytes == 4, "panel layout: values must be FP64 or FP32");
     VK_REQUIRE(groups >= 1 && groups <= 8 && (groups & (groups - 1)) == 0, "panel layout: groups must be 1, 2, 4 or 8");
-    VK_REQUIRE(threads == 256 || threads == 512, "panel layout: 256 or 512 threads per workgroup");
+    VK_REQUIRE(panel_kb_valid(panel_kb), "panel layout: panels must be 48, 24 or 16 KB");
+    VK_REQUIRE(threads == 512 || (threads == 256 && panel_kb == 48),
+               "panel layout: 512 lanes per workgroup, or 256 with 48 KB panels");
     VK_REQUIRE(rows > 0 && cols > 0, "panel layout: empty operand");
-    const int P = value_bytes == 8 ? panel_width<double>() : panel_width<float>();
+    const int P = value_bytes == 8 ? panel_width<double>(panel_kb) : panel_width<float>(panel_kb);
+    const int max_rows = panel_max_rows(panel_kb);
     std::vector<int> rp(static_cast<std::size_t>(rows) + 1), ci(static_cast<std::size_t>(std::max(nnz, 1)));
     VK_HIP_CHECK(hipMemcpyAsync(rp.data(), rp_d, sizeof(int) * (rows + 1ll), hipMemcpyDeviceToHost, st));
     if (nnz) VK_HIP_CHECK(hipMemcpyAsync(ci.data(), ci_d, sizeof(int) * static_cast<std::size_t>(nnz), hipMemcpyDeviceToHost, st));
@@ -375,7 +433,7 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
         // workgroups added) until no range exceeds the LDS accumulators.
         // (More than 256 workgroups in all run in two rounds: only add them
         // when the rows cannot be balanced under the accumulator limit.)
-        int nw = std::max(base_nw, (rows + kPanelMaxRows - 1) / kPanelMaxRows);
+        int nw = std::max(base_nw, (rows + max_rows - 1) / max_rows);
         std::vector<int> bounds;
         for (bool ok = false; !ok;) {
             for (long long lambda : {2ll, 8ll, 32ll, 1ll << 30}) {
@@ -391,7 +449,7 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
                 }
                 int widest = 0;
                 for (int i = 0; i < nw; ++i) widest = std::max(widest, bounds[i + 1] - bounds[i]);
-                if (widest <= kPanelMaxRows) {
+                if (widest <= max_rows) {
                     ok = true;
                     break;
                 }
@@ -515,6 +573,7 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
     const std::size_t esize = value_bytes == 8 ? sizeof(panel_u32x3) : sizeof(panel_u32x2);
     PanelLayout L;
     L.groups = groups;
+    L.panel_kb = panel_kb;
     L.threads = threads;
     L.steps = steps;
     L.fast_steps = fast_steps;
@@ -607,13 +666,8 @@ int panel_tune_steps(PanelLayout &L, int repeats, double *best_ms, hipStream_t s
 int panel_spmv_launch(const PanelLayout &L, const void *state, int select, const void *x0, const void *x1,
                       double *out, hipStream_t st) {
     const DevState *s = static_cast<const DevState *>(state);
-    if (L.value_bytes == 8) {
-        if (L.threads == 256) launch_steps<double, 256>(L, s, select, x0, x1, out, st);
-        else launch_steps<double, 512>(L, s, select, x0, x1, out, st);
-    } else {
-        if (L.threads == 256) launch_steps<float, 256>(L, s, select, x0, x1, out, st);
-        else launch_steps<float, 512>(L, s, select, x0, x1, out, st);
-    }
+    if (L.value_bytes == 8) launch_shape<double>(L, s, select, x0, x1, out, st);
+    else launch_shape<float>(L, s, select, x0, x1, out, st);
     return VK_OK;
 }
 
@@ -621,12 +675,16 @@ int panel_spmv_launch(const PanelLayout &L, const void *state, int select, const
 
 using namespace vk;
 
-extern "C" int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int precision, int groups, int threads,
-                                     const double *x, double *partial, int repeats, double *ms) {
+extern "C" int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int precision, int groups, int panel_kb,
+                                     int threads, const double *x, double *partial, int repeats, double *ms) {
     VK_REQUIRE(A && x && partial, "vk_hip_csr_panel_spmv: null argument");
     VK_REQUIRE(A->has_values, "vk_hip_csr_panel_spmv: matrix values were never set");
     VK_REQUIRE(precision == VK_PREC_FP64 || precision == VK_PREC_MIXED, "vk_hip_csr_panel_spmv: unknown precision");
     VK_REQUIRE(repeats >= 0, "vk_hip_csr_panel_spmv: negative repeat count");
+    VK_REQUIRE(groups == 1 || groups == 2 || groups == 4 || groups == 8,
+               "vk_hip_csr_panel_spmv: groups must be 1, 2, 4 or 8");
+    VK_REQUIRE(panel_kb == 0 || panel_kb_valid(panel_kb), "vk_hip_csr_panel_spmv: panels must be 0 (automatic), 48, 24 or 16 KB");
+    VK_REQUIRE(threads == 256 || threads == 512, "vk_hip_csr_panel_spmv: 256 or 512 lanes per workgroup");
     if (int rc = ensure_init()) return rc;
     if (int rc = bind_device()) return rc;
     hipStream_t st = default_stream();
@@ -640,11 +698,13 @@ extern "C" int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int pre
     const void *val = mixed ? static_cast<const void *>((transpose ? A->at32 : A->a32).val)
                             : static_cast<const void *>(op.val);
     const int vb = mixed ? 4 : 8;
+    if (panel_kb == 0) panel_auto_shape(rows, cols, static_cast<long long>(A->nnz), vb, threads, &groups, &panel_kb);
     PanelLayout L;
-    int rc = panel_build(op.row_ptr, op.col, val, vb, rows, cols, static_cast<int>(A->nnz), groups,
+    int rc = panel_build(op.row_ptr, op.col, val, vb, rows, cols, static_cast<int>(A->nnz), groups, panel_kb,
                          threads, L, st);
-    if (rc == VK_EINVAL && threads == 512)   // a lane needs more than 16 entries in a panel
-        rc = panel_build(op.row_ptr, op.col, val, vb, rows, cols, static_cast<int>(A->nnz), groups, 256, L, st);
+    if (rc == VK_EINVAL && threads == 512 && panel_kb == 48)   // a lane needs more than 24 entries in a panel
+        rc = panel_build(op.row_ptr, op.col, val, vb, rows, cols, static_cast<int>(A->nnz), groups, 48, 256, L,
+                         st);
     if (rc == VK_EINVAL) {
         set_error("vk_hip_csr_panel_spmv: the operand does not fit the panel layout");
         return rc;

@@ -12,6 +12,12 @@
 // panel gather from LDS.  The L1 traffic becomes coalesced lines of the
 // vector and of the matrix stream.
 //
+// Panels are 48, 24 or 16 KB; the rest of the 160 KB of LDS holds the FP64
+// row accumulators.  Every workgroup stages its group's whole slice of the
+// vector, so the staged bytes of a launch are (workgroups x slice): more
+// groups with more rows per workgroup (smaller panels, more accumulators)
+// stage less, at the price of more partial sums and more panels.
+//
 // Columns are split into G groups (workgroup b works on group b % G, so
 // each XCD's workgroups stage one group's slice and keep it in their L2);
 // each group writes FP64 partial sums of its rows, added in group order by
@@ -48,13 +54,19 @@ namespace vk {
 constexpr int kPanelStepCap = 24;        // entries per lane and panel a layout may need
 constexpr int kPanelMaxPanels = 128;     // panels per workgroup (headers held in 2 VGPR sets)
 constexpr int kPanelColBits = 14;        // column-in-panel field of an entry
-constexpr int kPanelBytes = 48 * 1024;   // LDS per staged panel (two are staged)
+constexpr int kPanelLds = 160 * 1024;    // LDS of a workgroup (one per CU)
 
+// Panel sizes a layout may use (KB of LDS per staged panel; two are staged).
+constexpr int kPanelKbs[3] = {48, 24, 16};
+
+constexpr bool panel_kb_valid(int kb) { return kb == 48 || kb == 24 || kb == 16; }
+
+// Entries of the gathered vector per panel.
 template <class V>
-constexpr int panel_width() { return kPanelBytes / static_cast<int>(sizeof(V)); }
+constexpr int panel_width(int kb) { return kb * 1024 / static_cast<int>(sizeof(V)); }
 
 // FP64 accumulators a workgroup can hold next to its two staged panels.
-constexpr int kPanelMaxRows = (160 * 1024 - 2 * kPanelBytes) / 8 - 64;
+constexpr int panel_max_rows(int kb) { return (kPanelLds - 2 * kb * 1024) / 8 - 64; }
 
 typedef unsigned panel_u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned panel_u32x3 __attribute__((ext_vector_type(3)));
@@ -102,7 +114,8 @@ struct PanelView {
 // The device layout of one operand in one value type.
 struct PanelLayout {
     int groups = 0;          // G (0: not built)
-    int threads = 0;         // lanes per workgroup (256 or 512)
+    int panel_kb = 0;        // KB per staged panel (48, 24 or 16)
+    int threads = 0;         // lanes per workgroup (256 or 512; 256 only with 48 KB panels)
     int steps = 0;           // most entries any lane has in one panel
     int fast_steps = 0;      // steps the kernel unrolls (4, 8, 12 or 16); a chunk with
                              // more runs the rest in a loop
@@ -118,11 +131,12 @@ struct PanelLayout {
 
 // Builds the layout of the CSR operand (device pattern rp, ci; rows x cols;
 // values val of `value_bytes` = 8 (double) or 4 (float)) with `groups`
-// column groups and `threads` lanes per workgroup.  Returns VK_OK, VK_EINVAL
-// (the operand does not fit the layout: a row with too many entries in one
-// panel, or too many panels; nothing is allocated), or an error code.
+// column groups, panels of `panel_kb` KB and `threads` lanes per workgroup.
+// Returns VK_OK, VK_EINVAL (the operand does not fit the layout: a row with
+// too many entries in one panel, or too many panels; nothing is allocated),
+// or an error code.
 int panel_build(const int *rp, const int *ci, const void *val, int value_bytes, int rows, int cols, int nnz,
-                int groups, int threads, PanelLayout &out, hipStream_t st);
+                int groups, int panel_kb, int threads, PanelLayout &out, hipStream_t st);
 
 // Times `repeats` launches of L (on a zero vector) for each unroll depth
 // among 4, 8, 12, 16 not below L's automatic choice by more than one step,
@@ -130,10 +144,18 @@ int panel_build(const int *rp, const int *ci, const void *val, int value_bytes, 
 // The depth never changes results.
 int panel_tune_steps(PanelLayout &L, int repeats, double *best_ms, hipStream_t st);
 
-// Automatic group count for an operand (rows x cols, gathered entries of
-// value_bytes): the smallest of 1, 2, 4, 8 that minimises the modelled
-// staging + partial-sum traffic.
-int panel_auto_groups(long long rows, long long cols, int value_bytes);
+// Automatic shape of an operand (rows x cols, nnz entries, gathered values
+// of value_bytes, `threads` lanes): the group count and the panel size,
+// each unless given (*groups > 0, *panel_kb > 0).  For each group count G
+// the largest panel size whose accumulators hold a 256-workgroup row range
+// (rows G / 256) is taken; shapes that leave a lane fewer than 3 entries
+// per panel on average are skipped; among the rest the smallest modelled
+// traffic wins: 16 G rows bytes of partial sums (stored and read back) plus
+// half the staged bytes 256 cols value_bytes / G (served by L2).  When no
+// shape qualifies: 48 KB panels and the group count of the staging model
+// with staged bytes weighted 0.1.
+void panel_auto_shape(long long rows, long long cols, long long nnz, int value_bytes, int threads, int *groups,
+                      int *panel_kb);
 
 // Launches y-partials = A x on `st` (FP64 partial sums, out[g * rows + r]).
 // Gathered vector: x1 when select != 0 and the device state's (k + 1) is

@@ -935,51 +935,72 @@ int build_blocked(vk_hip_pdhg *s) {
 
 
 // Builds the panel layouts of A (A x) and A' (A' y) in the solver's storage
-// precision.  Lanes per workgroup: the caller's, or 512 (measured on the
-// 400k x 1M LP: 512 is as fast or faster for both operands in both
-// precisions, by up to 22% for FP32 A x).  With panel_tune the tuner builds
-// both lane counts, times each over the unroll depths around the automatic
-// one on the actual matrix, and keeps the fastest; none of these choices
-// changes a result bit.  VK_EINVAL: the matrix does not fit the layout
-// (nothing is kept).
+// precision.  Shape: the caller's group counts / panel size / lane count
+// where given, else panel_auto_shape with 512 lanes (measured on the 400k x
+// 1M LP with 48 KB panels: 512 is as fast or faster than 256 for both
+// operands in both precisions, by up to 22% for FP32 A x).  With panel_tune
+// the tuner builds the automatic shape and, with its group count, 48 KB
+// panels with 512 lanes and with 256 lanes, times each over the unroll
+// depths around its automatic one on the actual matrix, and keeps the
+// fastest; none of these choices changes a result bit.  VK_EINVAL: the
+// matrix does not fit the layout (nothing is kept).
 int build_panels(vk_hip_pdhg *s) {
     const vk_hip_csr *A = s->A;
     const int vb = s->mixed ? 4 : 8;
     const int nnz = static_cast<int>(A->nnz);
-    const int ga = s->params.panel_groups > 0 ? s->params.panel_groups : panel_auto_groups(s->m, s->n, vb);
-    const int gt = s->params.panel_groups_t > 0 ? s->params.panel_groups_t : panel_auto_groups(s->n, s->m, vb);
     const int threads = s->params.panel_threads > 0 ? s->params.panel_threads : 512;
+    struct Shape {
+        int groups, kb, threads;
+    };
+    auto shape = [&](long long rows, long long cols, int groups, int kb, int th) {
+        panel_auto_shape(rows, cols, nnz, vb, th, &groups, &kb);
+        return Shape{groups, kb, th};
+    };
     auto build = [&](const int *rp, const int *ci, const void *val, int rows, int cols, int groups,
                      PanelLayout &out) -> int {
-        if (!s->params.panel_tune || s->params.panel_threads > 0)
-            return panel_build(rp, ci, val, vb, rows, cols, nnz, groups, threads, out, s->stream);
+        const Shape automatic = shape(rows, cols, groups, s->params.panel_kb, threads);
+        // Candidates: the automatic shape, then (unless the caller fixed the
+        // panel size or lane count) 48 KB panels with 512 and 256 lanes, all
+        // with the automatic shape's group count (the groups set the order
+        // of the partial sums; panel size and lanes change no bit): the
+        // first that fits is used, or with panel_tune the fastest.
+        std::vector<Shape> cands{automatic};
+        if (s->params.panel_threads == 0 && s->params.panel_kb == 0)
+            for (int th : {512, 256}) {
+                const Shape c{automatic.groups, 48, th};
+                if (c.groups != automatic.groups || c.kb != automatic.kb || c.threads != automatic.threads)
+                    cands.push_back(c);
+            }
         PanelLayout best;
         double best_ms = 1e300;
-        for (int th : {256, 512}) {
+        for (const Shape &c : cands) {
             PanelLayout L;
-            int rc = panel_build(rp, ci, val, vb, rows, cols, nnz, groups, th, L, s->stream);
-            double ms = 1e300;
-            if (rc == VK_OK) rc = panel_tune_steps(L, 5, &ms, s->stream);
+            int rc = panel_build(rp, ci, val, vb, rows, cols, nnz, c.groups, c.kb, c.threads, L, s->stream);
+            if (rc == VK_EINVAL) continue;   // this shape does not fit; another may
+            double ms = 0.0;
+            if (rc == VK_OK && s->params.panel_tune) rc = panel_tune_steps(L, 5, &ms, s->stream);
             if (rc != VK_OK) {
                 L.release();
                 best.release();
                 return rc;
             }
-            if (ms < best_ms) {
+            if (!best.groups || ms < best_ms) {
                 best.release();
                 best = L;
                 best_ms = ms;
             } else {
                 L.release();
             }
+            if (!s->params.panel_tune) break;
         }
+        if (!best.groups) return VK_EINVAL;
         out = best;
         return VK_OK;
     };
     const void *va = s->mixed ? static_cast<const void *>(A->a32.val) : static_cast<const void *>(A->a.val);
     const void *vat = s->mixed ? static_cast<const void *>(A->at32.val) : static_cast<const void *>(A->at.val);
-    int rc = build(A->a.row_ptr, A->a.col, va, s->m, s->n, ga, s->pan_a);
-    if (!rc) rc = build(A->at.row_ptr, A->at.col, vat, s->n, s->m, gt, s->pan_at);
+    int rc = build(A->a.row_ptr, A->a.col, va, s->m, s->n, s->params.panel_groups, s->pan_a);
+    if (!rc) rc = build(A->at.row_ptr, A->at.col, vat, s->n, s->m, s->params.panel_groups_t, s->pan_at);
     if (rc) {
         s->pan_a.release();
         s->pan_at.release();
@@ -1514,6 +1535,10 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
                "vk_hip_pdhg_create: panel groups must be 0 (automatic), 1, 2, 4 or 8");
     VK_REQUIRE(params->panel_threads == 0 || params->panel_threads == 256 || params->panel_threads == 512,
                "vk_hip_pdhg_create: panel_threads must be 0 (automatic), 256 or 512");
+    VK_REQUIRE(params->panel_kb == 0 || panel_kb_valid(params->panel_kb),
+               "vk_hip_pdhg_create: panel_kb must be 0 (automatic), 48, 24 or 16");
+    VK_REQUIRE(!(params->panel_threads == 256 && params->panel_kb > 0 && params->panel_kb != 48),
+               "vk_hip_pdhg_create: 256 lanes per panel workgroup need 48 KB panels");
     VK_REQUIRE(params->panel_tune == 0 || params->panel_tune == 1, "vk_hip_pdhg_create: panel_tune must be 0 or 1");
     if (int rc = ensure_init()) return rc;
     auto *s = new (std::nothrow) vk_hip_pdhg();
@@ -1675,10 +1700,11 @@ int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *s, int *info) {
     VK_REQUIRE(s && info, "vk_hip_pdhg_panel_layout: null argument");
     const PanelLayout *L[2] = {&s->pan_a, &s->pan_at};
     for (int i = 0; i < 2; ++i) {
-        info[4 * i] = s->panels ? L[i]->groups : 0;
-        info[4 * i + 1] = s->panels ? L[i]->threads : 0;
-        info[4 * i + 2] = s->panels ? L[i]->steps : 0;
-        info[4 * i + 3] = s->panels ? L[i]->fast_steps : 0;
+        info[5 * i] = s->panels ? L[i]->groups : 0;
+        info[5 * i + 1] = s->panels ? L[i]->threads : 0;
+        info[5 * i + 2] = s->panels ? L[i]->steps : 0;
+        info[5 * i + 3] = s->panels ? L[i]->fast_steps : 0;
+        info[5 * i + 4] = s->panels ? L[i]->panel_kb : 0;
     }
     return VK_OK;
 }

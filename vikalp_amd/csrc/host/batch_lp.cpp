@@ -263,6 +263,7 @@ std::unique_ptr<TailJob> prepare_tail(const Model &model, const Prepared &pr, co
     p.panel_groups = options.spmv_panel_groups;
     p.panel_groups_t = options.spmv_panel_groups_t;
     p.panel_threads = options.spmv_panel_threads;
+    p.panel_kb = options.spmv_panel_kb;
     p.panel_tune = options.spmv_autotune ? 1 : 0;
     p.norm_b = st.nb[i];
     p.norm_c = st.nc[i];

@@ -49,6 +49,7 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.spmv_panel_groups = o.spmv_panel_groups;
     s.spmv_panel_groups_t = o.spmv_panel_groups_t;
     s.spmv_panel_threads = o.spmv_panel_threads;
+    s.spmv_panel_kb = o.spmv_panel_kb;
     return s;
 }
 
@@ -118,6 +119,7 @@ void vk_solver_options_default(vk_solver_options *o) {
     o->spmv_panel_groups = d.spmv_panel_groups;
     o->spmv_panel_groups_t = d.spmv_panel_groups_t;
     o->spmv_panel_threads = d.spmv_panel_threads;
+    o->spmv_panel_kb = d.spmv_panel_kb;
 }
 
 static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,
@@ -167,7 +169,7 @@ int vk_lp_profile(int64_t m, int64_t n, const int64_t *row_offsets, const int64_
         out[7] = p.dual_pass_ms;
         out[8] = p.primal_spmv_ms;
         out[9] = p.primal_pass_ms;
-        for (int i = 0; i < 8; ++i) out[10 + i] = p.panel[i];
+        for (int i = 0; i < 10; ++i) out[10 + i] = p.panel[i];
     } catch (const std::exception &e) {
         return -2;
     }

@@ -73,11 +73,13 @@ struct SolverOptions {
     // 0 = automatic (once m or n exceeds 2^18, if the matrix fits), 1 = never,
     // 2 = always.  Groups of columns of A x / A' y (0 = automatic; 1, 2, 4, 8)
     // set the partial-sum order; lanes per workgroup (0 = automatic, 256 or
-    // 512) never change results.
+    // 512) and KB per staged panel (0 = automatic, 48, 24 or 16; 256 lanes
+    // only with 48) never change results.
     int spmv_panels = 0;
     int spmv_panel_groups = 0;
     int spmv_panel_groups_t = 0;
     int spmv_panel_threads = 0;
+    int spmv_panel_kb = 0;
 };
 
 }  // namespace vikalp

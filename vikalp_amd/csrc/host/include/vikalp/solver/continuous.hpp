@@ -64,8 +64,9 @@ struct KernelProfile {
     int col_blocks = 1;   // layout the solver used
     int split_passes = 0;
     // Panel layout (zeros when not used): groups, lanes per workgroup, most
-    // entries per lane and panel and the unrolled entries, of A x then A' y.
-    int panel[8] = {};
+    // entries per lane and panel, the unrolled entries and KB per panel, of
+    // A x then A' y.
+    int panel[10] = {};
 };
 KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &options,
                                  Index warm_iterations, Index iterations);

@@ -53,6 +53,8 @@ bool valid_options(const SolverOptions &o) {
            o.spmv_panels >= 0 && o.spmv_panels <= 2 && valid_groups(o.spmv_panel_groups) &&
            valid_groups(o.spmv_panel_groups_t) &&
            (o.spmv_panel_threads == 0 || o.spmv_panel_threads == 256 || o.spmv_panel_threads == 512) &&
+           (o.spmv_panel_kb == 0 || o.spmv_panel_kb == 48 || o.spmv_panel_kb == 24 || o.spmv_panel_kb == 16) &&
+           !(o.spmv_panel_threads == 256 && o.spmv_panel_kb != 0 && o.spmv_panel_kb != 48) &&
            (o.pdhg_policy == PdhgPolicy::Reference || o.pdhg_policy == PdhgPolicy::Adaptive ||
             o.pdhg_policy == PdhgPolicy::Halpern);
 }
@@ -543,6 +545,7 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     p.panel_groups = options.spmv_panel_groups;
     p.panel_groups_t = options.spmv_panel_groups_t;
     p.panel_threads = options.spmv_panel_threads;
+    p.panel_kb = options.spmv_panel_kb;
     p.panel_tune = options.spmv_autotune ? 1 : 0;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);

@@ -331,6 +331,7 @@ void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, co
     p.panel_groups = options.spmv_panel_groups;
     p.panel_groups_t = options.spmv_panel_groups_t;
     p.panel_threads = options.spmv_panel_threads;
+    p.panel_kb = options.spmv_panel_kb;
     p.panel_tune = options.spmv_autotune ? 1 : 0;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);

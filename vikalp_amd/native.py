@@ -76,7 +76,7 @@ class _SolverOptions(C.Structure):
         ("restart_period", C.c_int64), ("spmv_col_blocks", C.c_int), ("spmv_autotune", C.c_int),
         ("spmv_split_passes", C.c_int), ("spmv_cache_budget_mb", C.c_int),
         ("spmv_panels", C.c_int), ("spmv_panel_groups", C.c_int), ("spmv_panel_groups_t", C.c_int),
-        ("spmv_panel_threads", C.c_int),
+        ("spmv_panel_threads", C.c_int), ("spmv_panel_kb", C.c_int),
     ]
 
 
@@ -104,6 +104,7 @@ class PdhgParams(C.Structure):
         ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
         ("cache_budget_mb", C.c_int), ("spmv_panels", C.c_int), ("panel_groups", C.c_int),
         ("panel_groups_t", C.c_int), ("panel_threads", C.c_int), ("panel_tune", C.c_int),
+        ("panel_kb", C.c_int),
     ]
 
 
@@ -160,7 +161,8 @@ HIP_SYMBOLS = {
     "vk_hip_pdhg_profile": (C.c_int, [_vp, C.c_int64, _dp]),
     "vk_hip_pdhg_layout": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vk_hip_pdhg_panel_layout": (C.c_int, [_vp, C.POINTER(C.c_int)]),
-    "vk_hip_csr_panel_spmv": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]),
+    "vk_hip_csr_panel_spmv": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
+                                        _dp]),
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
     "vk_hip_csr_tune_col_blocks": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -380,6 +382,7 @@ class Options:
     spmv_panel_groups: int = 0
     spmv_panel_groups_t: int = 0
     spmv_panel_threads: int = 0
+    spmv_panel_kb: int = 0
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
@@ -462,7 +465,7 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     ci = np.ascontiguousarray(lp.column_indices, dtype=np.int64)
     f = [np.ascontiguousarray(a, dtype=np.float64) for a in
          (lp.values, lp.objective, lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper)]
-    out = np.zeros(18)
+    out = np.zeros(20)
     copt = (options or Options())._c()
     rc = L.vk_lp_profile(lp.m, lp.n, _ptr(rp, _lp), _ptr(ci, _lp), *[_ptr(a) for a in f],
                          C.byref(copt), int(warm_iterations), int(iterations), _ptr(out))
@@ -471,7 +474,7 @@ def profile_kernels(lp: LP, options: Options | None = None, warm_iterations: int
     return {"dual_ms": out[0], "primal_ms": out[1], "iteration_ms": out[2], "step_size": out[3],
             "col_blocks": int(out[4]), "split_passes": int(out[5]), "dual_spmv_ms": out[6],
             "dual_pass_ms": out[7], "primal_spmv_ms": out[8], "primal_pass_ms": out[9],
-            "panels": [int(v) for v in out[10:18]]}
+            "panels": [int(v) for v in out[10:20]]}
 
 
 def _result(info, x, y):
@@ -669,16 +672,17 @@ class Device:
         return best.value, dict(zip((1, 2, 4, 8, 16), ms.tolist()))
 
     def panel_spmv(self, h, rows, cols, x, transpose=False, precision=PdhgPrecision.FP64, groups=1,
-                   threads=256, repeats=0):
+                   threads=256, repeats=0, panel_kb=48):
         """vk_hip_csr_panel_spmv: the (groups, rows) FP64 partial sums of A x
-        (A' x when transpose) on the panel layout, and the mean ms of
-        `repeats` timed launches (None when repeats == 0)."""
+        (A' x when transpose) on the panel layout (panel_kb 48, 24, 16, or 0
+        for the automatic size), and the mean ms of `repeats` timed launches
+        (None when repeats == 0)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         out_rows = cols if transpose else rows
         part = np.zeros((groups, out_rows))
         ms = C.c_double()
-        check(self.L.vk_hip_csr_panel_spmv(h, int(bool(transpose)), int(precision), int(groups), int(threads),
-                                           _ptr(x), _ptr(part), int(repeats), C.byref(ms)),
+        check(self.L.vk_hip_csr_panel_spmv(h, int(bool(transpose)), int(precision), int(groups), int(panel_kb),
+                                           int(threads), _ptr(x), _ptr(part), int(repeats), C.byref(ms)),
               "vk_hip_csr_panel_spmv")
         return part, (ms.value if repeats else None)
 
