@@ -22,7 +22,7 @@ void PanelLayout::release() {
     meta = nullptr;
     hdr = nullptr;
     ent = nullptr;
-    groups = panel_kb = threads = steps = fast_steps = nwg = rows = cols = value_bytes = 0;
+    groups = panel_kb = threads = steps = fast_steps = nwg = resident = rows = cols = value_bytes = 0;
     ent_bytes = 0;
 }
 
@@ -40,11 +40,12 @@ __device__ __forceinline__ float entry_value(const panel_u32x2 &e) { return __ui
 // to the row accumulators, panel p+1's slice is written to the other LDS
 // buffer and a barrier ends the panel.  Entries a lane does not own (lane
 // >= the step's count) are loaded from a clamped, valid address and never
-// stored, so every load is unconditional and in bounds.
-template <class V, int NT, int KM, int KB>
-__global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevState *__restrict__ state, int select,
-                                                   const V *__restrict__ x0, const V *__restrict__ x1,
-                                                   double *__restrict__ out) {
+// stored, so every load is unconditional and in bounds.  KEEP: the entries
+// are loaded with cache-allocating loads (they stay in the MALL from one
+// iteration to the next), else streamed.
+template <class V, int NT, int KM, int KB, bool KEEP>
+__device__ __forceinline__ void panel_rows(const PanelView<V> &pv, const V *__restrict__ x, double *__restrict__ out,
+                                           V (*xbuf)[panel_width<V>(KB)], double *acc) {
 #pragma clang fp contract(off)
     using E = typename PanelEntry<V>::E;
     constexpr int P = panel_width<V>(KB);
@@ -54,14 +55,6 @@ __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevSta
     constexpr int kPer = P / kVec / NT;   // 16-byte loads per thread and panel
     static_assert(kPer * kVec * NT == P, "panel width must split evenly over the lanes");
     constexpr int kHW = 2 + (KM + 3) / 4;   // header words used
-    __shared__ __attribute__((aligned(16))) V xbuf[2][P];
-    __shared__ double acc[kRows];
-
-    const V *x = x0;
-    if (state) {
-        if (state->status != VK_PDHG_RUNNING) return;
-        if (select && ((state->k + 1) & 1)) x = x1;
-    }
     using CMeta = const __attribute__((address_space(4))) PanelMeta;
     CMeta *mp = (CMeta *)(pv.meta) + blockIdx.x;
     const int r0 = mp->r0, r1 = mp->r1, c0 = mp->c0, np = mp->np, hfirst = mp->hfirst, g = mp->g;
@@ -102,7 +95,9 @@ __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevSta
 #pragma unroll
         for (int j = 0; j < KM; ++j) {
             const int c = step_count(w, j);
-            ce[j] = __builtin_nontemporal_load(pv.ent + off + min(lane, max(c - 1, 0)));
+            const E *ep = pv.ent + off + min(lane, max(c - 1, 0));
+            if constexpr (KEEP) ce[j] = *ep;
+            else ce[j] = __builtin_nontemporal_load(ep);
             off += c;
         }
     };
@@ -167,6 +162,22 @@ __global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevSta
     }
     double *o = out + static_cast<long long>(g) * pv.rows + r0;
     for (int i = t; i < nr; i += NT) __builtin_nontemporal_store(acc[i], o + i);
+}
+
+// Workgroups [0, pv.resident) keep their entries in the MALL.
+template <class V, int NT, int KM, int KB>
+__global__ __launch_bounds__(NT) void k_panel_spmv(PanelView<V> pv, const DevState *__restrict__ state, int select,
+                                                   const V *__restrict__ x0, const V *__restrict__ x1,
+                                                   double *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) V xbuf[2][panel_width<V>(KB)];
+    __shared__ double acc[panel_max_rows(KB)];
+    const V *x = x0;
+    if (state) {
+        if (state->status != VK_PDHG_RUNNING) return;
+        if (select && ((state->k + 1) & 1)) x = x1;
+    }
+    if (static_cast<int>(blockIdx.x) < pv.resident) panel_rows<V, NT, KM, KB, true>(pv, x, out, xbuf, acc);
+    else panel_rows<V, NT, KM, KB, false>(pv, x, out, xbuf, acc);
 }
 
 template <class V, class E>
@@ -306,7 +317,7 @@ template <class V, int NT, int KM, int KB>
 void launch_kernel(const PanelLayout &L, const DevState *state, int select, const void *x0, const void *x1,
                    double *out, hipStream_t st) {
     PanelView<V> v{L.meta, L.hdr, static_cast<const typename PanelEntry<V>::E *>(L.ent), L.rows,
-                   (L.cols * static_cast<int>(sizeof(V)) + 15) / 16};
+                   (L.cols * static_cast<int>(sizeof(V)) + 15) / 16, L.resident};
     k_panel_spmv<V, NT, KM, KB><<<L.nwg, NT, 0, st>>>(v, state, select, static_cast<const V *>(x0),
                                                       static_cast<const V *>(x1), out);
 }

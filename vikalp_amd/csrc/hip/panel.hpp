@@ -109,6 +109,7 @@ struct PanelView {
     const typename PanelEntry<V>::E *ent;
     int rows;                                    // output rows (stride of the partial sums)
     int xvecs;                                   // 16-byte vectors readable from the gathered vector
+    int resident;                                // workgroups [0, resident) keep their entries cached
 };
 
 // The device layout of one operand in one value type.
@@ -120,6 +121,9 @@ struct PanelLayout {
     int fast_steps = 0;      // steps the kernel unrolls (4, 8, 12 or 16); a chunk with
                              // more runs the rest in a loop
     int nwg = 0;             // workgroups
+    int resident = 0;        // workgroups [0, resident) load their entries with
+                             // cache-allocating loads (MALL-resident), the rest
+                             // stream them (a load hint: results do not change)
     int rows = 0, cols = 0;  // output rows, gathered entries
     int value_bytes = 0;
     PanelMeta *meta = nullptr;

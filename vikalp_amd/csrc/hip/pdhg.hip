@@ -871,6 +871,13 @@ void plan_residency(vk_hip_pdhg *s) {
     const long long nnz = s->A->nnz;
     const long long per_nz = static_cast<long long>(sizeof(T)) + 4;
     s->resident_a = s->resident_at = 0;
+    if (s->panels) {
+        // Panel layouts: a prefix of each layout's workgroups (whose entry
+        // counts are balanced) keeps its entries; A x's first.
+        for (PanelLayout *L : {&s->pan_a, &s->pan_at})
+            L->resident = resident_prefix(L->nwg, static_cast<long long>(L->ent_bytes), left);
+        return;
+    }
     if (s->col_blocks > 1) {
         BlockedView<T> &B = blocked<T>(s);
         const long long entries = B.tile_first[B.nb];
