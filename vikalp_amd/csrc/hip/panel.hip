@@ -75,19 +75,21 @@ __device__ __forceinline__ void panel_rows(const PanelView<V> &pv, const V *__re
             hreg2[i] = h2[i];
         }
     }
-    auto hword = [&](int i, int p) {
-        return p < kWave ? __builtin_amdgcn_readlane(hreg[i], p) : __builtin_amdgcn_readlane(hreg2[i], p - kWave);
-    };
+    // (A select of the register set, then one readlane: no branch.)
+    auto hword = [&](int i, int p) { return __builtin_amdgcn_readlane(p < kWave ? hreg[i] : hreg2[i], p & (kWave - 1)); };
     auto step_count = [](const int (&w)[kHW], int j) { return (w[2 + j / 4] >> (8 * (j % 4))) & 0xff; };
 
-    const panel_u32x4 *xv = reinterpret_cast<const panel_u32x4 *>(x);
+    // Byte offsets of the 16-byte vectors fit 32 bits (panel_build checks),
+    // so each load is a uniform base plus one 32-bit lane offset.
+    const char *xb0 = reinterpret_cast<const char *>(x);
     const int xlast = pv.xvecs - 1;
     panel_u32x4 stg[kPer];
     E ce[KM];
     auto load_panel = [&](int p) {
         const int v0 = (c0 + p * P) / kVec;
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) stg[u] = xv[min(v0 + t + u * NT, xlast)];
+        for (int u = 0; u < kPer; ++u)
+            stg[u] = *reinterpret_cast<const panel_u32x4 *>(xb0 + (static_cast<unsigned>(min(v0 + t + u * NT, xlast)) << 4));
         int w[kHW];
 #pragma unroll
         for (int i = 0; i < kHW; ++i) w[i] = hword(i, p);
@@ -412,6 +414,8 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
     VK_REQUIRE(threads == 512 || (threads == 256 && panel_kb == 48),
                "panel layout: 512 lanes per workgroup, or 256 with 48 KB panels");
     VK_REQUIRE(rows > 0 && cols > 0, "panel layout: empty operand");
+    VK_REQUIRE(static_cast<long long>(cols) * value_bytes < (1ll << 32) - 64,
+               "panel layout: the gathered vector must stay below 4 GiB");
     const int P = value_bytes == 8 ? panel_width<double>(panel_kb) : panel_width<float>(panel_kb);
     const int max_rows = panel_max_rows(panel_kb);
     std::vector<int> rp(static_cast<std::size_t>(rows) + 1), ci(static_cast<std::size_t>(std::max(nnz, 1)));
