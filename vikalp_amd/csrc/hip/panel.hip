@@ -122,8 +122,13 @@ __device__ __forceinline__ void panel_rows(const PanelView<V> &pv, const V *__re
 #pragma unroll
         for (int j = 0; j < KM; ++j) {
             const unsigned e0 = ce[j].x;
-            const int col = min(static_cast<int>(e0 & ((1u << kPanelColBits) - 1u)), P - 1);
-            rows[j] = min(static_cast<int>(e0 >> kPanelColBits), kRows - 1);
+            // In range by construction: panel_build checks every entry's row
+            // against its workgroup's rows (below kRows) and its column
+            // against the panel, and a clamped load reads a checked entry
+            // (an empty step's reads the next chunk's first entry, or the
+            // zero padding).
+            const int col = static_cast<int>(e0 & ((1u << kPanelColBits) - 1u));
+            rows[j] = static_cast<int>(e0 >> kPanelColBits);
             prod[j] = static_cast<double>(entry_value(ce[j])) * static_cast<double>(xb[col]);
         }
         // The last panel reloads itself: the loop body has no branch around
