@@ -133,11 +133,31 @@ def fused_iteration_bytes(m, n, nnz, sv, halpern=True):
     return spmv + rows + cols
 
 
-# HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, scaled by a 256 MiB
-# device copy) measured with `bench.py --counters` under rocprofv3 --pmc;
-# profiles/pdhg_traffic_zipf_400k_x_1m.md holds the passes and commands.
-MEASURED_TRAFFIC = {"k_blocks<double>": 256.5e6, "k_dual_rows<double>": 45.3e6,
-                    "k_primal_own<double>": 386.5e6}
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by
+# what it reads for a 256 MiB device copy: 2.0 and 1.0) measured with
+# `bench.py --counters` under rocprofv3 --pmc on the 400k x 1M LP, keyed by
+# kernel and by the panel shape it ran with (groups, KB per panel):
+# profiles/pdhg_traffic_zipf_400k_x_1m.md holds the passes and commands.  A
+# kernel run with another shape reports no traffic (None).
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, per dispatch, scaled by a 256 MiB copy "
+                  "(profiles/pdhg_traffic_zipf_400k_x_1m.md); FETCH_SIZE includes MALL hits")
+MEASURED_TRAFFIC = {
+    ("k_panel_spmv[A x]<double>", 8, 24): 320.4e6, ("k_dual_rows<double>", 8, 24): 57.8e6,
+    ("k_panel_spmv[A'y]<double>", 2, 48): 317.0e6, ("k_primal_cols<double>", 2, 48): 112.4e6,
+    ("k_panel_spmv[A x]<float>", 8, 24): 172.9e6, ("k_dual_rows<float>", 8, 24): 41.8e6,
+    ("k_panel_spmv[A'y]<float>", 4, 16): 179.7e6, ("k_primal_cols<float>", 4, 16): 80.4e6,
+}
+
+
+def measured_traffic(kernel, panels):
+    """The measured HBM bytes of `kernel` (a kernel_table label) when the
+    profile's panel layout (vk_hip_pdhg_panel_layout's ten ints) has the shape
+    it was measured with, else None."""
+    if not panels or panels[0] <= 0:
+        return None
+    a_side = "A x" in kernel or kernel.startswith("k_dual_rows")
+    g, kb = (panels[0], panels[4]) if a_side else (panels[5], panels[9])
+    return MEASURED_TRAFFIC.get((kernel, g, kb))
 
 
 def counters_driver(V, args):
@@ -152,7 +172,8 @@ def counters_driver(V, args):
     lp = V.generate_lp(WORKLOAD["family"], args.m, args.n, 0, WORKLOAD["seed"])
     for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
         p = V.profile_kernels(lp, V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision,
-                                            relative_kkt_tolerance=WORKLOAD["tolerance"]), 2, 6)
+                                            relative_kkt_tolerance=WORKLOAD["tolerance"],
+                                            spmv_panels=args.spmv_panels), 2, 6)
         log(f"profiled {label}: col_blocks={p['col_blocks']} split={p['split_passes']} panels={p['panels']}")
 
 
@@ -474,6 +495,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel tracing: "
                          "rocprofv3 --kernel-trace does not follow graph replays with a forked stream)")
+    ap.add_argument("--spmv-panels", type=int, default=0, choices=(0, 1, 2),
+                    help="SpMV layout of the single-LP solves: 0 automatic (the panel layout on the "
+                         "benchmark LP), 1 the tile layouts, 2 panels always")
     ap.add_argument("--counters", action="store_true",
                     help="driver for the rocprofv3 --pmc passes: three 256 MiB device copies "
                          "(counter calibration), then 2 + 6 profiled iterations in FP64, then in "
@@ -500,13 +524,16 @@ def main():
                          relative_kkt_tolerance=WORKLOAD["tolerance"],
                          mixed_phase_tolerance=args.phase_tolerance,
                          iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit,
-                         use_graphs=not args.no_graphs)
+                         use_graphs=not args.no_graphs, spmv_panels=args.spmv_panels)
 
     # Kernel timing for the roofline (HIP events on the solver's stream).
     prof = V.profile_kernels(lp, options(V.PdhgPrecision.FP64), warm_iterations=30, iterations=60)
     prof32 = V.profile_kernels(lp, options(V.PdhgPrecision.Mixed), warm_iterations=30, iterations=60)
     k64 = kernel_table(prof, lp.m, lp.n, lp.nnz, 8, "double")
     k32 = kernel_table(prof32, lp.m, lp.n, lp.nnz, 4, "float")
+    for table, p in ((k64, prof), (k32, prof32)):
+        for name, row in table.items():
+            row["traffic"] = measured_traffic(name, p["panels"])
     for label, p, table in (("fp64", prof, k64), ("mixed", prof32, k32)):
         log(f"{label}: iteration {p['iteration_ms']:.4f} ms, col_blocks={p['col_blocks']} "
             f"split={p['split_passes']} panels={p['panels']}; " +
@@ -621,7 +648,8 @@ def main():
         "ms_per_iteration_fp64": prof["iteration_ms"],
         "ms_per_iteration_mixed": prof32["iteration_ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dominant),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, prof["panels"]),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "kernel": dominant, "bytes_per_launch": dom_bytes, "mean_ms": dom_ms},
         "layout": {"fp64": {"col_blocks": prof["col_blocks"], "split_passes": prof["split_passes"],
                             "panels": prof["panels"]},
