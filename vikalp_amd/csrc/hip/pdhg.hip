@@ -673,6 +673,9 @@ struct vk_hip_pdhg {
     bool panels = false;
     vk::PanelLayout pan_a, pan_at;
     double *tpart = nullptr;
+    // FP32 solvers: FP64 staging of n + m entries for uploads and the
+    // solution download, allocated once with the solver.
+    double *stage64 = nullptr;
 
     template <class T>
     T *alloc(size_t count) {
@@ -1079,7 +1082,14 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     }
     // Host vectors -> device (FP64 staging, converted on the device in mixed mode).
     double *stage = nullptr;
-    VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * std::max(m, n) + 8));
+    if constexpr (sizeof(T) != sizeof(double)) {
+        s->stage64 = s->alloc<double>(static_cast<size_t>(n) + m + 1);
+        if (!s->stage64) {
+            set_error("vk_hip_pdhg_create: device allocation failed");
+            return VK_ENOMEM;
+        }
+        stage = s->stage64;
+    }
     std::vector<double> unscale(static_cast<size_t>(m));
     for (int i = 0; i < m; ++i) unscale[i] = 1.0 / row_scale[i];
     std::vector<double> zeros_n(static_cast<size_t>(n), 0.0), zeros_m(static_cast<size_t>(m), 0.0);
@@ -1105,7 +1115,6 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     rc = rc ? rc : put(b.x[0], x0 ? x0 : zeros_n.data(), n);
     rc = rc ? rc : put(b.y, y0 ? y0 : zeros_m.data(), m);
     VK_HIP_CHECK(hipStreamSynchronize(st));
-    (void)hipFree(stage);
     if (rc) return rc;
 
     if (s->panels) {
@@ -1286,14 +1295,11 @@ int download_solution(vk_hip_pdhg *s, double *x, double *y) {
         if (x && s->n) VK_HIP_CHECK(hipMemcpyAsync(x, xs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
         if (y && s->m) VK_HIP_CHECK(hipMemcpyAsync(y, b.y, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->stream));
     } else {
-        double *stage = nullptr;
-        VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * (s->n + s->m) + 8));
+        double *stage = s->stage64;   // n + m entries, allocated with the solver
         if (s->n) k_f2d<<<egrid(s->n), kBlock, 0, s->stream>>>(s->n, xs, stage);
         if (s->m) k_f2d<<<egrid(s->m), kBlock, 0, s->stream>>>(s->m, b.y, stage + s->n);
         if (x && s->n) VK_HIP_CHECK(hipMemcpyAsync(x, stage, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
         if (y && s->m) VK_HIP_CHECK(hipMemcpyAsync(y, stage + s->n, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->stream));
-        VK_HIP_CHECK(hipStreamSynchronize(s->stream));
-        (void)hipFree(stage);
     }
     VK_HIP_CHECK(hipStreamSynchronize(s->stream));
     return VK_OK;
