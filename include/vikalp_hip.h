@@ -251,6 +251,10 @@ typedef struct {
                                       auto (per operand, with the group count
                                       when that is auto too), 48, 24 or 16
                                       (results do not depend on it)          */
+    int tile_geometry;             /* batched solver: row-tile geometry of the
+                                      instances' tiles, 0..3 as in
+                                      vk_hip_csr_set_tile_geometry (the
+                                      single-LP solver uses its matrix's)    */
 } vk_pdhg_params;
 
 typedef struct {
@@ -316,6 +320,27 @@ int vk_hip_pdhg_layout(const vk_hip_pdhg *solver, int *col_blocks, int *split_pa
  * info[5..9] the same for A' y. */
 int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *solver, int *info);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
+
+/* Row-tile geometry (new; the reference hands SpMV to cuSPARSE, which picks
+ * its own, cuda_ops.cu:473-528).  The tile kernels cut rows into tiles of
+ * whole rows; the geometry bounds a tile's nonzeros and rows and sets the
+ * row length above which a row has a tile of its own (reduced by the whole
+ * wave):
+ *   0  512 nonzeros, 64 rows, rows over 512 alone (the default)
+ *   1  256 nonzeros, 64 rows, rows over 256 alone
+ *   2  512 nonzeros, 32 rows, rows over 512 alone
+ *   3  512 nonzeros, 64 rows, rows over 64 alone
+ * It sets which rows share a tile and so the order of each row's sum: the
+ * results of SpMVs and solves on the matrix depend on it bitwise (each
+ * geometry is exact to round-off).  vk_hip_csr_set_tile_geometry re-cuts A's
+ * and A' tiles; call it before creating solvers on the matrix.
+ * vk_hip_csr_tune_tiles times one A x and one A' y (`repeats` times, in the
+ * storage precision of `precision`) with each geometry, writes the mean ms
+ * of each to ms[0..3] (may be NULL), leaves the fastest set on the matrix
+ * and returns its index in *best. */
+int vk_hip_csr_set_tile_geometry(vk_hip_csr *A, int geometry);
+int vk_hip_csr_tile_geometry(const vk_hip_csr *A, int *geometry);
+int vk_hip_csr_tune_tiles(vk_hip_csr *A, int precision, int repeats, int *best, double *ms);
 
 /* Layout tuner (new; the reference hands SpMV to cuSPARSE,
  * cuda_ops.cu:473-528, and has no layout choice): times y = A x in the

@@ -579,6 +579,8 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
                "vk_hip_batch_create: unknown policy");
     VK_REQUIRE(params->precision == VK_PREC_FP64 || params->precision == VK_PREC_MIXED,
                "vk_hip_batch_create: unknown precision");
+    VK_REQUIRE(params->tile_geometry >= 0 && params->tile_geometry < kTileGeometries,
+               "vk_hip_batch_create: tile_geometry must be 0..3");
     VK_REQUIRE(params->iteration_limit >= 0 && params->restart_period > 0,
                "vk_hip_batch_create: bad iteration limit or restart period");
     const int64_t m = row_starts[count], n = col_starts[count];
@@ -739,11 +741,12 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
         }
     }
     std::vector<int2> ta, tt;
+    const TileGeometry &geom = kTileGeometry[params->tile_geometry];
     h->inst_host.resize(static_cast<size_t>(count));
     if (src.shared) {
         // One tile table over the shared matrix, used by every instance.
-        append_tiles(rp.data(), 0, static_cast<int>(mat_rows), ta);
-        append_tiles(trp.data(), 0, static_cast<int>(mat_cols), tt);
+        append_tiles(rp.data(), 0, static_cast<int>(mat_rows), ta, geom);
+        append_tiles(trp.data(), 0, static_cast<int>(mat_cols), tt, geom);
         ta.pop_back();
         tt.pop_back();
     }
@@ -764,12 +767,12 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
         // range's start (ranges are contiguous) or the final sentinel.
         R.ta0 = static_cast<int>(ta.size());
         std::vector<int2> tmp;
-        append_tiles(rp.data(), R.r0, R.r1, tmp);
+        append_tiles(rp.data(), R.r0, R.r1, tmp, geom);
         R.ta1 = R.ta0 + static_cast<int>(tmp.size()) - 1;
         ta.insert(ta.end(), tmp.begin(), tmp.end() - 1);
         R.tt0 = static_cast<int>(tt.size());
         tmp.clear();
-        append_tiles(trp.data(), R.c0, R.c1, tmp);
+        append_tiles(trp.data(), R.c0, R.c1, tmp, geom);
         R.tt1 = R.tt0 + static_cast<int>(tmp.size()) - 1;
         tt.insert(tt.end(), tmp.begin(), tmp.end() - 1);
     }

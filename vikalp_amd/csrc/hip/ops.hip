@@ -66,19 +66,20 @@ int ensure_init() {
     return VK_OK;
 }
 
-void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles) {
+void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles, const TileGeometry &g) {
     int cur_nnz = 0, cur_rows = 0;
     tiles.push_back(make_int2(begin, rp[begin]));
     for (int r = begin; r < end; ++r) {
         const int len = rp[r + 1] - rp[r];
-        if (cur_rows > 0 && (cur_nnz + len > kTileNnz || cur_rows == kTileRows)) {
+        const bool alone = len > g.long_row;
+        if (cur_rows > 0 && (alone || cur_nnz + len > g.max_nnz || cur_rows == g.max_rows)) {
             tiles.push_back(make_int2(r, rp[r]));
             cur_nnz = 0;
             cur_rows = 0;
         }
         cur_nnz += len;
         ++cur_rows;
-        if (len > kTileNnz) {
+        if (alone) {
             tiles.push_back(make_int2(r + 1, rp[r + 1]));
             cur_nnz = 0;
             cur_rows = 0;
@@ -87,9 +88,9 @@ void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles) {
     if (cur_rows > 0) tiles.push_back(make_int2(end, rp[end]));
 }
 
-std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows) {
+std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows, const TileGeometry &g) {
     std::vector<int2> tiles;
-    append_tiles(row_ptr.data(), 0, rows, tiles);
+    append_tiles(row_ptr.data(), 0, rows, tiles, g);
     return tiles;
 }
 
@@ -532,6 +533,56 @@ int vk_hip_csr_set_values(vk_hip_csr *A, const double *values) {
     VK_HIP_CHECK(hipStreamSynchronize(s));
     A->has_values = true;
     A->scaled = false;
+    return VK_OK;
+}
+
+int vk_hip_csr_set_tile_geometry(vk_hip_csr *A, int geometry) {
+    VK_REQUIRE(A, "vk_hip_csr_set_tile_geometry: null matrix");
+    VK_REQUIRE(geometry >= 0 && geometry < kTileGeometries, "vk_hip_csr_set_tile_geometry: unknown geometry");
+    if (A->tile_geometry == geometry) return VK_OK;
+    hipStream_t s = default_stream();
+    const TileGeometry &g = kTileGeometry[geometry];
+    // Both operands' offsets, cut on the host, then the new tables replace
+    // the old ones (the FP32 copies share them).
+    std::vector<int> rp(static_cast<size_t>(A->rows) + 1), trp(static_cast<size_t>(A->cols) + 1);
+    VK_HIP_CHECK(hipMemcpyAsync(rp.data(), A->a.row_ptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost, s));
+    VK_HIP_CHECK(hipMemcpyAsync(trp.data(), A->at.row_ptr, sizeof(int) * trp.size(), hipMemcpyDeviceToHost, s));
+    VK_HIP_CHECK(hipStreamSynchronize(s));
+    const auto ta = build_tiles(rp, static_cast<int>(A->rows), g);
+    const auto tt = build_tiles(trp, static_cast<int>(A->cols), g);
+    int2 *da = nullptr, *dt = nullptr;
+    if (hipMalloc(&da, sizeof(int2) * ta.size()) != hipSuccess || hipMalloc(&dt, sizeof(int2) * tt.size()) != hipSuccess) {
+        if (da) (void)hipFree(da);
+        set_error("vk_hip_csr_set_tile_geometry: device allocation failed");
+        return VK_ENOMEM;
+    }
+    if (hipMemcpyAsync(da, ta.data(), sizeof(int2) * ta.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dt, tt.data(), sizeof(int2) * tt.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFree(da);
+        (void)hipFree(dt);
+        set_error("vk_hip_csr_set_tile_geometry: upload failed");
+        return VK_EDEVICE;
+    }
+    (void)hipFree(A->a.tiles);
+    (void)hipFree(A->at.tiles);
+    A->a.tiles = da;
+    A->a.ntiles = static_cast<int>(ta.size()) - 1;
+    A->at.tiles = dt;
+    A->at.ntiles = static_cast<int>(tt.size()) - 1;
+    if (A->a32.val) {
+        A->a32.tiles = A->a.tiles;
+        A->a32.ntiles = A->a.ntiles;
+        A->at32.tiles = A->at.tiles;
+        A->at32.ntiles = A->at.ntiles;
+    }
+    A->tile_geometry = geometry;
+    return VK_OK;
+}
+
+int vk_hip_csr_tile_geometry(const vk_hip_csr *A, int *geometry) {
+    VK_REQUIRE(A && geometry, "vk_hip_csr_tile_geometry: null argument");
+    *geometry = A->tile_geometry;
     return VK_OK;
 }
 

@@ -1735,6 +1735,80 @@ int vk_hip_pdhg_last_run_ms(const vk_hip_pdhg *s, double *ms) {
     return VK_OK;
 }
 
+int vk_hip_csr_tune_tiles(vk_hip_csr *A, int precision, int repeats, int *best, double *ms) {
+    VK_REQUIRE(A && best && repeats > 0, "vk_hip_csr_tune_tiles: bad arguments");
+    VK_REQUIRE(A->has_values, "vk_hip_csr_tune_tiles: matrix values were never set");
+    VK_REQUIRE(precision == VK_PREC_FP64 || precision == VK_PREC_MIXED, "vk_hip_csr_tune_tiles: unknown precision");
+    if (int rc = ensure_init()) return rc;
+    hipStream_t st = default_stream();
+    const bool f32 = precision == VK_PREC_MIXED;
+    if (f32)
+        if (int rc = csr_make_f32(A, st)) return rc;
+    const size_t m = static_cast<size_t>(A->rows), n = static_cast<size_t>(A->cols);
+    const size_t vb = f32 ? sizeof(float) : sizeof(double);
+    void *x = nullptr, *y = nullptr, *ax = nullptr, *aty = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = VK_OK;
+    auto check = [&](hipError_t e) {
+        if (e != hipSuccess && rc == VK_OK) {
+            set_error(std::string("vk_hip_csr_tune_tiles: ") + hipGetErrorString(e));
+            rc = e == hipErrorOutOfMemory ? VK_ENOMEM : VK_EDEVICE;
+        }
+    };
+    check(hipMalloc(&x, vb * n + 8));
+    check(hipMalloc(&y, vb * m + 8));
+    check(hipMalloc(&ax, vb * m + 8));
+    check(hipMalloc(&aty, vb * n + 8));
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    if (rc == VK_OK) check(hipMemsetAsync(x, 0, vb * n + 8, st));
+    if (rc == VK_OK) check(hipMemsetAsync(y, 0, vb * m + 8, st));
+    // One A x and one A' y per repeat, as an iteration runs them.
+    auto launch = [&] {
+        auto grid = [](int ntiles) { return std::max(1, std::min((ntiles + kWaves - 1) / kWaves, kMaxGrid)); };
+        if (f32) {
+            const CsrView<float> va = A->a32.view(), vt = A->at32.view();
+            if (va.ntiles > 0) k_spmv_store<float><<<grid(va.ntiles), kBlock, 0, st>>>(va, static_cast<float *>(x), static_cast<float *>(ax));
+            if (vt.ntiles > 0) k_spmv_store<float><<<grid(vt.ntiles), kBlock, 0, st>>>(vt, static_cast<float *>(y), static_cast<float *>(aty));
+        } else {
+            const CsrView<double> va = A->a.view(), vt = A->at.view();
+            if (va.ntiles > 0) k_spmv_store<double><<<grid(va.ntiles), kBlock, 0, st>>>(va, static_cast<double *>(x), static_cast<double *>(ax));
+            if (vt.ntiles > 0) k_spmv_store<double><<<grid(vt.ntiles), kBlock, 0, st>>>(vt, static_cast<double *>(y), static_cast<double *>(aty));
+        }
+    };
+    double best_ms = std::numeric_limits<double>::infinity();
+    int chosen = 0;
+    for (int g = 0; g < kTileGeometries && rc == VK_OK; ++g) {
+        if (int grc = vk_hip_csr_set_tile_geometry(A, g)) {
+            rc = grc;
+            break;
+        }
+        launch();   // warm-up
+        check(hipEventRecord(e0, st));
+        for (int r = 0; r < repeats; ++r) launch();
+        check(hipEventRecord(e1, st));
+        check(hipEventSynchronize(e1));
+        check(hipGetLastError());
+        float t = 0.0f;
+        check(hipEventElapsedTime(&t, e0, e1));
+        const double mean = static_cast<double>(t) / repeats;
+        if (ms) ms[g] = mean;
+        if (rc == VK_OK && mean < best_ms) {
+            best_ms = mean;
+            chosen = g;
+        }
+    }
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    if (ax) (void)hipFree(ax);
+    if (aty) (void)hipFree(aty);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *best = chosen;
+    return vk_hip_csr_set_tile_geometry(A, chosen);
+}
+
 int vk_hip_csr_tune_col_blocks(vk_hip_csr *A, int precision, int repeats, int *best, double *ms) {
     VK_REQUIRE(A && best && repeats > 0, "vk_hip_csr_tune_col_blocks: bad arguments");
     VK_REQUIRE(A->has_values, "vk_hip_csr_tune_col_blocks: matrix values were never set");

@@ -37,14 +37,32 @@ struct DeviceCsr {
     CsrView<V> view() const { return CsrView<V>{row_ptr, col, val, tiles, rows, ntiles, 0}; }
 };
 
-// Splits rows into tiles of whole rows with at most kTileNnz nonzeros and
-// kTileRows rows; a row longer than kTileNnz is a tile of its own.
-std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows);
+// How rows are cut into tiles: whole rows, at most max_nnz nonzeros and
+// max_rows rows per tile (max_nnz <= kTileNnz, max_rows <= kTileRows, what
+// the tile kernels hold), and a row with more than long_row nonzeros in a
+// tile of its own, reduced by the whole wave (long_row <= max_nnz).  The
+// geometry sets which rows share a tile and so the order in which a row's
+// products are added: results depend on it (bitwise), correctness does not.
+struct TileGeometry {
+    int max_nnz = kTileNnz;
+    int max_rows = kTileRows;
+    int long_row = kTileNnz;
+};
+
+// The geometries a caller or the tuner may pick (index 0 is the default):
+// 512 nonzeros x 64 rows; 256 x 64; 512 x 32; 512 x 64 with rows longer
+// than 64 nonzeros alone.
+constexpr int kTileGeometries = 4;
+constexpr TileGeometry kTileGeometry[kTileGeometries] = {
+    {kTileNnz, kTileRows, kTileNnz}, {256, kTileRows, 256}, {kTileNnz, 32, kTileNnz}, {kTileNnz, kTileRows, 64}};
+
+// Splits rows into tiles of whole rows by geometry g.
+std::vector<int2> build_tiles(const std::vector<int> &row_ptr, int rows, const TileGeometry &g = TileGeometry{});
 
 // Appends the tiles of rows [begin, end) of the CSR offsets `rp` to `tiles`
 // as [first, boundaries..., end] ({row, rp[row]} each): tile t spans entries
 // t and t+1, so the range's last entry is its end sentinel.
-void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles);
+void append_tiles(const int *rp, int begin, int end, std::vector<int2> &tiles, const TileGeometry &g = TileGeometry{});
 
 }  // namespace vk
 
@@ -64,6 +82,7 @@ struct vk_hip_csr {
     // factors, multiplied into them): col_passes rows of cols doubles.
     double *col_factors = nullptr;
     int col_passes = 0;
+    int tile_geometry = 0;            // index into kTileGeometry of a's and at's tiles
 };
 
 #define VK_HIP_CHECK(expr)                                                      \

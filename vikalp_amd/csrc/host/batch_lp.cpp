@@ -44,6 +44,8 @@ struct Stacked {
     std::vector<double> warm;                    // scaled start x of every instance, or empty
     vk_hip_csr *A = nullptr;
     bool shared = false;   // A is one matrix every instance runs on
+    int tile_geometry = 0;     // the instances' row tiles (vk_pdhg_params.tile_geometry)
+    std::string tiles_note;    // SolveResult::message note of a non-default geometry
     ~Stacked() { vk_hip_csr_destroy(A); }
 };
 
@@ -247,6 +249,7 @@ std::unique_ptr<TailJob> prepare_tail(const Model &model, const Prepared &pr, co
                                model.constraint_matrix.pattern.column_indices.data(), &job->A.a),
              "vk_hip_csr_create");
     check_vk(vk_hip_csr_set_values(job->A.a, model.constraint_matrix.values.data()), "vk_hip_csr_set_values");
+    check_vk(vk_hip_csr_set_tile_geometry(job->A.a, st.tile_geometry), "vk_hip_csr_set_tile_geometry");
     job->sc = device_scale(job->A.a, model, pr.bounds, options.pdhg_policy != PdhgPolicy::Reference);
     vk_pdhg_params &p = job->p;
     p.policy = options.pdhg_policy == PdhgPolicy::Adaptive ? VK_PDHG_POLICY_ADAPTIVE : VK_PDHG_POLICY_HALPERN;
@@ -350,6 +353,7 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
     p.relative_kkt_tolerance = rel_tol;
     p.restart_period = options.restart_period;
+    p.tile_geometry = st.tile_geometry;
     const std::vector<double> &weights = weights_in ? *weights_in : st.weights;
     DeviceBatch batch;
     const double *rl = m ? sc.rl.data() : nullptr, *ru = m ? sc.ru.data() : nullptr, *r = m ? sc.r.data() : nullptr;
@@ -499,6 +503,7 @@ void run_stacked(const std::vector<const Model *> &models, const std::vector<Pre
         pr.g.assign(all.g.begin() + col0, all.g.begin() + col0 + n);
         postsolve(model, prep[i].bounds, options, x, y, pr, options.relative_kkt_tolerance > 0.0, res);
         if (infeasible) report_infeasible(model, prep[i].bounds, y, res);
+        if (!st.tiles_note.empty()) res.message = res.message.empty() ? st.tiles_note : res.message + "; " + st.tiles_note;
         res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
         results[prep[i].index] = std::move(res);
     }
@@ -541,6 +546,10 @@ std::vector<SolveResult> solve_lp_batch(const std::vector<const Model *> &models
     if (prep.empty()) return results;
     Stacked st;
     stack(models, prep, options, st);
+    // The stacked matrix has every instance's rows: the geometry tuned on it
+    // is the one the instances' tiles are cut with.
+    st.tile_geometry = apply_tile_geometry(st.A, options);
+    st.tiles_note = tile_geometry_note(st.tile_geometry, options.spmv_tile_geometry < 0);
     run_stacked(models, prep, st, options, started, results);
     return results;
 }
@@ -600,6 +609,8 @@ std::vector<SolveResult> solve_lp_nodes(const Model &model, const std::vector<st
     if (prep.empty()) return results;
     Stacked st;
     stack_shared(model, prep, warm, options, st);
+    st.tile_geometry = apply_tile_geometry(st.A, options);
+    st.tiles_note = tile_geometry_note(st.tile_geometry, options.spmv_tile_geometry < 0);
     const std::vector<const Model *> models(count, &model);
     run_stacked(models, prep, st, options, started, results);
     return results;

@@ -80,6 +80,14 @@ struct SolverOptions {
     int spmv_panel_groups_t = 0;
     int spmv_panel_threads = 0;
     int spmv_panel_kb = 0;
+    // Row-tile geometry of the tile kernels (the fused path's matrix and the
+    // batched solver's instances): 0 = 512 nonzeros x 64 rows per tile, rows
+    // over 512 alone (the default); 1 = 256 x 64, over 256 alone; 2 = 512 x
+    // 32; 3 = 512 x 64, rows over 64 alone; -1 = time the four on the matrix
+    // at setup and use the fastest.  The geometry sets the order of each
+    // row's sum, so results depend on it bitwise; when it is not the default
+    // it is named in SolveResult::message.
+    int spmv_tile_geometry = 0;
 };
 
 }  // namespace vikalp

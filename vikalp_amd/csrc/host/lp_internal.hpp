@@ -59,6 +59,12 @@ Products device_products(const vk_hip_csr *A, const Model &model, const std::vec
 // device (vk_hip_lp_scale; bitwise equal): A's device values become the
 // scaled ones and the returned Scaled has no host values.
 Scaled device_scale(vk_hip_csr *A, const Model &model, const Bounds &bounds, bool equilibrated);
+// Applies options.spmv_tile_geometry to A (the default geometry leaves A as
+// it is; -1 times the geometries on A in the solve's storage precision and
+// keeps the fastest) and returns the geometry used.
+int apply_tile_geometry(vk_hip_csr *A, const SolverOptions &options);
+// "" for the default geometry, else the note SolveResult::message carries.
+std::string tile_geometry_note(int geometry, bool tuned);
 void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
                const std::vector<Scalar> &x, const std::vector<Scalar> &y, const Products &products,
                bool relative_mode, SolveResult &res);                      // pdhg.cpp:153-264

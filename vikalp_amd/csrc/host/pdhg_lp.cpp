@@ -55,6 +55,7 @@ bool valid_options(const SolverOptions &o) {
            (o.spmv_panel_threads == 0 || o.spmv_panel_threads == 256 || o.spmv_panel_threads == 512) &&
            (o.spmv_panel_kb == 0 || o.spmv_panel_kb == 48 || o.spmv_panel_kb == 24 || o.spmv_panel_kb == 16) &&
            !(o.spmv_panel_threads == 256 && o.spmv_panel_kb != 0 && o.spmv_panel_kb != 48) &&
+           o.spmv_tile_geometry >= -1 && o.spmv_tile_geometry < 4 &&
            (o.pdhg_policy == PdhgPolicy::Reference || o.pdhg_policy == PdhgPolicy::Adaptive ||
             o.pdhg_policy == PdhgPolicy::Halpern);
 }
@@ -367,6 +368,24 @@ Products device_products(const vk_hip_csr *A, const Model &model, const std::vec
                                            pr.g.data()),
              "vk_hip_csr_postsolve_products");
     return pr;
+}
+
+int apply_tile_geometry(vk_hip_csr *A, const SolverOptions &options) {
+    int g = options.spmv_tile_geometry;
+    if (g < 0) {
+        const int precision = options.pdhg_precision == PdhgPrecision::Mixed ? VK_PREC_MIXED : VK_PREC_FP64;
+        check_vk(vk_hip_csr_tune_tiles(A, precision, 10, &g, nullptr), "vk_hip_csr_tune_tiles");
+    } else {
+        check_vk(vk_hip_csr_set_tile_geometry(A, g), "vk_hip_csr_set_tile_geometry");
+    }
+    return g;
+}
+
+std::string tile_geometry_note(int geometry, bool tuned) {
+    static const char *const kNames[] = {"512 nonzeros x 64 rows", "256 nonzeros x 64 rows, rows over 256 alone",
+                                         "512 nonzeros x 32 rows", "512 nonzeros x 64 rows, rows over 64 alone"};
+    if (geometry == 0 && !tuned) return "";
+    return std::string("spmv tiles: ") + kNames[geometry] + (tuned ? " (tuned)" : "");
 }
 
 Scaled device_scale(vk_hip_csr *A, const Model &model, const Bounds &bounds, bool equilibrated) {
@@ -820,9 +839,12 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend, const Solver
     // The fused path scales on the device from the original values; the
     // op-by-op path keeps the reference's host scaling (pdhg.cpp:25).
     Scaled sc;
+    std::string tiles_note;
     if (handle) {
         backend.set_values(model.constraint_matrix.values, *matrix);
         sc = device_scale(handle, model, bounds, options.pdhg_policy != PdhgPolicy::Reference);
+        const int g = apply_tile_geometry(handle, options);
+        tiles_note = tile_geometry_note(g, options.spmv_tile_geometry < 0);
     } else {
         sc = scale(model, bounds);
         backend.set_values(sc.values, *matrix);
@@ -832,6 +854,7 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend, const Solver
     const std::vector<Scalar> *warm_ptr = warm_start.empty() ? nullptr : &warm;
     SolveResult res = handle ? solve_fused(model, bounds, sc, handle, options, warm_ptr, started)
                              : solve_ops(model, bounds, sc, backend, *matrix, options, warm_ptr, started);
+    if (!tiles_note.empty()) res.message = res.message.empty() ? tiles_note : res.message + "; " + tiles_note;
     res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
     return res;
 }

@@ -417,14 +417,20 @@ SolveResult solve_qp(const Model &model, ExecutionBackend &backend, const Solver
     res.solver = "PDQP";
     res.backend = "provided";
     std::vector<Scalar> xs, ys;
-    if (a_handle && q_handle && options.fused_iterations)
+    std::string tiles_note;
+    if (a_handle && q_handle && options.fused_iterations) {
+        // The matrix belongs to this solve: its tiles may be re-cut.
+        const int g = apply_tile_geometry(const_cast<vk_hip_csr *>(a_handle), options);
+        tiles_note = tile_geometry_note(g, options.spmv_tile_geometry < 0);
         iterate_fused(model, sc, a_handle, q_handle, max_step, options, warm_ptr, started, xs, ys, res);
-    else
+    } else {
         iterate_ops(model, sc, backend, *matrix, *quadratic, max_step, options, warm_ptr, started, xs, ys, res);
+    }
     std::vector<Scalar> x(xs.size()), y(ys.size());
     for (std::size_t j = 0; j < xs.size(); ++j) x[j] = xs[j] * sc.s[j];
     for (std::size_t i = 0; i < ys.size(); ++i) y[i] = ys[i] * sc.r[i];
     postsolve_qp(model, bounds, options, x, y, res);
+    if (!tiles_note.empty()) res.message = res.message.empty() ? tiles_note : res.message + "; " + tiles_note;
     res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
     return res;
 }

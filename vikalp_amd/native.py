@@ -76,7 +76,7 @@ class _SolverOptions(C.Structure):
         ("restart_period", C.c_int64), ("spmv_col_blocks", C.c_int), ("spmv_autotune", C.c_int),
         ("spmv_split_passes", C.c_int), ("spmv_cache_budget_mb", C.c_int),
         ("spmv_panels", C.c_int), ("spmv_panel_groups", C.c_int), ("spmv_panel_groups_t", C.c_int),
-        ("spmv_panel_threads", C.c_int), ("spmv_panel_kb", C.c_int),
+        ("spmv_panel_threads", C.c_int), ("spmv_panel_kb", C.c_int), ("spmv_tile_geometry", C.c_int),
     ]
 
 
@@ -104,7 +104,7 @@ class PdhgParams(C.Structure):
         ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
         ("cache_budget_mb", C.c_int), ("spmv_panels", C.c_int), ("panel_groups", C.c_int),
         ("panel_groups_t", C.c_int), ("panel_threads", C.c_int), ("panel_tune", C.c_int),
-        ("panel_kb", C.c_int),
+        ("panel_kb", C.c_int), ("tile_geometry", C.c_int),
     ]
 
 
@@ -163,6 +163,9 @@ HIP_SYMBOLS = {
     "vk_hip_pdhg_panel_layout": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "vk_hip_csr_panel_spmv": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
                                         _dp]),
+    "vk_hip_csr_set_tile_geometry": (C.c_int, [_vp, C.c_int]),
+    "vk_hip_csr_tile_geometry": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "vk_hip_csr_tune_tiles": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
     "vk_hip_csr_tune_col_blocks": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -383,6 +386,7 @@ class Options:
     spmv_panel_groups_t: int = 0
     spmv_panel_threads: int = 0
     spmv_panel_kb: int = 0
+    spmv_tile_geometry: int = 0
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
@@ -698,6 +702,24 @@ class Device:
                                      _ptr(out["var_lower"]), _ptr(out["var_upper"]), _ptr(out["row_lower"]),
                                      _ptr(out["row_upper"])), "vk_hip_lp_scale")
         return out
+
+    def set_tile_geometry(self, h, geometry: int):
+        """vk_hip_csr_set_tile_geometry: re-cut A's and A' row tiles (0..3)."""
+        check(self.L.vk_hip_csr_set_tile_geometry(h, int(geometry)), "vk_hip_csr_set_tile_geometry")
+
+    def tile_geometry(self, h) -> int:
+        g = C.c_int()
+        check(self.L.vk_hip_csr_tile_geometry(h, C.byref(g)), "vk_hip_csr_tile_geometry")
+        return g.value
+
+    def tune_tiles(self, h, precision=PdhgPrecision.FP64, repeats=10):
+        """vk_hip_csr_tune_tiles: (best geometry, mean ms of one A x + A' y
+        per geometry); the best is left set on the matrix."""
+        best = C.c_int()
+        ms = np.zeros(4)
+        check(self.L.vk_hip_csr_tune_tiles(h, int(precision), int(repeats), C.byref(best), _ptr(ms)),
+              "vk_hip_csr_tune_tiles")
+        return best.value, ms.tolist()
 
     def operator_bound(self, h) -> float:
         r = C.c_double()
