@@ -321,6 +321,15 @@ int vk_hip_pdhg_layout(const vk_hip_pdhg *solver, int *col_blocks, int *split_pa
 int vk_hip_pdhg_panel_layout(const vk_hip_pdhg *solver, int *info);
 int vk_hip_pdhg_destroy(vk_hip_pdhg *solver);
 
+/* Test hook (new; the reference's device errors become exceptions,
+ * cuda_ops.cu:35-46, and then NumericalFailure, Solver.cpp:81-86): after
+ * `after` more successful device allocations of this library, every
+ * allocation fails with hipErrorOutOfMemory until the hook is called again
+ * with after < 0 (never fail).  *made (may be NULL) receives the number of
+ * device allocations made since the previous call.  For tests of the
+ * allocation-failure paths. */
+int vk_hip_debug_fail_allocations(int64_t after, int64_t *made);
+
 /* Row-tile geometry (new; the reference hands SpMV to cuSPARSE, which picks
  * its own, cuda_ops.cu:473-528).  The tile kernels cut rows into tiles of
  * whole rows; the geometry bounds a tile's nonzeros and rows and sets the

@@ -425,9 +425,11 @@ struct vk_hip_batch {
     bool wide = false;   // the kWideBlock kernel fits (LDS) for this batch's precision
 
     template <class T>
+    // A failed allocation is listed as null, so a scan of `allocations`
+    // after a run of alloc calls finds it.
     T *alloc(size_t count_) {
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count_, 1)) != hipSuccess) return nullptr;
+        if (device_malloc(&p, sizeof(T) * std::max<size_t>(count_, 1)) != hipSuccess) p = nullptr;
         allocations.push_back(p);
         return static_cast<T *>(p);
     }
@@ -476,8 +478,13 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
         }
     // host_vectors: c, l, u, rl, ru, 1/r, s, x0, y0 (FP64)
     T *dst[9] = {c, l, u, rl, ru, runs, cs, b.x[0], b.y};
-    double *stage = nullptr;
-    VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * std::max<int64_t>(std::max(m, n), 1)));
+    // The FP64 staging buffer is one of the handle's allocations (freed with
+    // it), so an upload that fails part way leaks nothing.
+    double *stage = h->alloc<double>(static_cast<size_t>(std::max<int64_t>(std::max(m, n), 1)));
+    if (!stage) {
+        set_error("vk_hip_batch_create: device allocation failed");
+        return VK_ENOMEM;
+    }
     for (int v = 0; v < 9; ++v) {
         const auto &src = host_vectors[v];
         if (src.empty()) continue;
@@ -490,7 +497,6 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
         }
         VK_HIP_CHECK(hipStreamSynchronize(h->stream));
     }
-    (void)hipFree(stage);
     const int mr = static_cast<int>(h->mat_rows), mc = static_cast<int>(h->mat_cols);
     a.A = CsrView<T>{h->rp, h->ci, val_a, h->tiles_a, mr, h->ntiles_a, 0};
     a.AT = CsrView<T>{h->trp, h->tci, val_at, h->tiles_at, mc, h->ntiles_at, 0};
@@ -503,7 +509,10 @@ int setup_batch_bufs(vk_hip_batch *h, BatchArgs<T> &a, const T *val_a, const T *
     a.pw_cols = h->alloc<double>(n);
     a.count = h->count;
     a.power_iterations = 100;
-    if (!a.pw_rows || !a.pw_cols) return VK_ENOMEM;
+    if (!a.pw_rows || !a.pw_cols) {
+        set_error("vk_hip_batch_create: device allocation failed");
+        return VK_ENOMEM;
+    }
     // Dynamic LDS for the gathered vector: what the largest instance needs,
     // up to what half the device's LDS leaves beside the kernel's static LDS.
     int max_dim = 0;
@@ -981,7 +990,7 @@ int vk_hip_batch_download(vk_hip_batch *h, double *x, double *y) {
     std::vector<double> xs[2], ys(static_cast<size_t>(h->m));
     for (int p = 0; p < 2; ++p) xs[p].resize(static_cast<size_t>(h->n));
     double *stage = nullptr;
-    VK_HIP_CHECK(hipMalloc(&stage, sizeof(double) * std::max<int64_t>(std::max(h->m, h->n), 1)));
+    VK_HIP_CHECK(device_malloc(&stage, sizeof(double) * std::max<int64_t>(std::max(h->m, h->n), 1)));
     auto fetch = [&](const void *src, int64_t count, double *dst) -> int {
         if (count == 0) return VK_OK;
         if (h->mixed) {

@@ -112,7 +112,7 @@ struct Buffers {
     }
     void *get(size_t bytes) {
         void *p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
+        if (device_malloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
         blocks.push_back(p);
         return p;
     }

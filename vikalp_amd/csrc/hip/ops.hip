@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -41,6 +42,23 @@ void set_error(const std::string &message) { g_error = message; }
 
 hipStream_t default_stream() { return g_stream; }
 
+namespace {
+// Device allocations left before every allocation fails (-1: never), and the
+// allocations made since the hook was last called.
+std::atomic<long long> g_allocs_left{-1};
+std::atomic<long long> g_allocs_made{0};
+}  // namespace
+
+hipError_t device_alloc_bytes(void **p, std::size_t bytes) {
+    *p = nullptr;
+    for (long long left = g_allocs_left.load(); left >= 0;) {
+        if (left == 0) return hipErrorOutOfMemory;
+        if (g_allocs_left.compare_exchange_weak(left, left - 1)) break;
+    }
+    g_allocs_made.fetch_add(1);
+    return hipMalloc(p, bytes);
+}
+
 int bind_device() {
     // The current device is per host thread: a thread other than the one
     // that initialised the library starts on device 0.
@@ -60,8 +78,8 @@ int ensure_init() {
     VK_REQUIRE(count > 0, "no HIP device is visible");
     VK_REQUIRE(g_device >= 0 && g_device < count, "device index out of range");
     VK_HIP_CHECK(hipSetDevice(g_device));
-    VK_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    VK_HIP_CHECK(hipMalloc(&g_reduce, sizeof(double) * (kReduceBlocks + 8)));
+    if (!g_stream) VK_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    VK_HIP_CHECK(device_malloc(&g_reduce, sizeof(double) * (kReduceBlocks + 8)));
     g_ready = true;
     return VK_OK;
 }
@@ -226,10 +244,10 @@ int upload_csr(DeviceCsr<V> &d, const std::vector<int> &rp, const std::vector<in
     const auto tiles = build_tiles(rp, static_cast<int>(rp.size()) - 1);
     d.rows = static_cast<int>(rp.size()) - 1;
     d.ntiles = static_cast<int>(tiles.size()) - 1;
-    VK_HIP_CHECK(hipMalloc(&d.row_ptr, sizeof(int) * rp.size()));
-    VK_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * std::max<size_t>(1, ci.size())));
-    VK_HIP_CHECK(hipMalloc(&d.val, sizeof(V) * std::max<size_t>(1, ci.size())));
-    VK_HIP_CHECK(hipMalloc(&d.tiles, sizeof(int2) * tiles.size()));
+    VK_HIP_CHECK(device_malloc(&d.row_ptr, sizeof(int) * rp.size()));
+    VK_HIP_CHECK(device_malloc(&d.col, sizeof(int) * std::max<size_t>(1, ci.size())));
+    VK_HIP_CHECK(device_malloc(&d.val, sizeof(V) * std::max<size_t>(1, ci.size())));
+    VK_HIP_CHECK(device_malloc(&d.tiles, sizeof(int2) * tiles.size()));
     VK_HIP_CHECK(hipMemcpyAsync(d.row_ptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice, s));
     if (!ci.empty())
         VK_HIP_CHECK(hipMemcpyAsync(d.col, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice, s));
@@ -255,10 +273,10 @@ int build_transpose(vk_hip_csr *A, hipStream_t s) {
     DeviceCsr<double> &t = A->at;
     const int cols = static_cast<int>(A->cols), nnz = static_cast<int>(A->nnz);
     const size_t slots = static_cast<size_t>(std::max(1, nnz));
-    VK_HIP_CHECK(hipMalloc(&t.row_ptr, sizeof(int) * (static_cast<size_t>(cols) + 1)));
-    VK_HIP_CHECK(hipMalloc(&t.col, sizeof(int) * slots));
-    VK_HIP_CHECK(hipMalloc(&t.val, sizeof(double) * slots));
-    VK_HIP_CHECK(hipMalloc(&A->tperm, sizeof(int) * slots));
+    VK_HIP_CHECK(device_malloc(&t.row_ptr, sizeof(int) * (static_cast<size_t>(cols) + 1)));
+    VK_HIP_CHECK(device_malloc(&t.col, sizeof(int) * slots));
+    VK_HIP_CHECK(device_malloc(&t.val, sizeof(double) * slots));
+    VK_HIP_CHECK(device_malloc(&A->tperm, sizeof(int) * slots));
     std::vector<int> trp;
     if (int rc = transpose_on_device(static_cast<int>(A->rows), cols, nnz, A->a.row_ptr, A->a.col, t.row_ptr, t.col,
                                      A->tperm, trp, s))
@@ -266,7 +284,7 @@ int build_transpose(vk_hip_csr *A, hipStream_t s) {
     const auto tiles = build_tiles(trp, cols);
     t.rows = cols;
     t.ntiles = static_cast<int>(tiles.size()) - 1;
-    VK_HIP_CHECK(hipMalloc(&t.tiles, sizeof(int2) * tiles.size()));
+    VK_HIP_CHECK(device_malloc(&t.tiles, sizeof(int2) * tiles.size()));
     VK_HIP_CHECK(hipMemcpyAsync(t.tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
     VK_HIP_CHECK(hipMemsetAsync(t.val, 0, sizeof(double) * slots, s));
     VK_HIP_CHECK(hipStreamSynchronize(s));
@@ -286,17 +304,26 @@ int csr_make_f32(vk_hip_csr *A, hipStream_t s) {
         dst.tiles = src.tiles;
         dst.rows = src.rows;
         dst.ntiles = src.ntiles;
-        VK_HIP_CHECK(hipMalloc(&dst.val, sizeof(float) * nnz));
+        VK_HIP_CHECK(device_malloc(&dst.val, sizeof(float) * nnz));
         k_to_f32<<<elementwise_grid(nnz), kBlock, 0, s>>>(nnz, src.val, dst.val);
         VK_HIP_CHECK(hipGetLastError());
         return VK_OK;
     };
     int rc = shallow(A->a, A->a32);
-    if (rc) return rc;
-    rc = shallow(A->at, A->at32);
-    if (rc) return rc;
-    VK_HIP_CHECK(hipStreamSynchronize(s));
-    return VK_OK;
+    if (!rc) rc = shallow(A->at, A->at32);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("csr_make_f32: conversion failed");
+        rc = VK_EDEVICE;
+    }
+    if (rc) {
+        // Both copies or neither: a later call starts over.
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(A->a32.val);
+        (void)hipFree(A->at32.val);
+        A->a32 = DeviceCsr<float>{};
+        A->at32 = DeviceCsr<float>{};
+    }
+    return rc;
 }
 
 }  // namespace vk
@@ -341,7 +368,7 @@ int vk_hip_alloc(void **ptr, size_t bytes) {
     VK_REQUIRE(ptr, "vk_hip_alloc: null output");
     if (int rc = ensure_init()) return rc;
     *ptr = nullptr;
-    const hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 8));
+    const hipError_t e = device_malloc(ptr, std::max<size_t>(bytes, 8));
     if (e != hipSuccess) {
         set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
         return VK_ENOMEM;
@@ -536,6 +563,12 @@ int vk_hip_csr_set_values(vk_hip_csr *A, const double *values) {
     return VK_OK;
 }
 
+int vk_hip_debug_fail_allocations(int64_t after, int64_t *made) {
+    if (made) *made = g_allocs_made.exchange(0);
+    g_allocs_left.store(after < 0 ? -1 : after);
+    return VK_OK;
+}
+
 int vk_hip_csr_set_tile_geometry(vk_hip_csr *A, int geometry) {
     VK_REQUIRE(A, "vk_hip_csr_set_tile_geometry: null matrix");
     VK_REQUIRE(geometry >= 0 && geometry < kTileGeometries, "vk_hip_csr_set_tile_geometry: unknown geometry");
@@ -551,7 +584,7 @@ int vk_hip_csr_set_tile_geometry(vk_hip_csr *A, int geometry) {
     const auto ta = build_tiles(rp, static_cast<int>(A->rows), g);
     const auto tt = build_tiles(trp, static_cast<int>(A->cols), g);
     int2 *da = nullptr, *dt = nullptr;
-    if (hipMalloc(&da, sizeof(int2) * ta.size()) != hipSuccess || hipMalloc(&dt, sizeof(int2) * tt.size()) != hipSuccess) {
+    if (device_malloc(&da, sizeof(int2) * ta.size()) != hipSuccess || device_malloc(&dt, sizeof(int2) * tt.size()) != hipSuccess) {
         if (da) (void)hipFree(da);
         set_error("vk_hip_csr_set_tile_geometry: device allocation failed");
         return VK_ENOMEM;
@@ -645,8 +678,12 @@ int vk_hip_csr_norm2_estimate(const vk_hip_csr *A, int iterations, double *resul
     }
     hipStream_t s = default_stream();
     double *v = nullptr, *w = nullptr;
-    VK_HIP_CHECK(hipMalloc(&v, sizeof(double) * A->cols));
-    VK_HIP_CHECK(hipMalloc(&w, sizeof(double) * A->rows));
+    if (device_malloc(&v, sizeof(double) * A->cols) != hipSuccess ||
+        device_malloc(&w, sizeof(double) * A->rows) != hipSuccess) {
+        (void)hipFree(v);
+        set_error("vk_hip_csr_norm2_estimate: device allocation failed");
+        return VK_ENOMEM;
+    }
     // k_spmv forms alpha A x + beta y literally (cpu_ops.cpp:426), so with
     // beta = 0 a NaN or Inf left in fresh memory would still reach the sum.
     VK_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(double) * A->rows, s));

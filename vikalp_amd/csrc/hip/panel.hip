@@ -623,11 +623,11 @@ int panel_build(const int *rp_d, const int *ci_d, const void *val_d, int value_b
         return e == hipErrorOutOfMemory ? VK_ENOMEM : VK_EDEVICE;
     };
     hipError_t e;
-    if ((e = hipMalloc(&L.meta, sizeof(PanelMeta) * nwg)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&L.hdr, sizeof(PanelHeader) * std::max<std::size_t>(nh, 1))) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&L.ent, L.ent_bytes)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&idx_d, sizeof(unsigned) * idx.size())) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&perm_d, sizeof(int) * perm.size())) != hipSuccess) return fail(e);
+    if ((e = device_malloc(&L.meta, sizeof(PanelMeta) * nwg)) != hipSuccess) return fail(e);
+    if ((e = device_malloc(&L.hdr, sizeof(PanelHeader) * std::max<std::size_t>(nh, 1))) != hipSuccess) return fail(e);
+    if ((e = device_malloc(&L.ent, L.ent_bytes)) != hipSuccess) return fail(e);
+    if ((e = device_malloc(&idx_d, sizeof(unsigned) * idx.size())) != hipSuccess) return fail(e);
+    if ((e = device_malloc(&perm_d, sizeof(int) * perm.size())) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(L.meta, meta.data(), sizeof(PanelMeta) * nwg, hipMemcpyHostToDevice, st)) != hipSuccess ||
         (nh && (e = hipMemcpyAsync(L.hdr, hdr.data(), sizeof(PanelHeader) * nh, hipMemcpyHostToDevice, st)) != hipSuccess) ||
         (e = hipMemcpyAsync(idx_d, idx.data(), sizeof(unsigned) * idx.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
@@ -659,8 +659,8 @@ int panel_tune_steps(PanelLayout &L, int repeats, double *best_ms, hipStream_t s
             rc = e == hipErrorOutOfMemory ? VK_ENOMEM : VK_EDEVICE;
         }
     };
-    check(hipMalloc(&x, static_cast<std::size_t>(L.value_bytes) * (L.cols + 4)));
-    check(hipMalloc(&out, sizeof(double) * L.groups * static_cast<std::size_t>(L.rows)));
+    check(device_malloc(&x, static_cast<std::size_t>(L.value_bytes) * (L.cols + 4)));
+    check(device_malloc(&out, sizeof(double) * L.groups * static_cast<std::size_t>(L.rows)));
     check(hipEventCreate(&e0));
     check(hipEventCreate(&e1));
     if (rc == VK_OK) check(hipMemsetAsync(x, 0, static_cast<std::size_t>(L.value_bytes) * (L.cols + 4), st));
@@ -753,8 +753,8 @@ extern "C" int vk_hip_csr_panel_spmv(const vk_hip_csr *A, int transpose, int pre
         return code;
     };
     const std::size_t nout = static_cast<std::size_t>(L.groups) * rows;
-    if (hipMalloc(&xd, static_cast<std::size_t>(vb) * (cols + 4)) != hipSuccess ||
-        hipMalloc(&out, sizeof(double) * nout) != hipSuccess) {
+    if (device_malloc(&xd, static_cast<std::size_t>(vb) * (cols + 4)) != hipSuccess ||
+        device_malloc(&out, sizeof(double) * nout) != hipSuccess) {
         set_error("vk_hip_csr_panel_spmv: device allocation failed");
         return done(VK_ENOMEM);
     }

@@ -678,9 +678,11 @@ struct vk_hip_pdhg {
     double *stage64 = nullptr;
 
     template <class T>
+    // A failed allocation is listed as null, so a scan of `allocations`
+    // after a run of alloc calls finds it.
     T *alloc(size_t count) {
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) return nullptr;
+        if (device_malloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) p = nullptr;
         allocations.push_back(p);
         return static_cast<T *>(p);
     }
@@ -906,7 +908,7 @@ int build_blocked(vk_hip_pdhg *s) {
     s->blk_part = s->alloc<double>(static_cast<size_t>(nb) * m);
     int *perm = nullptr;
     if (!row_ptr || !col || !val || !s->blk_part ||
-        hipMalloc(&perm, sizeof(int) * std::max(1, nnz)) != hipSuccess) {
+        device_malloc(&perm, sizeof(int) * std::max(1, nnz)) != hipSuccess) {
         set_error("vk_hip_pdhg_create: column-blocked layout allocation failed");
         return VK_ENOMEM;
     }
@@ -1072,14 +1074,10 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     b.grid_rows = s->grid_rows;
     b.grid_cols = s->grid_cols;
     for (void *p : s->allocations)
-        if (!p) return VK_ENOMEM;
-    if (!b.x[0] || !b.x[1] || !b.y || !b.ax || !b.aty || !b.atyd || !c_d || !l_d || !u_d || !rl_d ||
-        !ru_d || !runs || !cs || !b.part_rows || !b.part_cols || !b.part_erows || !b.part_ecols ||
-        (adaptive && (!b.xsum || !b.ysum)) || ((adaptive || halpern) && (!b.xlast || !b.ylast || !b.ax_avg || !b.aty_avg)) ||
-        (halpern && (!b.xh || !b.yh || !b.x0h || !b.y0h || !b.ax0h || !b.aty0h))) {
-        set_error("vk_hip_pdhg_create: device allocation failed");
-        return VK_ENOMEM;
-    }
+        if (!p) {
+            set_error("vk_hip_pdhg_create: device allocation failed");
+            return VK_ENOMEM;
+        }
     // Host vectors -> device (FP64 staging, converted on the device in mixed mode).
     double *stage = nullptr;
     if constexpr (sizeof(T) != sizeof(double)) {
@@ -1388,7 +1386,7 @@ int tune_col_blocks(vk_hip_csr *A, const CsrView<T> &view, int repeats, int *bes
     };
     auto dalloc = [&](size_t bytes) -> void * {
         void *p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        if (device_malloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
         owned.push_back(p);
         return p;
     };
@@ -1590,7 +1588,7 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
         hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&s->ev_start) != hipSuccess || hipEventCreate(&s->ev_stop) != hipSuccess ||
-        hipMalloc(&s->st, sizeof(DevState)) != hipSuccess ||
+        device_malloc(&s->st, sizeof(DevState)) != hipSuccess ||
         hipHostMalloc(&s->host_state, sizeof(DevState)) != hipSuccess ||
         hipHostMalloc(&s->snap[0], sizeof(DevState)) != hipSuccess ||
         hipHostMalloc(&s->snap[1], sizeof(DevState)) != hipSuccess ||
@@ -1755,10 +1753,10 @@ int vk_hip_csr_tune_tiles(vk_hip_csr *A, int precision, int repeats, int *best, 
             rc = e == hipErrorOutOfMemory ? VK_ENOMEM : VK_EDEVICE;
         }
     };
-    check(hipMalloc(&x, vb * n + 8));
-    check(hipMalloc(&y, vb * m + 8));
-    check(hipMalloc(&ax, vb * m + 8));
-    check(hipMalloc(&aty, vb * n + 8));
+    check(device_malloc(&x, vb * n + 8));
+    check(device_malloc(&y, vb * m + 8));
+    check(device_malloc(&ax, vb * m + 8));
+    check(device_malloc(&aty, vb * n + 8));
     check(hipEventCreate(&e0));
     check(hipEventCreate(&e1));
     if (rc == VK_OK) check(hipMemsetAsync(x, 0, vb * n + 8, st));

@@ -221,7 +221,7 @@ struct Scratch {
     template <class T>
     T *get(size_t count) {
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(1, count)) != hipSuccess) return nullptr;
+        if (device_malloc(&p, sizeof(T) * std::max<size_t>(1, count)) != hipSuccess) return nullptr;
         blocks.push_back(p);
         return static_cast<T *>(p);
     }
@@ -279,17 +279,8 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
                "vk_hip_lp_scale: null column vector");
     VK_REQUIRE(m == 0 || (rl && ru && row_scale && rl_out && ru_out), "vk_hip_lp_scale: null row vector");
     hipStream_t s = default_stream();
-    // Start from the values last set: save them on the first scaling, put
-    // them back on a later one.
-    if (nnz) {
-        if (A->scaled) {
-            VK_HIP_CHECK(hipMemcpyAsync(A->a.val, A->orig, sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
-        } else {
-            if (!A->orig) VK_HIP_CHECK(hipMalloc(&A->orig, sizeof(double) * nnz));
-            VK_HIP_CHECK(hipMemcpyAsync(A->orig, A->a.val, sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
-        }
-    }
-    A->scaled = true;
+    // Every allocation comes before the matrix is touched: a failed one
+    // leaves A as it was.
     Scratch sb;
     double *dc = sb.get<double>(n), *dl = sb.get<double>(n), *du = sb.get<double>(n);
     double *ds = sb.get<double>(n), *cf = sb.get<double>(n);
@@ -298,6 +289,27 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
         set_error("vk_hip_lp_scale: scratch allocation failed");
         return VK_ENOMEM;
     }
+    const int passes = 1 + ruiz_passes + (pock_chambolle ? 1 : 0);
+    double *factors = nullptr;
+    if (device_malloc(&factors, sizeof(double) * std::max<int64_t>(1, n * passes)) != hipSuccess ||
+        (nnz && !A->scaled && !A->orig && device_malloc(&A->orig, sizeof(double) * nnz) != hipSuccess)) {
+        (void)hipFree(factors);
+        set_error("vk_hip_lp_scale: device allocation failed");
+        return VK_ENOMEM;
+    }
+    (void)hipFree(A->col_factors);
+    A->col_factors = factors;
+    A->col_passes = 0;
+    // Start from the values last set: save them on the first scaling, put
+    // them back on a later one.
+    if (nnz) {
+        if (A->scaled) {
+            VK_HIP_CHECK(hipMemcpyAsync(A->a.val, A->orig, sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
+        } else {
+            VK_HIP_CHECK(hipMemcpyAsync(A->orig, A->a.val, sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    A->scaled = true;
     if (n) {
         VK_HIP_CHECK(hipMemcpyAsync(dc, c, sizeof(double) * n, hipMemcpyHostToDevice, s));
         VK_HIP_CHECK(hipMemcpyAsync(dl, l, sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -308,11 +320,6 @@ int vk_hip_lp_scale(vk_hip_csr *A, const double *c, const double *l, const doubl
         VK_HIP_CHECK(hipMemcpyAsync(dru, ru, sizeof(double) * m, hipMemcpyHostToDevice, s));
     }
     const int mi = static_cast<int>(m), ni = static_cast<int>(n);
-    const int passes = 1 + ruiz_passes + (pock_chambolle ? 1 : 0);
-    (void)hipFree(A->col_factors);
-    A->col_factors = nullptr;
-    A->col_passes = 0;
-    VK_HIP_CHECK(hipMalloc(&A->col_factors, sizeof(double) * std::max<int64_t>(1, n * passes)));
     // continuous_common.hpp:133-205 on the original values (the transpose
     // holds them too: set_values and the restore above both propagate).
     if (int rc = propagate_values(A, s)) return rc;

@@ -19,6 +19,14 @@ int bind_device();
 hipStream_t default_stream();
 int ensure_init();
 
+// Every device allocation of the library goes through device_malloc, so a
+// test can make them fail on demand (vk_hip_debug_fail_allocations).
+hipError_t device_alloc_bytes(void **p, std::size_t bytes);
+template <class T>
+hipError_t device_malloc(T **p, std::size_t bytes) {
+    return device_alloc_bytes(reinterpret_cast<void **>(p), bytes);
+}
+
 inline hipStream_t as_stream(void *s) {
     return s ? static_cast<hipStream_t>(s) : default_stream();
 }
@@ -90,7 +98,7 @@ struct vk_hip_csr {
         hipError_t vk_err_ = (expr);                                            \
         if (vk_err_ != hipSuccess) {                                            \
             vk::set_error(std::string(#expr) + ": " + hipGetErrorString(vk_err_)); \
-            return VK_EDEVICE;                                                  \
+            return vk_err_ == hipErrorOutOfMemory ? VK_ENOMEM : VK_EDEVICE;     \
         }                                                                       \
     } while (0)
 

@@ -191,9 +191,11 @@ struct vk_hip_unrolled {
     std::vector<void *> allocations;
 
     template <class T>
+    // A failed allocation is listed as null, so a scan of `allocations`
+    // after a run of alloc calls finds it.
     T *alloc(size_t count) {
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) return nullptr;
+        if (device_malloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) p = nullptr;
         allocations.push_back(p);
         return static_cast<T *>(p);
     }

@@ -166,6 +166,7 @@ HIP_SYMBOLS = {
     "vk_hip_csr_set_tile_geometry": (C.c_int, [_vp, C.c_int]),
     "vk_hip_csr_tile_geometry": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "vk_hip_csr_tune_tiles": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
+    "vk_hip_debug_fail_allocations": (C.c_int, [C.c_int64, C.POINTER(C.c_int64)]),
     "vk_hip_pdhg_destroy": (C.c_int, [_vp]),
     "vk_hip_csr_tune_col_blocks": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp]),
     "vk_hip_batch_create": (C.c_int, [C.c_int, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -720,6 +721,14 @@ class Device:
         check(self.L.vk_hip_csr_tune_tiles(h, int(precision), int(repeats), C.byref(best), _ptr(ms)),
               "vk_hip_csr_tune_tiles")
         return best.value, ms.tolist()
+
+    def fail_allocations(self, after=-1) -> int:
+        """vk_hip_debug_fail_allocations: the device allocations made since
+        the previous call; from now on the first `after` allocations
+        succeed and every later one fails (after < 0: none fails)."""
+        made = C.c_int64()
+        check(self.L.vk_hip_debug_fail_allocations(int(after), C.byref(made)), "vk_hip_debug_fail_allocations")
+        return made.value
 
     def operator_bound(self, h) -> float:
         r = C.c_double()
