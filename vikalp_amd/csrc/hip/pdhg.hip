@@ -838,11 +838,14 @@ bool control_after(const vk_hip_pdhg *s, long long k) {
 // written by the primal kernel) runs on the main stream; the next dual pass
 // waits for both.  The x+ buffer is chosen by iteration parity on the host,
 // so the overlap needs an even block length (graph replays then keep the
-// parity).  The same kernels compute the same sums either way.
+// parity).  The same kernels compute the same sums either way.  The panel
+// layout keeps the finalizer in-stream: its A x workgroups hold every CU's
+// LDS, so a finalizer beside them only starts when they end, and the fork
+// and join cost more (about 6 us each) than the finalizer itself.
 template <class T>
 int enqueue_iterations(vk_hip_pdhg *s, long long k_begin, long long count) {
     Bufs<T> &b = bufs<T>(s);
-    const bool overlap = s->side && (s->col_blocks > 1 || s->panels) && block_length(s) % 2 == 0;
+    const bool overlap = s->side && s->col_blocks > 1 && block_length(s) % 2 == 0;
     bool spmv_done = false;   // iteration k's A x+ was launched beside the previous finalizer
     for (long long k = k_begin; k < k_begin + count; ++k) {
         launch_dual<T>(s, s->stream, spmv_done ? 2 : 3);
