@@ -617,9 +617,11 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
         p32.precision = VK_PREC_MIXED;
         p32.col_blocks = col_blocks_for(A, options, VK_PREC_MIXED);
         p32.relative_kkt_tolerance = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
-        // A Halpern FP32 phase stalls near 1e-4..1e-5 (measured on the
-        // 400k x 1M LP; its iterate, anchor and their A x, A' y carried by
-        // linearity are all FP32), so it hands over no later than 1e-5.
+        // A Halpern FP32 phase stalls between 6e-6 and 4e-5 (measured on the
+        // 400k x 1M LP: the FP32 rounding of every output, accumulated by
+        // the Halpern iteration between restarts; an FP64 Halpern state or
+        // stochastic rounding of the outputs does not lower it), so it hands
+        // over no later than 1e-5.
         if (p.policy == VK_PDHG_POLICY_HALPERN)
             p32.relative_kkt_tolerance = std::max(p32.relative_kkt_tolerance, kHalpernFp32Floor);
         status = run_device(A, sc, p32, warm_scaled, nullptr, started, options, xs, ys, state, ms, timed);
