@@ -470,8 +470,12 @@ def main():
     ap.add_argument("--c1-cpu-runs", type=int, default=5,
                     help="runs of the CPU oracle's full solve of the 1,000 x 3,000 LP (median; 0 skips "
                          "the config-1 measurement)")
-    ap.add_argument("--phase-tolerance", type=float, default=1e-5,
-                    help="relative KKT target of the FP32 phase of the mixed solve")
+    ap.add_argument("--phase-tolerance", type=float, default=0.0,
+                    help="relative KKT target of the FP32 phase of the mixed solve (0: automatic, the "
+                         "final target for the recentred FP32 iteration)")
+    ap.add_argument("--no-recentre", action="store_true",
+                    help="mixed solve with plain FP32 iterates (FP32 phase to 1e-5 at most) instead of the "
+                         "recentred FP32 iteration")
     ap.add_argument("--batch-instances", type=int, default=512,
                     help="block-angular instances of the batched solve (0 = skip)")
     ap.add_argument("--batch-iterations", type=int, default=40_000)
@@ -522,7 +526,7 @@ def main():
     def options(precision, policy=V.PdhgPolicy.Halpern):
         return V.Options(pdhg_policy=policy, pdhg_precision=precision,
                          relative_kkt_tolerance=WORKLOAD["tolerance"],
-                         mixed_phase_tolerance=args.phase_tolerance,
+                         mixed_phase_tolerance=args.phase_tolerance, mixed_recentre=not args.no_recentre,
                          iteration_limit=args.max_iterations, time_limit_seconds=args.time_limit,
                          use_graphs=not args.no_graphs, spmv_panels=args.spmv_panels)
 

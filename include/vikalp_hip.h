@@ -255,6 +255,17 @@ typedef struct {
                                       instances' tiles, 0..3 as in
                                       vk_hip_csr_set_tile_geometry (the
                                       single-LP solver uses its matrix's)    */
+    int recentre;                  /* 1 (Halpern, VK_PREC_MIXED, single LP
+                                      only): store the FP32 iterates as
+                                      corrections to an FP64 base point that
+                                      every restart moves to the restart
+                                      point, with the LP shifted to the base
+                                      (rounded to FP32 once per restart), so
+                                      that FP32 rounding is relative to the
+                                      correction and the FP32 iteration does
+                                      not stall near 1e-5 relative KKT;
+                                      download returns the total point.
+                                      0 = plain FP32 iterates              */
 } vk_pdhg_params;
 
 typedef struct {

@@ -51,6 +51,7 @@ typedef struct {
     int spmv_panel_threads;         /* lanes per panel workgroup: 0 auto, 256, 512 */
     int spmv_panel_kb;              /* KB per staged panel: 0 auto, 48, 24, 16 */
     int spmv_tile_geometry;         /* row tiles: 0..3 (vk_hip_csr_set_tile_geometry), -1 tune */
+    int mixed_recentre;             /* mixed Halpern FP32 phase as corrections to an FP64 base */
 } vk_solver_options;
 
 /* Fills `o` with the defaults of vikalp::SolverOptions. */

@@ -590,6 +590,7 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
                "vk_hip_batch_create: unknown precision");
     VK_REQUIRE(params->tile_geometry >= 0 && params->tile_geometry < kTileGeometries,
                "vk_hip_batch_create: tile_geometry must be 0..3");
+    VK_REQUIRE(params->recentre == 0, "vk_hip_batch_create: the batched solver has no recentred FP32 iteration");
     VK_REQUIRE(params->iteration_limit >= 0 && params->restart_period > 0,
                "vk_hip_batch_create: bad iteration limit or restart period");
     const int64_t m = row_starts[count], n = col_starts[count];

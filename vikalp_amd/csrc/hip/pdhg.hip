@@ -436,9 +436,12 @@ __global__ __launch_bounds__(kBlock) void k_infeas_cols(Bufs<T> b, const T *__re
     __shared__ double scratch[kSlotScratch];
     Partials part;
     part.zero();
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock)
-        farkas_column(static_cast<double>(__builtin_nontemporal_load(g + j)), static_cast<double>(b.l[j]),
-                      static_cast<double>(b.u[j]), part);
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
+        double gj = static_cast<double>(__builtin_nontemporal_load(g + j));
+        // Recentred: g is A' of the dual correction; c - cobj adds A' ybase.
+        if (b.cobj) gj += static_cast<double>(b.c[j]) - static_cast<double>(b.cobj[j]);
+        farkas_column(gj, static_cast<double>(b.l[j]), static_cast<double>(b.u[j]), part);
+    }
     store_partials(part, b.part_ecols, scratch);
 }
 
@@ -469,6 +472,150 @@ __global__ __launch_bounds__(kBlock) void k_halpern_apply(Bufs<T> b) {
         if (t < b.n) halpern_apply_column(b, s, t, xp, xnext);
         if (t < b.m) halpern_apply_row(b, s, t);
     }
+}
+
+// ------------------------------------------------------------------ recentring
+//
+// FP32 storage bounds how close a plain FP32 iteration gets to the solution:
+// every output is rounded relative to the iterate's own size (about 6e-8 of
+// it), and the Halpern iteration averages those errors down only as far as
+// its anchor weight allows; on the 400k x 1M LP the relative KKT error then
+// stalls between 6e-6 and 4e-5.  The recentred solver stores instead the
+// correction to an FP64 base point: x = xbase + u, y = ybase + v.  PDHG,
+// the Halpern step and the restart rule are affine-invariant, so u and v
+// follow the same iteration on the LP shifted to the base (objective c, the
+// gradient offset c + A'ybase, bounds l - xbase, u - xbase, rl - A xbase,
+// ru - A xbase), with the dual step taken on the total ybase + v.  Rounding
+// is then relative to the correction, which shrinks with the error.  At
+// every restart the restart point T(z_k) becomes the new base: the
+// corrections fold into (xbase, ybase) in FP64, A xbase and A'ybase are
+// recomputed with the FP64 matrix, the shifted LP is rounded to FP32 once,
+// and every correction restarts from zero.
+template <class T>
+struct RecentreBufs {
+    double *xb, *yb;                 // base point (n, m)
+    const double *c, *l, *u;         // scaled LP in FP64 (n)
+    const double *rl, *ru;           // (m)
+    double *ax, *aty;                // A xb, A' yb (FP64 sums)
+    T *cs, *ls, *us, *rls, *rus;     // the shifted LP (aliases of Bufs c, l, u, rl, ru)
+};
+
+// The dual sign rule of the Moreau step (y+ = 0 inside [rl, ru], y+ <= 0 at
+// rl, y+ >= 0 at ru): a total formed as ybase + v keeps it up to the
+// rounding of v, which this removes (a wrong-signed dual on a one-sided row
+// would make the dual objective -inf).
+__device__ __forceinline__ double dual_sign_project(double y, double rl, double ru) {
+    if (y > 0.0 && isinf(ru)) return 0.0;
+    if (y < 0.0 && isinf(rl)) return 0.0;
+    return y;
+}
+
+// Whether the recentring kernels after a restart check act: at a restart
+// that the finalizer applied (or always, when forced at setup).
+__device__ __forceinline__ bool recentre_now(const DevState *st, int force) {
+    return force || (st->status == VK_PDHG_RUNNING && st->restart_apply);
+}
+
+// Folds the restart point T(z_k) (the corrections x+ = x[k & 1] and y) into
+// the base and clears every correction and its products.  Runs after
+// k_halpern_apply, which copied T(z_k) into the Halpern state.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_recentre_base(Bufs<T> b, RecentreBufs<T> r, int force) {
+    if (!recentre_now(b.st, force)) return;
+    const long long k = b.st->k;
+    T *xp = b.x[k & 1];
+    const int total = b.n > b.m ? b.n : b.m;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        if (t < b.n) {
+            r.xb[t] = clamp_box(r.xb[t] + static_cast<double>(xp[t]), r.l[t], r.u[t]);
+            xp[t] = T(0);
+            b.xh[t] = T(0);
+            b.x0h[t] = T(0);
+            b.xlast[t] = T(0);
+            b.aty[t] = T(0);
+            b.aty0h[t] = T(0);
+        }
+        if (t < b.m) {
+            r.yb[t] = dual_sign_project(r.yb[t] + static_cast<double>(b.y[t]), r.rl[t], r.ru[t]);
+            b.y[t] = T(0);
+            b.yh[t] = T(0);
+            b.y0h[t] = T(0);
+            b.ylast[t] = T(0);
+            b.ax[t] = T(0);
+            b.ax0h[t] = T(0);
+        }
+    }
+}
+
+// y = A x on the FP64 tiles of A or A' (the base's products), when
+// recentre_now.
+__global__ __launch_bounds__(kBlock) void k_spmv_recentre(const DevState *st, int force, CsrView<double> A,
+                                                         const double *x, double *y) {
+    if (!recentre_now(st, force)) return;
+    __shared__ double lds[kWaves][kTileNnz];
+    StoreEpi<double> epi{y};
+    Partials part;
+    part.zero();
+    tile_spmv(A, x, epi, part, lds[threadIdx.x / kWave]);
+}
+
+// The LP shifted to the new base, rounded to T once, and the next primal
+// step from the cleared Halpern state (k_halpern_apply's step with xh = 0
+// and A'yh = 0); partial slot 5 collects c'xbase.  Launched over grid_cols
+// workgroups (the size of part_ecols).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_recentre_data(Bufs<T> b, RecentreBufs<T> r, int force) {
+    if (!recentre_now(b.st, force)) return;
+    __shared__ double scratch[kSlotScratch];
+    const long long k = b.st->k;
+    const double tau = b.st->tau;
+    T *xnext = b.x[(k + 1) & 1];
+    Partials part;
+    part.zero();
+    const int total = b.n > b.m ? b.n : b.m;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        if (t < b.n) {
+            const double xb = r.xb[t];
+            const T g = static_cast<T>(r.c[t] + r.aty[t]);
+            const T lo = static_cast<T>(r.l[t] - xb), hi = static_cast<T>(r.u[t] - xb);
+            r.cs[t] = g;
+            r.ls[t] = lo;
+            r.us[t] = hi;
+            xnext[t] = static_cast<T>(clamp_box((0.0 - tau * static_cast<double>(g)) - tau * 0.0,
+                                                static_cast<double>(lo), static_cast<double>(hi)));
+            part.v[5] += r.c[t] * xb;
+        }
+        if (t < b.m) {
+            const double ax = r.ax[t];
+            r.rls[t] = static_cast<T>(r.rl[t] - ax);
+            r.rus[t] = static_cast<T>(r.ru[t] - ax);
+        }
+    }
+    store_partials(part, b.part_ecols, scratch);
+}
+
+template <class T>
+__global__ __launch_bounds__(kFinalBlock) void k_recentre_finish(Bufs<T> b, int force) {
+    if (!recentre_now(b.st, force)) return;
+    __shared__ double scratch[kFinalScratch];
+    __shared__ double red[kSlots];
+    reduce_partials(b.part_ecols, b.grid_cols, red, scratch);
+    if (threadIdx.x == 0) b.st->cx_base = red[5];
+}
+
+// The total point (xbase + x, ybase + y) in FP64, x kept in [l, u] and y on
+// the dual sign rule (both only remove rounding of the corrections).
+template <class T>
+__global__ void k_recentre_total(int n, int m, const T *x, const T *y, RecentreBufs<T> r, double *xo, double *yo) {
+    const int total = n > m ? n : m;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        if (t < n) xo[t] = clamp_box(r.xb[t] + static_cast<double>(x[t]), r.l[t], r.u[t]);
+        if (t < m) yo[t] = dual_sign_project(r.yb[t] + static_cast<double>(y[t]), r.rl[t], r.ru[t]);
+    }
+}
+
+__global__ void k_clamp64(int n, double *x, const double *l, const double *u) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) x[j] = clamp_box(x[j], l[j], u[j]);
 }
 
 template <class T>
@@ -676,6 +823,12 @@ struct vk_hip_pdhg {
     // FP32 solvers: FP64 staging of n + m entries for uploads and the
     // solution download, allocated once with the solver.
     double *stage64 = nullptr;
+    // Recentred FP32 Halpern solver (vk_pdhg_params.recentre): the base
+    // point, the FP64 LP and the launch widths of the FP64 tile SpMVs that
+    // recompute the base's products.
+    bool recentre = false;
+    RecentreBufs<float> rec{};
+    int grid_a64 = 1, grid_at64 = 1;
 
     template <class T>
     // A failed allocation is listed as null, so a scan of `allocations`
@@ -792,6 +945,21 @@ void launch_eval_rows(vk_hip_pdhg *s, hipStream_t st) {
     }
 }
 
+// The recentring after a restart check (or, forced, at setup): fold the
+// restart point into the base, A xbase and A'ybase on the FP64 tiles, the
+// shifted LP in FP32 with the next primal step, and c'xbase.  Unforced, each
+// kernel returns at once unless the finalizer applied a restart.
+void launch_recentre(vk_hip_pdhg *s, hipStream_t st, int force) {
+    Bufs<float> &b = s->b32;
+    k_recentre_base<float><<<egrid(std::max(s->n, s->m)), kBlock, 0, st>>>(b, s->rec, force);
+    if (s->m > 0) {
+        k_spmv_recentre<<<s->grid_a64, kBlock, 0, st>>>(b.st, force, s->A->a.view(), s->rec.xb, s->rec.ax);
+        k_spmv_recentre<<<s->grid_at64, kBlock, 0, st>>>(b.st, force, s->A->at.view(), s->rec.yb, s->rec.aty);
+    }
+    k_recentre_data<float><<<s->grid_cols, kBlock, 0, st>>>(b, s->rec, force);
+    k_recentre_finish<float><<<1, kFinalBlock, 0, st>>>(b, force);
+}
+
 // The kernels that follow the primal kernel of iteration k: the finalizer,
 // then the policy's step-size or restart kernels.
 template <class T>
@@ -806,6 +974,8 @@ void launch_control(vk_hip_pdhg *s, long long k, hipStream_t st) {
             k_infeas_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, b.aty_avg);
             k_infeas_check<T><<<1, kFinalBlock, 0, st>>>(b);
             k_halpern_apply<T><<<egrid(std::max(s->n, s->m)), kBlock, 0, st>>>(b);
+            if constexpr (sizeof(T) == sizeof(float))
+                if (s->recentre) launch_recentre(s, st, 0);
         }
     } else if ((k + 1) % s->params.restart_period == 0) {
         k_infeas_cols<T><<<s->grid_cols, kBlock, 0, st>>>(b, b.aty);   // A' y+ of this iteration
@@ -1073,6 +1243,27 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     b.part_cols = s->alloc<double>(kSlots * s->grid_cols);
     b.part_erows = s->alloc<double>(kSlots * s->grid_rows);
     b.part_ecols = s->alloc<double>(kSlots * s->grid_cols);
+    RecentreBufs<float> &rec = s->rec;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (s->recentre) {
+            rec.xb = s->alloc<double>(n);
+            rec.yb = s->alloc<double>(m);
+            rec.c = s->alloc<double>(n);
+            rec.l = s->alloc<double>(n);
+            rec.u = s->alloc<double>(n);
+            rec.rl = s->alloc<double>(m);
+            rec.ru = s->alloc<double>(m);
+            rec.ax = s->alloc<double>(m);
+            rec.aty = s->alloc<double>(n);
+            rec.cs = c_d;
+            rec.ls = l_d;
+            rec.us = u_d;
+            rec.rls = rl_d;
+            rec.rus = ru_d;
+            b.cobj = s->alloc<T>(n);
+            b.ybase = rec.yb;
+        }
+    }
     b.st = s->st;
     b.grid_rows = s->grid_rows;
     b.grid_cols = s->grid_cols;
@@ -1113,10 +1304,43 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     rc = rc ? rc : put(ru_d, ru, m);
     rc = rc ? rc : put(runs, unscale.data(), m);
     rc = rc ? rc : put(cs, col_scale, n);
-    rc = rc ? rc : put(b.x[0], x0 ? x0 : zeros_n.data(), n);
-    rc = rc ? rc : put(b.y, y0 ? y0 : zeros_m.data(), m);
+    if (s->recentre) {
+        // The start is the first base point (x0 projected in FP64); the
+        // corrections start at zero.
+        rc = rc ? rc : put(b.x[0], zeros_n.data(), n);
+        rc = rc ? rc : put(b.y, zeros_m.data(), m);
+        if constexpr (sizeof(T) == sizeof(float)) {
+            rc = rc ? rc : put(const_cast<T *>(b.cobj), c, n);
+            auto put64 = [&](const double *dst, const double *src, int count) -> int {
+                if (count)
+                    VK_HIP_CHECK(hipMemcpyAsync(const_cast<double *>(dst), src, sizeof(double) * count,
+                                                hipMemcpyHostToDevice, st));
+                return VK_OK;
+            };
+            rc = rc ? rc : put64(rec.c, c, n);
+            rc = rc ? rc : put64(rec.l, l, n);
+            rc = rc ? rc : put64(rec.u, u, n);
+            rc = rc ? rc : put64(rec.rl, rl, m);
+            rc = rc ? rc : put64(rec.ru, ru, m);
+            rc = rc ? rc : put64(rec.xb, x0 ? x0 : zeros_n.data(), n);
+            rc = rc ? rc : put64(rec.yb, y0 ? y0 : zeros_m.data(), m);
+        }
+    } else {
+        rc = rc ? rc : put(b.x[0], x0 ? x0 : zeros_n.data(), n);
+        rc = rc ? rc : put(b.y, y0 ? y0 : zeros_m.data(), m);
+    }
     VK_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) return rc;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (s->recentre) {
+            // The LP shifted to the start (c, l, u, rl, ru above become the
+            // shifted ones) and c'xbase; the start projection, A x0, A' y0
+            // and the first primal step below then act on zero corrections.
+            k_clamp64<<<egrid(n), kBlock, 0, st>>>(n, rec.xb, rec.l, rec.u);
+            launch_recentre(s, st, 1);
+            VK_HIP_CHECK(hipGetLastError());
+        }
+    }
 
     if (s->panels) {
         s->blk_part = s->alloc<double>(static_cast<size_t>(s->pan_a.groups) * m);
@@ -1297,8 +1521,14 @@ int download_solution(vk_hip_pdhg *s, double *x, double *y) {
         if (y && s->m) VK_HIP_CHECK(hipMemcpyAsync(y, b.y, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->stream));
     } else {
         double *stage = s->stage64;   // n + m entries, allocated with the solver
-        if (s->n) k_f2d<<<egrid(s->n), kBlock, 0, s->stream>>>(s->n, xs, stage);
-        if (s->m) k_f2d<<<egrid(s->m), kBlock, 0, s->stream>>>(s->m, b.y, stage + s->n);
+        if (s->recentre) {
+            // The total point xbase + x, ybase + y.
+            k_recentre_total<T><<<egrid(std::max(s->n, s->m)), kBlock, 0, s->stream>>>(s->n, s->m, xs, b.y, s->rec,
+                                                                                      stage, stage + s->n);
+        } else {
+            if (s->n) k_f2d<<<egrid(s->n), kBlock, 0, s->stream>>>(s->n, xs, stage);
+            if (s->m) k_f2d<<<egrid(s->m), kBlock, 0, s->stream>>>(s->m, b.y, stage + s->n);
+        }
         if (x && s->n) VK_HIP_CHECK(hipMemcpyAsync(x, stage, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
         if (y && s->m) VK_HIP_CHECK(hipMemcpyAsync(y, stage + s->n, sizeof(double) * s->m, hipMemcpyDeviceToHost, s->stream));
     }
@@ -1554,6 +1784,10 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     VK_REQUIRE(!(params->panel_threads == 256 && params->panel_kb > 0 && params->panel_kb != 48),
                "vk_hip_pdhg_create: 256 lanes per panel workgroup need 48 KB panels");
     VK_REQUIRE(params->panel_tune == 0 || params->panel_tune == 1, "vk_hip_pdhg_create: panel_tune must be 0 or 1");
+    VK_REQUIRE(params->recentre == 0 || params->recentre == 1, "vk_hip_pdhg_create: recentre must be 0 or 1");
+    VK_REQUIRE(params->recentre == 0 || (params->policy == VK_PDHG_POLICY_HALPERN &&
+                                         params->precision == VK_PREC_MIXED && !Q),
+               "vk_hip_pdhg_create: recentre needs the Halpern policy in VK_PREC_MIXED without a quadratic objective");
     if (int rc = ensure_init()) return rc;
     auto *s = new (std::nothrow) vk_hip_pdhg();
     if (!s) return VK_ENOMEM;
@@ -1563,6 +1797,9 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     s->m = static_cast<int>(A->rows);
     s->n = static_cast<int>(A->cols);
     s->mixed = params->precision == VK_PREC_MIXED;
+    s->recentre = params->recentre == 1;
+    s->grid_a64 = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, kMaxGrid));
+    s->grid_at64 = std::max(1, std::min((A->at.ntiles + kWaves - 1) / kWaves, kMaxGrid));
     const int cap = std::min(params->grid_blocks > 0 ? params->grid_blocks : 2048, kMaxGrid);
     // One tile per wave, kWaves waves per block.
     s->grid_rows = std::max(1, std::min((A->a.ntiles + kWaves - 1) / kWaves, cap));

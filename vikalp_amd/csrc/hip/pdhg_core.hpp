@@ -37,6 +37,9 @@ struct DevState {
     // Row part of the dual objective of the last dual iterate, sum of its
     // term magnitudes and count of -inf terms (primal infeasibility test).
     double d0_rows, d0_rows_abs, d0_rows_ninf;
+    // Recentred solver: c'x of the base point, added to both objectives (the
+    // partial sums hold those of the LP shifted to the base); 0 otherwise.
+    double cx_base;
 };
 
 // Partial-slot layout (slot 0 combines with max, the rest with +).
@@ -68,6 +71,13 @@ struct Bufs {
     // aty0h).  x[] and y then hold the PDHG outputs T(z), which the SpMVs
     // gather; ax_avg / aty_avg keep A x+ and A' y+ at restart checks.
     T *xh, *yh, *x0h, *y0h, *ax0h, *aty0h;
+    // Recentred Halpern solver (null otherwise): every iterate above is a
+    // correction to an FP64 base point (xbase, ybase), and c, l, u, rl, ru
+    // hold the LP shifted to that base (c + A'ybase, l - xbase, u - xbase,
+    // rl - A xbase, ru - A xbase).  The dual step acts on the total dual
+    // ybase + y; cobj is the objective c itself (primal objective terms).
+    const double *ybase;
+    const T *cobj;
     double *part_rows, *part_cols, *part_erows, *part_ecols;
     DevState *st;
     int m, n;
@@ -173,6 +183,7 @@ struct HalpernDualEpi {
     bool check;
     struct Pre {
         T ax, yh, lo, hi, unscale, y0, ax0, ylast;
+        double yb;
     };
     __device__ Pre prefetch(int i) const {
         Pre p;
@@ -184,6 +195,7 @@ struct HalpernDualEpi {
         p.y0 = vec_load<Stream>(b.y0h + i);
         p.ax0 = vec_load<Stream>(b.ax0h + i);
         p.ylast = check ? vec_load<Stream>(b.ylast + i) : T(0);
+        p.yb = b.ybase ? vec_load<Stream>(b.ybase + i) : 0.0;
         return p;
     }
     __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
@@ -191,8 +203,13 @@ struct HalpernDualEpi {
         const double ax_old = static_cast<double>(q.ax), y_old = static_cast<double>(q.yh);
         const double lo = static_cast<double>(q.lo), hi = static_cast<double>(q.hi);
         const double v = y_old + sigma * (2.0 * axn - ax_old);
-        const double t = inv_sigma * v;   // sign-exact Moreau step, as for the adaptive policy
-        const double yn = (t < lo) ? fmin(v - sigma * lo, 0.0) : ((hi < t) ? fmax(v - sigma * hi, 0.0) : 0.0);
+        // Recentred: the step acts on the total dual ybase + v and the stored
+        // y+ is its correction, both formed in FP64, so that only the
+        // correction is rounded to T.  yt is the exact (sign-exact) total.
+        const double vt = b.ybase ? v + q.yb : v;
+        const double t = inv_sigma * vt;   // sign-exact Moreau step, as for the adaptive policy
+        const double yt = (t < lo) ? fmin(vt - sigma * lo, 0.0) : ((hi < t) ? fmax(vt - sigma * hi, 0.0) : 0.0);
+        const double yn = b.ybase ? yt - q.yb : yt;
         const T y_stored = static_cast<T>(yn);
         b.y[i] = y_stored;
         const double yp = static_cast<double>(y_stored);
@@ -203,10 +220,10 @@ struct HalpernDualEpi {
         const double du = d * static_cast<double>(q.unscale);
         p.v[1] += du * du;
         p.v[2] += d * d;
-        if (yn > 0.0) {
-            if (isinf(hi)) p.v[4] += 1.0; else { p.v[3] -= yn * hi; p.v[7] += fabs(yn * hi); }
-        } else if (yn < 0.0) {
-            if (isinf(lo)) p.v[4] += 1.0; else { p.v[3] -= yn * lo; p.v[7] += fabs(yn * lo); }
+        if (yt > 0.0) {
+            if (isinf(hi)) p.v[4] += 1.0; else { p.v[3] -= yt * hi; p.v[7] += fabs(yt * hi); }
+        } else if (yt < 0.0) {
+            if (isinf(lo)) p.v[4] += 1.0; else { p.v[3] -= yt * lo; p.v[7] += fabs(yt * lo); }
         }
         const double dz = yp - y_old;
         p.v[6] += dz * dz;
@@ -297,7 +314,7 @@ struct HalpernPrimalEpi {
     bool fuse, check;
     HalpernWeights w;
     struct Pre {
-        T xp, xh, aty, c, l, u, s, x0, aty0, xlast;
+        T xp, xh, aty, c, l, u, s, x0, aty0, xlast, cobj;
     };
     __device__ Pre prefetch(int j) const {
         Pre p;
@@ -311,13 +328,17 @@ struct HalpernPrimalEpi {
         p.x0 = vec_load<Stream>(b.x0h + j);
         p.aty0 = vec_load<Stream>(b.aty0h + j);
         p.xlast = check ? vec_load<Stream>(b.xlast + j) : T(0);
+        p.cobj = b.cobj ? vec_load<Stream>(b.cobj + j) : T(0);
         return p;
     }
     __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
         const double xp = static_cast<double>(q.xp), x_old = static_cast<double>(q.xh);
         const double cj = static_cast<double>(q.c);
         const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
-        column_terms(xp, cj, cj + acc, lj, uj, static_cast<double>(q.s), check, static_cast<double>(q.xlast), p);
+        // Recentred: c holds c + A'ybase, so the total gradient is c + acc
+        // as before, while the primal objective terms take c itself.
+        const double cobj = b.cobj ? static_cast<double>(q.cobj) : cj;
+        column_terms(xp, cobj, cj + acc, lj, uj, static_cast<double>(q.s), check, static_cast<double>(q.xlast), p);
         const double dz = xp - x_old;
         p.v[7] += dz * dz;
         const T xh_new = static_cast<T>(w.w1 * (2.0 * xp - x_old) + w.w0 * static_cast<double>(q.x0));
@@ -393,14 +414,17 @@ __device__ __forceinline__ void relative_kkt(const DevState &s, const double *r,
                                              double &gap_abs, double &pobj, double &dobj) {
     rel_p = sqrt(r[1]) / (1.0 + s.norm_b);
     rel_d = sqrt(c[1]) / (1.0 + s.norm_c);
-    pobj = c[5];
+    // Both objectives of an LP shifted by a base point x_b drop c'x_b (the
+    // dual one by shifting the bounds it takes), so the gap is taken on the
+    // shifted sums and cx_base added back to each objective.
+    pobj = s.cx_base + c[5];
     const bool neg_inf = (r[4] + c[4]) > 0.0;
-    dobj = neg_inf ? -INFINITY : r[3] + c[3];
+    dobj = neg_inf ? -INFINITY : s.cx_base + (r[3] + c[3]);
     if (neg_inf) {
         rel_g = 1.0;
         gap_abs = INFINITY;
     } else {
-        gap_abs = fabs(pobj - dobj);
+        gap_abs = fabs(c[5] - (r[3] + c[3]));
         rel_g = gap_abs / (1.0 + fabs(pobj) + fabs(dobj));
     }
 }

@@ -421,8 +421,8 @@ void run_stacked(const std::vector<const Model *> &models, const std::vector<Pre
     std::vector<char> infeasible32(prep.size(), 0);
     std::vector<double> ys32;
     if (options.pdhg_precision == PdhgPrecision::Mixed) {
-        double phase_tol = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
-        if (options.pdhg_policy == PdhgPolicy::Halpern) phase_tol = std::max(phase_tol, kHalpernFp32Floor);
+        // The batched FP32 phase stores plain FP32 iterates (no recentring).
+        const double phase_tol = fp32_phase_tolerance(options, false);
         run_batch(models, prep, st, options, VK_PREC_MIXED, phase_tol, options.iteration_limit, warm0, nullptr, nullptr,
                   nullptr, started, states, xs, ys, ms, timed);
         total_ms += ms;

@@ -215,7 +215,7 @@ int main(int argc, char **argv) {
                                                   : vikalp::PdhgPolicy::Halpern;
     options.pdhg_precision = precision == "mixed" ? vikalp::PdhgPrecision::Mixed : vikalp::PdhgPrecision::FP64;
     options.relative_kkt_tolerance = policy == "reference" ? 0.0 : tolerance;
-    options.mixed_phase_tolerance = 1e-5;
+    options.mixed_phase_tolerance = 0.0;   // automatic (SolverOptions)
     options.iteration_limit = iteration_limit;
     auto wanted = [&](const std::string &name) {
         return cases.empty() || ("," + cases + ",").find("," + name + ",") != std::string::npos;

@@ -51,6 +51,7 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.spmv_panel_threads = o.spmv_panel_threads;
     s.spmv_panel_kb = o.spmv_panel_kb;
     s.spmv_tile_geometry = o.spmv_tile_geometry;
+    s.mixed_recentre = o.mixed_recentre != 0;
     return s;
 }
 
@@ -122,6 +123,7 @@ void vk_solver_options_default(vk_solver_options *o) {
     o->spmv_panel_threads = d.spmv_panel_threads;
     o->spmv_panel_kb = d.spmv_panel_kb;
     o->spmv_tile_geometry = d.spmv_tile_geometry;
+    o->mixed_recentre = d.mixed_recentre ? 1 : 0;
 }
 
 static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,

@@ -45,9 +45,18 @@ struct SolverOptions {
     // > 0: stop when the relative KKT error (original coordinates) is at or
     // below this, instead of the reference's scaled infinity-norm test.
     Scalar relative_kkt_tolerance = 0.0;
-    // Mixed precision: relative KKT target of the FP32 phase (at least 1e-5
-    // under the Halpern policy, whose FP32 phase stalls near 1e-4 to 1e-5).
-    Scalar mixed_phase_tolerance = 1e-4;
+    // Mixed precision: relative KKT target of the FP32 phase.  0 = automatic:
+    // the final target when the FP32 phase is recentred (below), else 1e-4.
+    // A plain FP32 Halpern phase stalls between 6e-6 and 4e-5 and so stops
+    // no later than 1e-5.
+    Scalar mixed_phase_tolerance = 0.0;
+    // Mixed precision under the Halpern policy: the FP32 phase stores its
+    // iterates as corrections to an FP64 base point that moves to the
+    // restart point at every restart (FP32 rounding is then relative to the
+    // correction), so it converges like the FP64 iteration instead of
+    // stalling; the FP64 phase then only certifies.  false = plain FP32
+    // iterates.
+    bool mixed_recentre = true;
     // Adaptive and Halpern policies: iterations between restart checks.
     Index restart_period = 64;
     // Run whole iterations on the device (two fused kernels) instead of the

@@ -17,8 +17,13 @@ namespace vikalp::solver::detail {
 
 using Clock = std::chrono::steady_clock;
 constexpr int kRuizPasses = 10;
-// Smallest relative KKT target of a Halpern FP32 phase (see solve_lp's mixed driver).
+// Smallest relative KKT target of a plain (not recentred) Halpern FP32 phase
+// (see solve_lp's mixed driver).
 constexpr Scalar kHalpernFp32Floor = 1e-5;
+// Relative KKT target of the FP32 phase of a mixed solve
+// (SolverOptions::mixed_phase_tolerance; `recentred`: the phase runs the
+// recentred FP32 iteration).
+Scalar fp32_phase_tolerance(const SolverOptions &o, bool recentred);
 
 struct Bounds {
     std::vector<Scalar> lower, upper;

@@ -541,6 +541,19 @@ int drive_device(DeviceSolver &solver, const vk_pdhg_params &params, std::size_t
     return state.status;
 }
 
+Scalar fp32_phase_tolerance(const SolverOptions &o, bool recentred) {
+    Scalar t = o.mixed_phase_tolerance > 0.0 ? o.mixed_phase_tolerance : (recentred ? 0.0 : 1e-4);
+    t = std::max(t, o.relative_kkt_tolerance);
+    // A plain FP32 Halpern phase stalls between 6e-6 and 4e-5 (measured on
+    // the 400k x 1M LP: the FP32 rounding of every output, relative to the
+    // iterate, accumulated by the Halpern iteration between restarts; an
+    // FP64 Halpern state or stochastic rounding of the outputs does not lower
+    // it), so it hands over no later than 1e-5.  The recentred phase rounds
+    // relative to the correction and has no such floor.
+    if (!recentred && o.pdhg_policy == PdhgPolicy::Halpern) t = std::max(t, kHalpernFp32Floor);
+    return t;
+}
+
 }  // namespace detail
 
 namespace {
@@ -612,18 +625,14 @@ SolveResult solve_fused(const Model &model, const Bounds &bounds, const Scaled &
         // always runs: it certifies the returned point in FP64 (an FP32 phase
         // that met the target on its FP64-evaluated average can miss it by
         // the rounding of that average to FP32) and usually stops after its
-        // first iterations.
+        // first iterations.  Under the Halpern policy the FP32 phase is
+        // recentred (unless options.mixed_recentre is off) and by default
+        // runs to the final target itself.
         vk_pdhg_params p32 = p;
         p32.precision = VK_PREC_MIXED;
         p32.col_blocks = col_blocks_for(A, options, VK_PREC_MIXED);
-        p32.relative_kkt_tolerance = std::max(options.mixed_phase_tolerance, options.relative_kkt_tolerance);
-        // A Halpern FP32 phase stalls between 6e-6 and 4e-5 (measured on the
-        // 400k x 1M LP: the FP32 rounding of every output, accumulated by
-        // the Halpern iteration between restarts; an FP64 Halpern state or
-        // stochastic rounding of the outputs does not lower it), so it hands
-        // over no later than 1e-5.
-        if (p.policy == VK_PDHG_POLICY_HALPERN)
-            p32.relative_kkt_tolerance = std::max(p32.relative_kkt_tolerance, kHalpernFp32Floor);
+        p32.recentre = (p.policy == VK_PDHG_POLICY_HALPERN && options.mixed_recentre) ? 1 : 0;
+        p32.relative_kkt_tolerance = fp32_phase_tolerance(options, p32.recentre == 1);
         status = run_device(A, sc, p32, warm_scaled, nullptr, started, options, xs, ys, state, ms, timed);
         res.low_precision_iterations = state.iterations;
         Index used = state.iterations;
@@ -780,6 +789,8 @@ KernelProfile profile_lp_kernels(const Model &model, const SolverOptions &option
     vk_pdhg_params p = make_params(model, sc, A, options);
     p.precision = options.pdhg_precision == PdhgPrecision::Mixed ? VK_PREC_MIXED : VK_PREC_FP64;
     p.col_blocks = col_blocks_for(A, options, p.precision);
+    // The FP32 kernels as the mixed solve runs them (recentred under Halpern).
+    p.recentre = (p.precision == VK_PREC_MIXED && p.policy == VK_PDHG_POLICY_HALPERN && options.mixed_recentre) ? 1 : 0;
     p.iteration_limit = warm_iterations + iterations + 1;
     p.relative_kkt_tolerance = 0.0;
     p.metric_tolerance = 0.0;

@@ -77,6 +77,7 @@ class _SolverOptions(C.Structure):
         ("spmv_split_passes", C.c_int), ("spmv_cache_budget_mb", C.c_int),
         ("spmv_panels", C.c_int), ("spmv_panel_groups", C.c_int), ("spmv_panel_groups_t", C.c_int),
         ("spmv_panel_threads", C.c_int), ("spmv_panel_kb", C.c_int), ("spmv_tile_geometry", C.c_int),
+        ("mixed_recentre", C.c_int),
     ]
 
 
@@ -104,7 +105,7 @@ class PdhgParams(C.Structure):
         ("grid_blocks", C.c_int), ("col_blocks", C.c_int), ("split_passes", C.c_int),
         ("cache_budget_mb", C.c_int), ("spmv_panels", C.c_int), ("panel_groups", C.c_int),
         ("panel_groups_t", C.c_int), ("panel_threads", C.c_int), ("panel_tune", C.c_int),
-        ("panel_kb", C.c_int), ("tile_geometry", C.c_int),
+        ("panel_kb", C.c_int), ("tile_geometry", C.c_int), ("recentre", C.c_int),
     ]
 
 
@@ -376,7 +377,7 @@ class Options:
     fused_iterations: bool = True
     use_graphs: bool = True
     relative_kkt_tolerance: float = 0.0
-    mixed_phase_tolerance: float = 1e-4
+    mixed_phase_tolerance: float = 0.0
     restart_period: int = 64
     spmv_col_blocks: int = 0
     spmv_autotune: bool = False
@@ -388,6 +389,7 @@ class Options:
     spmv_panel_threads: int = 0
     spmv_panel_kb: int = 0
     spmv_tile_geometry: int = 0
+    mixed_recentre: bool = True
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
