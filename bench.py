@@ -8,16 +8,19 @@ seeded (generator family 2 of include/vikalp_lpgen.h).
 
 A step is one solve from a cold start until the relative KKT error (original
 coordinates) is <= 1e-6, with restarted Halpern PDHG (reflection, fixed-point
-residual restarts) on the fused HIP path; the FP32 phase of the mixed solve
-runs to 1e-5.  The same solve with the adaptive policy (restarted averaged
-PDHG) is reported beside it.  `value` is the mixed-precision device time of the iteration phase (the
-LP is already resident in HBM when it starts; setup, scaling and postsolve
-are reported separately), `fp64_seconds` the same for the FP64 path.
+residual restarts) on the fused HIP path.  The mixed solve's FP32 phase is
+recentred (FP32 corrections to an FP64 base point moved to the restart point
+at every restart) and runs to 1e-6 itself; its FP64 phase then certifies the
+point.  The same solve with the adaptive policy (restarted averaged PDHG) is
+reported beside it.  `value` is the mixed-precision device time of the
+iteration phase (the LP is already resident in HBM when it starts; setup,
+scaling and postsolve are reported separately), `fp64_seconds` the same for
+the FP64 path.
 
-roofline: the dominant fused kernel (A' y with the residual terms and the
-next primal step, or A x with the dual update) timed with HIP events on the
-solver's stream; achieved = its algorithmic HBM bytes per launch / mean
-duration.  cpu_baseline: the CPU oracle (the reference's single-threaded
+roofline: the dominant kernel of the timed (mixed) solve's iteration, timed
+with HIP events on the solver's stream; achieved = its algorithmic HBM bytes
+per launch / mean duration.  roofline_fp64: the same for the FP64 drop-in's
+iteration.  cpu_baseline: the CPU oracle (the reference's single-threaded
 algorithm restated in C) timed on a bounded sample of iterations of the same
 LP on one host core, extrapolated to the GPU's iteration count.
 
@@ -73,7 +76,7 @@ def host_cpu_model():
     return f"{name} ({', '.join(extra)})" if extra else name
 
 
-def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True):
+def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True, recentred=False):
     """Algorithmic (compulsory) HBM bytes and achieved GB/s per kernel of one
     iteration, for the layout the solver used.
 
@@ -97,11 +100,13 @@ def kernel_table(prof, m, n, nnz, sv, suffix, halpern=True):
     (the packed entries are s_v + 4 bytes each; there are no row offsets to
     read, and the partial sums of the groups beyond the first are not
     counted), then k_dual_rows and k_primal_cols the row and column passes.
+    Recentred FP32 passes read one vector more each: the FP64 dual base
+    (8 m) and the objective c beside the shifted gradient (s_v n).
     """
     spmv_a = (sv + 4) * nnz + 4 * (m + 1) + sv * n + 8 * m
     spmv_at = (sv + 4) * nnz + 4 * (n + 1) + sv * m + 8 * n
-    pass_rows = 8 * m + (10 * sv * m if halpern else 8 * sv * m + 16 * m)
-    pass_cols = 8 * n + (12 * sv * n if halpern else 8 * sv * n + 16 * n)
+    pass_rows = 8 * m + (10 * sv * m if halpern else 8 * sv * m + 16 * m) + (8 * m if recentred else 0)
+    pass_cols = 8 * n + (12 * sv * n if halpern else 8 * sv * n + 16 * n) + (sv * n if recentred else 0)
     rows = []
     if prof.get("panels") and prof["panels"][0] > 0:
         rows.append(("k_panel_spmv[A x]", prof["dual_spmv_ms"], spmv_a))
@@ -142,18 +147,21 @@ def fused_iteration_bytes(m, n, nnz, sv, halpern=True):
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, per dispatch, scaled by a 256 MiB copy "
                   "(profiles/pdhg_traffic_zipf_400k_x_1m.md); FETCH_SIZE includes MALL hits")
 MEASURED_TRAFFIC = {
-    ("k_panel_spmv[A x]<double>", 8, 24): 320.4e6, ("k_dual_rows<double>", 8, 24): 57.8e6,
+    ("k_panel_spmv[A x]<double>", 8, 24): 319.6e6, ("k_dual_rows<double>", 8, 24): 57.8e6,
     ("k_panel_spmv[A'y]<double>", 2, 48): 317.0e6, ("k_primal_cols<double>", 2, 48): 112.4e6,
-    ("k_panel_spmv[A x]<float>", 8, 24): 172.9e6, ("k_dual_rows<float>", 8, 24): 41.8e6,
-    ("k_panel_spmv[A'y]<float>", 4, 16): 179.7e6, ("k_primal_cols<float>", 4, 16): 80.4e6,
+    ("k_panel_spmv[A x]<float>", 8, 24): 172.7e6, ("k_dual_rows<float>", 8, 24): 45.0e6,
+    ("k_panel_spmv[A'y]<float>", 4, 16): 179.5e6, ("k_primal_cols<float>", 4, 16): 84.4e6,
 }
 
 
-def measured_traffic(kernel, panels):
+def measured_traffic(kernel, panels, recentred=False):
     """The measured HBM bytes of `kernel` (a kernel_table label) when the
     profile's panel layout (vk_hip_pdhg_panel_layout's ten ints) has the shape
-    it was measured with, else None."""
+    it was measured with, else None.  The FP32 passes were measured
+    recentred (the mixed solve's default)."""
     if not panels or panels[0] <= 0:
+        return None
+    if kernel.endswith("<float>") and not kernel.startswith("k_panel_spmv") and not recentred:
         return None
     a_side = "A x" in kernel or kernel.startswith("k_dual_rows")
     g, kb = (panels[0], panels[4]) if a_side else (panels[5], panels[9])
@@ -534,10 +542,10 @@ def main():
     prof = V.profile_kernels(lp, options(V.PdhgPrecision.FP64), warm_iterations=30, iterations=60)
     prof32 = V.profile_kernels(lp, options(V.PdhgPrecision.Mixed), warm_iterations=30, iterations=60)
     k64 = kernel_table(prof, lp.m, lp.n, lp.nnz, 8, "double")
-    k32 = kernel_table(prof32, lp.m, lp.n, lp.nnz, 4, "float")
-    for table, p in ((k64, prof), (k32, prof32)):
+    k32 = kernel_table(prof32, lp.m, lp.n, lp.nnz, 4, "float", recentred=not args.no_recentre)
+    for table, p, rec in ((k64, prof, False), (k32, prof32, not args.no_recentre)):
         for name, row in table.items():
-            row["traffic"] = measured_traffic(name, p["panels"])
+            row["traffic"] = measured_traffic(name, p["panels"], rec)
     for label, p, table in (("fp64", prof, k64), ("mixed", prof32, k32)):
         log(f"{label}: iteration {p['iteration_ms']:.4f} ms, col_blocks={p['col_blocks']} "
             f"split={p['split_passes']} panels={p['panels']}; " +
@@ -574,9 +582,13 @@ def main():
     r32, r64 = mixed_runs[-1], fp64_runs[-1]
     obj_err = abs(r32.objective_value - r64.objective_value) / max(1.0, abs(r64.objective_value))
 
-    dominant = max(k64, key=lambda k: k64[k]["ms"])
-    dom_ms, dom_bytes = k64[dominant]["ms"], k64[dominant]["bytes"]
-    achieved = k64[dominant]["GBs"]
+    def roofline(table, panels, rec):
+        dominant = max(table, key=lambda k: table[k]["ms"])
+        row = table[dominant]
+        return {"bound": "hbm", "achieved": row["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": row["GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, panels, rec),
+                "traffic_source": TRAFFIC_SOURCE, "kernel": dominant, "bytes_per_launch": row["bytes"],
+                "mean_ms": row["ms"]}
 
     adaptive = None
     if not args.skip_solve and not args.skip_adaptive:
@@ -649,12 +661,13 @@ def main():
         "rel_kkt_fp64": r64.kkt_relative,
         "rel_kkt_mixed": r32.kkt_relative,
         "objective_rel_diff_mixed_vs_fp64": obj_err,
-        "ms_per_iteration_fp64": prof["iteration_ms"],
-        "ms_per_iteration_mixed": prof32["iteration_ms"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, prof["panels"]),
-                     "traffic_source": TRAFFIC_SOURCE,
-                     "kernel": dominant, "bytes_per_launch": dom_bytes, "mean_ms": dom_ms},
+        # Per iteration in the solves (graph replays) and, kernel by kernel
+        # with events between them, in the profile.
+        "ms_per_iteration_fp64": t64 * 1e3 / max(1, r64.iterations),
+        "ms_per_iteration_mixed": t32 * 1e3 / max(1, r32.iterations),
+        "ms_per_iteration_profiled": {"fp64": prof["iteration_ms"], "mixed": prof32["iteration_ms"]},
+        "roofline": roofline(k32, prof32["panels"], not args.no_recentre),
+        "roofline_fp64": roofline(k64, prof["panels"], False),
         "layout": {"fp64": {"col_blocks": prof["col_blocks"], "split_passes": prof["split_passes"],
                             "panels": prof["panels"]},
                    "mixed": {"col_blocks": prof32["col_blocks"], "split_passes": prof32["split_passes"],

@@ -266,6 +266,12 @@ typedef struct {
                                       not stall near 1e-5 relative KKT;
                                       download returns the total point.
                                       0 = plain FP32 iterates              */
+    double primal_weight_gain;     /* adaptive / Halpern: proportional gain of
+                                      the primal-weight update at restarts,
+                                      log w <- log w - gain e - integral sum(e),
+                                      e = log w - log(|dy| / |dx|); 0 = 1/2
+                                      (PDLP's smoothing)                     */
+    double primal_weight_integral; /* integral gain (0 = none)               */
 } vk_pdhg_params;
 
 typedef struct {

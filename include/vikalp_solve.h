@@ -52,6 +52,9 @@ typedef struct {
     int spmv_panel_kb;              /* KB per staged panel: 0 auto, 48, 24, 16 */
     int spmv_tile_geometry;         /* row tiles: 0..3 (vk_hip_csr_set_tile_geometry), -1 tune */
     int mixed_recentre;             /* mixed Halpern FP32 phase as corrections to an FP64 base */
+    int reserved2;
+    double pdhg_primal_weight_gain;     /* 0 = 1/2 */
+    double pdhg_primal_weight_integral; /* 0 = none */
 } vk_solver_options;
 
 /* Fills `o` with the defaults of vikalp::SolverOptions. */

@@ -591,6 +591,9 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
     VK_REQUIRE(params->tile_geometry >= 0 && params->tile_geometry < kTileGeometries,
                "vk_hip_batch_create: tile_geometry must be 0..3");
     VK_REQUIRE(params->recentre == 0, "vk_hip_batch_create: the batched solver has no recentred FP32 iteration");
+    VK_REQUIRE(params->primal_weight_gain >= 0.0 && params->primal_weight_gain <= 1.0 &&
+                   params->primal_weight_integral >= 0.0 && params->primal_weight_integral <= 1.0,
+               "vk_hip_batch_create: primal-weight gains must be in [0, 1]");
     VK_REQUIRE(params->iteration_limit >= 0 && params->restart_period > 0,
                "vk_hip_batch_create: bad iteration limit or restart period");
     const int64_t m = row_starts[count], n = col_starts[count];
@@ -836,6 +839,8 @@ int batch_create(int count, const int64_t *row_starts, const int64_t *col_starts
             S.sigma = step * S.omega;
         }
         S.inv_sigma = step > 0.0 ? 1.0 / S.sigma : 0.0;
+        S.pw_kp = params->primal_weight_gain > 0.0 ? params->primal_weight_gain : 0.5;
+        S.pw_ki = params->primal_weight_integral;
     }
     if (up(h->states, h->states_host.data(), sizeof(DevState) * count) != hipSuccess) {
         set_error("vk_hip_batch_create: state upload failed");

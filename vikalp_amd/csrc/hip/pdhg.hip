@@ -1785,6 +1785,9 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
                "vk_hip_pdhg_create: 256 lanes per panel workgroup need 48 KB panels");
     VK_REQUIRE(params->panel_tune == 0 || params->panel_tune == 1, "vk_hip_pdhg_create: panel_tune must be 0 or 1");
     VK_REQUIRE(params->recentre == 0 || params->recentre == 1, "vk_hip_pdhg_create: recentre must be 0 or 1");
+    VK_REQUIRE(params->primal_weight_gain >= 0.0 && params->primal_weight_gain <= 1.0 &&
+                   params->primal_weight_integral >= 0.0 && params->primal_weight_integral <= 1.0,
+               "vk_hip_pdhg_create: primal-weight gains must be in [0, 1]");
     VK_REQUIRE(params->recentre == 0 || (params->policy == VK_PDHG_POLICY_HALPERN &&
                                          params->precision == VK_PREC_MIXED && !Q),
                "vk_hip_pdhg_create: recentre needs the Halpern policy in VK_PREC_MIXED without a quadratic objective");
@@ -1860,6 +1863,8 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
         init.sigma = init.eta * init.omega;
     }
     init.inv_sigma = 1.0 / init.sigma;
+    init.pw_kp = params->primal_weight_gain > 0.0 ? params->primal_weight_gain : 0.5;
+    init.pw_ki = params->primal_weight_integral;
     init.kkt_ref = std::numeric_limits<double>::infinity();
     init.kkt_prev = std::numeric_limits<double>::infinity();
     init.rel_kkt = std::numeric_limits<double>::infinity();

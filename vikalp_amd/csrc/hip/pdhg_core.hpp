@@ -40,6 +40,9 @@ struct DevState {
     // Recentred solver: c'x of the base point, added to both objectives (the
     // partial sums hold those of the LP shifted to the base); 0 otherwise.
     double cx_base;
+    // Primal-weight update at restarts (adaptive and Halpern): proportional
+    // and integral gains on the log-balance error, and the error's sum.
+    double pw_kp, pw_ki, pw_int;
 };
 
 // Partial-slot layout (slot 0 combines with max, the rest with +).
@@ -83,6 +86,19 @@ struct Bufs {
     int m, n;
     int grid_rows, grid_cols;
 };
+
+// Primal weight after a restart whose primal and dual moves since the last
+// restart point were dx and dy (both > 0).  PDLP's smoothed update, w <-
+// exp(log(dy/dx)/2 + log(w)/2), is a proportional controller on the
+// log-balance error e = log w - log(dy/dx) with gain 1/2; this is its
+// proportional-integral form, log w <- log w - kp e - ki sum(e), with
+// kp = 1/2, ki = 0 by default (PID control of the primal weight: Lu, Peng
+// and Yang, "cuPDLPx", 2025).
+__device__ __forceinline__ double primal_weight_update(DevState &S, double dx, double dy) {
+    const double target = log(dy / dx);
+    S.pw_int += log(S.omega) - target;
+    return exp(S.pw_kp * target + (1.0 - S.pw_kp) * log(S.omega) - S.pw_ki * S.pw_int);
+}
 
 __device__ __forceinline__ double kkt_weighted(double w, double pr2, double dr2, double gap) {
     return sqrt(w * w * pr2 + dr2 / (w * w) + gap * gap);
@@ -486,7 +502,7 @@ __device__ inline void halpern_rule(DevState &S, const double *red_rows, const d
     S.kkt_prev = r;
     if (!restart) return;
     const double dx = sqrt(red_cols[6]), dy = sqrt(red_rows[5]);
-    if (dx > 1e-10 && dy > 1e-10) S.omega = exp(0.5 * log(dy / dx) + 0.5 * log(S.omega));
+    if (dx > 1e-10 && dy > 1e-10) S.omega = primal_weight_update(S, dx, dy);
     S.tau = S.eta / S.omega;
     S.sigma = S.eta * S.omega;
     S.inv_sigma = 1.0 / S.sigma;
@@ -589,7 +605,7 @@ __device__ inline void check_step(DevState &S, const double *red_rows, const dou
     if (!restart) return;
     const double dx = sqrt(use_avg ? red_cols[6] : S.cur_dx2);
     const double dy = sqrt(use_avg ? red_rows[5] : S.cur_dy2);
-    if (dx > 1e-10 && dy > 1e-10) S.omega = exp(0.5 * log(dy / dx) + 0.5 * log(S.omega));
+    if (dx > 1e-10 && dy > 1e-10) S.omega = primal_weight_update(S, dx, dy);
     S.tau = S.eta / S.omega;
     S.sigma = S.eta * S.omega;
     S.inv_sigma = 1.0 / S.sigma;

@@ -254,6 +254,8 @@ std::unique_ptr<TailJob> prepare_tail(const Model &model, const Prepared &pr, co
     vk_pdhg_params &p = job->p;
     p.policy = options.pdhg_policy == PdhgPolicy::Adaptive ? VK_PDHG_POLICY_ADAPTIVE : VK_PDHG_POLICY_HALPERN;
     p.precision = precision;
+    p.primal_weight_gain = options.pdhg_primal_weight_gain;
+    p.primal_weight_integral = options.pdhg_primal_weight_integral;
     p.iteration_limit = std::max<Index>(0, limit - state.iterations);
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
     p.relative_kkt_tolerance = rel_tol;
@@ -349,6 +351,8 @@ void run_batch(const std::vector<const Model *> &models, const std::vector<Prepa
                : options.pdhg_policy == PdhgPolicy::Halpern ? VK_PDHG_POLICY_HALPERN
                                                             : VK_PDHG_POLICY_REFERENCE;
     p.precision = precision;
+    p.primal_weight_gain = options.pdhg_primal_weight_gain;
+    p.primal_weight_integral = options.pdhg_primal_weight_integral;
     p.iteration_limit = iteration_limit;
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
     p.relative_kkt_tolerance = rel_tol;

@@ -52,6 +52,8 @@ SolverOptions to_options(const vk_solver_options &o) {
     s.spmv_panel_kb = o.spmv_panel_kb;
     s.spmv_tile_geometry = o.spmv_tile_geometry;
     s.mixed_recentre = o.mixed_recentre != 0;
+    s.pdhg_primal_weight_gain = o.pdhg_primal_weight_gain;
+    s.pdhg_primal_weight_integral = o.pdhg_primal_weight_integral;
     return s;
 }
 
@@ -124,6 +126,8 @@ void vk_solver_options_default(vk_solver_options *o) {
     o->spmv_panel_kb = d.spmv_panel_kb;
     o->spmv_tile_geometry = d.spmv_tile_geometry;
     o->mixed_recentre = d.mixed_recentre ? 1 : 0;
+    o->pdhg_primal_weight_gain = d.pdhg_primal_weight_gain;
+    o->pdhg_primal_weight_integral = d.pdhg_primal_weight_integral;
 }
 
 static bool build_model(int64_t m, int64_t n, const int64_t *row_offsets,

@@ -57,6 +57,12 @@ struct SolverOptions {
     // stalling; the FP64 phase then only certifies.  false = plain FP32
     // iterates.
     bool mixed_recentre = true;
+    // Adaptive and Halpern policies: gains of the primal-weight update at
+    // restarts, log w <- log w - gain e - integral sum(e) with the balance
+    // error e = log w - log(|dy| / |dx|) of the moves since the last
+    // restart.  gain 0 = 1/2 (PDLP's smoothing); integral 0 = none.
+    Scalar pdhg_primal_weight_gain = 0.0;
+    Scalar pdhg_primal_weight_integral = 0.0;
     // Adaptive and Halpern policies: iterations between restart checks.
     Index restart_period = 64;
     // Run whole iterations on the device (two fused kernels) instead of the

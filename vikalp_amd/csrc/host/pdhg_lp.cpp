@@ -56,6 +56,8 @@ bool valid_options(const SolverOptions &o) {
            (o.spmv_panel_kb == 0 || o.spmv_panel_kb == 48 || o.spmv_panel_kb == 24 || o.spmv_panel_kb == 16) &&
            !(o.spmv_panel_threads == 256 && o.spmv_panel_kb != 0 && o.spmv_panel_kb != 48) &&
            o.spmv_tile_geometry >= -1 && o.spmv_tile_geometry < 4 &&
+           o.pdhg_primal_weight_gain >= 0.0 && o.pdhg_primal_weight_gain <= 1.0 &&
+           o.pdhg_primal_weight_integral >= 0.0 && o.pdhg_primal_weight_integral <= 1.0 &&
            (o.pdhg_policy == PdhgPolicy::Reference || o.pdhg_policy == PdhgPolicy::Adaptive ||
             o.pdhg_policy == PdhgPolicy::Halpern);
 }
@@ -579,6 +581,8 @@ vk_pdhg_params make_params(const Model &model, const Scaled &sc, const vk_hip_cs
     p.panel_threads = options.spmv_panel_threads;
     p.panel_kb = options.spmv_panel_kb;
     p.panel_tune = options.spmv_autotune ? 1 : 0;
+    p.primal_weight_gain = options.pdhg_primal_weight_gain;
+    p.primal_weight_integral = options.pdhg_primal_weight_integral;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
     if (p.policy == VK_PDHG_POLICY_REFERENCE) {

@@ -77,7 +77,8 @@ class _SolverOptions(C.Structure):
         ("spmv_split_passes", C.c_int), ("spmv_cache_budget_mb", C.c_int),
         ("spmv_panels", C.c_int), ("spmv_panel_groups", C.c_int), ("spmv_panel_groups_t", C.c_int),
         ("spmv_panel_threads", C.c_int), ("spmv_panel_kb", C.c_int), ("spmv_tile_geometry", C.c_int),
-        ("mixed_recentre", C.c_int),
+        ("mixed_recentre", C.c_int), ("reserved2", C.c_int),
+        ("pdhg_primal_weight_gain", C.c_double), ("pdhg_primal_weight_integral", C.c_double),
     ]
 
 
@@ -106,6 +107,7 @@ class PdhgParams(C.Structure):
         ("cache_budget_mb", C.c_int), ("spmv_panels", C.c_int), ("panel_groups", C.c_int),
         ("panel_groups_t", C.c_int), ("panel_threads", C.c_int), ("panel_tune", C.c_int),
         ("panel_kb", C.c_int), ("tile_geometry", C.c_int), ("recentre", C.c_int),
+        ("primal_weight_gain", C.c_double), ("primal_weight_integral", C.c_double),
     ]
 
 
@@ -390,6 +392,8 @@ class Options:
     spmv_panel_kb: int = 0
     spmv_tile_geometry: int = 0
     mixed_recentre: bool = True
+    pdhg_primal_weight_gain: float = 0.0
+    pdhg_primal_weight_integral: float = 0.0
 
     def _c(self) -> _SolverOptions:
         o = _SolverOptions()
