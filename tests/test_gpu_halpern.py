@@ -33,9 +33,11 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 HIGHS = json.load(open(os.path.join(GOLDEN, "highs_optima.json")))
 
 
-def halpern_np(lp, eta, omega, period, iterations):
+def halpern_np(lp, eta, omega, period, iterations, kp=0.5, ki=0.0):
     """The device's Halpern iteration restated with numpy on the unscaled LP,
-    from x0 = P_X(0), y0 = 0.  Returns (x+, y+, restarts) of the last
+    from x0 = P_X(0), y0 = 0, with the primal-weight update
+    log w <- log w - kp e - ki sum(e), e = log w - log(dy/dx) (kp = 1/2,
+    ki = 0: PDLP's smoothing).  Returns (x+, y+, restarts) of the last
     iteration."""
     A = sp.csr_matrix((lp.values, lp.column_indices, lp.row_offsets), shape=(lp.m, lp.n))
     AT = A.T.tocsr()
@@ -46,7 +48,7 @@ def halpern_np(lp, eta, omega, period, iterations):
     tau, sigma = eta / omega, eta * omega
     z0 = (x.copy(), y.copy(), ax.copy(), aty.copy())
     xr, yr = x.copy(), y.copy()
-    ks, r0, prev, restarts = 0, None, math.inf, 0
+    ks, r0, prev, restarts, esum = 0, None, math.inf, 0, 0.0
     xp = yp = None
     for k in range(iterations):
         xp = np.clip((x - tau * c) - tau * aty, l, u)
@@ -64,7 +66,9 @@ def halpern_np(lp, eta, omega, period, iterations):
             if restart:
                 dx, dy = np.linalg.norm(xp - xr), np.linalg.norm(yp - yr)
                 if dx > 1e-10 and dy > 1e-10:
-                    omega = math.exp(0.5 * math.log(dy / dx) + 0.5 * math.log(omega))
+                    target = math.log(dy / dx)
+                    esum += math.log(omega) - target
+                    omega = math.exp(kp * target + (1.0 - kp) * math.log(omega) - ki * esum)
                 tau, sigma = eta / omega, eta * omega
                 x, y, ax, aty = xp, yp, axp, atyp
                 z0 = (x.copy(), y.copy(), ax.copy(), aty.copy())
@@ -80,7 +84,7 @@ def halpern_np(lp, eta, omega, period, iterations):
     return xp, yp, restarts
 
 
-def device_halpern(lp, eta, omega, period, iterations, col_blocks, split):
+def device_halpern(lp, eta, omega, period, iterations, col_blocks, split, kp=0.0, ki=0.0):
     dev = V.Device()
     L = dev.L
     h = dev.csr(lp.m, lp.n, lp.row_offsets, lp.column_indices, lp.values)
@@ -93,6 +97,7 @@ def device_halpern(lp, eta, omega, period, iterations, col_blocks, split):
     p.norm_b, p.norm_c = 1.0, 1.0
     p.use_graphs = 1
     p.col_blocks, p.split_passes = col_blocks, split
+    p.primal_weight_gain, p.primal_weight_integral = kp, ki
     arr = [np.ascontiguousarray(a, dtype=np.float64) for a in
            (lp.objective, lp.var_lower, lp.var_upper, lp.row_lower, lp.row_upper, np.ones(lp.m), np.ones(lp.n))]
     s = C.c_void_p()
@@ -132,6 +137,21 @@ def test_device_iterates_match_restatement(gen, layout):
     x, y, st = device_halpern(lp, eta, omega, period, K, blocks, split)
     assert st.iterations == K and st.status == 2   # VK_PDHG_ITERATION_LIMIT
     assert st.restarts == restarts and restarts >= 2, (st.restarts, restarts)
+    sx = max(1.0, float(np.max(np.abs(xr))))
+    sy = max(1.0, float(np.max(np.abs(yr))))
+    assert np.max(np.abs(x - xr)) <= 1e-9 * sx and np.max(np.abs(y - yr)) <= 1e-9 * sy
+
+
+@pytest.mark.parametrize("gains", [(0.5, 0.02), (0.3, 0.05), (0.9, 0.0)])
+def test_primal_weight_gains_match_restatement(gains):
+    # The proportional-integral primal-weight update (the Halpern policy's
+    # default integral gain is 0.02) against the restatement.
+    kp, ki = gains
+    lp = V.generate_lp(2, 2000, 5000, 0, 7)
+    eta, omega, period, K = step_of(lp), 1.0, 64, 1500
+    xr, yr, restarts = halpern_np(lp, eta, omega, period, K, kp, ki)
+    x, y, st = device_halpern(lp, eta, omega, period, K, 1, 1, kp, ki)
+    assert st.restarts == restarts and restarts >= 3, (st.restarts, restarts)
     sx = max(1.0, float(np.max(np.abs(xr))))
     sy = max(1.0, float(np.max(np.abs(yr))))
     assert np.max(np.abs(x - xr)) <= 1e-9 * sx and np.max(np.abs(y - yr)) <= 1e-9 * sy
