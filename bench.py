@@ -201,7 +201,8 @@ def measure_batch(V, np, args):
     for label, precision in (("fp64", V.PdhgPrecision.FP64), ("mixed", V.PdhgPrecision.Mixed)):
         o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision,
                       relative_kkt_tolerance=WORKLOAD["tolerance"], iteration_limit=args.batch_iterations,
-                      mixed_phase_tolerance=1e-4, time_limit_seconds=args.time_limit,
+                      mixed_phase_tolerance=args.phase_tolerance, mixed_recentre=not args.no_recentre,
+                      time_limit_seconds=args.time_limit,
                       use_graphs=not getattr(args, "no_graphs", False))
         t0 = time.time()
         res = V.solve_batch(lps, o)

@@ -256,3 +256,4 @@ def test_recentre_is_refused_outside_mixed_halpern():
             assert rc != 0 and not s.value, (policy, precision)
         finally:
             L.vk_hip_csr_destroy(h)
+

@@ -72,13 +72,27 @@ struct StoreEpi {
     }
 };
 
-// The dual and primal epilogues of one iteration: H selects the Halpern
-// policy's, otherwise the reference / adaptive ones.
-template <class T, bool H, bool Stream>
-struct IterEpis;
+// Epilogue modes of the iteration kernels: the reference / adaptive
+// epilogues, the Halpern ones, and the Halpern ones of the recentred FP32
+// solver (which read the dual base and the objective beside the shifted
+// gradient).
+constexpr int kEpiPlain = 0, kEpiHalpern = 1, kEpiRecentred = 2;
+
+// The dual and primal epilogues of one iteration in mode M.
+template <class T, int M, bool Stream>
+struct IterEpis {
+    static constexpr bool R = M == kEpiRecentred;
+    static __device__ HalpernDualEpi<T, Stream, R> dual(const Bufs<T> &b, const DevState &s) {
+        return HalpernDualEpi<T, Stream, R>{b, s.sigma, s.inv_sigma, halpern_weights(s.ks), check_iteration(s)};
+    }
+    static __device__ HalpernPrimalEpi<T, Stream, R> primal(const Bufs<T> &b, const DevState &s, bool) {
+        return HalpernPrimalEpi<T, Stream, R>{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, may_fuse(s),
+                                              check_iteration(s), halpern_weights(s.ks)};
+    }
+};
 
 template <class T, bool Stream>
-struct IterEpis<T, false, Stream> {
+struct IterEpis<T, kEpiPlain, Stream> {
     static __device__ DualEpi<T, Stream> dual(const Bufs<T> &b, const DevState &s) {
         return DualEpi<T, Stream>{b, s.sigma, s.inv_sigma, s.policy == VK_PDHG_POLICY_REFERENCE && s.restart_flag,
                                   s.policy == VK_PDHG_POLICY_ADAPTIVE};
@@ -88,38 +102,27 @@ struct IterEpis<T, false, Stream> {
     }
 };
 
-template <class T, bool Stream>
-struct IterEpis<T, true, Stream> {
-    static __device__ HalpernDualEpi<T, Stream> dual(const Bufs<T> &b, const DevState &s) {
-        return HalpernDualEpi<T, Stream>{b, s.sigma, s.inv_sigma, halpern_weights(s.ks), check_iteration(s)};
-    }
-    static __device__ HalpernPrimalEpi<T, Stream> primal(const Bufs<T> &b, const DevState &s, bool) {
-        return HalpernPrimalEpi<T, Stream>{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, may_fuse(s),
-                                           check_iteration(s), halpern_weights(s.ks)};
-    }
-};
-
-template <class T, bool H>
+template <class T, int M>
 __global__ __launch_bounds__(kBlock) void k_dual(Bufs<T> b, CsrView<T> A) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kSlotScratch];
     const T *xin = b.x[(s.k + 1) & 1];
-    auto epi = IterEpis<T, H, false>::dual(b, s);
+    auto epi = IterEpis<T, M, false>::dual(b, s);
     Partials part;
     part.zero();
     tile_spmv(A, xin, epi, part, lds[threadIdx.x / kWave]);
     store_partials(part, b.part_rows, scratch);
 }
 
-template <class T, bool H>
+template <class T, int M>
 __global__ __launch_bounds__(kBlock) void k_primal(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double lds[kWaves][kTileNnz];
     __shared__ double scratch[kSlotScratch];
-    auto epi = IterEpis<T, H, false>::primal(b, s, true);
+    auto epi = IterEpis<T, M, false>::primal(b, s, true);
     Partials part;
     part.zero();
     tile_spmv(AT, b.y, epi, part, lds[threadIdx.x / kWave]);
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_spmv(Bufs<T> b, CsrView<T> A, d
 // after a workgroup barrier, the workgroup runs the column pass over exactly
 // those rows: no second launch, and the pass's streaming overlaps other
 // workgroups' gathers.  The sums and iterates are the split kernels' own.
-template <class T, bool H>
+template <class T, int M>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
 void k_primal_own(Bufs<T> b, CsrView<T> AT) {
     const DevState s = *b.st;
@@ -169,7 +172,7 @@ void k_primal_own(Bufs<T> b, CsrView<T> AT) {
     }
     __syncthreads();   // the sums of this workgroup's rows are visible to all its threads
     const int r0 = AT.tiles[t0].x, r1 = AT.tiles[t1].x;
-    auto epi = IterEpis<T, H, true>::primal(b, s, sizeof(T) != sizeof(double));
+    auto epi = IterEpis<T, M, true>::primal(b, s, sizeof(T) != sizeof(double));
     Partials part;
     part.zero();
     for (int j = r0 + static_cast<int>(threadIdx.x); j < r1; j += kBlock) {
@@ -292,12 +295,12 @@ __device__ __forceinline__ void pass_items(Epi &epi, const double *__restrict__ 
 
 // Dual step over rows from the column-blocked (or panel group) partial sums
 // (k_dual's epilogue; the nb partials are added in block order).
-template <class T, bool H>
+template <class T, int M>
 __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *__restrict__ part, int nb) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
-    auto epi = IterEpis<T, H, true>::dual(b, s);
+    auto epi = IterEpis<T, M, true>::dual(b, s);
     Partials acc_part;
     acc_part.zero();
     pass_items(epi, part, b.m, nb, acc_part);
@@ -307,12 +310,12 @@ __global__ __launch_bounds__(kBlock) void k_dual_rows(Bufs<T> b, const double *_
 // Primal step over columns from the panel layout's partial sums of A' y+
 // (k_primal_own's column pass; the nb group partials are added in group
 // order).
-template <class T, bool H>
+template <class T, int M>
 __global__ __launch_bounds__(kBlock) void k_primal_cols(Bufs<T> b, const double *__restrict__ part, int nb) {
     const DevState s = *b.st;
     if (s.status != VK_PDHG_RUNNING) return;
     __shared__ double scratch[kSlotScratch];
-    auto epi = IterEpis<T, H, true>::primal(b, s, true);
+    auto epi = IterEpis<T, M, true>::primal(b, s, true);
     Partials acc_part;
     acc_part.zero();
     pass_items(epi, part, b.n, nb, acc_part);
@@ -491,25 +494,6 @@ __global__ __launch_bounds__(kBlock) void k_halpern_apply(Bufs<T> b) {
 // corrections fold into (xbase, ybase) in FP64, A xbase and A'ybase are
 // recomputed with the FP64 matrix, the shifted LP is rounded to FP32 once,
 // and every correction restarts from zero.
-template <class T>
-struct RecentreBufs {
-    double *xb, *yb;                 // base point (n, m)
-    const double *c, *l, *u;         // scaled LP in FP64 (n)
-    const double *rl, *ru;           // (m)
-    double *ax, *aty;                // A xb, A' yb (FP64 sums)
-    T *cs, *ls, *us, *rls, *rus;     // the shifted LP (aliases of Bufs c, l, u, rl, ru)
-};
-
-// The dual sign rule of the Moreau step (y+ = 0 inside [rl, ru], y+ <= 0 at
-// rl, y+ >= 0 at ru): a total formed as ybase + v keeps it up to the
-// rounding of v, which this removes (a wrong-signed dual on a one-sided row
-// would make the dual objective -inf).
-__device__ __forceinline__ double dual_sign_project(double y, double rl, double ru) {
-    if (y > 0.0 && isinf(ru)) return 0.0;
-    if (y < 0.0 && isinf(rl)) return 0.0;
-    return y;
-}
-
 // Whether the recentring kernels after a restart check act: at a restart
 // that the finalizer applied (or always, when forced at setup).
 __device__ __forceinline__ bool recentre_now(const DevState *st, int force) {
@@ -881,33 +865,27 @@ Bufs<float> &bufs<float>(vk_hip_pdhg *s) { return s->b32; }
 // column-blocked SpMV and its row pass.
 // `part` selects the kernels: 1 = the SpMV (or the fused kernel), 2 = the
 // row pass of a split layout (nothing when fused), 3 = both.
-template <class T, bool H>
-void launch_dual_policy(vk_hip_pdhg *s, hipStream_t st, int part) {
+template <class T, int M>
+void launch_dual_mode(vk_hip_pdhg *s, hipStream_t st, int part) {
     Bufs<T> &b = bufs<T>(s);
     if (s->panels) {
         if (part & 1) (void)panel_spmv_launch(s->pan_a, b.st, 1, b.x[0], b.x[1], s->blk_part, st);
-        if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->pan_a.groups);
+        if (part & 2) k_dual_rows<T, M><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->pan_a.groups);
     } else if (s->col_blocks > 1) {
         if (part & 1) k_blocks<T><<<s->grid_blk, kBlock, 0, st>>>(b, blocked<T>(s), s->blk_part, 0);
-        if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
+        if (part & 2) k_dual_rows<T, M><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, s->col_blocks);
     } else if (s->split) {
         if (part & 1) k_dual_spmv<T><<<s->grid_spmv_rows, kBlock, 0, st>>>(b, view_a<T>(s), s->blk_part);
-        if (part & 2) k_dual_rows<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, 1);
+        if (part & 2) k_dual_rows<T, M><<<s->grid_rows, kBlock, 0, st>>>(b, s->blk_part, 1);
     } else if (part & 1) {
-        k_dual<T, H><<<s->grid_rows, kBlock, 0, st>>>(b, view_a<T>(s));
+        k_dual<T, M><<<s->grid_rows, kBlock, 0, st>>>(b, view_a<T>(s));
     }
 }
 
-template <class T>
-void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
-    if (s->params.policy == VK_PDHG_POLICY_HALPERN) launch_dual_policy<T, true>(s, st, part);
-    else launch_dual_policy<T, false>(s, st, part);
-}
-
 // A' y+ with the residual terms and the fused next primal step (`part` as
-// for launch_dual).
-template <class T>
-void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+// for launch_dual_mode).
+template <class T, int M>
+void launch_primal_mode(vk_hip_pdhg *s, hipStream_t st, int part) {
     Bufs<T> &b = bufs<T>(s);
     if constexpr (sizeof(T) == sizeof(double)) {
         if (s->Q && (part & 1)) {
@@ -915,22 +893,38 @@ void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
             k_qx<T><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(b, Q);
         }
     }
-    const bool h = s->params.policy == VK_PDHG_POLICY_HALPERN;
     if (s->panels) {
         if (part & 1) (void)panel_spmv_launch(s->pan_at, b.st, 0, b.y, b.y, s->tpart, st);
-        if (part & 2) {
-            if (h) k_primal_cols<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, s->tpart, s->pan_at.groups);
-            else k_primal_cols<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, s->tpart, s->pan_at.groups);
-        }
+        if (part & 2) k_primal_cols<T, M><<<s->grid_cols, kBlock, 0, st>>>(b, s->tpart, s->pan_at.groups);
     } else if (s->split) {
-        if (part & 1) {
-            if (h) k_primal_own<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
-            else k_primal_own<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
-        }
+        if (part & 1) k_primal_own<T, M><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
     } else if (part & 1) {
-        if (h) k_primal<T, true><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
-        else k_primal<T, false><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
+        k_primal<T, M><<<s->grid_cols, kBlock, 0, st>>>(b, view_at<T>(s));
     }
+}
+
+// The epilogue mode of the solver's iteration kernels: the recentred form
+// only exists for FP32 storage.
+template <class T>
+int epi_mode(const vk_hip_pdhg *s) {
+    if (s->params.policy != VK_PDHG_POLICY_HALPERN) return kEpiPlain;
+    return (sizeof(T) == sizeof(float) && s->recentre) ? kEpiRecentred : kEpiHalpern;
+}
+
+template <class T>
+void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+    const int mode = epi_mode<T>(s);
+    if (mode == kEpiPlain) launch_dual_mode<T, kEpiPlain>(s, st, part);
+    else if (mode == kEpiHalpern) launch_dual_mode<T, kEpiHalpern>(s, st, part);
+    else if constexpr (sizeof(T) == sizeof(float)) launch_dual_mode<T, kEpiRecentred>(s, st, part);
+}
+
+template <class T>
+void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
+    const int mode = epi_mode<T>(s);
+    if (mode == kEpiPlain) launch_primal_mode<T, kEpiPlain>(s, st, part);
+    else if (mode == kEpiHalpern) launch_primal_mode<T, kEpiHalpern>(s, st, part);
+    else if constexpr (sizeof(T) == sizeof(float)) launch_primal_mode<T, kEpiRecentred>(s, st, part);
 }
 
 // A xbar of the running average and its row terms.

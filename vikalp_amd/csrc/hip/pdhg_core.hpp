@@ -100,6 +100,27 @@ __device__ __forceinline__ double primal_weight_update(DevState &S, double dx, d
     return exp(S.pw_kp * target + (1.0 - S.pw_kp) * log(S.omega) - S.pw_ki * S.pw_int);
 }
 
+// The recentred FP32 solver's FP64 base point (x = xb + correction, y = yb +
+// correction) and the FP64 scaled LP the shifted one is rounded from.
+template <class T>
+struct RecentreBufs {
+    double *xb, *yb;                 // base point (n, m)
+    const double *c, *l, *u;         // scaled LP in FP64 (n)
+    const double *rl, *ru;           // (m)
+    double *ax, *aty;                // A xb, A' yb (FP64 sums)
+    T *cs, *ls, *us, *rls, *rus;     // the shifted LP (aliases of Bufs c, l, u, rl, ru)
+};
+
+// The dual sign rule of the Moreau step (y+ = 0 inside [rl, ru], y+ <= 0 at
+// rl, y+ >= 0 at ru): a total formed as ybase + v keeps it up to the
+// rounding of v, which this removes (a wrong-signed dual on a one-sided row
+// would make the dual objective -inf).
+__device__ __forceinline__ double dual_sign_project(double y, double rl, double ru) {
+    if (y > 0.0 && isinf(ru)) return 0.0;
+    if (y < 0.0 && isinf(rl)) return 0.0;
+    return y;
+}
+
 __device__ __forceinline__ double kkt_weighted(double w, double pr2, double dr2, double gap) {
     return sqrt(w * w * pr2 + dr2 / (w * w) + gap * gap);
 }
@@ -191,7 +212,8 @@ __device__ __forceinline__ bool check_iteration(const DevState &s) { return (s.k
 // (y_k, A x_k) with A xbar = 2 A x+ - A x_k; then the Halpern update of y
 // and of A x.  Row slots as DualEpi on (A x+, y+), plus slot 6
 // sum (y+ - y_k)^2 and, at checks, slot 5 sum (y+ - y_restart)^2.
-template <class T, bool Stream = false>
+// R: the recentred solver's form (Bufs::ybase; FP32 storage only).
+template <class T, bool Stream = false, bool R = false>
 struct HalpernDualEpi {
     const Bufs<T> &b;
     double sigma, inv_sigma;
@@ -211,7 +233,8 @@ struct HalpernDualEpi {
         p.y0 = vec_load<Stream>(b.y0h + i);
         p.ax0 = vec_load<Stream>(b.ax0h + i);
         p.ylast = check ? vec_load<Stream>(b.ylast + i) : T(0);
-        p.yb = b.ybase ? vec_load<Stream>(b.ybase + i) : 0.0;
+        if constexpr (R) p.yb = vec_load<Stream>(b.ybase + i);
+        else p.yb = 0.0;
         return p;
     }
     __device__ void row(int i, double acc, const Pre &q, Partials &p) const {
@@ -222,10 +245,10 @@ struct HalpernDualEpi {
         // Recentred: the step acts on the total dual ybase + v and the stored
         // y+ is its correction, both formed in FP64, so that only the
         // correction is rounded to T.  yt is the exact (sign-exact) total.
-        const double vt = b.ybase ? v + q.yb : v;
+        const double vt = R ? v + q.yb : v;
         const double t = inv_sigma * vt;   // sign-exact Moreau step, as for the adaptive policy
         const double yt = (t < lo) ? fmin(vt - sigma * lo, 0.0) : ((hi < t) ? fmax(vt - sigma * hi, 0.0) : 0.0);
-        const double yn = b.ybase ? yt - q.yb : yt;
+        const double yn = R ? yt - q.yb : yt;
         const T y_stored = static_cast<T>(yn);
         b.y[i] = y_stored;
         const double yp = static_cast<double>(y_stored);
@@ -321,7 +344,8 @@ struct PrimalEpi {
 // terms of (x+, c + A' y+) (slot 6 at checks: sum (x+ - x_restart)^2), slot 7
 // sum (x+ - x_k)^2, the Halpern update of x and of A' y, and (fused) the next
 // PDHG primal step x+ = P_X((x_{k+1} - tau c) - tau A' y_{k+1}).
-template <class T, bool Stream = false>
+// R: the recentred solver's form (Bufs::cobj; FP32 storage only).
+template <class T, bool Stream = false, bool R = false>
 struct HalpernPrimalEpi {
     const Bufs<T> &b;
     const T *xcur;   // x+ of this iteration
@@ -344,7 +368,8 @@ struct HalpernPrimalEpi {
         p.x0 = vec_load<Stream>(b.x0h + j);
         p.aty0 = vec_load<Stream>(b.aty0h + j);
         p.xlast = check ? vec_load<Stream>(b.xlast + j) : T(0);
-        p.cobj = b.cobj ? vec_load<Stream>(b.cobj + j) : T(0);
+        if constexpr (R) p.cobj = vec_load<Stream>(b.cobj + j);
+        else p.cobj = T(0);
         return p;
     }
     __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
@@ -353,7 +378,7 @@ struct HalpernPrimalEpi {
         const double lj = static_cast<double>(q.l), uj = static_cast<double>(q.u);
         // Recentred: c holds c + A'ybase, so the total gradient is c + acc
         // as before, while the primal objective terms take c itself.
-        const double cobj = b.cobj ? static_cast<double>(q.cobj) : cj;
+        const double cobj = R ? static_cast<double>(q.cobj) : cj;
         column_terms(xp, cobj, cj + acc, lj, uj, static_cast<double>(q.s), check, static_cast<double>(q.xlast), p);
         const double dz = xp - x_old;
         p.v[7] += dz * dz;
