@@ -76,18 +76,20 @@ struct StoreEpi {
 // epilogues, the Halpern ones, and the Halpern ones of the recentred FP32
 // solver (which read the dual base and the objective beside the shifted
 // gradient).
-constexpr int kEpiPlain = 0, kEpiHalpern = 1, kEpiRecentred = 2;
+// kEpiHalpernQP: the Halpern ones with a quadratic objective (FP64 only).
+constexpr int kEpiPlain = 0, kEpiHalpern = 1, kEpiRecentred = 2, kEpiHalpernQP = 3;
 
 // The dual and primal epilogues of one iteration in mode M.
 template <class T, int M, bool Stream>
 struct IterEpis {
     static constexpr bool R = M == kEpiRecentred;
+    static constexpr bool QP = M == kEpiHalpernQP;
     static __device__ HalpernDualEpi<T, Stream, R> dual(const Bufs<T> &b, const DevState &s) {
         return HalpernDualEpi<T, Stream, R>{b, s.sigma, s.inv_sigma, halpern_weights(s.ks), check_iteration(s)};
     }
-    static __device__ HalpernPrimalEpi<T, Stream, R> primal(const Bufs<T> &b, const DevState &s, bool) {
-        return HalpernPrimalEpi<T, Stream, R>{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, may_fuse(s),
-                                              check_iteration(s), halpern_weights(s.ks)};
+    static __device__ HalpernPrimalEpi<T, Stream, R, QP> primal(const Bufs<T> &b, const DevState &s, bool) {
+        return HalpernPrimalEpi<T, Stream, R, QP>{b, b.x[(s.k + 1) & 1], b.x[s.k & 1], s.tau, may_fuse(s),
+                                                  check_iteration(s), halpern_weights(s.ks)};
     }
 };
 
@@ -182,6 +184,15 @@ void k_primal_own(Bufs<T> b, CsrView<T> AT) {
     store_partials(part, b.part_cols, scratch);
 }
 
+
+// sum_k |q_jk| of every row j of Q (the Halpern policy's column steps).
+__global__ __launch_bounds__(kBlock) void k_row_abs_sums(CsrView<double> Q, double *out) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < Q.rows; j += gridDim.x * kBlock) {
+        double sum = 0.0;
+        for (int k = Q.row_ptr[j]; k < Q.row_ptr[j + 1]; ++k) sum += fabs(Q.val[k]);
+        out[j] = sum;
+    }
+}
 
 // Q x+ for a quadratic objective (pdqp.cpp:116, :149): FP64 sums into b.qx,
 // before the primal kernel that uses them.
@@ -472,7 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_halpern_apply(Bufs<T> b) {
     T *xnext = b.x[(s.k + 1) & 1];
     const int total = b.n > b.m ? b.n : b.m;
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
-        if (t < b.n) halpern_apply_column(b, s, t, xp, xnext);
+        if (t < b.n) {
+            if (b.qxh) halpern_apply_column<T, true>(b, s, t, xp, xnext);
+            else halpern_apply_column<T, false>(b, s, t, xp, xnext);
+        }
         if (t < b.m) halpern_apply_row(b, s, t);
     }
 }
@@ -904,19 +918,27 @@ void launch_primal_mode(vk_hip_pdhg *s, hipStream_t st, int part) {
 }
 
 // The epilogue mode of the solver's iteration kernels: the recentred form
-// only exists for FP32 storage.
+// only exists for FP32 storage, the quadratic one for FP64.
 template <class T>
 int epi_mode(const vk_hip_pdhg *s) {
     if (s->params.policy != VK_PDHG_POLICY_HALPERN) return kEpiPlain;
+    if (sizeof(T) == sizeof(double) && s->Q) return kEpiHalpernQP;
     return (sizeof(T) == sizeof(float) && s->recentre) ? kEpiRecentred : kEpiHalpern;
 }
 
 template <class T>
 void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
     const int mode = epi_mode<T>(s);
-    if (mode == kEpiPlain) launch_dual_mode<T, kEpiPlain>(s, st, part);
-    else if (mode == kEpiHalpern) launch_dual_mode<T, kEpiHalpern>(s, st, part);
-    else if constexpr (sizeof(T) == sizeof(float)) launch_dual_mode<T, kEpiRecentred>(s, st, part);
+    if (mode == kEpiPlain) {
+        launch_dual_mode<T, kEpiPlain>(s, st, part);
+    } else if (mode == kEpiHalpern) {
+        launch_dual_mode<T, kEpiHalpern>(s, st, part);
+    } else if constexpr (sizeof(T) == sizeof(float)) {
+        launch_dual_mode<T, kEpiRecentred>(s, st, part);
+    } else {
+        // The dual epilogue has no quadratic term: the kernels of kEpiHalpern.
+        launch_dual_mode<T, kEpiHalpern>(s, st, part);
+    }
 }
 
 template <class T>
@@ -925,6 +947,7 @@ void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
     if (mode == kEpiPlain) launch_primal_mode<T, kEpiPlain>(s, st, part);
     else if (mode == kEpiHalpern) launch_primal_mode<T, kEpiHalpern>(s, st, part);
     else if constexpr (sizeof(T) == sizeof(float)) launch_primal_mode<T, kEpiRecentred>(s, st, part);
+    else launch_primal_mode<T, kEpiHalpernQP>(s, st, part);
 }
 
 // A xbar of the running average and its row terms.
@@ -1413,6 +1436,23 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
             if (Q.ntiles > 0)
                 k_spmv_store<double><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(
                     Q, b.x[0], b.qx);
+            if (s->params.policy == VK_PDHG_POLICY_HALPERN) {
+                // Q x0 is Q x of the Halpern iterate and of the anchor; the
+                // absolute row sums of Q set the column steps.
+                b.qxh = s->alloc<double>(n);
+                b.qx0h = s->alloc<double>(n);
+                b.qx_avg = s->alloc<double>(n);
+                double *qabs = s->alloc<double>(n);
+                if (!b.qxh || !b.qx0h || !b.qx_avg || !qabs) {
+                    set_error("vk_hip_pdqp_create: device allocation failed");
+                    return VK_ENOMEM;
+                }
+                k_copy<double><<<egrid(n), kBlock, 0, st>>>(n, b.qx, b.qxh);
+                k_copy<double><<<egrid(n), kBlock, 0, st>>>(n, b.qx, b.qx0h);
+                k_copy<double><<<egrid(n), kBlock, 0, st>>>(n, b.qx, b.qx_avg);
+                k_row_abs_sums<<<egrid(n), kBlock, 0, st>>>(Q, qabs);
+                b.qabs = qabs;
+            }
         }
     }
     // First primal step x1 = P(x0 - tau (c + A'y0)) into x[1].
@@ -1743,10 +1783,11 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     if (Q) {
         VK_REQUIRE(Q->has_values, "vk_hip_pdqp_create: Q values were never set");
         VK_REQUIRE(Q->rows == A->cols && Q->cols == A->cols, "vk_hip_pdqp_create: Q must be n x n");
-        VK_REQUIRE(params->policy == VK_PDHG_POLICY_REFERENCE && params->precision == VK_PREC_FP64 &&
-                       params->relative_kkt_tolerance == 0.0,
-                   "vk_hip_pdqp_create: a quadratic objective runs the reference policy in FP64 "
-                   "(pdqp.cpp) without the relative-KKT test");
+        VK_REQUIRE(params->precision == VK_PREC_FP64 &&
+                       ((params->policy == VK_PDHG_POLICY_REFERENCE && params->relative_kkt_tolerance == 0.0) ||
+                        params->policy == VK_PDHG_POLICY_HALPERN),
+                   "vk_hip_pdqp_create: a quadratic objective runs in FP64 either the reference policy "
+                   "(pdqp.cpp) without the relative-KKT test or the Halpern policy");
     }
     VK_REQUIRE(A->has_values, "vk_hip_pdhg_create: matrix values were never set");
     VK_REQUIRE(A->cols > 0, "vk_hip_pdhg_create: the LP needs at least one variable");

@@ -81,6 +81,12 @@ struct Bufs {
     // ybase + y; cobj is the objective c itself (primal objective terms).
     const double *ybase;
     const T *cobj;
+    // Halpern with a quadratic objective (FP64; null otherwise): Q x of the
+    // Halpern iterate and of the anchor, carried by linearity like A'y, Q x+
+    // kept at restart checks, and the absolute row sums of Q that set each
+    // column's primal step (qp_column_step).
+    double *qxh, *qx0h, *qx_avg;
+    const double *qabs;
     double *part_rows, *part_cols, *part_erows, *part_ecols;
     DevState *st;
     int m, n;
@@ -274,6 +280,26 @@ struct HalpernDualEpi {
     }
 };
 
+// The PDQP primal step before projection, x - tau (A'y + (c + Q x))
+// (pdqp.cpp:115-122, the order of pstep_element's quadratic form).
+__device__ __forceinline__ double qp_step(double x, double c, double qx, double aty, double tau) {
+    return -tau * (aty + (c + qx)) + x;
+}
+
+// Primal step of column j under the Halpern policy with a quadratic
+// objective: 1/tau_j = 1/tau + sum_k |q_jk|, tau = eta/w the LP step, sigma =
+// eta w unchanged.  The linearised PDHG step is then a proximal point step in
+// the metric [T^-1 - Q, -A'; -A, I/sigma] with T = diag(tau_j), positive
+// definite because T^-1 - Q - sigma A'A = (diag(sum_k |q_jk|) - Q) + (I/tau
+// - sigma A'A): a diagonally dominant matrix with a non-negative diagonal
+// (PSD) plus (1 - eta^2 ||A||^2)/tau I (eta ||A|| = 0.99).  So the map is
+// firmly nonexpansive in that metric and the reflected Halpern step stays
+// nonexpansive.  Per column, unlike a scalar 1/(1/tau + ||Q||), the step
+// follows the curvature each column has after the column scaling.
+__device__ __forceinline__ double qp_column_step(double tau, double qabs) {
+    return 1.0 / (1.0 / tau + qabs);
+}
+
 // Column terms of the KKT sums for (x_j, g_j = c_j + aty_j) (shared by the
 // per-iteration and the average evaluation).  s = col_scale[j].
 __device__ __forceinline__ void column_terms(double xj, double cj, double g, double lj, double uj,
@@ -344,8 +370,11 @@ struct PrimalEpi {
 // terms of (x+, c + A' y+) (slot 6 at checks: sum (x+ - x_restart)^2), slot 7
 // sum (x+ - x_k)^2, the Halpern update of x and of A' y, and (fused) the next
 // PDHG primal step x+ = P_X((x_{k+1} - tau c) - tau A' y_{k+1}).
-// R: the recentred solver's form (Bufs::cobj; FP32 storage only).
-template <class T, bool Stream = false, bool R = false>
+// R: the recentred solver's form (Bufs::cobj; FP32 storage only).  QP: a
+// quadratic objective (Bufs::qx = Q x+ from k_qx; gradient A'y + (c + Q x),
+// as pdqp.cpp:148-152; objectives with the 1/2 x'Qx terms; Q x of the
+// Halpern iterate carried like A'y; FP64 only).
+template <class T, bool Stream = false, bool R = false, bool QP = false>
 struct HalpernPrimalEpi {
     const Bufs<T> &b;
     const T *xcur;   // x+ of this iteration
@@ -355,6 +384,7 @@ struct HalpernPrimalEpi {
     HalpernWeights w;
     struct Pre {
         T xp, xh, aty, c, l, u, s, x0, aty0, xlast, cobj;
+        double qx, qxh, qx0, qabs;
     };
     __device__ Pre prefetch(int j) const {
         Pre p;
@@ -370,6 +400,14 @@ struct HalpernPrimalEpi {
         p.xlast = check ? vec_load<Stream>(b.xlast + j) : T(0);
         if constexpr (R) p.cobj = vec_load<Stream>(b.cobj + j);
         else p.cobj = T(0);
+        if constexpr (QP) {
+            p.qx = vec_load<Stream>(b.qx + j);
+            p.qxh = vec_load<Stream>(b.qxh + j);
+            p.qx0 = vec_load<Stream>(b.qx0h + j);
+            p.qabs = vec_load<Stream>(b.qabs + j);
+        } else {
+            p.qx = p.qxh = p.qx0 = p.qabs = 0.0;
+        }
         return p;
     }
     __device__ void row(int j, double acc, const Pre &q, Partials &p) const {
@@ -379,7 +417,17 @@ struct HalpernPrimalEpi {
         // Recentred: c holds c + A'ybase, so the total gradient is c + acc
         // as before, while the primal objective terms take c itself.
         const double cobj = R ? static_cast<double>(q.cobj) : cj;
-        column_terms(xp, cobj, cj + acc, lj, uj, static_cast<double>(q.s), check, static_cast<double>(q.xlast), p);
+        if constexpr (QP) {
+            column_terms(xp, cobj, acc + (cj + q.qx), lj, uj, static_cast<double>(q.s), check,
+                         static_cast<double>(q.xlast), p);
+            // 1/2 x'Qx: added to the primal objective and taken from the
+            // (Wolfe) dual objective -1/2 x'Qx + min_X g'x - ...
+            const double half = 0.5 * xp * q.qx;
+            p.v[5] += half;
+            p.v[3] -= half;
+        } else {
+            column_terms(xp, cobj, cj + acc, lj, uj, static_cast<double>(q.s), check, static_cast<double>(q.xlast), p);
+        }
         const double dz = xp - x_old;
         p.v[7] += dz * dz;
         const T xh_new = static_cast<T>(w.w1 * (2.0 * xp - x_old) + w.w0 * static_cast<double>(q.x0));
@@ -388,10 +436,20 @@ struct HalpernPrimalEpi {
         vec_store<Stream>(b.xh + j, xh_new);
         vec_store<Stream>(b.aty + j, aty_new);
         if (check) vec_store<Stream>(b.aty_avg + j, static_cast<T>(acc));
-        if (fuse)
+        if constexpr (QP) {
+            const double qxh_new = w.w1 * (2.0 * q.qx - q.qxh) + w.w0 * q.qx0;
+            vec_store<Stream>(b.qxh + j, qxh_new);
+            if (check) vec_store<Stream>(b.qx_avg + j, q.qx);
+            if (fuse)
+                xnext[j] = static_cast<T>(clamp_box(qp_step(static_cast<double>(xh_new), cj, qxh_new,
+                                                            static_cast<double>(aty_new),
+                                                            qp_column_step(tau, q.qabs)),
+                                                    lj, uj));
+        } else if (fuse) {
             xnext[j] = static_cast<T>(clamp_box((static_cast<double>(xh_new) - tau * cj) -
                                                     tau * static_cast<double>(aty_new),
                                                 lj, uj));
+        }
     }
 };
 
@@ -687,7 +745,9 @@ __device__ __forceinline__ void apply_row(const Bufs<T> &b, const DevState &s, i
 // next step): on a restart z = z0 = T(z_k), i.e. x+ (xp) and the A' y+ kept
 // in aty_avg, which also become the primal weight's restart point; then the
 // next PDHG primal step x+ = P_X((xh - tau c) - tau A' yh) into xnext.
-template <class T>
+// QP: a quadratic objective (Bufs::qxh; FP64), whose Q x of the Halpern
+// iterate and anchor restart at the Q x+ kept in qx_avg.
+template <class T, bool QP = false>
 __device__ __forceinline__ void halpern_apply_column(const Bufs<T> &b, const DevState &s, int j, const T *xp,
                                                      T *xnext) {
     T xh = b.xh[j], aty = b.aty[j];
@@ -699,6 +759,18 @@ __device__ __forceinline__ void halpern_apply_column(const Bufs<T> &b, const Dev
         b.x0h[j] = xh;
         b.aty0h[j] = aty;
         b.xlast[j] = xh;
+    }
+    if constexpr (QP) {
+        double qxh = b.qxh[j];
+        if (s.restart_apply) {
+            qxh = b.qx_avg[j];
+            b.qxh[j] = qxh;
+            b.qx0h[j] = qxh;
+        }
+        xnext[j] = static_cast<T>(clamp_box(qp_step(static_cast<double>(xh), static_cast<double>(b.c[j]), qxh,
+                                                    static_cast<double>(aty), qp_column_step(s.tau, b.qabs[j])),
+                                            static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
+        return;
     }
     xnext[j] = static_cast<T>(clamp_box((static_cast<double>(xh) - s.tau * static_cast<double>(b.c[j])) -
                                             s.tau * static_cast<double>(aty),
@@ -723,7 +795,9 @@ template <class T>
 __device__ __forceinline__ void pstep_element(const Bufs<T> &b, const DevState &s, int j, const T *xcur, T *xnext) {
     const double xj = static_cast<double>(xcur[j]);
     const double cj = static_cast<double>(b.c[j]), aj = static_cast<double>(b.aty[j]);
-    const double z = b.qx ? -s.tau * (aj + (cj + b.qx[j])) + xj : (xj - s.tau * cj) - s.tau * aj;
+    double z;
+    if (b.qabs) z = qp_step(xj, cj, b.qx[j], aj, qp_column_step(s.tau, b.qabs[j]));   // Halpern with Q
+    else z = b.qx ? -s.tau * (aj + (cj + b.qx[j])) + xj : (xj - s.tau * cj) - s.tau * aj;
     const T xn = static_cast<T>(clamp_box(z, static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
     xnext[j] = xn;
     if (b.xsum) b.xsum[j] += static_cast<double>(xn);

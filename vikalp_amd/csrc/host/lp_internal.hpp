@@ -70,6 +70,10 @@ Scaled device_scale(vk_hip_csr *A, const Model &model, const Bounds &bounds, boo
 int apply_tile_geometry(vk_hip_csr *A, const SolverOptions &options);
 // "" for the default geometry, else the note SolveResult::message carries.
 std::string tile_geometry_note(int geometry, bool tuned);
+// Relative KKT error in original coordinates into res.kkt_* (pr.g the
+// stationarity c + A'y, or c + Qx + A'y for a QP with xqx = x'Qx).
+void relative_kkt(const Model &model, const Bounds &bounds, const std::vector<Scalar> &x,
+                  const std::vector<Scalar> &y, const Products &pr, SolveResult &res, Scalar xqx);
 void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &options,
                const std::vector<Scalar> &x, const std::vector<Scalar> &y, const Products &products,
                bool relative_mode, SolveResult &res);                      // pdhg.cpp:153-264

@@ -300,9 +300,11 @@ Scalar norm2(const std::vector<Scalar> &v) {
 //   rel_d = ||x - P_[l,u](x - g)||_2 / (1 + ||c||_2)
 //   gap   = |c'x - D(y)| / (1 + |c'x| + |D(y)|),
 //   D(y)  = sum_j min(g_j l_j, g_j u_j) - sum_i (max(y_i,0) ru_i - max(-y_i,0) rl_i)
-// with 0*inf = 0; D = -inf gives gap = 1.
+// with 0*inf = 0; D = -inf gives gap = 1.  A QP passes xqx = x'Qx (with
+// g = c + Qx + A'y): the primal objective gains 1/2 x'Qx and the (Wolfe)
+// dual objective -1/2 x'Qx + D(y).
 void relative_kkt(const Model &model, const Bounds &bounds, const std::vector<Scalar> &x,
-                  const std::vector<Scalar> &y, const Products &pr, SolveResult &res) {
+                  const std::vector<Scalar> &y, const Products &pr, SolveResult &res, Scalar xqx) {
     const auto m = static_cast<std::size_t>(model.constraint_matrix.pattern.rows);
     const auto n = static_cast<std::size_t>(model.constraint_matrix.pattern.columns);
     const std::vector<Scalar> &g = pr.g;
@@ -331,6 +333,8 @@ void relative_kkt(const Model &model, const Bounds &bounds, const std::vector<Sc
             if (std::isinf(u)) neg_inf = true; else dual += g[j] * u;
         }
     }
+    pobj += 0.5 * xqx;
+    dual -= 0.5 * xqx;
     res.kkt_relative_primal =
         std::sqrt(pres) / (1.0 + row_bound_norm(model.constraint_lower, model.constraint_upper));
     res.kkt_relative_dual = std::sqrt(dres) / (1.0 + norm2(model.linear_objective));
@@ -492,7 +496,7 @@ void postsolve(const Model &model, const Bounds &bounds, const SolverOptions &op
         res.absolute_gap = 0.0;
         res.relative_gap = 0.0;
     }
-    relative_kkt(model, bounds, x, y, pr, res);
+    relative_kkt(model, bounds, x, y, pr, res, 0.0);
 }
 
 void check_vk(int rc, const char *what) {

@@ -6,7 +6,11 @@
 // and the postsolve pdqp.cpp:178-300.  Two executions: the fused device
 // iteration (vk_hip_pdqp_create: one more SpMV per iteration, Q x+, ahead of
 // the primal kernel) and the reference's op-by-op sequence on any
-// ExecutionBackend.
+// ExecutionBackend.  Beyond the reference, SolverOptions::pdhg_policy =
+// Halpern runs restarted Halpern PDHG on the QP (fused, FP64): the LP
+// path's equilibrated device scaling, per-column primal steps 1 / (w / eta +
+// sum_k |q_jk|) that keep the PDHG metric positive definite, and optionally
+// the relative-KKT test with the 1/2 x'Qx terms.
 
 #include "lp_internal.hpp"
 
@@ -110,9 +114,13 @@ ScaledQ scale_q(const CsrMatrix &q, const std::vector<Scalar> &s) {
     return out;
 }
 
-// pdqp.cpp:178-300 on original-coordinate x, y (reference sign).
+// pdqp.cpp:178-300 on original-coordinate x, y (reference sign), and the
+// relative KKT error of the QP.  relative_mode: the device stopped on the
+// relative-KKT test, itself in original coordinates, so the absolute
+// re-check of pdqp.cpp does not apply.
 void postsolve_qp(const Model &model, const Bounds &bounds, const SolverOptions &options,
-                  const std::vector<Scalar> &x, const std::vector<Scalar> &y, SolveResult &res) {
+                  const std::vector<Scalar> &x, const std::vector<Scalar> &y, bool relative_mode,
+                  SolveResult &res) {
     const auto &A = model.constraint_matrix;
     const auto &Q = model.quadratic_objective;
     const auto m = static_cast<std::size_t>(A.pattern.rows);
@@ -197,7 +205,7 @@ void postsolve_qp(const Model &model, const Bounds &bounds, const SolverOptions 
     if (!finite) {
         res.status = SolveStatus::NumericalFailure;
         res.message = "PDQP produced a non-finite iterate";
-    } else if (res.status == SolveStatus::Optimal &&
+    } else if (res.status == SolveStatus::Optimal && !relative_mode &&
                (res.primal_residual > options.primal_tolerance ||
                 res.stationarity_residual > options.dual_tolerance)) {
         res.status = SolveStatus::NumericalFailure;
@@ -209,6 +217,10 @@ void postsolve_qp(const Model &model, const Bounds &bounds, const SolverOptions 
         res.absolute_gap = 0.0;
         res.relative_gap = 0.0;
     }
+    Products pr;
+    pr.ax = std::move(row_values);
+    pr.g = std::move(g);
+    relative_kkt(model, bounds, x, y, pr, res, quad);
 }
 
 // pdqp.cpp:110-176 issued call for call on an ExecutionBackend.
@@ -311,10 +323,14 @@ struct DeviceSolverQ {
     ~DeviceSolverQ() { vk_hip_pdhg_destroy(h); }
 };
 
-// The fused device iteration with Q (reference policy, FP64).
-void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, const vk_hip_csr *Q,
-                   Scalar max_step, const SolverOptions &options, const std::vector<Scalar> *warm_scaled,
-                   Clock::time_point started, std::vector<Scalar> &xs, std::vector<Scalar> &ys, SolveResult &res) {
+// The fused device iteration with Q in FP64: the reference policy with its
+// step max_step, or (halpern) restarted Halpern PDHG with eta = 0.99 /
+// ||A||_2 (the device sets each column's primal step from Q's absolute row
+// sums).  Returns the VK_PDHG_* status.
+int iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, const vk_hip_csr *Q,
+                  Scalar max_step, bool halpern, const SolverOptions &options,
+                  const std::vector<Scalar> *warm_scaled, Clock::time_point started, std::vector<Scalar> &xs,
+                  std::vector<Scalar> &ys, SolveResult &res) {
     vk_pdhg_params p{};
     p.policy = VK_PDHG_POLICY_REFERENCE;
     p.precision = VK_PREC_FP64;
@@ -322,6 +338,17 @@ void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, co
     p.metric_tolerance = std::max(options.primal_tolerance, options.dual_tolerance);
     p.step_size = max_step;
     p.primal_weight = 1.0;
+    if (halpern) {
+        p.policy = VK_PDHG_POLICY_HALPERN;
+        p.relative_kkt_tolerance = options.relative_kkt_tolerance;
+        double norm = 0.0;
+        check_vk(vk_hip_csr_norm2_estimate(A, 100, &norm), "vk_hip_csr_norm2_estimate");
+        p.step_size = 0.99 / std::max(norm, 1e-12);
+        const Scalar nc = norm2(sc.c), nb = row_bound_norm(sc.rl, sc.ru);
+        p.primal_weight = (nc > 1e-10 && nb > 1e-10) ? nc / nb : 1.0;
+        p.primal_weight_gain = options.pdhg_primal_weight_gain;
+        p.primal_weight_integral = options.pdhg_primal_weight_integral;
+    }
     p.restart_period = options.restart_period;
     p.use_graphs = options.use_graphs ? 1 : 0;
     p.col_blocks = options.spmv_col_blocks;
@@ -362,6 +389,7 @@ void iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, co
     xs.resize(static_cast<std::size_t>(model.num_variables()));
     ys.resize(static_cast<std::size_t>(model.num_constraints()));
     check_vk(vk_hip_pdhg_download(solver.h, xs.data(), ys.data()), "vk_hip_pdhg_download");
+    return state.status;
 }
 
 }  // namespace
@@ -385,14 +413,13 @@ SolveResult solve_qp(const Model &model, ExecutionBackend &backend, const Solver
         return status_result(SolveStatus::InvalidModel, "solver limits and tolerances must be non-negative");
     if (options.continuous_method != ContinuousMethod::Auto && options.continuous_method != ContinuousMethod::PDHG)
         return status_result(SolveStatus::UnsupportedModel, "requested continuous method is not implemented for QP");
-    if (options.pdhg_policy != PdhgPolicy::Reference || options.pdhg_precision != PdhgPrecision::FP64 ||
-        options.relative_kkt_tolerance > 0.0)
+    const bool halpern = options.pdhg_policy == PdhgPolicy::Halpern;
+    if (options.pdhg_policy == PdhgPolicy::Adaptive || options.pdhg_precision != PdhgPrecision::FP64 ||
+        (!halpern && options.relative_kkt_tolerance > 0.0))
         return status_result(SolveStatus::UnsupportedModel,
-                             "PDQP runs the reference policy in FP64 (pdqp.cpp); the adaptive, mixed-precision "
-                             "and relative-KKT options are for LPs");
-
-    const Scaled sc = scale(model, bounds);
-    std::vector<Scalar> warm;
+                             "PDQP runs in FP64 the reference policy (pdqp.cpp) or the Halpern policy; the "
+                             "adaptive policy and mixed precision are for LPs, and the relative-KKT test needs "
+                             "the Halpern policy");
     if (!warm_start.empty()) {   // pdqp.cpp:29-34
         if (static_cast<Index>(warm_start.size()) != model.num_variables())
             return status_result(SolveStatus::InvalidModel, "warm start must be empty or finite and sized to the model");
@@ -400,36 +427,57 @@ SolveResult solve_qp(const Model &model, ExecutionBackend &backend, const Solver
             if (!std::isfinite(v))
                 return status_result(SolveStatus::InvalidModel,
                                      "warm start must be empty or finite and sized to the model");
-        warm.assign(warm_start.begin(), warm_start.end());
-        for (std::size_t j = 0; j < warm.size(); ++j) warm[j] /= sc.s[j];   // pdqp.cpp:77-82
     }
-    const std::vector<Scalar> *warm_ptr = warm_start.empty() ? nullptr : &warm;
-    const ScaledQ sq = scale_q(model.quadratic_objective, sc.s);
-    const Scalar bound = operator_bound(model.constraint_matrix.pattern, sc.values);
-    const Scalar max_step = 0.99 / std::max<Scalar>(1.0, bound * bound + sq.norm_inf);   // pdqp.cpp:98
 
     auto matrix = backend.create_matrix(model.constraint_matrix.pattern);
     auto quadratic = backend.create_matrix(model.quadratic_objective.pattern);
-    backend.set_values(sc.values, *matrix);
+    vk_hip_csr *a_handle = options.fused_iterations ? hip_matrix_handle(*matrix) : nullptr;
+    const vk_hip_csr *q_handle = options.fused_iterations ? hip_matrix_handle(*quadratic) : nullptr;
+    if (halpern && !(a_handle && q_handle))
+        return status_result(SolveStatus::UnsupportedModel, "Halpern PDQP runs only fused on a HipBackend");
+    // Halpern: the LP path's device scaling with Ruiz equilibration; the
+    // reference policy: pdqp.cpp's host scaling.
+    Scaled sc;
+    if (halpern) {
+        backend.set_values(model.constraint_matrix.values, *matrix);
+        sc = device_scale(a_handle, model, bounds, true);
+    } else {
+        sc = scale(model, bounds);
+        backend.set_values(sc.values, *matrix);
+    }
+    std::vector<Scalar> warm(warm_start.begin(), warm_start.end());
+    for (std::size_t j = 0; j < warm.size(); ++j) warm[j] /= sc.s[j];   // pdqp.cpp:77-82
+    const std::vector<Scalar> *warm_ptr = warm_start.empty() ? nullptr : &warm;
+    const ScaledQ sq = scale_q(model.quadratic_objective, sc.s);
     backend.set_values(sq.values, *quadratic);
-    const vk_hip_csr *a_handle = hip_matrix_handle(*matrix), *q_handle = hip_matrix_handle(*quadratic);
+    Scalar max_step = 0.0;
+    if (!halpern) {
+        const Scalar bound = operator_bound(model.constraint_matrix.pattern, sc.values);
+        max_step = 0.99 / std::max<Scalar>(1.0, bound * bound + sq.norm_inf);   // pdqp.cpp:98
+    }
     SolveResult res;
     res.solver = "PDQP";
     res.backend = "provided";
     std::vector<Scalar> xs, ys;
     std::string tiles_note;
-    if (a_handle && q_handle && options.fused_iterations) {
+    int status = VK_PDHG_RUNNING;
+    if (a_handle && q_handle) {
         // The matrix belongs to this solve: its tiles may be re-cut.
-        const int g = apply_tile_geometry(const_cast<vk_hip_csr *>(a_handle), options);
+        const int g = apply_tile_geometry(a_handle, options);
         tiles_note = tile_geometry_note(g, options.spmv_tile_geometry < 0);
-        iterate_fused(model, sc, a_handle, q_handle, max_step, options, warm_ptr, started, xs, ys, res);
+        status = iterate_fused(model, sc, a_handle, q_handle, max_step, halpern, options, warm_ptr, started, xs, ys,
+                               res);
     } else {
         iterate_ops(model, sc, backend, *matrix, *quadratic, max_step, options, warm_ptr, started, xs, ys, res);
     }
     std::vector<Scalar> x(xs.size()), y(ys.size());
     for (std::size_t j = 0; j < xs.size(); ++j) x[j] = xs[j] * sc.s[j];
     for (std::size_t i = 0; i < ys.size(); ++i) y[i] = ys[i] * sc.r[i];
-    postsolve_qp(model, bounds, options, x, y, res);
+    const bool relative_mode = halpern && options.relative_kkt_tolerance > 0.0;
+    postsolve_qp(model, bounds, options, x, y, relative_mode, res);
+    // A Farkas ray does not involve the objective: the LP check applies.
+    if (status == VK_PDHG_PRIMAL_INFEASIBLE && res.status != SolveStatus::TimeLimit)
+        report_infeasible(model, bounds, y, res);
     if (!tiles_note.empty()) res.message = res.message.empty() ? tiles_note : res.message + "; " + tiles_note;
     res.solve_seconds = std::chrono::duration<Scalar>(Clock::now() - started).count();
     return res;
