@@ -30,9 +30,10 @@ including setup and postsolve ("solve_wall_seconds"), the batched solve of
 512 block-angular LPs ("batch"), 256 branch-and-bound node relaxations of
 one LP on one device matrix against per-node solve_lp calls ("nodes"), the
 unrolled-PDHG adjoint on a 100,000 x 250,000 LP with its central-difference
-check ("adjoint"), a fixed count of PDQP iterations on the benchmark LP plus
-a sparse convex Q against the same count without Q ("qp"), and MPS files
-given with --mps or written by --mps-generated ("mps").
+check ("adjoint"), the benchmark LP plus a sparse convex Q solved by Halpern
+PDQP to the same relative-KKT target, beside the reference policy after as
+many iterations ("qp"), and MPS files given with --mps or written by
+--mps-generated ("mps").
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
 """
@@ -308,15 +309,16 @@ def measure_nodes(V, np, args):
     return out
 
 
-def measure_qp(V, np, lp, args):
+def measure_qp(V, np, lp, args, lp_ms_per_iteration):
     """PDQP (pdqp.cpp:20-331) on the benchmark LP plus a sparse convex Q:
     every column gets one off-diagonal partner with a value in
     (-0.25, 0.25), mirrored, and a diagonal of 0.2 + its row's absolute
     off-diagonal sum + U(0, 1) (strictly diagonally dominant, so positive
-    definite).  PDQP runs the reference policy in FP64, which does not reach
-    the benchmark tolerance, so the measure is a fixed count of iterations,
-    timed beside the same count of the LP without Q (same policy, same
-    layout rules): the cost of Q x per iteration."""
+    definite).  Restarted Halpern PDHG with per-column primal steps solves it
+    in FP64 to the benchmark's relative-KKT target; the reference policy then
+    runs as many iterations, for its relative KKT at that count.  The cost of
+    Q x per iteration is the Halpern solve's ms/iteration beside the FP64 LP
+    solve's."""
     import scipy.sparse as sp
     rng = np.random.default_rng(WORKLOAD["seed"] + 7)
     n = lp.n
@@ -332,18 +334,24 @@ def measure_qp(V, np, lp, args):
     q = (off + sp.diags(diag)).tocsr()
     q.sort_indices()
     quadratic = (q.indptr.astype(np.int64), q.indices.astype(np.int64), q.data.astype(np.float64))
-    o = V.Options(iteration_limit=args.qp_iterations, time_limit_seconds=args.time_limit,
+    o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=WORKLOAD["tolerance"],
+                  iteration_limit=args.qp_iterations, time_limit_seconds=args.time_limit,
                   use_graphs=not args.no_graphs)
     rq = V.solve_qp(lp, quadratic, o)
-    rl = V.solve_lp(lp, o)
-    out = {"n": n, "q_nnz": int(q.nnz), "iterations": rq.iterations, "status": rq.status,
-           "qp_device_seconds": rq.device_seconds, "lp_device_seconds": rl.device_seconds,
-           "qp_ms_per_iteration": rq.device_seconds / max(1, rq.iterations) * 1e3,
-           "lp_ms_per_iteration": rl.device_seconds / max(1, rl.iterations) * 1e3,
-           "qp_wall_seconds": rq.solve_seconds, "policy": "reference", "precision": "fp64"}
-    log(f"qp: {rq.iterations} iterations with Q ({q.nnz} nonzeros): {out['qp_ms_per_iteration']:.3f} ms/iteration "
-        f"vs LP {out['lp_ms_per_iteration']:.3f}; {rq.status}, wall {rq.solve_seconds:.2f}s")
-    return out
+    log(f"qp: Halpern {rq.status} in {rq.iterations} iterations, device {rq.device_seconds:.3f}s, "
+        f"rel KKT {rq.kkt_relative:.3e}, objective {rq.objective_value:.10g}, wall {rq.solve_seconds:.2f}s")
+    rr = V.solve_qp(lp, quadratic, V.Options(iteration_limit=max(1, rq.iterations),
+                                             time_limit_seconds=args.time_limit, use_graphs=not args.no_graphs))
+    log(f"qp: reference policy after {rr.iterations} iterations: {rr.status}, rel KKT {rr.kkt_relative:.3e}")
+    return {"n": n, "q_nnz": int(q.nnz), "policy": "halpern", "precision": "fp64",
+            "tolerance": WORKLOAD["tolerance"], "status": rq.status, "iterations": rq.iterations,
+            "device_seconds": rq.device_seconds, "wall_seconds": rq.solve_seconds,
+            "rel_kkt": rq.kkt_relative, "objective": rq.objective_value,
+            "qp_ms_per_iteration": rq.device_seconds / max(1, rq.iterations) * 1e3,
+            "lp_ms_per_iteration": lp_ms_per_iteration,
+            "reference_policy_same_iterations": {"status": rr.status, "iterations": rr.iterations,
+                                                 "rel_kkt": rr.kkt_relative,
+                                                 "device_seconds": rr.device_seconds}}
 
 
 def write_generated_mps(V, np, path):
@@ -493,8 +501,8 @@ def main():
     ap.add_argument("--adjoint-iterations", type=int, default=1000,
                     help="unrolled iterations of the adjoint measurement (0 = skip)")
     ap.add_argument("--fd-directions", type=int, default=64)
-    ap.add_argument("--qp-iterations", type=int, default=2000,
-                    help="fixed PDQP iteration count on the benchmark LP plus Q (0 skips)")
+    ap.add_argument("--qp-iterations", type=int, default=100_000,
+                    help="iteration limit of the Halpern PDQP solve of the benchmark LP plus Q (0 skips)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adaptive", action="store_true",
                     help="skip the FP64 solve with the adaptive policy (comparison)")
@@ -634,7 +642,8 @@ def main():
         import shutil
         shutil.rmtree(generated_dir, ignore_errors=True)
     nodes = measure_nodes(V, np, args) if args.nodes > 0 and not args.skip_solve else None
-    qp = measure_qp(V, np, lp, args) if args.qp_iterations > 0 and not args.skip_solve else None
+    qp = (measure_qp(V, np, lp, args, t64 * 1e3 / max(1, r64.iterations))
+          if args.qp_iterations > 0 and not args.skip_solve else None)
 
     line = {
         "metric": "time to 1e-6 rel. KKT, mixed-FP32 vs FP64 PDHG, 1 MI355X; SpMV HBM GB/s",
