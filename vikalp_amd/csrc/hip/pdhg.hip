@@ -381,8 +381,10 @@ __global__ __launch_bounds__(kBlock) void k_pstep(Bufs<T> b) {
     if (s.status != VK_PDHG_RUNNING) return;
     const T *xcur = b.x[s.k & 1];
     T *xnext = b.x[(s.k + 1) & 1];
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock)
-        pstep_element(b, s, j, xcur, xnext);
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < b.n; j += gridDim.x * kBlock) {
+        if (b.qabs) pstep_element<T, true>(b, s, j, xcur, xnext);
+        else pstep_element<T, false>(b, s, j, xcur, xnext);
+    }
 }
 
 template <class T>

@@ -790,13 +790,14 @@ __device__ __forceinline__ void halpern_apply_row(const Bufs<T> &b, const DevSta
 }
 
 // One element of the standalone primal step x[k+1] = P_X(x[k] - tau (c + aty))
-// (with Q: x[k] - tau (aty + (c + Q x[k])), pdqp.cpp:115-122).
-template <class T>
+// (with Q: x[k] - tau (aty + (c + Q x[k])), pdqp.cpp:115-122; HalpernQP: the
+// Halpern policy's column step qp_column_step).
+template <class T, bool HalpernQP = false>
 __device__ __forceinline__ void pstep_element(const Bufs<T> &b, const DevState &s, int j, const T *xcur, T *xnext) {
     const double xj = static_cast<double>(xcur[j]);
     const double cj = static_cast<double>(b.c[j]), aj = static_cast<double>(b.aty[j]);
     double z;
-    if (b.qabs) z = qp_step(xj, cj, b.qx[j], aj, qp_column_step(s.tau, b.qabs[j]));   // Halpern with Q
+    if constexpr (HalpernQP) z = qp_step(xj, cj, b.qx[j], aj, qp_column_step(s.tau, b.qabs[j]));
     else z = b.qx ? -s.tau * (aj + (cj + b.qx[j])) + xj : (xj - s.tau * cj) - s.tau * aj;
     const T xn = static_cast<T>(clamp_box(z, static_cast<double>(b.l[j]), static_cast<double>(b.u[j])));
     xnext[j] = xn;
