@@ -302,12 +302,14 @@ int vk_hip_pdhg_create(const vk_hip_csr *A, const double *c, const double *l,
                        const vk_pdhg_params *params, vk_hip_pdhg **out);
 /* PDQP (pdqp.cpp:110-176): as vk_hip_pdhg_create with the SCALED quadratic
  * objective Q (n x n, values q_ij s_i s_j, stored symmetrically).  One more
- * SpMV per iteration forms Q x+ ahead of the primal kernel.  FP64, with
- * either the reference policy (the caller's step 0.99 / max(1, bound^2 +
+ * SpMV per iteration forms Q x+ ahead of the primal kernel.  Either the
+ * reference policy in FP64 (the caller's step 0.99 / max(1, bound^2 +
  * |Q|_inf), pdqp.cpp:98; no relative-KKT test) or the Halpern policy (new:
  * step_size = eta = 0.99 / ||A||_2 as for an LP, sigma = eta w, and column
  * j's primal step 1 / (w / eta + sum_k |q_jk|); the relative KKT has the
- * 1/2 x'Qx objective terms).  Q = NULL is vk_hip_pdhg_create. */
+ * 1/2 x'Qx objective terms) in FP64 or, with recentre = 1, in VK_PREC_MIXED
+ * (corrections to an FP64 base whose Q xbase joins the gradient offset).
+ * Q = NULL is vk_hip_pdhg_create. */
 int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c,
                        const double *l, const double *u, const double *rl, const double *ru,
                        const double *row_scale, const double *col_scale, const double *x0,

@@ -76,14 +76,15 @@ struct StoreEpi {
 // epilogues, the Halpern ones, and the Halpern ones of the recentred FP32
 // solver (which read the dual base and the objective beside the shifted
 // gradient).
-// kEpiHalpernQP: the Halpern ones with a quadratic objective (FP64 only).
-constexpr int kEpiPlain = 0, kEpiHalpern = 1, kEpiRecentred = 2, kEpiHalpernQP = 3;
+// kEpiHalpernQP / kEpiRecentredQP: the Halpern ones with a quadratic
+// objective (FP64 / the recentred FP32 solver).
+constexpr int kEpiPlain = 0, kEpiHalpern = 1, kEpiRecentred = 2, kEpiHalpernQP = 3, kEpiRecentredQP = 4;
 
 // The dual and primal epilogues of one iteration in mode M.
 template <class T, int M, bool Stream>
 struct IterEpis {
-    static constexpr bool R = M == kEpiRecentred;
-    static constexpr bool QP = M == kEpiHalpernQP;
+    static constexpr bool R = M == kEpiRecentred || M == kEpiRecentredQP;
+    static constexpr bool QP = M == kEpiHalpernQP || M == kEpiRecentredQP;
     static __device__ HalpernDualEpi<T, Stream, R> dual(const Bufs<T> &b, const DevState &s) {
         return HalpernDualEpi<T, Stream, R>{b, s.sigma, s.inv_sigma, halpern_weights(s.ks), check_iteration(s)};
     }
@@ -534,6 +535,10 @@ __global__ __launch_bounds__(kBlock) void k_recentre_base(Bufs<T> b, RecentreBuf
             b.xlast[t] = T(0);
             b.aty[t] = T(0);
             b.aty0h[t] = T(0);
+            if (b.qxh) {   // Q x of the cleared Halpern iterate and anchor
+                b.qxh[t] = 0.0;
+                b.qx0h[t] = 0.0;
+            }
         }
         if (t < b.m) {
             r.yb[t] = dual_sign_project(r.yb[t] + static_cast<double>(b.y[t]), r.rl[t], r.ru[t]);
@@ -561,8 +566,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_recentre(const DevState *st, in
 
 // The LP shifted to the new base, rounded to T once, and the next primal
 // step from the cleared Halpern state (k_halpern_apply's step with xh = 0
-// and A'yh = 0); partial slot 5 collects c'xbase.  Launched over grid_cols
-// workgroups (the size of part_ecols).
+// and A'yh = 0); partial slot 5 collects c'xbase.  With a quadratic
+// objective the gradient offset is c + Q xbase + A'ybase, the objective's
+// gradient at the base c + Q xbase goes to cobj, slot 5 collects c'xbase +
+// 1/2 xbase'Q xbase, and the step is the column step.  Launched over
+// grid_cols workgroups (the size of part_ecols).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_recentre_data(Bufs<T> b, RecentreBufs<T> r, int force) {
     if (!recentre_now(b.st, force)) return;
@@ -576,14 +584,25 @@ __global__ __launch_bounds__(kBlock) void k_recentre_data(Bufs<T> b, RecentreBuf
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
         if (t < b.n) {
             const double xb = r.xb[t];
-            const T g = static_cast<T>(r.c[t] + r.aty[t]);
             const T lo = static_cast<T>(r.l[t] - xb), hi = static_cast<T>(r.u[t] - xb);
-            r.cs[t] = g;
             r.ls[t] = lo;
             r.us[t] = hi;
-            xnext[t] = static_cast<T>(clamp_box((0.0 - tau * static_cast<double>(g)) - tau * 0.0,
-                                                static_cast<double>(lo), static_cast<double>(hi)));
-            part.v[5] += r.c[t] * xb;
+            if (r.qxb) {
+                const double cq = r.c[t] + r.qxb[t];
+                const T g = static_cast<T>(cq + r.aty[t]);
+                r.cs[t] = g;
+                r.cobj[t] = static_cast<T>(cq);
+                xnext[t] = static_cast<T>(clamp_box(qp_step(0.0, static_cast<double>(g), 0.0, 0.0,
+                                                            qp_column_step(tau, b.qabs[t])),
+                                                    static_cast<double>(lo), static_cast<double>(hi)));
+                part.v[5] += r.c[t] * xb + 0.5 * xb * r.qxb[t];
+            } else {
+                const T g = static_cast<T>(r.c[t] + r.aty[t]);
+                r.cs[t] = g;
+                xnext[t] = static_cast<T>(clamp_box((0.0 - tau * static_cast<double>(g)) - tau * 0.0,
+                                                    static_cast<double>(lo), static_cast<double>(hi)));
+                part.v[5] += r.c[t] * xb;
+            }
         }
         if (t < b.m) {
             const double ax = r.ax[t];
@@ -862,6 +881,13 @@ template <>
 const CsrView<float> view_at<float>(const vk_hip_pdhg *s) {
     return with_resident(s->A->at32.view(), s->resident_at);
 }
+// The quadratic objective's tiles in the storage precision.
+template <class T>
+const CsrView<T> view_q(const vk_hip_pdhg *s);
+template <>
+const CsrView<double> view_q<double>(const vk_hip_pdhg *s) { return s->Q->a.view(); }
+template <>
+const CsrView<float> view_q<float>(const vk_hip_pdhg *s) { return s->Q->a32.view(); }
 
 template <class T>
 BlockedView<T> &blocked(vk_hip_pdhg *s);
@@ -903,11 +929,10 @@ void launch_dual_mode(vk_hip_pdhg *s, hipStream_t st, int part) {
 template <class T, int M>
 void launch_primal_mode(vk_hip_pdhg *s, hipStream_t st, int part) {
     Bufs<T> &b = bufs<T>(s);
-    if constexpr (sizeof(T) == sizeof(double)) {
-        if (s->Q && (part & 1)) {
-            const CsrView<double> Q = s->Q->a.view();
-            k_qx<T><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(b, Q);
-        }
+    if (s->Q && (part & 1)) {
+        // Q in the storage precision (FP32 values in the recentred solver), FP64 sums.
+        const CsrView<T> Q = view_q<T>(s);
+        k_qx<T><<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(b, Q);
     }
     if (s->panels) {
         if (part & 1) (void)panel_spmv_launch(s->pan_at, b.st, 0, b.y, b.y, s->tpart, st);
@@ -919,13 +944,14 @@ void launch_primal_mode(vk_hip_pdhg *s, hipStream_t st, int part) {
     }
 }
 
-// The epilogue mode of the solver's iteration kernels: the recentred form
-// only exists for FP32 storage, the quadratic one for FP64.
+// The epilogue mode of the solver's iteration kernels: the recentred forms
+// only exist for FP32 storage, the plain quadratic one for FP64.
 template <class T>
 int epi_mode(const vk_hip_pdhg *s) {
     if (s->params.policy != VK_PDHG_POLICY_HALPERN) return kEpiPlain;
-    if (sizeof(T) == sizeof(double) && s->Q) return kEpiHalpernQP;
-    return (sizeof(T) == sizeof(float) && s->recentre) ? kEpiRecentred : kEpiHalpern;
+    if (sizeof(T) == sizeof(double)) return s->Q ? kEpiHalpernQP : kEpiHalpern;
+    if (!s->recentre) return kEpiHalpern;
+    return s->Q ? kEpiRecentredQP : kEpiRecentred;
 }
 
 template <class T>
@@ -936,9 +962,11 @@ void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
     } else if (mode == kEpiHalpern) {
         launch_dual_mode<T, kEpiHalpern>(s, st, part);
     } else if constexpr (sizeof(T) == sizeof(float)) {
+        // The dual epilogue has no quadratic term: kEpiRecentredQP runs the
+        // kernels of kEpiRecentred.
         launch_dual_mode<T, kEpiRecentred>(s, st, part);
     } else {
-        // The dual epilogue has no quadratic term: the kernels of kEpiHalpern.
+        // The same for kEpiHalpernQP and kEpiHalpern.
         launch_dual_mode<T, kEpiHalpern>(s, st, part);
     }
 }
@@ -946,10 +974,16 @@ void launch_dual(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
 template <class T>
 void launch_primal(vk_hip_pdhg *s, hipStream_t st, int part = 3) {
     const int mode = epi_mode<T>(s);
-    if (mode == kEpiPlain) launch_primal_mode<T, kEpiPlain>(s, st, part);
-    else if (mode == kEpiHalpern) launch_primal_mode<T, kEpiHalpern>(s, st, part);
-    else if constexpr (sizeof(T) == sizeof(float)) launch_primal_mode<T, kEpiRecentred>(s, st, part);
-    else launch_primal_mode<T, kEpiHalpernQP>(s, st, part);
+    if (mode == kEpiPlain) {
+        launch_primal_mode<T, kEpiPlain>(s, st, part);
+    } else if (mode == kEpiHalpern) {
+        launch_primal_mode<T, kEpiHalpern>(s, st, part);
+    } else if constexpr (sizeof(T) == sizeof(float)) {
+        if (mode == kEpiRecentredQP) launch_primal_mode<T, kEpiRecentredQP>(s, st, part);
+        else launch_primal_mode<T, kEpiRecentred>(s, st, part);
+    } else {
+        launch_primal_mode<T, kEpiHalpernQP>(s, st, part);
+    }
 }
 
 // A xbar of the running average and its row terms.
@@ -974,6 +1008,11 @@ void launch_recentre(vk_hip_pdhg *s, hipStream_t st, int force) {
     if (s->m > 0) {
         k_spmv_recentre<<<s->grid_a64, kBlock, 0, st>>>(b.st, force, s->A->a.view(), s->rec.xb, s->rec.ax);
         k_spmv_recentre<<<s->grid_at64, kBlock, 0, st>>>(b.st, force, s->A->at.view(), s->rec.yb, s->rec.aty);
+    }
+    if (s->Q) {
+        const CsrView<double> Q = s->Q->a.view();
+        k_spmv_recentre<<<std::max(1, std::min((Q.ntiles + kWaves - 1) / kWaves, kMaxGrid)), kBlock, 0, st>>>(
+            b.st, force, Q, s->rec.xb, s->rec.qxb);
     }
     k_recentre_data<float><<<s->grid_cols, kBlock, 0, st>>>(b, s->rec, force);
     k_recentre_finish<float><<<1, kFinalBlock, 0, st>>>(b, force);
@@ -1351,6 +1390,25 @@ int setup_solver(vk_hip_pdhg *s, const double *c, const double *l, const double 
     VK_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) return rc;
     if constexpr (sizeof(T) == sizeof(float)) {
+        if (s->recentre && s->Q) {
+            // Recentred Halpern PDQP: Q x of the zero corrections (x+, the
+            // Halpern iterate, the anchor) is zero; Q's absolute row sums
+            // set the column steps; Q xbase is the recentring's.
+            b.qx = s->alloc<double>(n);
+            b.qxh = s->alloc<double>(n);
+            b.qx0h = s->alloc<double>(n);
+            b.qx_avg = s->alloc<double>(n);
+            double *qabs = s->alloc<double>(n);
+            rec.qxb = s->alloc<double>(n);
+            rec.cobj = const_cast<T *>(b.cobj);
+            if (!b.qx || !b.qxh || !b.qx0h || !b.qx_avg || !qabs || !rec.qxb) {
+                set_error("vk_hip_pdqp_create: device allocation failed");
+                return VK_ENOMEM;
+            }
+            for (double *v : {b.qx, b.qxh, b.qx0h, b.qx_avg}) k_zero<double><<<egrid(n), kBlock, 0, st>>>(n, v);
+            k_row_abs_sums<<<egrid(n), kBlock, 0, st>>>(s->Q->a.view(), qabs);
+            b.qabs = qabs;
+        }
         if (s->recentre) {
             // The LP shifted to the start (c, l, u, rl, ru above become the
             // shifted ones) and c'xbase; the start projection, A x0, A' y0
@@ -1785,11 +1843,12 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     if (Q) {
         VK_REQUIRE(Q->has_values, "vk_hip_pdqp_create: Q values were never set");
         VK_REQUIRE(Q->rows == A->cols && Q->cols == A->cols, "vk_hip_pdqp_create: Q must be n x n");
-        VK_REQUIRE(params->precision == VK_PREC_FP64 &&
-                       ((params->policy == VK_PDHG_POLICY_REFERENCE && params->relative_kkt_tolerance == 0.0) ||
-                        params->policy == VK_PDHG_POLICY_HALPERN),
-                   "vk_hip_pdqp_create: a quadratic objective runs in FP64 either the reference policy "
-                   "(pdqp.cpp) without the relative-KKT test or the Halpern policy");
+        VK_REQUIRE((params->precision == VK_PREC_FP64 && params->policy == VK_PDHG_POLICY_REFERENCE &&
+                    params->relative_kkt_tolerance == 0.0) ||
+                       (params->policy == VK_PDHG_POLICY_HALPERN &&
+                        (params->precision == VK_PREC_FP64 || params->recentre == 1)),
+                   "vk_hip_pdqp_create: a quadratic objective runs the reference policy (pdqp.cpp) in FP64 "
+                   "without the relative-KKT test, or the Halpern policy in FP64 or recentred FP32");
     }
     VK_REQUIRE(A->has_values, "vk_hip_pdhg_create: matrix values were never set");
     VK_REQUIRE(A->cols > 0, "vk_hip_pdhg_create: the LP needs at least one variable");
@@ -1825,9 +1884,9 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
     VK_REQUIRE(params->primal_weight_gain >= 0.0 && params->primal_weight_gain <= 1.0 &&
                    params->primal_weight_integral >= 0.0 && params->primal_weight_integral <= 1.0,
                "vk_hip_pdhg_create: primal-weight gains must be in [0, 1]");
-    VK_REQUIRE(params->recentre == 0 || (params->policy == VK_PDHG_POLICY_HALPERN &&
-                                         params->precision == VK_PREC_MIXED && !Q),
-               "vk_hip_pdhg_create: recentre needs the Halpern policy in VK_PREC_MIXED without a quadratic objective");
+    VK_REQUIRE(params->recentre == 0 ||
+                   (params->policy == VK_PDHG_POLICY_HALPERN && params->precision == VK_PREC_MIXED),
+               "vk_hip_pdhg_create: recentre needs the Halpern policy in VK_PREC_MIXED");
     if (int rc = ensure_init()) return rc;
     auto *s = new (std::nothrow) vk_hip_pdhg();
     if (!s) return VK_ENOMEM;
@@ -1911,6 +1970,7 @@ int vk_hip_pdqp_create(const vk_hip_csr *A, const vk_hip_csr *Q, const double *c
         return fail(VK_EDEVICE);
     }
     int rc = s->mixed ? csr_make_f32(const_cast<vk_hip_csr *>(A), s->stream) : VK_OK;
+    if (!rc && s->mixed && Q) rc = csr_make_f32(const_cast<vk_hip_csr *>(Q), s->stream);
     if (rc) return fail(rc);
     if (params->spmv_panels == 2 || (params->spmv_panels == 0 && auto_panels(s->m, s->n))) {
         rc = build_panels(s);

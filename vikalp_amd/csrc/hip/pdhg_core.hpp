@@ -115,6 +115,10 @@ struct RecentreBufs {
     const double *rl, *ru;           // (m)
     double *ax, *aty;                // A xb, A' yb (FP64 sums)
     T *cs, *ls, *us, *rls, *rus;     // the shifted LP (aliases of Bufs c, l, u, rl, ru)
+    // A quadratic objective (null for an LP): Q xb (FP64 sums) and the
+    // objective's gradient at the base, c + Q xb (alias of Bufs cobj).
+    double *qxb;
+    T *cobj;
 };
 
 // The dual sign rule of the Moreau step (y+ = 0 inside [rl, ru], y+ <= 0 at

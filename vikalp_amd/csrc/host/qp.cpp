@@ -7,10 +7,11 @@
 // iteration (vk_hip_pdqp_create: one more SpMV per iteration, Q x+, ahead of
 // the primal kernel) and the reference's op-by-op sequence on any
 // ExecutionBackend.  Beyond the reference, SolverOptions::pdhg_policy =
-// Halpern runs restarted Halpern PDHG on the QP (fused, FP64): the LP
-// path's equilibrated device scaling, per-column primal steps 1 / (w / eta +
-// sum_k |q_jk|) that keep the PDHG metric positive definite, and optionally
-// the relative-KKT test with the 1/2 x'Qx terms.
+// Halpern runs restarted Halpern PDHG on the QP (fused): the LP path's
+// equilibrated device scaling, per-column primal steps 1 / (w / eta +
+// sum_k |q_jk|) that keep the PDHG metric positive definite, optionally the
+// relative-KKT test with the 1/2 x'Qx terms, in FP64 or mixed (the recentred
+// FP32 phase with the base's Q x, then a certifying FP64 phase).
 
 #include "lp_internal.hpp"
 
@@ -318,15 +319,26 @@ void iterate_ops(const Model &model, const Scaled &sc, ExecutionBackend &be, Bac
     be.download(*y, ys);
 }
 
-struct DeviceSolverQ {
-    vk_hip_pdhg *h = nullptr;
-    ~DeviceSolverQ() { vk_hip_pdhg_destroy(h); }
-};
+// One device PDQP solve with `p` from the scaled start (x0, y0; null = 0):
+// the VK_PDHG_* status, the scaled iterates, the final state and the device
+// time (run_device's loop).
+int run_qp(const vk_hip_csr *A, const vk_hip_csr *Q, const Scaled &sc, const vk_pdhg_params &p,
+           const std::vector<Scalar> *x0, const std::vector<Scalar> *y0, Clock::time_point started,
+           const SolverOptions &options, std::vector<Scalar> &xs, std::vector<Scalar> &ys, vk_pdhg_state &state,
+           double &device_ms, bool &timed) {
+    DeviceSolver solver;
+    check_vk(vk_hip_pdqp_create(A, Q, sc.c.data(), sc.l.data(), sc.u.data(), sc.rl.data(), sc.ru.data(), sc.r.data(),
+                                sc.s.data(), x0 ? x0->data() : nullptr, y0 ? y0->data() : nullptr, &p, &solver.h),
+             "vk_hip_pdqp_create");
+    return drive_device(solver, p, sc.c.size(), sc.r.size(), started, options, xs, ys, state, device_ms, timed);
+}
 
-// The fused device iteration with Q in FP64: the reference policy with its
+// The fused device iteration with Q: the reference policy in FP64 with its
 // step max_step, or (halpern) restarted Halpern PDHG with eta = 0.99 /
 // ||A||_2 (the device sets each column's primal step from Q's absolute row
-// sums).  Returns the VK_PDHG_* status.
+// sums), in FP64 or mixed: a recentred FP32 phase, then an FP64 phase
+// warm-started from its iterates that certifies the point (solve_lp's mixed
+// driver).  Returns the VK_PDHG_* status.
 int iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, const vk_hip_csr *Q,
                   Scalar max_step, bool halpern, const SolverOptions &options,
                   const std::vector<Scalar> *warm_scaled, Clock::time_point started, std::vector<Scalar> &xs,
@@ -362,34 +374,38 @@ int iterate_fused(const Model &model, const Scaled &sc, const vk_hip_csr *A, con
     p.panel_tune = options.spmv_autotune ? 1 : 0;
     p.norm_b = row_bound_norm(model.constraint_lower, model.constraint_upper);
     p.norm_c = norm2(model.linear_objective);
-    DeviceSolverQ solver;
-    check_vk(vk_hip_pdqp_create(A, Q, sc.c.data(), sc.l.data(), sc.u.data(), sc.rl.data(), sc.ru.data(), sc.r.data(),
-                                sc.s.data(), warm_scaled ? warm_scaled->data() : nullptr, nullptr, &p, &solver.h),
-             "vk_hip_pdqp_create");
-    int status = VK_PDHG_RUNNING;
-    const Index chunk = std::isfinite(options.time_limit_seconds) ? 512 : std::max<Index>(1, p.iteration_limit);
-    double device_ms = 0.0;
     vk_pdhg_state state{};
-    check_vk(vk_hip_pdhg_read_state(solver.h, &state), "vk_hip_pdhg_read_state");
-    status = state.status;
-    while (status == VK_PDHG_RUNNING) {
-        if (timed_out(started, options)) {
-            res.status = SolveStatus::TimeLimit;
-            break;
+    double ms = 0.0;
+    bool timed = false;
+    int status;
+    if (halpern && options.pdhg_precision == PdhgPrecision::Mixed) {
+        vk_pdhg_params p32 = p;
+        p32.precision = VK_PREC_MIXED;
+        p32.recentre = 1;
+        p32.relative_kkt_tolerance = fp32_phase_tolerance(options, true);
+        status = run_qp(A, Q, sc, p32, warm_scaled, nullptr, started, options, xs, ys, state, ms, timed);
+        res.low_precision_iterations = state.iterations;
+        Index used = state.iterations;
+        double total_ms = ms;
+        if (!timed && used < options.iteration_limit && status != VK_PDHG_PRIMAL_INFEASIBLE) {
+            const auto x32 = xs, y32 = ys;
+            vk_pdhg_params p64 = p;
+            p64.iteration_limit = options.iteration_limit - used;
+            p64.primal_weight = state.primal_weight;
+            status = run_qp(A, Q, sc, p64, &x32, &y32, started, options, xs, ys, state, ms, timed);
+            total_ms += ms;
+            used += state.iterations;
         }
-        check_vk(vk_hip_pdhg_run(solver.h, chunk, &status), "vk_hip_pdhg_run");
-        double ms = 0.0;
-        check_vk(vk_hip_pdhg_last_run_ms(solver.h, &ms), "vk_hip_pdhg_last_run_ms");
-        device_ms += ms;
+        state.iterations = used;
+        ms = total_ms;
+    } else {
+        status = run_qp(A, Q, sc, p, warm_scaled, nullptr, started, options, xs, ys, state, ms, timed);
     }
-    check_vk(vk_hip_pdhg_read_state(solver.h, &state), "vk_hip_pdhg_read_state");
-    if (state.status == VK_PDHG_CONVERGED) res.status = SolveStatus::Optimal;
     res.iterations = state.iterations;
-    res.device_seconds = device_ms * 1e-3;
-    xs.resize(static_cast<std::size_t>(model.num_variables()));
-    ys.resize(static_cast<std::size_t>(model.num_constraints()));
-    check_vk(vk_hip_pdhg_download(solver.h, xs.data(), ys.data()), "vk_hip_pdhg_download");
-    return state.status;
+    res.device_seconds = ms * 1e-3;
+    if (timed) res.status = SolveStatus::TimeLimit;
+    else if (status == VK_PDHG_CONVERGED) res.status = SolveStatus::Optimal;
+    return status;
 }
 
 }  // namespace
@@ -414,12 +430,13 @@ SolveResult solve_qp(const Model &model, ExecutionBackend &backend, const Solver
     if (options.continuous_method != ContinuousMethod::Auto && options.continuous_method != ContinuousMethod::PDHG)
         return status_result(SolveStatus::UnsupportedModel, "requested continuous method is not implemented for QP");
     const bool halpern = options.pdhg_policy == PdhgPolicy::Halpern;
-    if (options.pdhg_policy == PdhgPolicy::Adaptive || options.pdhg_precision != PdhgPrecision::FP64 ||
-        (!halpern && options.relative_kkt_tolerance > 0.0))
+    const bool mixed = options.pdhg_precision == PdhgPrecision::Mixed;
+    if (options.pdhg_policy == PdhgPolicy::Adaptive || (!halpern && (mixed || options.relative_kkt_tolerance > 0.0)) ||
+        (mixed && !options.mixed_recentre))
         return status_result(SolveStatus::UnsupportedModel,
-                             "PDQP runs in FP64 the reference policy (pdqp.cpp) or the Halpern policy; the "
-                             "adaptive policy and mixed precision are for LPs, and the relative-KKT test needs "
-                             "the Halpern policy");
+                             "PDQP runs the reference policy (pdqp.cpp) in FP64, or the Halpern policy in FP64 or "
+                             "mixed precision with the recentred FP32 phase; the adaptive policy is for LPs, and "
+                             "the relative-KKT test needs the Halpern policy");
     if (!warm_start.empty()) {   // pdqp.cpp:29-34
         if (static_cast<Index>(warm_start.size()) != model.num_variables())
             return status_result(SolveStatus::InvalidModel, "warm start must be empty or finite and sized to the model");
