@@ -315,10 +315,11 @@ def measure_qp(V, np, lp, args, lp_ms_per_iteration):
     (-0.25, 0.25), mirrored, and a diagonal of 0.2 + its row's absolute
     off-diagonal sum + U(0, 1) (strictly diagonally dominant, so positive
     definite).  Restarted Halpern PDHG with per-column primal steps solves it
-    in FP64 to the benchmark's relative-KKT target; the reference policy then
-    runs as many iterations, for its relative KKT at that count.  The cost of
-    Q x per iteration is the Halpern solve's ms/iteration beside the FP64 LP
-    solve's."""
+    to the benchmark's relative-KKT target in FP64 and in mixed precision
+    (the recentred FP32 phase, then the certifying FP64 phase); the reference
+    policy then runs as many iterations as the FP64 solve, for its relative
+    KKT at that count.  The cost of Q x per iteration is the FP64 solve's
+    ms/iteration beside the FP64 LP solve's."""
     import scipy.sparse as sp
     rng = np.random.default_rng(WORKLOAD["seed"] + 7)
     n = lp.n
@@ -340,6 +341,11 @@ def measure_qp(V, np, lp, args, lp_ms_per_iteration):
     rq = V.solve_qp(lp, quadratic, o)
     log(f"qp: Halpern {rq.status} in {rq.iterations} iterations, device {rq.device_seconds:.3f}s, "
         f"rel KKT {rq.kkt_relative:.3e}, objective {rq.objective_value:.10g}, wall {rq.solve_seconds:.2f}s")
+    o.pdhg_precision = V.PdhgPrecision.Mixed
+    rm = V.solve_qp(lp, quadratic, o)
+    log(f"qp: Halpern mixed {rm.status} in {rm.low_precision_iterations} FP32 + "
+        f"{rm.iterations - rm.low_precision_iterations} FP64 iterations, device {rm.device_seconds:.3f}s, "
+        f"rel KKT {rm.kkt_relative:.3e}, objective {rm.objective_value:.10g}")
     rr = V.solve_qp(lp, quadratic, V.Options(iteration_limit=max(1, rq.iterations),
                                              time_limit_seconds=args.time_limit, use_graphs=not args.no_graphs))
     log(f"qp: reference policy after {rr.iterations} iterations: {rr.status}, rel KKT {rr.kkt_relative:.3e}")
@@ -349,6 +355,13 @@ def measure_qp(V, np, lp, args, lp_ms_per_iteration):
             "rel_kkt": rq.kkt_relative, "objective": rq.objective_value,
             "qp_ms_per_iteration": rq.device_seconds / max(1, rq.iterations) * 1e3,
             "lp_ms_per_iteration": lp_ms_per_iteration,
+            "mixed": {"status": rm.status, "iterations": rm.iterations,
+                      "iterations_fp32_phase": rm.low_precision_iterations,
+                      "device_seconds": rm.device_seconds, "wall_seconds": rm.solve_seconds,
+                      "rel_kkt": rm.kkt_relative,
+                      "objective_rel_diff_vs_fp64": abs(rm.objective_value - rq.objective_value) /
+                      max(1.0, abs(rq.objective_value)),
+                      "speedup_vs_fp64": rq.device_seconds / rm.device_seconds if rm.device_seconds > 0 else None},
             "reference_policy_same_iterations": {"status": rr.status, "iterations": rr.iterations,
                                                  "rel_kkt": rr.kkt_relative,
                                                  "device_seconds": rr.device_seconds}}

@@ -255,8 +255,8 @@ typedef struct {
                                       instances' tiles, 0..3 as in
                                       vk_hip_csr_set_tile_geometry (the
                                       single-LP solver uses its matrix's)    */
-    int recentre;                  /* 1 (Halpern, VK_PREC_MIXED, single LP
-                                      only): store the FP32 iterates as
+    int recentre;                  /* 1 (Halpern, VK_PREC_MIXED, single LP or
+                                      QP only): store the FP32 iterates as
                                       corrections to an FP64 base point that
                                       every restart moves to the restart
                                       point, with the LP shifted to the base
