@@ -369,3 +369,22 @@ def test_halpern_qp_refusals():
                 dict(pdhg_policy=H, fused_iterations=False),
                 dict(pdhg_policy=V.PdhgPolicy.Adaptive, relative_kkt_tolerance=1e-6)):
         assert V.solve_qp(lp, q, V.Options(**bad)).status == "UnsupportedModel", bad
+
+
+@pytest.mark.parametrize("precision", [V.PdhgPrecision.FP64, V.PdhgPrecision.Mixed])
+@pytest.mark.parametrize("make", ["small", "branched"])
+def test_halpern_qp_detects_primal_infeasibility(make, precision):
+    # The feasible set does not involve Q, so an infeasible LP stays
+    # infeasible with any convex Q, and the device's Farkas test (which reads
+    # A'y only) applies unchanged: the returned row duals must be a Farkas ray
+    # of the original data.
+    from test_gpu_infeasible import branched_infeasible, farkas_d0, small_infeasible
+    lp = small_infeasible() if make == "small" else branched_infeasible()
+    q = make_q(lp.n, 21)
+    print(f"halpern qp infeasible {make} precision={precision.name}", flush=True)
+    r = V.solve_qp(lp, q, V.Options(pdhg_policy=V.PdhgPolicy.Halpern, pdhg_precision=precision,
+                                    relative_kkt_tolerance=1e-6, iteration_limit=200_000))
+    assert r.status == "Infeasible", (r.status, r.message, r.iterations)
+    assert r.iterations < 200_000
+    d0 = farkas_d0(lp, r.y)
+    assert np.isfinite(d0) and d0 > 0.0, d0
