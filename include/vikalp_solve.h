@@ -215,6 +215,25 @@ int vk_model_export(const vk_model *model, int64_t *row_offsets, int64_t *column
                     double *values, double *objective, double *row_lower, double *row_upper,
                     double *var_lower, double *var_upper, int32_t *variable_types,
                     double *objective_offset);
+/* Branch-and-bound over batches of node relaxations (new; solver::
+ * solve_mip_batched): the reference's engine (src/solver/mip/
+ * BranchAndBound.cpp:216-465, one RelaxationOracle::solve per node) with up
+ * to `batch` open nodes per round solved together by vk_lp_solve_nodes' path
+ * on one device matrix.  variable_types: 0 continuous, 1 integer, 2 binary.
+ * x_out (n; NULL to skip) gets the incumbent, info the result (iterations =
+ * nodes processed), stats (NULL to skip) 9 counters: nodes processed,
+ * pruned, integral, infeasible, incumbent updates, rounds, relaxations,
+ * relaxation PDHG iterations, largest round.  Returns 0, or -1 for unusable
+ * arguments. */
+int vk_mip_solve_batched(int64_t m, int64_t n, const int64_t *row_offsets,
+                         const int64_t *column_indices, const double *values,
+                         const double *objective, double objective_offset,
+                         const double *row_lower, const double *row_upper,
+                         const double *var_lower, const double *var_upper,
+                         const int32_t *variable_types, int64_t batch,
+                         const vk_solver_options *options, double *x_out,
+                         vk_solve_info *info, int64_t *stats);
+
 /* The model's NAME into buf (NUL-terminated, truncated to size); returns
  * the name's full length. */
 int64_t vk_model_name(const vk_model *model, char *buf, int64_t size);

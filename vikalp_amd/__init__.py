@@ -13,12 +13,12 @@ from .native import (  # noqa: F401
     BackendPreference, ContinuousMethod, PdhgPolicy, PdhgPrecision, SolveStatus,
     STATUS_NAMES, LP, Options, Result, Device, hip_lib, host_lib, generate_lp,
     solve, solve_lp, solve_qp, load_libraries, profile_kernels, solve_batch, UnrolledPdhg, read_mps,
-    solve_nodes,
+    solve_nodes, solve_mip, MIP_STATS,
 )
 
 __all__ = [
     "BackendPreference", "ContinuousMethod", "PdhgPolicy", "PdhgPrecision", "SolveStatus",
     "STATUS_NAMES", "LP", "Options", "Result", "Device", "hip_lib", "host_lib",
     "generate_lp", "solve", "solve_lp", "load_libraries", "profile_kernels", "solve_batch",
-    "UnrolledPdhg", "solve_qp", "read_mps", "solve_nodes",
+    "UnrolledPdhg", "solve_qp", "read_mps", "solve_nodes", "solve_mip", "MIP_STATS",
 ]

@@ -284,6 +284,41 @@ int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets, const in
     return 0;
 }
 
+int vk_mip_solve_batched(int64_t m, int64_t n, const int64_t *row_offsets, const int64_t *column_indices,
+                         const double *values, const double *objective, double objective_offset,
+                         const double *row_lower, const double *row_upper, const double *var_lower,
+                         const double *var_upper, const int32_t *variable_types, int64_t batch,
+                         const vk_solver_options *options, double *x_out, vk_solve_info *info, int64_t *stats) {
+    if (!info || !options || !variable_types) return -1;
+    Model model;
+    if (!build_model(m, n, row_offsets, column_indices, values, objective, objective_offset, row_lower,
+                     row_upper, var_lower, var_upper, model))
+        return -1;
+    for (int64_t j = 0; j < n; ++j) {
+        if (variable_types[j] < 0 || variable_types[j] > 2) return -1;
+        model.variable_types[static_cast<std::size_t>(j)] = static_cast<VariableType>(variable_types[j]);
+    }
+    solver::MipStats st;
+    SolveResult r;
+    try {
+        r = solver::solve_mip_batched(model, to_options(*options), batch, &st);
+    } catch (const std::exception &e) {
+        r = SolveResult{};
+        r.status = SolveStatus::NumericalFailure;
+        r.message = e.what();
+    }
+    fill_info(r, *info);
+    if (x_out && r.primal_solution.size() == static_cast<std::size_t>(n))
+        std::memcpy(x_out, r.primal_solution.data(), sizeof(double) * static_cast<std::size_t>(n));
+    if (stats) {
+        const int64_t v[9] = {st.nodes_processed, st.nodes_pruned, st.nodes_integral, st.nodes_infeasible,
+                              st.incumbent_updates, st.rounds, st.relaxations, st.relaxation_iterations,
+                              st.max_batch};
+        std::memcpy(stats, v, sizeof(v));
+    }
+    return 0;
+}
+
 int vk_lp_solve(int mode, int64_t m, int64_t n, const int64_t *row_offsets,
                 const int64_t *column_indices, const double *values, const double *objective,
                 double objective_offset, const double *row_lower, const double *row_upper,

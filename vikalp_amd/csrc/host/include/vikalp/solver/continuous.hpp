@@ -6,6 +6,7 @@
 #include "vikalp/contracts/SolveResult.hpp"
 #include "vikalp/contracts/SolverOptions.hpp"
 
+#include <cstdint>
 #include <memory>
 #include <span>
 #include <vector>
@@ -25,7 +26,8 @@ SolveResult solve_lp(const Model &model, ExecutionBackend &backend,
 // Q x + c + A'y and the step 0.99 / max(1, bound^2 + |Q|_inf).  On a
 // HipBackend with options.fused_iterations the iteration runs on the device;
 // otherwise the reference's op-by-op sequence is issued.  The reference
-// policy in FP64 only.
+// policy runs in FP64; the Halpern policy (fused only) in FP64 or mixed
+// precision, with per-column primal steps and the relative-KKT test.
 SolveResult solve_qp(const Model &model, ExecutionBackend &backend,
                      const SolverOptions &options = {},
                      std::span<const Scalar> warm_start = {},
@@ -52,6 +54,25 @@ std::vector<SolveResult> solve_lp_nodes(const Model &model, const std::vector<st
                                         const std::vector<std::vector<Scalar>> &upper,
                                         const std::vector<std::vector<Scalar>> &warm_start = {},
                                         const SolverOptions &options = {});
+
+// Counters of a batched branch-and-bound search.
+struct MipStats {
+    std::int64_t nodes_processed = 0, nodes_pruned = 0, nodes_integral = 0, nodes_infeasible = 0;
+    std::int64_t incumbent_updates = 0;
+    std::int64_t rounds = 0;                  // solve_lp_nodes calls
+    std::int64_t relaxations = 0;             // node relaxations solved
+    std::int64_t relaxation_iterations = 0;   // PDHG iterations over all of them
+    std::int64_t max_batch = 0;               // most relaxations in one round
+};
+
+// Branch-and-bound (BranchAndBound.cpp:216-465's rules: best-bound-first
+// queue, pruning, integral and rounded incumbents, most-fractional then
+// pseudocost branching, gap termination) whose node relaxations are solved
+// `batch` at a time by solve_lp_nodes on one device matrix, instead of one
+// RelaxationOracle::solve per node.  batch = 1 gives the reference's node
+// sequence.  Linear objectives only; options go to every relaxation.
+SolveResult solve_mip_batched(const Model &model, const SolverOptions &options = {}, Index batch = 64,
+                              MipStats *stats = nullptr);
 
 // Mean device time (ms, HIP events on the solver's stream) of the fused dual
 // and primal kernels and of a whole iteration, after `warm_iterations`

@@ -203,6 +203,9 @@ HOST_SYMBOLS = {
     "vk_lp_solve_nodes": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
                                     C.c_int, _dp, _dp, _dp, C.POINTER(_SolverOptions), _dp, _dp,
                                     C.POINTER(_SolveInfo)]),
+    "vk_mip_solve_batched": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
+                                       _ip, C.c_int64, C.POINTER(_SolverOptions), _dp, C.POINTER(_SolveInfo),
+                                       _lp]),
     "vk_lp_profile": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 C.POINTER(_SolverOptions), C.c_int64, C.c_int64, _dp]),
     "vk_lp_generate": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_vp)]),
@@ -553,6 +556,37 @@ def solve_nodes(lp: LP, lower=None, upper=None, warm_start=None, options: Option
         raise ValueError(f"vk_lp_solve_nodes rejected its arguments ({rc})")
     return [_result(infos[i], x[i * lp.n:(i + 1) * lp.n].copy(), y[i * lp.m:(i + 1) * lp.m].copy())
             for i in range(count)]
+
+
+MIP_STATS = ("nodes_processed", "nodes_pruned", "nodes_integral", "nodes_infeasible", "incumbent_updates",
+             "rounds", "relaxations", "relaxation_iterations", "max_batch")
+
+
+def solve_mip(lp: LP, options: Options | None = None, batch: int = 64, variable_types=None):
+    """solver::solve_mip_batched through vk_mip_solve_batched: branch-and-bound
+    whose node relaxations are solved `batch` at a time on one device matrix.
+    variable_types (default lp.variable_types): 0 continuous, 1 integer,
+    2 binary.  Returns (Result with the incumbent as x, dict of MIP_STATS)."""
+    L = host_lib()
+    vt = variable_types if variable_types is not None else lp.variable_types
+    if vt is None:
+        raise ValueError("solve_mip needs variable types")
+    vt = np.ascontiguousarray(vt, dtype=np.int32)
+    k = [np.ascontiguousarray(a, dtype=t) for a, t in (
+        (lp.row_offsets, np.int64), (lp.column_indices, np.int64), (lp.values, np.float64),
+        (lp.objective, np.float64), (lp.row_lower, np.float64), (lp.row_upper, np.float64),
+        (lp.var_lower, np.float64), (lp.var_upper, np.float64))]
+    x = np.zeros(lp.n)
+    info = _SolveInfo()
+    stats = np.zeros(len(MIP_STATS), np.int64)
+    copt = (options or Options())._c()
+    rc = L.vk_mip_solve_batched(lp.m, lp.n, _ptr(k[0], _lp), _ptr(k[1], _lp), _ptr(k[2]), _ptr(k[3]),
+                                float(lp.objective_offset), _ptr(k[4]), _ptr(k[5]), _ptr(k[6]), _ptr(k[7]),
+                                _ptr(vt, _ip), int(batch), C.byref(copt), _ptr(x), C.byref(info),
+                                _ptr(stats, _lp))
+    if rc != 0:
+        raise ValueError(f"vk_mip_solve_batched rejected its arguments ({rc})")
+    return _result(info, x, np.zeros(lp.m)), dict(zip(MIP_STATS, (int(v) for v in stats)))
 
 
 def solve(lp: LP, options: Options | None = None, quadratic=None) -> Result:
