@@ -32,8 +32,10 @@ one LP on one device matrix against per-node solve_lp calls ("nodes"), the
 unrolled-PDHG adjoint on a 100,000 x 250,000 LP with its central-difference
 check ("adjoint"), the benchmark LP plus a sparse convex Q solved by Halpern
 PDQP to the same relative-KKT target, beside the reference policy after as
-many iterations ("qp"), and MPS files given with --mps or written by
---mps-generated ("mps").
+many iterations ("qp"), branch-and-bound on a seeded MILP with node
+relaxations solved one at a time against rounds of 16 and 64 on one matrix
+("mip"), and MPS files given with --mps or written by --mps-generated
+("mps").
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 0]
 """
@@ -266,6 +268,51 @@ def measure_config1(V, np, args):
         f"({out['cpu_status']}, {out['cpu_iterations']} iterations); GPU drop-in {gpu.status} "
         f"{gpu.iterations} iterations in {gpu.solve_seconds:.3f} s wall; Halpern 1e-6 FP64 "
         f"{out['halpern_fp64']['device_seconds']:.3f} s, mixed {out['halpern_mixed']['device_seconds']:.3f} s")
+    return out
+
+
+def seeded_milp(V, np, m, n, seed, integer_share=0.7, upper=6):
+    """A feasible MILP: rows A x <= b with 6 coefficients in 1..8 per row and
+    b = A x* + U{0..6} for an integer point x* in [0, upper]^n, 70% of the
+    variables integer, objective -U(1, 10) per variable."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    dense = np.zeros((m, n))
+    for i in range(m):
+        cols = rng.choice(n, size=min(n, 6), replace=False)
+        dense[i, cols] = rng.integers(1, 9, size=cols.size)
+    xstar = rng.integers(0, upper + 1, size=n).astype(float)
+    b = dense @ xstar + rng.integers(0, 7, size=m)
+    A = sp.csr_matrix(dense)
+    types = (rng.random(n) < integer_share).astype(np.int32)
+    c = -rng.uniform(1.0, 10.0, size=n)
+    return V.LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data.astype(np.float64), c,
+                np.full(m, -np.inf), b.astype(np.float64), np.zeros(n), np.full(n, float(upper)),
+                variable_types=types)
+
+
+def measure_mip(V, np, args):
+    """Branch-and-bound with batched node relaxations (solve_mip_batched) on a
+    seeded 30 x 40 MILP (28 integer variables, about a thousand nodes): the
+    serial search (batch 1, the reference's one relaxation per node, each a
+    fresh upload and scaling) against rounds of 16 and 64 nodes in the batched
+    kernel on one matrix.  Halpern relaxations to 1e-9 relative KKT."""
+    lp = seeded_milp(V, np, 30, 40, 3)   # the 30 x 40 instance of tests/test_gpu_mip.py
+    o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=1e-9, iteration_limit=200_000,
+                  absolute_gap_tolerance=1e-6, relative_gap_tolerance=1e-9, use_graphs=not args.no_graphs)
+    out = {"m": lp.m, "n": lp.n, "integer_variables": int(np.sum(lp.variable_types)), "runs": {}}
+    for batch in args.mip_batches:
+        t = time.time()
+        r, st = V.solve_mip(lp, o, batch=batch)
+        wall = time.time() - t
+        out["runs"][str(batch)] = {"status": r.status, "objective": r.objective_value, "wall_seconds": wall,
+                                   "device_seconds": r.device_seconds, **st}
+        log(f"mip: batch {batch}: {r.status} objective {r.objective_value:.9g}, {st['nodes_processed']} nodes in "
+            f"{st['rounds']} rounds, wall {wall:.2f}s")
+    runs = out["runs"]
+    if "1" in runs:
+        for b, v in runs.items():
+            v["speedup_vs_serial"] = runs["1"]["wall_seconds"] / v["wall_seconds"]
     return out
 
 
@@ -509,6 +556,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=512,
                     help="block-angular instances of the batched solve (0 = skip)")
     ap.add_argument("--batch-iterations", type=int, default=40_000)
+    ap.add_argument("--mip-batches", type=lambda s: [int(v) for v in s.split(",") if v], default=[1, 16, 64],
+                    help="batch sizes of the branch-and-bound measurement (comma-separated; empty skips)")
     ap.add_argument("--nodes", type=int, default=256,
                     help="branch-and-bound node relaxations of one LP on one matrix (0 = skip)")
     ap.add_argument("--adjoint-iterations", type=int, default=1000,
@@ -655,6 +704,7 @@ def main():
         import shutil
         shutil.rmtree(generated_dir, ignore_errors=True)
     nodes = measure_nodes(V, np, args) if args.nodes > 0 and not args.skip_solve else None
+    mip = measure_mip(V, np, args) if args.mip_batches and not args.skip_solve else None
     qp = (measure_qp(V, np, lp, args, t64 * 1e3 / max(1, r64.iterations))
           if args.qp_iterations > 0 and not args.skip_solve else None)
 
@@ -703,6 +753,7 @@ def main():
         "adjoint": adjoint,
         "mps": mps,
         "nodes": nodes,
+        "mip": mip,
         "qp": qp,
     }
     print(json.dumps(line), flush=True)
