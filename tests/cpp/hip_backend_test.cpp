@@ -10,6 +10,7 @@
 // Exit code 0 = all pass.
 
 #include "vikalp/backend/HipBackend.hpp"
+#include "vikalp/solver/Solver.hpp"
 #include "pdhg_oracle.h"
 
 #include <cmath>
@@ -249,6 +250,39 @@ void random_consistency(vikalp::ExecutionBackend &be, std::int64_t rows, std::in
     check_eq("random norm_inf", be.norm_inf(*va), orc_norm_inf(cols, a.data()), 0.0);
 }
 
+// vikalp::solve routes a MILP to branch-and-bound (Solver.cpp:130-134): min
+// -x1 - x2 with 2 x1 + 2 x2 <= 3, x integer in [0, 2].  The relaxation's
+// optimum is -1.5; the integer optimum is -1 (x1 + x2 = 1).  Relaxations by
+// Halpern PDHG to 1e-9 relative KKT: under the reference step rule one node
+// relaxation of this model fails the original-coordinate residual check
+// (NumericalFailure), which ends a search as it does in the reference.
+void router_milp() {
+    vikalp::Model model;
+    model.name = "router-milp";
+    model.linear_objective = {-1.0, -1.0};
+    model.variable_lower = {0.0, 0.0};
+    model.variable_upper = {2.0, 2.0};
+    model.variable_types = {vikalp::VariableType::Integer, vikalp::VariableType::Integer};
+    model.constraint_lower = {-vikalp::Model::infinity()};
+    model.constraint_upper = {3.0};
+    model.constraint_matrix.pattern.rows = 1;
+    model.constraint_matrix.pattern.columns = 2;
+    model.constraint_matrix.pattern.row_offsets = {0, 2};
+    model.constraint_matrix.pattern.column_indices = {0, 1};
+    model.constraint_matrix.values = {2.0, 2.0};
+    vikalp::SolverOptions options;
+    options.backend = vikalp::BackendPreference::HIP;
+    options.pdhg_policy = vikalp::PdhgPolicy::Halpern;
+    options.relative_kkt_tolerance = 1e-9;
+    const auto r = vikalp::solve(model, options);
+    check_true("router milp optimal", r.status == vikalp::SolveStatus::Optimal);
+    check_eq("router milp objective", r.objective_value, -1.0, 1e-6);
+    check_true("router milp integral", r.primal_solution.size() == 2 &&
+                                           std::abs(r.primal_solution[0] - std::round(r.primal_solution[0])) <= 1e-6 &&
+                                           std::abs(r.primal_solution[1] - std::round(r.primal_solution[1])) <= 1e-6);
+    check_true("router milp nodes", r.nodes >= 2);
+}
+
 }  // namespace
 
 int main() {
@@ -258,6 +292,7 @@ int main() {
     random_consistency(*be, 300, 200, 0.05, 1);
     random_consistency(*be, 3000, 1000, 0.01, 2);
     random_consistency(*be, 1, 5000, 0.5, 3);
+    router_milp();
     if (g_failures == 0) {
         std::printf("ALL HIPBACKEND TESTS PASSED\n");
         return 0;

@@ -1,7 +1,9 @@
 // vikalp::solve for the HIP library: the routing of src/solver/Solver.cpp:28-99
 // with BackendPreference::HIP.  Exceptions below become NumericalFailure
 // (Solver.cpp:81-86); the backend label is "hip" (Solver.cpp:79,96 use
-// "cpu"/"cuda").
+// "cpu"/"cuda").  MILP models go to branch-and-bound as in Solver.cpp:130-134,
+// here the batched engine (solve_mip_batched, rounds of kRouterBatch node
+// relaxations); MIQP and MINLP are not served.
 
 #include "vikalp/solver/Solver.hpp"
 
@@ -13,6 +15,10 @@
 #include <span>
 
 namespace vikalp {
+
+// Node relaxations per branch-and-bound round when vikalp::solve routes a
+// MILP (the batch size that measured fastest on the benchmark MILP).
+constexpr Index kRouterBatch = 16;
 
 SolveResult solve(const Model &model, const SolverOptions &options) {
     SolveResult result;
@@ -28,9 +34,19 @@ SolveResult solve(const Model &model, const SolverOptions &options) {
         return result;
     }
     const ProblemClass pc = model.problem_class();
+    if (pc == ProblemClass::MILP) {
+        try {
+            return solver::solve_mip_batched(model, options, kRouterBatch);
+        } catch (const std::exception &error) {
+            result.status = SolveStatus::NumericalFailure;
+            result.message = error.what();
+            result.backend = "hip";
+            return result;
+        }
+    }
     if (pc != ProblemClass::LP && pc != ProblemClass::QP) {
         result.status = SolveStatus::UnsupportedModel;
-        result.message = "the HIP path solves LP (PDHG) and QP (PDQP) models only";
+        result.message = "the HIP path solves LP (PDHG), QP (PDQP) and MILP (batched branch-and-bound) models";
         return result;
     }
     try {
