@@ -43,19 +43,23 @@ struct Stacked {
     std::vector<double> steps, weights, nb, nc;  // per instance
     std::vector<double> warm;                    // scaled start x of every instance, or empty
     vk_hip_csr *A = nullptr;
+    bool owns_A = true;    // false: A belongs to a NodeCache
     bool shared = false;   // A is one matrix every instance runs on
     int tile_geometry = 0;     // the instances' row tiles (vk_pdhg_params.tile_geometry)
     std::string tiles_note;    // SolveResult::message note of a non-default geometry
-    ~Stacked() { vk_hip_csr_destroy(A); }
+    ~Stacked() {
+        if (owns_A) vk_hip_csr_destroy(A);
+    }
 };
 
 // Node batch: every instance is `model` with its own variable bounds (and
 // warm start).  The matrix is uploaded, transposed and scaled once; each
 // node's bounds are scaled by the same operations solve_lp applies to them
 // (vk_hip_lp_scale_bounds), so each node starts from exactly the data of its
-// own solve_lp.
+// own solve_lp.  The matrix, its scaling, tile geometry and step live in
+// `cache`, built at its first use and reused by later calls.
 void stack_shared(const Model &model, const std::vector<Prepared> &prep, const std::vector<std::vector<Scalar>> &warm,
-                  const SolverOptions &options, Stacked &st) {
+                  const SolverOptions &options, Stacked &st, NodeCache &cache) {
     const int count = static_cast<int>(prep.size());
     const int64_t m = model.num_constraints(), n = model.num_variables();
     st.shared = true;
@@ -65,18 +69,41 @@ void stack_shared(const Model &model, const std::vector<Prepared> &prep, const s
         st.row_starts[i] = i * m;
         st.col_starts[i] = i * n;
     }
-    const auto &pat = model.constraint_matrix.pattern;
-    check_vk(vk_hip_csr_create(pat.rows, pat.columns, pat.row_offsets.data(),
-                               pat.column_indices.empty() ? nullptr : pat.column_indices.data(), &st.A),
-             "vk_hip_csr_create");
-    check_vk(vk_hip_csr_set_values(st.A, model.constraint_matrix.values.empty() ? nullptr
-                                                                               : model.constraint_matrix.values.data()),
-             "vk_hip_csr_set_values");
     const bool equilibrated = options.pdhg_policy != PdhgPolicy::Reference;
+    if (!cache.A) {
+        const auto &pat = model.constraint_matrix.pattern;
+        check_vk(vk_hip_csr_create(pat.rows, pat.columns, pat.row_offsets.data(),
+                                   pat.column_indices.empty() ? nullptr : pat.column_indices.data(), &cache.A),
+                 "vk_hip_csr_create");
+        check_vk(vk_hip_csr_set_values(cache.A, model.constraint_matrix.values.empty()
+                                                    ? nullptr
+                                                    : model.constraint_matrix.values.data()),
+                 "vk_hip_csr_set_values");
+        // Scaling depends on A only: once, with the first node's bounds;
+        // every node's bounds then go through the same recorded column
+        // factors.
+        cache.base = device_scale(cache.A, model, prep[0].bounds, equilibrated);
+        cache.tile_geometry = apply_tile_geometry(cache.A, options);
+        cache.tiles_note = tile_geometry_note(cache.tile_geometry, options.spmv_tile_geometry < 0);
+        // The step of solve_lp: the reference operator bound
+        // (pdhg.cpp:75-78), or 0.99 over the power-iteration estimate of
+        // ||A||_2.
+        if (!equilibrated) {
+            double bound = 0.0;
+            check_vk(vk_hip_csr_operator_bound(cache.A, &bound), "vk_hip_csr_operator_bound");
+            cache.step = 0.99 / std::max<Scalar>(1.0, bound);
+        } else {
+            double norm = 0.0;
+            check_vk(vk_hip_csr_norm2_estimate(cache.A, 100, &norm), "vk_hip_csr_norm2_estimate");
+            cache.step = 0.99 / std::max(norm, 1e-12);
+        }
+    }
+    st.A = cache.A;
+    st.owns_A = false;
+    st.tile_geometry = cache.tile_geometry;
+    st.tiles_note = cache.tiles_note;
     Scaled &sc = st.sc;
-    // Scaling depends on A only: once, with the first node's bounds; every
-    // node's bounds then go through the same recorded column factors.
-    const Scaled s = device_scale(st.A, model, prep[0].bounds, equilibrated);
+    const Scaled &s = cache.base;
     std::vector<double> lo, hi;
     for (int i = 0; i < count; ++i) {
         lo.insert(lo.end(), prep[i].bounds.lower.begin(), prep[i].bounds.lower.end());
@@ -101,19 +128,7 @@ void stack_shared(const Model &model, const std::vector<Prepared> &prep, const s
                 st.warm[static_cast<std::size_t>(i * n) + j] = w[j] / sc.s[static_cast<std::size_t>(i * n) + j];   // pdhg.cpp:64-69
         }
     }
-    // The step of solve_lp: the reference operator bound (pdhg.cpp:75-78),
-    // or 0.99 over the power-iteration estimate of ||A||_2.
-    double step = 0.0;
-    if (!equilibrated) {
-        double bound = 0.0;
-        check_vk(vk_hip_csr_operator_bound(st.A, &bound), "vk_hip_csr_operator_bound");
-        step = 0.99 / std::max<Scalar>(1.0, bound);
-    } else {
-        double norm = 0.0;
-        check_vk(vk_hip_csr_norm2_estimate(st.A, 100, &norm), "vk_hip_csr_norm2_estimate");
-        step = 0.99 / std::max(norm, 1e-12);
-    }
-    st.steps.assign(count, step);
+    st.steps.assign(count, cache.step);
     const std::vector<Scalar> c0(sc.c.begin(), sc.c.begin() + n), rl0(sc.rl.begin(), sc.rl.begin() + m),
         ru0(sc.ru.begin(), sc.ru.begin() + m);
     const Scalar cn = norm2(c0), bn = row_bound_norm(rl0, ru0);
@@ -562,6 +577,15 @@ std::vector<SolveResult> solve_lp_nodes(const Model &model, const std::vector<st
                                         const std::vector<std::vector<Scalar>> &upper,
                                         const std::vector<std::vector<Scalar>> &warm_start,
                                         const SolverOptions &options) {
+    detail::NodeCache cache;
+    return detail::solve_lp_nodes_cached(model, lower, upper, warm_start, options, cache);
+}
+
+std::vector<SolveResult> detail::solve_lp_nodes_cached(const Model &model,
+                                                       const std::vector<std::vector<Scalar>> &lower,
+                                                       const std::vector<std::vector<Scalar>> &upper,
+                                                       const std::vector<std::vector<Scalar>> &warm_start,
+                                                       const SolverOptions &options, NodeCache &cache) {
     const auto started = Clock::now();
     const std::size_t count = std::max(lower.size(), upper.size());
     std::vector<SolveResult> results(count);
@@ -612,9 +636,7 @@ std::vector<SolveResult> solve_lp_nodes(const Model &model, const std::vector<st
     }
     if (prep.empty()) return results;
     Stacked st;
-    stack_shared(model, prep, warm, options, st);
-    st.tile_geometry = apply_tile_geometry(st.A, options);
-    st.tiles_note = tile_geometry_note(st.tile_geometry, options.spmv_tile_geometry < 0);
+    stack_shared(model, prep, warm, options, st, cache);
     const std::vector<const Model *> models(count, &model);
     run_stacked(models, prep, st, options, started, results);
     return results;

@@ -96,6 +96,25 @@ int run_device(const vk_hip_csr *A, const Scaled &sc, const vk_pdhg_params &para
 int drive_device(DeviceSolver &solver, const vk_pdhg_params &params, std::size_t n, std::size_t m,
                  Clock::time_point started, const SolverOptions &options, std::vector<Scalar> &xs,
                  std::vector<Scalar> &ys, vk_pdhg_state &state, double &device_ms, bool &timed);
+// The device matrix of one model's node relaxations, scaled once, with its
+// scaling and step, kept across the solve_lp_nodes rounds of a
+// branch-and-bound search (the options must not change between rounds).
+struct NodeCache {
+    vk_hip_csr *A = nullptr;
+    Scaled base;             // device_scale's result (values on the device)
+    double step = 0.0;
+    int tile_geometry = 0;
+    std::string tiles_note;
+    NodeCache() = default;
+    NodeCache(const NodeCache &) = delete;
+    NodeCache &operator=(const NodeCache &) = delete;
+    ~NodeCache() { vk_hip_csr_destroy(A); }
+};
+// solve_lp_nodes on the cache's matrix (built at the first call).
+std::vector<SolveResult> solve_lp_nodes_cached(const Model &model, const std::vector<std::vector<Scalar>> &lower,
+                                               const std::vector<std::vector<Scalar>> &upper,
+                                               const std::vector<std::vector<Scalar>> &warm_start,
+                                               const SolverOptions &options, NodeCache &cache);
 // continuous_common.hpp:242-275 (dense) up to n = 4096; beyond that a
 // diagonally dominant Q with a non-negative diagonal.  `message` says why not.
 bool positive_semidefinite(const CsrMatrix &q, std::string &message);

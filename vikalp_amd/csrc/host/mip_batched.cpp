@@ -7,9 +7,9 @@
 // (Solver.cpp:37-51).  Here each round takes up to `batch` open nodes in the
 // reference's queue order (BbQueue.hpp:20-33: lower relaxation bound first,
 // then the deeper node, then the lower id), solves their relaxations in one
-// solve_lp_nodes call (one upload and scaling per round, all nodes in the
-// batched kernel), and then processes the results in that order with the
-// reference's rules:
+// call of the batched node solver (all nodes in the batched kernel, on a
+// device matrix uploaded, scaled and sized for its step once per search),
+// and then processes the results in that order with the reference's rules:
 //   - a node whose bounds cross is infeasible (BranchAndBound.cpp:273-283);
 //   - an infeasible relaxation is counted, a failed one ends the search with
 //     its status, an unsolved or invalid one with NumericalFailure
@@ -242,6 +242,8 @@ SolveResult solve_mip_batched(const Model &model, const SolverOptions &options, 
         row_tol = std::max(row_tol, options.relative_kkt_tolerance *
                                         (1.0 + detail::row_bound_norm(model.constraint_lower, model.constraint_upper)));
 
+    // One device matrix, scaling and step for every round.
+    detail::NodeCache cache;
     OpenSet open;
     Pseudocosts pc(n);
     std::int64_t next_id = 0;
@@ -295,7 +297,7 @@ SolveResult solve_mip_batched(const Model &model, const SolverOptions &options, 
         }
         std::vector<SolveResult> relax;
         if (!lower.empty()) {
-            relax = solve_lp_nodes(model, lower, upper, warm, options);
+            relax = detail::solve_lp_nodes_cached(model, lower, upper, warm, options, cache);
             ++st.rounds;
             st.relaxations += static_cast<std::int64_t>(lower.size());
             st.max_batch = std::max<std::int64_t>(st.max_batch, static_cast<std::int64_t>(lower.size()));
