@@ -293,10 +293,10 @@ def seeded_milp(V, np, m, n, seed, integer_share=0.7, upper=6):
 
 def measure_mip(V, np, args):
     """Branch-and-bound with batched node relaxations (solve_mip_batched) on a
-    seeded 30 x 40 MILP (28 integer variables, about a thousand nodes): the
-    serial search (batch 1, the reference's one relaxation per node, each a
-    fresh upload and scaling) against rounds of 16 and 64 nodes in the batched
-    kernel on one matrix.  Halpern relaxations to 1e-9 relative KKT."""
+    seeded 30 x 40 MILP (27 integer variables, about a thousand nodes): the
+    serial search (batch 1, the reference's node order, one relaxation per
+    round) against rounds of 16 and 64 nodes in the batched kernel, all on one
+    cached device matrix.  Halpern relaxations to 1e-9 relative KKT."""
     lp = seeded_milp(V, np, 30, 40, 3)   # the 30 x 40 instance of tests/test_gpu_mip.py
     o = V.Options(pdhg_policy=V.PdhgPolicy.Halpern, relative_kkt_tolerance=1e-9, iteration_limit=200_000,
                   absolute_gap_tolerance=1e-6, relative_gap_tolerance=1e-9, use_graphs=not args.no_graphs)
