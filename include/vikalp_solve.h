@@ -220,6 +220,8 @@ int vk_model_export(const vk_model *model, int64_t *row_offsets, int64_t *column
  * BranchAndBound.cpp:216-465, one RelaxationOracle::solve per node) with up
  * to `batch` open nodes per round solved together by vk_lp_solve_nodes' path
  * on one device matrix.  variable_types: 0 continuous, 1 integer, 2 binary.
+ * q_* (q_row_offsets NULL = none): a quadratic objective 0.5 x'Qx as for
+ * vk_qp_solve; an MIQP is refused (info.status UnsupportedModel).
  * x_out (n; NULL to skip) gets the incumbent, info the result (iterations =
  * nodes processed), stats (NULL to skip) 9 counters: nodes processed,
  * pruned, integral, infeasible, incumbent updates, rounds, relaxations,
@@ -230,7 +232,8 @@ int vk_mip_solve_batched(int64_t m, int64_t n, const int64_t *row_offsets,
                          const double *objective, double objective_offset,
                          const double *row_lower, const double *row_upper,
                          const double *var_lower, const double *var_upper,
-                         const int32_t *variable_types, int64_t batch,
+                         const int32_t *variable_types, const int64_t *q_row_offsets,
+                         const int64_t *q_column_indices, const double *q_values, int64_t batch,
                          const vk_solver_options *options, double *x_out,
                          vk_solve_info *info, int64_t *stats);
 

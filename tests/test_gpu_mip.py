@@ -10,7 +10,10 @@ reference does not have, so it is pinned against the mathematics:
 - serial (batch 1, the reference's node order) and batched searches reaching
   the same optimum, the batched one in fewer relaxation rounds;
 - an integer-infeasible MILP whose LP relaxation is feasible, the node limit,
-  and the refused models.
+  and the refused models;
+- an MIQP (QP relaxations) against the optimum found by enumerating its
+  integer assignments, with the continuous variable in closed form (skipped:
+  the MIQP path is switched off), and the MIQP refusal.
 """
 import numpy as np
 import pytest
@@ -121,3 +124,67 @@ def test_node_limit_and_refusals():
     assert r.status == "UnsupportedModel"
     with pytest.raises(ValueError):
         V.solve_mip(cont, V.Options(**OPTS), variable_types=np.full(cont.n, 7, np.int32))
+
+
+def small_miqp():
+    """min c'x + 1/2 x'Qx with x1..x4 integer in [0, 3], x5 continuous in [0, 5],
+    two rows A x <= b, and Q strictly diagonally dominant (positive definite)."""
+    n = 5
+    Q = np.array([[2.0, 0.5, 0.0, 0.0, 0.3],
+                  [0.5, 1.5, 0.2, 0.0, 0.0],
+                  [0.0, 0.2, 1.0, 0.3, 0.0],
+                  [0.0, 0.0, 0.3, 2.5, 0.4],
+                  [0.3, 0.0, 0.0, 0.4, 1.2]])
+    c = np.array([-4.0, -3.5, -2.0, -5.0, -3.0])
+    A = np.array([[1.0, 1.0, 1.0, 1.0, 0.5],
+                  [2.0, 0.0, 1.0, 3.0, 1.0]])
+    b = np.array([5.5, 9.0])
+    As, Qs = sp.csr_matrix(A), sp.csr_matrix(Q)
+    lp = V.LP(2, n, As.indptr.astype(np.int64), As.indices.astype(np.int64), As.data, c,
+              np.full(2, -np.inf), b, np.zeros(n), np.array([3.0, 3.0, 3.0, 3.0, 5.0]),
+              variable_types=np.array([1, 1, 1, 1, 0], np.int32))
+    q = (Qs.indptr.astype(np.int64), Qs.indices.astype(np.int64), Qs.data)
+    return lp, q, Q, c, A, b
+
+
+def brute_force_miqp(Q, c, A, b):
+    """Every integer assignment of x1..x4; x5 then minimises a 1-D convex
+    quadratic over the interval the rows and its bounds leave (closed form)."""
+    import itertools
+    best = np.inf
+    for xi in itertools.product(range(4), repeat=4):
+        xi = np.array(xi, float)
+        lo, hi = 0.0, 5.0
+        for a, bi in zip(A, b):
+            rest = bi - a[:4] @ xi
+            if a[4] > 0:
+                hi = min(hi, rest / a[4])
+            elif rest < 0:
+                lo = np.inf
+        if lo > hi:
+            continue
+        x5 = min(max(-(c[4] + Q[4, :4] @ xi) / Q[4, 4], lo), hi)
+        x = np.append(xi, x5)
+        best = min(best, c @ x + 0.5 * x @ Q @ x)
+    return best
+
+
+@pytest.mark.skip(reason="MIQP through serial solve_qp relaxations: the first run of this case ended the "
+                         "process with a host segmentation fault whose cause is not established; the path is "
+                         "switched off (UnsupportedModel) until it is")
+@pytest.mark.parametrize("batch", [1, 8])
+def test_miqp_reaches_the_enumerated_optimum(batch):
+    lp, q, Q, c, A, b = small_miqp()
+    opt = brute_force_miqp(Q, c, A, b)
+    r, st = V.solve_mip(lp, V.Options(**OPTS), batch=batch, quadratic=q)
+    print(f"miqp batch={batch}: {r.status} {r.objective_value:.9g} vs enumerated {opt:.9g}; {st}", flush=True)
+    assert r.status == "Optimal", (r.status, r.message)
+    assert abs(r.objective_value - opt) <= 1e-6 * max(1.0, abs(opt)), (r.objective_value, opt)
+    check_incumbent(lp, r.x)
+    assert abs(c @ r.x + 0.5 * r.x @ Q @ r.x - r.objective_value) <= 1e-9 * max(1.0, abs(opt))
+
+
+def test_miqp_is_refused():
+    lp, q, *_ = small_miqp()
+    r, st = V.solve_mip(lp, V.Options(**OPTS), batch=8, quadratic=q)
+    assert r.status == "UnsupportedModel" and st["rounds"] == 0, (r.status, r.message)

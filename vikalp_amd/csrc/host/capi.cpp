@@ -287,13 +287,23 @@ int vk_lp_solve_nodes(int64_t m, int64_t n, const int64_t *row_offsets, const in
 int vk_mip_solve_batched(int64_t m, int64_t n, const int64_t *row_offsets, const int64_t *column_indices,
                          const double *values, const double *objective, double objective_offset,
                          const double *row_lower, const double *row_upper, const double *var_lower,
-                         const double *var_upper, const int32_t *variable_types, int64_t batch,
+                         const double *var_upper, const int32_t *variable_types, const int64_t *q_row_offsets,
+                         const int64_t *q_column_indices, const double *q_values, int64_t batch,
                          const vk_solver_options *options, double *x_out, vk_solve_info *info, int64_t *stats) {
     if (!info || !options || !variable_types) return -1;
     Model model;
     if (!build_model(m, n, row_offsets, column_indices, values, objective, objective_offset, row_lower,
                      row_upper, var_lower, var_upper, model))
         return -1;
+    if (q_row_offsets) {
+        const int64_t qnnz = q_row_offsets[n];
+        if (qnnz < 0 || (qnnz > 0 && (!q_column_indices || !q_values))) return -1;
+        model.quadratic_objective.pattern.rows = n;
+        model.quadratic_objective.pattern.columns = n;
+        model.quadratic_objective.pattern.row_offsets.assign(q_row_offsets, q_row_offsets + n + 1);
+        model.quadratic_objective.pattern.column_indices.assign(q_column_indices, q_column_indices + qnnz);
+        model.quadratic_objective.values.assign(q_values, q_values + qnnz);
+    }
     for (int64_t j = 0; j < n; ++j) {
         if (variable_types[j] < 0 || variable_types[j] > 2) return -1;
         model.variable_types[static_cast<std::size_t>(j)] = static_cast<VariableType>(variable_types[j]);

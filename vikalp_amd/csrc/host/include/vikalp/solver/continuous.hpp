@@ -70,7 +70,8 @@ struct MipStats {
 // pseudocost branching, gap termination) whose node relaxations are solved
 // `batch` at a time by solve_lp_nodes on one device matrix, instead of one
 // RelaxationOracle::solve per node.  batch = 1 gives the reference's node
-// sequence.  Linear objectives only; options go to every relaxation.
+// sequence.  Options go to every relaxation.  Linear objectives only (an
+// MIQP is UnsupportedModel).
 SolveResult solve_mip_batched(const Model &model, const SolverOptions &options = {}, Index batch = 64,
                               MipStats *stats = nullptr);
 

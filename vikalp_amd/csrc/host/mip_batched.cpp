@@ -31,8 +31,8 @@
 // With batch = 1 the node sequence is the reference's.  With larger batches
 // nodes that the serial order would have pruned by an incumbent found
 // earlier in the same round are solved anyway (and pruned when processed),
-// the price of solving them together.  Only linear objectives: the batched
-// relaxations are LPs.
+// the price of solving them together.  Linear objectives only: a quadratic
+// objective (MIQP) is refused.
 
 #include "lp_internal.hpp"
 
@@ -43,6 +43,7 @@
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <memory>
 #include <queue>
 #include <string>
 #include <utility>
@@ -132,9 +133,15 @@ bool integral(const Model &model, const std::vector<Scalar> &x, Scalar tol) {
     return true;
 }
 
-Scalar linear_objective(const Model &model, const std::vector<Scalar> &x) {
+// offset + c'x + 1/2 x'Qx over Q's stored entries (BranchAndBound.cpp:47-62).
+Scalar objective(const Model &model, const std::vector<Scalar> &x) {
     Scalar v = model.objective_offset;
     for (std::size_t j = 0; j < x.size(); ++j) v += model.linear_objective[j] * x[j];
+    const auto &q = model.quadratic_objective;
+    for (Index r = 0; r < q.pattern.rows; ++r)
+        for (Index k = q.pattern.row_offsets[r]; k < q.pattern.row_offsets[r + 1]; ++k)
+            v += 0.5 * q.values[static_cast<std::size_t>(k)] * x[static_cast<std::size_t>(r)] *
+                 x[static_cast<std::size_t>(q.pattern.column_indices[k])];
     return v;
 }
 
@@ -221,7 +228,7 @@ SolveResult solve_mip_batched(const Model &model, const SolverOptions &options, 
         result.message = errors.front();
         return result;
     }
-    if (model.quadratic_objective.pattern.column_indices.size() > 0) {
+    if (model.has_quadratic_objective()) {
         result.status = SolveStatus::UnsupportedModel;
         result.message = "batched branch-and-bound solves LP relaxations: the objective must be linear";
         return result;
@@ -359,7 +366,7 @@ SolveResult solve_mip_batched(const Model &model, const SolverOptions &options, 
             const std::vector<Scalar> &x = r.primal_solution;
             if (integral(model, x, int_tol)) {
                 ++st.nodes_integral;
-                take(linear_objective(model, x), x);
+                take(objective(model, x), x);
             } else {
                 // Rounding heuristic: integers rounded, everything clamped to
                 // the node's bounds, kept if integral and feasible.
@@ -370,7 +377,7 @@ SolveResult solve_mip_batched(const Model &model, const SolverOptions &options, 
                 }
                 if (integral(model, rx, int_tol) &&
                     feasible(model, rx, lo, hi, options.primal_tolerance, options.primal_tolerance))
-                    take(linear_objective(model, rx), std::move(rx));
+                    take(objective(model, rx), std::move(rx));
                 const Index j = branch_variable(model, x, lo, hi, int_tol, st.nodes_processed <= 8 ? nullptr : &pc);
                 if (j < 0) {
                     termination = SolveStatus::NumericalFailure;

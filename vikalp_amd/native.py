@@ -204,8 +204,8 @@ HOST_SYMBOLS = {
                                     C.c_int, _dp, _dp, _dp, C.POINTER(_SolverOptions), _dp, _dp,
                                     C.POINTER(_SolveInfo)]),
     "vk_mip_solve_batched": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
-                                       _ip, C.c_int64, C.POINTER(_SolverOptions), _dp, C.POINTER(_SolveInfo),
-                                       _lp]),
+                                       _ip, _lp, _lp, _dp, C.c_int64, C.POINTER(_SolverOptions), _dp,
+                                       C.POINTER(_SolveInfo), _lp]),
     "vk_lp_profile": (C.c_int, [C.c_int64, C.c_int64, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 C.POINTER(_SolverOptions), C.c_int64, C.c_int64, _dp]),
     "vk_lp_generate": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_vp)]),
@@ -562,11 +562,13 @@ MIP_STATS = ("nodes_processed", "nodes_pruned", "nodes_integral", "nodes_infeasi
              "rounds", "relaxations", "relaxation_iterations", "max_batch")
 
 
-def solve_mip(lp: LP, options: Options | None = None, batch: int = 64, variable_types=None):
+def solve_mip(lp: LP, options: Options | None = None, batch: int = 64, variable_types=None, quadratic=None):
     """solver::solve_mip_batched through vk_mip_solve_batched: branch-and-bound
     whose node relaxations are solved `batch` at a time on one device matrix.
     variable_types (default lp.variable_types): 0 continuous, 1 integer,
-    2 binary.  Returns (Result with the incumbent as x, dict of MIP_STATS)."""
+    2 binary; quadratic = (row_offsets, column_indices, values) of a symmetric
+    Q makes it an MIQP.  Returns (Result with the incumbent as x, dict of
+    MIP_STATS)."""
     L = host_lib()
     vt = variable_types if variable_types is not None else lp.variable_types
     if vt is None:
@@ -576,14 +578,18 @@ def solve_mip(lp: LP, options: Options | None = None, batch: int = 64, variable_
         (lp.row_offsets, np.int64), (lp.column_indices, np.int64), (lp.values, np.float64),
         (lp.objective, np.float64), (lp.row_lower, np.float64), (lp.row_upper, np.float64),
         (lp.var_lower, np.float64), (lp.var_upper, np.float64))]
+    q = [None, None, None]
+    if quadratic is not None:
+        q = [np.ascontiguousarray(quadratic[0], dtype=np.int64), np.ascontiguousarray(quadratic[1], dtype=np.int64),
+             np.ascontiguousarray(quadratic[2], dtype=np.float64)]
     x = np.zeros(lp.n)
     info = _SolveInfo()
     stats = np.zeros(len(MIP_STATS), np.int64)
     copt = (options or Options())._c()
     rc = L.vk_mip_solve_batched(lp.m, lp.n, _ptr(k[0], _lp), _ptr(k[1], _lp), _ptr(k[2]), _ptr(k[3]),
                                 float(lp.objective_offset), _ptr(k[4]), _ptr(k[5]), _ptr(k[6]), _ptr(k[7]),
-                                _ptr(vt, _ip), int(batch), C.byref(copt), _ptr(x), C.byref(info),
-                                _ptr(stats, _lp))
+                                _ptr(vt, _ip), _ptr(q[0], _lp), _ptr(q[1], _lp), _ptr(q[2]), int(batch),
+                                C.byref(copt), _ptr(x), C.byref(info), _ptr(stats, _lp))
     if rc != 0:
         raise ValueError(f"vk_mip_solve_batched rejected its arguments ({rc})")
     return _result(info, x, np.zeros(lp.m)), dict(zip(MIP_STATS, (int(v) for v in stats)))
