@@ -217,7 +217,7 @@ int vk_model_export(const vk_model *model, int64_t *row_offsets, int64_t *column
                     double *objective_offset);
 /* Branch-and-bound over batches of node relaxations (new; solver::
  * solve_mip_batched): the reference's engine (src/solver/mip/
- * BranchAndBound.cpp:216-465, one RelaxationOracle::solve per node) with up
+ * BranchAndBound.cpp:222-465, one RelaxationOracle::solve per node) with up
  * to `batch` open nodes per round solved together by vk_lp_solve_nodes' path
  * on one device matrix.  variable_types: 0 continuous, 1 integer, 2 binary.
  * q_* (q_row_offsets NULL = none): a quadratic objective 0.5 x'Qx as for

@@ -112,6 +112,21 @@ def test_host_validation_without_device():
                               np.array([1.0])), V.Options())
 
 
+def test_mip_entry_decides_refusals_on_the_host():
+    # solve_mip_batched's entry checks (BranchAndBound.cpp:228-243 and the
+    # batched engine's own) run before any device call.
+    lp = one_variable()
+    r, st = V.solve_mip(lp, variable_types=np.array([0], np.int32))
+    assert r.status == "UnsupportedModel" and "no integer variables" in r.message
+    r, st = V.solve_mip(lp, batch=0, variable_types=np.array([1], np.int32))
+    assert r.status == "InvalidModel" and st["rounds"] == 0
+    q = (np.array([0, 1]), np.array([0]), np.array([2.0]))
+    r, _ = V.solve_mip(lp, variable_types=np.array([1], np.int32), quadratic=q)
+    assert r.status == "UnsupportedModel" and "linear" in r.message
+    with pytest.raises(ValueError):      # a variable type outside 0..2
+        V.solve_mip(lp, variable_types=np.array([3], np.int32))
+
+
 def test_options_defaults_match_the_contract():
     o = native._SolverOptions()
     native.host_lib().vk_solver_options_default(o)

@@ -65,7 +65,7 @@ struct MipStats {
     std::int64_t max_batch = 0;               // most relaxations in one round
 };
 
-// Branch-and-bound (BranchAndBound.cpp:216-465's rules: best-bound-first
+// Branch-and-bound (BranchAndBound.cpp:222-465's rules: best-bound-first
 // queue, pruning, integral and rounded incumbents, most-fractional then
 // pseudocost branching, gap termination) whose node relaxations are solved
 // `batch` at a time by solve_lp_nodes on one device matrix, instead of one

@@ -1,33 +1,33 @@
 // solver::solve_mip_batched: branch-and-bound whose node relaxations run in
 // batches on one device-resident matrix (solve_lp_nodes).
 //
-// The reference's engine (src/solver/mip/BranchAndBound.cpp:216-465) takes
+// The reference's engine (src/solver/mip/BranchAndBound.cpp:222-465) takes
 // one open node at a time and calls RelaxationOracle::solve for it, which
 // builds a backend, scales and uploads the model for every node
 // (Solver.cpp:37-51).  Here each round takes up to `batch` open nodes in the
-// reference's queue order (BbQueue.hpp:20-33: lower relaxation bound first,
+// reference's queue order (BbQueue.hpp:24-34: lower relaxation bound first,
 // then the deeper node, then the lower id), solves their relaxations in one
 // call of the batched node solver (all nodes in the batched kernel, on a
 // device matrix uploaded, scaled and sized for its step once per search),
 // and then processes the results in that order with the reference's rules:
-//   - a node whose bounds cross is infeasible (BranchAndBound.cpp:273-283);
+//   - a node whose bounds cross is infeasible (BranchAndBound.cpp:276-286);
 //   - an infeasible relaxation is counted, a failed one ends the search with
 //     its status, an unsolved or invalid one with NumericalFailure
-//     (:293-318);
+//     (:298-324);
 //   - the node bound is the relaxation's dual bound, else its objective when
-//     optimal (:355-359), and updates the pseudocosts of the branching that
-//     made the node (:360-371);
+//     optimal (:358-361), and updates the pseudocosts of the branching that
+//     made the node (:362-373);
 //   - a node whose bound is within the absolute gap of the incumbent is
-//     pruned (:373-377);
+//     pruned (:375-379);
 //   - an integral relaxation is an incumbent candidate; otherwise the
 //     rounding heuristic (:145-163) may give one, and the node branches on
 //     the most fractional variable for the first 8 processed nodes, then by
-//     pseudocost score among reliable variables (:395-398), into a down child
+//     pseudocost score among reliable variables (:403-405), into a down child
 //     (upper bound floor) and an up child (lower bound ceil), each warm-started
-//     from the node's relaxation (:405-434);
+//     from the node's relaxation (:414-444);
 //   - the search stops as optimal once every bound so far was certified and
 //     the incumbent is within the absolute or relative gap of the best open
-//     bound (:437-447).
+//     bound (:447-457).
 // With batch = 1 the node sequence is the reference's.  With larger batches
 // nodes that the serial order would have pruned by an incumbent found
 // earlier in the same round are solved anyway (and pruned when processed),
@@ -95,7 +95,7 @@ private:
     std::priority_queue<OpenNode, std::vector<OpenNode>, After> heap_;
 };
 
-// Pseudocosts (Pseudocosts.hpp:14-93): per variable and direction the mean
+// Pseudocosts (Pseudocosts.hpp:23-92): per variable and direction the mean
 // bound change per unit of fractional distance, 1 when unobserved; reliable
 // after 4 observations each way; score (1 - mu) min(d, u) + mu max(d, u)
 // with d, u the costs times the distances and mu = 1/6.
